@@ -1,0 +1,236 @@
+/*
+ * rinnd_hip.h - C ABI of the MI355X (gfx950) NN-Descent k-NN graph builder and greedy graph search.
+ *
+ * Every entry point replaces one seam of lmcinnes/rinnd's Rust API.  The reference interface each
+ * function stands in for is cited as `file:line` (paths relative to the rinnd repository root,
+ * "core" = crates/rinnd-core/src, "py" = crates/rinnd/src/lib.rs).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Array arguments named `data`, `queries` and `out_*` may be host
+ *    pointers or HIP device pointers; the library detects which (hipPointerGetAttributes) and stages
+ *    host buffers through device memory.  Device-resident inputs avoid every host copy.
+ *  - Row-major layouts: data is n x dim float32, neighbour arrays are n x k.
+ *  - Every call returns an rinnd_status; on failure rinnd_last_error() holds a message.  The library
+ *    never aborts the host process (the reference panics, core/index.rs:65-76, 687-704).
+ *  - A context owns one HIP stream on one device.  Calls on one context are serialised by an internal
+ *    mutex; use one context per host thread for concurrency.
+ */
+#ifndef RINND_HIP_H
+#define RINND_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rinnd_status {
+    RINND_OK = 0,
+    RINND_ERR_INVALID_ARG = 1, /* py: PyValueError paths, py:643-709, 787-792 */
+    RINND_ERR_OOM = 2,
+    RINND_ERR_HIP = 3,
+    RINND_ERR_UNSUPPORTED = 4,
+    RINND_ERR_INTERNAL = 5
+} rinnd_status;
+
+/* Distance functors of the reference (core/distance/traits.rs:7-47 `Distance<f32>`).  The GPU kernels
+ * are compiled per functor instead of dispatching through a trait object. */
+typedef enum rinnd_distance {
+    RINND_DIST_SQEUCLIDEAN = 0,       /* SquaredEuclidean        core/distance/euclidean.rs:14-33, 66-110 */
+    RINND_DIST_COSINE = 1,            /* Cosine                  core/distance/cosine.rs:44-118 */
+    RINND_DIST_INNER_PRODUCT = 2,     /* InnerProduct            core/distance/inner_product.rs:64-107 */
+    RINND_DIST_ALT_DOT = 3,           /* AlternativeDot          core/distance/alternatives.rs:39-124 */
+    RINND_DIST_DIRECT_COSINE = 4,     /* DirectNormalizedCosine  core/distance/alternatives.rs:131-154 */
+    RINND_DIST_ALT_COSINE = 5,        /* AlternativeCosine       core/distance/alternatives.rs:200-231 */
+    RINND_DIST_ALT_INNER_PRODUCT = 6  /* AlternativeInnerProduct core/distance/alternatives.rs:160-193 */
+} rinnd_distance;
+
+/* Output distance corrections (core/index.rs:669-673, core/distance/alternatives.rs:276-288). */
+typedef enum rinnd_correction {
+    RINND_CORR_NONE = 0,
+    RINND_CORR_SQRT = 1,             /* euclidean: d.sqrt()                     py:1241-1243 */
+    RINND_CORR_ALT_COSINE = 2,       /* 1 - 2^(-d)  correct_alternative_cosine   alternatives.rs:276-278 */
+    RINND_CORR_ALT_INNER_PRODUCT = 3 /* -1/d        correct_alternative_inner_product alternatives.rs:282-288 */
+} rinnd_correction;
+
+/* Where the random draws of RP-tree splits and candidate priorities come from. */
+typedef enum rinnd_rng_mode {
+    /* The reference's FastRng streams (core/rng.rs:15-80): split choices consumed in depth-first order
+     * per tree (core/tree/builder.rs:172-234) and candidate priorities per Rayon thread block
+     * (core/nndescent/candidates.rs:84-131, 170-213).  Results are bit-identical to the reference on
+     * the FP32 path; the forest is then built depth-first, one workgroup per tree. */
+    RINND_RNG_REFERENCE = 0,
+    /* Counter-based draws keyed by (seed, tree, node) and (seed, iteration, edge): the forest is built
+     * level-synchronously across all trees and candidate sampling needs no stream jump-ahead.  Same
+     * algorithm, different random streams. */
+    RINND_RNG_COUNTER = 1
+} rinnd_rng_mode;
+
+/* Arithmetic of the local-join distance tiles. */
+typedef enum rinnd_join_precision {
+    RINND_JOIN_FP32 = 0, /* every candidate pair in FP32, the reference's AVX2 reduction order */
+    /* BF16 rows on MFMA (||x||^2 + ||y||^2 - 2 x.y with FP32 accumulation) screen every pair; a pair
+     * whose screened distance lies within the proven error bound of its threshold is re-evaluated in
+     * FP32 (reference order), so stored distances and indices equal the FP32 path's. */
+    RINND_JOIN_BF16 = 1
+} rinnd_join_precision;
+
+/* Mirrors NNDescentParams (core/nndescent/mod.rs:21-40) plus the knobs the reference reads implicitly. */
+typedef struct rinnd_params {
+    int64_t n_neighbors;    /* k */
+    int64_t n_trees;        /* RP trees for initialisation */
+    int64_t leaf_size;      /* maximum leaf size */
+    int64_t max_candidates; /* effective value is min(max_candidates, 60, k)  mod.rs:89 */
+    int64_t n_iters;        /* maximum NN-Descent iterations */
+    float delta;            /* convergence: stop when changes <= (delta*k*n) as usize  mod.rs:157 */
+    int32_t angular;        /* angular (cosine/IP) hyperplanes  builder.rs:281-308 */
+    int64_t max_depth;      /* maximum tree depth (200 in the reference builder, index.rs:635) */
+    int32_t verbose;        /* print per-phase timings like mod.rs:169-194 */
+    int32_t n_threads;      /* rayon::current_num_threads() of the reference run being reproduced;
+                               only candidate sampling depends on it (candidates.rs:57-61, 171) */
+    int32_t rng_mode;       /* rinnd_rng_mode */
+    int32_t join_precision; /* rinnd_join_precision */
+} rinnd_params;
+
+/* Per-build counters and phase timings (HIP events on the context stream). */
+typedef struct rinnd_build_stats {
+    int32_t iterations;          /* NN-Descent iterations run */
+    int32_t converged;           /* 1 if stopped by the delta test */
+    int64_t changes[64];         /* successful heap pushes per iteration (first 64) */
+    int64_t join_pairs[64];      /* candidate pairs evaluated per iteration */
+    int64_t join_rescored[64];   /* pairs re-evaluated in FP32 by the BF16 join */
+    int64_t join_updates[64];    /* updates that passed the threshold filter */
+    double ms_forest;            /* RP forest */
+    double ms_init;              /* leaf initialisation */
+    double ms_candidates;        /* candidate sampling, all iterations */
+    double ms_join;              /* local join (distance tiles + filter), all iterations */
+    double ms_apply;             /* ordered heap updates, all iterations */
+    double ms_total;             /* whole nn_descent call */
+    double join_kernel_ms;       /* summed duration of the dominant join kernel launches */
+    int64_t join_kernel_launches;
+    double join_bytes;           /* algorithmic bytes of those launches (gathered rows + lists) */
+} rinnd_build_stats;
+
+typedef struct rinnd_ctx rinnd_ctx;
+typedef struct rinnd_forest rinnd_forest;
+typedef struct rinnd_index rinnd_index;
+
+/* ---- context ----------------------------------------------------------------------------------- */
+/* No reference counterpart (the reference runs on the implicit Rayon pool, core/nndescent/mod.rs:17). */
+rinnd_status rinnd_ctx_create(int device, rinnd_ctx** out);
+void rinnd_ctx_destroy(rinnd_ctx* ctx);
+const char* rinnd_last_error(const rinnd_ctx* ctx);
+/* Library version string (py: `version()`, crates/rinnd/src/lib.rs:1300-1305). */
+const char* rinnd_version(void);
+
+/* ---- reference RNG ----------------------------------------------------------------------------- */
+/* FastRng::new(seed) state (core/rng.rs:22-26: Xoshiro256PlusPlus::seed_from_u64). */
+void rinnd_rng_seed(uint64_t seed, uint64_t state_out[4]);
+/* Advance a FastRng state by `steps` next_u64 draws in O(log steps) (GF(2) jump-ahead). */
+void rinnd_rng_jump(uint64_t state[4], uint64_t steps);
+
+/* ---- distances --------------------------------------------------------------------------------- */
+/* out[p] = distance(data[a[p]], data[b[p]]) for p < n_pairs, bit-exact to the reference's AVX2
+ * kernel of `dist` (core/distance/euclidean.rs:66-110 etc.).  Parity probe for every functor. */
+rinnd_status rinnd_pair_distances(rinnd_ctx* ctx, rinnd_distance dist, const float* data, int64_t n,
+                                  int64_t dim, const int32_t* a, const int32_t* b, int64_t n_pairs,
+                                  float* out);
+
+/* ---- NN-Descent build -------------------------------------------------------------------------- */
+/* Replaces `nn_descent(data, n_points, dim, &distance, &params, &mut rng) -> (NeighborHeap,
+ * Vec<FlatTree>)` (core/nndescent/mod.rs:79-197).  `rng_state` is the caller's FastRng, consumed and
+ * advanced exactly as the reference does (forest seeds first, then each iteration's candidates).
+ * Outputs are the unsorted heap arrays (n x k): indices (-1 = empty), distances (+inf = empty),
+ * flags (1 = new).  `out_forest` (may be NULL) receives the RP forest; free with rinnd_forest_free. */
+rinnd_status rinnd_nn_descent(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim,
+                              rinnd_distance dist, const rinnd_params* params, uint64_t rng_state[4],
+                              int32_t* out_idx, float* out_dist, uint8_t* out_flags,
+                              rinnd_forest** out_forest, rinnd_build_stats* stats);
+
+/* RP forest built alone: `build_rp_forest` (core/tree/builder.rs:89-118). */
+rinnd_status rinnd_build_forest(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim,
+                                int64_t n_trees, int64_t leaf_size, int32_t angular,
+                                int64_t max_depth, int32_t rng_mode, uint64_t rng_state[4],
+                                rinnd_forest** out_forest);
+
+/* FlatTree (core/tree/flat_tree.rs:14-28) accessors; copies go to host buffers. */
+int64_t rinnd_forest_n_trees(const rinnd_forest* f);
+rinnd_status rinnd_forest_tree_info(const rinnd_forest* f, int64_t tree, int64_t* n_nodes,
+                                    int64_t* dim);
+/* hyperplanes n_nodes x dim, offsets n_nodes, children n_nodes x 2, indices n (any may be NULL). */
+rinnd_status rinnd_forest_tree_copy(const rinnd_forest* f, int64_t tree, float* hyperplanes,
+                                    float* offsets, int32_t* children, int32_t* indices);
+void rinnd_forest_free(rinnd_forest* f);
+
+/* `NeighborHeap::sort_all` (core/heap/neighbor_heap.rs:293-320): stable ascending sort of each row by
+ * distance, in place. */
+rinnd_status rinnd_sort_rows(rinnd_ctx* ctx, int32_t* idx, float* dist, uint8_t* flags, int64_t n,
+                             int64_t k);
+
+/* `SearchGraph::from_dense_diversified` (core/graph/search_graph.rs:263-360) on rows sorted by
+ * rinnd_sort_rows.  Writes CSR indptr (n+1) and indices (capacity n*round(mult*k)); returns the
+ * edge count and min_distance. */
+rinnd_status rinnd_search_graph_build(rinnd_ctx* ctx, const int32_t* sorted_idx,
+                                      const float* sorted_dist, const float* data, int64_t n,
+                                      int64_t k, int64_t dim, rinnd_distance dist,
+                                      float diversify_prob, float pruning_degree_multiplier,
+                                      int32_t* out_indptr, int32_t* out_indices, int64_t* out_nnz,
+                                      float* out_min_distance);
+
+/* ---- index (build_with_distance + query) -------------------------------------------------------- */
+/* Mirrors NNDescentBuilder (core/index.rs:449-583) with the optional fields resolved by the caller:
+ * leaf_size/max_candidates/n_iters <= 0 select the reference defaults (index.rs:617-625). */
+typedef struct rinnd_index_params {
+    int64_t n_neighbors;
+    int64_t n_trees;
+    int64_t n_search_trees;
+    int64_t search_tree_leaf_budget;
+    int64_t leaf_size;
+    int64_t max_candidates;
+    int64_t n_iters;
+    float delta;
+    float diversify_prob;
+    float pruning_degree_multiplier;
+    int32_t angular;
+    uint64_t random_seed;
+    int32_t verbose;
+    int32_t n_threads;
+    int32_t rng_mode;
+    int32_t join_precision;
+} rinnd_index_params;
+
+/* `NNDescentBuilder::build_with_distance` (core/index.rs:601-733): nn_descent, sort_all, diversified
+ * search graph, correction, first-tree vertex reordering; data, CSR graph and trees stay resident in
+ * HBM for rinnd_index_query. */
+rinnd_status rinnd_index_build(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim,
+                               rinnd_distance dist, rinnd_correction corr,
+                               const rinnd_index_params* params, rinnd_index** out,
+                               rinnd_build_stats* stats);
+void rinnd_index_free(rinnd_index* index);
+
+/* `NNDescentIndex::query` (core/index.rs:406-439): greedy search per query with FastRng(seed+qid),
+ * ids mapped to input order, distances corrected, rows padded with (-1, +inf).  out_idx/out_dist are
+ * nq x k. */
+rinnd_status rinnd_index_query(rinnd_index* index, const float* queries, int64_t nq, int64_t k,
+                               float epsilon, int32_t* out_idx, float* out_dist);
+/* `query_one_with_workspace` (core/index.rs:253-261): one query with query id 0. */
+rinnd_status rinnd_index_query_one(rinnd_index* index, const float* query, int64_t k, float epsilon,
+                                   int32_t* out_idx, float* out_dist);
+
+/* `neighbor_graph` getter (py:1087-1103): sorted, corrected n x k arrays in input order. */
+rinnd_status rinnd_index_neighbor_graph(const rinnd_index* index, int32_t* out_idx, float* out_dist);
+/* Search graph CSR sizes and `search_graph_original_order` (core/index.rs:137-141, py:1013-1025). */
+rinnd_status rinnd_index_search_graph_info(const rinnd_index* index, int64_t* n, int64_t* nnz,
+                                           float* min_distance);
+rinnd_status rinnd_index_search_graph(const rinnd_index* index, int32_t* indptr, int32_t* indices);
+/* `vertex_order` (core/index.rs:47, 679-685): internal id -> input id. */
+rinnd_status rinnd_index_vertex_order(const rinnd_index* index, int64_t* out);
+/* Retained search trees (core/index.rs:679-680, 712-715) in the internal (reordered) id space. */
+const rinnd_forest* rinnd_index_trees(const rinnd_index* index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RINND_HIP_H */

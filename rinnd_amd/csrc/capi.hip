@@ -1,0 +1,681 @@
+// capi.hip - host orchestration of the gfx950 NN-Descent build and the C ABI (include/rinnd_hip.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "internal.h"
+
+
+namespace rinnd {
+
+// ---- GF(2) jump tables for FastRng ----------------------------------------------------------------
+static void build_jump_tables(std::vector<uint64_t>& tab) {
+    tab.assign((size_t)kJumpLevels * 1024, 0);
+    // M: column c = transition of basis vector e_c; store as rows.
+    std::vector<uint64_t> cols(256 * 4, 0);
+    for (int c = 0; c < 256; ++c) {
+        Xoshiro x;
+        x.s[0] = x.s[1] = x.s[2] = x.s[3] = 0;
+        x.s[c >> 6] = 1ull << (c & 63);
+        (void)x.next_u64();
+        for (int w = 0; w < 4; ++w) cols[c * 4 + w] = x.s[w];
+    }
+    uint64_t* m0 = tab.data();
+    for (int r = 0; r < 256; ++r)
+        for (int c = 0; c < 256; ++c)
+            if ((cols[c * 4 + (r >> 6)] >> (r & 63)) & 1) m0[r * 4 + (c >> 6)] |= 1ull << (c & 63);
+    for (int j = 1; j < kJumpLevels; ++j) {
+        const uint64_t* a = tab.data() + (size_t)(j - 1) * 1024;
+        uint64_t* out = tab.data() + (size_t)j * 1024;
+        for (int r = 0; r < 256; ++r) {
+            uint64_t acc[4] = {0, 0, 0, 0};
+            for (int t = 0; t < 256; ++t)
+                if ((a[r * 4 + (t >> 6)] >> (t & 63)) & 1)
+                    for (int w = 0; w < 4; ++w) acc[w] ^= a[t * 4 + w];
+            for (int w = 0; w < 4; ++w) out[r * 4 + w] = acc[w];
+        }
+    }
+}
+
+const uint64_t* host_jump_tables() {
+    static std::once_flag once;
+    static std::vector<uint64_t> tab;
+    std::call_once(once, [] { build_jump_tables(tab); });
+    return tab.data();
+}
+
+// ---- pointers ------------------------------------------------------------------------------------
+static bool is_device_ptr(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.isManaged;
+}
+
+// Device view of a caller array: the pointer itself if it is device memory, else a staged copy.
+template <class T>
+struct InView {
+    const T* d = nullptr;
+    DBuf<T> own;
+    InView(hipStream_t s, const T* p, size_t count) {
+        if (count == 0) return;
+        if (is_device_ptr(p)) {
+            d = p;
+        } else {
+            own.alloc(count);
+            HIP_TRY(hipMemcpyAsync(own.p, p, count * sizeof(T), hipMemcpyHostToDevice, s));
+            d = own.p;
+        }
+    }
+};
+template <class T>
+static void copy_out(hipStream_t s, T* dst, const T* src, size_t count) {
+    if (count == 0 || !dst) return;
+    HIP_TRY(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDefault, s));
+}
+
+static double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0;
+    }
+    return (double)ms;
+}
+
+static double wall_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---- padded device copy of the data --------------------------------------------------------------
+struct DataDev {
+    DBuf<float> X;
+    int64_t ld = 0;
+};
+static void make_data(hipStream_t s, const float* data, int64_t n, int dim, DataDev& out) {
+    out.ld = round_up(std::max(dim, 1), 8);
+    out.X.alloc((size_t)std::max<int64_t>(n, 1) * out.ld);
+    InView<float> in(s, data, (size_t)n * dim);
+    launch_pad_rows(s, in.d, n, dim, out.X.p, out.ld);
+    HIP_TRY(hipStreamSynchronize(s));  // the staged copy dies with `in`
+}
+
+// ---- forest ------------------------------------------------------------------------------------------
+static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t n, int dim, int64_t n_trees,
+                             int64_t leaf_size, int angular, int64_t max_depth, int rng_mode,
+                             Xoshiro& main, rinnd_forest& F) {
+    hipStream_t s = ctx->stream;
+    F.dim = dim;
+    F.n = n;
+    F.trees.clear();
+    F.trees.resize(n_trees);
+    if (n_trees == 0) return;
+    RINND_CHECK_ARG(max_depth <= 500, "max_depth above 500 is not supported by the GPU tree builder");
+    RINND_CHECK_ARG(dim <= 8192, "dim above 8192 is not supported by the GPU tree builder");
+    std::vector<uint64_t> states(n_trees * 4);
+    for (int64_t t = 0; t < n_trees; ++t) {  // build_rp_forest seeds (builder.rs:100)
+        const uint64_t seed = main.next_u64();
+        Xoshiro r = xoshiro_seed(rng_mode == RINND_RNG_COUNTER ? mix64(seed) : seed);
+        for (int w = 0; w < 4; ++w) states[t * 4 + w] = r.s[w];
+    }
+    const int64_t ls = std::max<int64_t>(leaf_size, 1);
+    int64_t cap = std::min<int64_t>(2 * n + 1, 8 * ceil_div(n, ls) + 1024);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        for (auto& T : F.trees) {
+            T.cap_nodes = cap;
+            T.hyper.alloc((size_t)cap * dim);
+            T.offsets.alloc(cap);
+            T.children.alloc((size_t)cap * 2);
+            T.indices.alloc(std::max<int64_t>(n, 1));
+            HIP_TRY(hipMemsetAsync(T.hyper.p, 0, T.hyper.bytes(), s));
+            HIP_TRY(hipMemsetAsync(T.offsets.p, 0, T.offsets.bytes(), s));
+        }
+        const size_t N1 = (size_t)(n + 1);
+        DBuf<uint8_t> side(n_trees * N1);
+        DBuf<int32_t> rankR(n_trees * N1), PQ(n_trees * 2 * N1), tmp(n_trees * N1);
+        std::vector<float*> hp(n_trees), of(n_trees);
+        std::vector<int32_t*> chp(n_trees), ixp(n_trees);
+        std::vector<int64_t> caps(n_trees, cap);
+        for (int64_t t = 0; t < n_trees; ++t) {
+            hp[t] = F.trees[t].hyper.p;
+            of[t] = F.trees[t].offsets.p;
+            chp[t] = F.trees[t].children.p;
+            ixp[t] = F.trees[t].indices.p;
+        }
+        DBuf<float*> d_hp(n_trees), d_of(n_trees);
+        DBuf<int32_t*> d_ch(n_trees), d_ix(n_trees);
+        DBuf<int64_t> d_cap(n_trees), d_nn(n_trees);
+        DBuf<int32_t> d_status(n_trees);
+        DBuf<uint64_t> d_states(n_trees * 4);
+        HIP_TRY(hipMemcpyAsync(d_hp.p, hp.data(), n_trees * sizeof(float*), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_of.p, of.data(), n_trees * sizeof(float*), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_ch.p, chp.data(), n_trees * sizeof(int32_t*), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_ix.p, ixp.data(), n_trees * sizeof(int32_t*), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_cap.p, caps.data(), n_trees * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_states.p, states.data(), n_trees * 32, hipMemcpyHostToDevice, s));
+        ForestDFSArgs a{};
+        a.X = DX;
+        a.ld = Dld;
+        a.dim = dim;
+        a.n = n;
+        a.leaf_size = leaf_size;
+        a.max_depth = max_depth;
+        a.angular = angular;
+        a.n_trees = (int)n_trees;
+        a.tree_states = d_states.p;
+        a.hyper = d_hp.p;
+        a.offsets = d_of.p;
+        a.children = d_ch.p;
+        a.indices = d_ix.p;
+        a.cap_nodes = d_cap.p;
+        a.n_nodes_out = d_nn.p;
+        a.status = d_status.p;
+        a.side = side.p;
+        a.rankR = rankR.p;
+        a.PQ = PQ.p;
+        a.tmp = tmp.p;
+        launch_forest_dfs(s, a);
+        HIP_TRY(hipGetLastError());
+        std::vector<int64_t> nn(n_trees);
+        std::vector<int32_t> st(n_trees);
+        HIP_TRY(hipMemcpyAsync(nn.data(), d_nn.p, n_trees * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(st.data(), d_status.p, n_trees * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        bool overflow = false;
+        for (int64_t t = 0; t < n_trees; ++t) {
+            if (st[t] == 2) throw Error{RINND_ERR_INTERNAL, "RP tree stack overflow"};
+            if (st[t] == 1) overflow = true;
+            F.trees[t].n_nodes = nn[t];
+        }
+        if (!overflow) return;
+        cap = 2 * n + 1;  // every split leaves two non-empty children: at most 2n-1 nodes
+    }
+    throw Error{RINND_ERR_INTERNAL, "RP tree node capacity exceeded"};
+}
+
+// ---- NN-Descent ---------------------------------------------------------------------------------
+struct Pipeline {
+    DBuf<URec> U;
+    DBuf<unsigned long long> U_count;
+    DBuf<unsigned long long> cnt, cur;
+    DBuf<uint64_t> off;
+    DBuf<SegRec> seg, seg2;
+    DBuf<int32_t> big_list, big_count, overflow;
+    DBuf<unsigned long long> changes;
+    DBuf<char> scan_tmp;
+    size_t scan_bytes = 0;
+    int64_t n = 0;
+    void init(int64_t n_, uint64_t u_cap) {
+        n = n_;
+        U.ensure(std::max<uint64_t>(u_cap, 1));
+        seg.ensure(2 * std::max<uint64_t>(u_cap, 1));
+        seg2.ensure(2 * std::max<uint64_t>(u_cap, 1));
+        U_count.ensure(1);
+        cnt.ensure(n + 1);
+        cur.ensure(n + 1);
+        off.ensure(n + 1);
+        big_list.ensure(std::max<int64_t>(n, 1));
+        big_count.ensure(1);
+        overflow.ensure(1);
+        changes.ensure(1);
+        size_t b = scan_u64_tmp_bytes(n + 1);
+        if (b > scan_bytes) {
+            scan_tmp.alloc(b);
+            scan_bytes = b;
+        }
+    }
+    void begin(hipStream_t s) {
+        HIP_TRY(hipMemsetAsync(cnt.p, 0, (n + 1) * 8, s));
+        HIP_TRY(hipMemsetAsync(U_count.p, 0, 8, s));
+        HIP_TRY(hipMemsetAsync(overflow.p, 0, 4, s));
+    }
+    // counts -> offsets -> grouped records
+    void group(hipStream_t s) {
+        scan_u64_exclusive(s, cnt.p, off.p, n + 1, scan_tmp.p, scan_bytes);
+        HIP_TRY(hipMemsetAsync(cur.p, 0, (n + 1) * 8, s));
+        launch_scatter(s, U.p, U_count.p, off.p, cur.p, seg.p);
+    }
+    void check_overflow(hipStream_t s) {
+        int32_t ov = 0;
+        HIP_TRY(hipMemcpyAsync(&ov, overflow.p, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (ov) throw Error{RINND_ERR_INTERNAL, "update buffer overflow"};
+    }
+};
+
+struct HeapDev {
+    DBuf<int32_t> idx;
+    DBuf<float> dist;
+    DBuf<uint8_t> flags;
+};
+
+static uint64_t d2h_u64(hipStream_t s, const void* p) {
+    uint64_t v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return v;
+}
+
+static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t n, int dim, int metric,
+                           const rinnd_params& p, Xoshiro& main, int32_t* h_idx, float* h_dist,
+                           uint8_t* h_flags, rinnd_forest& F, rinnd_build_stats* stats) {
+    hipStream_t s = ctx->stream;
+    const int k = (int)p.n_neighbors;
+    const int C = (int)std::min<int64_t>(std::min<int64_t>(p.max_candidates, 60), p.n_neighbors);
+    const double t_start = wall_ms();
+    hipEvent_t* ev = ctx->ev;
+    rinnd_build_stats st{};
+
+    HIP_TRY(hipEventRecord(ev[0], s));
+    build_forest_dev(ctx, DX, Dld, n, dim, p.n_trees, p.leaf_size, p.angular, p.max_depth, p.rng_mode, main, F);
+    HIP_TRY(hipEventRecord(ev[1], s));
+
+    launch_fill_heap(s, h_idx, h_dist, h_flags, (int64_t)n * k);
+
+    // ---- initialize_from_leaves (mod.rs:203-300), tree by tree ----
+    Pipeline PL;
+    const uint64_t join_rows_cap = (uint64_t)n;  // whole iteration in one generation pass
+    uint64_t u_cap = std::max<uint64_t>((uint64_t)2 * n * k, join_rows_cap * (uint64_t)C * 2 * C);
+    {
+        DBuf<int32_t> mark(n + 1), end_at(n + 1);
+        DBuf<int64_t> rank(2 * (n + 1)), leaves(2 * (n + 1));
+        DBuf<uint64_t> pc(n + 2), pb(n + 2);
+        size_t lt_bytes = std::max(scan_u64_tmp_bytes(n + 2), (size_t)16);
+        DBuf<char> ltmp(lt_bytes);
+        std::vector<int64_t> n_leaves(F.trees.size()), n_pairs(F.trees.size());
+        uint64_t max_pairs = 0;
+        for (size_t t = 0; t < F.trees.size(); ++t) {  // sizes first, to size the buffers once
+            launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
+                             leaves.p, ltmp.p, lt_bytes);
+            n_leaves[t] = (int64_t)d2h_u64(s, rank.p + n);
+            launch_leaf_pair_base(s, leaves.p, n_leaves[t], pc.p, pb.p, ltmp.p, lt_bytes);
+            n_pairs[t] = (int64_t)d2h_u64(s, pb.p + n_leaves[t]);
+            max_pairs = std::max<uint64_t>(max_pairs, (uint64_t)n_pairs[t]);
+        }
+        u_cap = std::max(u_cap, max_pairs);
+        PL.init(n, u_cap);
+        for (size_t t = 0; t < F.trees.size(); ++t) {
+            launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
+                             leaves.p, ltmp.p, lt_bytes);
+            launch_leaf_pair_base(s, leaves.p, n_leaves[t], pc.p, pb.p, ltmp.p, lt_bytes);
+            PL.begin(s);
+            launch_leaf_pairs(s, metric, DX, Dld, dim, F.trees[t].indices.p, leaves.p, n_leaves[t],
+                              pb.p, h_dist, k, PL.U.p, PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
+            PL.group(s);
+            launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
+                              PL.changes.p, PL.big_list.p, PL.big_count.p);
+        }
+        PL.check_overflow(s);
+    }
+    HIP_TRY(hipEventRecord(ev[2], s));
+
+    // ---- iterations (mod.rs:137-167) ----
+    DBuf<int32_t> new_idx((size_t)n * C), old_idx((size_t)n * C);
+    DBuf<uint64_t> rvp(n + 1);
+    DBuf<unsigned long long> rvc(n + 1);
+    DBuf<float> prio((size_t)2 * n * k + 1);
+    DBuf<RngStream> d_streams(64);
+    DBuf<int64_t> d_bbase(64);
+    DBuf<uint64_t> d_jt;
+    const int T = std::max(1, (int)p.n_threads);
+    const int T_eff = (n >= 256 && T > 1) ? T : 1;
+    RINND_CHECK_ARG(T_eff <= 64, "n_threads above 64 is not supported");
+    const int64_t threshold = (int64_t)(p.delta * (float)p.n_neighbors * (float)n);
+    double ms_cand = 0, ms_join = 0, ms_apply = 0;
+    int it = 0;
+    bool converged = false;
+    for (; it < p.n_iters; ++it) {
+        HIP_TRY(hipEventRecord(ev[3], s));
+        // CandidateSets::build_from_graph
+        launch_row_valid_counts(s, h_idx, n, k, (uint64_t*)rvc.p);
+        scan_u64_exclusive(s, rvc.p, rvp.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
+        int mode = 2;
+        uint64_t counter_key = 0;
+        int64_t block_rows = n;
+        if (p.rng_mode == RINND_RNG_REFERENCE) {
+            std::vector<RngStream> streams;
+            if (T_eff == 1) {
+                mode = 0;
+                const uint64_t V = d2h_u64(s, rvp.p + n);
+                RngStream r{};
+                for (int w = 0; w < 4; ++w) r.s[w] = main.s[w];
+                r.count = (int64_t)(2 * V);
+                r.out_offset = 0;
+                streams.push_back(r);
+                xoshiro_jump(host_jump_tables(), main.s, 2 * V);
+            } else {
+                mode = 1;
+                block_rows = ceil_div(n, T_eff);
+                std::vector<uint64_t> hv(n + 1);
+                HIP_TRY(hipMemcpyAsync(hv.data(), rvp.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                std::vector<int64_t> bbase(T_eff);
+                int64_t acc = 0;
+                for (int t = 0; t < T_eff; ++t) {
+                    const uint64_t seed = main.next_u64();  // candidates.rs:171
+                    const int64_t r0 = std::min<int64_t>(t * block_rows, n);
+                    const int64_t r1 = std::min<int64_t>((t + 1) * block_rows, n);
+                    RngStream r{};
+                    Xoshiro x = xoshiro_seed(seed);
+                    for (int w = 0; w < 4; ++w) r.s[w] = x.s[w];
+                    r.count = (int64_t)(hv[r1] - hv[r0]);
+                    r.out_offset = acc;
+                    bbase[t] = acc;
+                    acc += r.count;
+                    streams.push_back(r);
+                }
+                HIP_TRY(hipMemcpyAsync(d_bbase.p, bbase.data(), T_eff * 8, hipMemcpyHostToDevice, s));
+            }
+            if (!d_jt.p) {
+                d_jt.alloc((size_t)kJumpLevels * 1024);
+                HIP_TRY(hipMemcpyAsync(d_jt.p, host_jump_tables(), d_jt.bytes(), hipMemcpyHostToDevice, s));
+            }
+            int64_t maxc = 0;
+            for (auto& r : streams) maxc = std::max(maxc, r.count);
+            HIP_TRY(hipMemcpyAsync(d_streams.p, streams.data(), streams.size() * sizeof(RngStream),
+                                   hipMemcpyHostToDevice, s));
+            launch_rng_streams(s, d_jt.p, d_streams.p, (int)streams.size(), maxc, prio.p);
+            HIP_TRY(hipStreamSynchronize(s));  // host vectors above go out of scope
+        } else {
+            counter_key = main.next_u64();
+        }
+        PL.begin(s);
+        launch_cand_gen(s, h_idx, h_flags, n, k, rvp.p, prio.p, mode, T_eff, block_rows, d_bbase.p,
+                        counter_key, PL.U.p, PL.cnt.p);
+        {
+            const unsigned long long total = (unsigned long long)2 * n * k;
+            HIP_TRY(hipMemcpyAsync(PL.U_count.p, &total, 8, hipMemcpyHostToDevice, s));
+            PL.group(s);
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        launch_fill_i32(s, new_idx.p, -1, (int64_t)n * C);
+        launch_fill_i32(s, old_idx.p, -1, (int64_t)n * C);
+        launch_cand_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, new_idx.p, old_idx.p, C, PL.big_list.p,
+                          PL.big_count.p);
+        launch_mark_old(s, h_idx, h_flags, new_idx.p, n, k, C);
+        HIP_TRY(hipEventRecord(ev[4], s));
+        // update_iteration: generation + ordered apply
+        HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
+        PL.begin(s);
+        launch_join_fp32(s, metric, DX, Dld, dim, new_idx.p, old_idx.p, C, 0, n, h_dist, k, PL.U.p,
+                         PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
+        HIP_TRY(hipEventRecord(ev[5], s));
+        PL.group(s);
+        launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
+                          PL.changes.p, PL.big_list.p, PL.big_count.p);
+        HIP_TRY(hipEventRecord(ev[6], s));
+        uint64_t ch = 0, nup = 0;
+        HIP_TRY(hipMemcpyAsync(&ch, PL.changes.p, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&nup, PL.U_count.p, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        PL.check_overflow(s);
+        ms_cand += elapsed_ms(ev[3], ev[4]);
+        const double jm = elapsed_ms(ev[4], ev[5]);
+        ms_join += jm;
+        ms_apply += elapsed_ms(ev[5], ev[6]);
+        if (it < 64) {
+            st.changes[it] = (int64_t)ch;
+            st.join_updates[it] = (int64_t)nup;
+        }
+        st.join_kernel_ms += jm;
+        st.join_kernel_launches += 1;
+        if (p.verbose) std::printf("Iteration %d: %llu updates\n", it + 1, (unsigned long long)ch);
+        if ((int64_t)ch <= threshold) {
+            converged = true;
+            ++it;
+            break;
+        }
+    }
+    st.iterations = it;
+    st.converged = converged ? 1 : 0;
+    st.ms_forest = elapsed_ms(ev[0], ev[1]);
+    st.ms_init = elapsed_ms(ev[1], ev[2]);
+    st.ms_candidates = ms_cand;
+    st.ms_join = ms_join;
+    st.ms_apply = ms_apply;
+    st.ms_total = wall_ms() - t_start;
+    if (p.verbose)
+        std::printf("forest %.3f ms, init %.3f ms, candidates %.3f ms, join %.3f ms, apply %.3f ms\n",
+                    st.ms_forest, st.ms_init, ms_cand, ms_join, ms_apply);
+    if (stats) *stats = st;
+}
+
+void make_padded(hipStream_t s, const float* data, int64_t n, int dim, DBuf<float>& X, int64_t& ld) {
+    DataDev D;
+    make_data(s, data, n, dim, D);
+    X = std::move(D.X);
+    ld = D.ld;
+}
+bool device_pointer(const void* p) { return is_device_ptr(p); }
+static void validate_params(const rinnd_params* p, int64_t n, int64_t dim);
+void index_nn_descent(rinnd_ctx* ctx, const float* X, int64_t ld, int64_t n, int dim, int metric,
+                      const rinnd_params& p, uint64_t state[4], int32_t* idx, float* dist, uint8_t* flags,
+                      rinnd_forest& F, rinnd_build_stats* stats) {
+    validate_params(&p, n, dim);
+    Xoshiro main;
+    for (int w = 0; w < 4; ++w) main.s[w] = state[w];
+    nn_descent_dev(ctx, X, ld, n, dim, metric, p, main, idx, dist, flags, F, stats);
+    for (int w = 0; w < 4; ++w) state[w] = main.s[w];
+}
+
+static void validate_params(const rinnd_params* p, int64_t n, int64_t dim) {
+    RINND_CHECK_ARG(p, "params is NULL");
+    RINND_CHECK_ARG(n >= 0 && dim > 0, "invalid data shape");
+    RINND_CHECK_ARG(n < ((int64_t)1 << 31), "more than 2^31-1 points");
+    RINND_CHECK_ARG(p->n_neighbors >= 1 && p->n_neighbors <= 256, "n_neighbors must be in [1, 256]");
+    RINND_CHECK_ARG(p->n_trees >= 0 && p->n_trees <= 256, "n_trees must be in [0, 256]");
+    RINND_CHECK_ARG(p->leaf_size >= 1, "leaf_size must be positive");
+    RINND_CHECK_ARG(p->max_candidates >= 1, "max_candidates must be positive");
+    RINND_CHECK_ARG(p->n_iters >= 0, "n_iters must be non-negative");
+    RINND_CHECK_ARG(p->rng_mode == RINND_RNG_REFERENCE || p->rng_mode == RINND_RNG_COUNTER,
+                    "unknown rng_mode");
+    RINND_CHECK_ARG(p->join_precision == RINND_JOIN_FP32 || p->join_precision == RINND_JOIN_BF16,
+                    "unknown join_precision");
+}
+
+}  // namespace rinnd
+
+using namespace rinnd;
+
+// =================================================================================================
+// C ABI
+// =================================================================================================
+template <class F>
+static rinnd_status guarded(rinnd_ctx* ctx, F&& f) {
+    if (!ctx) return RINND_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        f();
+        ctx->last_error.clear();
+        return RINND_OK;
+    } catch (const Error& e) {
+        ctx->last_error = e.msg;
+        (void)hipGetLastError();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "host allocation failed";
+        return RINND_ERR_OOM;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return RINND_ERR_INTERNAL;
+    }
+}
+
+extern "C" {
+
+const char* rinnd_version(void) { return "rinnd-mi355x 0.1.0"; }
+
+rinnd_status rinnd_ctx_create(int device, rinnd_ctx** out) {
+    if (!out) return RINND_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return RINND_ERR_HIP;
+    }
+    if (device < 0 || device >= count) return RINND_ERR_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return RINND_ERR_HIP;
+    auto* c = new rinnd_ctx;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RINND_ERR_HIP;
+    }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return RINND_ERR_HIP;
+    *out = c;
+    return RINND_OK;
+}
+
+void rinnd_ctx_destroy(rinnd_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    ctx->jump_tables.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* rinnd_last_error(const rinnd_ctx* ctx) {
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+void rinnd_rng_seed(uint64_t seed, uint64_t state_out[4]) {
+    Xoshiro x = xoshiro_seed(seed);
+    for (int w = 0; w < 4; ++w) state_out[w] = x.s[w];
+}
+
+void rinnd_rng_jump(uint64_t state[4], uint64_t steps) { xoshiro_jump(host_jump_tables(), state, steps); }
+
+rinnd_status rinnd_pair_distances(rinnd_ctx* ctx, rinnd_distance dist, const float* data, int64_t n,
+                                  int64_t dim, const int32_t* a, const int32_t* b, int64_t n_pairs,
+                                  float* out) {
+    return guarded(ctx, [&] {
+        RINND_CHECK_ARG(n > 0 && dim > 0 && n_pairs >= 0, "invalid sizes");
+        RINND_CHECK_ARG((int)dist >= 0 && (int)dist <= 6, "unknown distance");
+        hipStream_t s = ctx->stream;
+        DataDev D;
+        make_data(s, data, n, (int)dim, D);
+        InView<int32_t> da(s, a, n_pairs), db(s, b, n_pairs);
+        DBuf<float> o(std::max<int64_t>(n_pairs, 1));
+        launch_pair_distances(s, (int)dist, D.X.p, D.ld, (int)dim, da.d, db.d, n_pairs, o.p);
+        HIP_TRY(hipGetLastError());
+        copy_out(s, out, o.p, n_pairs);
+        HIP_TRY(hipStreamSynchronize(s));
+    });
+}
+
+rinnd_status rinnd_build_forest(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim,
+                                int64_t n_trees, int64_t leaf_size, int32_t angular,
+                                int64_t max_depth, int32_t rng_mode, uint64_t rng_state[4],
+                                rinnd_forest** out_forest) {
+    return guarded(ctx, [&] {
+        RINND_CHECK_ARG(out_forest && rng_state, "null output");
+        RINND_CHECK_ARG(n >= 1 && dim >= 1 && leaf_size >= 1, "invalid sizes");
+        hipStream_t s = ctx->stream;
+        DataDev D;
+        make_data(s, data, n, (int)dim, D);
+        Xoshiro main;
+        for (int w = 0; w < 4; ++w) main.s[w] = rng_state[w];
+        auto F = std::make_unique<rinnd_forest>();
+        build_forest_dev(ctx, D.X.p, D.ld, n, (int)dim, n_trees, leaf_size, angular, max_depth, rng_mode, main, *F);
+        for (int w = 0; w < 4; ++w) rng_state[w] = main.s[w];
+        *out_forest = F.release();
+    });
+}
+
+int64_t rinnd_forest_n_trees(const rinnd_forest* f) { return f ? (int64_t)f->trees.size() : 0; }
+
+rinnd_status rinnd_forest_tree_info(const rinnd_forest* f, int64_t tree, int64_t* n_nodes,
+                                    int64_t* dim) {
+    if (!f || tree < 0 || tree >= (int64_t)f->trees.size()) return RINND_ERR_INVALID_ARG;
+    if (n_nodes) *n_nodes = f->trees[tree].n_nodes;
+    if (dim) *dim = f->dim;
+    return RINND_OK;
+}
+
+rinnd_status rinnd_forest_tree_copy(const rinnd_forest* f, int64_t tree, float* hyperplanes,
+                                    float* offsets, int32_t* children, int32_t* indices) {
+    if (!f || tree < 0 || tree >= (int64_t)f->trees.size()) return RINND_ERR_INVALID_ARG;
+    const TreeDev& T = f->trees[tree];
+    const size_t nn = (size_t)T.n_nodes;
+    if (hyperplanes && nn) (void)hipMemcpy(hyperplanes, T.hyper.p, nn * f->dim * 4, hipMemcpyDefault);
+    if (offsets && nn) (void)hipMemcpy(offsets, T.offsets.p, nn * 4, hipMemcpyDefault);
+    if (children && nn) (void)hipMemcpy(children, T.children.p, nn * 8, hipMemcpyDefault);
+    if (indices && f->n) (void)hipMemcpy(indices, T.indices.p, f->n * 4, hipMemcpyDefault);
+    return hipGetLastError() == hipSuccess ? RINND_OK : RINND_ERR_HIP;
+}
+
+void rinnd_forest_free(rinnd_forest* f) { delete f; }
+
+rinnd_status rinnd_nn_descent(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim,
+                              rinnd_distance dist, const rinnd_params* params, uint64_t rng_state[4],
+                              int32_t* out_idx, float* out_dist, uint8_t* out_flags,
+                              rinnd_forest** out_forest, rinnd_build_stats* stats) {
+    return guarded(ctx, [&] {
+        validate_params(params, n, dim);
+        RINND_CHECK_ARG(rng_state, "rng_state is NULL");
+        RINND_CHECK_ARG((int)dist >= 0 && (int)dist <= 6, "unknown distance");
+        hipStream_t s = ctx->stream;
+        auto F = std::make_unique<rinnd_forest>();
+        Xoshiro main;
+        for (int w = 0; w < 4; ++w) main.s[w] = rng_state[w];
+        const int64_t k = params->n_neighbors;
+        if (n == 0) {
+            F->dim = dim;
+            if (out_forest) *out_forest = F.release();
+            if (stats) *stats = rinnd_build_stats{};
+            return;
+        }
+        DataDev D;
+        make_data(s, data, n, (int)dim, D);
+        HeapDev H;
+        H.idx.alloc((size_t)n * k);
+        H.dist.alloc((size_t)n * k);
+        H.flags.alloc((size_t)n * k);
+        nn_descent_dev(ctx, D.X.p, D.ld, n, (int)dim, (int)dist, *params, main, H.idx.p, H.dist.p, H.flags.p, *F,
+                       stats);
+        copy_out(s, out_idx, H.idx.p, (size_t)n * k);
+        copy_out(s, out_dist, H.dist.p, (size_t)n * k);
+        copy_out(s, out_flags, H.flags.p, (size_t)n * k);
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int w = 0; w < 4; ++w) rng_state[w] = main.s[w];
+        if (out_forest) *out_forest = F.release();
+    });
+}
+
+rinnd_status rinnd_sort_rows(rinnd_ctx* ctx, int32_t* idx, float* dist, uint8_t* flags, int64_t n,
+                             int64_t k) {
+    return guarded(ctx, [&] {
+        RINND_CHECK_ARG(n >= 0 && k >= 1 && k <= 256, "invalid sizes");
+        hipStream_t s = ctx->stream;
+        const size_t cnt = (size_t)n * k;
+        DBuf<int32_t> di(cnt);
+        DBuf<float> dd(cnt);
+        DBuf<uint8_t> df(cnt);
+        HIP_TRY(hipMemcpyAsync(di.p, idx, cnt * 4, hipMemcpyDefault, s));
+        HIP_TRY(hipMemcpyAsync(dd.p, dist, cnt * 4, hipMemcpyDefault, s));
+        if (flags) HIP_TRY(hipMemcpyAsync(df.p, flags, cnt, hipMemcpyDefault, s));
+        else HIP_TRY(hipMemsetAsync(df.p, 0, cnt, s));
+        launch_sort_rows(s, di.p, dd.p, df.p, n, (int)k);
+        HIP_TRY(hipGetLastError());
+        copy_out(s, idx, di.p, cnt);
+        copy_out(s, dist, dd.p, cnt);
+        if (flags) copy_out(s, flags, df.p, cnt);
+        HIP_TRY(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,382 @@
+// index.hip - NNDescentBuilder::build_with_distance (core/index.rs:601-733) and NNDescentIndex::query
+// (core/index.rs:157-202, 406-439) with everything resident in HBM.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "graph.h"
+#include "search.h"
+
+namespace rinnd {
+// defined in capi.hip
+void index_nn_descent(rinnd_ctx* ctx, const float* data_dev_padded, int64_t ld, int64_t n, int dim,
+                      int metric, const rinnd_params& p, uint64_t state[4], int32_t* idx, float* dist,
+                      uint8_t* flags, rinnd_forest& F, rinnd_build_stats* stats);
+void make_padded(hipStream_t s, const float* data, int64_t n, int dim, DBuf<float>& X, int64_t& ld);
+bool device_pointer(const void* p);
+}  // namespace rinnd
+
+using namespace rinnd;
+
+struct rinnd_index {
+    rinnd_ctx* ctx = nullptr;
+    int64_t n = 0, dim = 0, k = 0, ld = 0;
+    int metric = 0, corr = 0;
+    DBuf<float> X;  // reordered, padded
+    DBuf<int32_t> indptr, indices;
+    int64_t nnz = 0;
+    float min_distance = 0.f;
+    rinnd_forest trees;  // retained search trees, ids in the reordered space
+    std::vector<DBuf<float>> hyper_padded;
+    DBuf<const float*> d_hyper, d_off;
+    DBuf<const int32_t*> d_ch, d_tix;
+    DBuf<int64_t> vertex_order, inv;
+    DBuf<int32_t> nb_idx;
+    DBuf<float> nb_dist;
+    int64_t leaf_budget = 1;
+    uint64_t seed = 42;
+    // query workspace (grown on demand)
+    int slots = 0;
+    DBuf<uint8_t> visited;
+    DBuf<int32_t> epochs;
+    DBuf<float> spill_d;
+    DBuf<int32_t> spill_i;
+    int64_t seed_cap = 0;
+    DBuf<int32_t> status;
+    DBuf<float> qbuf;
+    DBuf<int32_t> obuf_i;
+    DBuf<float> obuf_d;
+};
+
+namespace rinnd {
+
+static void index_build_impl(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim, int metric, int corr,
+                             const rinnd_index_params& ip, rinnd_index& ix, rinnd_build_stats* stats) {
+    hipStream_t s = ctx->stream;
+    const int64_t k = ip.n_neighbors;
+    rinnd_params p{};
+    p.n_neighbors = k;
+    p.n_trees = ip.n_trees;
+    p.leaf_size = ip.leaf_size > 0 ? ip.leaf_size : std::max<int64_t>(60, std::min<int64_t>(256, 5 * k));
+    p.max_candidates = ip.max_candidates > 0 ? ip.max_candidates : std::min<int64_t>(k, 60);
+    p.n_iters = ip.n_iters > 0 ? ip.n_iters
+                               : std::max<int64_t>(5, (int64_t)std::ceil(std::log2((double)std::max<int64_t>(n, 1))));
+    p.delta = ip.delta;
+    p.angular = ip.angular;
+    p.max_depth = 200;
+    p.verbose = ip.verbose;
+    p.n_threads = ip.n_threads;
+    p.rng_mode = ip.rng_mode;
+    p.join_precision = ip.join_precision;
+    ix.ctx = ctx;
+    ix.n = n;
+    ix.dim = dim;
+    ix.k = k;
+    ix.metric = metric;
+    ix.corr = corr;
+    ix.leaf_budget = std::max<int64_t>(1, ip.search_tree_leaf_budget);
+    ix.seed = ip.random_seed;
+    RINND_CHECK_ARG(ix.leaf_budget <= search_max_budget(), "search_tree_leaf_budget above 64");
+
+    DBuf<float> X0;
+    int64_t ld = 0;
+    make_padded(s, data, n, (int)dim, X0, ld);
+    ix.ld = ld;
+    uint64_t st[4];
+    rinnd_rng_seed(ip.random_seed, st);  // FastRng::new(random_seed) (index.rs:639)
+    DBuf<int32_t> hidx((size_t)n * k);
+    DBuf<float> hdist((size_t)n * k);
+    DBuf<uint8_t> hflags((size_t)n * k);
+    rinnd_forest forest;
+    index_nn_descent(ctx, X0.p, ld, n, (int)dim, metric, p, st, hidx.p, hdist.p, hflags.p, forest, stats);
+    launch_sort_rows(s, hidx.p, hdist.p, hflags.p, n, (int)k);  // sort_all (index.rs:653)
+    SearchGraphDev G;
+    build_search_graph_dev(s, metric, X0.p, ld, (int)dim, hidx.p, hdist.p, n, (int)k, ip.diversify_prob,
+                           ip.pruning_degree_multiplier, ip.random_seed, G);
+    ix.min_distance = G.min_distance;
+    launch_correct(s, hdist.p, n * k, corr);  // index.rs:669-673
+    ix.nb_idx = std::move(hidx);
+    ix.nb_dist = std::move(hdist);
+    // retained trees and the first tree's leaf order (index.rs:679-715)
+    const int64_t nst = std::min<int64_t>(ip.n_search_trees, (int64_t)forest.trees.size());
+    ix.trees.dim = dim;
+    ix.trees.n = n;
+    for (int64_t t = 0; t < nst; ++t) ix.trees.trees.push_back(std::move(forest.trees[t]));
+    if (nst > 0) {
+        vertex_order_from_tree(s, ix.trees.trees[0].indices.p, n, ix.vertex_order, ix.inv);
+    } else {
+        std::vector<int64_t> h(n);
+        for (int64_t i = 0; i < n; ++i) h[i] = i;
+        ix.vertex_order.alloc(std::max<int64_t>(n, 1));
+        ix.inv.alloc(std::max<int64_t>(n, 1));
+        HIP_TRY(hipMemcpyAsync(ix.vertex_order.p, h.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(ix.inv.p, h.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    ix.X.alloc((size_t)std::max<int64_t>(n, 1) * ld);
+    gather_rows(s, X0.p, ld, ix.vertex_order.p, n, ix.X.p);
+    search_graph_csr(s, G, ix.vertex_order.p, ix.inv.p, ix.indptr, ix.indices, ix.nnz);
+    // trees: ids remapped (remap_indices, flat_tree.rs:329-334); hyperplanes re-laid with stride ld
+    std::vector<const float*> hp, of;
+    std::vector<const int32_t*> ch, tx;
+    for (auto& T : ix.trees.trees) {
+        remap_ids(s, T.indices.p, n, ix.inv.p);
+        DBuf<float> h((size_t)std::max<int64_t>(T.n_nodes, 1) * ld);
+        HIP_TRY(hipMemsetAsync(h.p, 0, h.bytes(), s));
+        if (T.n_nodes > 0)
+            HIP_TRY(hipMemcpy2DAsync(h.p, ld * 4, T.hyper.p, dim * 4, dim * 4, T.n_nodes, hipMemcpyDeviceToDevice, s));
+        hp.push_back(h.p);
+        of.push_back(T.offsets.p);
+        ch.push_back(T.children.p);
+        tx.push_back(T.indices.p);
+        ix.hyper_padded.push_back(std::move(h));
+    }
+    const size_t nt = std::max<size_t>(hp.size(), 1);
+    ix.d_hyper.alloc(nt);
+    ix.d_off.alloc(nt);
+    ix.d_ch.alloc(nt);
+    ix.d_tix.alloc(nt);
+    if (!hp.empty()) {
+        HIP_TRY(hipMemcpyAsync(ix.d_hyper.p, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(ix.d_off.p, of.data(), of.size() * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(ix.d_ch.p, ch.data(), ch.size() * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(ix.d_tix.p, tx.data(), tx.size() * 8, hipMemcpyHostToDevice, s));
+    }
+    ix.status.alloc(1);
+    HIP_TRY(hipMemsetAsync(ix.status.p, 0, 4, s));
+    HIP_TRY(hipStreamSynchronize(s));
+}
+
+static void ensure_query_workspace(rinnd_index& ix, int slots) {
+    hipStream_t s = ix.ctx->stream;
+    if (slots <= ix.slots) return;
+    ix.visited.alloc((size_t)slots * std::max<int64_t>(ix.n, 1));
+    ix.epochs.alloc(slots);
+    HIP_TRY(hipMemsetAsync(ix.visited.p, 0, ix.visited.bytes(), s));
+    HIP_TRY(hipMemsetAsync(ix.epochs.p, 0, ix.epochs.bytes(), s));
+    ix.seed_cap = std::max<int64_t>(ix.n, 1);
+    ix.spill_d.alloc((size_t)slots * ix.seed_cap);
+    ix.spill_i.alloc((size_t)slots * ix.seed_cap);
+    ix.slots = slots;
+}
+
+static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_t k, float eps, int64_t qid_base,
+                       int32_t* out_idx, float* out_dist) {
+    hipStream_t s = ix.ctx->stream;
+    RINND_CHECK_ARG(k >= 0 && k <= search_max_k(), "k must be in [0, 512]");
+    if (nq == 0 || k == 0) return;
+    // workspace: one slot per resident wave; memory per slot is n bytes + 8n bytes of spill
+    const int64_t per_slot = ix.n * 9;
+    int64_t max_slots = std::max<int64_t>(64, std::min<int64_t>(2048, ((int64_t)16 << 30) / std::max<int64_t>(per_slot, 1)));
+    const int slots = (int)std::min<int64_t>(nq, max_slots);
+    ensure_query_workspace(ix, std::max(slots, ix.slots));
+    DBuf<float> qtmp;
+    int64_t ldq = 0;
+    make_padded(s, queries, nq, (int)ix.dim, qtmp, ldq);
+    const bool dev_out = device_pointer(out_idx) && device_pointer(out_dist);
+    DBuf<int32_t> oi;
+    DBuf<float> od;
+    int32_t* oip = out_idx;
+    float* odp = out_dist;
+    if (!dev_out) {
+        oi.alloc((size_t)nq * k);
+        od.alloc((size_t)nq * k);
+        oip = oi.p;
+        odp = od.p;
+    }
+    SearchArgs a{};
+    a.metric = ix.metric;
+    a.X = ix.X.p;
+    a.ld = ix.ld;
+    a.dim = (int)ix.dim;
+    a.n = ix.n;
+    a.indptr = ix.indptr.p;
+    a.indices = ix.indices.p;
+    a.n_trees = (int)ix.trees.trees.size();
+    a.hyper = ix.d_hyper.p;
+    a.hdim = ix.ld;
+    a.offsets = ix.d_off.p;
+    a.children = ix.d_ch.p;
+    a.tindices = ix.d_tix.p;
+    a.leaf_budget = (int)ix.leaf_budget;
+    a.Q = qtmp.p;
+    a.ldq = ldq;
+    a.nq = nq;
+    a.qid_base = qid_base;
+    a.k = (int)k;
+    a.eps = eps;
+    a.min_distance = ix.min_distance;
+    a.seed = ix.seed;
+    a.vertex_order = ix.vertex_order.p;
+    a.corr = ix.corr;
+    a.out_idx = oip;
+    a.out_dist = odp;
+    a.visited = ix.visited.p;
+    a.epochs = ix.epochs.p;
+    a.seed_spill_d = ix.spill_d.p;
+    a.seed_spill_i = ix.spill_i.p;
+    a.seed_cap = ix.seed_cap;
+    a.status = ix.status.p;
+    launch_search(s, a, slots);
+    HIP_TRY(hipGetLastError());
+    if (!dev_out) {
+        HIP_TRY(hipMemcpyAsync(out_idx, oip, (size_t)nq * k * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(out_dist, odp, (size_t)nq * k * 4, hipMemcpyDeviceToHost, s));
+    }
+    int32_t st = 0;
+    HIP_TRY(hipMemcpyAsync(&st, ix.status.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (st != 0) {
+        HIP_TRY(hipMemsetAsync(ix.status.p, 0, 4, s));
+        throw Error{RINND_ERR_INTERNAL, "search workspace capacity exceeded"};
+    }
+}
+
+template <class F>
+static rinnd_status guard(rinnd_ctx* ctx, F&& f) {
+    if (!ctx) return RINND_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        f();
+        ctx->last_error.clear();
+        return RINND_OK;
+    } catch (const Error& e) {
+        ctx->last_error = e.msg;
+        (void)hipGetLastError();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "host allocation failed";
+        return RINND_ERR_OOM;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return RINND_ERR_INTERNAL;
+    }
+}
+
+}  // namespace rinnd
+
+extern "C" {
+
+rinnd_status rinnd_search_graph_build(rinnd_ctx* ctx, const int32_t* sorted_idx, const float* sorted_dist,
+                                      const float* data, int64_t n, int64_t k, int64_t dim, rinnd_distance dist,
+                                      float diversify_prob, float pruning_degree_multiplier, int32_t* out_indptr,
+                                      int32_t* out_indices, int64_t* out_nnz, float* out_min_distance) {
+    return guard(ctx, [&] {
+        RINND_CHECK_ARG(n >= 1 && k >= 1 && dim >= 1, "invalid sizes");
+        hipStream_t s = ctx->stream;
+        DBuf<float> X;
+        int64_t ld = 0;
+        make_padded(s, data, n, (int)dim, X, ld);
+        DBuf<int32_t> di((size_t)n * k);
+        DBuf<float> dd((size_t)n * k);
+        HIP_TRY(hipMemcpyAsync(di.p, sorted_idx, (size_t)n * k * 4, hipMemcpyDefault, s));
+        HIP_TRY(hipMemcpyAsync(dd.p, sorted_dist, (size_t)n * k * 4, hipMemcpyDefault, s));
+        SearchGraphDev G;
+        build_search_graph_dev(s, (int)dist, X.p, ld, (int)dim, di.p, dd.p, n, (int)k, diversify_prob,
+                               pruning_degree_multiplier, 42, G);
+        DBuf<int32_t> ip, ic;
+        int64_t nnz = 0;
+        search_graph_csr(s, G, nullptr, nullptr, ip, ic, nnz);
+        HIP_TRY(hipMemcpyAsync(out_indptr, ip.p, (n + 1) * 4, hipMemcpyDefault, s));
+        if (nnz) HIP_TRY(hipMemcpyAsync(out_indices, ic.p, nnz * 4, hipMemcpyDefault, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (out_nnz) *out_nnz = nnz;
+        if (out_min_distance) *out_min_distance = G.min_distance;
+    });
+}
+
+rinnd_status rinnd_index_build(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim, rinnd_distance dist,
+                               rinnd_correction corr, const rinnd_index_params* params, rinnd_index** out,
+                               rinnd_build_stats* stats) {
+    return guard(ctx, [&] {
+        RINND_CHECK_ARG(params && out, "null argument");
+        RINND_CHECK_ARG(n >= 1 && dim >= 1, "the index needs at least one point");
+        RINND_CHECK_ARG((int)dist >= 0 && (int)dist <= 6, "unknown distance");
+        RINND_CHECK_ARG((int)corr >= 0 && (int)corr <= 3, "unknown correction");
+        auto ix = std::make_unique<rinnd_index>();
+        index_build_impl(ctx, data, n, dim, (int)dist, (int)corr, *params, *ix, stats);
+        *out = ix.release();
+    });
+}
+
+void rinnd_index_free(rinnd_index* index) {
+    if (!index) return;
+    (void)hipSetDevice(index->ctx->device);
+    (void)hipStreamSynchronize(index->ctx->stream);
+    delete index;
+}
+
+rinnd_status rinnd_index_query(rinnd_index* ix, const float* queries, int64_t nq, int64_t k, float epsilon,
+                               int32_t* out_idx, float* out_dist) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        RINND_CHECK_ARG(nq >= 0, "negative query count");
+        query_impl(*ix, queries, nq, k, epsilon, 0, out_idx, out_dist);
+    });
+}
+
+rinnd_status rinnd_index_query_one(rinnd_index* ix, const float* query, int64_t k, float epsilon,
+                                   int32_t* out_idx, float* out_dist) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] { query_impl(*ix, query, 1, k, epsilon, 0, out_idx, out_dist); });
+}
+
+rinnd_status rinnd_index_neighbor_graph(const rinnd_index* ix, int32_t* out_idx, float* out_dist) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        hipStream_t s = ix->ctx->stream;
+        const size_t cnt = (size_t)ix->n * ix->k;
+        if (out_idx) HIP_TRY(hipMemcpyAsync(out_idx, ix->nb_idx.p, cnt * 4, hipMemcpyDefault, s));
+        if (out_dist) HIP_TRY(hipMemcpyAsync(out_dist, ix->nb_dist.p, cnt * 4, hipMemcpyDefault, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    });
+}
+
+rinnd_status rinnd_index_search_graph_info(const rinnd_index* ix, int64_t* n, int64_t* nnz, float* min_distance) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    if (n) *n = ix->n;
+    if (nnz) *nnz = ix->nnz;
+    if (min_distance) *min_distance = ix->min_distance;
+    return RINND_OK;
+}
+
+// search_graph_original_order (core/index.rs:137-141): the internal CSR reordered by inv
+rinnd_status rinnd_index_search_graph(const rinnd_index* ix, int32_t* indptr, int32_t* indices) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        hipStream_t s = ix->ctx->stream;
+        const int64_t n = ix->n;
+        std::vector<int32_t> p(n + 1), c(std::max<int64_t>(ix->nnz, 1));
+        std::vector<int64_t> inv(n), order(n);
+        HIP_TRY(hipMemcpyAsync(p.data(), ix->indptr.p, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+        if (ix->nnz) HIP_TRY(hipMemcpyAsync(c.data(), ix->indices.p, ix->nnz * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(order.data(), ix->vertex_order.p, n * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        // new (original) row o = internal row inv_orig[o]; ids mapped through vertex_order
+        for (int64_t i = 0; i < n; ++i) inv[order[i]] = i;
+        std::vector<int32_t> op(n + 1, 0), oc;
+        oc.reserve(ix->nnz);
+        for (int64_t o = 0; o < n; ++o) {
+            const int64_t r = inv[o];
+            for (int32_t t = p[r]; t < p[r + 1]; ++t) oc.push_back((int32_t)order[c[t]]);
+            op[o + 1] = (int32_t)oc.size();
+        }
+        std::memcpy(indptr, op.data(), (n + 1) * 4);
+        if (!oc.empty()) std::memcpy(indices, oc.data(), oc.size() * 4);
+    });
+}
+
+rinnd_status rinnd_index_vertex_order(const rinnd_index* ix, int64_t* out) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        HIP_TRY(hipMemcpyAsync(out, ix->vertex_order.p, ix->n * 8, hipMemcpyDefault, ix->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ix->ctx->stream));
+    });
+}
+
+const rinnd_forest* rinnd_index_trees(const rinnd_index* ix) { return ix ? &ix->trees : nullptr; }
+
+}  // extern "C"

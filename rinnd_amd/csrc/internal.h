@@ -1,0 +1,204 @@
+// internal.h - declarations shared by the kernel and host translation units.
+#pragma once
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace rinnd {
+
+// ---- device memory -------------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{RINND_ERR_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                           " bytes failed: " + hipGetErrorString(e)};
+        }
+        n = count;
+    }
+    // grow-only reuse
+    void ensure(size_t count) {
+        if (count > n) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Records of the ordered-apply pipeline.  A generator emits URec (one update, optionally applied to
+// both endpoints); the scatter groups them by destination as SegRec; a per-destination pass sorts
+// each group by `key` (the reference's generation order) and applies it sequentially.
+struct URec {
+    uint64_t key;
+    int32_t a;        // destination (or first endpoint)
+    int32_t b;        // value pushed into a (and destination of the mirrored record)
+    float v;          // distance or priority
+    int32_t two_sided;
+};
+struct SegRec {
+    uint64_t key;
+    int32_t x;
+    float v;
+};
+static_assert(sizeof(SegRec) == 16, "SegRec layout");
+static_assert(sizeof(URec) == 24, "URec layout");
+
+// ---- forest --------------------------------------------------------------------------------------
+struct TreeDev {
+    int64_t n_nodes = 0;
+    int64_t cap_nodes = 0;
+    DBuf<float> hyper;       // cap_nodes x dim
+    DBuf<float> offsets;     // cap_nodes
+    DBuf<int32_t> children;  // cap_nodes x 2
+    DBuf<int32_t> indices;   // n
+};
+
+}  // namespace rinnd
+
+struct rinnd_forest {
+    int64_t dim = 0, n = 0;
+    std::vector<rinnd::TreeDev> trees;
+};
+
+struct rinnd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    std::mutex mu;
+    rinnd::DBuf<uint64_t> jump_tables;  // 64 x 256 x 4 words
+    hipEvent_t ev[8] = {};
+};
+
+namespace rinnd {
+
+// ---- launchers (kernels.hip) ---------------------------------------------------------------------
+void launch_pad_rows(hipStream_t s, const float* src, int64_t n, int dim, float* dst, int64_t ld);
+void launch_fill_heap(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t count);
+void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t count);
+void launch_pair_distances(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                           const int32_t* a, const int32_t* b, int64_t n_pairs, float* out);
+
+// forest, reference streams (depth-first per tree)
+struct ForestDFSArgs {
+    const float* X;
+    int64_t ld;
+    int dim;
+    int64_t n;
+    int64_t leaf_size;
+    int64_t max_depth;
+    int angular;
+    int n_trees;
+    const uint64_t* tree_states;  // 4 words per tree
+    float** hyper;
+    float** offsets;
+    int32_t** children;
+    int32_t** indices;
+    int64_t* cap_nodes;
+    int64_t* n_nodes_out;
+    int32_t* status;
+    // scratch, n_trees x (n+1) each
+    uint8_t* side;
+    int32_t* rankR;
+    int32_t* PQ;
+    int32_t* tmp;
+};
+void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a);
+
+// forest, counter streams (level-synchronous across trees)
+struct ForestLevelArgs {
+    const float* X;
+    int64_t ld;
+    int dim;
+    int64_t n;
+    int64_t leaf_size;
+    int64_t max_depth;
+    int angular;
+    int n_trees;
+    const uint64_t* tree_keys;  // one 64-bit key per tree
+};
+
+// leaves of one tree in depth-first order: (start, end) pairs, returns count via device int
+void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, int64_t n,
+                      int32_t* leaf_mark /* n+1 scratch */, int64_t* leaf_rank /* n+1 */,
+                      int32_t* leaf_end_at /* n */, int64_t* leaves /* 2 per leaf */,
+                      void* scan_tmp, size_t scan_tmp_bytes);
+
+// ordered-apply pipeline
+void launch_leaf_pairs(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                       const int32_t* tree_indices, const int64_t* leaves, int64_t n_leaves,
+                       const uint64_t* pair_base, const float* heap_dist, int k, URec* U,
+                       unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
+                       int32_t* overflow);
+void launch_leaf_pair_base(hipStream_t s, const int64_t* leaves, int64_t n_leaves,
+                           uint64_t* pair_base_in, uint64_t* pair_base_out, void* scan_tmp,
+                           size_t scan_tmp_bytes);
+void launch_scatter(hipStream_t s, const URec* U, const unsigned long long* U_count,
+                    const uint64_t* off, unsigned long long* cur, SegRec* seg);
+void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
+                       int64_t n_dest, int32_t* idx, float* dist, uint8_t* flags, int k,
+                       unsigned long long* changes, int32_t* big_list, int32_t* big_count);
+void launch_cand_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
+                       int64_t n_dest, int32_t* new_idx, int32_t* old_idx, int C,
+                       int32_t* big_list, int32_t* big_count);
+
+// candidates
+void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt);
+struct RngStream {
+    uint64_t s[4];
+    int64_t count;
+    int64_t out_offset;
+};
+void launch_rng_streams(hipStream_t s, const uint64_t* jump_tables, const RngStream* streams_dev,
+                        int n_streams, int64_t max_count, float* out);
+void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, int64_t n, int k,
+                     const uint64_t* row_valid_prefix, const float* prio, int mode, int T,
+                     int64_t block_rows, const int64_t* block_prio_base, uint64_t counter_key,
+                     URec* U, unsigned long long* cnt);
+void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const int32_t* new_idx,
+                     int64_t n, int k, int C);
+
+// local join
+void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                      int64_t row_end, const float* heap_dist, int k, URec* U,
+                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
+                      int32_t* overflow);
+
+// rows
+void launch_sort_rows(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t n, int k);
+void launch_correct(hipStream_t s, float* dist, int64_t count, int corr);
+
+// scans (hipcub)
+size_t scan_u64_tmp_bytes(int64_t n);
+void scan_u64_exclusive(hipStream_t s, const unsigned long long* in, uint64_t* out, int64_t n,
+                        void* tmp, size_t tmp_bytes);
+
+}  // namespace rinnd

@@ -1,0 +1,1066 @@
+// kernels.hip - gfx950 kernels of the NN-Descent build path.
+//
+// Phases and the reference code they reproduce:
+//   k_forest_dfs        RP trees with the reference's FastRng streams (core/tree/builder.rs:89-348)
+//   k_leaf_pairs        leaf-pair distances of initialize_from_leaves (core/nndescent/mod.rs:203-300)
+//   k_cand_gen/_apply   CandidateSets::build_from_graph (core/nndescent/candidates.rs:51-484)
+//   k_join_fp32         update_iteration's candidate-pair distances + filter (mod.rs:321-448)
+//   k_scatter/_apply    per-vertex heap pushes in the reference's generation order (mod.rs:450-487)
+//   k_sort_rows         NeighborHeap::sort_all (core/heap/neighbor_heap.rs:293-320)
+//
+// Ordering argument used by the apply pipeline: the reference applies, for every vertex, its pushes
+// in global generation order (Rayon buckets are concatenated in source-thread order), and its
+// threshold filter only drops pushes the heap would reject anyway (roots only decrease).  Each
+// update therefore carries its generation key; a destination sorts its updates by key and replays
+// checked_flagged_push sequentially, which gives the reference's heaps and change counts exactly.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.h"
+
+namespace rinnd {
+
+static inline unsigned grid_for(int64_t work, int block, int64_t cap = 1 << 20) {
+    int64_t g = ceil_div(work, block);
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// ---- fills / copies ------------------------------------------------------------------------------
+__global__ void k_pad_rows(const float* __restrict__ src, int64_t n, int dim,
+                           float* __restrict__ dst, int64_t ld) {
+    const int64_t total = n * ld;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / ld, c = t - r * ld;
+        dst[t] = c < dim ? src[r * dim + c] : 0.f;
+    }
+}
+void launch_pad_rows(hipStream_t s, const float* src, int64_t n, int dim, float* dst, int64_t ld) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_pad_rows, dim3(grid_for(n * ld, 256)), dim3(256), 0, s, src, n, dim, dst, ld);
+}
+
+// NeighborHeap::new (neighbor_heap.rs:35-44): (-1, +inf, 0)
+__global__ void k_fill_heap(int32_t* idx, float* dist, uint8_t* flags, int64_t count) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        idx[t] = -1;
+        dist[t] = __builtin_inff();
+        flags[t] = 0;
+    }
+}
+void launch_fill_heap(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t count) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_fill_heap, dim3(grid_for(count, 256)), dim3(256), 0, s, idx, dist, flags, count);
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t count) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t count) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256)), dim3(256), 0, s, p, v, count);
+}
+
+__global__ void k_pair_distances(int metric, const float* __restrict__ X, int64_t ld, int dim,
+                                 const int32_t* __restrict__ a, const int32_t* __restrict__ b,
+                                 int64_t n_pairs, float* __restrict__ out) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_pairs;
+         t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = distance_exact(metric, X + (int64_t)a[t] * ld, X + (int64_t)b[t] * ld, dim);
+}
+void launch_pair_distances(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                           const int32_t* a, const int32_t* b, int64_t n_pairs, float* out) {
+    if (n_pairs == 0) return;
+    hipLaunchKernelGGL(k_pair_distances, dim3(grid_for(n_pairs, 256)), dim3(256), 0, s, metric, X, ld,
+                       dim, a, b, n_pairs, out);
+}
+
+// ---- RP forest with the reference streams ---------------------------------------------------------
+// One workgroup per tree walks the tree depth-first exactly like TreeBuilder::build_recursive
+// (builder.rs:172-226): node ids in pre-order, split points from the tree's FastRng, hyperplane
+// sums in the reference's sequential order, margins in the AVX2 dot order (8 lanes per point).
+// The two-pointer partition (builder.rs:312-348) is computed in parallel from its closed form:
+// with s = #L, a left element at p < s stays; the b-th left element counted from the back fills
+// the b-th right element counted from the front; the r-th right element from the front (p <= s)
+// moves to (r == 0 ? m : Q[r-1]) - 1 where Q[b] is the position of the b-th left element from the
+// back; a right element at p > s moves to p - 1.  A node containing a margin tie (|m| < 1e-8)
+// consumes next_bool() in examination order, so it is partitioned sequentially instead.
+constexpr int kForestThreads = 1024;
+constexpr int kForestStack = 512;
+struct StackEnt {
+    int32_t start, end, depth, pslot;
+};
+
+__global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) {
+    extern __shared__ float fsm[];  // hp[dim], p1[dim], p2[dim]
+    __shared__ StackEnt stack[kForestStack];
+    __shared__ uint64_t rs[4];
+    __shared__ int sh_sp, sh_ctr, sh_abort, sh_tie, sh_node, sh_split, sh_leaf, sh_scale;
+    __shared__ int sh_i1, sh_i2;
+    __shared__ float sh_off, sh_inv, sh_n1, sh_n2;
+    __shared__ int wave_tot[kForestThreads / RINND_WAVE];
+    __shared__ int sh_running;
+
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int64_t n = a.n;
+    const int dim = a.dim;
+    float* hp = fsm;
+    float* p1s = fsm + dim;
+    float* p2s = fsm + 2 * dim;
+    float* hyper = a.hyper[t];
+    float* offs = a.offsets[t];
+    int32_t* ch = a.children[t];
+    int32_t* idx = a.indices[t];
+    uint8_t* side = a.side + (int64_t)t * (n + 1);
+    int32_t* rankR = a.rankR + (int64_t)t * (n + 1);
+    int32_t* P = a.PQ + (int64_t)t * 2 * (n + 1);
+    int32_t* Q = P + (n + 1);
+    int32_t* tmp = a.tmp + (int64_t)t * (n + 1);
+    const int64_t cap = a.cap_nodes[t];
+
+    for (int64_t i = tid; i < n; i += blockDim.x) idx[i] = (int32_t)i;
+    if (tid == 0) {
+        for (int w = 0; w < 4; ++w) rs[w] = a.tree_states[t * 4 + w];
+        stack[0] = StackEnt{0, (int32_t)n, 0, -1};
+        sh_sp = 1;
+        sh_ctr = 0;
+        sh_abort = 0;
+    }
+    __syncthreads();
+
+    for (;;) {
+        if (sh_sp == 0 || sh_abort) break;
+        const StackEnt e = stack[sh_sp - 1];
+        __syncthreads();
+        const int m = e.end - e.start;
+        if (tid == 0) {
+            sh_sp -= 1;
+            const int node = sh_ctr;
+            if (node >= cap) {
+                sh_abort = 1;
+            } else {
+                sh_ctr = node + 1;
+                if (e.pslot >= 0) ch[e.pslot] = node;
+            }
+            sh_node = node;
+            sh_leaf = (m <= a.leaf_size || e.depth >= a.max_depth) ? 1 : 0;
+            sh_tie = 0;
+            if (!sh_leaf && !sh_abort) {  // make_split (builder.rs:228-247)
+                Xoshiro r;
+                for (int w = 0; w < 4; ++w) r.s[w] = rs[w];
+                const uint64_t i1 = r.next_index((uint64_t)m);
+                uint64_t i2 = r.next_index((uint64_t)m);
+                while (i2 == i1 && m > 1) i2 = r.next_index((uint64_t)m);
+                for (int w = 0; w < 4; ++w) rs[w] = r.s[w];
+                sh_i1 = (int)i1;
+                sh_i2 = (int)i2;
+            }
+        }
+        __syncthreads();
+        if (sh_abort) break;
+        const int node = sh_node;
+        if (sh_leaf) {
+            if (tid == 0) {
+                ch[2 * node] = -e.start;
+                ch[2 * node + 1] = -e.end;
+            }
+            continue;  // next iteration re-reads sh_sp after the barrier below
+        }
+        const float* x1 = a.X + (int64_t)idx[e.start + sh_i1] * a.ld;
+        const float* x2 = a.X + (int64_t)idx[e.start + sh_i2] * a.ld;
+        for (int i = tid; i < dim; i += blockDim.x) {
+            p1s[i] = x1[i];
+            p2s[i] = x2[i];
+        }
+        __syncthreads();
+        if (!a.angular) {  // make_euclidean_split (builder.rs:249-279)
+            for (int i = tid; i < dim; i += blockDim.x) hp[i] = p2s[i] - p1s[i];
+            __syncthreads();
+            if (tid == 0) {
+                float norm_sq = 0.f, dot_nm = 0.f;
+                for (int i = 0; i < dim; ++i) {
+                    const float h = hp[i];
+                    const float mid = (p1s[i] + p2s[i]) * 0.5f;
+                    norm_sq += h * h;
+                    dot_nm += h * mid;
+                }
+                const float norm = sqrtf(norm_sq);
+                if (norm > 1e-8f) {
+                    const float inv = 1.0f / norm;
+                    sh_inv = inv;
+                    sh_off = -dot_nm * inv;
+                    sh_scale = 1;
+                } else {
+                    sh_off = 0.f;
+                    sh_scale = 0;
+                }
+            }
+            __syncthreads();
+            if (sh_scale)
+                for (int i = tid; i < dim; i += blockDim.x) hp[i] *= sh_inv;
+        } else {  // make_angular_split (builder.rs:281-308)
+            if (tid == 0) {
+                float n1 = 0.f, n2 = 0.f;
+                for (int i = 0; i < dim; ++i) n1 += p1s[i] * p1s[i];
+                for (int i = 0; i < dim; ++i) n2 += p2s[i] * p2s[i];
+                sh_n1 = sqrtf(n1);
+                sh_n2 = sqrtf(n2);
+                sh_off = 0.f;
+            }
+            __syncthreads();
+            const bool deg = sh_n1 < 1e-8f || sh_n2 < 1e-8f;
+            for (int i = tid; i < dim; i += blockDim.x)
+                hp[i] = deg ? 0.f : p2s[i] / sh_n2 - p1s[i] / sh_n1;
+            __syncthreads();
+            if (!deg) {
+                if (tid == 0) {
+                    float nn = 0.f;
+                    for (int i = 0; i < dim; ++i) nn += hp[i] * hp[i];
+                    sh_n1 = sqrtf(nn);
+                }
+                __syncthreads();
+                const float nn = sh_n1;
+                if (nn > 1e-8f)
+                    for (int i = tid; i < dim; i += blockDim.x) hp[i] /= nn;
+            }
+        }
+        __syncthreads();
+        // margins: 8 lanes per point in the AVX2 lane order (dot_product_avx2, builder.rs:29-60)
+        {
+            const float off = sh_off;
+            const int g = tid >> 3, l = tid & 7;
+            const int groups = blockDim.x >> 3;
+            const int chunks = dim >> 3;
+            for (int base = e.start; base < e.end; base += groups) {
+                const int pos = base + g;
+                const bool act = pos < e.end;
+                float acc = 0.f;
+                const float* x = a.X + (int64_t)(act ? idx[pos] : 0) * a.ld;
+                if (act)
+                    for (int c = 0; c < chunks; ++c) acc = __builtin_fmaf(x[c * 8 + l], hp[c * 8 + l], acc);
+                const float s4 = acc + __shfl_xor(acc, 4);
+                const float s1 = s4 + __shfl_xor(s4, 1);
+                const float tot = s1 + __shfl_xor(s1, 2);
+                if (act && l == 0) {
+                    float r = tot;
+                    for (int i = chunks * 8; i < dim; ++i) r += x[i] * hp[i];
+                    const float margin = r + off;
+                    uint8_t sd;
+                    if (fabsf(margin) < 1e-8f) {
+                        sd = 2;
+                        sh_tie = 1;
+                    } else {
+                        sd = margin < 0.0f ? 0 : 1;
+                    }
+                    side[pos - e.start] = sd;
+                }
+            }
+        }
+        __syncthreads();
+        if (sh_tie) {
+            if (tid == 0) {  // partition_inplace verbatim (builder.rs:319-347)
+                Xoshiro r;
+                for (int w = 0; w < 4; ++w) r.s[w] = rs[w];
+                int left = 0, right = m;
+                while (left < right) {
+                    const uint8_t sd = side[left];
+                    const bool goes_left = sd == 2 ? !r.next_bool() : sd == 0;
+                    if (goes_left) {
+                        ++left;
+                    } else {
+                        --right;
+                        const int32_t ti = idx[e.start + left];
+                        idx[e.start + left] = idx[e.start + right];
+                        idx[e.start + right] = ti;
+                        const uint8_t ts = side[left];
+                        side[left] = side[right];
+                        side[right] = ts;
+                    }
+                }
+                for (int w = 0; w < 4; ++w) rs[w] = r.s[w];
+                sh_split = left;
+            }
+            __syncthreads();
+        } else {
+            // rankR[p] = number of right elements before p (exclusive scan of 1-bit flags)
+            if (tid == 0) sh_running = 0;
+            __syncthreads();
+            for (int base = 0; base < m; base += blockDim.x) {
+                const int p = base + tid;
+                const bool isR = p < m && side[p] == 1;
+                const unsigned long long bal = __ballot(isR);
+                const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+                const int lpre = __popcll(bal & lt);
+                if (lane == 0) wave_tot[wid] = __popcll(bal);
+                __syncthreads();
+                int wpre = 0;
+                for (int w = 0; w < wid; ++w) wpre += wave_tot[w];
+                if (p < m) rankR[p] = sh_running + wpre + lpre;
+                __syncthreads();
+                if (tid == 0) {
+                    int tot = 0;
+                    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += wave_tot[w];
+                    sh_running += tot;
+                }
+                __syncthreads();
+            }
+            const int totR = sh_running;
+            const int split = m - totR;
+            if (tid == 0) rankR[m] = totR;
+            __syncthreads();
+            for (int p = tid; p < m; p += blockDim.x) {
+                const uint8_t sd = side[p];
+                if (sd == 1 && p < split) P[rankR[p]] = p;
+                if (sd == 0 && p >= split) Q[(m - p - 1) - (totR - rankR[p + 1])] = p;
+            }
+            __syncthreads();
+            for (int p = tid; p < m; p += blockDim.x) {
+                const uint8_t sd = side[p];
+                int dest;
+                if (sd == 0) {
+                    dest = p < split ? p : P[(m - p - 1) - (totR - rankR[p + 1])];
+                } else {
+                    const int r = rankR[p];
+                    dest = p <= split ? ((r == 0 ? m : Q[r - 1]) - 1) : p - 1;
+                }
+                tmp[dest] = idx[e.start + p];
+            }
+            __syncthreads();
+            for (int p = tid; p < m; p += blockDim.x) idx[e.start + p] = tmp[p];
+            if (tid == 0) sh_split = split;
+            __syncthreads();
+        }
+        const int split = sh_split;
+        if (split == 0 || split == m) {  // degenerate split -> leaf (builder.rs:198-210)
+            if (tid == 0) {
+                ch[2 * node] = -e.start;
+                ch[2 * node + 1] = -e.end;
+            }
+        } else {
+            for (int i = tid; i < dim; i += blockDim.x) hyper[(int64_t)node * dim + i] = hp[i];
+            if (tid == 0) {
+                offs[node] = sh_off;
+                if (sh_sp + 2 > kForestStack) {
+                    sh_abort = 2;
+                } else {
+                    stack[sh_sp] = StackEnt{e.start + split, e.end, e.depth + 1, 2 * node + 1};
+                    stack[sh_sp + 1] = StackEnt{e.start, e.start + split, e.depth + 1, 2 * node};
+                    sh_sp += 2;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (tid == 0) {
+        a.n_nodes_out[t] = sh_ctr;
+        a.status[t] = sh_abort;
+    }
+}
+
+void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a) {
+    const size_t lds = (size_t)3 * a.dim * sizeof(float);
+    hipLaunchKernelGGL(k_forest_dfs, dim3(a.n_trees), dim3(kForestThreads), lds, s, a);
+}
+
+// ---- leaves in depth-first order -------------------------------------------------------------------
+__global__ void k_leaf_mark(const int32_t* ch, int64_t n_nodes, int32_t* mark, int32_t* end_at) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n_nodes;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c0 = ch[2 * v];
+        if (c0 <= 0) {
+            const int32_t s = -c0, e = -ch[2 * v + 1];
+            if (e > s) {
+                mark[s] = 1;
+                end_at[s] = e;
+            }
+        }
+    }
+}
+__global__ void k_i32_to_u64(const int32_t* in, unsigned long long* out, int64_t n) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = (unsigned long long)in[t];
+}
+__global__ void k_leaf_emit(const int32_t* mark, const uint64_t* rank, const int32_t* end_at,
+                            int64_t n, int64_t* leaves) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x)
+        if (mark[p]) {
+            const uint64_t r = rank[p];
+            leaves[2 * r] = p;
+            leaves[2 * r + 1] = end_at[p];
+        }
+}
+void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, int64_t n,
+                      int32_t* leaf_mark, int64_t* leaf_rank, int32_t* leaf_end_at, int64_t* leaves,
+                      void* scan_tmp, size_t scan_tmp_bytes) {
+    HIP_TRY(hipMemsetAsync(leaf_mark, 0, (n + 1) * sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_leaf_mark, dim3(grid_for(n_nodes, 256)), dim3(256), 0, s, children, n_nodes,
+                       leaf_mark, leaf_end_at);
+    // widen marks into the second half of the rank buffer (2(n+1) words), scan into the first half;
+    // rank[n] = number of leaves
+    unsigned long long* wide = (unsigned long long*)(leaf_rank + (n + 1));
+    hipLaunchKernelGGL(k_i32_to_u64, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, leaf_mark, wide, n + 1);
+    size_t bytes = scan_tmp_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, wide, (unsigned long long*)leaf_rank,
+                                             (int)(n + 1), s));
+    hipLaunchKernelGGL(k_leaf_emit, dim3(grid_for(n, 256)), dim3(256), 0, s, leaf_mark,
+                       (const uint64_t*)leaf_rank, leaf_end_at, n, leaves);
+}
+
+__global__ void k_leaf_pair_counts(const int64_t* leaves, int64_t n_leaves, uint64_t* out) {
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l <= n_leaves;
+         l += (int64_t)gridDim.x * blockDim.x) {
+        if (l == n_leaves) {
+            out[l] = 0;
+        } else {
+            const uint64_t m = (uint64_t)(leaves[2 * l + 1] - leaves[2 * l]);
+            out[l] = m * (m - 1) / 2;
+        }
+    }
+}
+void launch_leaf_pair_base(hipStream_t s, const int64_t* leaves, int64_t n_leaves,
+                           uint64_t* pair_base_in, uint64_t* pair_base_out, void* scan_tmp,
+                           size_t scan_tmp_bytes) {
+    hipLaunchKernelGGL(k_leaf_pair_counts, dim3(grid_for(n_leaves + 1, 256)), dim3(256), 0, s, leaves,
+                       n_leaves, pair_base_in);
+    size_t bytes = scan_tmp_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, (unsigned long long*)pair_base_in,
+                                             (unsigned long long*)pair_base_out, (int)(n_leaves + 1), s));
+}
+
+// ---- emission helper: wave-aggregated append into U ---------------------------------------------
+__device__ __forceinline__ void emit_update(bool keep, uint64_t key, int32_t p, int32_t q, float d,
+                                            URec* U, unsigned long long* U_count, uint64_t U_cap,
+                                            unsigned long long* cnt, int32_t* overflow) {
+    const unsigned long long bal = __ballot(keep);
+    if (bal == 0) return;
+    const int lane = lane_id();
+    const int total = __popcll(bal);
+    const int leader = __ffsll((long long)bal) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(U_count, (unsigned long long)total);
+    base = __shfl(base, leader);
+    if (keep) {
+        const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const uint64_t pos = base + __popcll(bal & lt);
+        if (pos < U_cap) {
+            U[pos] = URec{key, p, q, d, 1};
+            atomicAdd(&cnt[p], 1ull);
+            atomicAdd(&cnt[q], 1ull);
+        } else {
+            *overflow = 1;
+        }
+    }
+}
+
+// ---- leaf initialisation pairs (initialize_from_leaves, mod.rs:239-266) ----------------------------
+// One workgroup per leaf; pair (i, j), i < j, key = pair_base[leaf] + rank of (i, j) in the leaf's
+// row-major upper triangle, so leaves keep forest order and pairs keep loop order.
+__global__ __launch_bounds__(256) void k_leaf_pairs(int metric, const float* __restrict__ X,
+                                                    int64_t ld, int dim,
+                                                    const int32_t* __restrict__ tidx,
+                                                    const int64_t* __restrict__ leaves,
+                                                    int64_t n_leaves, const uint64_t* pair_base,
+                                                    const float* __restrict__ hd, int k, URec* U,
+                                                    unsigned long long* U_count, uint64_t U_cap,
+                                                    unsigned long long* cnt, int32_t* overflow) {
+    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
+        const int64_t s = leaves[2 * leaf], e = leaves[2 * leaf + 1];
+        const int64_t m = e - s;
+        const uint64_t base = pair_base[leaf];
+        const int64_t total = m * m;
+        for (int64_t t0 = 0; t0 < total; t0 += blockDim.x) {
+            const int64_t t = t0 + threadIdx.x;
+            const int64_t i = t / (m ? m : 1), j = t - i * m;
+            bool keep = false;
+            int32_t p = 0, q = 0;
+            float d = 0.f;
+            uint64_t key = 0;
+            if (t < total && j > i) {
+                p = tidx[s + i];
+                q = tidx[s + j];
+                d = distance_exact(metric, X + (int64_t)p * ld, X + (int64_t)q * ld, dim);
+                const float thr = fmaxf(hd[(int64_t)p * k], hd[(int64_t)q * k]);
+                keep = d < thr;
+                key = base + (uint64_t)(i * m - i * (i + 1) / 2 + (j - i - 1));
+            }
+            emit_update(keep, key, p, q, d, U, U_count, U_cap, cnt, overflow);
+        }
+    }
+}
+void launch_leaf_pairs(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                       const int32_t* tree_indices, const int64_t* leaves, int64_t n_leaves,
+                       const uint64_t* pair_base, const float* heap_dist, int k, URec* U,
+                       unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
+                       int32_t* overflow) {
+    if (n_leaves == 0) return;
+    hipLaunchKernelGGL(k_leaf_pairs, dim3(grid_for(n_leaves, 1, 1 << 16)), dim3(256), 0, s, metric, X,
+                       ld, dim, tree_indices, leaves, n_leaves, pair_base, heap_dist, k, U, U_count,
+                       U_cap, cnt, overflow);
+}
+
+// ---- scatter by destination -----------------------------------------------------------------------
+__global__ void k_scatter(const URec* __restrict__ U, const unsigned long long* U_count,
+                          const uint64_t* __restrict__ off, unsigned long long* cur, SegRec* seg) {
+    const uint64_t total = *U_count;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const URec u = U[t];
+        if (u.a < 0) continue;
+        const uint64_t sa = off[u.a] + atomicAdd(&cur[u.a], 1ull);
+        seg[sa] = SegRec{u.key, u.b, u.v};
+        if (u.two_sided) {
+            const uint64_t sb = off[u.b] + atomicAdd(&cur[u.b], 1ull);
+            seg[sb] = SegRec{u.key, u.a, u.v};
+        }
+    }
+}
+void launch_scatter(hipStream_t s, const URec* U, const unsigned long long* U_count,
+                    const uint64_t* off, unsigned long long* cur, SegRec* seg) {
+    hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, s, U, U_count, off, cur, seg);
+}
+
+// ---- per-destination sort + sequential apply ------------------------------------------------------
+constexpr int kApplyWaves = 4;
+constexpr int kSortCap = 512;           // records sorted in LDS by one wave
+constexpr int kApplyScratch = 4096;     // per-wave heap scratch bytes
+constexpr int kPerWave = kSortCap * 16 + kApplyScratch;
+constexpr int kBigThreads = 1024;
+constexpr int kBigChunk = 2048;         // records per LDS chunk in the large-group path
+
+__device__ __forceinline__ int next_pow2(int64_t x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+__device__ __forceinline__ bool rec_gt(const SegRec& a, const SegRec& b) { return a.key > b.key; }
+
+// checked_flagged_push (neighbor_heap.rs:149-177) on an LDS row; returns true on insertion.
+__device__ __forceinline__ bool heap_push(int32_t* hi, float* hd, uint8_t* hf, int k, int32_t x,
+                                          float d, uint8_t flag) {
+    if (d >= hd[0]) return false;
+    for (int j = 0; j < k; ++j)
+        if (hi[j] == x) return false;
+    // replace root, sift_down (neighbor_heap.rs:207-243)
+    int pos = 0, child = 1;
+    while (child < k) {
+        const int right = child + 1;
+        if (right < k && hd[child] < hd[right]) child = right;
+        if (d >= hd[child]) break;
+        hd[pos] = hd[child];
+        hi[pos] = hi[child];
+        hf[pos] = hf[child];
+        pos = child;
+        child = 2 * pos + 1;
+    }
+    hd[pos] = d;
+    hi[pos] = x;
+    hf[pos] = flag;
+    return true;
+}
+
+struct HeapApply {
+    int32_t* idx;
+    float* dist;
+    uint8_t* flags;
+    int k;
+    unsigned long long* changes;
+    __device__ void apply(int64_t v, const SegRec* recs, int64_t L, char* scratch, int lane) const {
+        int32_t* hi = (int32_t*)scratch;
+        float* hd = (float*)(hi + k);
+        uint8_t* hf = (uint8_t*)(hd + k);
+        const int64_t o = v * k;
+        for (int j = lane; j < k; j += RINND_WAVE) {
+            hi[j] = idx[o + j];
+            hd[j] = dist[o + j];
+            hf[j] = flags[o + j];
+        }
+        wave_sync();
+        if (lane == 0) {
+            unsigned long long ch = 0;
+            for (int64_t r = 0; r < L; ++r) {
+                const SegRec rec = recs[r];
+                if (heap_push(hi, hd, hf, k, rec.x, rec.v, 1)) ++ch;
+            }
+            if (ch) atomicAdd(changes, ch);
+        }
+        wave_sync();
+        for (int j = lane; j < k; j += RINND_WAVE) {
+            idx[o + j] = hi[j];
+            dist[o + j] = hd[j];
+            flags[o + j] = hf[j];
+        }
+    }
+};
+
+// checked_heap_push_flat (candidates.rs:433-484)
+__device__ __forceinline__ void cand_push(float* pri, int32_t* ix, int n, float priority,
+                                          int32_t index) {
+    if (priority >= pri[0]) return;
+    for (int i = 0; i < n; ++i)
+        if (ix[i] == index) return;
+    pri[0] = priority;
+    ix[0] = index;
+    int pos = 0;
+    for (;;) {
+        const int l = 2 * pos + 1, r = 2 * pos + 2;
+        int largest = pos;
+        if (l < n && pri[l] > pri[largest]) largest = l;
+        if (r < n && pri[r] > pri[largest]) largest = r;
+        if (largest == pos) break;
+        pri[pos] = pri[largest];
+        ix[pos] = ix[largest];
+        pri[largest] = priority;
+        ix[largest] = index;
+        pos = largest;
+    }
+}
+
+struct CandApply {
+    int32_t* new_idx;
+    int32_t* old_idx;
+    int C;
+    __device__ void apply(int64_t v, const SegRec* recs, int64_t L, char* scratch, int lane) const {
+        int32_t* ni = (int32_t*)scratch;
+        int32_t* oi = ni + C;
+        float* np = (float*)(oi + C);
+        float* op = np + C;
+        for (int j = lane; j < C; j += RINND_WAVE) {
+            ni[j] = -1;
+            oi[j] = -1;
+            np[j] = __builtin_inff();
+            op[j] = __builtin_inff();
+        }
+        wave_sync();
+        if (lane == 0) {
+            for (int64_t r = 0; r < L; ++r) {
+                const SegRec rec = recs[r];
+                const int32_t x = rec.x & 0x7fffffff;
+                if (rec.x < 0) cand_push(np, ni, C, rec.v, x);  // flag bit set: "new" edge
+                else cand_push(op, oi, C, rec.v, x);
+            }
+        }
+        wave_sync();
+        for (int j = lane; j < C; j += RINND_WAVE) {
+            new_idx[v * C + j] = ni[j];
+            old_idx[v * C + j] = oi[j];
+        }
+    }
+};
+
+template <class Ap>
+__global__ __launch_bounds__(kApplyWaves * RINND_WAVE) void k_seg_apply(
+    const uint64_t* __restrict__ off, const SegRec* __restrict__ seg, int64_t n_dest, Ap ap,
+    int32_t* big_list, int32_t* big_count) {
+    extern __shared__ __align__(16) char asm_[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    SegRec* recs = (SegRec*)(asm_ + w * kPerWave);
+    char* scratch = asm_ + w * kPerWave + kSortCap * 16;
+    for (int64_t v = (int64_t)blockIdx.x * kApplyWaves + w; v < n_dest;
+         v += (int64_t)gridDim.x * kApplyWaves) {
+        const uint64_t o0 = off[v];
+        const int64_t L = (int64_t)(off[v + 1] - o0);
+        if (L == 0) continue;
+        if (L > kSortCap) {
+            if (lane == 0) big_list[atomicAdd(big_count, 1)] = (int32_t)v;
+            continue;
+        }
+        const int S = next_pow2(L);
+        for (int i = lane; i < S; i += RINND_WAVE)
+            recs[i] = i < L ? seg[o0 + i] : SegRec{~0ull, 0, 0.f};
+        wave_sync();
+        for (int kk = 2; kk <= S; kk <<= 1)
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < S; i += RINND_WAVE) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & kk) == 0;
+                        const SegRec A = recs[i], B = recs[ixj];
+                        if (rec_gt(A, B) == up) {
+                            recs[i] = B;
+                            recs[ixj] = A;
+                        }
+                    }
+                }
+                wave_sync();
+            }
+        ap.apply(v, recs, L, scratch, lane);
+    }
+}
+
+// Large groups: chunks sorted in LDS, merged pairwise in global memory (merge path), then applied.
+template <class Ap>
+__global__ __launch_bounds__(kBigThreads) void k_seg_apply_big(const uint64_t* __restrict__ off,
+                                                               SegRec* seg, SegRec* seg2, Ap ap,
+                                                               const int32_t* big_list,
+                                                               const int32_t* big_count) {
+    extern __shared__ __align__(16) char bsm[];
+    SegRec* buf = (SegRec*)bsm;
+    char* scratch = bsm + kBigChunk * 16;
+    const int nb = *big_count;
+    const int tid = threadIdx.x;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int64_t v = big_list[b];
+        const uint64_t o0 = off[v];
+        const int64_t L = (int64_t)(off[v + 1] - o0);
+        SegRec* A = seg + o0;
+        SegRec* B = seg2 + o0;
+        for (int64_t c0 = 0; c0 < L; c0 += kBigChunk) {
+            const int64_t cl = min((int64_t)kBigChunk, L - c0);
+            const int S = next_pow2(cl);
+            for (int i = tid; i < S; i += blockDim.x) buf[i] = i < cl ? A[c0 + i] : SegRec{~0ull, 0, 0.f};
+            __syncthreads();
+            for (int kk = 2; kk <= S; kk <<= 1)
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    for (int i = tid; i < S; i += blockDim.x) {
+                        const int ixj = i ^ j;
+                        if (ixj > i) {
+                            const bool up = (i & kk) == 0;
+                            const SegRec x = buf[i], y = buf[ixj];
+                            if (rec_gt(x, y) == up) {
+                                buf[i] = y;
+                                buf[ixj] = x;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            for (int i = tid; i < cl; i += blockDim.x) A[c0 + i] = buf[i];
+            __syncthreads();
+        }
+        SegRec* src = A;
+        SegRec* dst = B;
+        for (int64_t w = kBigChunk; w < L; w *= 2) {
+            for (int64_t o = tid; o < L; o += blockDim.x) {
+                const int64_t base = (o / (2 * w)) * (2 * w);
+                const int64_t la = min(w, L - base);
+                const int64_t lb = max((int64_t)0, min(w, L - base - w));
+                const int64_t ol = o - base;
+                const SegRec* Aa = src + base;
+                const SegRec* Bb = src + base + w;
+                int64_t lo = max((int64_t)0, ol - lb), hi = min(ol, la);
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (Aa[mid].key < Bb[ol - 1 - mid].key) lo = mid + 1;
+                    else hi = mid;
+                }
+                const int64_t i = lo, j = ol - lo;
+                const bool takeA = i < la && (j >= lb || Aa[i].key < Bb[j].key);
+                dst[o] = takeA ? Aa[i] : Bb[j];
+            }
+            __syncthreads();
+            SegRec* t = src;
+            src = dst;
+            dst = t;
+        }
+        if (tid < RINND_WAVE) ap.apply(v, src, L, scratch, tid);
+        __syncthreads();
+    }
+}
+
+template <class Ap>
+static void run_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2, int64_t n_dest,
+                      const Ap& ap, int32_t* big_list, int32_t* big_count) {
+    HIP_TRY(hipMemsetAsync(big_count, 0, sizeof(int32_t), s));
+    const unsigned grid = grid_for(n_dest, kApplyWaves, 1 << 16);
+    hipLaunchKernelGGL(k_seg_apply<Ap>, dim3(grid), dim3(kApplyWaves * RINND_WAVE),
+                       kApplyWaves * kPerWave, s, off, seg, n_dest, ap, big_list, big_count);
+    hipLaunchKernelGGL(k_seg_apply_big<Ap>, dim3(256), dim3(kBigThreads),
+                       kBigChunk * 16 + kApplyScratch, s, off, seg, seg2, ap, big_list, big_count);
+}
+
+void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
+                       int64_t n_dest, int32_t* idx, float* dist, uint8_t* flags, int k,
+                       unsigned long long* changes, int32_t* big_list, int32_t* big_count) {
+    run_apply(s, off, seg, seg2, n_dest, HeapApply{idx, dist, flags, k, changes}, big_list, big_count);
+}
+void launch_cand_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
+                       int64_t n_dest, int32_t* new_idx, int32_t* old_idx, int C,
+                       int32_t* big_list, int32_t* big_count) {
+    run_apply(s, off, seg, seg2, n_dest, CandApply{new_idx, old_idx, C}, big_list, big_count);
+}
+
+// ---- candidates -----------------------------------------------------------------------------------
+__global__ void k_row_valid(const int32_t* __restrict__ idx, int64_t n, int k,
+                            unsigned long long* cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long c = 0;
+        if (i < n)
+            for (int j = 0; j < k; ++j) c += idx[i * k + j] >= 0;
+        cnt[i] = c;
+    }
+}
+void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt) {
+    hipLaunchKernelGGL(k_row_valid, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, idx, n, k,
+                       (unsigned long long*)cnt);
+}
+
+// FastRng streams at arbitrary offsets: thread = chunk of kRngChunk draws, jumped to its start.
+constexpr int kRngChunk = 2048;
+__global__ void k_rng_streams(const uint64_t* __restrict__ tables, const RngStream* streams,
+                              float* out) {
+    const RngStream st = streams[blockIdx.y];
+    const int64_t chunk = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t first = chunk * kRngChunk;
+    if (first >= st.count) return;
+    uint64_t s4[4] = {st.s[0], st.s[1], st.s[2], st.s[3]};
+    xoshiro_jump(tables, s4, (uint64_t)first);
+    Xoshiro r;
+    for (int w = 0; w < 4; ++w) r.s[w] = s4[w];
+    const int64_t last = min(first + kRngChunk, st.count);
+    float* o = out + st.out_offset;
+    for (int64_t t = first; t < last; ++t) o[t] = r.next_float();
+}
+void launch_rng_streams(hipStream_t s, const uint64_t* jump_tables, const RngStream* streams_dev,
+                        int n_streams, int64_t max_count, float* out) {
+    if (n_streams == 0 || max_count == 0) return;
+    const int64_t chunks = ceil_div(max_count, kRngChunk);
+    hipLaunchKernelGGL(k_rng_streams, dim3(grid_for(chunks, 64, 1 << 30), n_streams), dim3(64), 0, s,
+                       jump_tables, streams_dev, out);
+}
+
+// One thread per heap row: forward record (to the row) and reverse record (to the neighbour) per
+// valid edge, with the reference's priorities and per-destination push order encoded in `key`:
+//   mode 0 (one stream, candidates.rs:84-131): draws 2r, 2r+1 of the main stream, key = edge
+//   mode 1 (T blocks, candidates.rs:170-340): draw r of block stream, key = (rotated block, edge)
+//   mode 2 (counter draws): priorities from mix64(counter_key, edge, direction), key = edge
+__global__ void k_cand_gen(const int32_t* __restrict__ idx, const uint8_t* __restrict__ flags,
+                           int64_t n, int k, const uint64_t* __restrict__ rvp,
+                           const float* __restrict__ prio, int mode, int T, int64_t block_rows,
+                           const int64_t* __restrict__ block_prio_base, uint64_t counter_key,
+                           URec* U, unsigned long long* cnt) {
+    const uint64_t nk = (uint64_t)n * k;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t r = rvp[i];
+        const int64_t blk = mode == 1 ? i / block_rows : 0;
+        const uint64_t blk_first = mode == 1 ? rvp[blk * block_rows] : 0;
+        for (int j = 0; j < k; ++j) {
+            const uint64_t e = (uint64_t)i * k + j;
+            const int32_t nb = idx[e];
+            if (nb < 0) {
+                U[2 * e].a = -1;
+                U[2 * e + 1].a = -1;
+                continue;
+            }
+            const int32_t fbit = flags[e] ? (int32_t)0x80000000u : 0;
+            float pf, pr;
+            uint64_t kf = e, kr = e;
+            if (mode == 0) {
+                pf = prio[2 * r];
+                pr = prio[2 * r + 1];
+            } else if (mode == 1) {
+                pf = pr = prio[block_prio_base[blk] + (int64_t)(r - blk_first)];
+                const int64_t db = nb / block_rows;
+                kr = (db == blk ? 0 : (uint64_t)(blk + 1)) * nk + e;
+            } else {
+                const uint64_t h1 = mix64(counter_key ^ (2 * e));
+                const uint64_t h2 = mix64(counter_key ^ (2 * e + 1));
+                pf = (float)(uint32_t)(h1 >> 40) * (1.0f / 16777216.0f);
+                pr = (float)(uint32_t)(h2 >> 40) * (1.0f / 16777216.0f);
+            }
+            U[2 * e] = URec{kf, (int32_t)i, nb | fbit, pf, 0};
+            U[2 * e + 1] = URec{kr, nb, (int32_t)i | fbit, pr, 0};
+            atomicAdd(&cnt[i], 1ull);
+            atomicAdd(&cnt[nb], 1ull);
+            ++r;
+        }
+    }
+}
+void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, int64_t n, int k,
+                     const uint64_t* row_valid_prefix, const float* prio, int mode, int T,
+                     int64_t block_rows, const int64_t* block_prio_base, uint64_t counter_key,
+                     URec* U, unsigned long long* cnt) {
+    hipLaunchKernelGGL(k_cand_gen, dim3(grid_for(n, 128)), dim3(128), 0, s, idx, flags, n, k,
+                       row_valid_prefix, prio, mode, T, block_rows, block_prio_base, counter_key, U, cnt);
+}
+
+// mark_old_flags (candidates.rs:354-417)
+__global__ void k_mark_old(const int32_t* __restrict__ idx, uint8_t* flags,
+                           const int32_t* __restrict__ new_idx, int64_t n, int k, int C) {
+    const int64_t total = n * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t nb = idx[e];
+        if (nb < 0 || flags[e] == 0) continue;
+        const int64_t i = e / k;
+        for (int c = 0; c < C; ++c)
+            if (new_idx[i * C + c] == nb) {
+                flags[e] = 0;
+                break;
+            }
+    }
+}
+void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const int32_t* new_idx,
+                     int64_t n, int k, int C) {
+    hipLaunchKernelGGL(k_mark_old, dim3(grid_for(n * k, 256)), dim3(256), 0, s, idx, flags, new_idx, n,
+                       k, C);
+}
+
+// ---- local join, FP32 (update_iteration generation, mod.rs:383-447) --------------------------------
+// One workgroup per row i: the row's new (C) and old (C) candidate rows are staged in LDS, every
+// pair (j, kk) with kk in (j+1..C) over new and C+(0..C) over old is evaluated in the reference
+// order and kept if d <= max(root_p, root_q); key = (i - row_begin, j, kk) in loop order.
+constexpr int kJoinThreads = 256;
+constexpr int kJoinLdsBytes = 64 * 1024;
+
+__global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
+    int metric, const float* __restrict__ X, int64_t ld, int dim,
+    const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx, int C,
+    int64_t row_begin, int64_t row_end, const float* __restrict__ hd, int k, URec* U,
+    unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt, int32_t* overflow) {
+    extern __shared__ __align__(16) float jsm[];
+    __shared__ int32_t ids[128];
+    __shared__ float roots[128];
+    const int C2 = 2 * C;
+    const int64_t ldl = ld + 4;  // padded LDS row stride
+    const bool staged = (int64_t)C2 * ldl * 4 <= kJoinLdsBytes;
+    for (int64_t i = row_begin + blockIdx.x; i < row_end; i += gridDim.x) {
+        for (int c = threadIdx.x; c < C2; c += blockDim.x) {
+            const int32_t id = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
+            ids[c] = id;
+            roots[c] = id >= 0 ? hd[(int64_t)id * k] : 0.f;
+        }
+        __syncthreads();
+        if (staged) {
+            const int vec = (int)(ld >> 2);
+            for (int t = threadIdx.x; t < C2 * vec; t += blockDim.x) {
+                const int c = t / vec, e4 = t - c * vec;
+                const int32_t id = ids[c];
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (id >= 0) v = *reinterpret_cast<const float4*>(X + (int64_t)id * ld + e4 * 4);
+                *reinterpret_cast<float4*>(jsm + c * ldl + e4 * 4) = v;
+            }
+            __syncthreads();
+        }
+        const int total = C * C2;
+        const uint64_t rowkey = (uint64_t)(i - row_begin) * (uint64_t)(C * C2);
+        for (int t0 = 0; t0 < total; t0 += blockDim.x) {
+            const int t = t0 + threadIdx.x;
+            bool keep = false;
+            int32_t p = 0, q = 0;
+            float d = 0.f;
+            if (t < total) {
+                const int j = t / C2, kk = t - j * C2;
+                p = ids[j];
+                q = ids[kk];
+                const bool valid = p >= 0 && q >= 0 && (kk < C ? kk > j : p != q);
+                if (valid) {
+                    const float* xp = staged ? jsm + j * ldl : X + (int64_t)p * ld;
+                    const float* xq = staged ? jsm + kk * ldl : X + (int64_t)q * ld;
+                    d = distance_exact(metric, xp, xq, dim);
+                    keep = d <= fmaxf(roots[j], roots[kk]);
+                }
+            }
+            emit_update(keep, rowkey + (uint64_t)t, p, q, d, U, U_count, U_cap, cnt, overflow);
+        }
+        __syncthreads();
+    }
+}
+void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                      int64_t row_end, const float* heap_dist, int k, URec* U,
+                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
+                      int32_t* overflow) {
+    if (row_end <= row_begin) return;
+    const int64_t ldl = ld + 4;
+    const bool staged = (int64_t)2 * C * ldl * 4 <= kJoinLdsBytes;
+    const size_t lds = staged ? (size_t)2 * C * ldl * 4 : 16;
+    hipLaunchKernelGGL(k_join_fp32, dim3(grid_for(row_end - row_begin, 1, 1 << 18)), dim3(kJoinThreads),
+                       lds, s, metric, X, ld, dim, new_idx, old_idx, C, row_begin, row_end, heap_dist, k,
+                       U, U_count, U_cap, cnt, overflow);
+}
+
+// ---- sort_all (neighbor_heap.rs:293-320): stable ascending sort per row -------------------------
+__global__ __launch_bounds__(256) void k_sort_rows(int32_t* idx, float* dist, uint8_t* flags,
+                                                   int64_t n, int k) {
+    __shared__ uint64_t keys[4][256];
+    __shared__ int32_t si[4][256];
+    __shared__ float sd[4][256];
+    __shared__ uint8_t sf[4][256];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int S = 1;
+    while (S < k) S <<= 1;
+    for (int64_t row = (int64_t)blockIdx.x * 4 + w; row < n; row += (int64_t)gridDim.x * 4) {
+        const int64_t o = row * k;
+        for (int j = lane; j < S; j += 64) {
+            if (j < k) {
+                si[w][j] = idx[o + j];
+                sd[w][j] = dist[o + j];
+                sf[w][j] = flags[o + j];
+                keys[w][j] = ((uint64_t)float_order_key(sd[w][j]) << 32) | (uint32_t)j;
+            } else {
+                keys[w][j] = ~0ull;
+            }
+        }
+        wave_sync();
+        for (int kk = 2; kk <= S; kk <<= 1)
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < S; i += 64) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & kk) == 0;
+                        const uint64_t A = keys[w][i], B = keys[w][ixj];
+                        if ((A > B) == up) {
+                            keys[w][i] = B;
+                            keys[w][ixj] = A;
+                        }
+                    }
+                }
+                wave_sync();
+            }
+        for (int j = lane; j < k; j += 64) {
+            const int src = (int)(keys[w][j] & 0xffffffffu);
+            idx[o + j] = si[w][src];
+            dist[o + j] = sd[w][src];
+            flags[o + j] = sf[w][src];
+        }
+        wave_sync();
+    }
+}
+void launch_sort_rows(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t n, int k) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_sort_rows, dim3(grid_for(n, 4, 1 << 16)), dim3(256), 0, s, idx, dist, flags, n, k);
+}
+
+// distance corrections (core/index.rs:669-673)
+__global__ void k_correct(float* d, int64_t count, int corr) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const float x = d[t];
+        float y = x;
+        if (corr == RINND_CORR_SQRT) y = sqrtf(x);
+        else if (corr == RINND_CORR_ALT_COSINE) y = 1.0f - powf(2.0f, -x);
+        else if (corr == RINND_CORR_ALT_INNER_PRODUCT) y = x >= 3.40282347e+38f ? 0.0f : -1.0f / x;
+        d[t] = y;
+    }
+}
+void launch_correct(hipStream_t s, float* dist, int64_t count, int corr) {
+    if (count == 0 || corr == RINND_CORR_NONE) return;
+    hipLaunchKernelGGL(k_correct, dim3(grid_for(count, 256)), dim3(256), 0, s, dist, count, corr);
+}
+
+// ---- scans ---------------------------------------------------------------------------------------
+size_t scan_u64_tmp_bytes(int64_t n) {
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (unsigned long long*)nullptr,
+                                             (unsigned long long*)nullptr, (int)n));
+    return bytes;
+}
+void scan_u64_exclusive(hipStream_t s, const unsigned long long* in, uint64_t* out, int64_t n,
+                        void* tmp, size_t tmp_bytes) {
+    size_t bytes = tmp_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, (unsigned long long*)out, (int)n, s));
+}
+
+}  // namespace rinnd

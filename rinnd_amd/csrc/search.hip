@@ -1,0 +1,411 @@
+// search.hip - batched greedy graph search (core/search/greedy.rs:70-241) on the GPU.
+//
+// One wavefront per query, persistent over the batch.  Per query the wave keeps in LDS:
+//   - the result heap: a bounded std::collections::BinaryHeap<(OrderedFloat<f32>, i32)> replayed
+//     with Rust's sift_up / sift_down_to_bottom so that into_sorted's tie order (a stable sort of
+//     the heap's internal layout by distance, candidate_heap.rs:140-148) is reproduced;
+//   - the seed heap: a min-heap over (distance, id); ids are unique (visited set), so its pop order
+//     is fixed by the (distance, id) order alone.  It spills to a per-slot global region when its
+//     LDS capacity is exceeded;
+//   - the visited set: an epoch byte per vertex in a per-slot global array (VisitedSet,
+//     core/visited.rs:8-63, with an 8-bit epoch).
+// A popped vertex's CSR neighbours are checked, marked and measured by the lanes in parallel; the
+// insertion decisions, which depend on the bound updated after every insert, are then replayed
+// sequentially by lane 0 in CSR order (greedy.rs:176-200).
+#include "search.h"
+
+namespace rinnd {
+
+constexpr int kSeedLds = 768;
+constexpr int kMaxK = 512;
+constexpr int kMaxDeg = 256;
+constexpr int kFrontier = 256;
+constexpr int kQLds = 2048;
+
+__device__ __forceinline__ int ord_cmp(float ad, int32_t ai, float bd, int32_t bi) {
+    if (ad < bd) return -1;
+    if (ad > bd) return 1;
+    return ai < bi ? -1 : (ai > bi ? 1 : 0);
+}
+
+// ---- Rust std BinaryHeap (max-heap) ----
+__device__ void bh_push(float* hd, int32_t* hi, int& len, float d, int32_t x) {
+    int pos = len++;
+    while (pos > 0) {  // sift_up(0, pos)
+        const int parent = (pos - 1) / 2;
+        if (ord_cmp(d, x, hd[parent], hi[parent]) <= 0) break;
+        hd[pos] = hd[parent];
+        hi[pos] = hi[parent];
+        pos = parent;
+    }
+    hd[pos] = d;
+    hi[pos] = x;
+}
+__device__ void bh_pop(float* hd, int32_t* hi, int& len) {
+    // item = data.pop(); if !empty { swap(item, data[0]); sift_down_to_bottom(0) }
+    --len;
+    if (len == 0) return;
+    const float ed = hd[len];
+    const int32_t ei = hi[len];
+    const int end = len;
+    int pos = 0, child = 1;
+    while (end >= 2 && child <= end - 2) {
+        child += ord_cmp(hd[child], hi[child], hd[child + 1], hi[child + 1]) <= 0 ? 1 : 0;
+        hd[pos] = hd[child];
+        hi[pos] = hi[child];
+        pos = child;
+        child = 2 * pos + 1;
+    }
+    if (child == end - 1) {
+        hd[pos] = hd[child];
+        hi[pos] = hi[child];
+        pos = child;
+    }
+    while (pos > 0) {  // sift_up(0, pos)
+        const int parent = (pos - 1) / 2;
+        if (ord_cmp(ed, ei, hd[parent], hi[parent]) <= 0) break;
+        hd[pos] = hd[parent];
+        hi[pos] = hi[parent];
+        pos = parent;
+    }
+    hd[pos] = ed;
+    hi[pos] = ei;
+}
+// BoundedHeap::push (candidate_heap.rs:97-113)
+__device__ bool bounded_push(float* hd, int32_t* hi, int& len, int k, float d, int32_t x) {
+    if (len < k) {
+        bh_push(hd, hi, len, d, x);
+        return true;
+    }
+    if (len > 0 && d < hd[0]) {
+        bh_pop(hd, hi, len);
+        bh_push(hd, hi, len, d, x);
+        return true;
+    }
+    return false;
+}
+
+// ---- seed min-heap ----
+__device__ void mh_push(float* hd, int32_t* hi, int64_t& len, float d, int32_t x) {
+    int64_t pos = len++;
+    while (pos > 0) {
+        const int64_t parent = (pos - 1) >> 1;
+        if (ord_cmp(hd[parent], hi[parent], d, x) <= 0) break;
+        hd[pos] = hd[parent];
+        hi[pos] = hi[parent];
+        pos = parent;
+    }
+    hd[pos] = d;
+    hi[pos] = x;
+}
+__device__ void mh_pop(float* hd, int32_t* hi, int64_t& len, float& d, int32_t& x) {
+    d = hd[0];
+    x = hi[0];
+    --len;
+    if (len == 0) return;
+    const float ld = hd[len];
+    const int32_t li = hi[len];
+    int64_t pos = 0;
+    for (;;) {
+        int64_t c = 2 * pos + 1;
+        if (c >= len) break;
+        if (c + 1 < len && ord_cmp(hd[c + 1], hi[c + 1], hd[c], hi[c]) < 0) ++c;
+        if (ord_cmp(ld, li, hd[c], hi[c]) <= 0) break;
+        hd[pos] = hd[c];
+        hi[pos] = hi[c];
+        pos = c;
+    }
+    hd[pos] = ld;
+    hi[pos] = li;
+}
+
+struct WaveState {
+    __align__(16) float q[kQLds];
+    float rh_d[kMaxK];
+    int32_t rh_i[kMaxK];
+    float sh_d[kSeedLds];
+    int32_t sh_i[kSeedLds];
+    int32_t nb_i[kMaxDeg];
+    float nb_d[kMaxDeg];
+    uint8_t nb_ok[kMaxDeg];
+    float fr_p[kFrontier];
+    int32_t fr_n[kFrontier];
+    int64_t leaves[64 * 2];
+    int rh_len;
+    int64_t sh_len;
+    int n_leaves;
+    int done;
+    int32_t v;
+    float bound;
+    int global_mode;
+};
+
+__device__ __forceinline__ float corrected(int corr, float x) {
+    if (corr == RINND_CORR_SQRT) return sqrtf(x);
+    if (corr == RINND_CORR_ALT_COSINE) return 1.0f - powf(2.0f, -x);
+    if (corr == RINND_CORR_ALT_INNER_PRODUCT) return x >= 3.40282347e+38f ? 0.0f : -1.0f / x;
+    return x;
+}
+
+__global__ __launch_bounds__(64) void k_search(SearchArgs a) {
+    __shared__ WaveState S;
+    const int lane = threadIdx.x;
+    const int slot = blockIdx.x;
+    uint8_t* vis = a.visited + (int64_t)slot * a.n;
+    float* gsd = a.seed_spill_d + (int64_t)slot * a.seed_cap;
+    int32_t* gsi = a.seed_spill_i + (int64_t)slot * a.seed_cap;
+    const int dim = a.dim;
+    const bool q_in_lds = dim <= kQLds;
+    for (int64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
+        const int64_t qid = a.qid_base + qi;
+        const float* qg = a.Q + qi * a.ldq;
+        if (q_in_lds)
+            for (int i = lane; i < dim; i += 64) S.q[i] = qg[i];
+        const float* q = q_in_lds ? S.q : qg;
+        // epoch (VisitedSet::clear, visited.rs:51-57)
+        int ep = a.epochs[slot] + 1;
+        if (ep > 255) {
+            for (int64_t i = lane; i < a.n; i += 64) vis[i] = 0;
+            ep = 1;
+        }
+        if (lane == 0) {
+            a.epochs[slot] = ep;
+            S.rh_len = 0;
+            S.sh_len = 0;
+            S.global_mode = 0;
+        }
+        __threadfence_block();
+        wave_sync();
+        Xoshiro rng = xoshiro_seed(a.seed + (uint64_t)qid);  // FastRng::new(seed + query_id)
+        const int k = a.k;
+
+        // seed heap storage (LDS, or the per-slot global region after an overflow)
+        auto seed_push = [&](float d, int32_t x) {  // lane 0 only
+            if (!S.global_mode && S.sh_len >= kSeedLds) {
+                for (int64_t t = 0; t < S.sh_len; ++t) {
+                    gsd[t] = S.sh_d[t];
+                    gsi[t] = S.sh_i[t];
+                }
+                S.global_mode = 1;
+            }
+            if (S.global_mode) {
+                if (S.sh_len >= a.seed_cap) {
+                    *a.status = 1;
+                    return;
+                }
+                mh_push(gsd, gsi, S.sh_len, d, x);
+            } else {
+                mh_push(S.sh_d, S.sh_i, S.sh_len, d, x);
+            }
+        };
+
+        // ---- initialize_fp32 (greedy.rs:70-99) ----
+        for (int t = 0; t < a.n_trees; ++t) {
+            const int32_t* ch = a.children[t];
+            const float* hp = a.hyper[t];
+            const float* off = a.offsets[t];
+            if (lane == 0) {
+                int nl = 0;
+                if (a.leaf_budget == 1) {  // hard descent (flat_tree.rs:253-273)
+                    int64_t node = 0;
+                    while (ch[2 * node] > 0) {
+                        const float margin = d_dot8(q, hp + node * a.hdim, dim) + off[node];
+                        const int side = fabsf(margin) < 1e-8f ? (int)rng.next_bool() : (int)(margin > 0.0f);
+                        node = ch[2 * node + side];
+                    }
+                    S.leaves[0] = -ch[2 * node];
+                    S.leaves[1] = -ch[2 * node + 1];
+                    nl = 1;
+                } else {  // bounded best-first spill (flat_tree.rs:275-308)
+                    int nf = 1;
+                    S.fr_p[0] = 0.f;
+                    S.fr_n[0] = 0;
+                    while (nl < a.leaf_budget && nf > 0) {
+                        int best = 0;
+                        for (int i = 1; i < nf; ++i) {  // min_by(total_cmp): first minimum
+                            int32_t bi = __builtin_bit_cast(int32_t, S.fr_p[i]);
+                            int32_t bb = __builtin_bit_cast(int32_t, S.fr_p[best]);
+                            bi ^= (int32_t)(((uint32_t)(bi >> 31)) >> 1);
+                            bb ^= (int32_t)(((uint32_t)(bb >> 31)) >> 1);
+                            if (bi < bb) best = i;
+                        }
+                        const float prio = S.fr_p[best];
+                        const int32_t node = S.fr_n[best];
+                        for (int i = best; i + 1 < nf; ++i) {
+                            S.fr_p[i] = S.fr_p[i + 1];
+                            S.fr_n[i] = S.fr_n[i + 1];
+                        }
+                        --nf;
+                        if (ch[2 * node] <= 0) {
+                            S.leaves[2 * nl] = -ch[2 * node];
+                            S.leaves[2 * nl + 1] = -ch[2 * node + 1];
+                            ++nl;
+                            continue;
+                        }
+                        const float margin = d_dot8(q, hp + (int64_t)node * a.hdim, dim) + off[node];
+                        const int side = fabsf(margin) < 1e-8f ? (int)rng.next_bool() : (int)(margin > 0.0f);
+                        if (nf + 2 > kFrontier) {
+                            *a.status = 2;
+                            break;
+                        }
+                        S.fr_p[nf] = prio;
+                        S.fr_n[nf] = ch[2 * node + side];
+                        S.fr_p[nf + 1] = fmaxf(prio, fabsf(margin));
+                        S.fr_n[nf + 1] = ch[2 * node + 1 - side];
+                        nf += 2;
+                    }
+                }
+                S.n_leaves = nl;
+            }
+            wave_sync();
+            const int nl = S.n_leaves;
+            const int32_t* tix = a.tindices[t];
+            for (int l = 0; l < nl; ++l) {
+                const int64_t s0 = S.leaves[2 * l], s1 = S.leaves[2 * l + 1];
+                for (int64_t c0 = s0; c0 < s1; c0 += 64) {
+                    const int64_t p = c0 + lane;
+                    bool ok = false;
+                    int32_t x = -1;
+                    float d = 0.f;
+                    if (p < s1) {
+                        x = tix[p];
+                        if (x >= 0) {
+                            ok = vis[x] != ep;
+                            vis[x] = (uint8_t)ep;
+                        }
+                        if (ok) d = distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+                    }
+                    S.nb_i[lane] = x;
+                    S.nb_d[lane] = d;
+                    S.nb_ok[lane] = ok;
+                    __threadfence_block();
+                    wave_sync();
+                    if (lane == 0) {
+                        const int cnt = (int)min((int64_t)64, s1 - c0);
+                        for (int j = 0; j < cnt; ++j)
+                            if (S.nb_ok[j]) {
+                                bounded_push(S.rh_d, S.rh_i, S.rh_len, k, S.nb_d[j], S.nb_i[j]);
+                                seed_push(S.nb_d[j], S.nb_i[j]);
+                            }
+                    }
+                    wave_sync();
+                }
+            }
+        }
+        // ---- fill_random (greedy.rs:148-163) ----
+        if (lane == 0) {
+            const int64_t nfill = (int64_t)k > S.sh_len ? (int64_t)k - S.sh_len : 0;
+            for (int64_t r = 0; r < nfill; ++r) {
+                const int32_t x = (int32_t)rng.next_index((uint64_t)a.n);
+                if (vis[x] != ep) {
+                    vis[x] = (uint8_t)ep;
+                    const float d = distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+                    bounded_push(S.rh_d, S.rh_i, S.rh_len, k, d, x);
+                    seed_push(d, x);
+                }
+            }
+            const float maxd = S.rh_len > 0 ? S.rh_d[0] : __builtin_inff();
+            S.bound = maxd + a.eps * (maxd - a.min_distance);
+        }
+        __threadfence_block();
+        wave_sync();
+        // ---- traverse (greedy.rs:165-201) ----
+        for (;;) {
+            if (lane == 0) {
+                S.done = 1;
+                if (S.sh_len > 0) {
+                    float dv;
+                    int32_t v;
+                    if (S.global_mode) mh_pop(gsd, gsi, S.sh_len, dv, v);
+                    else mh_pop(S.sh_d, S.sh_i, S.sh_len, dv, v);
+                    if (!(S.rh_len >= k && dv >= S.bound)) {
+                        S.done = 0;
+                        S.v = v;
+                    }
+                }
+            }
+            wave_sync();
+            if (S.done) break;
+            const int32_t v = S.v;
+            const int32_t beg = a.indptr[v];
+            const int deg = a.indptr[v + 1] - beg;
+            for (int c0 = 0; c0 < deg; c0 += 64) {
+                const int t = c0 + lane;
+                bool ok = false;
+                int32_t x = -1;
+                float d = 0.f;
+                if (t < deg) {
+                    x = a.indices[beg + t];
+                    if (x >= 0) {
+                        ok = vis[x] != ep;
+                        vis[x] = (uint8_t)ep;
+                    }
+                    if (ok) d = distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+                }
+                S.nb_i[lane] = x;
+                S.nb_d[lane] = d;
+                S.nb_ok[lane] = ok;
+                __threadfence_block();
+                wave_sync();
+                if (lane == 0) {
+                    const int cnt = min(64, deg - c0);
+                    float bound = S.bound;
+                    for (int j = 0; j < cnt; ++j) {
+                        if (!S.nb_ok[j]) continue;
+                        const float dj = S.nb_d[j];
+                        if (dj < bound) {
+                            const bool ins = bounded_push(S.rh_d, S.rh_i, S.rh_len, k, dj, S.nb_i[j]);
+                            seed_push(dj, S.nb_i[j]);
+                            if (ins) {
+                                const float maxd = S.rh_d[0];
+                                bound = maxd + a.eps * (maxd - a.min_distance);
+                            }
+                        }
+                    }
+                    S.bound = bound;
+                }
+                wave_sync();
+            }
+        }
+        // ---- into_sorted + id remap + correction + padding (index.rs:182-199) ----
+        if (lane == 0) {
+            const int m = S.rh_len;
+            for (int i = 1; i < m; ++i) {  // stable insertion sort by distance
+                const float d = S.rh_d[i];
+                const int32_t x = S.rh_i[i];
+                int j = i - 1;
+                while (j >= 0 && S.rh_d[j] > d) {
+                    S.rh_d[j + 1] = S.rh_d[j];
+                    S.rh_i[j + 1] = S.rh_i[j];
+                    --j;
+                }
+                S.rh_d[j + 1] = d;
+                S.rh_i[j + 1] = x;
+            }
+        }
+        wave_sync();
+        const int m = S.rh_len;
+        for (int j = lane; j < k; j += 64) {
+            int32_t oi = -1;
+            float od = __builtin_inff();
+            if (j < m) {
+                const int32_t x = S.rh_i[j];
+                oi = x >= 0 ? (int32_t)a.vertex_order[x] : x;
+                od = corrected(a.corr, S.rh_d[j]);
+            }
+            a.out_idx[qi * k + j] = oi;
+            a.out_dist[qi * k + j] = od;
+        }
+        wave_sync();
+    }
+}
+
+void launch_search(hipStream_t s, const SearchArgs& a, int slots) {
+    hipLaunchKernelGGL(k_search, dim3(slots), dim3(64), 0, s, a);
+}
+
+int search_max_k() { return kMaxK; }
+int search_max_degree() { return kMaxDeg; }
+int search_max_budget() { return 64; }
+
+}  // namespace rinnd

@@ -1,0 +1,167 @@
+"""GPU parity of the NN-Descent build path against the oracle (CPU restatement of the reference).
+
+Bit-exact on the FP32 path: distances, RP forest (hyperplanes, offsets, children, leaf order), and
+the full nn_descent heap (indices, distances, flags) plus per-iteration change counts, for the
+reference's sequential candidate path (T=1) and its Rayon block path (T=8).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rinnd_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def uniform(n, d, seed=0):
+    return np.random.default_rng(seed).random((n, d), dtype=np.float32)
+
+
+def gpu_pair_distances(ctx, metric, X, a, b):
+    out = np.zeros(len(a), np.float32)
+    st = L.lib().rinnd_pair_distances(ctx.h, L.DIST[metric], L.ptr(X), X.shape[0], X.shape[1],
+                                      L.ptr(a), L.ptr(b), len(a), L.ptr(out))
+    ctx.check(st, "rinnd_pair_distances")
+    return out
+
+
+@pytest.mark.parametrize("metric", list(L.DIST))
+@pytest.mark.parametrize("d", [1, 7, 8, 13, 32, 37, 64, 96, 128, 130])
+def test_pair_distances_bit_exact(ctx, metric, d):
+    rng = np.random.default_rng(d)
+    X = (rng.standard_normal((64, d)) * 3).astype(np.float32)
+    if metric in ("alt_dot", "direct_cosine"):
+        X /= np.linalg.norm(X, axis=1, keepdims=True) + np.float32(1e-6)
+    a = rng.integers(0, 64, 500).astype(np.int32)
+    b = rng.integers(0, 64, 500).astype(np.int32)
+    got = gpu_pair_distances(ctx, metric, X, a, b)
+    want = O.pair_distances(metric, X, a, b)
+    if metric in ("alt_dot", "alt_cosine"):
+        # log2f: device libm vs glibc may differ by an ulp; the dot/norm inputs are bit-exact
+        np.testing.assert_allclose(got, want, rtol=4e-7, atol=0)
+    else:
+        np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def gpu_forest(ctx, X, n_trees, leaf, seed, angular=False):
+    st = L.rng_seed(seed)
+    h = C.c_void_p()
+    rc = L.lib().rinnd_build_forest(ctx.h, L.ptr(X), X.shape[0], X.shape[1], n_trees, leaf, int(angular), 200,
+                                    0, L.ptr(st), C.byref(h))
+    ctx.check(rc, "rinnd_build_forest")
+    return L.Forest(h), st
+
+
+@pytest.mark.parametrize("angular", [False, True])
+@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (2000, 32, 20), (3000, 37, 60)])
+def test_forest_matches_reference_streams(ctx, n, d, leaf, angular):
+    X = uniform(n, d, seed=n + d)
+    if angular:
+        X = X - 0.5
+    F, st_gpu = gpu_forest(ctx, X, 4, leaf, 42, angular)
+    st_ref = O.rng_state(42)
+    R = O.build_forest(X, 4, leaf, st_ref, angular=angular, threads=4)
+    np.testing.assert_array_equal(st_gpu, st_ref)
+    assert F.n_trees == R.n_trees == 4
+    for t in range(4):
+        g = F.tree(t, n)
+        r = R.tree(t)
+        assert g["n_nodes"] == r["n_nodes"]
+        np.testing.assert_array_equal(g["indices"], r["indices"])
+        np.testing.assert_array_equal(g["children"], r["children"])
+        np.testing.assert_array_equal(g["offsets"].view(np.uint32), r["offsets"].view(np.uint32))
+        np.testing.assert_array_equal(g["hyperplanes"].view(np.uint32), R.hyperplanes(t, d).view(np.uint32))
+
+
+def test_forest_handles_duplicate_points(ctx):
+    # all-equal rows: zero hyperplanes, margins tie -> next_bool() in examination order
+    X = np.ones((300, 16), np.float32)
+    X[150:] = 2.0
+    F, st_gpu = gpu_forest(ctx, X, 3, 10, 7)
+    st_ref = O.rng_state(7)
+    R = O.build_forest(X, 3, 10, st_ref)
+    np.testing.assert_array_equal(st_gpu, st_ref)
+    for t in range(3):
+        g, r = F.tree(t, 300), R.tree(t)
+        np.testing.assert_array_equal(g["indices"], r["indices"])
+        np.testing.assert_array_equal(g["children"], r["children"])
+
+
+def gpu_nn_descent(ctx, X, k, n_trees, leaf, C_, iters, delta=0.001, metric="sqeuclidean", threads=1,
+                   rng_mode="reference", seed=42, angular=False):
+    n = X.shape[0]
+    p = L.make_params(k, n_trees, leaf, C_, iters, delta, angular=angular, n_threads=threads, rng_mode=rng_mode)
+    st = L.rng_seed(seed)
+    idx = np.zeros((n, k), np.int32)
+    dist = np.zeros((n, k), np.float32)
+    flags = np.zeros((n, k), np.uint8)
+    stats = L.BuildStats()
+    h = C.c_void_p()
+    rc = L.lib().rinnd_nn_descent(ctx.h, L.ptr(X), n, X.shape[1], L.DIST[metric], C.byref(p), L.ptr(st),
+                                  L.ptr(idx), L.ptr(dist), L.ptr(flags), C.byref(h), C.byref(stats))
+    ctx.check(rc, "rinnd_nn_descent")
+    return idx, dist, flags, stats.to_dict(), st, L.Forest(h)
+
+
+@pytest.mark.parametrize("threads", [1, 8])
+@pytest.mark.parametrize("n,d,k,leaf", [(600, 16, 10, 20), (2500, 32, 15, 40)])
+def test_nn_descent_bit_exact(ctx, n, d, k, leaf, threads):
+    X = uniform(n, d, seed=n)
+    C_ = min(k, 60)
+    gi, gd, gf, gs, gst, _ = gpu_nn_descent(ctx, X, k, 4, leaf, C_, 6, threads=threads)
+    r = O.nn_descent(X, k, 4, leaf, C_, 6, threads=threads)
+    assert gs["changes"] == list(r["changes"])
+    np.testing.assert_array_equal(gi, r["indices"])
+    np.testing.assert_array_equal(gd.view(np.uint32), r["distances"].view(np.uint32))
+    np.testing.assert_array_equal(gf, r["flags"])
+    np.testing.assert_array_equal(gst, r["rng_state"])
+
+
+@pytest.mark.parametrize("metric,angular", [("cosine", True), ("inner_product", True), ("alt_inner_product", True)])
+def test_nn_descent_other_metrics(ctx, metric, angular):
+    X = uniform(800, 24, seed=3) - np.float32(0.3)
+    gi, gd, gf, gs, _, _ = gpu_nn_descent(ctx, X, 8, 3, 30, 8, 5, metric=metric, angular=angular, threads=4)
+    r = O.nn_descent(X, 8, 3, 30, 8, 5, metric=metric, angular=angular, threads=4)
+    np.testing.assert_array_equal(gi, r["indices"])
+    np.testing.assert_array_equal(gd.view(np.uint32), r["distances"].view(np.uint32))
+
+
+def test_nn_descent_cfg1_bit_exact(ctx):
+    """10,000 x 128 U[0,1), k=15, leaf 60, 8 trees, <=10 iterations, delta 0.001 (T=1 and T=8)."""
+    X = uniform(10000, 128, seed=0)
+    for threads in (1, 8):
+        gi, gd, gf, gs, gst, _ = gpu_nn_descent(ctx, X, 15, 8, 60, 15, 10, threads=threads)
+        r = O.nn_descent(X, 15, 8, 60, 15, 10, threads=threads)
+        assert gs["changes"] == list(r["changes"])
+        np.testing.assert_array_equal(gi, r["indices"])
+        np.testing.assert_array_equal(gd.view(np.uint32), r["distances"].view(np.uint32))
+        np.testing.assert_array_equal(gf, r["flags"])
+
+
+def test_counter_mode_quality(ctx):
+    X = uniform(3000, 16, seed=5)
+    gi, gd, gf, gs, _, _ = gpu_nn_descent(ctx, X, 10, 4, 30, 10, 8, rng_mode="counter")
+    D = ((X[:, None, :] - X[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(D, np.inf)
+    true = np.argsort(D, axis=1)[:, :10]
+    rec = np.mean([len(set(gi[i]) & set(true[i])) / 10 for i in range(len(X))])
+    assert rec > 0.9
+    assert np.all(gi != np.arange(len(X))[:, None])
+
+
+def test_sort_rows_matches_sort_all(ctx):
+    rng = np.random.default_rng(0)
+    n, k = 500, 23
+    idx = rng.integers(-1, 1000, (n, k)).astype(np.int32)
+    dist = rng.integers(0, 6, (n, k)).astype(np.float32)  # many ties: stability matters
+    dist[idx < 0] = np.inf
+    flags = rng.integers(0, 2, (n, k)).astype(np.uint8)
+    ri, rd, rf = idx.copy(), dist.copy(), flags.copy()
+    O.sort_all(ri, rd, rf)
+    gi, gd, gf = idx.copy(), dist.copy(), flags.copy()
+    ctx.check(L.lib().rinnd_sort_rows(ctx.h, L.ptr(gi), L.ptr(gd), L.ptr(gf), n, k), "rinnd_sort_rows")
+    np.testing.assert_array_equal(gi, ri)
+    np.testing.assert_array_equal(gd, rd)
+    np.testing.assert_array_equal(gf, rf)
