@@ -1,0 +1,278 @@
+#!/usr/bin/env python3
+"""Benchmark: 1M x 128 k-NN graph build points/s (BF16-MFMA join vs FP32 join) at recall@15,
+plus batched query QPS and p50/p99 latency, on one MI355X.
+
+Workload: 1,000,000 x 128 float32 integer-valued Gaussian mixture (1,000 centres U[0,100)^128,
+sigma 5, rounded and clipped to 0..255), squared Euclidean, k=15, RP forest (8 trees, leaf 75) +
+NN-Descent (max_candidates 15, <= 20 iterations, delta 0.001).  A step is one full build from
+HBM-resident data to the sorted k-NN graph (forest, leaf init, iterations, sort_all).
+
+Prints one JSON line.  `value` is the points/s of the build with the headline join precision.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--k", type=int, default=15)
+    ap.add_argument("--trees", type=int, default=8)
+    ap.add_argument("--leaf", type=int, default=75)
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--threads", type=int, default=8, help="Rayon thread count reproduced by candidate sampling")
+    ap.add_argument("--rng", default="reference", choices=["reference", "counter"])
+    ap.add_argument("--precision", default="both", choices=["fp32", "bf16", "both"])
+    ap.add_argument("--recall-rows", type=int, default=10_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-query", action="store_true")
+    ap.add_argument("--queries", type=int, default=10_000)
+    ap.add_argument("--profile", action="store_true", help="builds only (for rocprofv3 runs)")
+    return ap.parse_args()
+
+
+def gpu_build(ctx, L, X, n, dim, args, precision, outs):
+    idx, dist, flags = outs
+    p = L.make_params(args.k, args.trees, args.leaf, min(args.k, 60), args.iters, 0.001, n_threads=args.threads,
+                      rng_mode=args.rng, join_precision=precision)
+    st = L.rng_seed(42)
+    stats = L.BuildStats()
+    h = C.c_void_p()
+    rc = L.lib().rinnd_nn_descent(ctx.h, L.ptr(X), n, dim, L.DIST["sqeuclidean"], C.byref(p), L.ptr(st), L.ptr(idx),
+                                  L.ptr(dist), L.ptr(flags), C.byref(h), C.byref(stats))
+    ctx.check(rc, "rinnd_nn_descent")
+    L.lib().rinnd_forest_free(h)
+    ctx.check(L.lib().rinnd_sort_rows(ctx.h, L.ptr(idx), L.ptr(dist), L.ptr(flags), n, args.k), "rinnd_sort_rows")
+    return stats.to_dict()
+
+
+def exact_knn_f64(torch, Xd, rows, k, chunk=1024):
+    """Exact k nearest neighbours (excluding self) of Xd[rows] in float64 on the GPU."""
+    X64 = Xd.double()
+    sq = (X64 * X64).sum(1)
+    out_d, out_i = [], []
+    for s in range(0, len(rows), chunk):
+        r = torch.as_tensor(rows[s:s + chunk], device=Xd.device)
+        q = X64[r]
+        D = sq[r][:, None] + sq[None, :] - 2.0 * (q @ X64.T)
+        D[torch.arange(len(r), device=Xd.device), r] = float("inf")
+        d, i = torch.topk(D, k, dim=1, largest=False)
+        out_d.append(d)
+        out_i.append(i)
+    return torch.cat(out_d), torch.cat(out_i)
+
+
+def exact_knn_queries_f64(torch, Xd, Q, k, chunk=1024):
+    X64 = Xd.double()
+    sq = (X64 * X64).sum(1)
+    out_d, out_i = [], []
+    for s in range(0, Q.shape[0], chunk):
+        q = Q[s:s + chunk].double()
+        D = (q * q).sum(1)[:, None] + sq[None, :] - 2.0 * (q @ X64.T)
+        d, i = torch.topk(D, k, dim=1, largest=False)
+        out_d.append(d)
+        out_i.append(i)
+    return torch.cat(out_d), torch.cat(out_i)
+
+
+def tie_aware_recall(torch, Xd, src, found_idx, true_d, k):
+    """hit if the FP64 distance of a returned id is <= the k-th true FP64 distance."""
+    X64 = Xd.double()
+    f = found_idx.long().clamp(min=0)
+    d = ((X64[f] - src.double()[:, None, :]) ** 2).sum(-1)
+    d[found_idx < 0] = float("inf")
+    hits = (d <= true_d[:, k - 1:k] * (1 + 1e-12)).sum(1).double() / k
+    return float(hits.mean())
+
+
+def cpu_baseline(X, args):
+    from oracle import oracle as O
+    cores = min(os.cpu_count() or 1, 64)
+    sample = min(args.n, 50_000)
+    t = 0.0
+    while True:
+        Xs = np.ascontiguousarray(X[:sample])
+        t0 = time.perf_counter()
+        O.nn_descent(Xs, args.k, args.trees, args.leaf, min(args.k, 60), args.iters, 0.001, threads=cores)
+        t = time.perf_counter() - t0
+        if t > 8.0 or sample >= min(args.n, 400_000):
+            break
+        sample = min(args.n, sample * 2 if t > 2.0 else sample * 4)
+    return {"value": sample / t, "unit": "points/s", "cores": cores, "kind": "port",
+            "sample": f"oracle nn_descent on the first {sample} rows of the same data, same parameters, "
+                      f"{cores} threads ({t:.2f} s)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); multi-GPU runs are not part of "
+              "this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from rinnd_amd import _lib as L
+    from rinnd_amd import datasets
+
+    n, dim, k = args.n, args.dim, args.k
+    log(f"[bench] generating {n}x{dim} integer Gaussian mixture")
+    X = datasets.cfg2(n=n, d=dim) if dim == 128 or True else None
+    Xd = torch.from_numpy(X).cuda()
+    ctx = L.Context.default(0)
+    outs = (torch.empty((n, k), dtype=torch.int32, device="cuda"),
+            torch.empty((n, k), dtype=torch.float32, device="cuda"),
+            torch.empty((n, k), dtype=torch.uint8, device="cuda"))
+    precisions = ["fp32", "bf16"] if args.precision == "both" else [args.precision]
+    results = {}
+    for prec in precisions:
+        try:
+            for w in range(args.warmup):
+                log(f"[bench] warmup {prec} {w + 1}/{args.warmup}")
+                gpu_build(ctx, L, Xd, n, dim, args, prec, outs)
+            torch.cuda.synchronize()
+            times, stats = [], []
+            for s in range(args.steps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                st = gpu_build(ctx, L, Xd, n, dim, args, prec, outs)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0)
+                stats.append(st)
+                log(f"[bench] {prec} step {s + 1}: {times[-1] * 1e3:.1f} ms, iters {st['iterations']}, "
+                    f"forest {st['ms_forest']:.1f} init {st['ms_init']:.1f} cand {st['ms_candidates']:.1f} "
+                    f"join {st['ms_join']:.1f} apply {st['ms_apply']:.1f} ms")
+            results[prec] = {"times": times, "stats": stats, "idx": outs[0].clone()}
+        except Exception as e:  # report what ran
+            log(f"[bench] {prec} build failed: {e}")
+    if not results:
+        raise SystemExit("no build succeeded")
+
+    rows = np.random.default_rng(3).choice(n, size=min(args.recall_rows, n), replace=False)
+    recall = {}
+    if not args.profile:
+        log("[bench] exact FP64 k-NN of the recall rows")
+        true_d, true_i = exact_knn_f64(torch, Xd, rows, k)
+        rt = torch.as_tensor(rows, device="cuda")
+        for prec, r in results.items():
+            recall[prec] = tie_aware_recall(torch, Xd, Xd[rt], r["idx"][rt], true_d, k)
+            log(f"[bench] recall@{k} {prec}: {recall[prec]:.4f}")
+
+    head = "bf16" if "bf16" in results else "fp32"
+    T = results[head]["times"]
+    ms = 1e3 * sum(T) / len(T)
+    st = results[head]["stats"]
+    launches = sum(s["join_kernel_launches"] for s in st)
+    kms = sum(s["join_kernel_ms"] for s in st)
+    jbytes = sum(s["join_bytes"] for s in st)
+    achieved = (jbytes / launches) / (kms / launches * 1e-3) / 1e9 if launches and kms > 0 else None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "kernel": "k_join_fp32" if head == "fp32" else "k_join_bf16",
+                "avg_launch_ms": kms / launches if launches else None,
+                "bytes_per_launch": jbytes / launches if launches else None}
+
+    line = {
+        "metric": "1Mx128 kNN build pts/s BF16-join vs FP32 at recall@15; query QPS + p99",
+        "value": n / (ms / 1e3), "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head == "fp32" else "bf16-join/f32", "data": "synthetic",
+        "config": {"workload": f"{n}x{dim} integer Gaussian mixture (1000 centres, sigma 5), sqeuclidean, "
+                               f"k={k}, {args.trees} trees, leaf {args.leaf}, <= {args.iters} iterations",
+                   "head_precision": head, "rng_mode": args.rng, "n_threads": args.threads},
+        "builds": {p: {"points_per_s": n / (sum(r["times"]) / len(r["times"])),
+                       "ms_per_build": 1e3 * sum(r["times"]) / len(r["times"]),
+                       "recall_at_k": recall.get(p), "iterations": r["stats"][-1]["iterations"],
+                       "phase_ms": {x: r["stats"][-1][x] for x in
+                                    ("ms_forest", "ms_init", "ms_candidates", "ms_join", "ms_apply")}}
+                   for p, r in results.items()},
+        "roofline": roofline,
+    }
+    if "fp32" in results and "bf16" in results:
+        line["bf16_speedup"] = line["builds"]["bf16"]["points_per_s"] / line["builds"]["fp32"]["points_per_s"]
+
+    if not args.profile and not args.no_query:
+        try:
+            line["query"] = query_bench(torch, L, ctx, X, Xd, args)
+        except Exception as e:
+            log(f"[bench] query bench failed: {e}")
+    if not args.profile and not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(X, args)
+        except Exception as e:
+            log(f"[bench] cpu baseline failed: {e}")
+    print(json.dumps(line), flush=True)
+
+
+def query_bench(torch, L, ctx, X, Xd, args):
+    from rinnd_amd import datasets
+    n, dim = X.shape
+    p = L.IndexParams(args.k, args.trees, 1, 1, args.leaf, min(args.k, 60), args.iters, 0.001, 1.0, 1.5, 0, 42, 0,
+                      args.threads, L.RNG[args.rng], L.JOIN["fp32"])
+    h = C.c_void_p()
+    stats = L.BuildStats()
+    t0 = time.perf_counter()
+    ctx.check(L.lib().rinnd_index_build(ctx.h, L.ptr(Xd), n, dim, L.DIST["sqeuclidean"], L.CORR["sqrt"],
+                                        C.byref(p), C.byref(h), C.byref(stats)), "rinnd_index_build")
+    t_index = time.perf_counter() - t0
+    log(f"[bench] index built in {t_index:.2f} s")
+    rng = np.random.default_rng(8)
+    Qn = np.clip(np.floor(X[rng.integers(0, n, args.queries)] + rng.normal(0, 2.0, (args.queries, dim)) + 0.5),
+                 0, 255).astype(np.float32)
+    Qd = torch.from_numpy(Qn).cuda()
+    kq = 10
+    oi = torch.empty((args.queries, kq), dtype=torch.int32, device="cuda")
+    od = torch.empty((args.queries, kq), dtype=torch.float32, device="cuda")
+    for _ in range(2):
+        ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(0.1), L.ptr(oi), L.ptr(od)),
+                  "rinnd_index_query")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(0.1), L.ptr(oi), L.ptr(od)),
+                  "rinnd_index_query")
+    torch.cuda.synchronize()
+    qps = args.queries * reps / (time.perf_counter() - t0)
+    true_d, _ = exact_knn_queries_f64(torch, Xd, Qd, kq)
+    rec = tie_aware_recall(torch, Xd, Qd, oi, true_d, kq)
+    lat = []
+    one_i = np.empty(kq, np.int32)
+    one_d = np.empty(kq, np.float32)
+    for j in range(200):
+        q = np.ascontiguousarray(Qn[j])
+        t0 = time.perf_counter()
+        ctx.check(L.lib().rinnd_index_query_one(h, L.ptr(q), kq, C.c_float(0.1), L.ptr(one_i), L.ptr(one_d)),
+                  "rinnd_index_query_one")
+        lat.append(time.perf_counter() - t0)
+    L.lib().rinnd_index_free(h)
+    lat_ms = np.array(lat) * 1e3
+    log(f"[bench] query: {qps:.0f} QPS, recall@10 {rec:.4f}, p50 {np.percentile(lat_ms, 50):.3f} ms, "
+        f"p99 {np.percentile(lat_ms, 99):.3f} ms")
+    return {"qps": qps, "batch": args.queries, "k": kq, "epsilon": 0.1, "recall_at_10": rec,
+            "p50_ms_single": float(np.percentile(lat_ms, 50)), "p99_ms_single": float(np.percentile(lat_ms, 99)),
+            "index_build_s": t_index}
+
+
+if __name__ == "__main__":
+    main()

@@ -33,6 +33,7 @@ EXPORTS = [
     "rinnd_sort_rows", "rinnd_search_graph_build", "rinnd_index_build", "rinnd_index_free",
     "rinnd_index_query", "rinnd_index_query_one", "rinnd_index_neighbor_graph",
     "rinnd_index_search_graph_info", "rinnd_index_search_graph", "rinnd_index_vertex_order",
+    "rinnd_index_trees",
 ]
 
 

@@ -324,8 +324,17 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     DBuf<unsigned long long> rvc(n + 1);
     DBuf<float> prio((size_t)2 * n * k + 1);
     DBuf<RngStream> d_streams(64);
+    DBuf<unsigned long long> cstat(2);
     DBuf<int64_t> d_bbase(64);
     DBuf<uint64_t> d_jt;
+    const bool use_bf16 = p.join_precision == RINND_JOIN_BF16;
+    Bf16Rows B16;
+    DBuf<unsigned long long> rescored(1);
+    if (use_bf16) {
+        if (metric != RINND_DIST_SQEUCLIDEAN)
+            throw Error{RINND_ERR_UNSUPPORTED, "the BF16 join is implemented for squared Euclidean"};
+        prepare_bf16_rows(s, DX, Dld, n, dim, B16);
+    }
     const int T = std::max(1, (int)p.n_threads);
     const int T_eff = (n >= 256 && T > 1) ? T : 1;
     RINND_CHECK_ARG(T_eff <= 64, "n_threads above 64 is not supported");
@@ -402,30 +411,49 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         launch_cand_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, new_idx.p, old_idx.p, C, PL.big_list.p,
                           PL.big_count.p);
         launch_mark_old(s, h_idx, h_flags, new_idx.p, n, k, C);
-        HIP_TRY(hipEventRecord(ev[4], s));
+        HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
+        launch_cand_stats(s, new_idx.p, old_idx.p, n, C, cstat.p);
         // update_iteration: generation + ordered apply
         HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
+        HIP_TRY(hipMemsetAsync(rescored.p, 0, 8, s));
         PL.begin(s);
-        launch_join_fp32(s, metric, DX, Dld, dim, new_idx.p, old_idx.p, C, 0, n, h_dist, k, PL.U.p,
-                         PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
+        HIP_TRY(hipEventRecord(ev[4], s));
+        if (use_bf16)
+            launch_join_bf16(s, DX, Dld, dim, B16, new_idx.p, old_idx.p, C, 0, n, h_dist, k, PL.U.p, PL.U_count.p,
+                             u_cap, PL.cnt.p, PL.overflow.p, rescored.p);
+        else
+            launch_join_fp32(s, metric, DX, Dld, dim, new_idx.p, old_idx.p, C, 0, n, h_dist, k, PL.U.p,
+                             PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
         HIP_TRY(hipEventRecord(ev[5], s));
         PL.group(s);
         launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
                           PL.changes.p, PL.big_list.p, PL.big_count.p);
         HIP_TRY(hipEventRecord(ev[6], s));
-        uint64_t ch = 0, nup = 0;
+        uint64_t ch = 0, nup = 0, cs[2] = {0, 0}, nres = 0;
+        HIP_TRY(hipMemcpyAsync(cs, cstat.p, 16, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&nres, rescored.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(&ch, PL.changes.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(&nup, PL.U_count.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         PL.check_overflow(s);
-        ms_cand += elapsed_ms(ev[3], ev[4]);
+        ms_cand += elapsed_ms(ev[3], ev[4]);  // includes the small counter resets
         const double jm = elapsed_ms(ev[4], ev[5]);
         ms_join += jm;
         ms_apply += elapsed_ms(ev[5], ev[6]);
         if (it < 64) {
             st.changes[it] = (int64_t)ch;
             st.join_updates[it] = (int64_t)nup;
+            st.join_pairs[it] = (int64_t)cs[1];
+            st.join_rescored[it] = use_bf16 && !B16.exact ? (int64_t)nres : 0;
         }
+        // algorithmic bytes of the join launch: gathered candidate rows, both candidate lists, the
+        // 2C heap roots read per row, and the 24-byte update records written
+        // (BF16: 2-byte rows + norms, plus the FP32 rows of re-evaluated pairs)
+        if (use_bf16)
+            st.join_bytes += (double)cs[0] * (B16.ldb * 2.0 + 4.0) + (double)n * 2 * C * 4.0 * 2.0 +
+                             (double)nup * 24.0 + (B16.exact ? 0.0 : (double)nres * 2.0 * dim * 4.0);
+        else
+            st.join_bytes += (double)cs[0] * dim * 4.0 + (double)n * 2 * C * 4.0 * 2.0 + (double)nup * 24.0;
         st.join_kernel_ms += jm;
         st.join_kernel_launches += 1;
         if (p.verbose) std::printf("Iteration %d: %llu updates\n", it + 1, (unsigned long long)ch);

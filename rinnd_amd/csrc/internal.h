@@ -182,6 +182,10 @@ void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, in
                      const uint64_t* row_valid_prefix, const float* prio, int mode, int T,
                      int64_t block_rows, const int64_t* block_prio_base, uint64_t counter_key,
                      URec* U, unsigned long long* cnt);
+// counts[0] += valid new+old candidates, counts[1] += pairs the join evaluates (per row:
+// nn(nn-1)/2 + nn*no)
+void launch_cand_stats(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t n, int C,
+                       unsigned long long* counts);
 void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const int32_t* new_idx,
                      int64_t n, int k, int C);
 
@@ -191,6 +195,21 @@ void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int
                       int64_t row_end, const float* heap_dist, int k, URec* U,
                       unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
                       int32_t* overflow);
+
+// local join, BF16 rows on MFMA with FP32 re-evaluation (join_bf16.hip)
+struct Bf16Rows {
+    DBuf<uint16_t> Xb;  // n x ldb bf16 (centred unless exact)
+    DBuf<float> nrm;    // FP32 squared norms of the rounded rows
+    DBuf<float> mu;     // column means used for centring
+    int64_t ldb = 0;
+    int exact = 0;      // integer data with |x| <= 256 and |x|^2 <= 2^23: screened distances are exact
+};
+void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R);
+void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const Bf16Rows& R,
+                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                      int64_t row_end, const float* heap_dist, int k, URec* U,
+                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
+                      int32_t* overflow, unsigned long long* rescored);
 
 // rows
 void launch_sort_rows(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t n, int k);

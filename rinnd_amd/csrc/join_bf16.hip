@@ -1,0 +1,321 @@
+// join_bf16.hip - local join with BF16 rows on MFMA and FP32 re-evaluation inside a proven margin.
+//
+// For row i the join needs d(p, q) for p in new[i] (C) and q in new[i] ++ old[i] (2C)
+// (update_iteration, core/nndescent/mod.rs:383-447).  One wavefront per row forms the Gram matrix
+// G = A B^T of the bf16-rounded candidate rows on v_mfma_f32_16x16x16bf16 (FP32 accumulation) and
+// the screened distance  dt = |a|^2 + |b|^2 - 2 G,  where |a|^2 are FP32 norms of the rounded rows.
+//
+// Error bound.  With x^ = bf16(x) (|x^ - x| <= u|x|, u = 2^-8), D = |x - y|^2, N = |x|^2 + |y|^2:
+//   | |x^ - y^|^2 - D |  <=  2u sqrt(2 D N) + 2u^2 N                      (rounding of the rows)
+//   | dt - |x^ - y^|^2 | <=  (2 g_d + 4 u24) N^                         (FP32 norms, Gram, final ops)
+//   | D_ref - D |        <=  g_r D                                       (the reference's FP32 sum)
+// with u24 = 2^-24, g_d = d u24 / (1 - d u24), g_r = 1.01 (d + 8) u24 and N^ <= (1 + 4.1u) N.
+// A pair can pass the reference filter D_ref <= thr only if D <= thr (1 + g_r) =: Db, so every
+// such pair has dt <= thr + E with
+//   E = 1.05 (2u sqrt(2 Db N') + 2u^2 N' + (2 g_d + 4 u24) N' + g_r Db),  N' = (1 + 4.1u) N^.
+// Those pairs are re-evaluated with the reference-order FP32 distance and filtered exactly, so
+// the emitted updates - and the final heaps - equal the FP32 path's.  Rows are centred by the
+// column mean first (distances are translation invariant; N shrinks).
+//
+// Exact mode.  If every coordinate is an integer with |x| <= 256 (exact in bf16) and every
+// |x|^2 <= 2^23, all products, partial sums, norms and dt are integers below 2^24: dt equals the
+// true distance, which also equals the reference's FP32 result.  No re-evaluation is needed.
+#include <cstring>
+
+#include "internal.h"
+
+namespace rinnd {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+// column sums (FP64) for centring
+__global__ void k_col_sums(const float* __restrict__ X, int64_t ld, int64_t n, int dim, double* sums) {
+    for (int c = threadIdx.x; c < dim; c += blockDim.x) {
+        double s = 0.0;
+        for (int64_t r = blockIdx.x; r < n; r += gridDim.x) s += X[r * ld + c];
+        atomicAdd(&sums[c], s);
+    }
+}
+__global__ void k_col_mean(const double* sums, int64_t n, int dim, float* mu) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < dim; c += gridDim.x * blockDim.x)
+        mu[c] = (float)(sums[c] / (double)n);
+}
+
+// exactness test: flag[0] = 1 if some coordinate is not an integer in [-256, 256];
+// maxnorm = max over rows of |x|^2 (as ordered bits of a non-negative double)
+__global__ void k_exact_check(const float* __restrict__ X, int64_t ld, int64_t n, int dim, int* flag,
+                              unsigned long long* maxnorm) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t r = wid; r < n; r += nw) {
+        double s = 0.0;
+        int bad = 0;
+        for (int c = lane; c < dim; c += 64) {
+            const float v = X[r * ld + c];
+            bad |= !(v == rintf(v) && fabsf(v) <= 256.0f);
+            s += (double)v * (double)v;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o);
+            bad |= __shfl_xor(bad, o);
+        }
+        if (lane == 0) {
+            if (bad) atomicOr(flag, 1);
+            atomicMax(maxnorm, (unsigned long long)__double_as_longlong(s));
+        }
+    }
+}
+
+// bf16 rows (optionally centred) and FP32 norms of the rounded rows; one wave per row
+__global__ void k_to_bf16(const float* __restrict__ X, int64_t ld, int64_t n, int dim,
+                          const float* __restrict__ mu, uint16_t* __restrict__ Xb, int64_t ldb,
+                          float* __restrict__ nrm) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t r = wid; r < n; r += nw) {
+        float s = 0.f;
+        for (int c = lane; c < ldb; c += 64) {
+            uint16_t h = 0;
+            if (c < dim) {
+                const float v = mu ? X[r * ld + c] - mu[c] : X[r * ld + c];
+                h = f2bf(v);
+                const float f = bf2f(h);
+                s = __builtin_fmaf(f, f, s);
+            }
+            Xb[r * ldb + c] = h;
+        }
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) nrm[r] = s;
+    }
+}
+
+struct JoinBF16Args {
+    const float* X;  // FP32 rows (reference-order re-evaluation)
+    int64_t ld;
+    int dim;
+    const uint16_t* Xb;  // bf16 rows
+    int64_t ldb;
+    const float* nrm;
+    const int32_t* new_idx;
+    const int32_t* old_idx;
+    int C;
+    int64_t row_begin, row_end;
+    const float* hd;
+    int k;
+    int exact;
+    float g_d, g_r;
+    URec* U;
+    unsigned long long* U_count;
+    uint64_t U_cap;
+    unsigned long long* cnt;
+    int32_t* overflow;
+    unsigned long long* rescored;
+};
+
+__device__ __forceinline__ void emit_bf16(bool keep, uint64_t key, int32_t p, int32_t q, float d,
+                                          const JoinBF16Args& a) {
+    const unsigned long long bal = __ballot(keep);
+    if (bal == 0) return;
+    const int lane = lane_id();
+    const int total = __popcll(bal);
+    const int leader = __ffsll((long long)bal) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.U_count, (unsigned long long)total);
+    base = __shfl(base, leader);
+    if (keep) {
+        const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const uint64_t pos = base + __popcll(bal & lt);
+        if (pos < a.U_cap) {
+            a.U[pos] = URec{key, p, q, d, 1};
+            atomicAdd(&a.cnt[p], 1ull);
+            atomicAdd(&a.cnt[q], 1ull);
+        } else {
+            *a.overflow = 1;
+        }
+    }
+}
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
+    constexpr int W = 4;
+    __shared__ int32_t ids[W][TB * 16];
+    __shared__ float roots[W][TB * 16];
+    __shared__ float nr[W][TB * 16];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int C = a.C, C2 = 2 * C;
+    const float u = 1.0f / 256.0f, u24 = 1.0f / 16777216.0f;
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    unsigned long long resc = 0;
+    for (int64_t i = a.row_begin + (int64_t)blockIdx.x * W + w; i < a.row_end; i += (int64_t)gridDim.x * W) {
+        for (int c = lane; c < TB * 16; c += 64) {
+            int32_t id = -1;
+            if (c < C) id = a.new_idx[i * C + c];
+            else if (c < C2) id = a.old_idx[i * C + (c - C)];
+            ids[w][c] = id;
+            roots[w][c] = id >= 0 ? a.hd[(int64_t)id * a.k] : 0.f;
+            nr[w][c] = id >= 0 ? a.nrm[id] : 0.f;
+        }
+        wave_sync();
+        const uint16_t* arow[TA];
+        const uint16_t* brow[TB];
+#pragma unroll
+        for (int t = 0; t < TA; ++t) {
+            const int32_t id = (16 * t + m < C) ? ids[w][16 * t + m] : -1;
+            arow[t] = id >= 0 ? a.Xb + (int64_t)id * a.ldb + kq : nullptr;
+        }
+#pragma unroll
+        for (int t = 0; t < TB; ++t) {
+            const int32_t id = ids[w][16 * t + m];
+            brow[t] = id >= 0 ? a.Xb + (int64_t)id * a.ldb + kq : nullptr;
+        }
+        v4f acc[TA][TB];
+#pragma unroll
+        for (int x = 0; x < TA; ++x)
+#pragma unroll
+            for (int y = 0; y < TB; ++y) acc[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int64_t k0 = 0; k0 < a.ldb; k0 += 16) {
+            v4s A[TA], B[TB];
+#pragma unroll
+            for (int t = 0; t < TA; ++t)
+                A[t] = arow[t] ? *reinterpret_cast<const v4s*>(arow[t] + k0) : v4s{0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+                B[t] = brow[t] ? *reinterpret_cast<const v4s*>(brow[t] + k0) : v4s{0, 0, 0, 0};
+#pragma unroll
+            for (int x = 0; x < TA; ++x)
+#pragma unroll
+                for (int y = 0; y < TB; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[x], B[y], acc[x][y], 0, 0, 0);
+        }
+        const uint64_t rowkey = (uint64_t)(i - a.row_begin) * (uint64_t)(C * C2);
+#pragma unroll
+        for (int x = 0; x < TA; ++x)
+#pragma unroll
+            for (int y = 0; y < TB; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = 16 * x + (lane >> 4) * 4 + r;  // D[row][col] layout of 16x16 MFMA
+                    const int kk = 16 * y + m;
+                    bool keep = false;
+                    int32_t p = 0, q = 0;
+                    float d = 0.f;
+                    if (j < C && kk < C2) {
+                        p = ids[w][j];
+                        q = ids[w][kk];
+                        const bool valid = p >= 0 && q >= 0 && (kk < C ? kk > j : p != q);
+                        if (valid) {
+                            const float thr = fmaxf(roots[w][j], roots[w][kk]);
+                            const float dt = (nr[w][j] + nr[w][kk]) - 2.0f * acc[x][y][r];
+                            if (a.exact) {
+                                d = dt;
+                                keep = d <= thr;
+                            } else {
+                                bool rescore;
+                                if (!(thr < __builtin_inff())) {
+                                    rescore = true;
+                                } else {
+                                    const float Np = (nr[w][j] + nr[w][kk]) * (1.0f + 4.1f * u);
+                                    const float Db = thr * (1.0f + a.g_r);
+                                    const float E = 1.05f * (2.0f * u * sqrtf(2.0f * fmaxf(Db, 0.f) * Np) +
+                                                             2.0f * u * u * Np + (2.0f * a.g_d + 4.0f * u24) * Np +
+                                                             a.g_r * fabsf(Db)) + 1e-30f;
+                                    rescore = dt <= thr + E;
+                                }
+                                if (rescore) {
+                                    ++resc;
+                                    d = d_sqeuclidean(a.X + (int64_t)p * a.ld, a.X + (int64_t)q * a.ld, a.dim);
+                                    keep = d <= thr;
+                                }
+                            }
+                        }
+                    }
+                    emit_bf16(keep, rowkey + (uint64_t)(j * C2 + kk), p, q, d, a);
+                }
+        wave_sync();
+    }
+    for (int o = 32; o > 0; o >>= 1) resc += __shfl_xor(resc, o);
+    if (lane == 0 && resc) atomicAdd(a.rescored, resc);
+}
+
+void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R) {
+    DBuf<int> flag(1);
+    DBuf<unsigned long long> mx(1);
+    HIP_TRY(hipMemsetAsync(flag.p, 0, 4, s));
+    HIP_TRY(hipMemsetAsync(mx.p, 0, 8, s));
+    hipLaunchKernelGGL(k_exact_check, dim3(1024), dim3(256), 0, s, X, ld, n, dim, flag.p, mx.p);
+    int f = 0;
+    unsigned long long mb = 0;
+    HIP_TRY(hipMemcpyAsync(&f, flag.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&mb, mx.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double maxnorm;
+    std::memcpy(&maxnorm, &mb, 8);
+    R.exact = (f == 0 && maxnorm <= 8388608.0) ? 1 : 0;  // integers in [-256, 256], |x|^2 <= 2^23
+    R.ldb = round_up(dim, 16);
+    R.Xb.alloc((size_t)std::max<int64_t>(n, 1) * R.ldb);
+    R.nrm.alloc(std::max<int64_t>(n, 1));
+    const float* mu = nullptr;
+    if (!R.exact) {
+        DBuf<double> sums(dim);
+        HIP_TRY(hipMemsetAsync(sums.p, 0, dim * 8, s));
+        R.mu.alloc(dim);
+        hipLaunchKernelGGL(k_col_sums, dim3(1024), dim3(256), 0, s, X, ld, n, dim, sums.p);
+        hipLaunchKernelGGL(k_col_mean, dim3(ceil_div(dim, 256)), dim3(256), 0, s, sums.p, n, dim, R.mu.p);
+        mu = R.mu.p;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)std::min<int64_t>(ceil_div(n, 4), 1 << 16)), dim3(256), 0, s, X, ld,
+                       n, dim, mu, R.Xb.p, R.ldb, R.nrm.p);
+    HIP_TRY(hipGetLastError());
+}
+
+void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const Bf16Rows& R,
+                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                      int64_t row_end, const float* heap_dist, int k, URec* U,
+                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
+                      int32_t* overflow, unsigned long long* rescored) {
+    if (row_end <= row_begin) return;
+    JoinBF16Args a{};
+    a.X = X;
+    a.ld = ld;
+    a.dim = dim;
+    a.Xb = R.Xb.p;
+    a.ldb = R.ldb;
+    a.nrm = R.nrm.p;
+    a.new_idx = new_idx;
+    a.old_idx = old_idx;
+    a.C = C;
+    a.row_begin = row_begin;
+    a.row_end = row_end;
+    a.hd = heap_dist;
+    a.k = k;
+    a.exact = R.exact;
+    const double u24 = 1.0 / 16777216.0;
+    a.g_d = (float)(dim * u24 / (1.0 - dim * u24));
+    a.g_r = (float)(1.01 * (dim + 8) * u24);
+    a.U = U;
+    a.U_count = U_count;
+    a.U_cap = U_cap;
+    a.cnt = cnt;
+    a.overflow = overflow;
+    a.rescored = rescored;
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(row_end - row_begin, 4), 1 << 18);
+    if (C <= 8) hipLaunchKernelGGL((k_join_bf16<1, 1>), dim3(grid), dim3(256), 0, s, a);
+    else if (C <= 16) hipLaunchKernelGGL((k_join_bf16<1, 2>), dim3(grid), dim3(256), 0, s, a);
+    else if (C <= 32) hipLaunchKernelGGL((k_join_bf16<2, 4>), dim3(grid), dim3(256), 0, s, a);
+    else if (C <= 48) hipLaunchKernelGGL((k_join_bf16<3, 6>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_join_bf16<4, 8>), dim3(grid), dim3(256), 0, s, a);
+}
+
+}  // namespace rinnd

@@ -885,6 +885,33 @@ void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, in
                        row_valid_prefix, prio, mode, T, block_rows, block_prio_base, counter_key, U, cnt);
 }
 
+__global__ void k_cand_stats(const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx,
+                             int64_t n, int C, unsigned long long* counts) {
+    unsigned long long rows = 0, pairs = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long nn = 0, no = 0;
+        for (int c = 0; c < C; ++c) {
+            nn += new_idx[i * C + c] >= 0;
+            no += old_idx[i * C + c] >= 0;
+        }
+        rows += nn + no;
+        pairs += nn * (nn - (nn ? 1 : 0)) / 2 + nn * no;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rows += __shfl_xor(rows, o);
+        pairs += __shfl_xor(pairs, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&counts[0], rows);
+        atomicAdd(&counts[1], pairs);
+    }
+}
+void launch_cand_stats(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t n, int C,
+                       unsigned long long* counts) {
+    hipLaunchKernelGGL(k_cand_stats, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, new_idx, old_idx, n, C, counts);
+}
+
 // mark_old_flags (candidates.rs:354-417)
 __global__ void k_mark_old(const int32_t* __restrict__ idx, uint8_t* flags,
                            const int32_t* __restrict__ new_idx, int64_t n, int k, int C) {

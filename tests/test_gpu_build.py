@@ -165,3 +165,39 @@ def test_sort_rows_matches_sort_all(ctx):
     np.testing.assert_array_equal(gi, ri)
     np.testing.assert_array_equal(gd, rd)
     np.testing.assert_array_equal(gf, rf)
+
+
+@pytest.mark.parametrize("kind", ["integer", "float"])
+@pytest.mark.parametrize("C_", [10, 15, 30])
+def test_bf16_join_equals_fp32_path(ctx, kind, C_):
+    """The BF16-MFMA join with FP32 re-evaluation reproduces the FP32 path's heaps bit-exactly."""
+    rng = np.random.default_rng(C_)
+    n, d = 4000, 96 if C_ != 30 else 160
+    if kind == "integer":  # exact mode: integer coordinates, |x|^2 <= 2^23
+        centres = rng.random((20, d)) * 100
+        X = np.clip(np.floor(centres[rng.integers(0, 20, n)] + 5 * rng.standard_normal((n, d)) + 0.5), 0, 255)
+        X = X.astype(np.float32)
+    else:
+        X = rng.random((n, d), dtype=np.float32)
+    k = C_
+    a = gpu_nn_descent(ctx, X, k, 3, 40, C_, 5, threads=4)
+    p = L.make_params(k, 3, 40, C_, 5, 0.001, n_threads=4, join_precision="bf16")
+    st = L.rng_seed(42)
+    idx = np.zeros((n, k), np.int32)
+    dist = np.zeros((n, k), np.float32)
+    flags = np.zeros((n, k), np.uint8)
+    stats = L.BuildStats()
+    h = C.c_void_p()
+    rc = L.lib().rinnd_nn_descent(ctx.h, L.ptr(X), n, d, L.DIST["sqeuclidean"], C.byref(p), L.ptr(st), L.ptr(idx),
+                                  L.ptr(dist), L.ptr(flags), C.byref(h), C.byref(stats))
+    ctx.check(rc, "rinnd_nn_descent(bf16)")
+    L.lib().rinnd_forest_free(h)
+    s = stats.to_dict()
+    assert s["changes"] == a[3]["changes"]
+    np.testing.assert_array_equal(idx, a[0])
+    np.testing.assert_array_equal(dist.view(np.uint32), a[1].view(np.uint32))
+    np.testing.assert_array_equal(flags, a[2])
+    if kind == "integer":
+        assert sum(s["join_rescored"]) == 0
+    else:
+        assert 0 < sum(s["join_rescored"]) < sum(s["join_pairs"])
