@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--trees", type=int, default=8)
     ap.add_argument("--leaf", type=int, default=75)
     ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--delta", type=float, default=0.001)
     ap.add_argument("--threads", type=int, default=8, help="Rayon thread count reproduced by candidate sampling")
     ap.add_argument("--rng", default="reference", choices=["reference", "counter"])
     ap.add_argument("--precision", default="both", choices=["fp32", "bf16", "both"])
@@ -55,7 +56,7 @@ def parse():
 
 def gpu_build(ctx, L, X, n, dim, args, precision, outs):
     idx, dist, flags = outs
-    p = L.make_params(args.k, args.trees, args.leaf, min(args.k, 60), args.iters, 0.001, n_threads=args.threads,
+    p = L.make_params(args.k, args.trees, args.leaf, min(args.k, 60), args.iters, args.delta, n_threads=args.threads,
                       rng_mode=args.rng, join_precision=precision)
     st = L.rng_seed(42)
     stats = L.BuildStats()

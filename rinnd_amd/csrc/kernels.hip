@@ -568,13 +568,90 @@ __device__ __forceinline__ bool heap_push(int32_t* hi, float* hd, uint8_t* hf, i
     return true;
 }
 
+// Where a destination's sorted records live: one per lane (groups of <= 64) or in memory.
+struct RecSrc {
+    const SegRec* mem;
+    int32_t rx;
+    float rv;
+    bool in_regs;
+    __device__ __forceinline__ void get(int r, int32_t& x, float& v) const {
+        if (in_regs) {
+            x = __builtin_amdgcn_readlane(rx, r);
+            v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, rv), r));
+        } else {
+            const SegRec q = mem[r];
+            x = q.x;
+            v = q.v;
+        }
+    }
+};
+
+__device__ __forceinline__ float readlane_f(float x, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, x), l));
+}
+
 struct HeapApply {
     int32_t* idx;
     float* dist;
     uint8_t* flags;
     int k;
     unsigned long long* changes;
+    __device__ static bool kRegs(const HeapApply& a) { return a.k <= RINND_WAVE; }
+    // k <= 64: heap slot j lives in lane j; the duplicate scan is one ballot and the sift-down of
+    // neighbor_heap.rs:207-243 walks the heap with uniform readlane reads.
+    __device__ void apply_regs(int64_t v, const RecSrc& src, int L, int lane) const {
+        const int64_t o = v * k;
+        float hd = lane < k ? dist[o + lane] : __builtin_inff();
+        int32_t hi = lane < k ? idx[o + lane] : -1;
+        int32_t hf = lane < k ? flags[o + lane] : 0;
+        unsigned long long ch = 0;
+        for (int r = 0; r < L; ++r) {
+            int32_t x;
+            float d;
+            src.get(r, x, d);
+            if (d >= readlane_f(hd, 0)) continue;          // checked_flagged_push: reject d >= root
+            if (__ballot(lane < k && hi == x)) continue;   // duplicate id
+            int pos = 0;
+            for (;;) {
+                int child = 2 * pos + 1;
+                if (child >= k) break;
+                float dc = readlane_f(hd, child);
+                if (child + 1 < k) {
+                    const float dr = readlane_f(hd, child + 1);
+                    if (dc < dr) {
+                        child = child + 1;
+                        dc = dr;
+                    }
+                }
+                if (d >= dc) break;
+                const int32_t ic = __builtin_amdgcn_readlane(hi, child);
+                const int32_t fc = __builtin_amdgcn_readlane(hf, child);
+                if (lane == pos) {
+                    hd = dc;
+                    hi = ic;
+                    hf = fc;
+                }
+                pos = child;
+            }
+            if (lane == pos) {
+                hd = d;
+                hi = x;
+                hf = 1;
+            }
+            ++ch;
+        }
+        if (lane < k) {
+            idx[o + lane] = hi;
+            dist[o + lane] = hd;
+            flags[o + lane] = (uint8_t)hf;
+        }
+        if (lane == 0 && ch) atomicAdd(changes, ch);
+    }
     __device__ void apply(int64_t v, const SegRec* recs, int64_t L, char* scratch, int lane) const {
+        if (k <= RINND_WAVE) {
+            apply_regs(v, RecSrc{recs, 0, 0.f, false}, (int)L, lane);
+            return;
+        }
         int32_t* hi = (int32_t*)scratch;
         float* hd = (float*)(hi + k);
         uint8_t* hf = (uint8_t*)(hd + k);
@@ -625,11 +702,69 @@ __device__ __forceinline__ void cand_push(float* pri, int32_t* ix, int n, float 
     }
 }
 
+// checked_heap_push_flat on a register-resident heap (slot j in lane j).
+__device__ __forceinline__ void cand_push_regs(float& pri, int32_t& ix, int n, float priority, int32_t index,
+                                               int lane) {
+    if (priority >= readlane_f(pri, 0)) return;
+    if (__ballot(lane < n && ix == index)) return;
+    int pos = 0;
+    for (;;) {
+        const int l = 2 * pos + 1, r = 2 * pos + 2;
+        int best = pos;
+        float bp = priority;  // the sifted value occupies `pos`
+        if (l < n) {
+            const float pl = readlane_f(pri, l);
+            if (pl > bp) {
+                best = l;
+                bp = pl;
+            }
+        }
+        if (r < n) {
+            const float pr = readlane_f(pri, r);
+            if (pr > bp) {
+                best = r;
+                bp = pr;
+            }
+        }
+        if (best == pos) break;
+        const int32_t ib = __builtin_amdgcn_readlane(ix, best);
+        if (lane == pos) {
+            pri = bp;
+            ix = ib;
+        }
+        pos = best;
+    }
+    if (lane == pos) {
+        pri = priority;
+        ix = index;
+    }
+}
+
 struct CandApply {
     int32_t* new_idx;
     int32_t* old_idx;
     int C;
+    __device__ static bool kRegs(const CandApply& a) { return a.C <= RINND_WAVE; }
+    __device__ void apply_regs(int64_t v, const RecSrc& src, int L, int lane) const {
+        float np = __builtin_inff(), op = __builtin_inff();
+        int32_t ni = -1, oi = -1;
+        for (int r = 0; r < L; ++r) {
+            int32_t x;
+            float pri;
+            src.get(r, x, pri);
+            if (x < 0) cand_push_regs(np, ni, C, pri, x & 0x7fffffff, lane);  // flag bit set: "new" edge
+            else cand_push_regs(op, oi, C, pri, x, lane);
+        }
+        if (lane < C) {
+            new_idx[v * C + lane] = ni;
+            old_idx[v * C + lane] = oi;
+        }
+    }
     __device__ void apply(int64_t v, const SegRec* recs, int64_t L, char* scratch, int lane) const {
+        if (C <= RINND_WAVE) {
+            apply_regs(v, RecSrc{recs, 0, 0.f, false}, (int)L, lane);
+            return;
+        }
         int32_t* ni = (int32_t*)scratch;
         int32_t* oi = ni + C;
         float* np = (float*)(oi + C);
@@ -672,6 +807,34 @@ __global__ __launch_bounds__(kApplyWaves * RINND_WAVE) void k_seg_apply(
         if (L == 0) continue;
         if (L > kSortCap) {
             if (lane == 0) big_list[atomicAdd(big_count, 1)] = (int32_t)v;
+            continue;
+        }
+        if (L <= RINND_WAVE && Ap::kRegs(ap)) {
+            // one record per lane, bitonic network across lanes (keys are unique; pads sort last)
+            uint64_t key = ~0ull;
+            int32_t rx = 0;
+            float rv = 0.f;
+            if (lane < L) {
+                const SegRec q = seg[o0 + lane];
+                key = q.key;
+                rx = q.x;
+                rv = q.v;
+            }
+            for (int kk = 2; kk <= RINND_WAVE; kk <<= 1)
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    const uint64_t pk = __shfl_xor(key, j);
+                    const int32_t px = __shfl_xor(rx, j);
+                    const float pv = __shfl_xor(rv, j);
+                    const bool up = (lane & kk) == 0;
+                    const bool lower = (lane & j) == 0;
+                    const bool gt = key > pk;
+                    if (key != pk && (lower ? (gt == up) : (gt != up))) {
+                        key = pk;
+                        rx = px;
+                        rv = pv;
+                    }
+                }
+            ap.apply_regs(v, RecSrc{nullptr, rx, rv, true}, (int)L, lane);
             continue;
         }
         const int S = next_pow2(L);
