@@ -139,6 +139,13 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
             HIP_TRY(hipMemsetAsync(T.hyper.p, 0, T.hyper.bytes(), s));
             HIP_TRY(hipMemsetAsync(T.offsets.p, 0, T.offsets.bytes(), s));
         }
+        if (rng_mode == RINND_RNG_COUNTER) {  // all trees and nodes of a level at once
+            std::vector<uint64_t> keys(n_trees);
+            for (int64_t t = 0; t < n_trees; ++t) keys[t] = states[t * 4] ^ states[t * 4 + 3];
+            if (build_forest_level(s, DX, Dld, dim, n, leaf_size, max_depth, angular, keys, F)) return;
+            cap = 2 * n + 1;
+            continue;
+        }
         const size_t N1 = (size_t)(n + 1);
         DBuf<uint8_t> side(n_trees * N1);
         DBuf<int32_t> rankR(n_trees * N1), PQ(n_trees * 2 * N1), tmp(n_trees * N1);
@@ -285,7 +292,20 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     Pipeline PL;
     const uint64_t join_rows_cap = (uint64_t)n;  // whole iteration in one generation pass
     uint64_t u_cap = std::max<uint64_t>((uint64_t)2 * n * k, join_rows_cap * (uint64_t)C * 2 * C);
-    {
+    if (leaf_init_supported(k, p.leaf_size)) {  // per-leaf fused pushes, tree by tree
+        DBuf<int32_t> mark(n + 1), end_at(n + 1);
+        DBuf<int64_t> rank(2 * (n + 1)), leaves(2 * (n + 1));
+        size_t lt_bytes = std::max(scan_u64_tmp_bytes(n + 2), (size_t)16);
+        DBuf<char> ltmp(lt_bytes);
+        for (size_t t = 0; t < F.trees.size(); ++t) {
+            launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
+                             leaves.p, ltmp.p, lt_bytes);
+            launch_leaf_init(s, metric, DX, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
+                             h_idx, h_dist, h_flags, k);
+        }
+        HIP_TRY(hipStreamSynchronize(s));  // the leaf buffers die here
+        PL.init(n, u_cap);
+    } else {
         DBuf<int32_t> mark(n + 1), end_at(n + 1);
         DBuf<int64_t> rank(2 * (n + 1)), leaves(2 * (n + 1));
         DBuf<uint64_t> pc(n + 2), pb(n + 2);

@@ -1,0 +1,369 @@
+// forest_level.hip - RP forest with counter-based draws, built level-synchronously over all trees.
+//
+// Same tree algorithm as TreeBuilder (core/tree/builder.rs:172-348): a node is a leaf when
+// m <= leaf_size or depth >= max_depth; otherwise it splits by the bisector of two random members
+// (euclidean) or by the normalised difference of their directions (angular), a point goes left iff
+// its margin is < 0 (ties |margin| < 1e-8 decided by a random bit), and a degenerate split (all
+// points on one side) makes the node a leaf.  The random draws come from mix64(tree key, node id,
+// position) instead of a depth-first FastRng stream, so every node of a level is independent and
+// one level is processed for all trees at once:
+//   k_lvl_planes   one workgroup per active node: split points, hyperplane, offset (or leaf)
+//   k_lvl_margins  16 lanes per (tree, position): side of the node's hyperplane
+//   scan           exclusive prefix of the left flags over all (tree, position)
+//   k_lvl_counts   left size per node; degenerate splits become leaves; children counted per tree
+//   k_lvl_assign   child node ids (breadth-first per tree) and the next level's active list
+//   k_lvl_scatter  stable partition of every split node's positions
+// Node ids are breadth-first per tree (root 0); leaves hold their [start, end) range of
+// tree.indices exactly as in the depth-first layout.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.h"
+
+namespace rinnd {
+
+struct LvlNode {
+    int32_t tree, node, start, end, depth;
+};
+
+struct LvlArgs {
+    const float* X;
+    int64_t ld;
+    int dim;
+    int64_t n;
+    int64_t leaf_size, max_depth;
+    int angular;
+    const uint64_t* keys;  // per tree
+    float** hyper;         // per tree, cap x dim
+    float** offsets;       // per tree, cap
+    int32_t** children;    // per tree, cap x 2
+};
+
+static inline unsigned lgrid(int64_t work, int block, int64_t cap = 1 << 20) {
+    const int64_t g = ceil_div(work, (int64_t)block);
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+__device__ __forceinline__ float block_sum(float v, float* red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    float t = 0.f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[i];
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* nodes, int64_t na,
+                                                    const int32_t* const* perm, uint8_t* split) {
+    __shared__ float red[4];
+    for (int64_t e = blockIdx.x; e < na; e += gridDim.x) {
+        const LvlNode nd = nodes[e];
+        const int64_t m = nd.end - nd.start;
+        int32_t* ch = a.children[nd.tree];
+        if (m <= a.leaf_size || nd.depth >= a.max_depth) {
+            if (threadIdx.x == 0) {
+                ch[2 * nd.node] = -nd.start;
+                ch[2 * nd.node + 1] = -nd.end;
+                split[e] = 0;
+            }
+            continue;  // uniform across the block
+        }
+        const uint64_t key = a.keys[nd.tree];
+        const uint64_t h1 = mix64(key ^ mix64(2 * (uint64_t)nd.node + 1));
+        const uint64_t h2 = mix64(key ^ mix64(2 * (uint64_t)nd.node + 2));
+        const int64_t i1 = (int64_t)(h1 % (uint64_t)m);
+        int64_t i2 = (int64_t)(h2 % (uint64_t)(m - 1));
+        if (i2 >= i1) ++i2;
+        const float* p1 = a.X + (int64_t)perm[nd.tree][nd.start + i1] * a.ld;
+        const float* p2 = a.X + (int64_t)perm[nd.tree][nd.start + i2] * a.ld;
+        float* hp = a.hyper[nd.tree] + (int64_t)nd.node * a.dim;
+        if (!a.angular) {
+            float nsq = 0.f, dnm = 0.f;
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
+                const float h = p2[i] - p1[i];
+                nsq += h * h;
+                dnm += h * ((p1[i] + p2[i]) * 0.5f);
+            }
+            nsq = block_sum(nsq, red);
+            dnm = block_sum(dnm, red);
+            const float norm = sqrtf(nsq);
+            const bool scale = norm > 1e-8f;
+            const float inv = scale ? 1.0f / norm : 0.f;
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x)
+                hp[i] = scale ? (p2[i] - p1[i]) * inv : p2[i] - p1[i];
+            if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = scale ? -dnm * inv : 0.f;
+        } else {
+            float n1 = 0.f, n2 = 0.f;
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
+                n1 += p1[i] * p1[i];
+                n2 += p2[i] * p2[i];
+            }
+            n1 = sqrtf(block_sum(n1, red));
+            n2 = sqrtf(block_sum(n2, red));
+            const bool deg = n1 < 1e-8f || n2 < 1e-8f;
+            float nn = 0.f;
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
+                const float h = deg ? 0.f : p2[i] / n2 - p1[i] / n1;
+                nn += h * h;
+            }
+            nn = sqrtf(block_sum(nn, red));
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
+                const float h = deg ? 0.f : p2[i] / n2 - p1[i] / n1;
+                hp[i] = nn > 1e-8f ? h / nn : h;
+            }
+            if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = 0.f;
+        }
+        if (threadIdx.x == 0) split[e] = 1;
+    }
+}
+
+// Left flag per (tree, position); positions outside split nodes count as left (flag 1) so the
+// prefix over any node's range counts only its own members.  16 lanes per item; V4 needs
+// dim % 4 == 0 (16-byte aligned hyperplane rows; data rows are padded to 8 floats).
+template <bool V4>
+__global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* nodes, int n_trees,
+                                                     const int32_t* const* perm, const int32_t* const* act,
+                                                     const uint8_t* split, uint32_t* flags) {
+    const int gl = threadIdx.x & 15;
+    const int64_t total = (int64_t)n_trees * a.n;
+    const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 4;
+    for (int64_t item = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4; item < total; item += groups) {
+        const int t = (int)(item / a.n);
+        const int64_t pos = item - (int64_t)t * a.n;
+        const int32_t e = act[t][pos];
+        const bool active = e >= 0 && split[e];
+        float acc = 0.f;
+        int32_t node = 0;
+        if (active) {
+            node = nodes[e].node;
+            const float* x = a.X + (int64_t)perm[t][pos] * a.ld;
+            const float* hp = a.hyper[t] + (int64_t)node * a.dim;
+            if (V4) {
+                for (int i = 4 * gl; i < a.dim; i += 64) {
+                    const float4 xv = *reinterpret_cast<const float4*>(x + i);
+                    const float4 hv = *reinterpret_cast<const float4*>(hp + i);
+                    acc += xv.x * hv.x + xv.y * hv.y + xv.z * hv.z + xv.w * hv.w;
+                }
+            } else {
+                for (int i = gl; i < a.dim; i += 16) acc += x[i] * hp[i];
+            }
+        }
+        for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // stays inside the 16-lane group
+        if (gl == 0) {
+            uint32_t left = 1;
+            if (active) {
+                const float margin = acc + a.offsets[t][node];
+                if (fabsf(margin) < 1e-8f)
+                    left = (uint32_t)(mix64(a.keys[t] ^ mix64(((uint64_t)(uint32_t)node << 32) ^ (uint64_t)pos)) & 1);
+                else
+                    left = margin < 0.0f ? 1u : 0u;
+            }
+            flags[item] = left;
+        }
+    }
+}
+
+__global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const uint32_t* prefix, uint8_t* split,
+                             uint32_t* nchild, uint32_t* leftsz, unsigned* tree_children) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
+        const LvlNode nd = nodes[e];
+        uint32_t c = 0;
+        if (split[e]) {
+            const int64_t off = (int64_t)nd.tree * a.n;
+            const uint32_t L = prefix[off + nd.end] - prefix[off + nd.start];
+            const uint32_t m = (uint32_t)(nd.end - nd.start);
+            leftsz[e] = L;
+            if (L == 0 || L == m) {  // degenerate split -> leaf with a zero hyperplane (builder.rs:198-210)
+                split[e] = 0;
+                a.children[nd.tree][2 * nd.node] = -nd.start;
+                a.children[nd.tree][2 * nd.node + 1] = -nd.end;
+                float* hp = a.hyper[nd.tree] + (int64_t)nd.node * a.dim;
+                for (int i = 0; i < a.dim; ++i) hp[i] = 0.f;
+                a.offsets[nd.tree][nd.node] = 0.f;
+            } else {
+                c = 2;
+                atomicAdd(&tree_children[nd.tree], 2u);
+            }
+        }
+        nchild[e] = c;
+    }
+}
+
+__global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* split, const uint32_t* cscan,
+                             const uint32_t* leftsz, const int32_t* tree_base_id, const uint32_t* tree_cum,
+                             int32_t** children, LvlNode* next) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
+        if (!split[e]) continue;
+        const LvlNode nd = nodes[e];
+        const uint32_t s = cscan[e];  // the active list stays grouped by tree, so s - tree_cum is per tree
+        const int32_t left = tree_base_id[nd.tree] + (int32_t)(s - tree_cum[nd.tree]);
+        children[nd.tree][2 * nd.node] = left;
+        children[nd.tree][2 * nd.node + 1] = left + 1;
+        const int32_t mid = nd.start + (int32_t)leftsz[e];
+        next[s] = LvlNode{nd.tree, left, nd.start, mid, nd.depth + 1};
+        next[s + 1] = LvlNode{nd.tree, left + 1, mid, nd.end, nd.depth + 1};
+    }
+}
+
+__global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, const uint8_t* split,
+                              const uint32_t* cscan, const uint32_t* leftsz, const uint32_t* prefix,
+                              const uint32_t* flags, const int32_t* const* perm, int32_t* const* perm2,
+                              const int32_t* const* act, int32_t* const* act2) {
+    const int64_t total = (int64_t)n_trees * n;
+    for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
+         item += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(item / n);
+        const int64_t pos = item - (int64_t)t * n;
+        const int32_t e = act[t][pos];
+        if (e < 0 || !split[e]) {
+            perm2[t][pos] = perm[t][pos];
+            act2[t][pos] = -1;
+            continue;
+        }
+        const LvlNode nd = nodes[e];
+        const int64_t off = (int64_t)t * n;
+        const uint32_t lr = prefix[off + pos] - prefix[off + nd.start];  // left members before pos
+        const bool left = flags[item] != 0;
+        const int64_t np = left ? nd.start + lr : nd.start + leftsz[e] + (pos - nd.start - lr);
+        perm2[t][np] = perm[t][pos];
+        act2[t][np] = (int32_t)cscan[e] + (left ? 0 : 1);
+    }
+}
+
+__global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act) {
+    const int64_t total = (int64_t)n_trees * n;
+    for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
+         item += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(item / n);
+        const int64_t pos = item - (int64_t)t * n;
+        perm[t][pos] = (int32_t)pos;
+        act[t][pos] = t;  // level 0: the root of tree t is active entry t
+    }
+}
+
+bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
+                        int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F) {
+    const int T = (int)F.trees.size();
+    const int64_t total = (int64_t)T * n;
+    if (T == 0) return true;
+    RINND_CHECK_ARG(total < (int64_t)INT32_MAX - 1, "n_trees x n must stay below 2^31 for the level forest builder");
+    const int64_t cap = F.trees[0].cap_nodes;
+    std::vector<int32_t*> perm_h(T), perm2_h(T), act_h(T), act2_h(T), ch_h(T);
+    std::vector<float*> hp_h(T), of_h(T);
+    DBuf<int32_t> perm2(total), act(total), act2(total);
+    for (int t = 0; t < T; ++t) {
+        perm_h[t] = F.trees[t].indices.p;
+        perm2_h[t] = perm2.p + (int64_t)t * n;
+        act_h[t] = act.p + (int64_t)t * n;
+        act2_h[t] = act2.p + (int64_t)t * n;
+        ch_h[t] = F.trees[t].children.p;
+        hp_h[t] = F.trees[t].hyper.p;
+        of_h[t] = F.trees[t].offsets.p;
+    }
+    // pointer tables [perm, perm2, act, act2] for the two ping-pong phases
+    std::vector<int32_t*> tab(8 * (size_t)T);
+    for (int t = 0; t < T; ++t) {
+        tab[0 * T + t] = perm_h[t];
+        tab[1 * T + t] = perm2_h[t];
+        tab[2 * T + t] = act_h[t];
+        tab[3 * T + t] = act2_h[t];
+        tab[4 * T + t] = perm2_h[t];
+        tab[5 * T + t] = perm_h[t];
+        tab[6 * T + t] = act2_h[t];
+        tab[7 * T + t] = act_h[t];
+    }
+    DBuf<int32_t*> d_tab(tab.size()), d_ch(T);
+    DBuf<float*> d_hp(T), d_of(T);
+    DBuf<uint64_t> d_keys(T);
+    HIP_TRY(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * sizeof(int32_t*), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_ch.p, ch_h.data(), T * sizeof(int32_t*), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_hp.p, hp_h.data(), T * sizeof(float*), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_of.p, of_h.data(), T * sizeof(float*), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_keys.p, keys.data(), T * 8, hipMemcpyHostToDevice, s));
+    const LvlArgs a{X, ld, dim, n, std::max<int64_t>(leaf_size, 1), max_depth, angular,
+                    d_keys.p, d_hp.p, d_of.p, d_ch.p};
+
+    DBuf<uint32_t> flags(total + 1), prefix(total + 1);
+    size_t scan_bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags.p, prefix.p, (int)(total + 1)));
+    DBuf<char> scan_tmp(std::max<size_t>(scan_bytes, 16));
+    // a level holds at most n nodes per tree (every active node has at least one member)
+    const int64_t max_nodes = std::max<int64_t>(1, std::min<int64_t>(cap, n)) * (int64_t)T;
+    DBuf<LvlNode> cur(max_nodes + 1), nxt(max_nodes + 1);
+    DBuf<uint8_t> split(max_nodes + 1);
+    DBuf<uint32_t> nchild(max_nodes + 1), cscan(max_nodes + 1), leftsz(max_nodes + 1);
+    DBuf<unsigned> tree_children(T);
+    DBuf<int32_t> d_base(T);
+    DBuf<uint32_t> d_cum(T);
+    size_t cscan_bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, cscan_bytes, nchild.p, cscan.p, (int)(max_nodes + 1)));
+    DBuf<char> cscan_tmp(std::max<size_t>(cscan_bytes, 16));
+
+    std::vector<LvlNode> roots(T);
+    for (int t = 0; t < T; ++t) roots[t] = LvlNode{t, 0, 0, (int32_t)n, 0};
+    HIP_TRY(hipMemcpyAsync(cur.p, roots.data(), T * sizeof(LvlNode), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T);
+    std::vector<int64_t> node_count(T, 1);
+    std::vector<unsigned> tc(T);
+    std::vector<int32_t> base(T);
+    std::vector<uint32_t> cum(T);
+    int64_t na = T;
+    int phase = 0;
+    const bool v4 = dim % 4 == 0;
+    while (na > 0) {
+        int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
+        hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1)), dim3(256), 0, s, a, cur.p, na,
+                           (const int32_t* const*)P, split.p);
+        const unsigned mg = lgrid(total * 16, 256, 1 << 16);
+        if (v4)
+            hipLaunchKernelGGL(k_lvl_margins<true>, dim3(mg), dim3(256), 0, s, a, cur.p, T,
+                               (const int32_t* const*)P, (const int32_t* const*)(P + 2 * T), split.p, flags.p);
+        else
+            hipLaunchKernelGGL(k_lvl_margins<false>, dim3(mg), dim3(256), 0, s, a, cur.p, T,
+                               (const int32_t* const*)P, (const int32_t* const*)(P + 2 * T), split.p, flags.p);
+        HIP_TRY(hipMemsetAsync(flags.p + total, 0, 4, s));
+        size_t b = scan_bytes;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
+        HIP_TRY(hipMemsetAsync(tree_children.p, 0, T * sizeof(unsigned), s));
+        hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur.p, na, prefix.p, split.p,
+                           nchild.p, leftsz.p, tree_children.p);
+        HIP_TRY(hipMemsetAsync(nchild.p + na, 0, 4, s));
+        size_t cb = cscan_bytes;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
+        HIP_TRY(hipMemcpyAsync(tc.data(), tree_children.p, T * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        uint32_t acc = 0;
+        int64_t next_na = 0;
+        bool over = false;
+        for (int t = 0; t < T; ++t) {
+            base[t] = (int32_t)node_count[t];
+            cum[t] = acc;
+            acc += tc[t];
+            node_count[t] += tc[t];
+            next_na += tc[t];
+            if (node_count[t] > cap) over = true;
+        }
+        if (over) return false;
+        HIP_TRY(hipMemcpyAsync(d_base.p, base.data(), T * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_cum.p, cum.data(), T * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur.p, na, split.p, cscan.p,
+                           leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt.p);
+        hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur.p, T, n, split.p,
+                           cscan.p, leftsz.p, prefix.p, flags.p, (const int32_t* const*)P, P + T,
+                           (const int32_t* const*)(P + 2 * T), P + 3 * T);
+        HIP_TRY(hipGetLastError());
+        std::swap(cur, nxt);
+        phase ^= 1;
+        na = next_na;
+    }
+    // after an odd number of levels the final order sits in the second buffer
+    for (int t = 0; t < T; ++t) {
+        if (phase == 1 && n > 0) HIP_TRY(hipMemcpyAsync(perm_h[t], perm2_h[t], n * 4, hipMemcpyDeviceToDevice, s));
+        F.trees[t].n_nodes = node_count[t];
+    }
+    HIP_TRY(hipStreamSynchronize(s));  // host vectors used by the copies above go out of scope
+    return true;
+}
+
+}  // namespace rinnd

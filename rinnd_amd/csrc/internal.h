@@ -132,24 +132,24 @@ struct ForestDFSArgs {
 };
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a);
 
-// forest, counter streams (level-synchronous across trees)
-struct ForestLevelArgs {
-    const float* X;
-    int64_t ld;
-    int dim;
-    int64_t n;
-    int64_t leaf_size;
-    int64_t max_depth;
-    int angular;
-    int n_trees;
-    const uint64_t* tree_keys;  // one 64-bit key per tree
-};
+// forest, counter streams (level-synchronous across trees, forest_level.hip).  Trees must be
+// allocated with cap_nodes each and zeroed hyperplanes/offsets; returns false when a tree needs
+// more than cap_nodes nodes.
+bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
+                        int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F);
 
 // leaves of one tree in depth-first order: (start, end) pairs, returns count via device int
 void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, int64_t n,
                       int32_t* leaf_mark /* n+1 scratch */, int64_t* leaf_rank /* n+1 */,
                       int32_t* leaf_end_at /* n */, int64_t* leaves /* 2 per leaf */,
                       void* scan_tmp, size_t scan_tmp_bytes);
+
+// initialize_from_leaves fused per leaf (leaf_init.hip); n_leaves is read on the device,
+// max_leaves bounds the grid.  Supported when a 64..256-thread block's heaps fit in LDS.
+bool leaf_init_supported(int k, int64_t leaf_size);
+void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
+                      const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
+                      int32_t* idx, float* dist, uint8_t* flags, int k);
 
 // ordered-apply pipeline
 void launch_leaf_pairs(hipStream_t s, int metric, const float* X, int64_t ld, int dim,

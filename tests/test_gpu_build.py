@@ -89,6 +89,88 @@ def test_forest_handles_duplicate_points(ctx):
         np.testing.assert_array_equal(g["children"], r["children"])
 
 
+def gpu_forest_counter(ctx, X, n_trees, leaf, seed, angular=False, max_depth=200):
+    st = L.rng_seed(seed)
+    h = C.c_void_p()
+    rc = L.lib().rinnd_build_forest(ctx.h, L.ptr(X), X.shape[0], X.shape[1], n_trees, leaf, int(angular), max_depth,
+                                    L.RNG["counter"], L.ptr(st), C.byref(h))
+    ctx.check(rc, "rinnd_build_forest(counter)")
+    return L.Forest(h)
+
+
+def check_tree_structure(X, tr, leaf, max_depth=200):
+    """Every point in exactly one leaf, leaves tile tree.indices, each member lies on its side of every
+    ancestor hyperplane (left iff margin < 0; near-zero margins may go either way).  Returns the number
+    of points in leaves above leaf_size that the depth limit does not explain (degenerate splits)."""
+    n = X.shape[0]
+    idx, ch, hp, off = tr["indices"], tr["children"], tr["hyperplanes"].astype(np.float64), tr["offsets"]
+    assert sorted(idx.tolist()) == list(range(n))
+    leaves = []
+    stack = [(0, 0, [])]
+    seen = set()
+    while stack:
+        node, depth, path = stack.pop()
+        assert node not in seen
+        seen.add(node)
+        a, b = int(ch[node, 0]), int(ch[node, 1])
+        if b <= 0:  # leaf [-a, -b); internal nodes hold child ids >= 1
+            leaves.append((-a, -b, depth, path, node))
+            continue
+        stack.append((b, depth + 1, path + [(node, 1)]))
+        stack.append((a, depth + 1, path + [(node, 0)]))
+    assert len(seen) == tr["n_nodes"]
+    leaves.sort()
+    assert leaves[0][0] == 0 and leaves[-1][1] == n
+    assert all(p[1] == q[0] for p, q in zip(leaves, leaves[1:]))
+    Xd = X.astype(np.float64)
+    oversized = 0
+    for s, e, depth, path, node in leaves:
+        rows = Xd[idx[s:e]]
+        assert not np.any(hp[node]) and off[node] == 0
+        if e - s > leaf and depth < max_depth:
+            oversized += e - s
+        for anc, side in path:
+            m = rows @ hp[anc] + off[anc]
+            tol = 1e-4 * (1 + np.abs(rows).sum(1))
+            if side == 0:
+                assert np.all(m < tol)
+            else:
+                assert np.all(m > -tol)
+    return oversized
+
+
+@pytest.mark.parametrize("angular", [False, True])
+@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (3000, 36, 30), (20000, 128, 60)])
+def test_counter_forest_structure(ctx, n, d, leaf, angular):
+    X = uniform(n, d, seed=n + d + 1)
+    if angular:
+        X = X - 0.5
+    F = gpu_forest_counter(ctx, X, 5, leaf, 11, angular)
+    G = gpu_forest_counter(ctx, X, 5, leaf, 11, angular)
+    for t in range(5):
+        tr = F.tree(t, n)
+        assert check_tree_structure(X, tr, leaf) == 0
+        g = G.tree(t, n)  # deterministic
+        np.testing.assert_array_equal(tr["indices"], g["indices"])
+        np.testing.assert_array_equal(tr["children"], g["children"])
+    if n > 100:  # different trees
+        assert not np.array_equal(F.tree(0, n)["indices"], F.tree(1, n)["indices"])
+
+
+def test_counter_forest_duplicates_and_depth_limit(ctx):
+    X = np.ones((400, 16), np.float32)
+    X[200:] = 3.0
+    F = gpu_forest_counter(ctx, X, 3, 10, 5)
+    for t in range(3):
+        assert check_tree_structure(X, F.tree(t, 400), 10) <= 40  # all-tie nodes split by random bits
+    X = uniform(5000, 8, seed=2)
+    F = gpu_forest_counter(ctx, X, 2, 10, 5, max_depth=3)
+    for t in range(2):
+        tr = F.tree(t, 5000)
+        assert check_tree_structure(X, tr, 10, max_depth=3) == 0
+        assert tr["n_nodes"] <= 15
+
+
 def gpu_nn_descent(ctx, X, k, n_trees, leaf, C_, iters, delta=0.001, metric="sqeuclidean", threads=1,
                    rng_mode="reference", seed=42, angular=False):
     n = X.shape[0]
