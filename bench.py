@@ -163,6 +163,9 @@ def main():
                 log(f"[bench] {prec} step {s + 1}: {times[-1] * 1e3:.1f} ms, iters {st['iterations']}, "
                     f"forest {st['ms_forest']:.1f} init {st['ms_init']:.1f} cand {st['ms_candidates']:.1f} "
                     f"join {st['ms_join']:.1f} apply {st['ms_apply']:.1f} ms")
+                it = st["iterations"]
+                log(f"[bench]   per iteration: pairs {st['join_pairs'][:it]} updates {st['join_updates'][:it]} "
+                    f"changes {st['changes'][:it]} rescored {st['join_rescored'][:it]}")
             results[prec] = {"times": times, "stats": stats, "idx": outs[0].clone()}
         except Exception as e:  # report what ran
             log(f"[bench] {prec} build failed: {e}")

@@ -12,7 +12,7 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librinnd_hip.so")
+LIB_PATH = os.environ.get("RINND_HIP_LIB") or os.path.join(HERE, "librinnd_hip.so")
 
 OK = 0
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "INTERNAL"}

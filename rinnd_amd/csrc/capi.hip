@@ -290,8 +290,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
 
     // ---- initialize_from_leaves (mod.rs:203-300), tree by tree ----
     Pipeline PL;
-    const uint64_t join_rows_cap = (uint64_t)n;  // whole iteration in one generation pass
-    uint64_t u_cap = std::max<uint64_t>((uint64_t)2 * n * k, join_rows_cap * (uint64_t)C * 2 * C);
+    uint64_t u_cap = (uint64_t)2 * n * k;  // candidate records per iteration
     if (leaf_init_supported(k, p.leaf_size)) {  // per-leaf fused pushes, tree by tree
         DBuf<int32_t> mark(n + 1), end_at(n + 1);
         DBuf<int64_t> rank(2 * (n + 1)), leaves(2 * (n + 1));
@@ -347,6 +346,10 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     DBuf<unsigned long long> cstat(2);
     DBuf<int64_t> d_bbase(64);
     DBuf<uint64_t> d_jt;
+    DBuf<unsigned long long> rpc(n + 1);
+    DBuf<uint64_t> rowoff(n + 1);
+    DBuf<uint32_t> kc(std::max<int64_t>(n, 1));
+    DBuf<JRec> JU;
     const bool use_bf16 = p.join_precision == RINND_JOIN_BF16;
     Bf16Rows B16;
     DBuf<unsigned long long> rescored(1);
@@ -433,29 +436,34 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         launch_mark_old(s, h_idx, h_flags, new_idx.p, n, k, C);
         HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
         launch_cand_stats(s, new_idx.p, old_idx.p, n, C, cstat.p);
-        // update_iteration: generation + ordered apply
+        // update_iteration: generation into per-row regions, grouping by destination, ordered apply
         HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
         HIP_TRY(hipMemsetAsync(rescored.p, 0, 8, s));
+        launch_row_pairs(s, new_idx.p, old_idx.p, 0, n, C, rpc.p, rowoff.p, PL.scan_tmp.p, PL.scan_bytes);
+        JU.ensure(std::max<uint64_t>(d2h_u64(s, rowoff.p + n), 1));
+        const JoinOut jo{rowoff.p, JU.p, kc.p, PL.cnt.p};
         PL.begin(s);
         HIP_TRY(hipEventRecord(ev[4], s));
         if (use_bf16)
-            launch_join_bf16(s, DX, Dld, dim, B16, new_idx.p, old_idx.p, C, 0, n, h_dist, k, PL.U.p, PL.U_count.p,
-                             u_cap, PL.cnt.p, PL.overflow.p, rescored.p);
+            launch_join_bf16(s, DX, Dld, dim, B16, new_idx.p, old_idx.p, C, 0, n, h_dist, k, jo, rescored.p);
         else
-            launch_join_fp32(s, metric, DX, Dld, dim, new_idx.p, old_idx.p, C, 0, n, h_dist, k, PL.U.p,
-                             PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
+            launch_join_fp32(s, metric, DX, Dld, dim, new_idx.p, old_idx.p, C, 0, n, h_dist, k, jo);
         HIP_TRY(hipEventRecord(ev[5], s));
-        PL.group(s);
+        scan_u64_exclusive(s, PL.cnt.p, PL.off.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
+        const uint64_t n_rec = d2h_u64(s, PL.off.p + n);
+        PL.seg.ensure(std::max<uint64_t>(n_rec, 1));
+        PL.seg2.ensure(std::max<uint64_t>(n_rec, 1));
+        HIP_TRY(hipMemsetAsync(PL.cur.p, 0, (n + 1) * 8, s));
+        launch_join_scatter(s, new_idx.p, old_idx.p, C, 0, n, jo, PL.off.p, PL.cur.p, PL.seg.p);
         launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
                           PL.changes.p, PL.big_list.p, PL.big_count.p);
         HIP_TRY(hipEventRecord(ev[6], s));
-        uint64_t ch = 0, nup = 0, cs[2] = {0, 0}, nres = 0;
+        uint64_t ch = 0, cs[2] = {0, 0}, nres = 0;
+        const uint64_t nup = n_rec / 2;
         HIP_TRY(hipMemcpyAsync(cs, cstat.p, 16, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(&nres, rescored.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(&ch, PL.changes.p, 8, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&nup, PL.U_count.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        PL.check_overflow(s);
         ms_cand += elapsed_ms(ev[3], ev[4]);  // includes the small counter resets
         const double jm = elapsed_ms(ev[4], ev[5]);
         ms_join += jm;
@@ -467,13 +475,13 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             st.join_rescored[it] = use_bf16 && !B16.exact ? (int64_t)nres : 0;
         }
         // algorithmic bytes of the join launch: gathered candidate rows, both candidate lists, the
-        // 2C heap roots read per row, and the 24-byte update records written
+        // 2C heap roots read per row, and the 8-byte kept-pair records written
         // (BF16: 2-byte rows + norms, plus the FP32 rows of re-evaluated pairs)
         if (use_bf16)
             st.join_bytes += (double)cs[0] * (B16.ldb * 2.0 + 4.0) + (double)n * 2 * C * 4.0 * 2.0 +
-                             (double)nup * 24.0 + (B16.exact ? 0.0 : (double)nres * 2.0 * dim * 4.0);
+                             (double)nup * 8.0 + (B16.exact ? 0.0 : (double)nres * 2.0 * dim * 4.0);
         else
-            st.join_bytes += (double)cs[0] * dim * 4.0 + (double)n * 2 * C * 4.0 * 2.0 + (double)nup * 24.0;
+            st.join_bytes += (double)cs[0] * dim * 4.0 + (double)n * 2 * C * 4.0 * 2.0 + (double)nup * 8.0;
         st.join_kernel_ms += jm;
         st.join_kernel_launches += 1;
         if (p.verbose) std::printf("Iteration %d: %llu updates\n", it + 1, (unsigned long long)ch);

@@ -71,6 +71,7 @@ __device__ __forceinline__ float d_sqeuclidean(const float* __restrict__ a,
                                                const float* __restrict__ b, int d) {
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int chunks = d >> 3;
+#pragma unroll 4
     for (int c = 0; c < chunks; ++c) {
         const float4 a0 = *reinterpret_cast<const float4*>(a + c * 8);
         const float4 a1 = *reinterpret_cast<const float4*>(a + c * 8 + 4);
@@ -99,6 +100,7 @@ __device__ __forceinline__ float d_dot8(const float* __restrict__ a, const float
                                         int d) {
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int chunks = d >> 3;
+#pragma unroll 4
     for (int c = 0; c < chunks; ++c) {
         const float4 a0 = *reinterpret_cast<const float4*>(a + c * 8);
         const float4 a1 = *reinterpret_cast<const float4*>(a + c * 8 + 4);
@@ -125,6 +127,7 @@ __device__ __forceinline__ float d_cosine(const float* __restrict__ a, const flo
     float va[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float vb[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int chunks = d >> 3;
+#pragma unroll 2
     for (int c = 0; c < chunks; ++c) {
 #pragma unroll
         for (int l = 0; l < 8; ++l) {
@@ -156,6 +159,7 @@ __device__ __forceinline__ float d_normdot(const float* __restrict__ a, const fl
 #pragma unroll
         for (int l = 0; l < 8; ++l) s[q][l] = 0.f;
     const int chunks = d / 32;
+#pragma unroll 2
     for (int c = 0; c < chunks; ++c) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)

@@ -189,12 +189,31 @@ void launch_cand_stats(hipStream_t s, const int32_t* new_idx, const int32_t* old
 void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const int32_t* new_idx,
                      int64_t n, int k, int C);
 
-// local join
+// local join.  Row r = i - row_begin owns the region U[rowoff[r], rowoff[r+1]) sized by its valid
+// pair count; the join writes its kept pairs there as (slot t = j*2C + kk, d) in any order, the kept
+// count to kc[r], and adds each candidate's share to cnt[id] (one atomic per candidate and row).
+struct JRec {
+    uint32_t t;
+    float d;
+};
+static_assert(sizeof(JRec) == 8, "JRec layout");
+struct JoinOut {
+    const uint64_t* rowoff;
+    JRec* U;
+    uint32_t* kc;
+    unsigned long long* cnt;
+};
+// rowoff[r] = exclusive prefix of nn(nn-1)/2 + nn*no over the rows (rowoff[n_rows] = total)
+void launch_row_pairs(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t row_begin,
+                      int64_t row_end, int C, unsigned long long* counts, uint64_t* rowoff, void* scan_tmp,
+                      size_t scan_tmp_bytes);
 void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                      int64_t row_end, const float* heap_dist, int k, URec* U,
-                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
-                      int32_t* overflow);
+                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out);
+// kept pairs -> both destinations' segments (off/cur as in launch_scatter), key = r*C*2C + t
+void launch_join_scatter(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                         int64_t row_end, const JoinOut& out, const uint64_t* off, unsigned long long* cur,
+                         SegRec* seg);
 
 // local join, BF16 rows on MFMA with FP32 re-evaluation (join_bf16.hip)
 struct Bf16Rows {
@@ -207,9 +226,8 @@ struct Bf16Rows {
 void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R);
 void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const Bf16Rows& R,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                      int64_t row_end, const float* heap_dist, int k, URec* U,
-                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
-                      int32_t* overflow, unsigned long long* rescored);
+                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out,
+                      unsigned long long* rescored);
 
 // rows
 void launch_sort_rows(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t n, int k);

@@ -57,22 +57,22 @@ __global__ void k_exact_check(const float* __restrict__ X, int64_t ld, int64_t n
     const int lane = threadIdx.x & 63;
     const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    double mx = 0.0;
+    int bad = 0;
     for (int64_t r = wid; r < n; r += nw) {
         double s = 0.0;
-        int bad = 0;
         for (int c = lane; c < dim; c += 64) {
             const float v = X[r * ld + c];
             bad |= !(v == rintf(v) && fabsf(v) <= 256.0f);
             s += (double)v * (double)v;
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            s += __shfl_xor(s, o);
-            bad |= __shfl_xor(bad, o);
-        }
-        if (lane == 0) {
-            if (bad) atomicOr(flag, 1);
-            atomicMax(maxnorm, (unsigned long long)__double_as_longlong(s));
-        }
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        mx = fmax(mx, s);
+    }
+    for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o);
+    if (lane == 0) {  // one pair of atomics per wave
+        if (bad) atomicOr(flag, 1);
+        atomicMax(maxnorm, (unsigned long long)__double_as_longlong(mx));
     }
 }
 
@@ -115,36 +115,9 @@ struct JoinBF16Args {
     int k;
     int exact;
     float g_d, g_r;
-    URec* U;
-    unsigned long long* U_count;
-    uint64_t U_cap;
-    unsigned long long* cnt;
-    int32_t* overflow;
+    JoinOut out;
     unsigned long long* rescored;
 };
-
-__device__ __forceinline__ void emit_bf16(bool keep, uint64_t key, int32_t p, int32_t q, float d,
-                                          const JoinBF16Args& a) {
-    const unsigned long long bal = __ballot(keep);
-    if (bal == 0) return;
-    const int lane = lane_id();
-    const int total = __popcll(bal);
-    const int leader = __ffsll((long long)bal) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(a.U_count, (unsigned long long)total);
-    base = __shfl(base, leader);
-    if (keep) {
-        const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const uint64_t pos = base + __popcll(bal & lt);
-        if (pos < a.U_cap) {
-            a.U[pos] = URec{key, p, q, d, 1};
-            atomicAdd(&a.cnt[p], 1ull);
-            atomicAdd(&a.cnt[q], 1ull);
-        } else {
-            *a.overflow = 1;
-        }
-    }
-}
 
 template <int TA, int TB>
 __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
@@ -152,7 +125,9 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
     __shared__ int32_t ids[W][TB * 16];
     __shared__ float roots[W][TB * 16];
     __shared__ float nr[W][TB * 16];
+    __shared__ unsigned cc[W][TB * 16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     const int C = a.C, C2 = 2 * C;
     const float u = 1.0f / 256.0f, u24 = 1.0f / 16777216.0f;
     const int m = lane & 15, kq = (lane >> 4) * 4;
@@ -165,8 +140,12 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
             ids[w][c] = id;
             roots[w][c] = id >= 0 ? a.hd[(int64_t)id * a.k] : 0.f;
             nr[w][c] = id >= 0 ? a.nrm[id] : 0.f;
+            cc[w][c] = 0;
         }
         wave_sync();
+        const int64_t r0 = i - a.row_begin;
+        JRec* U = a.out.U + a.out.rowoff[r0];
+        unsigned run = 0;
         const uint16_t* arow[TA];
         const uint16_t* brow[TB];
 #pragma unroll
@@ -198,7 +177,6 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                 for (int y = 0; y < TB; ++y)
                     acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[x], B[y], acc[x][y], 0, 0, 0);
         }
-        const uint64_t rowkey = (uint64_t)(i - a.row_begin) * (uint64_t)(C * C2);
 #pragma unroll
         for (int x = 0; x < TA; ++x)
 #pragma unroll
@@ -240,8 +218,19 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                             }
                         }
                     }
-                    emit_bf16(keep, rowkey + (uint64_t)(j * C2 + kk), p, q, d, a);
+                    // kept pairs go to the row's own region in any order (the slot carries the key)
+                    const unsigned long long bal = __ballot(keep);
+                    if (keep) {
+                        U[run + __popcll(bal & lt)] = JRec{(uint32_t)(j * C2 + kk), d};
+                        atomicAdd(&cc[w][j], 1u);
+                        atomicAdd(&cc[w][kk], 1u);
+                    }
+                    run += (unsigned)__popcll(bal);
                 }
+        wave_sync();
+        if (lane == 0) a.out.kc[r0] = run;
+        for (int c = lane; c < C2; c += 64)
+            if (cc[w][c]) atomicAdd(&a.out.cnt[ids[w][c]], (unsigned long long)cc[w][c]);
         wave_sync();
     }
     for (int o = 32; o > 0; o >>= 1) resc += __shfl_xor(resc, o);
@@ -282,9 +271,8 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
 
 void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const Bf16Rows& R,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                      int64_t row_end, const float* heap_dist, int k, URec* U,
-                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
-                      int32_t* overflow, unsigned long long* rescored) {
+                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out,
+                      unsigned long long* rescored) {
     if (row_end <= row_begin) return;
     JoinBF16Args a{};
     a.X = X;
@@ -304,11 +292,7 @@ void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const 
     const double u24 = 1.0 / 16777216.0;
     a.g_d = (float)(dim * u24 / (1.0 - dim * u24));
     a.g_r = (float)(1.01 * (dim + 8) * u24);
-    a.U = U;
-    a.U_count = U_count;
-    a.U_cap = U_cap;
-    a.cnt = cnt;
-    a.overflow = overflow;
+    a.out = out;
     a.rescored = rescored;
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(row_end - row_begin, 4), 1 << 18);
     if (C <= 8) hipLaunchKernelGGL((k_join_bf16<1, 1>), dim3(grid), dim3(256), 0, s, a);

@@ -1099,30 +1099,64 @@ void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const in
 
 // ---- local join, FP32 (update_iteration generation, mod.rs:383-447) --------------------------------
 // One workgroup per row i: the row's new (C) and old (C) candidate rows are staged in LDS, every
-// pair (j, kk) with kk in (j+1..C) over new and C+(0..C) over old is evaluated in the reference
-// order and kept if d <= max(root_p, root_q); key = (i - row_begin, j, kk) in loop order.
+// pair (j, kk) with kk in (j+1..C) over new and C+(0..C) over old is evaluated with the reference's
+// arithmetic and kept if d <= max(root_p, root_q).  Kept pairs go to the row's own region of U
+// (no global counter); per-candidate counts are summed in LDS and added to cnt once per row.
 constexpr int kJoinThreads = 256;
 constexpr int kJoinLdsBytes = 64 * 1024;
 
+__global__ void k_row_pairs(const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx,
+                            int64_t row_begin, int64_t row_end, int C, unsigned long long* out) {
+    const int64_t rows = row_end - row_begin;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        if (r == rows) {
+            out[r] = 0;
+            continue;
+        }
+        const int64_t i = row_begin + r;
+        unsigned long long nn = 0, no = 0;
+        for (int c = 0; c < C; ++c) {
+            nn += new_idx[i * C + c] >= 0;
+            no += old_idx[i * C + c] >= 0;
+        }
+        out[r] = nn * (nn - (nn > 0)) / 2 + nn * no;
+    }
+}
+void launch_row_pairs(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t row_begin,
+                      int64_t row_end, int C, unsigned long long* counts, uint64_t* rowoff, void* scan_tmp,
+                      size_t scan_tmp_bytes) {
+    const int64_t rows = std::max<int64_t>(row_end - row_begin, 0);
+    hipLaunchKernelGGL(k_row_pairs, dim3(grid_for(rows + 1, 256)), dim3(256), 0, s, new_idx, old_idx, row_begin,
+                       row_end, C, counts);
+    scan_u64_exclusive(s, counts, rowoff, rows + 1, scan_tmp, scan_tmp_bytes);
+}
+
+template <bool STAGED>
 __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
     int metric, const float* __restrict__ X, int64_t ld, int dim,
     const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx, int C,
-    int64_t row_begin, int64_t row_end, const float* __restrict__ hd, int k, URec* U,
-    unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt, int32_t* overflow) {
+    int64_t row_begin, int64_t row_end, const float* __restrict__ hd, int k, JoinOut out) {
     extern __shared__ __align__(16) float jsm[];
     __shared__ int32_t ids[128];
     __shared__ float roots[128];
+    __shared__ unsigned cc[128];
+    __shared__ unsigned run;
     const int C2 = 2 * C;
     const int64_t ldl = ld + 4;  // padded LDS row stride
-    const bool staged = (int64_t)C2 * ldl * 4 <= kJoinLdsBytes;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int64_t i = row_begin + blockIdx.x; i < row_end; i += gridDim.x) {
+        const int64_t r = i - row_begin;
         for (int c = threadIdx.x; c < C2; c += blockDim.x) {
             const int32_t id = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
             ids[c] = id;
             roots[c] = id >= 0 ? hd[(int64_t)id * k] : 0.f;
+            cc[c] = 0;
         }
+        if (threadIdx.x == 0) run = 0;
         __syncthreads();
-        if (staged) {
+        if (STAGED) {
             const int vec = (int)(ld >> 2);
             for (int t = threadIdx.x; t < C2 * vec; t += blockDim.x) {
                 const int c = t / vec, e4 = t - c * vec;
@@ -1133,42 +1167,113 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
             }
             __syncthreads();
         }
+        JRec* U = out.U + out.rowoff[r];
         const int total = C * C2;
-        const uint64_t rowkey = (uint64_t)(i - row_begin) * (uint64_t)(C * C2);
         for (int t0 = 0; t0 < total; t0 += blockDim.x) {
             const int t = t0 + threadIdx.x;
             bool keep = false;
-            int32_t p = 0, q = 0;
             float d = 0.f;
+            int j = 0, kk = 0;
             if (t < total) {
-                const int j = t / C2, kk = t - j * C2;
-                p = ids[j];
-                q = ids[kk];
+                j = t / C2;
+                kk = t - j * C2;
+                const int32_t p = ids[j], q = ids[kk];
                 const bool valid = p >= 0 && q >= 0 && (kk < C ? kk > j : p != q);
                 if (valid) {
-                    const float* xp = staged ? jsm + j * ldl : X + (int64_t)p * ld;
-                    const float* xq = staged ? jsm + kk * ldl : X + (int64_t)q * ld;
+                    const float* xp = STAGED ? jsm + j * ldl : X + (int64_t)p * ld;
+                    const float* xq = STAGED ? jsm + kk * ldl : X + (int64_t)q * ld;
                     d = distance_exact(metric, xp, xq, dim);
                     keep = d <= fmaxf(roots[j], roots[kk]);
                 }
             }
-            emit_update(keep, rowkey + (uint64_t)t, p, q, d, U, U_count, U_cap, cnt, overflow);
+            const unsigned long long bal = __ballot(keep);
+            if (bal) {
+                const int leader = __ffsll((long long)bal) - 1;
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(&run, (unsigned)__popcll(bal));
+                base = __shfl(base, leader);
+                if (keep) {
+                    U[base + __popcll(bal & lt)] = JRec{(uint32_t)t, d};
+                    atomicAdd(&cc[j], 1u);
+                    atomicAdd(&cc[kk], 1u);
+                }
+            }
         }
+        __syncthreads();
+        if (threadIdx.x == 0) out.kc[r] = run;
+        for (int c = threadIdx.x; c < C2; c += blockDim.x)
+            if (cc[c]) atomicAdd(&out.cnt[ids[c]], (unsigned long long)cc[c]);
         __syncthreads();
     }
 }
 void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                      int64_t row_end, const float* heap_dist, int k, URec* U,
-                      unsigned long long* U_count, uint64_t U_cap, unsigned long long* cnt,
-                      int32_t* overflow) {
+                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out) {
     if (row_end <= row_begin) return;
     const int64_t ldl = ld + 4;
     const bool staged = (int64_t)2 * C * ldl * 4 <= kJoinLdsBytes;
     const size_t lds = staged ? (size_t)2 * C * ldl * 4 : 16;
-    hipLaunchKernelGGL(k_join_fp32, dim3(grid_for(row_end - row_begin, 1, 1 << 18)), dim3(kJoinThreads),
-                       lds, s, metric, X, ld, dim, new_idx, old_idx, C, row_begin, row_end, heap_dist, k,
-                       U, U_count, U_cap, cnt, overflow);
+    const dim3 grid(grid_for(row_end - row_begin, 1, 1 << 18));
+    if (staged)
+        hipLaunchKernelGGL(k_join_fp32<true>, grid, dim3(kJoinThreads), lds, s, metric, X, ld, dim, new_idx,
+                           old_idx, C, row_begin, row_end, heap_dist, k, out);
+    else
+        hipLaunchKernelGGL(k_join_fp32<false>, grid, dim3(kJoinThreads), lds, s, metric, X, ld, dim, new_idx,
+                           old_idx, C, row_begin, row_end, heap_dist, k, out);
+}
+
+// Kept pairs of each row -> the segments of both endpoints.  One wave per row: per-candidate record
+// counts in LDS, one cursor atomic per candidate, then slots within the candidate's block.
+__global__ __launch_bounds__(256) void k_join_scatter(const int32_t* __restrict__ new_idx,
+                                                      const int32_t* __restrict__ old_idx, int C,
+                                                      int64_t row_begin, int64_t row_end, JoinOut out,
+                                                      const uint64_t* __restrict__ off, unsigned long long* cur,
+                                                      SegRec* seg) {
+    __shared__ int32_t ids[4][128];
+    __shared__ unsigned lc[4][128];
+    __shared__ uint64_t base[4][128];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int C2 = 2 * C;
+    for (int64_t i = row_begin + (int64_t)blockIdx.x * 4 + w; i < row_end; i += (int64_t)gridDim.x * 4) {
+        const int64_t r = i - row_begin;
+        for (int c = lane; c < C2; c += 64) {
+            ids[w][c] = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
+            lc[w][c] = 0;
+        }
+        wave_sync();
+        const uint32_t nk = out.kc[r];
+        const JRec* E = out.U + out.rowoff[r];
+        for (uint32_t e = lane; e < nk; e += 64) {
+            const uint32_t t = E[e].t;
+            const uint32_t j = t / (uint32_t)C2, kk = t - j * (uint32_t)C2;
+            atomicAdd(&lc[w][j], 1u);
+            atomicAdd(&lc[w][kk], 1u);
+        }
+        wave_sync();
+        for (int c = lane; c < C2; c += 64) {
+            const unsigned v = lc[w][c];
+            base[w][c] = v ? off[ids[w][c]] + atomicAdd(&cur[ids[w][c]], (unsigned long long)v) : 0;
+            lc[w][c] = 0;
+        }
+        wave_sync();
+        const uint64_t rowkey = (uint64_t)r * (uint64_t)(C * C2);
+        for (uint32_t e = lane; e < nk; e += 64) {
+            const JRec q = E[e];
+            const uint32_t j = q.t / (uint32_t)C2, kk = q.t - j * (uint32_t)C2;
+            const unsigned sj = atomicAdd(&lc[w][j], 1u);
+            seg[base[w][j] + sj] = SegRec{rowkey + q.t, ids[w][kk], q.d};
+            const unsigned sk = atomicAdd(&lc[w][kk], 1u);
+            seg[base[w][kk] + sk] = SegRec{rowkey + q.t, ids[w][j], q.d};
+        }
+        wave_sync();
+    }
+}
+void launch_join_scatter(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                         int64_t row_end, const JoinOut& out, const uint64_t* off, unsigned long long* cur,
+                         SegRec* seg) {
+    if (row_end <= row_begin) return;
+    hipLaunchKernelGGL(k_join_scatter, dim3(grid_for(ceil_div(row_end - row_begin, (int64_t)4), 1, 1 << 18)),
+                       dim3(256), 0, s, new_idx, old_idx, C, row_begin, row_end, out, off, cur, seg);
 }
 
 // ---- sort_all (neighbor_heap.rs:293-320): stable ascending sort per row -------------------------

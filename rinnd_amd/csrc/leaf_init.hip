@@ -22,6 +22,7 @@ constexpr int kInitLdsBudget = 96 * 1024;
 __device__ __forceinline__ void heap_push_col(float* hd, int32_t* hi, uint8_t* hf, int B, int c, int k, int32_t x,
                                               float d) {
     if (d >= hd[c]) return;
+#pragma unroll 4
     for (int j = 0; j < k; ++j)
         if (hi[j * B + c] == x) return;
     int pos = 0, child = 1;  // replace the root and sift down (neighbor_heap.rs:207-243)
@@ -81,7 +82,9 @@ __global__ __launch_bounds__(256) void k_leaf_init(int metric, const float* __re
                 const int32_t q = tidx[s + o];
                 if (!act || o == a) continue;
                 const float* ro = STAGED ? rows + (int64_t)o * ldl : X + (int64_t)q * ld;
-                const float d = a < o ? distance_exact(metric, ra, ro, dim) : distance_exact(metric, ro, ra, dim);
+                // every metric here is bitwise symmetric (products commute, the lane sums see the
+                // same terms), so distance(leaf[min], leaf[max]) == distance(ra, ro)
+                const float d = distance_exact(metric, ra, ro, dim);
                 heap_push_col(hd, hi, hf, B, tid, k, q, d);
             }
             if (act)
@@ -115,7 +118,7 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
     const int64_t ldl = ld + 4;
     const int64_t cap_rows = ((int64_t)kInitLdsBudget - (int64_t)heap) / (ldl * 4);
     const int stage_cap = (int)std::max<int64_t>(0, std::min<int64_t>(cap_rows, std::max<int64_t>(leaf_size, 1)));
-    const unsigned grid = (unsigned)std::min<int64_t>(max_leaves, 1 << 20);
+    const unsigned grid = (unsigned)std::min<int64_t>(max_leaves, 8192);  // persistent over the leaves
     if (stage_cap > 0)
         hipLaunchKernelGGL(k_leaf_init<true>, dim3(grid), dim3(B), heap + (size_t)stage_cap * ldl * 4, s, metric, X,
                            ld, dim, tree_indices, leaves, n_leaves_dev, stage_cap, ldl, idx, dist, flags, k);
