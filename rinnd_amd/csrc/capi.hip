@@ -217,6 +217,7 @@ struct Pipeline {
     DBuf<uint64_t> off;
     DBuf<SegRec> seg, seg2;
     DBuf<int32_t> big_list, big_count, overflow;
+    DBuf<uint32_t> bins;
     DBuf<unsigned long long> changes;
     DBuf<char> scan_tmp;
     size_t scan_bytes = 0;
@@ -232,6 +233,7 @@ struct Pipeline {
         off.ensure(n + 1);
         big_list.ensure(std::max<int64_t>(n, 1));
         big_count.ensure(1);
+        bins.ensure(128);
         overflow.ensure(1);
         changes.ensure(1);
         size_t b = scan_u64_tmp_bytes(n + 1);
@@ -331,7 +333,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
                               pb.p, h_dist, k, PL.U.p, PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
             PL.group(s);
             launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
-                              PL.changes.p, PL.big_list.p, PL.big_count.p);
+                              PL.changes.p, PL.big_list.p, PL.big_count.p, PL.bins.p);
         }
         PL.check_overflow(s);
     }
@@ -432,7 +434,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         launch_fill_i32(s, new_idx.p, -1, (int64_t)n * C);
         launch_fill_i32(s, old_idx.p, -1, (int64_t)n * C);
         launch_cand_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, new_idx.p, old_idx.p, C, PL.big_list.p,
-                          PL.big_count.p);
+                          PL.big_count.p, PL.bins.p);
         launch_mark_old(s, h_idx, h_flags, new_idx.p, n, k, C);
         HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
         launch_cand_stats(s, new_idx.p, old_idx.p, n, C, cstat.p);
@@ -456,7 +458,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         HIP_TRY(hipMemsetAsync(PL.cur.p, 0, (n + 1) * 8, s));
         launch_join_scatter(s, new_idx.p, old_idx.p, C, 0, n, jo, PL.off.p, PL.cur.p, PL.seg.p);
         launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
-                          PL.changes.p, PL.big_list.p, PL.big_count.p);
+                          PL.changes.p, PL.big_list.p, PL.big_count.p, PL.bins.p);
         HIP_TRY(hipEventRecord(ev[6], s));
         uint64_t ch = 0, cs[2] = {0, 0}, nres = 0;
         const uint64_t nup = n_rec / 2;

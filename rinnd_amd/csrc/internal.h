@@ -162,12 +162,13 @@ void launch_leaf_pair_base(hipStream_t s, const int64_t* leaves, int64_t n_leave
                            size_t scan_tmp_bytes);
 void launch_scatter(hipStream_t s, const URec* U, const unsigned long long* U_count,
                     const uint64_t* off, unsigned long long* cur, SegRec* seg);
+// big_list: n_dest ints, big_count: 1 int, bins: 128 counters (scratch of the grouping stages)
 void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
                        int64_t n_dest, int32_t* idx, float* dist, uint8_t* flags, int k,
-                       unsigned long long* changes, int32_t* big_list, int32_t* big_count);
+                       unsigned long long* changes, int32_t* big_list, int32_t* big_count, uint32_t* bins);
 void launch_cand_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
                        int64_t n_dest, int32_t* new_idx, int32_t* old_idx, int C,
-                       int32_t* big_list, int32_t* big_count);
+                       int32_t* big_list, int32_t* big_count, uint32_t* bins);
 
 // candidates
 void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt);

@@ -941,15 +941,368 @@ static void run_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* s
                        kBigChunk * 16 + kApplyScratch, s, off, seg, seg2, ap, big_list, big_count);
 }
 
+// ---- lane-per-destination apply --------------------------------------------------------------------
+// Stage 1 sorts every destination's records by key in place (wave per destination: a register
+// bitonic network up to 64 records, an LDS network up to kSortCap, chunk sort + merge beyond).
+// Stage 2 lists the destinations grouped by record count (power-of-two bins), so a wave's lanes
+// have similar sequence lengths.  Stage 3 gives each lane one destination: its heap lives in an LDS
+// column and its sorted records are pushed in order - the same sequential semantics as the
+// reference, 64 destinations per wave.
+__global__ __launch_bounds__(kApplyWaves * RINND_WAVE) void k_seg_sort(const uint64_t* __restrict__ off,
+                                                                      SegRec* seg, int64_t n_dest,
+                                                                      int32_t* big_list, int32_t* big_count) {
+    extern __shared__ __align__(16) char ssm[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    SegRec* recs = (SegRec*)(ssm + w * kSortCap * 16);
+    for (int64_t v = (int64_t)blockIdx.x * kApplyWaves + w; v < n_dest; v += (int64_t)gridDim.x * kApplyWaves) {
+        const uint64_t o0 = off[v];
+        const int64_t L = (int64_t)(off[v + 1] - o0);
+        if (L <= 1) continue;
+        if (L > kSortCap) {
+            if (lane == 0) big_list[atomicAdd(big_count, 1)] = (int32_t)v;
+            continue;
+        }
+        if (L <= RINND_WAVE) {
+            uint64_t key = ~0ull;
+            int32_t rx = 0;
+            float rv = 0.f;
+            if (lane < L) {
+                const SegRec q = seg[o0 + lane];
+                key = q.key;
+                rx = q.x;
+                rv = q.v;
+            }
+            const int S = next_pow2(L);
+            for (int kk = 2; kk <= S; kk <<= 1)
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    const uint64_t pk = __shfl_xor(key, j);
+                    const int32_t px = __shfl_xor(rx, j);
+                    const float pv = __shfl_xor(rv, j);
+                    const bool up = (lane & kk) == 0;
+                    const bool lower = (lane & j) == 0;
+                    const bool gt = key > pk;
+                    if (key != pk && (lower ? (gt == up) : (gt != up))) {
+                        key = pk;
+                        rx = px;
+                        rv = pv;
+                    }
+                }
+            if (lane < L) seg[o0 + lane] = SegRec{key, rx, rv};
+            continue;
+        }
+        const int S = next_pow2(L);
+        for (int i = lane; i < S; i += RINND_WAVE) recs[i] = i < L ? seg[o0 + i] : SegRec{~0ull, 0, 0.f};
+        wave_sync();
+        for (int kk = 2; kk <= S; kk <<= 1)
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < S; i += RINND_WAVE) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & kk) == 0;
+                        const SegRec A = recs[i], B = recs[ixj];
+                        if (rec_gt(A, B) == up) {
+                            recs[i] = B;
+                            recs[ixj] = A;
+                        }
+                    }
+                }
+                wave_sync();
+            }
+        for (int i = lane; i < L; i += RINND_WAVE) seg[o0 + i] = recs[i];
+        wave_sync();
+    }
+}
+
+__global__ __launch_bounds__(kBigThreads) void k_seg_sort_big(const uint64_t* __restrict__ off, SegRec* seg,
+                                                              SegRec* seg2, const int32_t* big_list,
+                                                              const int32_t* big_count) {
+    extern __shared__ __align__(16) char bsm[];
+    SegRec* buf = (SegRec*)bsm;
+    const int nb = *big_count;
+    const int tid = threadIdx.x;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int64_t v = big_list[b];
+        const uint64_t o0 = off[v];
+        const int64_t L = (int64_t)(off[v + 1] - o0);
+        SegRec* A = seg + o0;
+        SegRec* B = seg2 + o0;
+        for (int64_t c0 = 0; c0 < L; c0 += kBigChunk) {
+            const int64_t cl = min((int64_t)kBigChunk, L - c0);
+            const int S = next_pow2(cl);
+            for (int i = tid; i < S; i += blockDim.x) buf[i] = i < cl ? A[c0 + i] : SegRec{~0ull, 0, 0.f};
+            __syncthreads();
+            for (int kk = 2; kk <= S; kk <<= 1)
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    for (int i = tid; i < S; i += blockDim.x) {
+                        const int ixj = i ^ j;
+                        if (ixj > i) {
+                            const bool up = (i & kk) == 0;
+                            const SegRec x = buf[i], y = buf[ixj];
+                            if (rec_gt(x, y) == up) {
+                                buf[i] = y;
+                                buf[ixj] = x;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            for (int i = tid; i < cl; i += blockDim.x) A[c0 + i] = buf[i];
+            __syncthreads();
+        }
+        SegRec* src = A;
+        SegRec* dst = B;
+        for (int64_t w = kBigChunk; w < L; w *= 2) {
+            for (int64_t o = tid; o < L; o += blockDim.x) {  // merge path of runs [base, base+w) and [base+w, ..)
+                const int64_t base = (o / (2 * w)) * (2 * w);
+                const int64_t la = min(w, L - base);
+                const int64_t lb = max((int64_t)0, min(w, L - base - w));
+                const int64_t ol = o - base;
+                const SegRec* Aa = src + base;
+                const SegRec* Bb = src + base + w;
+                int64_t lo = max((int64_t)0, ol - lb), hi = min(ol, la);
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (Aa[mid].key < Bb[ol - 1 - mid].key) lo = mid + 1;
+                    else hi = mid;
+                }
+                const int64_t i = lo, j = ol - lo;
+                const bool takeA = i < la && (j >= lb || Aa[i].key < Bb[j].key);
+                dst[o] = takeA ? Aa[i] : Bb[j];
+            }
+            __syncthreads();
+            SegRec* t = src;
+            src = dst;
+            dst = t;
+        }
+        if (src != A)
+            for (int64_t o = tid; o < L; o += blockDim.x) A[o] = src[o];
+        __syncthreads();
+    }
+}
+
+constexpr int kBins = 40;  // bin of L >= 1 is 1 + floor(log2 L); bin 0 (no records) is not listed
+__device__ __forceinline__ int len_bin(uint64_t L) { return L ? 64 - __clzll((long long)L) : 0; }
+
+// bins[0..kBins): counts, bins[kBins..2kBins): offsets, bins[2kBins..3kBins): cursors
+__global__ __launch_bounds__(256) void k_bin_count(const uint64_t* __restrict__ off, int64_t n_dest, uint32_t* bins) {
+    __shared__ uint32_t h[kBins];
+    for (int i = threadIdx.x; i < kBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n_dest; v += (int64_t)gridDim.x * blockDim.x) {
+        const int b = len_bin(off[v + 1] - off[v]);
+        if (b) atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBins; i += blockDim.x)
+        if (h[i]) atomicAdd(&bins[i], h[i]);
+}
+__global__ void k_bin_scan(uint32_t* bins) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < kBins; ++i) {
+            bins[kBins + i] = acc;
+            bins[2 * kBins + i] = 0;
+            acc += bins[i];
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict__ off, int64_t n_dest, uint32_t* bins,
+                                                     int32_t* order) {
+    __shared__ uint32_t h[kBins], base[kBins];
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < n_dest; v0 += (int64_t)gridDim.x * blockDim.x) {
+        for (int i = threadIdx.x; i < kBins; i += blockDim.x) h[i] = 0;
+        __syncthreads();
+        const int64_t v = v0 + threadIdx.x;
+        int b = 0;
+        uint32_t r = 0;
+        if (v < n_dest) {
+            b = len_bin(off[v + 1] - off[v]);
+            if (b) r = atomicAdd(&h[b], 1u);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kBins; i += blockDim.x)
+            base[i] = h[i] ? bins[kBins + i] + atomicAdd(&bins[2 * kBins + i], h[i]) : 0;
+        __syncthreads();
+        if (b) order[base[b] + r] = (int32_t)v;
+        __syncthreads();
+    }
+}
+
+// checked_flagged_push (neighbor_heap.rs:149-177) with flag = new on an LDS heap column
+// (slot j of lane c at j * B + c); returns true on insertion.
+__device__ __forceinline__ bool heap_push_lane(float* hd, int32_t* hi, uint8_t* hf, int B, int c, int k, int32_t x,
+                                               float d) {
+    if (d >= hd[c]) return false;
+#pragma unroll 4
+    for (int j = 0; j < k; ++j)
+        if (hi[j * B + c] == x) return false;
+    int pos = 0, child = 1;  // replace the root, sift down (neighbor_heap.rs:207-243)
+    while (child < k) {
+        const int right = child + 1;
+        if (right < k && hd[child * B + c] < hd[right * B + c]) child = right;
+        const float dc = hd[child * B + c];
+        if (d >= dc) break;
+        hd[pos * B + c] = dc;
+        hi[pos * B + c] = hi[child * B + c];
+        hf[pos * B + c] = hf[child * B + c];
+        pos = child;
+        child = 2 * pos + 1;
+    }
+    hd[pos * B + c] = d;
+    hi[pos * B + c] = x;
+    hf[pos * B + c] = 1;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_heap_push_lanes(const int32_t* __restrict__ order, const uint32_t* bins,
+                                                         const uint64_t* __restrict__ off,
+                                                         const SegRec* __restrict__ seg, int32_t* idx, float* dist,
+                                                         uint8_t* flags, int k, unsigned long long* changes) {
+    extern __shared__ __align__(16) float hsm[];
+    const int B = blockDim.x, c = threadIdx.x;
+    float* hd = hsm;
+    int32_t* hi = (int32_t*)(hd + B * k);
+    uint8_t* hf = (uint8_t*)(hi + B * k);
+    const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
+    unsigned long long ch = 0;
+    for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
+        const int64_t e = e0 + c;
+        if (e < total) {
+            const int64_t v = order[e];
+            const uint64_t o0 = off[v];
+            const int64_t L = (int64_t)(off[v + 1] - o0);
+            const int64_t hb = v * k;
+            for (int j = 0; j < k; ++j) {
+                hd[j * B + c] = dist[hb + j];
+                hi[j * B + c] = idx[hb + j];
+                hf[j * B + c] = flags[hb + j];
+            }
+            const SegRec* R = seg + o0;
+            for (int64_t r = 0; r < L; ++r) {
+                const SegRec q = R[r];
+                ch += heap_push_lane(hd, hi, hf, B, c, k, q.x, q.v);
+            }
+            for (int j = 0; j < k; ++j) {
+                dist[hb + j] = hd[j * B + c];
+                idx[hb + j] = hi[j * B + c];
+                flags[hb + j] = hf[j * B + c];
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) ch += __shfl_xor(ch, o);
+    if ((threadIdx.x & 63) == 0 && ch) atomicAdd(changes, ch);
+}
+
+// checked_heap_push_flat (candidates.rs:433-484) on an LDS column: swap sift toward the larger child.
+__device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, int c, int n, float priority,
+                                               int32_t index) {
+    if (priority >= pri[c]) return;
+#pragma unroll 4
+    for (int i = 0; i < n; ++i)
+        if (ix[i * B + c] == index) return;
+    int pos = 0;
+    for (;;) {
+        const int l = 2 * pos + 1, r = 2 * pos + 2;
+        int best = pos;
+        float bp = priority;  // the sifted value occupies pos
+        if (l < n && pri[l * B + c] > bp) {
+            best = l;
+            bp = pri[l * B + c];
+        }
+        if (r < n && pri[r * B + c] > bp) {
+            best = r;
+            bp = pri[r * B + c];
+        }
+        if (best == pos) break;
+        pri[pos * B + c] = bp;
+        ix[pos * B + c] = ix[best * B + c];
+        pos = best;
+    }
+    pri[pos * B + c] = priority;
+    ix[pos * B + c] = index;
+}
+
+__global__ __launch_bounds__(128) void k_cand_push_lanes(const int32_t* __restrict__ order, const uint32_t* bins,
+                                                         const uint64_t* __restrict__ off,
+                                                         const SegRec* __restrict__ seg, int32_t* new_idx,
+                                                         int32_t* old_idx, int C) {
+    extern __shared__ __align__(16) float csm[];
+    const int B = blockDim.x, c = threadIdx.x;
+    float* np = csm;
+    int32_t* ni = (int32_t*)(np + B * C);
+    float* op = (float*)(ni + B * C);
+    int32_t* oi = (int32_t*)(op + B * C);
+    const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
+    for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
+        const int64_t e = e0 + c;
+        if (e >= total) continue;
+        const int64_t v = order[e];
+        const uint64_t o0 = off[v];
+        const int64_t L = (int64_t)(off[v + 1] - o0);
+        for (int j = 0; j < C; ++j) {
+            np[j * B + c] = __builtin_inff();
+            op[j * B + c] = __builtin_inff();
+            ni[j * B + c] = -1;
+            oi[j * B + c] = -1;
+        }
+        const SegRec* R = seg + o0;
+        for (int64_t r = 0; r < L; ++r) {
+            const SegRec q = R[r];
+            if (q.x < 0) cand_push_lane(np, ni, B, c, C, q.v, q.x & 0x7fffffff);  // flag bit set: "new" edge
+            else cand_push_lane(op, oi, B, c, C, q.v, q.x);
+        }
+        for (int j = 0; j < C; ++j) {
+            new_idx[v * C + j] = ni[j * B + c];
+            old_idx[v * C + j] = oi[j * B + c];
+        }
+    }
+}
+
+static void sort_and_bin(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2, int64_t n_dest,
+                         int32_t* big_list, int32_t* big_count, uint32_t* bins) {
+    HIP_TRY(hipMemsetAsync(big_count, 0, sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(bins, 0, 3 * kBins * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_seg_sort, dim3(grid_for(n_dest, kApplyWaves, 1 << 16)), dim3(kApplyWaves * RINND_WAVE),
+                       kApplyWaves * kSortCap * 16, s, off, seg, n_dest, big_list, big_count);
+    hipLaunchKernelGGL(k_seg_sort_big, dim3(256), dim3(kBigThreads), kBigChunk * 16, s, off, seg, seg2, big_list,
+                       big_count);
+    // big_list is free again: it receives the destinations grouped by bin
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n_dest, 256, 2048)), dim3(256), 0, s, off, n_dest, bins);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, bins);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n_dest, 256, 4096)), dim3(256), 0, s, off, n_dest, bins,
+                       big_list);
+}
+
+static int lanes_block(int slots, int bytes_per_slot, int cap) {
+    int B = cap;
+    while (B > 64 && (int64_t)B * slots * bytes_per_slot > 64 * 1024) B -= 64;
+    return (int64_t)B * slots * bytes_per_slot <= 64 * 1024 ? B : 0;
+}
+
 void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
                        int64_t n_dest, int32_t* idx, float* dist, uint8_t* flags, int k,
-                       unsigned long long* changes, int32_t* big_list, int32_t* big_count) {
-    run_apply(s, off, seg, seg2, n_dest, HeapApply{idx, dist, flags, k, changes}, big_list, big_count);
+                       unsigned long long* changes, int32_t* big_list, int32_t* big_count, uint32_t* bins) {
+    const int B = lanes_block(k, 9, 256);
+    if (B == 0) {  // very large k: wave per destination
+        run_apply(s, off, seg, seg2, n_dest, HeapApply{idx, dist, flags, k, changes}, big_list, big_count);
+        return;
+    }
+    sort_and_bin(s, off, seg, seg2, n_dest, big_list, big_count, bins);
+    const size_t lds = (size_t)round_up((int64_t)B * k * 2 + (B * k + 3) / 4, 4) * 4;
+    hipLaunchKernelGGL(k_heap_push_lanes, dim3(grid_for(n_dest, B, 8192)), dim3(B), lds, s, big_list, bins, off, seg,
+                       idx, dist, flags, k, changes);
 }
 void launch_cand_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
                        int64_t n_dest, int32_t* new_idx, int32_t* old_idx, int C,
-                       int32_t* big_list, int32_t* big_count) {
-    run_apply(s, off, seg, seg2, n_dest, CandApply{new_idx, old_idx, C}, big_list, big_count);
+                       int32_t* big_list, int32_t* big_count, uint32_t* bins) {
+    const int B = lanes_block(C, 16, 128);
+    if (B == 0) {
+        run_apply(s, off, seg, seg2, n_dest, CandApply{new_idx, old_idx, C}, big_list, big_count);
+        return;
+    }
+    sort_and_bin(s, off, seg, seg2, n_dest, big_list, big_count, bins);
+    hipLaunchKernelGGL(k_cand_push_lanes, dim3(grid_for(n_dest, B, 8192)), dim3(B), (size_t)B * C * 16, s, big_list,
+                       bins, off, seg, new_idx, old_idx, C);
 }
 
 // ---- candidates -----------------------------------------------------------------------------------
