@@ -298,11 +298,13 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         DBuf<int64_t> rank(2 * (n + 1)), leaves(2 * (n + 1));
         size_t lt_bytes = std::max(scan_u64_tmp_bytes(n + 2), (size_t)16);
         DBuf<char> ltmp(lt_bytes);
+        DBuf<float> dscr(leaf_init_scratch_floats(n, Dld, p.leaf_size));
+        const LeafInitWs lws{mark.p, rank.p, dscr.p};
         for (size_t t = 0; t < F.trees.size(); ++t) {
             launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
                              leaves.p, ltmp.p, lt_bytes);
             launch_leaf_init(s, metric, DX, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
-                             h_idx, h_dist, h_flags, k);
+                             h_idx, h_dist, h_flags, k, lws);
         }
         HIP_TRY(hipStreamSynchronize(s));  // the leaf buffers die here
         PL.init(n, u_cap);

@@ -146,10 +146,18 @@ void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, i
 
 // initialize_from_leaves fused per leaf (leaf_init.hip); n_leaves is read on the device,
 // max_leaves bounds the grid.  Supported when a 64..256-thread block's heaps fit in LDS.
+// ws: the leaf marks and start ranks of launch_leaf_list and a distance scratch of
+// leaf_init_scratch_floats() floats (the two-pass kernels are skipped when it is null).
+struct LeafInitWs {
+    const int32_t* mark;
+    const int64_t* rank;
+    float* D;
+};
 bool leaf_init_supported(int k, int64_t leaf_size);
+int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size);
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
                       const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
-                      int32_t* idx, float* dist, uint8_t* flags, int k);
+                      int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws);
 
 // ordered-apply pipeline
 void launch_leaf_pairs(hipStream_t s, int metric, const float* X, int64_t ld, int dim,

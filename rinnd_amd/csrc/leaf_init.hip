@@ -98,6 +98,112 @@ __global__ __launch_bounds__(256) void k_leaf_init(int metric, const float* __re
     }
 }
 
+// Two-pass form for leaves whose rows fit in LDS.  k_leaf_dist evaluates every pair of a leaf once
+// (all threads busy) into D, an n x Dl4 matrix whose row p holds the distances from tree position p
+// to the members of its leaf; k_leaf_push gives each position one lane that applies its pushes in
+// leaf order with its heap in an LDS column and the root in a register, so rejected pushes cost
+// one compare.
+__device__ __forceinline__ void tri_pair(int q, int m, int& a, int& o) {
+    // q-th pair (a < o) in row-major order: counted back from the last row, r = k(k+1)/2 + rest
+    const int r = m * (m - 1) / 2 - 1 - q;
+    int k = (int)((sqrtf(8.0f * (float)r + 1.0f) - 1.0f) * 0.5f);
+    while ((k + 1) * (k + 2) / 2 <= r) ++k;
+    while (k * (k + 1) / 2 > r) --k;
+    a = m - 2 - k;
+    o = m - 1 - (r - k * (k + 1) / 2);
+}
+
+__global__ __launch_bounds__(256) void k_leaf_dist(int metric, const float* __restrict__ X, int64_t ld, int dim,
+                                                   const int32_t* __restrict__ tidx,
+                                                   const int64_t* __restrict__ leaves,
+                                                   const int64_t* __restrict__ n_leaves_dev, int mcap, int64_t ldl,
+                                                   float* __restrict__ D, int64_t Dl4) {
+    extern __shared__ __align__(16) float rsm[];
+    const int B = blockDim.x, t = threadIdx.x;
+    const int64_t n_leaves = *n_leaves_dev;
+    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
+        const int64_t s = leaves[2 * leaf];
+        const int m = (int)(leaves[2 * leaf + 1] - s);
+        if (m > mcap || m < 2) continue;  // uniform; larger leaves go to the untiled kernel
+        const int vec = (int)(ld >> 2);
+        for (int i = t; i < m * vec; i += B) {
+            const int c = i / vec, e4 = i - c * vec;
+            *reinterpret_cast<float4*>(rsm + c * ldl + e4 * 4) =
+                *reinterpret_cast<const float4*>(X + (int64_t)tidx[s + c] * ld + e4 * 4);
+        }
+        __syncthreads();
+        // every metric here is bitwise symmetric, so one evaluation serves both members
+        const int T = m * (m - 1) / 2;
+        for (int q = t; q < T; q += B) {
+            int a, o;
+            tri_pair(q, m, a, o);
+            const float d = distance_exact(metric, rsm + a * ldl, rsm + o * ldl, dim);
+            D[(s + a) * Dl4 + o] = d;
+            D[(s + o) * Dl4 + a] = d;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
+                                                   const int32_t* __restrict__ mark, const int64_t* __restrict__ rank,
+                                                   int64_t n, int mcap, const float* __restrict__ D, int64_t Dl4,
+                                                   int32_t* idx, float* dist, uint8_t* flags, int k) {
+    extern __shared__ __align__(16) float psm[];
+    const int B = blockDim.x, c = threadIdx.x;
+    float* hd = psm;
+    int32_t* hi = (int32_t*)(hd + B * k);
+    uint8_t* hf = (uint8_t*)(hi + B * k);
+    for (int64_t pos = blockIdx.x * (int64_t)B + c; pos < n; pos += (int64_t)gridDim.x * B) {
+        const int64_t l = rank[pos] + mark[pos] - 1;  // leaf holding this position
+        const int64_t s = leaves[2 * l];
+        const int m = (int)(leaves[2 * l + 1] - s);
+        if (m > mcap || m < 2) continue;
+        const int a = (int)(pos - s);
+        const int64_t hb = (int64_t)tidx[pos] * k;
+        for (int j = 0; j < k; ++j) {
+            hd[j * B + c] = dist[hb + j];
+            hi[j * B + c] = idx[hb + j];
+            hf[j * B + c] = flags[hb + j];
+        }
+        float root = hd[c];
+        const float* Dr = D + pos * Dl4;
+        for (int o = 0; o < m; o += 4) {
+            const float4 dv = *reinterpret_cast<const float4*>(Dr + o);
+            const float du[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int oo = o + u;
+                if (oo < m && oo != a && du[u] < root) {
+                    heap_push_col(hd, hi, hf, B, c, k, tidx[s + oo], du[u]);
+                    root = hd[c];
+                }
+            }
+        }
+        for (int j = 0; j < k; ++j) {
+            dist[hb + j] = hd[j * B + c];
+            idx[hb + j] = hi[j * B + c];
+            flags[hb + j] = hf[j * B + c];
+        }
+    }
+}
+
+struct TilePlan {
+    int mcap = 0;
+    int64_t Dl4 = 0;
+    size_t rows_lds = 0;
+};
+static TilePlan tile_plan(int64_t ld, int64_t leaf_size) {
+    TilePlan P;
+    const int mcap = (int)std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 256);
+    const size_t rows = (size_t)mcap * (ld + 4) * 4;
+    if (rows > 150 * 1024) return P;
+    P.mcap = mcap;
+    P.Dl4 = round_up((int64_t)mcap, (int64_t)4);
+    P.rows_lds = rows;
+    return P;
+}
+
 static int init_block(int64_t leaf_size) {
     return (int)std::min<int64_t>(256, round_up(std::max<int64_t>(leaf_size, 1), (int64_t)64));
 }
@@ -108,22 +214,40 @@ static size_t init_heap_bytes(int B, int k) {
 bool leaf_init_supported(int k, int64_t leaf_size) {
     return init_heap_bytes(init_block(leaf_size), k) <= (size_t)kInitLdsBudget;
 }
+int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size) {
+    const TilePlan P = tile_plan(ld, leaf_size);
+    return P.mcap ? std::max<int64_t>(n, 1) * P.Dl4 : 0;
+}
 
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
                       const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
-                      int32_t* idx, float* dist, uint8_t* flags, int k) {
+                      int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws) {
     if (max_leaves <= 0) return;
     const int B = init_block(leaf_size);
     const size_t heap = init_heap_bytes(B, k);
     const int64_t ldl = ld + 4;
-    const int64_t cap_rows = ((int64_t)kInitLdsBudget - (int64_t)heap) / (ldl * 4);
-    const int stage_cap = (int)std::max<int64_t>(0, std::min<int64_t>(cap_rows, std::max<int64_t>(leaf_size, 1)));
+    const int64_t n = max_leaves;  // positions in the tree == points
     const unsigned grid = (unsigned)std::min<int64_t>(max_leaves, 8192);  // persistent over the leaves
-    if (stage_cap > 0)
-        hipLaunchKernelGGL(k_leaf_init<true>, dim3(grid), dim3(B), heap + (size_t)stage_cap * ldl * 4, s, metric, X,
-                           ld, dim, tree_indices, leaves, n_leaves_dev, stage_cap, ldl, idx, dist, flags, k);
+    const TilePlan P = tile_plan(ld, leaf_size);
+    int untiled_cap = 0;  // leaves with m <= untiled_cap are done by the two-pass kernels
+    int PB = 256;
+    while (PB > 64 && init_heap_bytes(PB, k) > 64 * 1024) PB -= 64;
+    if (P.mcap > 0 && ws.D && init_heap_bytes(PB, k) <= 64 * 1024) {
+        hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X, ld,
+                           dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4);
+        hipLaunchKernelGGL(k_leaf_push, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16)),
+                           dim3(PB), init_heap_bytes(PB, k), s, tree_indices, leaves, ws.mark, ws.rank, n, P.mcap,
+                           ws.D, P.Dl4, idx, dist, flags, k);
+        untiled_cap = P.mcap;
+    } else {
+        const int64_t cap_rows = ((int64_t)kInitLdsBudget - (int64_t)heap) / (ldl * 4);
+        untiled_cap = (int)std::max<int64_t>(0, std::min<int64_t>(cap_rows, std::max<int64_t>(leaf_size, 1)));
+        if (untiled_cap > 0)
+            hipLaunchKernelGGL(k_leaf_init<true>, dim3(grid), dim3(B), heap + (size_t)untiled_cap * ldl * 4, s, metric,
+                               X, ld, dim, tree_indices, leaves, n_leaves_dev, untiled_cap, ldl, idx, dist, flags, k);
+    }
     hipLaunchKernelGGL(k_leaf_init<false>, dim3(std::min<unsigned>(grid, 4096)), dim3(B), heap, s, metric, X, ld,
-                       dim, tree_indices, leaves, n_leaves_dev, stage_cap, ldl, idx, dist, flags, k);
+                       dim, tree_indices, leaves, n_leaves_dev, untiled_cap, ldl, idx, dist, flags, k);
     HIP_TRY(hipGetLastError());
 }
 

@@ -201,6 +201,18 @@ def test_nn_descent_bit_exact(ctx, n, d, k, leaf, threads):
     np.testing.assert_array_equal(gst, r["rng_state"])
 
 
+@pytest.mark.parametrize("leaf,d", [(200, 128), (140, 64), (40, 300), (150, 300)])
+def test_leaf_init_paths(ctx, leaf, d):
+    """Leaf initialisation through the two-pass kernels (rows of a leaf fit in LDS) and the single-pass
+    kernels (150 rows of 300 dims do not: small leaves staged, large ones read from memory)."""
+    X = uniform(4000, d, seed=leaf)
+    gi, gd, gf, gs, gst, _ = gpu_nn_descent(ctx, X, 15, 2, leaf, 15, 1, threads=4)
+    r = O.nn_descent(X, 15, 2, leaf, 15, 1, threads=4)
+    np.testing.assert_array_equal(gi, r["indices"])
+    np.testing.assert_array_equal(gd.view(np.uint32), r["distances"].view(np.uint32))
+    np.testing.assert_array_equal(gf, r["flags"])
+
+
 @pytest.mark.parametrize("metric,angular", [("cosine", True), ("inner_product", True), ("alt_inner_product", True)])
 def test_nn_descent_other_metrics(ctx, metric, angular):
     X = uniform(800, 24, seed=3) - np.float32(0.3)
