@@ -3,9 +3,11 @@
 plus batched query QPS and p50/p99 latency, on one MI355X.
 
 Workload: 1,000,000 x 128 float32 integer-valued Gaussian mixture (1,000 centres U[0,100)^128,
-sigma 5, rounded and clipped to 0..255), squared Euclidean, k=15, RP forest (8 trees, leaf 75) +
-NN-Descent (max_candidates 15, <= 20 iterations, delta 0.001).  A step is one full build from
-HBM-resident data to the sorted k-NN graph (forest, leaf init, iterations, sort_all).
+sigma 5, rounded and clipped to 0..255), squared Euclidean, k=15, RP forest (24 trees, leaf 150) +
+NN-Descent (max_candidates 15, <= 20 iterations, delta 0.001), counter-based draws (the GPU-native
+sampling mode).  A step is one full build from HBM-resident data to the sorted k-NN graph (forest,
+leaf init, iterations, sort_all).  One extra FP32 build with the reference's FastRng streams (the
+bit-exact mode) is timed and reported beside it.
 
 Prints one JSON line.  `value` is the points/s of the build with the headline join precision.
 """
@@ -39,18 +41,23 @@ def parse():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=15)
-    ap.add_argument("--trees", type=int, default=8)
-    ap.add_argument("--leaf", type=int, default=75)
+    ap.add_argument("--trees", type=int, default=24)
+    ap.add_argument("--leaf", type=int, default=150)
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--delta", type=float, default=0.001)
     ap.add_argument("--threads", type=int, default=8, help="Rayon thread count reproduced by candidate sampling")
-    ap.add_argument("--rng", default="reference", choices=["reference", "counter"])
+    ap.add_argument("--rng", default="counter", choices=["reference", "counter"])
+    ap.add_argument("--no-ref", action="store_true", help="skip the reference-stream FP32 build")
     ap.add_argument("--precision", default="both", choices=["fp32", "bf16", "both"])
     ap.add_argument("--recall-rows", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-query", action="store_true")
     ap.add_argument("--queries", type=int, default=10_000)
     ap.add_argument("--profile", action="store_true", help="builds only (for rocprofv3 runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy (3 times): the known-size kernel that "
+                         "scales FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--epsilons", default="0.1", help="comma-separated query epsilons to report")
     return ap.parse_args()
 
 
@@ -137,6 +144,13 @@ def main():
     from rinnd_amd import datasets
 
     n, dim, k = args.n, args.dim, args.k
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log(f"[bench] generating {n}x{dim} integer Gaussian mixture")
     X = datasets.cfg2(n=n, d=dim) if dim == 128 or True else None
     Xd = torch.from_numpy(X).cuda()
@@ -174,6 +188,7 @@ def main():
 
     rows = np.random.default_rng(3).choice(n, size=min(args.recall_rows, n), replace=False)
     recall = {}
+    true_d = None
     if not args.profile:
         log("[bench] exact FP64 k-NN of the recall rows")
         true_d, true_i = exact_knn_f64(torch, Xd, rows, k)
@@ -186,15 +201,8 @@ def main():
     T = results[head]["times"]
     ms = 1e3 * sum(T) / len(T)
     st = results[head]["stats"]
-    launches = sum(s["join_kernel_launches"] for s in st)
-    kms = sum(s["join_kernel_ms"] for s in st)
-    jbytes = sum(s["join_bytes"] for s in st)
-    achieved = (jbytes / launches) / (kms / launches * 1e-3) / 1e9 if launches and kms > 0 else None
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                "kernel": "k_join_fp32" if head == "fp32" else "k_join_bf16",
-                "avg_launch_ms": kms / launches if launches else None,
-                "bytes_per_launch": jbytes / launches if launches else None}
+    roofline = roofline_of(st, head)
+    kernels = {fam: kernel_line(st, fam, head) for fam in L.TIMED_KERNELS}
 
     line = {
         "metric": "1Mx128 kNN build pts/s BF16-join vs FP32 at recall@15; query QPS + p99",
@@ -211,10 +219,16 @@ def main():
                                     ("ms_forest", "ms_init", "ms_candidates", "ms_join", "ms_apply")}}
                    for p, r in results.items()},
         "roofline": roofline,
+        "kernels": kernels,
     }
     if "fp32" in results and "bf16" in results:
         line["bf16_speedup"] = line["builds"]["bf16"]["points_per_s"] / line["builds"]["fp32"]["points_per_s"]
 
+    if not args.profile and not args.no_ref and args.rng != "reference":
+        try:
+            line["reference_stream_build"] = reference_build(torch, ctx, L, Xd, n, dim, args, outs, rows, true_d)
+        except Exception as e:
+            log(f"[bench] reference-stream build failed: {e}")
     if not args.profile and not args.no_query:
         try:
             line["query"] = query_bench(torch, L, ctx, X, Xd, args)
@@ -226,6 +240,54 @@ def main():
         except Exception as e:
             log(f"[bench] cpu baseline failed: {e}")
     print(json.dumps(line), flush=True)
+
+
+KERNEL_NAMES = {"join": {"fp32": "k_join_fp32", "bf16": "k_join_bf16"}, "margins": "k_lvl_margins",
+                "leaf_dist": "k_leaf_dist", "leaf_push": "k_leaf_push"}
+
+
+def kernel_line(stats, fam, head):
+    ms = sum(s["timed"][fam]["ms"] for s in stats)
+    launches = sum(s["timed"][fam]["launches"] for s in stats)
+    nbytes = sum(s["timed"][fam]["bytes"] for s in stats)
+    name = KERNEL_NAMES[fam][head] if isinstance(KERNEL_NAMES[fam], dict) else KERNEL_NAMES[fam]
+    if not launches or ms <= 0:
+        return {"kernel": name, "launches": 0}
+    achieved = (nbytes / launches) / (ms / launches * 1e-3) / 1e9
+    return {"kernel": name, "launches": launches // len(stats), "ms_per_build": ms / len(stats),
+            "avg_launch_ms": ms / launches, "bytes_per_launch": nbytes / launches, "achieved_gbs": achieved,
+            "frac_hbm": achieved / HBM_PEAK_GBS}
+
+
+def roofline_of(stats, head):
+    """HBM roofline of the timed kernel family with the largest share of the build."""
+    fams = [(sum(s["timed"][f]["ms"] for s in stats), f) for f in ("join", "margins", "leaf_dist", "leaf_push")]
+    _, fam = max(fams)
+    k = kernel_line(stats, fam, head)
+    if not k.get("launches"):
+        return None
+    return {"bound": "hbm", "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": k["frac_hbm"], "traffic": None, "kernel": k["kernel"], "avg_launch_ms": k["avg_launch_ms"],
+            "bytes_per_launch": k["bytes_per_launch"]}
+
+
+def reference_build(torch, ctx, L, Xd, n, dim, args, outs, rows, true_d):
+    """One FP32 build with the reference's FastRng streams (bit-exact mode)."""
+    class A:
+        pass
+    a = A()
+    a.__dict__.update(vars(args))
+    a.rng = "reference"
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = gpu_build(ctx, L, Xd, n, dim, a, "fp32", outs)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    rt = torch.as_tensor(rows, device="cuda")
+    rec = tie_aware_recall(torch, Xd, Xd[rt], outs[0][rt], true_d, args.k) if true_d is not None else None
+    log(f"[bench] reference-stream fp32 build: {t * 1e3:.1f} ms, recall@{args.k} {rec}")
+    return {"points_per_s": n / t, "ms_per_build": t * 1e3, "recall_at_k": rec, "iterations": st["iterations"],
+            "phase_ms": {x: st[x] for x in ("ms_forest", "ms_init", "ms_candidates", "ms_join", "ms_apply")}}
 
 
 def query_bench(torch, L, ctx, X, Xd, args):
@@ -247,35 +309,40 @@ def query_bench(torch, L, ctx, X, Xd, args):
     kq = 10
     oi = torch.empty((args.queries, kq), dtype=torch.int32, device="cuda")
     od = torch.empty((args.queries, kq), dtype=torch.float32, device="cuda")
-    for _ in range(2):
-        ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(0.1), L.ptr(oi), L.ptr(od)),
-                  "rinnd_index_query")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reps = 3
-    for _ in range(reps):
-        ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(0.1), L.ptr(oi), L.ptr(od)),
-                  "rinnd_index_query")
-    torch.cuda.synchronize()
-    qps = args.queries * reps / (time.perf_counter() - t0)
     true_d, _ = exact_knn_queries_f64(torch, Xd, Qd, kq)
-    rec = tie_aware_recall(torch, Xd, Qd, oi, true_d, kq)
-    lat = []
     one_i = np.empty(kq, np.int32)
     one_d = np.empty(kq, np.float32)
-    for j in range(200):
-        q = np.ascontiguousarray(Qn[j])
+    runs = []
+    for eps in [float(e) for e in args.epsilons.split(",")]:
+        for _ in range(2):
+            ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(eps), L.ptr(oi),
+                                                L.ptr(od)), "rinnd_index_query")
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ctx.check(L.lib().rinnd_index_query_one(h, L.ptr(q), kq, C.c_float(0.1), L.ptr(one_i), L.ptr(one_d)),
-                  "rinnd_index_query_one")
-        lat.append(time.perf_counter() - t0)
+        reps = 3
+        for _ in range(reps):
+            ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(eps), L.ptr(oi),
+                                                L.ptr(od)), "rinnd_index_query")
+        torch.cuda.synchronize()
+        qps = args.queries * reps / (time.perf_counter() - t0)
+        rec = tie_aware_recall(torch, Xd, Qd, oi, true_d, kq)
+        lat = []
+        for j in range(200):
+            q = np.ascontiguousarray(Qn[j])
+            t0 = time.perf_counter()
+            ctx.check(L.lib().rinnd_index_query_one(h, L.ptr(q), kq, C.c_float(eps), L.ptr(one_i), L.ptr(one_d)),
+                      "rinnd_index_query_one")
+            lat.append(time.perf_counter() - t0)
+        lat_ms = np.array(lat) * 1e3
+        log(f"[bench] query eps {eps}: {qps:.0f} QPS, recall@10 {rec:.4f}, p50 {np.percentile(lat_ms, 50):.3f} ms, "
+            f"p99 {np.percentile(lat_ms, 99):.3f} ms")
+        runs.append({"qps": qps, "batch": args.queries, "k": kq, "epsilon": eps, "recall_at_10": rec,
+                     "p50_ms_single": float(np.percentile(lat_ms, 50)),
+                     "p99_ms_single": float(np.percentile(lat_ms, 99))})
     L.lib().rinnd_index_free(h)
-    lat_ms = np.array(lat) * 1e3
-    log(f"[bench] query: {qps:.0f} QPS, recall@10 {rec:.4f}, p50 {np.percentile(lat_ms, 50):.3f} ms, "
-        f"p99 {np.percentile(lat_ms, 99):.3f} ms")
-    return {"qps": qps, "batch": args.queries, "k": kq, "epsilon": 0.1, "recall_at_10": rec,
-            "p50_ms_single": float(np.percentile(lat_ms, 50)), "p99_ms_single": float(np.percentile(lat_ms, 99)),
-            "index_build_s": t_index}
+    ok = [r for r in runs if r["recall_at_10"] >= 0.95]
+    best = max(ok, key=lambda r: r["qps"]) if ok else max(runs, key=lambda r: r["recall_at_10"])
+    return dict(best, index_build_s=t_index, sweep=runs)
 
 
 if __name__ == "__main__":

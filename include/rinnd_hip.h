@@ -93,6 +93,22 @@ typedef struct rinnd_params {
     int32_t join_precision; /* rinnd_join_precision */
 } rinnd_params;
 
+/* Launch timing of one kernel family: summed duration of its launches (HIP events recorded on the
+ * stream the kernel runs on) and the algorithmic bytes those launches must move. */
+typedef struct rinnd_kernel_timing {
+    double ms;
+    int64_t launches;
+    double bytes;
+} rinnd_kernel_timing;
+
+typedef enum rinnd_timed_kernel {
+    RINND_TIMED_JOIN = 0,      /* local join (FP32 or BF16 tile kernel), one launch per iteration */
+    RINND_TIMED_MARGINS = 1,   /* RP-forest hyperplane side tests, one launch per tree level */
+    RINND_TIMED_LEAF_DIST = 2, /* leaf pair distances, one launch per tree */
+    RINND_TIMED_LEAF_PUSH = 3, /* leaf heap pushes, one launch per tree */
+    RINND_TIMED_COUNT = 4
+} rinnd_timed_kernel;
+
 /* Per-build counters and phase timings (HIP events on the context stream). */
 typedef struct rinnd_build_stats {
     int32_t iterations;          /* NN-Descent iterations run */
@@ -107,9 +123,7 @@ typedef struct rinnd_build_stats {
     double ms_join;              /* local join (distance tiles + filter), all iterations */
     double ms_apply;             /* ordered heap updates, all iterations */
     double ms_total;             /* whole nn_descent call */
-    double join_kernel_ms;       /* summed duration of the dominant join kernel launches */
-    int64_t join_kernel_launches;
-    double join_bytes;           /* algorithmic bytes of those launches (gathered rows + lists) */
+    rinnd_kernel_timing timed[RINND_TIMED_COUNT];  /* per kernel family, see rinnd_timed_kernel */
 } rinnd_build_stats;
 
 typedef struct rinnd_ctx rinnd_ctx;

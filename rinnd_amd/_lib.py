@@ -47,13 +47,21 @@ class Params(C.Structure):
     ]
 
 
+class KernelTiming(C.Structure):
+    _fields_ = [("ms", C.c_double), ("launches", C.c_int64), ("bytes", C.c_double)]
+
+
+# rinnd_timed_kernel order, with the kernel each entry times
+TIMED_KERNELS = ("join", "margins", "leaf_dist", "leaf_push")
+
+
 class BuildStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("converged", C.c_int32), ("changes", C.c_int64 * 64),
         ("join_pairs", C.c_int64 * 64), ("join_rescored", C.c_int64 * 64), ("join_updates", C.c_int64 * 64),
         ("ms_forest", C.c_double), ("ms_init", C.c_double), ("ms_candidates", C.c_double),
         ("ms_join", C.c_double), ("ms_apply", C.c_double), ("ms_total", C.c_double),
-        ("join_kernel_ms", C.c_double), ("join_kernel_launches", C.c_int64), ("join_bytes", C.c_double),
+        ("timed", KernelTiming * 4),
     ]
 
     def to_dict(self):
@@ -66,8 +74,8 @@ class BuildStats(C.Structure):
             "join_updates": list(self.join_updates[: min(it, 64)]),
             "ms_forest": self.ms_forest, "ms_init": self.ms_init, "ms_candidates": self.ms_candidates,
             "ms_join": self.ms_join, "ms_apply": self.ms_apply, "ms_total": self.ms_total,
-            "join_kernel_ms": self.join_kernel_ms, "join_kernel_launches": self.join_kernel_launches,
-            "join_bytes": self.join_bytes,
+            "timed": {name: {"ms": self.timed[i].ms, "launches": self.timed[i].launches, "bytes": self.timed[i].bytes}
+                      for i, name in enumerate(TIMED_KERNELS)},
         }
 
 

@@ -112,7 +112,7 @@ static void make_data(hipStream_t s, const float* data, int64_t n, int dim, Data
 // ---- forest ------------------------------------------------------------------------------------------
 static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t n, int dim, int64_t n_trees,
                              int64_t leaf_size, int angular, int64_t max_depth, int rng_mode,
-                             Xoshiro& main, rinnd_forest& F) {
+                             Xoshiro& main, rinnd_forest& F, rinnd_kernel_timing* margins = nullptr) {
     hipStream_t s = ctx->stream;
     F.dim = dim;
     F.n = n;
@@ -142,7 +142,9 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
         if (rng_mode == RINND_RNG_COUNTER) {  // all trees and nodes of a level at once
             std::vector<uint64_t> keys(n_trees);
             for (int64_t t = 0; t < n_trees; ++t) keys[t] = states[t * 4] ^ states[t * 4 + 3];
-            if (build_forest_level(s, DX, Dld, dim, n, leaf_size, max_depth, angular, keys, F)) return;
+            if (build_forest_level(s, DX, Dld, dim, n, leaf_size, max_depth, angular, keys, F,
+                                   KTimer{ctx->ev[8], ctx->ev[9], margins}))
+                return;
             cap = 2 * n + 1;
             continue;
         }
@@ -285,7 +287,8 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     rinnd_build_stats st{};
 
     HIP_TRY(hipEventRecord(ev[0], s));
-    build_forest_dev(ctx, DX, Dld, n, dim, p.n_trees, p.leaf_size, p.angular, p.max_depth, p.rng_mode, main, F);
+    build_forest_dev(ctx, DX, Dld, n, dim, p.n_trees, p.leaf_size, p.angular, p.max_depth, p.rng_mode, main, F,
+                     &st.timed[RINND_TIMED_MARGINS]);
     HIP_TRY(hipEventRecord(ev[1], s));
 
     launch_fill_heap(s, h_idx, h_dist, h_flags, (int64_t)n * k);
@@ -299,12 +302,16 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         size_t lt_bytes = std::max(scan_u64_tmp_bytes(n + 2), (size_t)16);
         DBuf<char> ltmp(lt_bytes);
         DBuf<float> dscr(leaf_init_scratch_floats(n, Dld, p.leaf_size));
-        const LeafInitWs lws{mark.p, rank.p, dscr.p};
+        DBuf<unsigned long long> lpairs(1);
+        HIP_TRY(hipMemsetAsync(lpairs.p, 0, 8, s));
+        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p};
+        const KTimer dist_timer{ev[8], ev[9], &st.timed[RINND_TIMED_LEAF_DIST]};
+        const KTimer push_timer{ev[10], ev[11], &st.timed[RINND_TIMED_LEAF_PUSH]};
         for (size_t t = 0; t < F.trees.size(); ++t) {
             launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
                              leaves.p, ltmp.p, lt_bytes);
             launch_leaf_init(s, metric, DX, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
-                             h_idx, h_dist, h_flags, k, lws);
+                             h_idx, h_dist, h_flags, k, lws, dist_timer, push_timer);
         }
         HIP_TRY(hipStreamSynchronize(s));  // the leaf buffers die here
         PL.init(n, u_cap);
@@ -481,13 +488,14 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         // algorithmic bytes of the join launch: gathered candidate rows, both candidate lists, the
         // 2C heap roots read per row, and the 8-byte kept-pair records written
         // (BF16: 2-byte rows + norms, plus the FP32 rows of re-evaluated pairs)
+        rinnd_kernel_timing& tj = st.timed[RINND_TIMED_JOIN];
         if (use_bf16)
-            st.join_bytes += (double)cs[0] * (B16.ldb * 2.0 + 4.0) + (double)n * 2 * C * 4.0 * 2.0 +
-                             (double)nup * 8.0 + (B16.exact ? 0.0 : (double)nres * 2.0 * dim * 4.0);
+            tj.bytes += (double)cs[0] * (B16.ldb * 2.0 + 4.0) + (double)n * 2 * C * 4.0 * 2.0 + (double)nup * 8.0 +
+                        (B16.exact ? 0.0 : (double)nres * 2.0 * dim * 4.0);
         else
-            st.join_bytes += (double)cs[0] * dim * 4.0 + (double)n * 2 * C * 4.0 * 2.0 + (double)nup * 8.0;
-        st.join_kernel_ms += jm;
-        st.join_kernel_launches += 1;
+            tj.bytes += (double)cs[0] * dim * 4.0 + (double)n * 2 * C * 4.0 * 2.0 + (double)nup * 8.0;
+        tj.ms += jm;
+        tj.launches += 1;
         if (p.verbose) std::printf("Iteration %d: %llu updates\n", it + 1, (unsigned long long)ch);
         if ((int64_t)ch <= threshold) {
             converged = true;

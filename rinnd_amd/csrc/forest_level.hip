@@ -8,7 +8,7 @@
 // position) instead of a depth-first FastRng stream, so every node of a level is independent and
 // one level is processed for all trees at once:
 //   k_lvl_planes   one workgroup per active node: split points, hyperplane, offset (or leaf)
-//   k_lvl_margins  16 lanes per (tree, position): side of the node's hyperplane
+//   k_lvl_margins  16 lanes per point, all trees: side of each tree's current hyperplane
 //   scan           exclusive prefix of the left flags over all (tree, position)
 //   k_lvl_counts   left size per node; degenerate splits become leaves; children counted per tree
 //   k_lvl_assign   child node ids (breadth-first per tree) and the next level's active list
@@ -55,8 +55,10 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 }
 
 __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* nodes, int64_t na,
-                                                    const int32_t* const* perm, uint8_t* split) {
+                                                    const int32_t* const* perm, uint8_t* split,
+                                                    unsigned long long* active_pairs) {
     __shared__ float red[4];
+    unsigned long long act = 0;  // members of split nodes: the (tree, point) pairs tested this level
     for (int64_t e = blockIdx.x; e < na; e += gridDim.x) {
         const LvlNode nd = nodes[e];
         const int64_t m = nd.end - nd.start;
@@ -115,51 +117,64 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
             if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = 0.f;
         }
         if (threadIdx.x == 0) split[e] = 1;
+        act += (unsigned long long)m;
     }
+    if (threadIdx.x == 0 && act) atomicAdd(active_pairs, act);
 }
 
-// Left flag per (tree, position); positions outside split nodes count as left (flag 1) so the
-// prefix over any node's range counts only its own members.  16 lanes per item; V4 needs
-// dim % 4 == 0 (16-byte aligned hyperplane rows; data rows are padded to 8 floats).
-template <bool V4>
+// Left flag of every (tree, point) whose node splits at this level, written at the point's position
+// (flags of other positions are never read: a split node's range holds only its own members).
+// Point-major: 16 lanes per point keep its row in registers (CH float4 per lane, dim <= 64 * CH) and
+// test it against the current hyperplane of each tree, so a row is read once per level.
+template <int CH>
 __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* nodes, int n_trees,
-                                                     const int32_t* const* perm, const int32_t* const* act,
-                                                     const uint8_t* split, uint32_t* flags) {
+                                                     const int32_t* __restrict__ nodeOf,
+                                                     const int32_t* __restrict__ posOf, const uint8_t* split,
+                                                     uint32_t* flags) {
     const int gl = threadIdx.x & 15;
-    const int64_t total = (int64_t)n_trees * a.n;
     const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 4;
-    for (int64_t item = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4; item < total; item += groups) {
-        const int t = (int)(item / a.n);
-        const int64_t pos = item - (int64_t)t * a.n;
-        const int32_t e = act[t][pos];
-        const bool active = e >= 0 && split[e];
-        float acc = 0.f;
-        int32_t node = 0;
-        if (active) {
-            node = nodes[e].node;
-            const float* x = a.X + (int64_t)perm[t][pos] * a.ld;
+    for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4; i < a.n; i += groups) {
+        float4 xr[CH > 0 ? CH : 1];
+        const float* x = a.X + i * a.ld;
+        bool loaded = false;
+        for (int t = 0; t < n_trees; ++t) {
+            const int64_t ti = (int64_t)t * a.n + i;
+            const int32_t e = nodeOf[ti];
+            if (e < 0 || !split[e]) continue;  // uniform within the 16-lane group
+            const int32_t node = nodes[e].node;
             const float* hp = a.hyper[t] + (int64_t)node * a.dim;
-            if (V4) {
-                for (int i = 4 * gl; i < a.dim; i += 64) {
-                    const float4 xv = *reinterpret_cast<const float4*>(x + i);
-                    const float4 hv = *reinterpret_cast<const float4*>(hp + i);
-                    acc += xv.x * hv.x + xv.y * hv.y + xv.z * hv.z + xv.w * hv.w;
+            float acc = 0.f;
+            if (CH > 0) {
+                if (!loaded) {
+#pragma unroll
+                    for (int c = 0; c < CH; ++c) {
+                        const int d0 = 4 * gl + 64 * c;
+                        xr[c] = d0 < a.dim ? *reinterpret_cast<const float4*>(x + d0) : make_float4(0.f, 0.f, 0.f, 0.f);
+                    }
+                    loaded = true;
+                }
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    const int d0 = 4 * gl + 64 * c;
+                    if (d0 < a.dim) {
+                        const float4 hv = *reinterpret_cast<const float4*>(hp + d0);
+                        acc += xr[c].x * hv.x + xr[c].y * hv.y + xr[c].z * hv.z + xr[c].w * hv.w;
+                    }
                 }
             } else {
-                for (int i = gl; i < a.dim; i += 16) acc += x[i] * hp[i];
+                for (int d = gl; d < a.dim; d += 16) acc += x[d] * hp[d];
             }
-        }
-        for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // stays inside the 16-lane group
-        if (gl == 0) {
-            uint32_t left = 1;
-            if (active) {
+            for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // stays inside the 16-lane group
+            if (gl == 0) {
+                const int32_t pos = posOf[ti];
                 const float margin = acc + a.offsets[t][node];
+                uint32_t left;
                 if (fabsf(margin) < 1e-8f)
                     left = (uint32_t)(mix64(a.keys[t] ^ mix64(((uint64_t)(uint32_t)node << 32) ^ (uint64_t)pos)) & 1);
                 else
                     left = margin < 0.0f ? 1u : 0u;
+                flags[(int64_t)t * a.n + pos] = left;
             }
-            flags[item] = left;
         }
     }
 }
@@ -209,7 +224,7 @@ __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* sp
 __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, const uint8_t* split,
                               const uint32_t* cscan, const uint32_t* leftsz, const uint32_t* prefix,
                               const uint32_t* flags, const int32_t* const* perm, int32_t* const* perm2,
-                              const int32_t* const* act, int32_t* const* act2) {
+                              const int32_t* const* act, int32_t* const* act2, int32_t* nodeOf, int32_t* posOf) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
          item += (int64_t)gridDim.x * blockDim.x) {
@@ -219,6 +234,7 @@ __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, cons
         if (e < 0 || !split[e]) {
             perm2[t][pos] = perm[t][pos];
             act2[t][pos] = -1;
+            if (e >= 0) nodeOf[(int64_t)t * n + perm[t][pos]] = -1;  // became a leaf at this level
             continue;
         }
         const LvlNode nd = nodes[e];
@@ -226,12 +242,16 @@ __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, cons
         const uint32_t lr = prefix[off + pos] - prefix[off + nd.start];  // left members before pos
         const bool left = flags[item] != 0;
         const int64_t np = left ? nd.start + lr : nd.start + leftsz[e] + (pos - nd.start - lr);
-        perm2[t][np] = perm[t][pos];
+        const int32_t pt = perm[t][pos];
+        perm2[t][np] = pt;
         act2[t][np] = (int32_t)cscan[e] + (left ? 0 : 1);
+        nodeOf[(int64_t)t * n + pt] = act2[t][np];
+        posOf[(int64_t)t * n + pt] = (int32_t)np;
     }
 }
 
-__global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act) {
+__global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act, int32_t* nodeOf,
+                           int32_t* posOf) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
          item += (int64_t)gridDim.x * blockDim.x) {
@@ -239,11 +259,14 @@ __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t
         const int64_t pos = item - (int64_t)t * n;
         perm[t][pos] = (int32_t)pos;
         act[t][pos] = t;  // level 0: the root of tree t is active entry t
+        nodeOf[item] = t;
+        posOf[item] = (int32_t)pos;
     }
 }
 
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
-                        int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F) {
+                        int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F,
+                        const KTimer& margins_timer) {
     const int T = (int)F.trees.size();
     const int64_t total = (int64_t)T * n;
     if (T == 0) return true;
@@ -251,7 +274,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     const int64_t cap = F.trees[0].cap_nodes;
     std::vector<int32_t*> perm_h(T), perm2_h(T), act_h(T), act2_h(T), ch_h(T);
     std::vector<float*> hp_h(T), of_h(T);
-    DBuf<int32_t> perm2(total), act(total), act2(total);
+    DBuf<int32_t> perm2(total), act(total), act2(total), nodeOf(total), posOf(total);
     for (int t = 0; t < T; ++t) {
         perm_h[t] = F.trees[t].indices.p;
         perm2_h[t] = perm2.p + (int64_t)t * n;
@@ -285,6 +308,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
                     d_keys.p, d_hp.p, d_of.p, d_ch.p};
 
     DBuf<uint32_t> flags(total + 1), prefix(total + 1);
+    HIP_TRY(hipMemsetAsync(flags.p, 0, (total + 1) * 4, s));  // only split nodes' positions are rewritten
     size_t scan_bytes = 0;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags.p, prefix.p, (int)(total + 1)));
     DBuf<char> scan_tmp(std::max<size_t>(scan_bytes, 16));
@@ -294,6 +318,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     DBuf<uint8_t> split(max_nodes + 1);
     DBuf<uint32_t> nchild(max_nodes + 1), cscan(max_nodes + 1), leftsz(max_nodes + 1);
     DBuf<unsigned> tree_children(T);
+    DBuf<unsigned long long> active_pairs(1);
+    HIP_TRY(hipMemsetAsync(active_pairs.p, 0, 8, s));
+    unsigned long long pairs_before = 0;
     DBuf<int32_t> d_base(T);
     DBuf<uint32_t> d_cum(T);
     size_t cscan_bytes = 0;
@@ -303,25 +330,33 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     std::vector<LvlNode> roots(T);
     for (int t = 0; t < T; ++t) roots[t] = LvlNode{t, 0, 0, (int32_t)n, 0};
     HIP_TRY(hipMemcpyAsync(cur.p, roots.data(), T * sizeof(LvlNode), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T);
+    hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T,
+                       nodeOf.p, posOf.p);
     std::vector<int64_t> node_count(T, 1);
     std::vector<unsigned> tc(T);
     std::vector<int32_t> base(T);
     std::vector<uint32_t> cum(T);
     int64_t na = T;
     int phase = 0;
-    const bool v4 = dim % 4 == 0;
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
-        hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1)), dim3(256), 0, s, a, cur.p, na,
-                           (const int32_t* const*)P, split.p);
-        const unsigned mg = lgrid(total * 16, 256, 1 << 16);
-        if (v4)
-            hipLaunchKernelGGL(k_lvl_margins<true>, dim3(mg), dim3(256), 0, s, a, cur.p, T,
-                               (const int32_t* const*)P, (const int32_t* const*)(P + 2 * T), split.p, flags.p);
+        hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur.p, na,
+                           (const int32_t* const*)P, split.p, active_pairs.p);
+        margins_timer.begin(s);
+        const unsigned mg = lgrid(n * 16, 256, 1 << 16);
+        if (dim % 4 == 0 && dim <= 64)
+            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+                               flags.p);
+        else if (dim % 4 == 0 && dim <= 128)
+            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+                               flags.p);
+        else if (dim % 4 == 0 && dim <= 256)
+            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+                               flags.p);
         else
-            hipLaunchKernelGGL(k_lvl_margins<false>, dim3(mg), dim3(256), 0, s, a, cur.p, T,
-                               (const int32_t* const*)P, (const int32_t* const*)(P + 2 * T), split.p, flags.p);
+            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+                               flags.p);
+        margins_timer.stop(s);
         HIP_TRY(hipMemsetAsync(flags.p + total, 0, 4, s));
         size_t b = scan_bytes;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
@@ -332,7 +367,17 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         size_t cb = cscan_bytes;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
         HIP_TRY(hipMemcpyAsync(tc.data(), tree_children.p, T * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        unsigned long long pairs_now = 0;
+        HIP_TRY(hipMemcpyAsync(&pairs_now, active_pairs.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        {
+            // algorithmic bytes: every point row once, each split node's hyperplane once, and per tested
+            // (tree, point) its node and position reads and its flag write
+            const double tested = (double)(pairs_now - pairs_before);
+            pairs_before = pairs_now;
+            margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim * 4.0 +
+                                  tested * 12.0);
+        }
         uint32_t acc = 0;
         int64_t next_na = 0;
         bool over = false;
@@ -351,7 +396,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
                            leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt.p);
         hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur.p, T, n, split.p,
                            cscan.p, leftsz.p, prefix.p, flags.p, (const int32_t* const*)P, P + T,
-                           (const int32_t* const*)(P + 2 * T), P + 3 * T);
+                           (const int32_t* const*)(P + 2 * T), P + 3 * T, nodeOf.p, posOf.p);
         HIP_TRY(hipGetLastError());
         std::swap(cur, nxt);
         phase ^= 1;

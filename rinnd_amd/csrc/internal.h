@@ -94,10 +94,33 @@ struct rinnd_ctx {
     std::string last_error;
     std::mutex mu;
     rinnd::DBuf<uint64_t> jump_tables;  // 64 x 256 x 4 words
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};
 };
 
 namespace rinnd {
+
+// Times one kernel launch with a pair of events on its stream and adds it to a rinnd_kernel_timing
+// (a null target makes it a no-op).  end() waits for the launch.
+struct KTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    rinnd_kernel_timing* out = nullptr;
+    void begin(hipStream_t s) const {
+        if (out) HIP_TRY(hipEventRecord(a, s));
+    }
+    void stop(hipStream_t s) const {
+        if (out) HIP_TRY(hipEventRecord(b, s));
+    }
+    // waits for the launch; bytes are the launch's algorithmic bytes
+    void collect(double bytes) const {
+        if (!out) return;
+        HIP_TRY(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        out->ms += ms;
+        out->launches += 1;
+        out->bytes += bytes;
+    }
+};
 
 // ---- launchers (kernels.hip) ---------------------------------------------------------------------
 void launch_pad_rows(hipStream_t s, const float* src, int64_t n, int dim, float* dst, int64_t ld);
@@ -136,7 +159,8 @@ void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a);
 // allocated with cap_nodes each and zeroed hyperplanes/offsets; returns false when a tree needs
 // more than cap_nodes nodes.
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
-                        int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F);
+                        int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F,
+                        const KTimer& margins_timer);
 
 // leaves of one tree in depth-first order: (start, end) pairs, returns count via device int
 void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, int64_t n,
@@ -152,12 +176,14 @@ struct LeafInitWs {
     const int32_t* mark;
     const int64_t* rank;
     float* D;
+    unsigned long long* pairs;  // 1 counter: ordered pairs m(m-1) of the two-pass leaves (accumulated)
 };
 bool leaf_init_supported(int k, int64_t leaf_size);
 int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size);
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
                       const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
-                      int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws);
+                      int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws,
+                      const KTimer& dist_timer, const KTimer& push_timer);
 
 // ordered-apply pipeline
 void launch_leaf_pairs(hipStream_t s, int metric, const float* X, int64_t ld, int dim,

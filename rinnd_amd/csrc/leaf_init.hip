@@ -117,14 +117,16 @@ __global__ __launch_bounds__(256) void k_leaf_dist(int metric, const float* __re
                                                    const int32_t* __restrict__ tidx,
                                                    const int64_t* __restrict__ leaves,
                                                    const int64_t* __restrict__ n_leaves_dev, int mcap, int64_t ldl,
-                                                   float* __restrict__ D, int64_t Dl4) {
+                                                   float* __restrict__ D, int64_t Dl4, unsigned long long* pairs) {
     extern __shared__ __align__(16) float rsm[];
     const int B = blockDim.x, t = threadIdx.x;
     const int64_t n_leaves = *n_leaves_dev;
+    unsigned long long np = 0;
     for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
         const int64_t s = leaves[2 * leaf];
         const int m = (int)(leaves[2 * leaf + 1] - s);
         if (m > mcap || m < 2) continue;  // uniform; larger leaves go to the untiled kernel
+        np += (unsigned long long)m * (m - 1);
         const int vec = (int)(ld >> 2);
         for (int i = t; i < m * vec; i += B) {
             const int c = i / vec, e4 = i - c * vec;
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(256) void k_leaf_dist(int metric, const float* __re
         }
         __syncthreads();
     }
+    if (t == 0 && np) atomicAdd(pairs, np);
 }
 
 __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
@@ -221,7 +224,8 @@ int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size) {
 
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
                       const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
-                      int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws) {
+                      int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws,
+                      const KTimer& dist_timer, const KTimer& push_timer) {
     if (max_leaves <= 0) return;
     const int B = init_block(leaf_size);
     const size_t heap = init_heap_bytes(B, k);
@@ -233,11 +237,29 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
     int PB = 256;
     while (PB > 64 && init_heap_bytes(PB, k) > 64 * 1024) PB -= 64;
     if (P.mcap > 0 && ws.D && init_heap_bytes(PB, k) <= 64 * 1024) {
+        unsigned long long p0 = 0, p1 = 0;
+        if (dist_timer.out) {
+            HIP_TRY(hipMemcpyAsync(&p0, ws.pairs, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        dist_timer.begin(s);
         hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X, ld,
-                           dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4);
+                           dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
+        dist_timer.stop(s);
+        push_timer.begin(s);
         hipLaunchKernelGGL(k_leaf_push, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16)),
                            dim3(PB), init_heap_bytes(PB, k), s, tree_indices, leaves, ws.mark, ws.rank, n, P.mcap,
                            ws.D, P.Dl4, idx, dist, flags, k);
+        push_timer.stop(s);
+        if (dist_timer.out || push_timer.out) {
+            HIP_TRY(hipMemcpyAsync(&p1, ws.pairs, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            const double pairs = (double)(p1 - p0);  // ordered pairs (a, o), a != o
+            // dist: each member row once (n rows at most), ids, both halves of D written
+            dist_timer.collect((double)n * (dim * 4.0 + 4.0) + pairs * 4.0);
+            // push: D rows read, the ids of the partners, every member's heap read and written
+            push_timer.collect(pairs * 8.0 + (double)n * k * 9.0 * 2.0 + n * 40.0);
+        }
         untiled_cap = P.mcap;
     } else {
         const int64_t cap_rows = ((int64_t)kInitLdsBudget - (int64_t)heap) / (ldl * 4);
