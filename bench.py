@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy (3 times): the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--epsilons", default="0.1", help="comma-separated query epsilons to report")
+    ap.add_argument("--epsilons", default="0.1,0.15,0.2",
+                    help="comma-separated query epsilons; the fastest one at recall@10 >= 0.95 is reported")
     return ap.parse_args()
 
 
@@ -259,6 +260,23 @@ def kernel_line(stats, fam, head):
             "frac_hbm": achieved / HBM_PEAK_GBS}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_build_1Mx128_hbm_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled
+    by the copy-kernel calibration recorded there), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic "):
+                kv = dict(x.split("=", 1) for x in line.split()[1:])
+                if kv["kernel"] == kernel:
+                    return float(kv["traffic_bytes"])
+    except OSError:
+        pass
+    return None
+
+
 def roofline_of(stats, head):
     """HBM roofline of the timed kernel family with the largest share of the build."""
     fams = [(sum(s["timed"][f]["ms"] for s in stats), f) for f in ("join", "margins", "leaf_dist", "leaf_push")]
@@ -267,8 +285,8 @@ def roofline_of(stats, head):
     if not k.get("launches"):
         return None
     return {"bound": "hbm", "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": k["frac_hbm"], "traffic": None, "kernel": k["kernel"], "avg_launch_ms": k["avg_launch_ms"],
-            "bytes_per_launch": k["bytes_per_launch"]}
+            "frac": k["frac_hbm"], "traffic": measured_traffic(k["kernel"]), "kernel": k["kernel"],
+            "avg_launch_ms": k["avg_launch_ms"], "bytes_per_launch": k["bytes_per_launch"]}
 
 
 def reference_build(torch, ctx, L, Xd, n, dim, args, outs, rows, true_d):
