@@ -124,56 +124,104 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
 
 // Left flag of every (tree, point) whose node splits at this level, written at the point's position
 // (flags of other positions are never read: a split node's range holds only its own members).
-// Point-major: 16 lanes per point keep its row in registers (CH float4 per lane, dim <= 64 * CH) and
-// test it against the current hyperplane of each tree, so a row is read once per level.
+// Point-major: 16 lanes per point.  First the lanes fetch, in parallel, the point's node, split flag,
+// offset and position for every tree (lane l owns trees l, l+16, ...); then the point's row, held in
+// registers (CH float4 per lane, dim <= 64 * CH), is tested against the split trees' hyperplanes, four
+// trees per batch so their hyperplane loads are in flight together.
 template <int CH>
 __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* nodes, int n_trees,
                                                      const int32_t* __restrict__ nodeOf,
                                                      const int32_t* __restrict__ posOf, const uint8_t* split,
                                                      uint32_t* flags) {
+    constexpr int kMaxT = 64;  // trees per point handled by the lane tables (4 per lane)
     const int gl = threadIdx.x & 15;
+    const int gbase = threadIdx.x & 48;  // first lane of this 16-lane group within the wave
     const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 4;
     for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4; i < a.n; i += groups) {
+        // per-lane tables for trees t = gl + 16 j
+        int32_t tnode[kMaxT / 16], tpos[kMaxT / 16];
+        float toff[kMaxT / 16];
+        uint32_t mask = 0;  // bit j: tree gl + 16 j splits this point's node
+#pragma unroll
+        for (int j = 0; j < kMaxT / 16; ++j) {
+            const int t = gl + 16 * j;
+            tnode[j] = 0;
+            tpos[j] = 0;
+            toff[j] = 0.f;
+            if (t < n_trees) {
+                const int64_t ti = (int64_t)t * a.n + i;
+                const int32_t e = nodeOf[ti];
+                if (e >= 0 && split[e]) {
+                    const int32_t node = nodes[e].node;
+                    tnode[j] = node;
+                    tpos[j] = posOf[ti];
+                    toff[j] = a.offsets[t][node];
+                    mask |= 1u << j;
+                }
+            }
+        }
         float4 xr[CH > 0 ? CH : 1];
         const float* x = a.X + i * a.ld;
-        bool loaded = false;
-        for (int t = 0; t < n_trees; ++t) {
-            const int64_t ti = (int64_t)t * a.n + i;
-            const int32_t e = nodeOf[ti];
-            if (e < 0 || !split[e]) continue;  // uniform within the 16-lane group
-            const int32_t node = nodes[e].node;
-            const float* hp = a.hyper[t] + (int64_t)node * a.dim;
-            float acc = 0.f;
-            if (CH > 0) {
-                if (!loaded) {
+        if (CH > 0) {
 #pragma unroll
-                    for (int c = 0; c < CH; ++c) {
-                        const int d0 = 4 * gl + 64 * c;
-                        xr[c] = d0 < a.dim ? *reinterpret_cast<const float4*>(x + d0) : make_float4(0.f, 0.f, 0.f, 0.f);
-                    }
-                    loaded = true;
-                }
-#pragma unroll
-                for (int c = 0; c < CH; ++c) {
-                    const int d0 = 4 * gl + 64 * c;
-                    if (d0 < a.dim) {
-                        const float4 hv = *reinterpret_cast<const float4*>(hp + d0);
-                        acc += xr[c].x * hv.x + xr[c].y * hv.y + xr[c].z * hv.z + xr[c].w * hv.w;
-                    }
-                }
-            } else {
-                for (int d = gl; d < a.dim; d += 16) acc += x[d] * hp[d];
+            for (int c = 0; c < CH; ++c) {
+                const int d0 = 4 * gl + 64 * c;
+                xr[c] = d0 < a.dim ? *reinterpret_cast<const float4*>(x + d0) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // stays inside the 16-lane group
-            if (gl == 0) {
-                const int32_t pos = posOf[ti];
-                const float margin = acc + a.offsets[t][node];
-                uint32_t left;
-                if (fabsf(margin) < 1e-8f)
-                    left = (uint32_t)(mix64(a.keys[t] ^ mix64(((uint64_t)(uint32_t)node << 32) ^ (uint64_t)pos)) & 1);
-                else
-                    left = margin < 0.0f ? 1u : 0u;
-                flags[(int64_t)t * a.n + pos] = left;
+        }
+        // walk the split trees of this point in batches of four
+        for (int j = 0; j < kMaxT / 16; ++j) {
+            // trees t = l + 16 j over the group's lanes l whose bit j is set
+            uint64_t wm = __ballot((mask >> j) & 1u);
+            uint32_t gm = (uint32_t)((wm >> gbase) & 0xffffu);
+            while (gm) {
+                int tl[4];
+                int cnt = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    tl[u] = gm ? __ffs((int)gm) - 1 : -1;
+                    if (gm) {
+                        gm &= gm - 1;
+                        ++cnt;
+                    }
+                }
+                float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (tl[u] < 0) continue;
+                    const int t = tl[u] + 16 * j;
+                    const int32_t node = __shfl(tnode[j], gbase + tl[u]);
+                    const float* hp = a.hyper[t] + (int64_t)node * a.dim;
+                    if (CH > 0) {
+#pragma unroll
+                        for (int c = 0; c < CH; ++c) {
+                            const int d0 = 4 * gl + 64 * c;
+                            if (d0 < a.dim) {
+                                const float4 hv = *reinterpret_cast<const float4*>(hp + d0);
+                                acc[u] += xr[c].x * hv.x + xr[c].y * hv.y + xr[c].z * hv.z + xr[c].w * hv.w;
+                            }
+                        }
+                    } else {
+                        for (int d = gl; d < a.dim; d += 16) acc[u] += x[d] * hp[d];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    for (int o = 8; o > 0; o >>= 1) acc[u] += __shfl_xor(acc[u], o);  // inside the 16-lane group
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (tl[u] != gl) continue;  // the lane that owns tree t writes its flag
+                    const int t = gl + 16 * j;
+                    const float margin = acc[u] + toff[j];
+                    uint32_t left;
+                    if (fabsf(margin) < 1e-8f)
+                        left = (uint32_t)(mix64(a.keys[t] ^ mix64(((uint64_t)(uint32_t)tnode[j] << 32) ^
+                                                                  (uint64_t)tpos[j])) & 1);
+                    else
+                        left = margin < 0.0f ? 1u : 0u;
+                    flags[(int64_t)t * a.n + tpos[j]] = left;
+                }
+                (void)cnt;
             }
         }
     }
@@ -271,6 +319,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     const int64_t total = (int64_t)T * n;
     if (T == 0) return true;
     RINND_CHECK_ARG(total < (int64_t)INT32_MAX - 1, "n_trees x n must stay below 2^31 for the level forest builder");
+    RINND_CHECK_ARG(T <= 64, "the level forest builder handles at most 64 trees per call");
     const int64_t cap = F.trees[0].cap_nodes;
     std::vector<int32_t*> perm_h(T), perm2_h(T), act_h(T), act2_h(T), ch_h(T);
     std::vector<float*> hp_h(T), of_h(T);

@@ -148,6 +148,96 @@ __global__ __launch_bounds__(256) void k_leaf_dist(int metric, const float* __re
     if (t == 0 && np) atomicAdd(pairs, np);
 }
 
+// Squared Euclidean with 4 x 4 register tiles: a thread evaluates rows 4A..4A+3 against rows
+// 4O..4O+3 (A <= O), loading each 8-float chunk of the 8 rows once for 16 pairs.  Each pair keeps the
+// reference's arithmetic: 8 lane accumulators (fma of the difference with itself) over the chunks in
+// order, the hsum of l2_sqr_avx2, then the scalar tail without fma.
+__global__ __launch_bounds__(256) void k_leaf_dist_sq(const float* __restrict__ X, int64_t ld, int dim,
+                                                      const int32_t* __restrict__ tidx,
+                                                      const int64_t* __restrict__ leaves,
+                                                      const int64_t* __restrict__ n_leaves_dev, int mcap, int64_t ldl,
+                                                      float* __restrict__ D, int64_t Dl4, unsigned long long* pairs) {
+    extern __shared__ __align__(16) float rsm[];
+    const int B = blockDim.x, t = threadIdx.x;
+    const int64_t n_leaves = *n_leaves_dev;
+    const int chunks = dim >> 3;
+    unsigned long long np = 0;
+    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
+        const int64_t s = leaves[2 * leaf];
+        const int m = (int)(leaves[2 * leaf + 1] - s);
+        if (m > mcap || m < 2) continue;  // uniform; larger leaves go to the untiled kernel
+        np += (unsigned long long)m * (m - 1);
+        const int vec = (int)(ld >> 2);
+        const int mp = round_up(m, 4);
+        for (int i = t; i < mp * vec; i += B) {  // rows m..mp-1 are zero padding for the edge tiles
+            const int c = i / vec, e4 = i - c * vec;
+            *reinterpret_cast<float4*>(rsm + c * ldl + e4 * 4) =
+                c < m ? *reinterpret_cast<const float4*>(X + (int64_t)tidx[s + c] * ld + e4 * 4)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        const int nb = mp >> 2;
+        const int T = nb * (nb + 1) / 2;
+        for (int q = t; q < T; q += B) {
+            int A, O;
+            tri_pair(q, nb + 1, A, O);  // pairs A < O' over nb + 1 blocks -> A <= O = O' - 1
+            O -= 1;
+            const float* ra = rsm + (4 * A) * ldl;
+            const float* ro = rsm + (4 * O) * ldl;
+            float acc[4][4][8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int l = 0; l < 8; ++l) acc[i][j][l] = 0.f;
+            for (int c = 0; c < chunks; ++c) {
+                float4 xa[4][2], xo[4][2];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    xa[i][0] = *reinterpret_cast<const float4*>(ra + i * ldl + c * 8);
+                    xa[i][1] = *reinterpret_cast<const float4*>(ra + i * ldl + c * 8 + 4);
+                    xo[i][0] = *reinterpret_cast<const float4*>(ro + i * ldl + c * 8);
+                    xo[i][1] = *reinterpret_cast<const float4*>(ro + i * ldl + c * 8 + 4);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        float d;
+                        d = xa[i][0].x - xo[j][0].x; acc[i][j][0] = __builtin_fmaf(d, d, acc[i][j][0]);
+                        d = xa[i][0].y - xo[j][0].y; acc[i][j][1] = __builtin_fmaf(d, d, acc[i][j][1]);
+                        d = xa[i][0].z - xo[j][0].z; acc[i][j][2] = __builtin_fmaf(d, d, acc[i][j][2]);
+                        d = xa[i][0].w - xo[j][0].w; acc[i][j][3] = __builtin_fmaf(d, d, acc[i][j][3]);
+                        d = xa[i][1].x - xo[j][1].x; acc[i][j][4] = __builtin_fmaf(d, d, acc[i][j][4]);
+                        d = xa[i][1].y - xo[j][1].y; acc[i][j][5] = __builtin_fmaf(d, d, acc[i][j][5]);
+                        d = xa[i][1].z - xo[j][1].z; acc[i][j][6] = __builtin_fmaf(d, d, acc[i][j][6]);
+                        d = xa[i][1].w - xo[j][1].w; acc[i][j][7] = __builtin_fmaf(d, d, acc[i][j][7]);
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int a = 4 * A + i, o = 4 * O + j;
+                    if (a < o && o < m) {
+                        float r = hsum8(acc[i][j]);
+                        for (int e = chunks * 8; e < dim; ++e) {
+                            const float df = ra[i * ldl + e] - ro[j * ldl + e];
+                            r += df * df;
+                        }
+                        D[(s + a) * Dl4 + o] = r;
+                        D[(s + o) * Dl4 + a] = r;
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    if (t == 0 && np) atomicAdd(pairs, np);
+}
+
+// One lane per tree position: its heap in an LDS column, its root in a register (a rejected push costs
+// one compare), its row of D read four distances at a time.
 __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
                                                    const int32_t* __restrict__ mark, const int64_t* __restrict__ rank,
                                                    int64_t n, int mcap, const float* __restrict__ D, int64_t Dl4,
@@ -199,7 +289,7 @@ struct TilePlan {
 static TilePlan tile_plan(int64_t ld, int64_t leaf_size) {
     TilePlan P;
     const int mcap = (int)std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 256);
-    const size_t rows = (size_t)mcap * (ld + 4) * 4;
+    const size_t rows = (size_t)round_up((int64_t)mcap, (int64_t)4) * (ld + 4) * 4;
     if (rows > 150 * 1024) return P;
     P.mcap = mcap;
     P.Dl4 = round_up((int64_t)mcap, (int64_t)4);
@@ -243,8 +333,12 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
             HIP_TRY(hipStreamSynchronize(s));
         }
         dist_timer.begin(s);
-        hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X, ld,
-                           dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
+        if (metric == RINND_DIST_SQEUCLIDEAN)
+            hipLaunchKernelGGL(k_leaf_dist_sq, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, X, ld,
+                               dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
+        else
+            hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X,
+                               ld, dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
         dist_timer.stop(s);
         push_timer.begin(s);
         hipLaunchKernelGGL(k_leaf_push, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16)),
