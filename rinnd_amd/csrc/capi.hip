@@ -292,6 +292,15 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     HIP_TRY(hipEventRecord(ev[1], s));
 
     launch_fill_heap(s, h_idx, h_dist, h_flags, (int64_t)n * k);
+    // BF16 rows (join tiles; leaf tiles too when the data are exact in BF16)
+    const bool use_bf16 = p.join_precision == RINND_JOIN_BF16;
+    Bf16Rows B16;
+    DBuf<unsigned long long> rescored(1);
+    if (use_bf16) {
+        if (metric != RINND_DIST_SQEUCLIDEAN)
+            throw Error{RINND_ERR_UNSUPPORTED, "the BF16 join is implemented for squared Euclidean"};
+        prepare_bf16_rows(s, DX, Dld, n, dim, B16);
+    }
 
     // ---- initialize_from_leaves (mod.rs:203-300), tree by tree ----
     Pipeline PL;
@@ -304,7 +313,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         DBuf<float> dscr(leaf_init_scratch_floats(n, Dld, p.leaf_size));
         DBuf<unsigned long long> lpairs(1);
         HIP_TRY(hipMemsetAsync(lpairs.p, 0, 8, s));
-        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p};
+        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, use_bf16 && B16.exact ? &B16 : nullptr};
         const KTimer dist_timer{ev[8], ev[9], &st.timed[RINND_TIMED_LEAF_DIST]};
         const KTimer push_timer{ev[10], ev[11], &st.timed[RINND_TIMED_LEAF_PUSH]};
         for (size_t t = 0; t < F.trees.size(); ++t) {
@@ -361,14 +370,6 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     DBuf<uint64_t> rowoff(n + 1);
     DBuf<uint32_t> kc(std::max<int64_t>(n, 1));
     DBuf<JRec> JU;
-    const bool use_bf16 = p.join_precision == RINND_JOIN_BF16;
-    Bf16Rows B16;
-    DBuf<unsigned long long> rescored(1);
-    if (use_bf16) {
-        if (metric != RINND_DIST_SQEUCLIDEAN)
-            throw Error{RINND_ERR_UNSUPPORTED, "the BF16 join is implemented for squared Euclidean"};
-        prepare_bf16_rows(s, DX, Dld, n, dim, B16);
-    }
     const int T = std::max(1, (int)p.n_threads);
     const int T_eff = (n >= 256 && T > 1) ? T : 1;
     RINND_CHECK_ARG(T_eff <= 64, "n_threads above 64 is not supported");

@@ -172,11 +172,13 @@ void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, i
 // max_leaves bounds the grid.  Supported when a 64..256-thread block's heaps fit in LDS.
 // ws: the leaf marks and start ranks of launch_leaf_list and a distance scratch of
 // leaf_init_scratch_floats() floats (the two-pass kernels are skipped when it is null).
+struct Bf16Rows;
 struct LeafInitWs {
     const int32_t* mark;
     const int64_t* rank;
     float* D;
     unsigned long long* pairs;  // 1 counter: ordered pairs m(m-1) of the two-pass leaves (accumulated)
+    const Bf16Rows* exact_bf16; // exact-mode bf16 rows: pair distances on MFMA (null: FP32 kernels)
 };
 bool leaf_init_supported(int k, int64_t leaf_size);
 int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size);
@@ -259,6 +261,11 @@ struct Bf16Rows {
     int exact = 0;      // integer data with |x| <= 256 and |x|^2 <= 2^23: screened distances are exact
 };
 void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R);
+// leaf pair distances on MFMA (exact mode only), same D layout as the FP32 two-pass kernel
+size_t leaf_dist_mfma_lds(int64_t ldb, int mcap);
+void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
+                           const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, float* D, int64_t Dl4,
+                           unsigned long long* pairs);
 void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const Bf16Rows& R,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
                       int64_t row_end, const float* heap_dist, int k, const JoinOut& out,

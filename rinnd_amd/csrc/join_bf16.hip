@@ -17,9 +17,11 @@
 // the emitted updates - and the final heaps - equal the FP32 path's.  Rows are centred by the
 // column mean first (distances are translation invariant; N shrinks).
 //
-// Exact mode.  If every coordinate is an integer with |x| <= 256 (exact in bf16) and every
-// |x|^2 <= 2^23, all products, partial sums, norms and dt are integers below 2^24: dt equals the
-// true distance, which also equals the reference's FP32 result.  No re-evaluation is needed.
+// Exact mode.  If every coordinate is an integer with |x| <= 256 (exact in bf16) and either every
+// |x|^2 <= 2^22, or no coordinate is negative and every |x|^2 <= 2^23, then all products, partial sums,
+// norms, dt and every partial sum of the reference's (x - y)^2 accumulation are integers of at most
+// 2^24: dt equals the true distance, which also equals the reference's FP32 result.  No
+// re-evaluation is needed.
 #include <cstring>
 
 #include "internal.h"
@@ -52,6 +54,8 @@ __global__ void k_col_mean(const double* sums, int64_t n, int dim, float* mu) {
 
 // exactness test: flag[0] = 1 if some coordinate is not an integer in [-256, 256];
 // maxnorm = max over rows of |x|^2 (as ordered bits of a non-negative double)
+// exactness test: flag bit 0 = some coordinate is not an integer in [-256, 256], bit 1 = some
+// coordinate is negative; maxnorm = max over rows of |x|^2 (as ordered bits of a non-negative double)
 __global__ void k_exact_check(const float* __restrict__ X, int64_t ld, int64_t n, int dim, int* flag,
                               unsigned long long* maxnorm) {
     const int lane = threadIdx.x & 63;
@@ -63,7 +67,8 @@ __global__ void k_exact_check(const float* __restrict__ X, int64_t ld, int64_t n
         double s = 0.0;
         for (int c = lane; c < dim; c += 64) {
             const float v = X[r * ld + c];
-            bad |= !(v == rintf(v) && fabsf(v) <= 256.0f);
+            bad |= !(v == rintf(v) && fabsf(v) <= 256.0f) ? 1 : 0;
+            bad |= v < 0.0f ? 2 : 0;
             s += (double)v * (double)v;
         }
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -71,7 +76,7 @@ __global__ void k_exact_check(const float* __restrict__ X, int64_t ld, int64_t n
     }
     for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o);
     if (lane == 0) {  // one pair of atomics per wave
-        if (bad) atomicOr(flag, 1);
+        if (bad) atomicOr(flag, bad);
         atomicMax(maxnorm, (unsigned long long)__double_as_longlong(mx));
     }
 }
@@ -237,6 +242,89 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
     if (lane == 0 && resc) atomicAdd(a.rescored, resc);
 }
 
+// Leaf pair distances on MFMA for exact-mode data (initialize_from_leaves, core/nndescent/mod.rs:203-300):
+// the leaf's bf16 rows are staged in LDS, one wave per 16 x 16 block (A <= O) of the leaf's Gram matrix
+// over 16x16x16 bf16 MFMAs, d = |a|^2 + |o|^2 - 2 G.  In exact mode this is the reference's FP32 value
+// bit for bit (see the header), so the scratch D gets exactly what k_leaf_dist would write.
+__device__ __forceinline__ void tri_pair16(int q, int m, int& a, int& o) {
+    const int r = m * (m - 1) / 2 - 1 - q;
+    int k = (int)((sqrtf(8.0f * (float)r + 1.0f) - 1.0f) * 0.5f);
+    while ((k + 1) * (k + 2) / 2 <= r) ++k;
+    while (k * (k + 1) / 2 > r) --k;
+    a = m - 2 - k;
+    o = m - 1 - (r - k * (k + 1) / 2);
+}
+
+__global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restrict__ Xb, int64_t ldb,
+                                                        const float* __restrict__ nrm, const int32_t* __restrict__ tidx,
+                                                        const int64_t* __restrict__ leaves,
+                                                        const int64_t* __restrict__ n_leaves_dev, int mcap,
+                                                        float* __restrict__ D, int64_t Dl4, unsigned long long* pairs) {
+    extern __shared__ __align__(16) uint16_t bsm16[];
+    const int B = blockDim.x, t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63, nw = B >> 6;
+    const int lds_ld = (int)ldb + 8;  // bf16 elements per staged row (16-byte aligned, skewed banks)
+    float* nr = (float*)(bsm16 + (int64_t)round_up((int64_t)mcap, (int64_t)16) * lds_ld);
+    const int64_t n_leaves = *n_leaves_dev;
+    const int mrow = lane & 15, kq = (lane >> 4) * 4;
+    unsigned long long np = 0;
+    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
+        const int64_t s = leaves[2 * leaf];
+        const int m = (int)(leaves[2 * leaf + 1] - s);
+        if (m > mcap || m < 2) continue;  // uniform
+        np += (unsigned long long)m * (m - 1);
+        const int mp = round_up(m, 16);
+        const int v8 = (int)(ldb >> 3);  // 16-byte chunks per row
+        for (int i = t; i < mp * v8; i += B) {
+            const int r = i / v8, e = i - r * v8;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (r < m) v = *reinterpret_cast<const uint4*>(Xb + (int64_t)tidx[s + r] * ldb + e * 8);
+            *reinterpret_cast<uint4*>(bsm16 + r * lds_ld + e * 8) = v;
+        }
+        for (int i = t; i < mp; i += B) nr[i] = i < m ? nrm[tidx[s + i]] : 0.f;
+        __syncthreads();
+        const int nb = mp >> 4;
+        const int T = nb * (nb + 1) / 2;
+        for (int q = w; q < T; q += nw) {
+            int A, O;
+            tri_pair16(q, nb + 1, A, O);  // A < O' over nb + 1 blocks -> A <= O = O' - 1
+            O -= 1;
+            const uint16_t* ra = bsm16 + (16 * A + mrow) * lds_ld + kq;
+            const uint16_t* ro = bsm16 + (16 * O + mrow) * lds_ld + kq;
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            for (int64_t k0 = 0; k0 < ldb; k0 += 16) {
+                const v4s av = *reinterpret_cast<const v4s*>(ra + k0);
+                const v4s bv = *reinterpret_cast<const v4s*>(ro + k0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = 16 * A + kq + r;  // D[row][col] layout of the 16x16 MFMA
+                const int o = 16 * O + mrow;
+                if (a < o && o < m) {
+                    const float d = (nr[a] + nr[o]) - 2.0f * acc[r];
+                    D[(s + a) * Dl4 + o] = d;
+                    D[(s + o) * Dl4 + a] = d;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0 && np) atomicAdd(pairs, np);
+}
+
+size_t leaf_dist_mfma_lds(int64_t ldb, int mcap) {
+    return (size_t)round_up((int64_t)mcap, (int64_t)16) * (ldb + 8) * 2 + (size_t)round_up((int64_t)mcap, (int64_t)16) * 4;
+}
+
+void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
+                           const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, float* D, int64_t Dl4,
+                           unsigned long long* pairs) {
+    const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(max_leaves, 1), 4096);
+    hipLaunchKernelGGL(k_leaf_dist_mfma, dim3(grid), dim3(256), leaf_dist_mfma_lds(R.ldb, mcap), s, R.Xb.p, R.ldb,
+                       R.nrm.p, tree_indices, leaves, n_leaves_dev, mcap, D, Dl4, pairs);
+}
+
 void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R) {
     DBuf<int> flag(1);
     DBuf<unsigned long long> mx(1);
@@ -250,7 +338,8 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
     HIP_TRY(hipStreamSynchronize(s));
     double maxnorm;
     std::memcpy(&maxnorm, &mb, 8);
-    R.exact = (f == 0 && maxnorm <= 8388608.0) ? 1 : 0;  // integers in [-256, 256], |x|^2 <= 2^23
+    // integers in [-256, 256] with |x|^2 <= 2^22, or non-negative with |x|^2 <= 2^23
+    R.exact = ((f & 1) == 0 && (maxnorm <= 4194304.0 || ((f & 2) == 0 && maxnorm <= 8388608.0))) ? 1 : 0;
     R.ldb = round_up(dim, 16);
     R.Xb.alloc((size_t)std::max<int64_t>(n, 1) * R.ldb);
     R.nrm.alloc(std::max<int64_t>(n, 1));

@@ -1178,9 +1178,25 @@ __global__ __launch_bounds__(256) void k_heap_push_lanes(const int32_t* __restri
                 hf[j * B + c] = flags[hb + j];
             }
             const SegRec* R = seg + o0;
-            for (int64_t r = 0; r < L; ++r) {
+            float root = hd[c];
+            int64_t r = 0;
+            for (; r + 8 <= L; r += 8) {  // eight records in flight per batch
+                SegRec q[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) q[u] = R[r + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (q[u].v < root && heap_push_lane(hd, hi, hf, B, c, k, q[u].x, q[u].v)) {
+                        ++ch;
+                        root = hd[c];
+                    }
+            }
+            for (; r < L; ++r) {
                 const SegRec q = R[r];
-                ch += heap_push_lane(hd, hi, hf, B, c, k, q.x, q.v);
+                if (q.v < root && heap_push_lane(hd, hi, hf, B, c, k, q.x, q.v)) {
+                    ++ch;
+                    root = hd[c];
+                }
             }
             for (int j = 0; j < k; ++j) {
                 dist[hb + j] = hd[j * B + c];
@@ -1246,7 +1262,18 @@ __global__ __launch_bounds__(128) void k_cand_push_lanes(const int32_t* __restri
             oi[j * B + c] = -1;
         }
         const SegRec* R = seg + o0;
-        for (int64_t r = 0; r < L; ++r) {
+        int64_t r = 0;
+        for (; r + 8 <= L; r += 8) {  // eight records in flight per batch
+            SegRec q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = R[r + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (q[u].x < 0) cand_push_lane(np, ni, B, c, C, q[u].v, q[u].x & 0x7fffffff);  // flag bit: "new"
+                else cand_push_lane(op, oi, B, c, C, q[u].v, q[u].x);
+            }
+        }
+        for (; r < L; ++r) {
             const SegRec q = R[r];
             if (q.x < 0) cand_push_lane(np, ni, B, c, C, q.v, q.x & 0x7fffffff);  // flag bit set: "new" edge
             else cand_push_lane(op, oi, B, c, C, q.v, q.x);

@@ -237,47 +237,87 @@ __global__ __launch_bounds__(256) void k_leaf_dist_sq(const float* __restrict__ 
 }
 
 // One lane per tree position: its heap in an LDS column, its root in a register (a rejected push costs
-// one compare), its row of D read four distances at a time.
+// one compare).  A wave's 64 positions are consecutive, so their rows of D form one contiguous block:
+// the wave streams it through a private LDS tile in 16-column chunks, four lanes per row segment
+// (coalesced, each byte read once), with the next chunk loaded into registers while the current one is
+// replayed.  Waves never wait for each other.
+constexpr int kPushCols = 16;
+constexpr int kPushTL = kPushCols + 1;  // padded tile row: lane-strided reads are conflict-free
 __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
                                                    const int32_t* __restrict__ mark, const int64_t* __restrict__ rank,
                                                    int64_t n, int mcap, const float* __restrict__ D, int64_t Dl4,
                                                    int32_t* idx, float* dist, uint8_t* flags, int k) {
     extern __shared__ __align__(16) float psm[];
     const int B = blockDim.x, c = threadIdx.x;
+    const int lane = c & 63, w = c >> 6;
     float* hd = psm;
     int32_t* hi = (int32_t*)(hd + B * k);
     uint8_t* hf = (uint8_t*)(hi + B * k);
-    for (int64_t pos = blockIdx.x * (int64_t)B + c; pos < n; pos += (int64_t)gridDim.x * B) {
-        const int64_t l = rank[pos] + mark[pos] - 1;  // leaf holding this position
-        const int64_t s = leaves[2 * l];
-        const int m = (int)(leaves[2 * l + 1] - s);
-        if (m > mcap || m < 2) continue;
-        const int a = (int)(pos - s);
-        const int64_t hb = (int64_t)tidx[pos] * k;
-        for (int j = 0; j < k; ++j) {
-            hd[j * B + c] = dist[hb + j];
-            hi[j * B + c] = idx[hb + j];
-            hf[j * B + c] = flags[hb + j];
+    float* tile = psm + round_up((int64_t)B * k * 2 + (B * k + 3) / 4, 4) + w * 64 * kPushTL;
+    const int rl = lane >> 2, q4 = (lane & 3) * 4;  // loader role: rows rl + 16 i, columns q4..q4+3
+    for (int64_t base = ((int64_t)blockIdx.x * B) + w * 64; base < n; base += (int64_t)gridDim.x * B) {
+        const int64_t pos = base + lane;
+        int m = 0, a = 0;
+        int64_t s = 0, hb = 0;
+        if (pos < n) {
+            const int64_t l = rank[pos] + mark[pos] - 1;  // leaf holding this position
+            s = leaves[2 * l];
+            m = (int)(leaves[2 * l + 1] - s);
+            if (m > mcap || m < 2) m = 0;  // not a two-pass leaf
+            a = (int)(pos - s);
         }
-        float root = hd[c];
-        const float* Dr = D + pos * Dl4;
-        for (int o = 0; o < m; o += 4) {
-            const float4 dv = *reinterpret_cast<const float4*>(Dr + o);
-            const float du[4] = {dv.x, dv.y, dv.z, dv.w};
+        if (m) {
+            hb = (int64_t)tidx[pos] * k;
+            for (int j = 0; j < k; ++j) {
+                hd[j * B + c] = dist[hb + j];
+                hi[j * B + c] = idx[hb + j];
+                hf[j * B + c] = flags[hb + j];
+            }
+        }
+        float root = m ? hd[c] : 0.f;
+        int mw = m;  // the wave's longest row
+        for (int o = 32; o > 0; o >>= 1) mw = max(mw, __shfl_xor(mw, o));
+        const int64_t rows = min((int64_t)64, n - base);
+        float4 nx[4];
+        auto load = [&](int c0) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int oo = o + u;
-                if (oo < m && oo != a && du[u] < root) {
-                    heap_push_col(hd, hi, hf, B, c, k, tidx[s + oo], du[u]);
-                    root = hd[c];
+            for (int i = 0; i < 4; ++i) {
+                const int r = rl + 16 * i;
+                nx[i] = (r < rows && c0 + q4 < Dl4)
+                            ? *reinterpret_cast<const float4*>(D + (base + r) * Dl4 + c0 + q4)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        };
+        if (mw) load(0);
+        for (int c0 = 0; c0 < mw; c0 += kPushCols) {
+            wave_sync();  // every lane is done with the previous chunk
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float* t = tile + (rl + 16 * i) * kPushTL + q4;
+                t[0] = nx[i].x;
+                t[1] = nx[i].y;
+                t[2] = nx[i].z;
+                t[3] = nx[i].w;
+            }
+            wave_sync();
+            if (c0 + kPushCols < mw) load(c0 + kPushCols);  // in flight while this chunk is replayed
+            if (m) {
+                const int oe = min(m, c0 + kPushCols);
+                for (int o = c0; o < oe; ++o) {
+                    const float d = tile[lane * kPushTL + (o - c0)];
+                    if (o != a && d < root) {
+                        heap_push_col(hd, hi, hf, B, c, k, tidx[s + o], d);
+                        root = hd[c];
+                    }
                 }
             }
         }
-        for (int j = 0; j < k; ++j) {
-            dist[hb + j] = hd[j * B + c];
-            idx[hb + j] = hi[j * B + c];
-            flags[hb + j] = hf[j * B + c];
-        }
+        if (m)
+            for (int j = 0; j < k; ++j) {
+                dist[hb + j] = hd[j * B + c];
+                idx[hb + j] = hi[j * B + c];
+                flags[hb + j] = hf[j * B + c];
+            }
     }
 }
 
@@ -333,7 +373,11 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
             HIP_TRY(hipStreamSynchronize(s));
         }
         dist_timer.begin(s);
-        if (metric == RINND_DIST_SQEUCLIDEAN)
+        if (ws.exact_bf16 && metric == RINND_DIST_SQEUCLIDEAN &&
+            leaf_dist_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024)
+            launch_leaf_dist_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, ws.D,
+                                  P.Dl4, ws.pairs);
+        else if (metric == RINND_DIST_SQEUCLIDEAN)
             hipLaunchKernelGGL(k_leaf_dist_sq, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, X, ld,
                                dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
         else
@@ -342,8 +386,8 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
         dist_timer.stop(s);
         push_timer.begin(s);
         hipLaunchKernelGGL(k_leaf_push, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16)),
-                           dim3(PB), init_heap_bytes(PB, k), s, tree_indices, leaves, ws.mark, ws.rank, n, P.mcap,
-                           ws.D, P.Dl4, idx, dist, flags, k);
+                           dim3(PB), init_heap_bytes(PB, k) + (size_t)PB * kPushTL * 4, s, tree_indices, leaves,
+                           ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
         push_timer.stop(s);
         if (dist_timer.out || push_timer.out) {
             HIP_TRY(hipMemcpyAsync(&p1, ws.pairs, 8, hipMemcpyDeviceToHost, s));
