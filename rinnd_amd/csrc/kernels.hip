@@ -1133,20 +1133,32 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict_
 __device__ __forceinline__ bool heap_push_lane(float* hd, int32_t* hi, uint8_t* hf, int B, int c, int k, int32_t x,
                                                float d) {
     if (d >= hd[c]) return false;
-#pragma unroll 4
-    for (int j = 0; j < k; ++j)
-        if (hi[j * B + c] == x) return false;
-    int pos = 0, child = 1;  // replace the root, sift down (neighbor_heap.rs:207-243)
-    while (child < k) {
-        const int right = child + 1;
-        if (right < k && hd[child * B + c] < hd[right * B + c]) child = right;
-        const float dc = hd[child * B + c];
+    bool dup = false;  // no early exit: the slot loads issue together
+#pragma unroll 8
+    for (int j = 0; j < k; ++j) dup |= hi[j * B + c] == x;
+    if (dup) return false;
+    int pos = 0;  // replace the root, sift down (neighbor_heap.rs:207-243); both children's slots load together
+    for (;;) {
+        const int l = 2 * pos + 1, r = l + 1;
+        if (l >= k) break;
+        const float dl = hd[l * B + c];
+        const int32_t il = hi[l * B + c];
+        const uint8_t fl = hf[l * B + c];
+        float dr = dl;
+        int32_t ir = il;
+        uint8_t fr = fl;
+        if (r < k) {
+            dr = hd[r * B + c];
+            ir = hi[r * B + c];
+            fr = hf[r * B + c];
+        }
+        const bool right = r < k && dl < dr;  // the right child only if strictly larger
+        const float dc = right ? dr : dl;
         if (d >= dc) break;
         hd[pos * B + c] = dc;
-        hi[pos * B + c] = hi[child * B + c];
-        hf[pos * B + c] = hf[child * B + c];
-        pos = child;
-        child = 2 * pos + 1;
+        hi[pos * B + c] = right ? ir : il;
+        hf[pos * B + c] = right ? fr : fl;
+        pos = right ? r : l;
     }
     hd[pos * B + c] = d;
     hi[pos * B + c] = x;
@@ -1213,25 +1225,38 @@ __global__ __launch_bounds__(256) void k_heap_push_lanes(const int32_t* __restri
 __device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, int c, int n, float priority,
                                                int32_t index) {
     if (priority >= pri[c]) return;
-#pragma unroll 4
-    for (int i = 0; i < n; ++i)
-        if (ix[i * B + c] == index) return;
+    bool dup = false;  // no early exit: the slot loads issue together
+#pragma unroll 8
+    for (int i = 0; i < n; ++i) dup |= ix[i * B + c] == index;
+    if (dup) return;
     int pos = 0;
-    for (;;) {
+    for (;;) {  // both children's slots load together
         const int l = 2 * pos + 1, r = 2 * pos + 2;
+        if (l >= n) break;
+        const float pl = pri[l * B + c];
+        const int32_t il = ix[l * B + c];
+        float pr = pl;
+        int32_t ir = il;
+        if (r < n) {
+            pr = pri[r * B + c];
+            ir = ix[r * B + c];
+        }
         int best = pos;
         float bp = priority;  // the sifted value occupies pos
-        if (l < n && pri[l * B + c] > bp) {
+        int32_t bi = 0;
+        if (pl > bp) {
             best = l;
-            bp = pri[l * B + c];
+            bp = pl;
+            bi = il;
         }
-        if (r < n && pri[r * B + c] > bp) {
+        if (r < n && pr > bp) {
             best = r;
-            bp = pri[r * B + c];
+            bp = pr;
+            bi = ir;
         }
         if (best == pos) break;
         pri[pos * B + c] = bp;
-        ix[pos * B + c] = ix[best * B + c];
+        ix[pos * B + c] = bi;
         pos = best;
     }
     pri[pos * B + c] = priority;

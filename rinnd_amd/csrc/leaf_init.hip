@@ -22,20 +22,32 @@ constexpr int kInitLdsBudget = 96 * 1024;
 __device__ __forceinline__ void heap_push_col(float* hd, int32_t* hi, uint8_t* hf, int B, int c, int k, int32_t x,
                                               float d) {
     if (d >= hd[c]) return;
-#pragma unroll 4
-    for (int j = 0; j < k; ++j)
-        if (hi[j * B + c] == x) return;
-    int pos = 0, child = 1;  // replace the root and sift down (neighbor_heap.rs:207-243)
-    while (child < k) {
-        const int right = child + 1;
-        if (right < k && hd[child * B + c] < hd[right * B + c]) child = right;
-        const float dc = hd[child * B + c];
+    bool dup = false;  // no early exit: the slot loads issue together
+#pragma unroll 8
+    for (int j = 0; j < k; ++j) dup |= hi[j * B + c] == x;
+    if (dup) return;
+    int pos = 0;  // replace the root, sift down (neighbor_heap.rs:207-243); both children's slots load together
+    for (;;) {
+        const int l = 2 * pos + 1, r = l + 1;
+        if (l >= k) break;
+        const float dl = hd[l * B + c];
+        const int32_t il = hi[l * B + c];
+        const uint8_t fl = hf[l * B + c];
+        float dr = dl;
+        int32_t ir = il;
+        uint8_t fr = fl;
+        if (r < k) {
+            dr = hd[r * B + c];
+            ir = hi[r * B + c];
+            fr = hf[r * B + c];
+        }
+        const bool right = r < k && dl < dr;  // the right child only if strictly larger
+        const float dc = right ? dr : dl;
         if (d >= dc) break;
         hd[pos * B + c] = dc;
-        hi[pos * B + c] = hi[child * B + c];
-        hf[pos * B + c] = hf[child * B + c];
-        pos = child;
-        child = 2 * pos + 1;
+        hi[pos * B + c] = right ? ir : il;
+        hf[pos * B + c] = right ? fr : fl;
+        pos = right ? r : l;
     }
     hd[pos * B + c] = d;
     hi[pos * B + c] = x;
