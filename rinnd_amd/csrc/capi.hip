@@ -291,7 +291,33 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
                      &st.timed[RINND_TIMED_MARGINS]);
     HIP_TRY(hipEventRecord(ev[1], s));
 
-    launch_fill_heap(s, h_idx, h_dist, h_flags, (int64_t)n * k);
+    // Counter draws: relabel the points in tree 0's leaf order for the rest of the build, so that
+    // points that are close in space are close in memory (leaf rows, candidate rows and heap rows
+    // gather from few cache lines).  The results are mapped back to the caller's labels at the end.
+    const bool reorder = p.rng_mode == RINND_RNG_COUNTER && !F.trees.empty() && n > 1;
+    DBuf<int32_t> perm0, inv0, r_idx;
+    DBuf<float> Xr, r_dist;
+    DBuf<uint8_t> r_flags;
+    const float* DXw = DX;
+    int32_t* w_idx = h_idx;
+    float* w_dist = h_dist;
+    uint8_t* w_flags = h_flags;
+    if (reorder) {
+        perm0.alloc(n);
+        inv0.alloc(n);
+        Xr.alloc((size_t)n * Dld);
+        r_idx.alloc((size_t)n * k);
+        r_dist.alloc((size_t)n * k);
+        r_flags.alloc((size_t)n * k);
+        HIP_TRY(hipMemcpyAsync(perm0.p, F.trees[0].indices.p, n * 4, hipMemcpyDeviceToDevice, s));
+        launch_relabel_points(s, DX, Dld, n, perm0.p, inv0.p, Xr.p);
+        for (auto& T : F.trees) launch_map_ids(s, T.indices.p, n, inv0.p);
+        DXw = Xr.p;
+        w_idx = r_idx.p;
+        w_dist = r_dist.p;
+        w_flags = r_flags.p;
+    }
+    launch_fill_heap(s, w_idx, w_dist, w_flags, (int64_t)n * k);
     // BF16 rows (join tiles; leaf tiles too when the data are exact in BF16)
     const bool use_bf16 = p.join_precision == RINND_JOIN_BF16;
     Bf16Rows B16;
@@ -299,7 +325,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     if (use_bf16) {
         if (metric != RINND_DIST_SQEUCLIDEAN)
             throw Error{RINND_ERR_UNSUPPORTED, "the BF16 join is implemented for squared Euclidean"};
-        prepare_bf16_rows(s, DX, Dld, n, dim, B16);
+        prepare_bf16_rows(s, DXw, Dld, n, dim, B16);
     }
 
     // ---- initialize_from_leaves (mod.rs:203-300), tree by tree ----
@@ -319,8 +345,8 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         for (size_t t = 0; t < F.trees.size(); ++t) {
             launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
                              leaves.p, ltmp.p, lt_bytes);
-            launch_leaf_init(s, metric, DX, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
-                             h_idx, h_dist, h_flags, k, lws, dist_timer, push_timer);
+            launch_leaf_init(s, metric, DXw, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
+                             w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer);
         }
         HIP_TRY(hipStreamSynchronize(s));  // the leaf buffers die here
         PL.init(n, u_cap);
@@ -347,10 +373,10 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
                              leaves.p, ltmp.p, lt_bytes);
             launch_leaf_pair_base(s, leaves.p, n_leaves[t], pc.p, pb.p, ltmp.p, lt_bytes);
             PL.begin(s);
-            launch_leaf_pairs(s, metric, DX, Dld, dim, F.trees[t].indices.p, leaves.p, n_leaves[t],
-                              pb.p, h_dist, k, PL.U.p, PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
+            launch_leaf_pairs(s, metric, DXw, Dld, dim, F.trees[t].indices.p, leaves.p, n_leaves[t],
+                              pb.p, w_dist, k, PL.U.p, PL.U_count.p, u_cap, PL.cnt.p, PL.overflow.p);
             PL.group(s);
-            launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
+            launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, w_idx, w_dist, w_flags, k,
                               PL.changes.p, PL.big_list.p, PL.big_count.p, PL.bins.p);
         }
         PL.check_overflow(s);
@@ -380,7 +406,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     for (; it < p.n_iters; ++it) {
         HIP_TRY(hipEventRecord(ev[3], s));
         // CandidateSets::build_from_graph
-        launch_row_valid_counts(s, h_idx, n, k, (uint64_t*)rvc.p);
+        launch_row_valid_counts(s, w_idx, n, k, (uint64_t*)rvc.p);
         scan_u64_exclusive(s, rvc.p, rvp.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
         int mode = 2;
         uint64_t counter_key = 0;
@@ -433,7 +459,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             counter_key = main.next_u64();
         }
         PL.begin(s);
-        launch_cand_gen(s, h_idx, h_flags, n, k, rvp.p, prio.p, mode, T_eff, block_rows, d_bbase.p,
+        launch_cand_gen(s, w_idx, w_flags, n, k, rvp.p, prio.p, mode, T_eff, block_rows, d_bbase.p,
                         counter_key, PL.U.p, PL.cnt.p);
         {
             const unsigned long long total = (unsigned long long)2 * n * k;
@@ -445,7 +471,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         launch_fill_i32(s, old_idx.p, -1, (int64_t)n * C);
         launch_cand_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, new_idx.p, old_idx.p, C, PL.big_list.p,
                           PL.big_count.p, PL.bins.p);
-        launch_mark_old(s, h_idx, h_flags, new_idx.p, n, k, C);
+        launch_mark_old(s, w_idx, w_flags, new_idx.p, n, k, C);
         HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
         launch_cand_stats(s, new_idx.p, old_idx.p, n, C, cstat.p);
         // update_iteration: generation into per-row regions, grouping by destination, ordered apply
@@ -457,9 +483,9 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         PL.begin(s);
         HIP_TRY(hipEventRecord(ev[4], s));
         if (use_bf16)
-            launch_join_bf16(s, DX, Dld, dim, B16, new_idx.p, old_idx.p, C, 0, n, h_dist, k, jo, rescored.p);
+            launch_join_bf16(s, DXw, Dld, dim, B16, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo, rescored.p);
         else
-            launch_join_fp32(s, metric, DX, Dld, dim, new_idx.p, old_idx.p, C, 0, n, h_dist, k, jo);
+            launch_join_fp32(s, metric, DXw, Dld, dim, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo);
         HIP_TRY(hipEventRecord(ev[5], s));
         scan_u64_exclusive(s, PL.cnt.p, PL.off.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
         const uint64_t n_rec = d2h_u64(s, PL.off.p + n);
@@ -467,7 +493,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         PL.seg2.ensure(std::max<uint64_t>(n_rec, 1));
         HIP_TRY(hipMemsetAsync(PL.cur.p, 0, (n + 1) * 8, s));
         launch_join_scatter(s, new_idx.p, old_idx.p, C, 0, n, jo, PL.off.p, PL.cur.p, PL.seg.p);
-        launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, h_idx, h_dist, h_flags, k,
+        launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, w_idx, w_dist, w_flags, k,
                           PL.changes.p, PL.big_list.p, PL.big_count.p, PL.bins.p);
         HIP_TRY(hipEventRecord(ev[6], s));
         uint64_t ch = 0, cs[2] = {0, 0}, nres = 0;
@@ -503,6 +529,11 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             ++it;
             break;
         }
+    }
+    if (reorder) {
+        launch_unpermute_heap(s, w_idx, w_dist, w_flags, perm0.p, n, k, h_idx, h_dist, h_flags);
+        for (auto& T : F.trees) launch_map_ids(s, T.indices.p, n, perm0.p);
+        HIP_TRY(hipStreamSynchronize(s));  // the relabelled buffers die here
     }
     st.iterations = it;
     st.converged = converged ? 1 : 0;

@@ -1681,6 +1681,51 @@ void launch_join_scatter(hipStream_t s, const int32_t* new_idx, const int32_t* o
                        dim3(256), 0, s, new_idx, old_idx, C, row_begin, row_end, out, off, cur, seg);
 }
 
+// ---- point relabelling (counter draws) --------------------------------------------------------------
+__global__ void k_relabel_rows(const float* __restrict__ X, int64_t ld, int64_t n, const int32_t* __restrict__ perm,
+                               int32_t* inv, float* __restrict__ Xr) {
+    const int64_t vec = ld >> 2;
+    const int64_t total = n * vec;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / vec, e = t - r * vec;
+        const int32_t src = perm[r];
+        *reinterpret_cast<float4*>(Xr + r * ld + e * 4) = *reinterpret_cast<const float4*>(X + (int64_t)src * ld + e * 4);
+        if (e == 0) inv[src] = (int32_t)r;
+    }
+}
+void launch_relabel_points(hipStream_t s, const float* X, int64_t ld, int64_t n, const int32_t* perm, int32_t* inv,
+                           float* Xr) {
+    hipLaunchKernelGGL(k_relabel_rows, dim3(grid_for(n * (ld >> 2), 256)), dim3(256), 0, s, X, ld, n, perm, inv, Xr);
+}
+__global__ void k_map_ids(int32_t* ids, int64_t count, const int32_t* __restrict__ map) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = ids[i];
+        if (v >= 0) ids[i] = map[v];
+    }
+}
+void launch_map_ids(hipStream_t s, int32_t* ids, int64_t count, const int32_t* map) {
+    if (count > 0) hipLaunchKernelGGL(k_map_ids, dim3(grid_for(count, 256)), dim3(256), 0, s, ids, count, map);
+}
+// row r of the relabelled heap is row perm[r] of the caller's, and its ids map through perm
+__global__ void k_unpermute_heap(const int32_t* __restrict__ idx, const float* __restrict__ dist,
+                                 const uint8_t* __restrict__ flags, const int32_t* __restrict__ perm, int64_t n, int k,
+                                 int32_t* oidx, float* odist, uint8_t* oflags) {
+    const int64_t total = n * k;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / k, j = t - r * k;
+        const int64_t o = (int64_t)perm[r] * k + j;
+        const int32_t v = idx[t];
+        oidx[o] = v >= 0 ? perm[v] : v;
+        odist[o] = dist[t];
+        oflags[o] = flags[t];
+    }
+}
+void launch_unpermute_heap(hipStream_t s, const int32_t* idx, const float* dist, const uint8_t* flags,
+                           const int32_t* perm, int64_t n, int k, int32_t* oidx, float* odist, uint8_t* oflags) {
+    hipLaunchKernelGGL(k_unpermute_heap, dim3(grid_for(n * k, 256)), dim3(256), 0, s, idx, dist, flags, perm, n, k,
+                       oidx, odist, oflags);
+}
+
 // ---- sort_all (neighbor_heap.rs:293-320): stable ascending sort per row -------------------------
 __global__ __launch_bounds__(256) void k_sort_rows(int32_t* idx, float* dist, uint8_t* flags,
                                                    int64_t n, int k) {

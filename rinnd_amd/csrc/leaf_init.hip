@@ -314,11 +314,20 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
             wave_sync();
             if (c0 + kPushCols < mw) load(c0 + kPushCols);  // in flight while this chunk is replayed
             if (m) {
-                const int oe = min(m, c0 + kPushCols);
-                for (int o = c0; o < oe; ++o) {
-                    const float d = tile[lane * kPushTL + (o - c0)];
-                    if (o != a && d < root) {
-                        heap_push_col(hd, hi, hf, B, c, k, tidx[s + o], d);
+                // survivors of this chunk against the current root (a superset of the accepted ones:
+                // the root only falls), then pushed in leaf order
+                uint32_t msk = 0;
+#pragma unroll
+                for (int j = 0; j < kPushCols; ++j) {
+                    const int o = c0 + j;
+                    if (o < m && o != a && tile[lane * kPushTL + j] < root) msk |= 1u << j;
+                }
+                while (msk) {
+                    const int j = __ffs((int)msk) - 1;
+                    msk &= msk - 1;
+                    const float d = tile[lane * kPushTL + j];
+                    if (d < root) {
+                        heap_push_col(hd, hi, hf, B, c, k, tidx[s + c0 + j], d);
                         root = hd[c];
                     }
                 }
