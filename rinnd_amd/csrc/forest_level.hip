@@ -4,7 +4,9 @@
 // m <= leaf_size or depth >= max_depth; otherwise it splits by the bisector of two random members
 // (euclidean) or by the normalised difference of their directions (angular), a point goes left iff
 // its margin is < 0 (ties |margin| < 1e-8 decided by a random bit), and a degenerate split (all
-// points on one side) makes the node a leaf.  The random draws come from mix64(tree key, node id,
+// points on one side) makes the node a leaf.  The unit normal is rounded to bf16 (the stored f32
+// hyperplane holds exactly those values, and a bf16 copy halves the bytes the side tests read);
+// the plane passes through the midpoint of the two members.  The random draws come from mix64(tree key, node id,
 // position) instead of a depth-first FastRng stream, so every node of a level is independent and
 // one level is processed for all trees at once:
 //   k_lvl_planes   one workgroup per active node: split points, hyperplane, offset (or leaf)
@@ -33,10 +35,19 @@ struct LvlArgs {
     int64_t leaf_size, max_depth;
     int angular;
     const uint64_t* keys;  // per tree
-    float** hyper;         // per tree, cap x dim
+    float** hyper;         // per tree, cap x dim (values exactly representable in bf16)
     float** offsets;       // per tree, cap
     int32_t** children;    // per tree, cap x 2
+    uint16_t** hyper_bf;   // per tree, cap x dim bf16 copy read by the side tests
 };
+
+__device__ __forceinline__ float bf16_round(float f) {  // round to nearest even, back to f32
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(float, u & 0xffff0000u);
+}
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
 
 static inline unsigned lgrid(int64_t work, int block, int64_t cap = 1 << 20) {
     const int64_t g = ceil_div(work, (int64_t)block);
@@ -80,21 +91,27 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
         const float* p1 = a.X + (int64_t)perm[nd.tree][nd.start + i1] * a.ld;
         const float* p2 = a.X + (int64_t)perm[nd.tree][nd.start + i2] * a.ld;
         float* hp = a.hyper[nd.tree] + (int64_t)nd.node * a.dim;
+        uint16_t* hb = a.hyper_bf[nd.tree] + (int64_t)nd.node * a.dim;
         if (!a.angular) {
-            float nsq = 0.f, dnm = 0.f;
+            float nsq = 0.f;
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
                 const float h = p2[i] - p1[i];
                 nsq += h * h;
-                dnm += h * ((p1[i] + p2[i]) * 0.5f);
             }
             nsq = block_sum(nsq, red);
-            dnm = block_sum(dnm, red);
             const float norm = sqrtf(nsq);
             const bool scale = norm > 1e-8f;
             const float inv = scale ? 1.0f / norm : 0.f;
-            for (int i = threadIdx.x; i < a.dim; i += blockDim.x)
-                hp[i] = scale ? (p2[i] - p1[i]) * inv : p2[i] - p1[i];
-            if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = scale ? -dnm * inv : 0.f;
+            // normal rounded to bf16 (what the side tests read); the plane passes through the midpoint
+            float dm = 0.f;
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
+                const float h = bf16_round(scale ? (p2[i] - p1[i]) * inv : p2[i] - p1[i]);
+                hp[i] = h;
+                hb[i] = (uint16_t)(__builtin_bit_cast(uint32_t, h) >> 16);
+                dm += h * ((p1[i] + p2[i]) * 0.5f);
+            }
+            dm = block_sum(dm, red);
+            if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = scale ? -dm : 0.f;
         } else {
             float n1 = 0.f, n2 = 0.f;
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
@@ -112,7 +129,9 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
             nn = sqrtf(block_sum(nn, red));
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
                 const float h = deg ? 0.f : p2[i] / n2 - p1[i] / n1;
-                hp[i] = nn > 1e-8f ? h / nn : h;
+                const float hr = bf16_round(nn > 1e-8f ? h / nn : h);
+                hp[i] = hr;
+                hb[i] = (uint16_t)(__builtin_bit_cast(uint32_t, hr) >> 16);
             }
             if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = 0.f;
         }
@@ -130,8 +149,7 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
 // trees per batch so their hyperplane loads are in flight together.
 template <int CH>
 __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* nodes, int n_trees,
-                                                     const int32_t* __restrict__ nodeOf,
-                                                     const int32_t* __restrict__ posOf, const uint8_t* split,
+                                                     const int2* __restrict__ nodePos, const uint8_t* split,
                                                      uint32_t* flags) {
     constexpr int kMaxT = 64;  // trees per point handled by the lane tables (4 per lane)
     const int gl = threadIdx.x & 15;
@@ -150,11 +168,12 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* n
             toff[j] = 0.f;
             if (t < n_trees) {
                 const int64_t ti = (int64_t)t * a.n + i;
-                const int32_t e = nodeOf[ti];
+                const int2 np = nodePos[ti];
+                const int32_t e = np.x;
                 if (e >= 0 && split[e]) {
                     const int32_t node = nodes[e].node;
                     tnode[j] = node;
-                    tpos[j] = posOf[ti];
+                    tpos[j] = np.y;
                     toff[j] = a.offsets[t][node];
                     mask |= 1u << j;
                 }
@@ -191,18 +210,19 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* n
                     if (tl[u] < 0) continue;
                     const int t = tl[u] + 16 * j;
                     const int32_t node = __shfl(tnode[j], gbase + tl[u]);
-                    const float* hp = a.hyper[t] + (int64_t)node * a.dim;
+                    const uint16_t* hb = a.hyper_bf[t] + (int64_t)node * a.dim;
                     if (CH > 0) {
 #pragma unroll
                         for (int c = 0; c < CH; ++c) {
                             const int d0 = 4 * gl + 64 * c;
                             if (d0 < a.dim) {
-                                const float4 hv = *reinterpret_cast<const float4*>(hp + d0);
-                                acc[u] += xr[c].x * hv.x + xr[c].y * hv.y + xr[c].z * hv.z + xr[c].w * hv.w;
+                                const uint2 hv = *reinterpret_cast<const uint2*>(hb + d0);
+                                acc[u] += xr[c].x * bf16_lo(hv.x) + xr[c].y * bf16_hi(hv.x) +
+                                          xr[c].z * bf16_lo(hv.y) + xr[c].w * bf16_hi(hv.y);
                             }
                         }
                     } else {
-                        for (int d = gl; d < a.dim; d += 16) acc[u] += x[d] * hp[d];
+                        for (int d = gl; d < a.dim; d += 16) acc[u] += x[d] * bf16_lo((uint32_t)hb[d]);
                     }
                 }
 #pragma unroll
@@ -269,20 +289,29 @@ __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* sp
     }
 }
 
+// act values: >= 0 active entry; -2 became a leaf one level ago (the perm buffers already agree, the
+// act buffers are synced now); -1 settled in both buffers (never touched again).
 __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, const uint8_t* split,
                               const uint32_t* cscan, const uint32_t* leftsz, const uint32_t* prefix,
                               const uint32_t* flags, const int32_t* const* perm, int32_t* const* perm2,
-                              const int32_t* const* act, int32_t* const* act2, int32_t* nodeOf, int32_t* posOf) {
+                              int32_t* const* act, int32_t* const* act2, int2* nodePos) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
          item += (int64_t)gridDim.x * blockDim.x) {
         const int t = (int)(item / n);
         const int64_t pos = item - (int64_t)t * n;
         const int32_t e = act[t][pos];
-        if (e < 0 || !split[e]) {
-            perm2[t][pos] = perm[t][pos];
+        if (e == -1) continue;
+        if (e == -2) {
+            act[t][pos] = -1;
             act2[t][pos] = -1;
-            if (e >= 0) nodeOf[(int64_t)t * n + perm[t][pos]] = -1;  // became a leaf at this level
+            continue;
+        }
+        if (!split[e]) {  // became a leaf at this level
+            const int32_t pt = perm[t][pos];
+            perm2[t][pos] = pt;
+            act2[t][pos] = -2;
+            nodePos[(int64_t)t * n + pt] = make_int2(-1, (int32_t)pos);
             continue;
         }
         const LvlNode nd = nodes[e];
@@ -291,15 +320,14 @@ __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, cons
         const bool left = flags[item] != 0;
         const int64_t np = left ? nd.start + lr : nd.start + leftsz[e] + (pos - nd.start - lr);
         const int32_t pt = perm[t][pos];
+        const int32_t ne = (int32_t)cscan[e] + (left ? 0 : 1);
         perm2[t][np] = pt;
-        act2[t][np] = (int32_t)cscan[e] + (left ? 0 : 1);
-        nodeOf[(int64_t)t * n + pt] = act2[t][np];
-        posOf[(int64_t)t * n + pt] = (int32_t)np;
+        act2[t][np] = ne;
+        nodePos[(int64_t)t * n + pt] = make_int2(ne, (int32_t)np);
     }
 }
 
-__global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act, int32_t* nodeOf,
-                           int32_t* posOf) {
+__global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act, int2* nodePos) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
          item += (int64_t)gridDim.x * blockDim.x) {
@@ -307,8 +335,7 @@ __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t
         const int64_t pos = item - (int64_t)t * n;
         perm[t][pos] = (int32_t)pos;
         act[t][pos] = t;  // level 0: the root of tree t is active entry t
-        nodeOf[item] = t;
-        posOf[item] = (int32_t)pos;
+        nodePos[item] = make_int2(t, (int32_t)pos);
     }
 }
 
@@ -323,7 +350,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     const int64_t cap = F.trees[0].cap_nodes;
     std::vector<int32_t*> perm_h(T), perm2_h(T), act_h(T), act2_h(T), ch_h(T);
     std::vector<float*> hp_h(T), of_h(T);
-    DBuf<int32_t> perm2(total), act(total), act2(total), nodeOf(total), posOf(total);
+    DBuf<int32_t> perm2(total), act(total), act2(total);
+    DBuf<int2> nodePos(total);
+    DBuf<uint16_t> hyper_bf((size_t)T * cap * dim);
     for (int t = 0; t < T; ++t) {
         perm_h[t] = F.trees[t].indices.p;
         perm2_h[t] = perm2.p + (int64_t)t * n;
@@ -353,8 +382,12 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     HIP_TRY(hipMemcpyAsync(d_hp.p, hp_h.data(), T * sizeof(float*), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_of.p, of_h.data(), T * sizeof(float*), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_keys.p, keys.data(), T * 8, hipMemcpyHostToDevice, s));
+    std::vector<uint16_t*> hb_h(T);
+    for (int t = 0; t < T; ++t) hb_h[t] = hyper_bf.p + (size_t)t * cap * dim;
+    DBuf<uint16_t*> d_hb(T);
+    HIP_TRY(hipMemcpyAsync(d_hb.p, hb_h.data(), T * sizeof(uint16_t*), hipMemcpyHostToDevice, s));
     const LvlArgs a{X, ld, dim, n, std::max<int64_t>(leaf_size, 1), max_depth, angular,
-                    d_keys.p, d_hp.p, d_of.p, d_ch.p};
+                    d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p};
 
     DBuf<uint32_t> flags(total + 1), prefix(total + 1);
     HIP_TRY(hipMemsetAsync(flags.p, 0, (total + 1) * 4, s));  // only split nodes' positions are rewritten
@@ -380,7 +413,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     for (int t = 0; t < T; ++t) roots[t] = LvlNode{t, 0, 0, (int32_t)n, 0};
     HIP_TRY(hipMemcpyAsync(cur.p, roots.data(), T * sizeof(LvlNode), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T,
-                       nodeOf.p, posOf.p);
+                       nodePos.p);
     std::vector<int64_t> node_count(T, 1);
     std::vector<unsigned> tc(T);
     std::vector<int32_t> base(T);
@@ -394,16 +427,16 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
         if (dim % 4 == 0 && dim <= 64)
-            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
                                flags.p);
         else if (dim % 4 == 0 && dim <= 128)
-            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
                                flags.p);
         else if (dim % 4 == 0 && dim <= 256)
-            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
                                flags.p);
         else
-            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodeOf.p, posOf.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
                                flags.p);
         margins_timer.stop(s);
         HIP_TRY(hipMemsetAsync(flags.p + total, 0, 4, s));
@@ -445,7 +478,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
                            leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt.p);
         hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur.p, T, n, split.p,
                            cscan.p, leftsz.p, prefix.p, flags.p, (const int32_t* const*)P, P + T,
-                           (const int32_t* const*)(P + 2 * T), P + 3 * T, nodeOf.p, posOf.p);
+                           P + 2 * T, P + 3 * T, nodePos.p);
         HIP_TRY(hipGetLastError());
         std::swap(cur, nxt);
         phase ^= 1;
