@@ -3,7 +3,7 @@
 plus batched query QPS and p50/p99 latency, on one MI355X.
 
 Workload: 1,000,000 x 128 float32 integer-valued Gaussian mixture (1,000 centres U[0,100)^128,
-sigma 5, rounded and clipped to 0..255), squared Euclidean, k=15, RP forest (24 trees, leaf 150) +
+sigma 5, rounded and clipped to 0..255), squared Euclidean, k=15, RP forest (16 trees, leaf 256) +
 NN-Descent (max_candidates 15, <= 20 iterations, delta 0.001), counter-based draws (the GPU-native
 sampling mode).  A step is one full build from HBM-resident data to the sorted k-NN graph (forest,
 leaf init, iterations, sort_all).  One extra FP32 build with the reference's FastRng streams (the
@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=15)
-    ap.add_argument("--trees", type=int, default=24)
-    ap.add_argument("--leaf", type=int, default=150)
+    ap.add_argument("--trees", type=int, default=16)
+    ap.add_argument("--leaf", type=int, default=256)
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--delta", type=float, default=0.001)
     ap.add_argument("--threads", type=int, default=8, help="Rayon thread count reproduced by candidate sampling")
