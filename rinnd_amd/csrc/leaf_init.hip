@@ -250,10 +250,10 @@ __global__ __launch_bounds__(256) void k_leaf_dist_sq(const float* __restrict__ 
 
 // One lane per tree position: its heap in an LDS column, its root in a register (a rejected push costs
 // one compare).  A wave's 64 positions are consecutive, so their rows of D form one contiguous block:
-// the wave streams it through a private LDS tile in 16-column chunks, four lanes per row segment
+// the wave streams it through a private LDS tile in 32-column chunks, eight lanes per row segment
 // (coalesced, each byte read once), with the next chunk loaded into registers while the current one is
 // replayed.  Waves never wait for each other.
-constexpr int kPushCols = 16;
+constexpr int kPushCols = 32;
 constexpr int kPushTL = kPushCols + 1;  // padded tile row: lane-strided reads are conflict-free
 __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
                                                    const int32_t* __restrict__ mark, const int64_t* __restrict__ rank,
@@ -266,7 +266,9 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
     int32_t* hi = (int32_t*)(hd + B * k);
     uint8_t* hf = (uint8_t*)(hi + B * k);
     float* tile = psm + round_up((int64_t)B * k * 2 + (B * k + 3) / 4, 4) + w * 64 * kPushTL;
-    const int rl = lane >> 2, q4 = (lane & 3) * 4;  // loader role: rows rl + 16 i, columns q4..q4+3
+    constexpr int LPR = kPushCols / 4;  // loader lanes per row (one float4 each)
+    constexpr int RPI = 64 / LPR;       // rows per load instruction
+    const int rl = lane / LPR, q4 = (lane % LPR) * 4;  // loader role: rows rl + RPI i, columns q4..q4+3
     for (int64_t base = ((int64_t)blockIdx.x * B) + w * 64; base < n; base += (int64_t)gridDim.x * B) {
         const int64_t pos = base + lane;
         int m = 0, a = 0;
@@ -290,11 +292,11 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
         int mw = m;  // the wave's longest row
         for (int o = 32; o > 0; o >>= 1) mw = max(mw, __shfl_xor(mw, o));
         const int64_t rows = min((int64_t)64, n - base);
-        float4 nx[4];
+        float4 nx[LPR];
         auto load = [&](int c0) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int r = rl + 16 * i;
+            for (int i = 0; i < LPR; ++i) {
+                const int r = rl + RPI * i;
                 nx[i] = (r < rows && c0 + q4 < Dl4)
                             ? *reinterpret_cast<const float4*>(D + (base + r) * Dl4 + c0 + q4)
                             : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -304,8 +306,8 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
         for (int c0 = 0; c0 < mw; c0 += kPushCols) {
             wave_sync();  // every lane is done with the previous chunk
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float* t = tile + (rl + 16 * i) * kPushTL + q4;
+            for (int i = 0; i < LPR; ++i) {
+                float* t = tile + (rl + RPI * i) * kPushTL + q4;
                 t[0] = nx[i].x;
                 t[1] = nx[i].y;
                 t[2] = nx[i].z;
