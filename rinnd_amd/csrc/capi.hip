@@ -349,7 +349,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
                              w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer);
         }
         HIP_TRY(hipStreamSynchronize(s));  // the leaf buffers die here
-        PL.init(n, u_cap);
+        PL.init(n, 1);  // the update buffer is only used by the grouped leaf path below
     } else {
         DBuf<int32_t> mark(n + 1), end_at(n + 1);
         DBuf<int64_t> rank(2 * (n + 1)), leaves(2 * (n + 1));
@@ -392,6 +392,13 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     DBuf<unsigned long long> cstat(2);
     DBuf<int64_t> d_bbase(64);
     DBuf<uint64_t> d_jt;
+    // candidate build scratch (forward priorities, reverse records before / after the destination sort)
+    DBuf<float> c_fpr((size_t)n * k);
+    DBuf<uint32_t> c_rkey((size_t)n * k), c_rkey2((size_t)n * k);
+    DBuf<uint64_t> c_rval((size_t)n * k), c_rval2((size_t)n * k);
+    int c_end_bit = 1;
+    while (((int64_t)1 << c_end_bit) <= n) ++c_end_bit;
+    DBuf<char> c_sort_tmp(std::max<size_t>(cand_sort_tmp_bytes((int64_t)n * k, c_end_bit), 16));
     DBuf<unsigned long long> rpc(n + 1);
     DBuf<uint64_t> rowoff(n + 1);
     DBuf<uint32_t> kc(std::max<int64_t>(n, 1));
@@ -458,19 +465,38 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         } else {
             counter_key = main.next_u64();
         }
-        PL.begin(s);
-        launch_cand_gen(s, w_idx, w_flags, n, k, rvp.p, prio.p, mode, T_eff, block_rows, d_bbase.p,
-                        counter_key, PL.U.p, PL.cnt.p);
-        {
-            const unsigned long long total = (unsigned long long)2 * n * k;
-            HIP_TRY(hipMemcpyAsync(PL.U_count.p, &total, 8, hipMemcpyHostToDevice, s));
-            PL.group(s);
-            HIP_TRY(hipStreamSynchronize(s));
-        }
         launch_fill_i32(s, new_idx.p, -1, (int64_t)n * C);
         launch_fill_i32(s, old_idx.p, -1, (int64_t)n * C);
-        launch_cand_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, new_idx.p, old_idx.p, C, PL.big_list.p,
-                          PL.big_count.p, PL.bins.p);
+        {
+            CandBuildArgs ca{};
+            ca.idx = w_idx;
+            ca.flags = w_flags;
+            ca.n = n;
+            ca.k = k;
+            ca.C = C;
+            ca.mode = mode;
+            ca.rvp = rvp.p;
+            ca.prio = prio.p;
+            ca.block_rows = block_rows;
+            ca.block_prio_base = d_bbase.p;
+            ca.counter_key = counter_key;
+            ca.fpr = c_fpr.p;
+            ca.rkey = c_rkey.p;
+            ca.rkey2 = c_rkey2.p;
+            ca.rval = c_rval.p;
+            ca.rval2 = c_rval2.p;
+            ca.cnt = PL.cnt.p;
+            ca.off = PL.off.p;
+            ca.order = PL.big_list.p;
+            ca.bins = PL.bins.p;
+            ca.sort_tmp = c_sort_tmp.p;
+            ca.sort_tmp_bytes = c_sort_tmp.bytes();
+            ca.scan_tmp = PL.scan_tmp.p;
+            ca.scan_tmp_bytes = PL.scan_bytes;
+            ca.new_idx = new_idx.p;
+            ca.old_idx = old_idx.p;
+            launch_cand_build(s, ca);
+        }
         launch_mark_old(s, w_idx, w_flags, new_idx.p, n, k, C);
         HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
         launch_cand_stats(s, new_idx.p, old_idx.p, n, C, cstat.p);

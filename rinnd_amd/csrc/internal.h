@@ -219,6 +219,35 @@ void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, in
                      const uint64_t* row_valid_prefix, const float* prio, int mode, int T,
                      int64_t block_rows, const int64_t* block_prio_base, uint64_t counter_key,
                      URec* U, unsigned long long* cnt);
+// Candidate sets (CandidateSets::build_from_graph, candidates.rs:51-351) with forward pushes read from
+// the heap row and reverse pushes grouped by one stable radix sort; writes new_idx / old_idx
+// (pre-filled with -1).  Scratch: fpr, rkey, rval, rkey2, rval2 (n*k each), cnt/off (n+1), order (n),
+// bins (128), sort/scan temporaries.  The forward draws need C <= 64 lanes of LDS per destination.
+struct CandBuildArgs {
+    const int32_t* idx;
+    const uint8_t* flags;
+    int64_t n;
+    int k, C, mode;
+    const uint64_t* rvp;  // exclusive prefix of valid edges per row (n+1)
+    const float* prio;
+    int64_t block_rows;
+    const int64_t* block_prio_base;
+    uint64_t counter_key;
+    float* fpr;
+    uint32_t *rkey, *rkey2;
+    uint64_t *rval, *rval2;
+    unsigned long long* cnt;
+    uint64_t* off;
+    int32_t* order;
+    uint32_t* bins;
+    void* sort_tmp;
+    size_t sort_tmp_bytes;
+    void* scan_tmp;
+    size_t scan_tmp_bytes;
+    int32_t *new_idx, *old_idx;
+};
+size_t cand_sort_tmp_bytes(int64_t items, int end_bit);
+void launch_cand_build(hipStream_t s, const CandBuildArgs& a);
 // counts[0] += valid new+old candidates, counts[1] += pairs the join evaluates (per row:
 // nn(nn-1)/2 + nn*no)
 void launch_cand_stats(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t n, int C,

@@ -1453,6 +1453,188 @@ void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, in
                        row_valid_prefix, prio, mode, T, block_rows, block_prio_base, counter_key, U, cnt);
 }
 
+// ---- candidate build with a stable destination sort -----------------------------------------------
+// A vertex v receives, in the reference's order, its own heap row's edges (forward pushes, keys
+// v*k + j) and the reverse pushes of every edge i -> v (keys i*k + j'; T > 1: own row block first,
+// then the other blocks ascending).  Forward pushes need no grouping: the push kernel reads v's row.
+// Reverse pushes are written in edge order as (destination, source|flag, priority) and sorted by
+// destination with a stable radix sort, so each destination's segment is ordered by source row -
+// exactly the key order.  The push kernel merges the two streams.
+__global__ void k_cand_gen_rev(const int32_t* __restrict__ idx, const uint8_t* __restrict__ flags, int64_t n, int k,
+                               const uint64_t* __restrict__ rvp, const float* __restrict__ prio, int mode,
+                               int64_t block_rows, const int64_t* __restrict__ block_prio_base, uint64_t counter_key,
+                               float* fpr, uint32_t* rkey, uint64_t* rval, unsigned long long* cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t r = rvp[i];
+        const int64_t blk = mode == 1 ? i / block_rows : 0;
+        const uint64_t blk_first = mode == 1 ? rvp[blk * block_rows] : 0;
+        for (int j = 0; j < k; ++j) {
+            const uint64_t e = (uint64_t)i * k + j;
+            const int32_t nb = idx[e];
+            if (nb < 0) {
+                rkey[e] = (uint32_t)n;  // sorts after every destination
+                continue;
+            }
+            const uint32_t fbit = flags[e] ? 0x80000000u : 0u;
+            float pf, pr;
+            if (mode == 0) {  // candidates.rs:84-131: two draws per edge from the main stream
+                pf = prio[2 * r];
+                pr = prio[2 * r + 1];
+            } else if (mode == 1) {  // candidates.rs:170-213: one draw per edge from the row block's stream
+                pf = pr = prio[block_prio_base[blk] + (int64_t)(r - blk_first)];
+            } else {  // counter draws
+                const uint64_t h1 = mix64(counter_key ^ (2 * e));
+                const uint64_t h2 = mix64(counter_key ^ (2 * e + 1));
+                pf = (float)(uint32_t)(h1 >> 40) * (1.0f / 16777216.0f);
+                pr = (float)(uint32_t)(h2 >> 40) * (1.0f / 16777216.0f);
+            }
+            fpr[e] = pf;
+            rkey[e] = (uint32_t)nb;
+            rval[e] = ((uint64_t)__builtin_bit_cast(uint32_t, pr) << 32) | ((uint32_t)i | fbit);
+            atomicAdd(&cnt[nb], 1ull);
+            ++r;
+        }
+    }
+}
+
+// first position in [lo, hi) whose record's source row is >= bound (records sorted by source)
+__device__ __forceinline__ uint64_t src_lower_bound(const uint64_t* rv, uint64_t lo, uint64_t hi, int64_t bound) {
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((int64_t)(uint32_t)(rv[mid] & 0x7fffffffu) < bound) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict__ order, const uint32_t* bins,
+                                                       const uint64_t* __restrict__ off, const uint64_t* __restrict__ rv,
+                                                       const int32_t* __restrict__ idx, const uint8_t* __restrict__ flags,
+                                                       const float* __restrict__ fpr, int k, int mode, int64_t block_rows,
+                                                       int32_t* new_idx, int32_t* old_idx, int C) {
+    extern __shared__ __align__(16) float csm[];
+    const int B = blockDim.x, c = threadIdx.x;
+    float* np = csm;
+    int32_t* ni = (int32_t*)(np + B * C);
+    float* op = (float*)(ni + B * C);
+    int32_t* oi = (int32_t*)(op + B * C);
+    const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
+    for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
+        const int64_t e = e0 + c;
+        if (e >= total) continue;
+        const int64_t v = order[e];
+        const uint64_t o0 = off[v], o1 = off[v + 1];
+        for (int j = 0; j < C; ++j) {
+            np[j * B + c] = __builtin_inff();
+            op[j * B + c] = __builtin_inff();
+            ni[j * B + c] = -1;
+            oi[j * B + c] = -1;
+        }
+        auto push = [&](uint32_t x, float pri) {
+            if ((int32_t)x < 0) cand_push_lane(np, ni, B, c, C, pri, (int32_t)(x & 0x7fffffffu));  // "new" edge
+            else cand_push_lane(op, oi, B, c, C, pri, (int32_t)x);
+        };
+        auto rev = [&](uint64_t a, uint64_t b) {
+            for (uint64_t r = a; r < b; ++r) {
+                const uint64_t w = rv[r];
+                push((uint32_t)w, __builtin_bit_cast(float, (uint32_t)(w >> 32)));
+            }
+        };
+        auto fwd = [&]() {
+            for (int j = 0; j < k; ++j) {
+                const int64_t f = v * k + j;
+                const int32_t nb = idx[f];
+                if (nb >= 0) push((uint32_t)nb | (flags[f] ? 0x80000000u : 0u), fpr[f]);
+            }
+        };
+        if (mode != 1) {  // key order = source row order; v's own row sits between sources < v and > v
+            const uint64_t pv = src_lower_bound(rv, o0, o1, v + 1);
+            rev(o0, pv);
+            fwd();
+            rev(pv, o1);
+        } else {  // own row block first (merged with v's row), then the other blocks ascending
+            const int64_t lo = (v / block_rows) * block_rows, hi = lo + block_rows;
+            const uint64_t pa = src_lower_bound(rv, o0, o1, lo);
+            const uint64_t pb = src_lower_bound(rv, pa, o1, hi);
+            const uint64_t pv = src_lower_bound(rv, pa, pb, v + 1);
+            rev(pa, pv);
+            fwd();
+            rev(pv, pb);
+            rev(o0, pa);
+            rev(pb, o1);
+        }
+        for (int j = 0; j < C; ++j) {
+            new_idx[v * C + j] = ni[j * B + c];
+            old_idx[v * C + j] = oi[j * B + c];
+        }
+    }
+}
+
+// destination lengths for the bins: reverse records plus the row's valid forward edges
+__global__ __launch_bounds__(256) void k_bin_count2(const uint64_t* __restrict__ off, const uint64_t* __restrict__ off2,
+                                                    int64_t n_dest, uint32_t* bins) {
+    __shared__ uint32_t h[kBins];
+    for (int i = threadIdx.x; i < kBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n_dest; v += (int64_t)gridDim.x * blockDim.x) {
+        const int b = len_bin((off[v + 1] - off[v]) + (off2[v + 1] - off2[v]));
+        if (b) atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBins; i += blockDim.x)
+        if (h[i]) atomicAdd(&bins[i], h[i]);
+}
+__global__ __launch_bounds__(256) void k_bin_scatter2(const uint64_t* __restrict__ off, const uint64_t* __restrict__ off2,
+                                                      int64_t n_dest, uint32_t* bins, int32_t* order) {
+    __shared__ uint32_t h[kBins], base[kBins];
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < n_dest; v0 += (int64_t)gridDim.x * blockDim.x) {
+        for (int i = threadIdx.x; i < kBins; i += blockDim.x) h[i] = 0;
+        __syncthreads();
+        const int64_t v = v0 + threadIdx.x;
+        int b = 0;
+        uint32_t r = 0;
+        if (v < n_dest) {
+            b = len_bin((off[v + 1] - off[v]) + (off2[v + 1] - off2[v]));
+            if (b) r = atomicAdd(&h[b], 1u);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kBins; i += blockDim.x)
+            base[i] = h[i] ? bins[kBins + i] + atomicAdd(&bins[2 * kBins + i], h[i]) : 0;
+        __syncthreads();
+        if (b) order[base[b] + r] = (int32_t)v;
+        __syncthreads();
+    }
+}
+
+size_t cand_sort_tmp_bytes(int64_t items, int end_bit) {
+    size_t b = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                               (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)items, 0, end_bit));
+    return b;
+}
+
+void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
+    const int64_t nk = a.n * a.k;
+    HIP_TRY(hipMemsetAsync(a.cnt, 0, (a.n + 1) * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_cand_gen_rev, dim3(grid_for(a.n, 128)), dim3(128), 0, s, a.idx, a.flags, a.n, a.k, a.rvp,
+                       a.prio, a.mode, a.block_rows, a.block_prio_base, a.counter_key, a.fpr, a.rkey, a.rval, a.cnt);
+    int end_bit = 1;
+    while (((int64_t)1 << end_bit) <= a.n) ++end_bit;  // destinations 0..n (n = no edge)
+    size_t tb = a.sort_tmp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2, (int)nk, 0, end_bit,
+                                               s));
+    scan_u64_exclusive(s, a.cnt, a.off, a.n + 1, a.scan_tmp, a.scan_tmp_bytes);
+    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_bin_count2, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
+    hipLaunchKernelGGL(k_bin_scatter2, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins,
+                       a.order);
+    const int B = lanes_block(a.C, 16, 128);
+    hipLaunchKernelGGL(k_cand_push_rev, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)B * a.C * 16, s, a.order,
+                       a.bins, a.off, a.rval2, a.idx, a.flags, a.fpr, a.k, a.mode, a.block_rows, a.new_idx, a.old_idx,
+                       a.C);
+}
+
 __global__ void k_cand_stats(const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx,
                              int64_t n, int C, unsigned long long* counts) {
     unsigned long long rows = 0, pairs = 0;
