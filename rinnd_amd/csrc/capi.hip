@@ -401,7 +401,12 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     DBuf<char> c_sort_tmp(std::max<size_t>(cand_sort_tmp_bytes((int64_t)n * k, c_end_bit), 16));
     DBuf<unsigned long long> rpc(n + 1);
     DBuf<uint64_t> rowoff(n + 1);
-    DBuf<uint32_t> kc(std::max<int64_t>(n, 1));
+    DBuf<unsigned long long> kc(n + 1);
+    HIP_TRY(hipMemsetAsync(kc.p, 0, (n + 1) * 8, s));  // kc[n] stays 0: the scan's total slot
+    DBuf<uint64_t> koff(n + 1);
+    DBuf<uint32_t> jr_key, jr_key2;
+    DBuf<uint64_t> jr_val, jr_val2;
+    DBuf<char> jr_tmp;
     DBuf<JRec> JU;
     const int T = std::max(1, (int)p.n_threads);
     const int T_eff = (n >= 256 && T > 1) ? T : 1;
@@ -514,13 +519,43 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             launch_join_fp32(s, metric, DXw, Dld, dim, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo);
         HIP_TRY(hipEventRecord(ev[5], s));
         scan_u64_exclusive(s, PL.cnt.p, PL.off.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
+        scan_u64_exclusive(s, kc.p, koff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
         const uint64_t n_rec = d2h_u64(s, PL.off.p + n);
-        PL.seg.ensure(std::max<uint64_t>(n_rec, 1));
-        PL.seg2.ensure(std::max<uint64_t>(n_rec, 1));
-        HIP_TRY(hipMemsetAsync(PL.cur.p, 0, (n + 1) * 8, s));
-        launch_join_scatter(s, new_idx.p, old_idx.p, C, 0, n, jo, PL.off.p, PL.cur.p, PL.seg.p);
-        launch_heap_apply(s, PL.off.p, PL.seg.p, PL.seg2.p, n, w_idx, w_dist, w_flags, k,
-                          PL.changes.p, PL.big_list.p, PL.big_count.p, PL.bins.p);
+        {
+            const uint64_t items = std::max<uint64_t>(n_rec, 1);
+            jr_key.ensure(items);
+            jr_key2.ensure(items);
+            jr_val.ensure(items);
+            jr_val2.ensure(items);
+            int eb = 1;
+            while (((int64_t)1 << eb) < n) ++eb;
+            jr_tmp.ensure(std::max<size_t>(join_sort_tmp_bytes((int64_t)items, eb), 16));
+            JoinApplyArgs ja{};
+            ja.new_idx = new_idx.p;
+            ja.old_idx = old_idx.p;
+            ja.C = C;
+            ja.n = n;
+            ja.out = jo;
+            ja.koff = koff.p;
+            ja.n_records = n_rec;
+            ja.rkey = jr_key.p;
+            ja.rkey2 = jr_key2.p;
+            ja.rval = jr_val.p;
+            ja.rval2 = jr_val2.p;
+            ja.off = PL.off.p;
+            ja.order = PL.big_list.p;
+            ja.bins = PL.bins.p;
+            ja.sort_tmp = jr_tmp.p;
+            ja.sort_tmp_bytes = jr_tmp.bytes();
+            ja.scan_tmp = PL.scan_tmp.p;
+            ja.scan_tmp_bytes = PL.scan_bytes;
+            ja.idx = w_idx;
+            ja.dist = w_dist;
+            ja.flags = w_flags;
+            ja.k = k;
+            ja.changes = PL.changes.p;
+            launch_join_apply(s, ja);
+        }
         HIP_TRY(hipEventRecord(ev[6], s));
         uint64_t ch = 0, cs[2] = {0, 0}, nres = 0;
         const uint64_t nup = n_rec / 2;

@@ -266,7 +266,7 @@ static_assert(sizeof(JRec) == 8, "JRec layout");
 struct JoinOut {
     const uint64_t* rowoff;
     JRec* U;
-    uint32_t* kc;
+    unsigned long long* kc;  // kept pairs per row
     unsigned long long* cnt;
 };
 // rowoff[r] = exclusive prefix of nn(nn-1)/2 + nn*no over the rows (rowoff[n_rows] = total)
@@ -276,6 +276,36 @@ void launch_row_pairs(hipStream_t s, const int32_t* new_idx, const int32_t* old_
 void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
                       int64_t row_end, const float* heap_dist, int k, const JoinOut& out);
+// Kept pairs -> destination-grouped (id, distance) records for the ordered heap update.  Each row's
+// kept pairs are laid out in loop order (slot t ascending) as two records each - (p; q, d) and
+// (q; p, d) - at 2 * koff[r] (koff: exclusive prefix of kc), then one stable radix sort by destination
+// leaves every destination's records in generation order; off = prefix of the per-destination counts
+// (cnt).  Then one lane per destination applies them (checked_flagged_push, flag new).
+struct JoinApplyArgs {
+    const int32_t* new_idx;
+    const int32_t* old_idx;
+    int C;
+    int64_t n;
+    JoinOut out;
+    uint64_t* koff;          // n + 1
+    uint64_t n_records;      // 2 * koff[n]
+    uint32_t *rkey, *rkey2;  // n_records each
+    uint64_t *rval, *rval2;
+    const uint64_t* off;     // n + 1, prefix of cnt
+    int32_t* order;          // n
+    uint32_t* bins;          // 128
+    void* sort_tmp;
+    size_t sort_tmp_bytes;
+    void* scan_tmp;
+    size_t scan_tmp_bytes;
+    int32_t* idx;
+    float* dist;
+    uint8_t* flags;
+    int k;
+    unsigned long long* changes;
+};
+size_t join_sort_tmp_bytes(int64_t items, int end_bit);
+void launch_join_apply(hipStream_t s, const JoinApplyArgs& a);
 // kept pairs -> both destinations' segments (off/cur as in launch_scatter), key = r*C*2C + t
 void launch_join_scatter(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
                          int64_t row_end, const JoinOut& out, const uint64_t* off, unsigned long long* cur,

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                     run += (unsigned)__popcll(bal);
                 }
         wave_sync();
-        if (lane == 0) a.out.kc[r0] = run;
+        if (lane == 0) a.out.kc[r0] = (unsigned long long)run;
         for (int c = lane; c < C2; c += 64)
             if (cc[w][c]) atomicAdd(&a.out.cnt[ids[w][c]], (unsigned long long)cc[w][c]);
         wave_sync();

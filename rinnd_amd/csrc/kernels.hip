@@ -1787,7 +1787,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) out.kc[r] = run;
+        if (threadIdx.x == 0) out.kc[r] = (unsigned long long)run;
         for (int c = threadIdx.x; c < C2; c += blockDim.x)
             if (cc[c]) atomicAdd(&out.cnt[ids[c]], (unsigned long long)cc[c]);
         __syncthreads();
@@ -1828,7 +1828,7 @@ __global__ __launch_bounds__(256) void k_join_scatter(const int32_t* __restrict_
             lc[w][c] = 0;
         }
         wave_sync();
-        const uint32_t nk = out.kc[r];
+        const uint32_t nk = (uint32_t)out.kc[r];
         const JRec* E = out.U + out.rowoff[r];
         for (uint32_t e = lane; e < nk; e += 64) {
             const uint32_t t = E[e].t;
@@ -1906,6 +1906,151 @@ void launch_unpermute_heap(hipStream_t s, const int32_t* idx, const float* dist,
                            const int32_t* perm, int64_t n, int k, int32_t* oidx, float* odist, uint8_t* oflags) {
     hipLaunchKernelGGL(k_unpermute_heap, dim3(grid_for(n * k, 256)), dim3(256), 0, s, idx, dist, flags, perm, n, k,
                        oidx, odist, oflags);
+}
+
+// ---- join records in generation order + stable destination sort + lane apply ----------------------
+// One wave per row.  The row's kept pairs sit in its region in any order; their slots t are unique, so a
+// bitmask over the C x 2C slots plus a distance per slot puts them back in loop order, and a wave prefix
+// of the mask's popcounts gives each pair its rank.
+__global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict__ new_idx,
+                                                      const int32_t* __restrict__ old_idx, int C, int64_t n,
+                                                      JoinOut out, const uint64_t* __restrict__ koff, uint32_t* rkey,
+                                                      uint64_t* rval) {
+    extern __shared__ __align__(16) char jrm[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int C2 = 2 * C, S = C * C2, W = (S + 63) >> 6;
+    const int per_wave = round_up(W * 8 + S * 4 + C2 * 4, 16);
+    unsigned long long* mask = (unsigned long long*)(jrm + w * per_wave);
+    float* dv = (float*)(mask + W);
+    int32_t* ids = (int32_t*)(dv + S);
+    for (int64_t i = (int64_t)blockIdx.x * 4 + w; i < n; i += (int64_t)gridDim.x * 4) {
+        for (int c = lane; c < C2; c += 64) ids[c] = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
+        for (int q = lane; q < W; q += 64) mask[q] = 0ull;
+        wave_sync();
+        const uint32_t nk = (uint32_t)out.kc[i];
+        const JRec* E = out.U + out.rowoff[i];
+        for (uint32_t e = lane; e < nk; e += 64) {
+            const JRec q = E[e];
+            dv[q.t] = q.d;
+            atomicOr(&mask[q.t >> 6], 1ull << (q.t & 63));
+        }
+        wave_sync();
+        uint64_t carry = koff[i];
+        for (int q0 = 0; q0 < W; q0 += 64) {
+            const int q = q0 + lane;
+            const unsigned long long bits = q < W ? mask[q] : 0ull;
+            const uint32_t pc = (uint32_t)__popcll(bits);
+            uint32_t incl = pc;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            uint64_t rank = carry + (incl - pc);
+            unsigned long long b = bits;
+            while (b) {
+                const int bit = __ffsll((long long)b) - 1;
+                b &= b - 1;
+                const int t = q * 64 + bit;
+                const int j = t / C2, kk = t - j * C2;
+                const int32_t p = ids[j], qq = ids[kk];
+                const uint32_t dbits = __builtin_bit_cast(uint32_t, dv[t]);
+                rkey[2 * rank] = (uint32_t)p;
+                rval[2 * rank] = ((uint64_t)dbits << 32) | (uint32_t)qq;
+                rkey[2 * rank + 1] = (uint32_t)qq;
+                rval[2 * rank + 1] = ((uint64_t)dbits << 32) | (uint32_t)p;
+                ++rank;
+            }
+            carry += __shfl(incl, 63);
+        }
+        wave_sync();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict__ order, const uint32_t* bins,
+                                                       const uint64_t* __restrict__ off, const uint64_t* __restrict__ rv,
+                                                       int32_t* idx, float* dist, uint8_t* flags, int k,
+                                                       unsigned long long* changes) {
+    extern __shared__ __align__(16) float hsm2[];
+    const int B = blockDim.x, c = threadIdx.x;
+    float* hd = hsm2;
+    int32_t* hi = (int32_t*)(hd + B * k);
+    uint8_t* hf = (uint8_t*)(hi + B * k);
+    const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
+    unsigned long long ch = 0;
+    for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
+        const int64_t e = e0 + c;
+        if (e < total) {
+            const int64_t v = order[e];
+            const uint64_t o0 = off[v];
+            const int64_t L = (int64_t)(off[v + 1] - o0);
+            const int64_t hb = v * k;
+            for (int j = 0; j < k; ++j) {
+                hd[j * B + c] = dist[hb + j];
+                hi[j * B + c] = idx[hb + j];
+                hf[j * B + c] = flags[hb + j];
+            }
+            const uint64_t* R = rv + o0;
+            float root = hd[c];
+            int64_t r = 0;
+            for (; r + 8 <= L; r += 8) {  // eight records in flight per batch
+                uint64_t q[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) q[u] = R[r + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float d = __builtin_bit_cast(float, (uint32_t)(q[u] >> 32));
+                    if (d < root && heap_push_lane(hd, hi, hf, B, c, k, (int32_t)(uint32_t)q[u], d)) {
+                        ++ch;
+                        root = hd[c];
+                    }
+                }
+            }
+            for (; r < L; ++r) {
+                const uint64_t q = R[r];
+                const float d = __builtin_bit_cast(float, (uint32_t)(q >> 32));
+                if (d < root && heap_push_lane(hd, hi, hf, B, c, k, (int32_t)(uint32_t)q, d)) {
+                    ++ch;
+                    root = hd[c];
+                }
+            }
+            for (int j = 0; j < k; ++j) {
+                dist[hb + j] = hd[j * B + c];
+                idx[hb + j] = hi[j * B + c];
+                flags[hb + j] = hf[j * B + c];
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) ch += __shfl_xor(ch, o);
+    if ((threadIdx.x & 63) == 0 && ch) atomicAdd(changes, ch);
+}
+
+size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
+    size_t b = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                               (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)items, 0, end_bit));
+    return b;
+}
+
+void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
+    const int C2 = 2 * a.C, S = a.C * C2, W = (S + 63) >> 6;
+    const size_t per_wave = (size_t)round_up(W * 8 + S * 4 + C2 * 4, 16);
+    hipLaunchKernelGGL(k_join_records, dim3(grid_for(ceil_div(a.n, (int64_t)4), 1, 1 << 18)), dim3(256), 4 * per_wave,
+                       s, a.new_idx, a.old_idx, a.C, a.n, a.out, a.koff, a.rkey, a.rval);
+    int end_bit = 1;
+    while (((int64_t)1 << end_bit) < a.n) ++end_bit;
+    if (a.n_records > 0) {
+        size_t tb = a.sort_tmp_bytes;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2,
+                                                   (int)a.n_records, 0, end_bit, s));
+    }
+    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.n, a.bins);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
+    const int B = lanes_block(a.k, 9, 256);
+    const size_t lds = (size_t)round_up((int64_t)B * a.k * 2 + (B * a.k + 3) / 4, 4) * 4;
+    hipLaunchKernelGGL(k_heap_push_rec, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins, a.off, a.rval2,
+                       a.idx, a.dist, a.flags, a.k, a.changes);
 }
 
 // ---- sort_all (neighbor_heap.rs:293-320): stable ascending sort per row -------------------------
