@@ -143,7 +143,7 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
             std::vector<uint64_t> keys(n_trees);
             for (int64_t t = 0; t < n_trees; ++t) keys[t] = states[t * 4] ^ states[t * 4 + 3];
             if (build_forest_level(s, DX, Dld, dim, n, leaf_size, max_depth, angular, keys, F,
-                                   KTimer{ctx->ev[8], ctx->ev[9], margins}))
+                                   KTimer{ctx->ev[8], ctx->ev[9], margins}, *ctx->ws))
                 return;
             cap = 2 * n + 1;
             continue;
@@ -212,56 +212,6 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
 }
 
 // ---- NN-Descent ---------------------------------------------------------------------------------
-struct Pipeline {
-    DBuf<URec> U;
-    DBuf<unsigned long long> U_count;
-    DBuf<unsigned long long> cnt, cur;
-    DBuf<uint64_t> off;
-    DBuf<SegRec> seg, seg2;
-    DBuf<int32_t> big_list, big_count, overflow;
-    DBuf<uint32_t> bins;
-    DBuf<unsigned long long> changes;
-    DBuf<char> scan_tmp;
-    size_t scan_bytes = 0;
-    int64_t n = 0;
-    void init(int64_t n_, uint64_t u_cap) {
-        n = n_;
-        U.ensure(std::max<uint64_t>(u_cap, 1));
-        seg.ensure(2 * std::max<uint64_t>(u_cap, 1));
-        seg2.ensure(2 * std::max<uint64_t>(u_cap, 1));
-        U_count.ensure(1);
-        cnt.ensure(n + 1);
-        cur.ensure(n + 1);
-        off.ensure(n + 1);
-        big_list.ensure(std::max<int64_t>(n, 1));
-        big_count.ensure(1);
-        bins.ensure(128);
-        overflow.ensure(1);
-        changes.ensure(1);
-        size_t b = scan_u64_tmp_bytes(n + 1);
-        if (b > scan_bytes) {
-            scan_tmp.alloc(b);
-            scan_bytes = b;
-        }
-    }
-    void begin(hipStream_t s) {
-        HIP_TRY(hipMemsetAsync(cnt.p, 0, (n + 1) * 8, s));
-        HIP_TRY(hipMemsetAsync(U_count.p, 0, 8, s));
-        HIP_TRY(hipMemsetAsync(overflow.p, 0, 4, s));
-    }
-    // counts -> offsets -> grouped records
-    void group(hipStream_t s) {
-        scan_u64_exclusive(s, cnt.p, off.p, n + 1, scan_tmp.p, scan_bytes);
-        HIP_TRY(hipMemsetAsync(cur.p, 0, (n + 1) * 8, s));
-        launch_scatter(s, U.p, U_count.p, off.p, cur.p, seg.p);
-    }
-    void check_overflow(hipStream_t s) {
-        int32_t ov = 0;
-        HIP_TRY(hipMemcpyAsync(&ov, overflow.p, 4, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (ov) throw Error{RINND_ERR_INTERNAL, "update buffer overflow"};
-    }
-};
 
 struct HeapDev {
     DBuf<int32_t> idx;
@@ -295,20 +245,21 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     // points that are close in space are close in memory (leaf rows, candidate rows and heap rows
     // gather from few cache lines).  The results are mapped back to the caller's labels at the end.
     const bool reorder = p.rng_mode == RINND_RNG_COUNTER && !F.trees.empty() && n > 1;
-    DBuf<int32_t> perm0, inv0, r_idx;
-    DBuf<float> Xr, r_dist;
-    DBuf<uint8_t> r_flags;
+    Workspace& W = *ctx->ws;
+    WBuf<int32_t> perm0, inv0, r_idx;
+    WBuf<float> Xr, r_dist;
+    WBuf<uint8_t> r_flags;
     const float* DXw = DX;
     int32_t* w_idx = h_idx;
     float* w_dist = h_dist;
     uint8_t* w_flags = h_flags;
     if (reorder) {
-        perm0.alloc(n);
-        inv0.alloc(n);
-        Xr.alloc((size_t)n * Dld);
-        r_idx.alloc((size_t)n * k);
-        r_dist.alloc((size_t)n * k);
-        r_flags.alloc((size_t)n * k);
+        perm0 = WBuf<int32_t>(W, 0, n);
+        inv0 = WBuf<int32_t>(W, 1, n);
+        Xr = WBuf<float>(W, 2, (size_t)n * Dld);
+        r_idx = WBuf<int32_t>(W, 3, (size_t)n * k);
+        r_dist = WBuf<float>(W, 4, (size_t)n * k);
+        r_flags = WBuf<uint8_t>(W, 5, (size_t)n * k);
         HIP_TRY(hipMemcpyAsync(perm0.p, F.trees[0].indices.p, n * 4, hipMemcpyDeviceToDevice, s));
         launch_relabel_points(s, DX, Dld, n, perm0.p, inv0.p, Xr.p);
         for (auto& T : F.trees) launch_map_ids(s, T.indices.p, n, inv0.p);
@@ -320,8 +271,8 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     launch_fill_heap(s, w_idx, w_dist, w_flags, (int64_t)n * k);
     // BF16 rows (join tiles; leaf tiles too when the data are exact in BF16)
     const bool use_bf16 = p.join_precision == RINND_JOIN_BF16;
-    Bf16Rows B16;
-    DBuf<unsigned long long> rescored(1);
+    Bf16Rows& B16 = W.b16;
+    WBuf<unsigned long long> rescored(W, 6, 1);
     if (use_bf16) {
         if (metric != RINND_DIST_SQEUCLIDEAN)
             throw Error{RINND_ERR_UNSUPPORTED, "the BF16 join is implemented for squared Euclidean"};
@@ -329,15 +280,16 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     }
 
     // ---- initialize_from_leaves (mod.rs:203-300), tree by tree ----
-    Pipeline PL;
+    if (!W.pipeline) W.pipeline.reset(new Pipeline());
+    Pipeline& PL = *W.pipeline;
     uint64_t u_cap = (uint64_t)2 * n * k;  // candidate records per iteration
     if (leaf_init_supported(k, p.leaf_size)) {  // per-leaf fused pushes, tree by tree
-        DBuf<int32_t> mark(n + 1), end_at(n + 1);
-        DBuf<int64_t> rank(2 * (n + 1)), leaves(2 * (n + 1));
+        WBuf<int32_t> mark(W, 7, n + 1), end_at(W, 8, n + 1);
+        WBuf<int64_t> rank(W, 9, 2 * (n + 1)), leaves(W, 10, 2 * (n + 1));
         size_t lt_bytes = std::max(scan_u64_tmp_bytes(n + 2), (size_t)16);
-        DBuf<char> ltmp(lt_bytes);
-        DBuf<float> dscr(leaf_init_scratch_floats(n, Dld, p.leaf_size));
-        DBuf<unsigned long long> lpairs(1);
+        WBuf<char> ltmp(W, 11, lt_bytes);
+        WBuf<float> dscr(W, 12, leaf_init_scratch_floats(n, Dld, p.leaf_size));
+        WBuf<unsigned long long> lpairs(W, 13, 1);
         HIP_TRY(hipMemsetAsync(lpairs.p, 0, 8, s));
         const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, use_bf16 && B16.exact ? &B16 : nullptr};
         const KTimer dist_timer{ev[8], ev[9], &st.timed[RINND_TIMED_LEAF_DIST]};
@@ -348,7 +300,6 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             launch_leaf_init(s, metric, DXw, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
                              w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer);
         }
-        HIP_TRY(hipStreamSynchronize(s));  // the leaf buffers die here
         PL.init(n, 1);  // the update buffer is only used by the grouped leaf path below
     } else {
         DBuf<int32_t> mark(n + 1), end_at(n + 1);
@@ -384,30 +335,30 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     HIP_TRY(hipEventRecord(ev[2], s));
 
     // ---- iterations (mod.rs:137-167) ----
-    DBuf<int32_t> new_idx((size_t)n * C), old_idx((size_t)n * C);
-    DBuf<uint64_t> rvp(n + 1);
-    DBuf<unsigned long long> rvc(n + 1);
-    DBuf<float> prio((size_t)2 * n * k + 1);
-    DBuf<RngStream> d_streams(64);
-    DBuf<unsigned long long> cstat(2);
-    DBuf<int64_t> d_bbase(64);
-    DBuf<uint64_t> d_jt;
+    WBuf<int32_t> new_idx(W, 14, (size_t)n * C), old_idx(W, 15, (size_t)n * C);
+    WBuf<uint64_t> rvp(W, 16, n + 1);
+    WBuf<unsigned long long> rvc(W, 17, n + 1);
+    WBuf<float> prio(W, 18, (size_t)2 * n * k + 1);
+    WBuf<RngStream> d_streams(W, 19, 64);
+    WBuf<unsigned long long> cstat(W, 20, 2);
+    WBuf<int64_t> d_bbase(W, 21, 64);
+    WBuf<uint64_t> d_jt;
     // candidate build scratch (forward priorities, reverse records before / after the destination sort)
-    DBuf<float> c_fpr((size_t)n * k);
-    DBuf<uint32_t> c_rkey((size_t)n * k), c_rkey2((size_t)n * k);
-    DBuf<uint64_t> c_rval((size_t)n * k), c_rval2((size_t)n * k);
+    WBuf<float> c_fpr(W, 22, (size_t)n * k);
+    WBuf<uint32_t> c_rkey(W, 23, (size_t)n * k), c_rkey2(W, 24, (size_t)n * k);
+    WBuf<uint64_t> c_rval(W, 25, (size_t)n * k), c_rval2(W, 26, (size_t)n * k);
     int c_end_bit = 1;
     while (((int64_t)1 << c_end_bit) <= n) ++c_end_bit;
-    DBuf<char> c_sort_tmp(std::max<size_t>(cand_sort_tmp_bytes((int64_t)n * k, c_end_bit), 16));
-    DBuf<unsigned long long> rpc(n + 1);
-    DBuf<uint64_t> rowoff(n + 1);
-    DBuf<unsigned long long> kc(n + 1);
+    WBuf<char> c_sort_tmp(W, 27, std::max<size_t>(cand_sort_tmp_bytes((int64_t)n * k, c_end_bit), 16));
+    WBuf<unsigned long long> rpc(W, 28, n + 1);
+    WBuf<uint64_t> rowoff(W, 29, n + 1);
+    WBuf<unsigned long long> kc(W, 30, n + 1);
     HIP_TRY(hipMemsetAsync(kc.p, 0, (n + 1) * 8, s));  // kc[n] stays 0: the scan's total slot
-    DBuf<uint64_t> koff(n + 1);
-    DBuf<uint32_t> jr_key, jr_key2;
-    DBuf<uint64_t> jr_val, jr_val2;
-    DBuf<char> jr_tmp;
-    DBuf<JRec> JU;
+    WBuf<uint64_t> koff(W, 31, n + 1);
+    WBuf<uint32_t> jr_key, jr_key2;
+    WBuf<uint64_t> jr_val, jr_val2;
+    WBuf<char> jr_tmp;
+    WBuf<JRec> JU;
     const int T = std::max(1, (int)p.n_threads);
     const int T_eff = (n >= 256 && T > 1) ? T : 1;
     RINND_CHECK_ARG(T_eff <= 64, "n_threads above 64 is not supported");
@@ -458,7 +409,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
                 HIP_TRY(hipMemcpyAsync(d_bbase.p, bbase.data(), T_eff * 8, hipMemcpyHostToDevice, s));
             }
             if (!d_jt.p) {
-                d_jt.alloc((size_t)kJumpLevels * 1024);
+                d_jt = WBuf<uint64_t>(W, 32, (size_t)kJumpLevels * 1024);
                 HIP_TRY(hipMemcpyAsync(d_jt.p, host_jump_tables(), d_jt.bytes(), hipMemcpyHostToDevice, s));
             }
             int64_t maxc = 0;
@@ -509,7 +460,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
         HIP_TRY(hipMemsetAsync(rescored.p, 0, 8, s));
         launch_row_pairs(s, new_idx.p, old_idx.p, 0, n, C, rpc.p, rowoff.p, PL.scan_tmp.p, PL.scan_bytes);
-        JU.ensure(std::max<uint64_t>(d2h_u64(s, rowoff.p + n), 1));
+        JU = WBuf<JRec>(W, 33, std::max<uint64_t>(d2h_u64(s, rowoff.p + n), 1));
         const JoinOut jo{rowoff.p, JU.p, kc.p, PL.cnt.p};
         PL.begin(s);
         HIP_TRY(hipEventRecord(ev[4], s));
@@ -523,13 +474,13 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         const uint64_t n_rec = d2h_u64(s, PL.off.p + n);
         {
             const uint64_t items = std::max<uint64_t>(n_rec, 1);
-            jr_key.ensure(items);
-            jr_key2.ensure(items);
-            jr_val.ensure(items);
-            jr_val2.ensure(items);
+            jr_key = WBuf<uint32_t>(W, 34, items);
+            jr_key2 = WBuf<uint32_t>(W, 35, items);
+            jr_val = WBuf<uint64_t>(W, 36, items);
+            jr_val2 = WBuf<uint64_t>(W, 37, items);
             int eb = 1;
             while (((int64_t)1 << eb) < n) ++eb;
-            jr_tmp.ensure(std::max<size_t>(join_sort_tmp_bytes((int64_t)items, eb), 16));
+            jr_tmp = WBuf<char>(W, 38, std::max<size_t>(join_sort_tmp_bytes((int64_t)items, eb), 16));
             JoinApplyArgs ja{};
             ja.new_idx = new_idx.p;
             ja.old_idx = old_idx.p;
@@ -687,6 +638,7 @@ rinnd_status rinnd_ctx_create(int device, rinnd_ctx** out) {
     if (device < 0 || device >= count) return RINND_ERR_INVALID_ARG;
     if (hipSetDevice(device) != hipSuccess) return RINND_ERR_HIP;
     auto* c = new rinnd_ctx;
+    c->ws = new rinnd::Workspace();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -706,6 +658,8 @@ void rinnd_ctx_destroy(rinnd_ctx* ctx) {
         if (e) (void)hipEventDestroy(e);
     ctx->jump_tables.release();
     (void)hipStreamDestroy(ctx->stream);
+    delete ctx->ws;  // frees the build scratch with the device current
+    ctx->ws = nullptr;
     delete ctx;
 }
 

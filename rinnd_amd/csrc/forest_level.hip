@@ -341,7 +341,7 @@ __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t
 
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
                         int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F,
-                        const KTimer& margins_timer) {
+                        const KTimer& margins_timer, Workspace& W) {
     const int T = (int)F.trees.size();
     const int64_t total = (int64_t)T * n;
     if (T == 0) return true;
@@ -350,9 +350,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     const int64_t cap = F.trees[0].cap_nodes;
     std::vector<int32_t*> perm_h(T), perm2_h(T), act_h(T), act2_h(T), ch_h(T);
     std::vector<float*> hp_h(T), of_h(T);
-    DBuf<int32_t> perm2(total), act(total), act2(total);
-    DBuf<int2> nodePos(total);
-    DBuf<uint16_t> hyper_bf((size_t)T * cap * dim);
+    WBuf<int32_t> perm2(W, kWsForest + 0, total), act(W, kWsForest + 1, total), act2(W, kWsForest + 2, total);
+    WBuf<int2> nodePos(W, kWsForest + 3, total);
+    WBuf<uint16_t> hyper_bf(W, kWsForest + 4, (size_t)T * cap * dim);
     for (int t = 0; t < T; ++t) {
         perm_h[t] = F.trees[t].indices.p;
         perm2_h[t] = perm2.p + (int64_t)t * n;
@@ -374,9 +374,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         tab[6 * T + t] = act2_h[t];
         tab[7 * T + t] = act_h[t];
     }
-    DBuf<int32_t*> d_tab(tab.size()), d_ch(T);
-    DBuf<float*> d_hp(T), d_of(T);
-    DBuf<uint64_t> d_keys(T);
+    WBuf<int32_t*> d_tab(W, kWsForest + 5, tab.size()), d_ch(W, kWsForest + 6, T);
+    WBuf<float*> d_hp(W, kWsForest + 7, T), d_of(W, kWsForest + 8, T);
+    WBuf<uint64_t> d_keys(W, kWsForest + 9, T);
     HIP_TRY(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * sizeof(int32_t*), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_ch.p, ch_h.data(), T * sizeof(int32_t*), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_hp.p, hp_h.data(), T * sizeof(float*), hipMemcpyHostToDevice, s));
@@ -384,34 +384,37 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     HIP_TRY(hipMemcpyAsync(d_keys.p, keys.data(), T * 8, hipMemcpyHostToDevice, s));
     std::vector<uint16_t*> hb_h(T);
     for (int t = 0; t < T; ++t) hb_h[t] = hyper_bf.p + (size_t)t * cap * dim;
-    DBuf<uint16_t*> d_hb(T);
+    WBuf<uint16_t*> d_hb(W, kWsForest + 10, T);
     HIP_TRY(hipMemcpyAsync(d_hb.p, hb_h.data(), T * sizeof(uint16_t*), hipMemcpyHostToDevice, s));
     const LvlArgs a{X, ld, dim, n, std::max<int64_t>(leaf_size, 1), max_depth, angular,
                     d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p};
 
-    DBuf<uint32_t> flags(total + 1), prefix(total + 1);
+    WBuf<uint32_t> flags(W, kWsForest + 11, total + 1), prefix(W, kWsForest + 12, total + 1);
     HIP_TRY(hipMemsetAsync(flags.p, 0, (total + 1) * 4, s));  // only split nodes' positions are rewritten
     size_t scan_bytes = 0;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags.p, prefix.p, (int)(total + 1)));
-    DBuf<char> scan_tmp(std::max<size_t>(scan_bytes, 16));
+    WBuf<char> scan_tmp(W, kWsForest + 13, std::max<size_t>(scan_bytes, 16));
     // a level holds at most n nodes per tree (every active node has at least one member)
     const int64_t max_nodes = std::max<int64_t>(1, std::min<int64_t>(cap, n)) * (int64_t)T;
-    DBuf<LvlNode> cur(max_nodes + 1), nxt(max_nodes + 1);
-    DBuf<uint8_t> split(max_nodes + 1);
-    DBuf<uint32_t> nchild(max_nodes + 1), cscan(max_nodes + 1), leftsz(max_nodes + 1);
-    DBuf<unsigned> tree_children(T);
-    DBuf<unsigned long long> active_pairs(1);
+    WBuf<LvlNode> curb(W, kWsForest + 14, max_nodes + 1), nxtb(W, kWsForest + 15, max_nodes + 1);
+    WBuf<uint8_t> split(W, kWsForest + 16, max_nodes + 1);
+    WBuf<uint32_t> nchild(W, kWsForest + 17, max_nodes + 1), cscan(W, kWsForest + 18, max_nodes + 1),
+        leftsz(W, kWsForest + 19, max_nodes + 1);
+    WBuf<unsigned> tree_children(W, kWsForest + 20, T);
+    LvlNode* cur = curb.p;
+    LvlNode* nxt = nxtb.p;
+    WBuf<unsigned long long> active_pairs(W, kWsForest + 21, 1);
     HIP_TRY(hipMemsetAsync(active_pairs.p, 0, 8, s));
     unsigned long long pairs_before = 0;
-    DBuf<int32_t> d_base(T);
-    DBuf<uint32_t> d_cum(T);
+    WBuf<int32_t> d_base(W, kWsForest + 22, T);
+    WBuf<uint32_t> d_cum(W, kWsForest + 23, T);
     size_t cscan_bytes = 0;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, cscan_bytes, nchild.p, cscan.p, (int)(max_nodes + 1)));
-    DBuf<char> cscan_tmp(std::max<size_t>(cscan_bytes, 16));
+    WBuf<char> cscan_tmp(W, kWsForest + 24, std::max<size_t>(cscan_bytes, 16));
 
     std::vector<LvlNode> roots(T);
     for (int t = 0; t < T; ++t) roots[t] = LvlNode{t, 0, 0, (int32_t)n, 0};
-    HIP_TRY(hipMemcpyAsync(cur.p, roots.data(), T * sizeof(LvlNode), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(cur, roots.data(), T * sizeof(LvlNode), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T,
                        nodePos.p);
     std::vector<int64_t> node_count(T, 1);
@@ -422,28 +425,28 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     int phase = 0;
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
-        hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur.p, na,
+        hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur, na,
                            (const int32_t* const*)P, split.p, active_pairs.p);
         margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
         if (dim % 4 == 0 && dim <= 64)
-            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
                                flags.p);
         else if (dim % 4 == 0 && dim <= 128)
-            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
                                flags.p);
         else if (dim % 4 == 0 && dim <= 256)
-            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
                                flags.p);
         else
-            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, cur.p, T, nodePos.p, split.p,
+            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
                                flags.p);
         margins_timer.stop(s);
         HIP_TRY(hipMemsetAsync(flags.p + total, 0, 4, s));
         size_t b = scan_bytes;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
         HIP_TRY(hipMemsetAsync(tree_children.p, 0, T * sizeof(unsigned), s));
-        hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur.p, na, prefix.p, split.p,
+        hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, prefix.p, split.p,
                            nchild.p, leftsz.p, tree_children.p);
         HIP_TRY(hipMemsetAsync(nchild.p + na, 0, 4, s));
         size_t cb = cscan_bytes;
@@ -474,9 +477,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         if (over) return false;
         HIP_TRY(hipMemcpyAsync(d_base.p, base.data(), T * 4, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(d_cum.p, cum.data(), T * 4, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur.p, na, split.p, cscan.p,
-                           leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt.p);
-        hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur.p, T, n, split.p,
+        hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
+                           leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt);
+        hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur, T, n, split.p,
                            cscan.p, leftsz.p, prefix.p, flags.p, (const int32_t* const*)P, P + T,
                            P + 2 * T, P + 3 * T, nodePos.p);
         HIP_TRY(hipGetLastError());

@@ -1,6 +1,7 @@
 // internal.h - declarations shared by the kernel and host translation units.
 #pragma once
 
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -88,6 +89,10 @@ struct rinnd_forest {
     std::vector<rinnd::TreeDev> trees;
 };
 
+namespace rinnd {
+struct Workspace;
+}
+
 struct rinnd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -95,6 +100,7 @@ struct rinnd_ctx {
     std::mutex mu;
     rinnd::DBuf<uint64_t> jump_tables;  // 64 x 256 x 4 words
     hipEvent_t ev[12] = {};
+    rinnd::Workspace* ws = nullptr;     // build scratch kept between calls (released by rinnd_ctx_destroy)
 };
 
 namespace rinnd {
@@ -158,9 +164,10 @@ void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a);
 // forest, counter streams (level-synchronous across trees, forest_level.hip).  Trees must be
 // allocated with cap_nodes each and zeroed hyperplanes/offsets; returns false when a tree needs
 // more than cap_nodes nodes.
+struct Workspace;
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
                         int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F,
-                        const KTimer& margins_timer);
+                        const KTimer& margins_timer, Workspace& W);
 
 // leaves of one tree in depth-first order: (start, end) pairs, returns count via device int
 void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, int64_t n,
@@ -320,6 +327,8 @@ struct Bf16Rows {
     int exact = 0;      // integer data with |x| <= 256 and |x|^2 <= 2^23: screened distances are exact
 };
 void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R);
+
+
 // leaf pair distances on MFMA (exact mode only), same D layout as the FP32 two-pass kernel
 size_t leaf_dist_mfma_lds(int64_t ldb, int mcap);
 void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
@@ -346,5 +355,81 @@ void launch_correct(hipStream_t s, float* dist, int64_t count, int corr);
 size_t scan_u64_tmp_bytes(int64_t n);
 void scan_u64_exclusive(hipStream_t s, const unsigned long long* in, uint64_t* out, int64_t n,
                         void* tmp, size_t tmp_bytes);
+
+// Grouping buffers of the ordered-apply pipeline (grow-only).
+struct Pipeline {
+    DBuf<URec> U;
+    DBuf<unsigned long long> U_count;
+    DBuf<unsigned long long> cnt, cur;
+    DBuf<uint64_t> off;
+    DBuf<SegRec> seg, seg2;
+    DBuf<int32_t> big_list, big_count, overflow;
+    DBuf<uint32_t> bins;
+    DBuf<unsigned long long> changes;
+    DBuf<char> scan_tmp;
+    size_t scan_bytes = 0;
+    int64_t n = 0;
+    void init(int64_t n_, uint64_t u_cap) {
+        n = n_;
+        U.ensure(std::max<uint64_t>(u_cap, 1));
+        seg.ensure(2 * std::max<uint64_t>(u_cap, 1));
+        seg2.ensure(2 * std::max<uint64_t>(u_cap, 1));
+        U_count.ensure(1);
+        cnt.ensure(n + 1);
+        cur.ensure(n + 1);
+        off.ensure(n + 1);
+        big_list.ensure(std::max<int64_t>(n, 1));
+        big_count.ensure(1);
+        bins.ensure(128);
+        overflow.ensure(1);
+        changes.ensure(1);
+        size_t b = scan_u64_tmp_bytes(n + 1);
+        if (b > scan_bytes) {
+            scan_tmp.alloc(b);
+            scan_bytes = b;
+        }
+    }
+    void begin(hipStream_t s) {
+        HIP_TRY(hipMemsetAsync(cnt.p, 0, (n + 1) * 8, s));
+        HIP_TRY(hipMemsetAsync(U_count.p, 0, 8, s));
+        HIP_TRY(hipMemsetAsync(overflow.p, 0, 4, s));
+    }
+    // counts -> offsets -> grouped records
+    void group(hipStream_t s) {
+        scan_u64_exclusive(s, cnt.p, off.p, n + 1, scan_tmp.p, scan_bytes);
+        HIP_TRY(hipMemsetAsync(cur.p, 0, (n + 1) * 8, s));
+        launch_scatter(s, U.p, U_count.p, off.p, cur.p, seg.p);
+    }
+    void check_overflow(hipStream_t s) {
+        int32_t ov = 0;
+        HIP_TRY(hipMemcpyAsync(&ov, overflow.p, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (ov) throw Error{RINND_ERR_INTERNAL, "update buffer overflow"};
+    }
+};
+
+// Grow-only device buffers a context keeps between builds: allocating and releasing the multi-GB
+// scratch of a build would otherwise stall every call.  Each use site owns a fixed slot; contents are
+// never assumed to survive from one build to the next.
+struct Workspace {
+    std::vector<DBuf<char>> slots;
+    Bf16Rows b16;
+    std::unique_ptr<Pipeline> pipeline;
+    void* get(int slot, size_t bytes) {
+        if ((int)slots.size() <= slot) slots.resize(slot + 1);
+        slots[slot].ensure(std::max<size_t>(bytes, 16));
+        return slots[slot].p;
+    }
+};
+template <class T>
+struct WBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    WBuf() = default;
+    WBuf(Workspace& w, int slot, size_t count) : p((T*)w.get(slot, count * sizeof(T))), n(count) {}
+    size_t bytes() const { return n * sizeof(T); }
+};
+// slot ranges: build (capi.hip) 0..99, level forest 100..149
+enum : int { kWsForest = 100 };
 
 }  // namespace rinnd

@@ -341,13 +341,13 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
     // integers in [-256, 256] with |x|^2 <= 2^22, or non-negative with |x|^2 <= 2^23
     R.exact = ((f & 1) == 0 && (maxnorm <= 4194304.0 || ((f & 2) == 0 && maxnorm <= 8388608.0))) ? 1 : 0;
     R.ldb = round_up(dim, 16);
-    R.Xb.alloc((size_t)std::max<int64_t>(n, 1) * R.ldb);
-    R.nrm.alloc(std::max<int64_t>(n, 1));
+    R.Xb.ensure((size_t)std::max<int64_t>(n, 1) * R.ldb);
+    R.nrm.ensure(std::max<int64_t>(n, 1));
     const float* mu = nullptr;
     if (!R.exact) {
         DBuf<double> sums(dim);
         HIP_TRY(hipMemsetAsync(sums.p, 0, dim * 8, s));
-        R.mu.alloc(dim);
+        R.mu.ensure(dim);
         hipLaunchKernelGGL(k_col_sums, dim3(1024), dim3(256), 0, s, X, ld, n, dim, sums.p);
         hipLaunchKernelGGL(k_col_mean, dim3(ceil_div(dim, 256)), dim3(256), 0, s, sums.p, n, dim, R.mu.p);
         mu = R.mu.p;
