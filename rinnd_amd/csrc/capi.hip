@@ -101,8 +101,14 @@ struct DataDev {
     DBuf<float> X;
     int64_t ld = 0;
 };
+// Device input whose rows already have the padded stride (dim a multiple of 8, 256-byte aligned base)
+// is used in place (read-only); anything else is copied into a zero-padded buffer.
 static void make_data(hipStream_t s, const float* data, int64_t n, int dim, DataDev& out) {
     out.ld = round_up(std::max(dim, 1), 8);
+    if (n > 0 && dim == out.ld && ((uintptr_t)data & 255) == 0 && is_device_ptr(data)) {
+        out.X.view(const_cast<float*>(data), (size_t)n * dim);
+        return;
+    }
     out.X.alloc((size_t)std::max<int64_t>(n, 1) * out.ld);
     InView<float> in(s, data, (size_t)n * dim);
     launch_pad_rows(s, in.d, n, dim, out.X.p, out.ld);
@@ -130,14 +136,26 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
     const int64_t ls = std::max<int64_t>(leaf_size, 1);
     int64_t cap = std::min<int64_t>(2 * n + 1, 8 * ceil_div(n, ls) + 1024);
     for (int attempt = 0; attempt < 2; ++attempt) {
-        for (auto& T : F.trees) {
-            T.cap_nodes = cap;
-            T.hyper.alloc((size_t)cap * dim);
-            T.offsets.alloc(cap);
-            T.children.alloc((size_t)cap * 2);
-            T.indices.alloc(std::max<int64_t>(n, 1));
-            HIP_TRY(hipMemsetAsync(T.hyper.p, 0, T.hyper.bytes(), s));
-            HIP_TRY(hipMemsetAsync(T.offsets.p, 0, T.offsets.bytes(), s));
+        {  // one pool: [hyper | offsets] of every tree (zeroed together), then children and indices
+            auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+            const size_t b_hyper = al((size_t)cap * dim * 4), b_off = al((size_t)cap * 4);
+            const size_t b_ch = al((size_t)cap * 8), b_ix = al((size_t)std::max<int64_t>(n, 1) * 4);
+            const size_t zeroed = (b_hyper + b_off) * n_trees;
+            auto pool = std::make_shared<DBuf<char>>(zeroed + (b_ch + b_ix) * n_trees);
+            HIP_TRY(hipMemsetAsync(pool->p, 0, zeroed, s));
+            char* z = pool->p;
+            char* r = pool->p + zeroed;
+            for (auto& T : F.trees) {
+                T = TreeDev{};
+                T.cap_nodes = cap;
+                T.pool = pool;
+                T.hyper.view((float*)z, (size_t)cap * dim);
+                T.offsets.view((float*)(z + b_hyper), cap);
+                T.children.view((int32_t*)r, (size_t)cap * 2);
+                T.indices.view((int32_t*)(r + b_ch), std::max<int64_t>(n, 1));
+                z += b_hyper + b_off;
+                r += b_ch + b_ix;
+            }
         }
         if (rng_mode == RINND_RNG_COUNTER) {  // all trees and nodes of a level at once
             std::vector<uint64_t> keys(n_trees);
@@ -212,12 +230,6 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
 }
 
 // ---- NN-Descent ---------------------------------------------------------------------------------
-
-struct HeapDev {
-    DBuf<int32_t> idx;
-    DBuf<float> dist;
-    DBuf<uint8_t> flags;
-};
 
 static uint64_t d2h_u64(hipStream_t s, const void* p) {
     uint64_t v = 0;
@@ -756,15 +768,27 @@ rinnd_status rinnd_nn_descent(rinnd_ctx* ctx, const float* data, int64_t n, int6
         }
         DataDev D;
         make_data(s, data, n, (int)dim, D);
-        HeapDev H;
-        H.idx.alloc((size_t)n * k);
-        H.dist.alloc((size_t)n * k);
-        H.flags.alloc((size_t)n * k);
-        nn_descent_dev(ctx, D.X.p, D.ld, n, (int)dim, (int)dist, *params, main, H.idx.p, H.dist.p, H.flags.p, *F,
-                       stats);
-        copy_out(s, out_idx, H.idx.p, (size_t)n * k);
-        copy_out(s, out_dist, H.dist.p, (size_t)n * k);
-        copy_out(s, out_flags, H.flags.p, (size_t)n * k);
+        // device outputs are written in place; otherwise the heap lives in the context's workspace
+        const size_t cnt = (size_t)n * k;
+        const bool direct = is_device_ptr(out_idx) && is_device_ptr(out_dist) && is_device_ptr(out_flags);
+        Workspace& W = *ctx->ws;
+        WBuf<int32_t> hi;
+        WBuf<float> hd;
+        WBuf<uint8_t> hf;
+        if (!direct) {
+            hi = WBuf<int32_t>(W, kWsHeap + 0, cnt);
+            hd = WBuf<float>(W, kWsHeap + 1, cnt);
+            hf = WBuf<uint8_t>(W, kWsHeap + 2, cnt);
+        }
+        int32_t* h_idx = direct ? out_idx : hi.p;
+        float* h_dist = direct ? out_dist : hd.p;
+        uint8_t* h_flags = direct ? out_flags : hf.p;
+        nn_descent_dev(ctx, D.X.p, D.ld, n, (int)dim, (int)dist, *params, main, h_idx, h_dist, h_flags, *F, stats);
+        if (!direct) {
+            copy_out(s, out_idx, h_idx, cnt);
+            copy_out(s, out_dist, h_dist, cnt);
+            copy_out(s, out_flags, h_flags, cnt);
+        }
         HIP_TRY(hipStreamSynchronize(s));
         for (int w = 0; w < 4; ++w) rng_state[w] = main.s[w];
         if (out_forest) *out_forest = F.release();
@@ -777,18 +801,37 @@ rinnd_status rinnd_sort_rows(rinnd_ctx* ctx, int32_t* idx, float* dist, uint8_t*
         RINND_CHECK_ARG(n >= 0 && k >= 1 && k <= 256, "invalid sizes");
         hipStream_t s = ctx->stream;
         const size_t cnt = (size_t)n * k;
-        DBuf<int32_t> di(cnt);
-        DBuf<float> dd(cnt);
-        DBuf<uint8_t> df(cnt);
-        HIP_TRY(hipMemcpyAsync(di.p, idx, cnt * 4, hipMemcpyDefault, s));
-        HIP_TRY(hipMemcpyAsync(dd.p, dist, cnt * 4, hipMemcpyDefault, s));
-        if (flags) HIP_TRY(hipMemcpyAsync(df.p, flags, cnt, hipMemcpyDefault, s));
-        else HIP_TRY(hipMemsetAsync(df.p, 0, cnt, s));
-        launch_sort_rows(s, di.p, dd.p, df.p, n, (int)k);
+        if (cnt == 0) return;
+        // device rows are sorted in place; host rows are staged through the context's workspace
+        Workspace& W = *ctx->ws;
+        const bool dev_i = is_device_ptr(idx), dev_d = is_device_ptr(dist), dev_f = flags && is_device_ptr(flags);
+        WBuf<int32_t> di;
+        WBuf<float> dd;
+        WBuf<uint8_t> df;
+        int32_t* pi = idx;
+        float* pd = dist;
+        uint8_t* pf = flags;
+        if (!dev_i) {
+            di = WBuf<int32_t>(W, kWsHeap + 0, cnt);
+            HIP_TRY(hipMemcpyAsync(di.p, idx, cnt * 4, hipMemcpyDefault, s));
+            pi = di.p;
+        }
+        if (!dev_d) {
+            dd = WBuf<float>(W, kWsHeap + 1, cnt);
+            HIP_TRY(hipMemcpyAsync(dd.p, dist, cnt * 4, hipMemcpyDefault, s));
+            pd = dd.p;
+        }
+        if (!dev_f) {
+            df = WBuf<uint8_t>(W, kWsHeap + 2, cnt);
+            if (flags) HIP_TRY(hipMemcpyAsync(df.p, flags, cnt, hipMemcpyDefault, s));
+            else HIP_TRY(hipMemsetAsync(df.p, 0, cnt, s));
+            pf = df.p;
+        }
+        launch_sort_rows(s, pi, pd, pf, n, (int)k);
         HIP_TRY(hipGetLastError());
-        copy_out(s, idx, di.p, cnt);
-        copy_out(s, dist, dd.p, cnt);
-        if (flags) copy_out(s, flags, df.p, cnt);
+        if (!dev_i) copy_out(s, idx, pi, cnt);
+        if (!dev_d) copy_out(s, dist, pd, cnt);
+        if (flags && !dev_f) copy_out(s, flags, pf, cnt);
         HIP_TRY(hipStreamSynchronize(s));
     });
 }

@@ -103,7 +103,22 @@ static void index_build_impl(rinnd_ctx* ctx, const float* data, int64_t n, int64
     const int64_t nst = std::min<int64_t>(ip.n_search_trees, (int64_t)forest.trees.size());
     ix.trees.dim = dim;
     ix.trees.n = n;
-    for (int64_t t = 0; t < nst; ++t) ix.trees.trees.push_back(std::move(forest.trees[t]));
+    for (int64_t t = 0; t < nst; ++t) {  // own, exactly sized copies: the forest's shared pool is released
+        const TreeDev& S = forest.trees[t];
+        TreeDev T;
+        T.n_nodes = T.cap_nodes = S.n_nodes;
+        const size_t nn = (size_t)std::max<int64_t>(S.n_nodes, 1);
+        T.hyper.alloc(nn * dim);
+        T.offsets.alloc(nn);
+        T.children.alloc(nn * 2);
+        T.indices.alloc((size_t)std::max<int64_t>(n, 1));
+        HIP_TRY(hipMemcpyAsync(T.hyper.p, S.hyper.p, (size_t)S.n_nodes * dim * 4, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(T.offsets.p, S.offsets.p, (size_t)S.n_nodes * 4, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(T.children.p, S.children.p, (size_t)S.n_nodes * 8, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(T.indices.p, S.indices.p, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        ix.trees.trees.push_back(std::move(T));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
     if (nst > 0) {
         vertex_order_from_tree(s, ix.trees.trees[0].indices.p, n, ix.vertex_order, ix.inv);
     } else {

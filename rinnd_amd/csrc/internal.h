@@ -15,16 +15,18 @@ template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = true;  // false for a view into memory someone else releases
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), owned(o.owned) { o.p = nullptr; o.n = 0; }
     DBuf& operator=(DBuf&& o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             n = o.n;
+            owned = o.owned;
             o.p = nullptr;
             o.n = 0;
         }
@@ -33,6 +35,7 @@ struct DBuf {
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
+        owned = true;
         if (count == 0) return;
         hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
         if (e != hipSuccess) {
@@ -42,12 +45,18 @@ struct DBuf {
         }
         n = count;
     }
+    void view(T* ptr, size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        owned = false;
+    }
     // grow-only reuse
     void ensure(size_t count) {
         if (count > n) alloc(count);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -73,9 +82,12 @@ static_assert(sizeof(SegRec) == 16, "SegRec layout");
 static_assert(sizeof(URec) == 24, "URec layout");
 
 // ---- forest --------------------------------------------------------------------------------------
+// A tree's arrays either own their memory or are views into `pool`, one allocation shared by all
+// trees of a forest (one hipMalloc/hipFree per forest instead of four per tree).
 struct TreeDev {
     int64_t n_nodes = 0;
     int64_t cap_nodes = 0;
+    std::shared_ptr<DBuf<char>> pool;
     DBuf<float> hyper;       // cap_nodes x dim
     DBuf<float> offsets;     // cap_nodes
     DBuf<int32_t> children;  // cap_nodes x 2
@@ -323,8 +335,9 @@ struct Bf16Rows {
     DBuf<uint16_t> Xb;  // n x ldb bf16 (centred unless exact)
     DBuf<float> nrm;    // FP32 squared norms of the rounded rows
     DBuf<float> mu;     // column means used for centring
+    DBuf<double> aux;   // [flag word | max norm | column sums] of the preparation pass
     int64_t ldb = 0;
-    int exact = 0;      // integer data with |x| <= 256 and |x|^2 <= 2^23: screened distances are exact
+    int exact = 0;      // integers with |x| <= 256 and |x|^2 <= 2^22 (2^23 if non-negative): exact screen
 };
 void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R);
 
@@ -429,7 +442,7 @@ struct WBuf {
     WBuf(Workspace& w, int slot, size_t count) : p((T*)w.get(slot, count * sizeof(T))), n(count) {}
     size_t bytes() const { return n * sizeof(T); }
 };
-// slot ranges: build (capi.hip) 0..99, level forest 100..149
-enum : int { kWsForest = 100 };
+// slot ranges: NN-Descent build 0..59, caller-facing heap staging 60..62, level forest 100..149
+enum : int { kWsHeap = 60, kWsForest = 100 };
 
 }  // namespace rinnd

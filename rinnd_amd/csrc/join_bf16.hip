@@ -326,15 +326,16 @@ void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree
 }
 
 void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int dim, Bf16Rows& R) {
-    DBuf<int> flag(1);
-    DBuf<unsigned long long> mx(1);
-    HIP_TRY(hipMemsetAsync(flag.p, 0, 4, s));
-    HIP_TRY(hipMemsetAsync(mx.p, 0, 8, s));
-    hipLaunchKernelGGL(k_exact_check, dim3(1024), dim3(256), 0, s, X, ld, n, dim, flag.p, mx.p);
+    R.aux.ensure(2 + (size_t)dim);
+    int* flag = (int*)R.aux.p;
+    unsigned long long* mx = (unsigned long long*)(R.aux.p + 1);
+    double* sums = R.aux.p + 2;
+    HIP_TRY(hipMemsetAsync(R.aux.p, 0, 16, s));
+    hipLaunchKernelGGL(k_exact_check, dim3(1024), dim3(256), 0, s, X, ld, n, dim, flag, mx);
     int f = 0;
     unsigned long long mb = 0;
-    HIP_TRY(hipMemcpyAsync(&f, flag.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&mb, mx.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&f, flag, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&mb, mx, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     double maxnorm;
     std::memcpy(&maxnorm, &mb, 8);
@@ -345,11 +346,10 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
     R.nrm.ensure(std::max<int64_t>(n, 1));
     const float* mu = nullptr;
     if (!R.exact) {
-        DBuf<double> sums(dim);
-        HIP_TRY(hipMemsetAsync(sums.p, 0, dim * 8, s));
+        HIP_TRY(hipMemsetAsync(sums, 0, dim * 8, s));
         R.mu.ensure(dim);
-        hipLaunchKernelGGL(k_col_sums, dim3(1024), dim3(256), 0, s, X, ld, n, dim, sums.p);
-        hipLaunchKernelGGL(k_col_mean, dim3(ceil_div(dim, 256)), dim3(256), 0, s, sums.p, n, dim, R.mu.p);
+        hipLaunchKernelGGL(k_col_sums, dim3(1024), dim3(256), 0, s, X, ld, n, dim, sums);
+        hipLaunchKernelGGL(k_col_mean, dim3(ceil_div(dim, 256)), dim3(256), 0, s, sums, n, dim, R.mu.p);
         mu = R.mu.p;
         HIP_TRY(hipStreamSynchronize(s));
     }
