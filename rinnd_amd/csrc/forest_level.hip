@@ -27,6 +27,12 @@ struct LvlNode {
     int32_t tree, node, start, end, depth;
 };
 
+// What the side tests of an active entry read: its node id (-1 if it does not split) and offset.
+struct PInfo {
+    int32_t node;
+    float off;
+};
+
 struct LvlArgs {
     const float* X;
     int64_t ld;
@@ -39,6 +45,8 @@ struct LvlArgs {
     float** offsets;       // per tree, cap
     int32_t** children;    // per tree, cap x 2
     uint16_t** hyper_bf;   // per tree, cap x dim bf16 copy read by the side tests
+    const uint16_t* hb0;   // the same copies as one array: tree t's node v at hb0 + (t * cap + v) * dim
+    int64_t cap;
 };
 
 __device__ __forceinline__ float bf16_round(float f) {  // round to nearest even, back to f32
@@ -66,7 +74,7 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 }
 
 __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* nodes, int64_t na,
-                                                    const int32_t* const* perm, uint8_t* split,
+                                                    const int32_t* const* perm, uint8_t* split, PInfo* pinfo,
                                                     unsigned long long* active_pairs) {
     __shared__ float red[4];
     unsigned long long act = 0;  // members of split nodes: the (tree, point) pairs tested this level
@@ -79,6 +87,7 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
                 ch[2 * nd.node] = -nd.start;
                 ch[2 * nd.node + 1] = -nd.end;
                 split[e] = 0;
+                pinfo[e] = PInfo{-1, 0.f};
             }
             continue;  // uniform across the block
         }
@@ -111,7 +120,10 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
                 dm += h * ((p1[i] + p2[i]) * 0.5f);
             }
             dm = block_sum(dm, red);
-            if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = scale ? -dm : 0.f;
+            if (threadIdx.x == 0) {
+                a.offsets[nd.tree][nd.node] = scale ? -dm : 0.f;
+                pinfo[e] = PInfo{nd.node, scale ? -dm : 0.f};
+            }
         } else {
             float n1 = 0.f, n2 = 0.f;
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
@@ -133,7 +145,10 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
                 hp[i] = hr;
                 hb[i] = (uint16_t)(__builtin_bit_cast(uint32_t, hr) >> 16);
             }
-            if (threadIdx.x == 0) a.offsets[nd.tree][nd.node] = 0.f;
+            if (threadIdx.x == 0) {
+                a.offsets[nd.tree][nd.node] = 0.f;
+                pinfo[e] = PInfo{nd.node, 0.f};
+            }
         }
         if (threadIdx.x == 0) split[e] = 1;
         act += (unsigned long long)m;
@@ -247,6 +262,183 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* n
     }
 }
 
+__device__ __forceinline__ uint32_t side_of(float margin, uint64_t key, int32_t node, int32_t pos) {
+    if (fabsf(margin) < 1e-8f)  // ties: a random bit per (tree, node, position)
+        return (uint32_t)(mix64(key ^ mix64(((uint64_t)(uint32_t)node << 32) ^ (uint64_t)(uint32_t)pos)) & 1);
+    return margin < 0.0f ? 1u : 0u;
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+// Lane u's value in every lane of its 16-lane DPP row (row_newbcast; u is a constant once unrolled).
+__device__ __forceinline__ int row_bcast(int v, int u) {
+    switch (u) {
+        case 0: return dpp_i<0x150>(v);
+        case 1: return dpp_i<0x151>(v);
+        case 2: return dpp_i<0x152>(v);
+        case 3: return dpp_i<0x153>(v);
+        case 4: return dpp_i<0x154>(v);
+        case 5: return dpp_i<0x155>(v);
+        case 6: return dpp_i<0x156>(v);
+        case 7: return dpp_i<0x157>(v);
+        case 8: return dpp_i<0x158>(v);
+        case 9: return dpp_i<0x159>(v);
+        case 10: return dpp_i<0x15A>(v);
+        case 11: return dpp_i<0x15B>(v);
+        case 12: return dpp_i<0x15C>(v);
+        case 13: return dpp_i<0x15D>(v);
+        case 14: return dpp_i<0x15E>(v);
+        default: return dpp_i<0x15F>(v);
+    }
+}
+// Sum over the 16 lanes of a DPP row (rotations by 8, 4, 2, 1), in every lane of the row.
+__device__ __forceinline__ float row16_sum(float v) {
+    v += dpp_f<0x128>(v);
+    v += dpp_f<0x124>(v);
+    v += dpp_f<0x122>(v);
+    v += dpp_f<0x121>(v);
+    return v;
+}
+
+// Dense levels, point-major: a 16-lane group (one DPP row) per point holds the point's row in registers
+// (lane l: dims 8 (l + 16 r) .. +8).  Trees go in batches of 16: lane u fetches tree u's nodePos word and
+// PInfo, the group broadcasts each tree's node from its lane (row_newbcast), reads that tree's bf16 plane
+// with one coalesced 16-byte load per lane, and reduces the 16 partial dot products with row rotations;
+// lane u keeps tree u's margin and writes its flag.  Needs dim % 8 == 0 and dim <= 128 R.
+template <int R>
+__global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
+                                                        const int2* __restrict__ nodePos,
+                                                        uint32_t* __restrict__ flags) {
+    const int gl = threadIdx.x & 15;
+    const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 4;
+    const int64_t tstride = a.cap * a.dim;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < a.n; i += groups) {  // group-uniform
+        float4 xr[2 * R];
+        const float* x = a.X + i * a.ld;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int d0 = 8 * (gl + 16 * r);
+            if (d0 < a.dim) {
+                xr[2 * r] = *reinterpret_cast<const float4*>(x + d0);
+                xr[2 * r + 1] = *reinterpret_cast<const float4*>(x + d0 + 4);
+            } else {
+                xr[2 * r] = xr[2 * r + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        for (int tb = 0; tb < n_trees; tb += 16) {
+            const int tl = tb + gl;
+            int32_t nodeL = -1, posL = 0;
+            float offL = 0.f;
+            if (tl < n_trees) {
+                const int2 np = nodePos[(int64_t)tl * a.n + i];
+                if (np.x >= 0) {
+                    const PInfo pi = pinfo[np.x];
+                    nodeL = pi.node;
+                    offL = pi.off;
+                    posL = np.y;
+                }
+            }
+            if (!((__ballot(nodeL >= 0) >> (threadIdx.x & 48)) & 0xffffull)) continue;  // group-uniform
+            float mine = 0.f;
+            constexpr int UB = R >= 4 ? 2 : 8 / R;  // planes in flight per batch (8 for dim <= 128)
+#pragma unroll
+            for (int ub = 0; ub < 16; ub += UB) {  // UB planes in flight, then their dot products
+                uint4 h[UB][R];
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    // trees past n_trees and nodes that do not split read a valid plane whose result is unused
+                    const int32_t nodeU = row_bcast(nodeL, ub + u);
+                    const int tu = tb + ub + u < n_trees ? tb + ub + u : n_trees - 1;
+                    const uint4* hb = reinterpret_cast<const uint4*>(
+                        a.hb0 + (int64_t)tu * tstride + (int64_t)(nodeU > 0 ? nodeU : 0) * a.dim);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int c = gl + 16 * r;
+                        h[u][r] = hb[8 * c < a.dim ? c : 0];  // chunks past dim meet zero row values
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const uint4 hv = h[u][r];
+                        acc = fmaf(xr[2 * r].x, bf16_lo(hv.x), acc);
+                        acc = fmaf(xr[2 * r].y, bf16_hi(hv.x), acc);
+                        acc = fmaf(xr[2 * r].z, bf16_lo(hv.y), acc);
+                        acc = fmaf(xr[2 * r].w, bf16_hi(hv.y), acc);
+                        acc = fmaf(xr[2 * r + 1].x, bf16_lo(hv.z), acc);
+                        acc = fmaf(xr[2 * r + 1].y, bf16_hi(hv.z), acc);
+                        acc = fmaf(xr[2 * r + 1].z, bf16_lo(hv.w), acc);
+                        acc = fmaf(xr[2 * r + 1].w, bf16_hi(hv.w), acc);
+                    }
+                    acc = row16_sum(acc);
+                    if (gl == ub + u) mine = acc;
+                }
+            }
+            if (nodeL >= 0) flags[(int64_t)tl * a.n + posL] = side_of(mine + offL, a.keys[tl], nodeL, posL);
+        }
+    }
+}
+
+// Sparse levels, position-major: a wave reads 64 consecutive (tree, position) items of the act table,
+// and its four 16-lane groups test the active ones four at a time: the member's row is gathered through
+// perm and the 16 lanes split the dimensions.  Needs dim % 4 == 0; nodePos is not used.
+__global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
+                                                         const int32_t* const* act, const int32_t* const* perm,
+                                                         uint32_t* __restrict__ flags) {
+    const int lane = threadIdx.x & 63, gl = lane & 15, g = lane >> 4;
+    const int64_t total = (int64_t)n_trees * a.n;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; w0 < total;
+         w0 += nw * 64) {  // wave-uniform
+        const int64_t item = w0 + lane;
+        int32_t node = -1, t = 0, pos = 0;
+        float off = 0.f;
+        if (item < total) {
+            t = (int32_t)(item / a.n);
+            pos = (int32_t)(item - (int64_t)t * a.n);
+            const int32_t e = act[t][pos];
+            if (e >= 0) {
+                const PInfo pi = pinfo[e];
+                node = pi.node;
+                off = pi.off;
+            }
+        }
+        uint64_t m = __ballot(node >= 0);
+        while (m) {  // wave-uniform
+            uint64_t mg = m;
+            for (int k = 0; k < g && mg; ++k) mg &= mg - 1;  // group g takes the g-th remaining item
+            const int src = mg ? __ffsll((unsigned long long)mg) - 1 : -1;
+            for (int k = 0; k < 4 && m; ++k) m &= m - 1;
+            const int sl = src < 0 ? 0 : src;
+            const int32_t nd = __shfl(node, sl), tt = __shfl(t, sl), ps = __shfl(pos, sl);
+            const float of = __shfl(off, sl);
+            float acc = 0.f;
+            if (src >= 0) {
+                const float* x = a.X + (int64_t)perm[tt][ps] * a.ld;
+                const uint16_t* hb = a.hyper_bf[tt] + (int64_t)nd * a.dim;
+                for (int d0 = 4 * gl; d0 < a.dim; d0 += 64) {
+                    const float4 xv = *reinterpret_cast<const float4*>(x + d0);
+                    const uint2 hv = *reinterpret_cast<const uint2*>(hb + d0);
+                    acc = fmaf(xv.x, bf16_lo(hv.x), acc);
+                    acc = fmaf(xv.y, bf16_hi(hv.x), acc);
+                    acc = fmaf(xv.z, bf16_lo(hv.y), acc);
+                    acc = fmaf(xv.w, bf16_hi(hv.y), acc);
+                }
+            }
+            for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // inside the 16-lane group
+            if (src >= 0 && gl == 0) flags[(int64_t)tt * a.n + ps] = side_of(acc + of, a.keys[tt], nd, ps);
+        }
+    }
+}
+
 __global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const uint32_t* prefix, uint8_t* split,
                              uint32_t* nchild, uint32_t* leftsz, unsigned* tree_children) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
@@ -294,7 +486,7 @@ __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* sp
 __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, const uint8_t* split,
                               const uint32_t* cscan, const uint32_t* leftsz, const uint32_t* prefix,
                               const uint32_t* flags, const int32_t* const* perm, int32_t* const* perm2,
-                              int32_t* const* act, int32_t* const* act2, int2* nodePos) {
+                              int32_t* const* act, int32_t* const* act2, int2* nodePos, int write_np) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
          item += (int64_t)gridDim.x * blockDim.x) {
@@ -311,7 +503,7 @@ __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, cons
             const int32_t pt = perm[t][pos];
             perm2[t][pos] = pt;
             act2[t][pos] = -2;
-            nodePos[(int64_t)t * n + pt] = make_int2(-1, (int32_t)pos);
+            if (write_np) nodePos[(int64_t)t * n + pt] = make_int2(-1, (int32_t)pos);
             continue;
         }
         const LvlNode nd = nodes[e];
@@ -323,7 +515,7 @@ __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, cons
         const int32_t ne = (int32_t)cscan[e] + (left ? 0 : 1);
         perm2[t][np] = pt;
         act2[t][np] = ne;
-        nodePos[(int64_t)t * n + pt] = make_int2(ne, (int32_t)np);
+        if (write_np) nodePos[(int64_t)t * n + pt] = make_int2(ne, (int32_t)np);
     }
 }
 
@@ -337,6 +529,19 @@ __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t
         act[t][pos] = t;  // level 0: the root of tree t is active entry t
         nodePos[item] = make_int2(t, (int32_t)pos);
     }
+}
+
+static void launch_margins_pm(hipStream_t s, const LvlArgs& a, const PInfo* pinfo, int T, const int2* nodePos,
+                              uint32_t* flags) {
+    const unsigned grid = lgrid(a.n * 16, 256, 1 << 16);
+    if (a.dim <= 128)
+        hipLaunchKernelGGL(k_lvl_margins_pm<1>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
+    else if (a.dim <= 256)
+        hipLaunchKernelGGL(k_lvl_margins_pm<2>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
+    else if (a.dim <= 512)
+        hipLaunchKernelGGL(k_lvl_margins_pm<4>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
+    else
+        hipLaunchKernelGGL(k_lvl_margins_pm<8>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
 }
 
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
@@ -387,7 +592,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<uint16_t*> d_hb(W, kWsForest + 10, T);
     HIP_TRY(hipMemcpyAsync(d_hb.p, hb_h.data(), T * sizeof(uint16_t*), hipMemcpyHostToDevice, s));
     const LvlArgs a{X, ld, dim, n, std::max<int64_t>(leaf_size, 1), max_depth, angular,
-                    d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p};
+                    d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p, hyper_bf.p, cap};
 
     WBuf<uint32_t> flags(W, kWsForest + 11, total + 1), prefix(W, kWsForest + 12, total + 1);
     HIP_TRY(hipMemsetAsync(flags.p, 0, (total + 1) * 4, s));  // only split nodes' positions are rewritten
@@ -398,6 +603,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     const int64_t max_nodes = std::max<int64_t>(1, std::min<int64_t>(cap, n)) * (int64_t)T;
     WBuf<LvlNode> curb(W, kWsForest + 14, max_nodes + 1), nxtb(W, kWsForest + 15, max_nodes + 1);
     WBuf<uint8_t> split(W, kWsForest + 16, max_nodes + 1);
+    WBuf<PInfo> pinfo(W, kWsForest + 25, max_nodes + 1);
     WBuf<uint32_t> nchild(W, kWsForest + 17, max_nodes + 1), cscan(W, kWsForest + 18, max_nodes + 1),
         leftsz(W, kWsForest + 19, max_nodes + 1);
     WBuf<unsigned> tree_children(W, kWsForest + 20, T);
@@ -423,13 +629,24 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     std::vector<uint32_t> cum(T);
     int64_t na = T;
     int phase = 0;
+    // Side tests: point-major (every point's row read once for all trees) while most (tree, point) pairs
+    // are tested; position-major (rows gathered per tested pair, no point -> position table) once the
+    // tested pairs of a level fall to kSparsePairs per point, which they never exceed again.
+    constexpr int64_t kSparsePairs = 2;
+    const bool point_major_dpp = dim % 8 == 0 && dim <= 1024;
+    bool sparse = false;
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
         hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur, na,
-                           (const int32_t* const*)P, split.p, active_pairs.p);
+                           (const int32_t* const*)P, split.p, pinfo.p, active_pairs.p);
         margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
-        if (dim % 4 == 0 && dim <= 64)
+        if (sparse)
+            hipLaunchKernelGGL(k_lvl_margins_pos, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, a, pinfo.p, T,
+                               (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags.p);
+        else if (point_major_dpp)
+            launch_margins_pm(s, a, pinfo.p, T, nodePos.p, flags.p);
+        else if (dim % 4 == 0 && dim <= 64)
             hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
                                flags.p);
         else if (dim % 4 == 0 && dim <= 128)
@@ -453,6 +670,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
         HIP_TRY(hipMemcpyAsync(tc.data(), tree_children.p, T * sizeof(unsigned), hipMemcpyDeviceToHost, s));
         unsigned long long pairs_now = 0;
+        bool sparse_next = sparse;
         HIP_TRY(hipMemcpyAsync(&pairs_now, active_pairs.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         {
@@ -462,6 +680,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
             pairs_before = pairs_now;
             margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim * 4.0 +
                                   tested * 12.0);
+            sparse_next = sparse || (dim % 4 == 0 && tested <= (double)(kSparsePairs * n));
         }
         uint32_t acc = 0;
         int64_t next_na = 0;
@@ -481,8 +700,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
                            leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt);
         hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur, T, n, split.p,
                            cscan.p, leftsz.p, prefix.p, flags.p, (const int32_t* const*)P, P + T,
-                           P + 2 * T, P + 3 * T, nodePos.p);
+                           P + 2 * T, P + 3 * T, nodePos.p, sparse_next ? 0 : 1);
         HIP_TRY(hipGetLastError());
+        sparse = sparse_next;
         std::swap(cur, nxt);
         phase ^= 1;
         na = next_na;

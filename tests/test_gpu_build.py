@@ -140,7 +140,8 @@ def check_tree_structure(X, tr, leaf, max_depth=200):
 
 
 @pytest.mark.parametrize("angular", [False, True])
-@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (3000, 36, 30), (20000, 128, 60)])
+@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (3000, 36, 30), (20000, 128, 60), (6000, 200, 40),
+                                      (3000, 600, 50)])
 def test_counter_forest_structure(ctx, n, d, leaf, angular):
     X = uniform(n, d, seed=n + d + 1)
     if angular:
@@ -155,6 +156,15 @@ def test_counter_forest_structure(ctx, n, d, leaf, angular):
         np.testing.assert_array_equal(tr["children"], g["children"])
     if n > 100:  # different trees
         assert not np.array_equal(F.tree(0, n)["indices"], F.tree(1, n)["indices"])
+
+
+@pytest.mark.parametrize("n_trees,d", [(20, 64), (37, 24)])
+def test_counter_forest_many_trees(ctx, n_trees, d):
+    """More trees than one 16-lane batch of the side tests (and a partial last batch)."""
+    X = uniform(5000, d, seed=n_trees)
+    F = gpu_forest_counter(ctx, X, n_trees, 40, 3)
+    for t in (0, 15, 16, n_trees - 1):
+        assert check_tree_structure(X, F.tree(t, 5000), 40) == 0
 
 
 def test_counter_forest_duplicates_and_depth_limit(ctx):
