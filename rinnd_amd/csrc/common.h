@@ -43,6 +43,10 @@ __device__ inline void wave_sync() {
 
 __device__ inline int lane_id() { return threadIdx.x & (RINND_WAVE - 1); }
 
+// operand and accumulator vectors of the 16x16x16 bf16 MFMA
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // Order-preserving float -> u32 map; -0.0 is folded onto +0.0 so that ties compare equal like
 // Rust's partial_cmp (neighbor_heap.rs:308-310).
 __host__ __device__ inline uint32_t float_order_key(float f) {

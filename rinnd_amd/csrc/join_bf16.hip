@@ -28,9 +28,6 @@
 
 namespace rinnd {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     uint32_t u = __builtin_bit_cast(uint32_t, f);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN

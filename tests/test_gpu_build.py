@@ -271,6 +271,40 @@ def test_sort_rows_matches_sort_all(ctx):
     np.testing.assert_array_equal(gf, rf)
 
 
+def _build(ctx, X, k, trees, leaf, C_, iters, rng_mode, precision):
+    n, d = X.shape
+    p = L.make_params(k, trees, leaf, C_, iters, 0.001, n_threads=4, rng_mode=rng_mode, join_precision=precision)
+    st = L.rng_seed(7)
+    idx = np.zeros((n, k), np.int32)
+    dist = np.zeros((n, k), np.float32)
+    flags = np.zeros((n, k), np.uint8)
+    stats = L.BuildStats()
+    h = C.c_void_p()
+    rc = L.lib().rinnd_nn_descent(ctx.h, L.ptr(X), n, d, L.DIST["sqeuclidean"], C.byref(p), L.ptr(st), L.ptr(idx),
+                                  L.ptr(dist), L.ptr(flags), C.byref(h), C.byref(stats))
+    ctx.check(rc, "rinnd_nn_descent")
+    L.lib().rinnd_forest_free(h)
+    return idx, dist, flags, stats.to_dict()
+
+
+@pytest.mark.parametrize("rng_mode", ["reference", "counter"])
+@pytest.mark.parametrize("leaf,k", [(256, 15), (100, 30), (17, 8)])
+def test_bf16_exact_leaf_init_equals_fp32(ctx, rng_mode, leaf, k):
+    """Integer data (exact in bf16): the MFMA leaf distances and the BF16 join give the FP32 path's heaps,
+    flags and change counts bit for bit, for leaves of up to 256 members."""
+    rng = np.random.default_rng(leaf)
+    n, d = 6000, 128
+    centres = rng.random((30, d)) * 100
+    X = np.clip(np.floor(centres[rng.integers(0, 30, n)] + 6 * rng.standard_normal((n, d)) + 0.5), 0, 255)
+    X = X.astype(np.float32)
+    a = _build(ctx, X, k, 4, leaf, k, 4, rng_mode, "fp32")
+    b = _build(ctx, X, k, 4, leaf, k, 4, rng_mode, "bf16")
+    assert b[3]["changes"] == a[3]["changes"]
+    np.testing.assert_array_equal(b[0], a[0])
+    np.testing.assert_array_equal(b[1].view(np.uint32), a[1].view(np.uint32))
+    np.testing.assert_array_equal(b[2], a[2])
+
+
 @pytest.mark.parametrize("kind", ["integer", "float"])
 @pytest.mark.parametrize("C_", [10, 15, 30])
 def test_bf16_join_equals_fp32_path(ctx, kind, C_):
