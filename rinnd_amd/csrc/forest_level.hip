@@ -45,6 +45,7 @@ struct LvlArgs {
     float** offsets;       // per tree, cap
     int32_t** children;    // per tree, cap x 2
     uint16_t** hyper_bf;   // per tree, cap x dim bf16 copy read by the side tests
+    int n_trees;
     const uint16_t* hb0;   // the same copies as one array: tree t's node v at hb0 + (t * cap + v) * dim
     int64_t cap;
 };
@@ -75,8 +76,9 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 
 __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* nodes, int64_t na,
                                                     const int32_t* const* perm, uint8_t* split, PInfo* pinfo,
-                                                    unsigned long long* active_pairs) {
+                                                    unsigned long long* active_pairs, unsigned* tree_children) {
     __shared__ float red[4];
+    if (blockIdx.x == 0 && threadIdx.x < a.n_trees) tree_children[threadIdx.x] = 0;  // counted by k_lvl_counts
     unsigned long long act = 0;  // members of split nodes: the (tree, point) pairs tested this level
     for (int64_t e = blockIdx.x; e < na; e += gridDim.x) {
         const LvlNode nd = nodes[e];
@@ -441,6 +443,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
 
 __global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const uint32_t* prefix, uint8_t* split,
                              uint32_t* nchild, uint32_t* leftsz, unsigned* tree_children) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) nchild[na] = 0;  // the scan's total
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
         const LvlNode nd = nodes[e];
         uint32_t c = 0;
@@ -462,6 +465,20 @@ __global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const 
             }
         }
         nchild[e] = c;
+    }
+}
+
+// Per tree: first child id of the next level (the tree's node count so far) and the exclusive prefix of
+// the children counts over the trees (the active list is grouped by tree).
+__global__ void k_lvl_tree_base(int n_trees, const unsigned* tree_children, int32_t* node_count, int32_t* base,
+                                uint32_t* cum) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (int t = 0; t < n_trees; ++t) {
+        base[t] = node_count[t];
+        cum[t] = acc;
+        acc += tree_children[t];
+        node_count[t] += (int32_t)tree_children[t];
     }
 }
 
@@ -592,7 +609,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<uint16_t*> d_hb(W, kWsForest + 10, T);
     HIP_TRY(hipMemcpyAsync(d_hb.p, hb_h.data(), T * sizeof(uint16_t*), hipMemcpyHostToDevice, s));
     const LvlArgs a{X, ld, dim, n, std::max<int64_t>(leaf_size, 1), max_depth, angular,
-                    d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p, hyper_bf.p, cap};
+                    d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p, T, hyper_bf.p, cap};
 
     WBuf<uint32_t> flags(W, kWsForest + 11, total + 1), prefix(W, kWsForest + 12, total + 1);
     HIP_TRY(hipMemsetAsync(flags.p, 0, (total + 1) * 4, s));  // only split nodes' positions are rewritten
@@ -606,13 +623,15 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<PInfo> pinfo(W, kWsForest + 25, max_nodes + 1);
     WBuf<uint32_t> nchild(W, kWsForest + 17, max_nodes + 1), cscan(W, kWsForest + 18, max_nodes + 1),
         leftsz(W, kWsForest + 19, max_nodes + 1);
-    WBuf<unsigned> tree_children(W, kWsForest + 20, T);
+    // level status read back once per level: [tested pairs so far (u64) | children per tree (u32 x T)]
+    WBuf<unsigned long long> lstat(W, kWsForest + 21, 1 + (T + 1) / 2);
+    unsigned long long* active_pairs = lstat.p;
+    unsigned* tree_children = (unsigned*)(lstat.p + 1);
+    HIP_TRY(hipMemsetAsync(active_pairs, 0, 8, s));
     LvlNode* cur = curb.p;
     LvlNode* nxt = nxtb.p;
-    WBuf<unsigned long long> active_pairs(W, kWsForest + 21, 1);
-    HIP_TRY(hipMemsetAsync(active_pairs.p, 0, 8, s));
     unsigned long long pairs_before = 0;
-    WBuf<int32_t> d_base(W, kWsForest + 22, T);
+    WBuf<int32_t> d_base(W, kWsForest + 22, T), d_count(W, kWsForest + 20, T);
     WBuf<uint32_t> d_cum(W, kWsForest + 23, T);
     size_t cscan_bytes = 0;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, cscan_bytes, nchild.p, cscan.p, (int)(max_nodes + 1)));
@@ -624,9 +643,12 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T,
                        nodePos.p);
     std::vector<int64_t> node_count(T, 1);
-    std::vector<unsigned> tc(T);
-    std::vector<int32_t> base(T);
-    std::vector<uint32_t> cum(T);
+    {
+        const std::vector<int32_t> ones(T, 1);
+        HIP_TRY(hipMemcpyAsync(d_count.p, ones.data(), T * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    std::vector<unsigned long long> lst(1 + (T + 1) / 2);
     int64_t na = T;
     int phase = 0;
     // Side tests: point-major (every point's row read once for all trees) while most (tree, point) pairs
@@ -638,7 +660,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
         hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur, na,
-                           (const int32_t* const*)P, split.p, pinfo.p, active_pairs.p);
+                           (const int32_t* const*)P, split.p, pinfo.p, active_pairs, tree_children);
         margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
         if (sparse)
@@ -659,20 +681,18 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
             hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
                                flags.p);
         margins_timer.stop(s);
-        HIP_TRY(hipMemsetAsync(flags.p + total, 0, 4, s));
-        size_t b = scan_bytes;
+        size_t b = scan_bytes;  // flags[total] stays 0 from the initial clear: the scan's total
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
-        HIP_TRY(hipMemsetAsync(tree_children.p, 0, T * sizeof(unsigned), s));
         hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, prefix.p, split.p,
-                           nchild.p, leftsz.p, tree_children.p);
-        HIP_TRY(hipMemsetAsync(nchild.p + na, 0, 4, s));
+                           nchild.p, leftsz.p, tree_children);
         size_t cb = cscan_bytes;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
-        HIP_TRY(hipMemcpyAsync(tc.data(), tree_children.p, T * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        unsigned long long pairs_now = 0;
-        bool sparse_next = sparse;
-        HIP_TRY(hipMemcpyAsync(&pairs_now, active_pairs.p, 8, hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_lvl_tree_base, dim3(1), dim3(64), 0, s, T, tree_children, d_count.p, d_base.p, d_cum.p);
+        HIP_TRY(hipMemcpyAsync(lst.data(), lstat.p, lst.size() * 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        const unsigned long long pairs_now = lst[0];
+        const unsigned* tc = reinterpret_cast<const unsigned*>(lst.data() + 1);
+        bool sparse_next = sparse;
         {
             // algorithmic bytes: every point row once, each split node's hyperplane once, and per tested
             // (tree, point) its node and position reads and its flag write
@@ -682,20 +702,14 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
                                   tested * 12.0);
             sparse_next = sparse || (dim % 4 == 0 && tested <= (double)(kSparsePairs * n));
         }
-        uint32_t acc = 0;
         int64_t next_na = 0;
         bool over = false;
-        for (int t = 0; t < T; ++t) {
-            base[t] = (int32_t)node_count[t];
-            cum[t] = acc;
-            acc += tc[t];
+        for (int t = 0; t < T; ++t) {  // the host's copy of the node counts k_lvl_tree_base keeps
             node_count[t] += tc[t];
             next_na += tc[t];
             if (node_count[t] > cap) over = true;
         }
         if (over) return false;
-        HIP_TRY(hipMemcpyAsync(d_base.p, base.data(), T * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(d_cum.p, cum.data(), T * 4, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
                            leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt);
         hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur, T, n, split.p,

@@ -221,9 +221,6 @@ void launch_scatter(hipStream_t s, const URec* U, const unsigned long long* U_co
 void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
                        int64_t n_dest, int32_t* idx, float* dist, uint8_t* flags, int k,
                        unsigned long long* changes, int32_t* big_list, int32_t* big_count, uint32_t* bins);
-void launch_cand_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
-                       int64_t n_dest, int32_t* new_idx, int32_t* old_idx, int C,
-                       int32_t* big_list, int32_t* big_count, uint32_t* bins);
 
 // candidates
 void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt);
@@ -234,10 +231,6 @@ struct RngStream {
 };
 void launch_rng_streams(hipStream_t s, const uint64_t* jump_tables, const RngStream* streams_dev,
                         int n_streams, int64_t max_count, float* out);
-void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, int64_t n, int k,
-                     const uint64_t* row_valid_prefix, const float* prio, int mode, int T,
-                     int64_t block_rows, const int64_t* block_prio_base, uint64_t counter_key,
-                     URec* U, unsigned long long* cnt);
 // Candidate sets (CandidateSets::build_from_graph, candidates.rs:51-351) with forward pushes read from
 // the heap row and reverse pushes grouped by one stable radix sort; writes new_idx / old_idx
 // (pre-filled with -1).  Scratch: fpr, rkey, rval, rkey2, rval2 (n*k each), cnt/off (n+1), order (n),
@@ -325,10 +318,6 @@ struct JoinApplyArgs {
 };
 size_t join_sort_tmp_bytes(int64_t items, int end_bit);
 void launch_join_apply(hipStream_t s, const JoinApplyArgs& a);
-// kept pairs -> both destinations' segments (off/cur as in launch_scatter), key = r*C*2C + t
-void launch_join_scatter(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                         int64_t row_end, const JoinOut& out, const uint64_t* off, unsigned long long* cur,
-                         SegRec* seg);
 
 // local join, BF16 rows on MFMA with FP32 re-evaluation (join_bf16.hip)
 struct Bf16Rows {

@@ -1,12 +1,15 @@
 // kernels.hip - gfx950 kernels of the NN-Descent build path.
 //
 // Phases and the reference code they reproduce:
-//   k_forest_dfs        RP trees with the reference's FastRng streams (core/tree/builder.rs:89-348)
-//   k_leaf_pairs        leaf-pair distances of initialize_from_leaves (core/nndescent/mod.rs:203-300)
-//   k_cand_gen/_apply   CandidateSets::build_from_graph (core/nndescent/candidates.rs:51-484)
-//   k_join_fp32         update_iteration's candidate-pair distances + filter (mod.rs:321-448)
-//   k_scatter/_apply    per-vertex heap pushes in the reference's generation order (mod.rs:450-487)
-//   k_sort_rows         NeighborHeap::sort_all (core/heap/neighbor_heap.rs:293-320)
+//   k_forest_dfs          RP trees with the reference's FastRng streams (core/tree/builder.rs:89-348)
+//   k_leaf_pairs          leaf pairs of initialize_from_leaves when the heaps do not fit in LDS
+//                         (core/nndescent/mod.rs:203-300; the LDS form is leaf_init.hip)
+//   k_scatter/k_seg_*     grouping + ordered apply of those leaf pairs (mod.rs:450-487)
+//   k_cand_gen_rev/_push_rev  CandidateSets::build_from_graph (core/nndescent/candidates.rs:51-484)
+//   k_join_fp32           update_iteration's candidate-pair distances + filter (mod.rs:321-448)
+//   k_join_records/k_heap_push_rec  per-vertex heap pushes in the reference's generation order
+//                         (mod.rs:450-487)
+//   k_sort_rows           NeighborHeap::sort_all (core/heap/neighbor_heap.rs:293-320)
 //
 // Ordering argument used by the apply pipeline: the reference applies, for every vertex, its pushes
 // in global generation order (Rayon buckets are concatenated in source-thread order), and its
@@ -679,119 +682,6 @@ struct HeapApply {
     }
 };
 
-// checked_heap_push_flat (candidates.rs:433-484)
-__device__ __forceinline__ void cand_push(float* pri, int32_t* ix, int n, float priority,
-                                          int32_t index) {
-    if (priority >= pri[0]) return;
-    for (int i = 0; i < n; ++i)
-        if (ix[i] == index) return;
-    pri[0] = priority;
-    ix[0] = index;
-    int pos = 0;
-    for (;;) {
-        const int l = 2 * pos + 1, r = 2 * pos + 2;
-        int largest = pos;
-        if (l < n && pri[l] > pri[largest]) largest = l;
-        if (r < n && pri[r] > pri[largest]) largest = r;
-        if (largest == pos) break;
-        pri[pos] = pri[largest];
-        ix[pos] = ix[largest];
-        pri[largest] = priority;
-        ix[largest] = index;
-        pos = largest;
-    }
-}
-
-// checked_heap_push_flat on a register-resident heap (slot j in lane j).
-__device__ __forceinline__ void cand_push_regs(float& pri, int32_t& ix, int n, float priority, int32_t index,
-                                               int lane) {
-    if (priority >= readlane_f(pri, 0)) return;
-    if (__ballot(lane < n && ix == index)) return;
-    int pos = 0;
-    for (;;) {
-        const int l = 2 * pos + 1, r = 2 * pos + 2;
-        int best = pos;
-        float bp = priority;  // the sifted value occupies `pos`
-        if (l < n) {
-            const float pl = readlane_f(pri, l);
-            if (pl > bp) {
-                best = l;
-                bp = pl;
-            }
-        }
-        if (r < n) {
-            const float pr = readlane_f(pri, r);
-            if (pr > bp) {
-                best = r;
-                bp = pr;
-            }
-        }
-        if (best == pos) break;
-        const int32_t ib = __builtin_amdgcn_readlane(ix, best);
-        if (lane == pos) {
-            pri = bp;
-            ix = ib;
-        }
-        pos = best;
-    }
-    if (lane == pos) {
-        pri = priority;
-        ix = index;
-    }
-}
-
-struct CandApply {
-    int32_t* new_idx;
-    int32_t* old_idx;
-    int C;
-    __device__ static bool kRegs(const CandApply& a) { return a.C <= RINND_WAVE; }
-    __device__ void apply_regs(int64_t v, const RecSrc& src, int L, int lane) const {
-        float np = __builtin_inff(), op = __builtin_inff();
-        int32_t ni = -1, oi = -1;
-        for (int r = 0; r < L; ++r) {
-            int32_t x;
-            float pri;
-            src.get(r, x, pri);
-            if (x < 0) cand_push_regs(np, ni, C, pri, x & 0x7fffffff, lane);  // flag bit set: "new" edge
-            else cand_push_regs(op, oi, C, pri, x, lane);
-        }
-        if (lane < C) {
-            new_idx[v * C + lane] = ni;
-            old_idx[v * C + lane] = oi;
-        }
-    }
-    __device__ void apply(int64_t v, const SegRec* recs, int64_t L, char* scratch, int lane) const {
-        if (C <= RINND_WAVE) {
-            apply_regs(v, RecSrc{recs, 0, 0.f, false}, (int)L, lane);
-            return;
-        }
-        int32_t* ni = (int32_t*)scratch;
-        int32_t* oi = ni + C;
-        float* np = (float*)(oi + C);
-        float* op = np + C;
-        for (int j = lane; j < C; j += RINND_WAVE) {
-            ni[j] = -1;
-            oi[j] = -1;
-            np[j] = __builtin_inff();
-            op[j] = __builtin_inff();
-        }
-        wave_sync();
-        if (lane == 0) {
-            for (int64_t r = 0; r < L; ++r) {
-                const SegRec rec = recs[r];
-                const int32_t x = rec.x & 0x7fffffff;
-                if (rec.x < 0) cand_push(np, ni, C, rec.v, x);  // flag bit set: "new" edge
-                else cand_push(op, oi, C, rec.v, x);
-            }
-        }
-        wave_sync();
-        for (int j = lane; j < C; j += RINND_WAVE) {
-            new_idx[v * C + j] = ni[j];
-            old_idx[v * C + j] = oi[j];
-        }
-    }
-};
-
 template <class Ap>
 __global__ __launch_bounds__(kApplyWaves * RINND_WAVE) void k_seg_apply(
     const uint64_t* __restrict__ off, const SegRec* __restrict__ seg, int64_t n_dest, Ap ap,
@@ -1263,52 +1153,6 @@ __device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, i
     ix[pos * B + c] = index;
 }
 
-__global__ __launch_bounds__(128) void k_cand_push_lanes(const int32_t* __restrict__ order, const uint32_t* bins,
-                                                         const uint64_t* __restrict__ off,
-                                                         const SegRec* __restrict__ seg, int32_t* new_idx,
-                                                         int32_t* old_idx, int C) {
-    extern __shared__ __align__(16) float csm[];
-    const int B = blockDim.x, c = threadIdx.x;
-    float* np = csm;
-    int32_t* ni = (int32_t*)(np + B * C);
-    float* op = (float*)(ni + B * C);
-    int32_t* oi = (int32_t*)(op + B * C);
-    const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
-    for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
-        const int64_t e = e0 + c;
-        if (e >= total) continue;
-        const int64_t v = order[e];
-        const uint64_t o0 = off[v];
-        const int64_t L = (int64_t)(off[v + 1] - o0);
-        for (int j = 0; j < C; ++j) {
-            np[j * B + c] = __builtin_inff();
-            op[j * B + c] = __builtin_inff();
-            ni[j * B + c] = -1;
-            oi[j * B + c] = -1;
-        }
-        const SegRec* R = seg + o0;
-        int64_t r = 0;
-        for (; r + 8 <= L; r += 8) {  // eight records in flight per batch
-            SegRec q[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) q[u] = R[r + u];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (q[u].x < 0) cand_push_lane(np, ni, B, c, C, q[u].v, q[u].x & 0x7fffffff);  // flag bit: "new"
-                else cand_push_lane(op, oi, B, c, C, q[u].v, q[u].x);
-            }
-        }
-        for (; r < L; ++r) {
-            const SegRec q = R[r];
-            if (q.x < 0) cand_push_lane(np, ni, B, c, C, q.v, q.x & 0x7fffffff);  // flag bit set: "new" edge
-            else cand_push_lane(op, oi, B, c, C, q.v, q.x);
-        }
-        for (int j = 0; j < C; ++j) {
-            new_idx[v * C + j] = ni[j * B + c];
-            old_idx[v * C + j] = oi[j * B + c];
-        }
-    }
-}
 
 static void sort_and_bin(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2, int64_t n_dest,
                          int32_t* big_list, int32_t* big_count, uint32_t* bins) {
@@ -1343,18 +1187,6 @@ void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* 
     const size_t lds = (size_t)round_up((int64_t)B * k * 2 + (B * k + 3) / 4, 4) * 4;
     hipLaunchKernelGGL(k_heap_push_lanes, dim3(grid_for(n_dest, B, 8192)), dim3(B), lds, s, big_list, bins, off, seg,
                        idx, dist, flags, k, changes);
-}
-void launch_cand_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
-                       int64_t n_dest, int32_t* new_idx, int32_t* old_idx, int C,
-                       int32_t* big_list, int32_t* big_count, uint32_t* bins) {
-    const int B = lanes_block(C, 16, 128);
-    if (B == 0) {
-        run_apply(s, off, seg, seg2, n_dest, CandApply{new_idx, old_idx, C}, big_list, big_count);
-        return;
-    }
-    sort_and_bin(s, off, seg, seg2, n_dest, big_list, big_count, bins);
-    hipLaunchKernelGGL(k_cand_push_lanes, dim3(grid_for(n_dest, B, 8192)), dim3(B), (size_t)B * C * 16, s, big_list,
-                       bins, off, seg, new_idx, old_idx, C);
 }
 
 // ---- candidates -----------------------------------------------------------------------------------
@@ -1397,61 +1229,6 @@ void launch_rng_streams(hipStream_t s, const uint64_t* jump_tables, const RngStr
                        jump_tables, streams_dev, out);
 }
 
-// One thread per heap row: forward record (to the row) and reverse record (to the neighbour) per
-// valid edge, with the reference's priorities and per-destination push order encoded in `key`:
-//   mode 0 (one stream, candidates.rs:84-131): draws 2r, 2r+1 of the main stream, key = edge
-//   mode 1 (T blocks, candidates.rs:170-340): draw r of block stream, key = (rotated block, edge)
-//   mode 2 (counter draws): priorities from mix64(counter_key, edge, direction), key = edge
-__global__ void k_cand_gen(const int32_t* __restrict__ idx, const uint8_t* __restrict__ flags,
-                           int64_t n, int k, const uint64_t* __restrict__ rvp,
-                           const float* __restrict__ prio, int mode, int T, int64_t block_rows,
-                           const int64_t* __restrict__ block_prio_base, uint64_t counter_key,
-                           URec* U, unsigned long long* cnt) {
-    const uint64_t nk = (uint64_t)n * k;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t r = rvp[i];
-        const int64_t blk = mode == 1 ? i / block_rows : 0;
-        const uint64_t blk_first = mode == 1 ? rvp[blk * block_rows] : 0;
-        for (int j = 0; j < k; ++j) {
-            const uint64_t e = (uint64_t)i * k + j;
-            const int32_t nb = idx[e];
-            if (nb < 0) {
-                U[2 * e].a = -1;
-                U[2 * e + 1].a = -1;
-                continue;
-            }
-            const int32_t fbit = flags[e] ? (int32_t)0x80000000u : 0;
-            float pf, pr;
-            uint64_t kf = e, kr = e;
-            if (mode == 0) {
-                pf = prio[2 * r];
-                pr = prio[2 * r + 1];
-            } else if (mode == 1) {
-                pf = pr = prio[block_prio_base[blk] + (int64_t)(r - blk_first)];
-                const int64_t db = nb / block_rows;
-                kr = (db == blk ? 0 : (uint64_t)(blk + 1)) * nk + e;
-            } else {
-                const uint64_t h1 = mix64(counter_key ^ (2 * e));
-                const uint64_t h2 = mix64(counter_key ^ (2 * e + 1));
-                pf = (float)(uint32_t)(h1 >> 40) * (1.0f / 16777216.0f);
-                pr = (float)(uint32_t)(h2 >> 40) * (1.0f / 16777216.0f);
-            }
-            U[2 * e] = URec{kf, (int32_t)i, nb | fbit, pf, 0};
-            U[2 * e + 1] = URec{kr, nb, (int32_t)i | fbit, pr, 0};
-            atomicAdd(&cnt[i], 1ull);
-            atomicAdd(&cnt[nb], 1ull);
-            ++r;
-        }
-    }
-}
-void launch_cand_gen(hipStream_t s, const int32_t* idx, const uint8_t* flags, int64_t n, int k,
-                     const uint64_t* row_valid_prefix, const float* prio, int mode, int T,
-                     int64_t block_rows, const int64_t* block_prio_base, uint64_t counter_key,
-                     URec* U, unsigned long long* cnt) {
-    hipLaunchKernelGGL(k_cand_gen, dim3(grid_for(n, 128)), dim3(128), 0, s, idx, flags, n, k,
-                       row_valid_prefix, prio, mode, T, block_rows, block_prio_base, counter_key, U, cnt);
-}
 
 // ---- candidate build with a stable destination sort -----------------------------------------------
 // A vertex v receives, in the reference's order, its own heap row's edges (forward pushes, keys
@@ -1809,59 +1586,6 @@ void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int
                            old_idx, C, row_begin, row_end, heap_dist, k, out);
 }
 
-// Kept pairs of each row -> the segments of both endpoints.  One wave per row: per-candidate record
-// counts in LDS, one cursor atomic per candidate, then slots within the candidate's block.
-__global__ __launch_bounds__(256) void k_join_scatter(const int32_t* __restrict__ new_idx,
-                                                      const int32_t* __restrict__ old_idx, int C,
-                                                      int64_t row_begin, int64_t row_end, JoinOut out,
-                                                      const uint64_t* __restrict__ off, unsigned long long* cur,
-                                                      SegRec* seg) {
-    __shared__ int32_t ids[4][128];
-    __shared__ unsigned lc[4][128];
-    __shared__ uint64_t base[4][128];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int C2 = 2 * C;
-    for (int64_t i = row_begin + (int64_t)blockIdx.x * 4 + w; i < row_end; i += (int64_t)gridDim.x * 4) {
-        const int64_t r = i - row_begin;
-        for (int c = lane; c < C2; c += 64) {
-            ids[w][c] = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
-            lc[w][c] = 0;
-        }
-        wave_sync();
-        const uint32_t nk = (uint32_t)out.kc[r];
-        const JRec* E = out.U + out.rowoff[r];
-        for (uint32_t e = lane; e < nk; e += 64) {
-            const uint32_t t = E[e].t;
-            const uint32_t j = t / (uint32_t)C2, kk = t - j * (uint32_t)C2;
-            atomicAdd(&lc[w][j], 1u);
-            atomicAdd(&lc[w][kk], 1u);
-        }
-        wave_sync();
-        for (int c = lane; c < C2; c += 64) {
-            const unsigned v = lc[w][c];
-            base[w][c] = v ? off[ids[w][c]] + atomicAdd(&cur[ids[w][c]], (unsigned long long)v) : 0;
-            lc[w][c] = 0;
-        }
-        wave_sync();
-        const uint64_t rowkey = (uint64_t)r * (uint64_t)(C * C2);
-        for (uint32_t e = lane; e < nk; e += 64) {
-            const JRec q = E[e];
-            const uint32_t j = q.t / (uint32_t)C2, kk = q.t - j * (uint32_t)C2;
-            const unsigned sj = atomicAdd(&lc[w][j], 1u);
-            seg[base[w][j] + sj] = SegRec{rowkey + q.t, ids[w][kk], q.d};
-            const unsigned sk = atomicAdd(&lc[w][kk], 1u);
-            seg[base[w][kk] + sk] = SegRec{rowkey + q.t, ids[w][j], q.d};
-        }
-        wave_sync();
-    }
-}
-void launch_join_scatter(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                         int64_t row_end, const JoinOut& out, const uint64_t* off, unsigned long long* cur,
-                         SegRec* seg) {
-    if (row_end <= row_begin) return;
-    hipLaunchKernelGGL(k_join_scatter, dim3(grid_for(ceil_div(row_end - row_begin, (int64_t)4), 1, 1 << 18)),
-                       dim3(256), 0, s, new_idx, old_idx, C, row_begin, row_end, out, off, cur, seg);
-}
 
 // ---- point relabelling (counter draws) --------------------------------------------------------------
 __global__ void k_relabel_rows(const float* __restrict__ X, int64_t ld, int64_t n, const int32_t* __restrict__ perm,
