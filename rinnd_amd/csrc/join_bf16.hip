@@ -262,6 +262,7 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
     const int w = t >> 6, lane = t & 63, nw = B >> 6;
     const int lds_ld = (int)ldb + 8;  // bf16 elements per staged row (16-byte aligned, skewed banks)
     float* nr = (float*)(bsm16 + (int64_t)round_up((int64_t)mcap, (int64_t)16) * lds_ld);
+    int32_t* rid = (int32_t*)(nr + round_up((int64_t)mcap, (int64_t)16));
     const int64_t n_leaves = *n_leaves_dev;
     const int mrow = lane & 15, kq = (lane >> 4) * 4;
     unsigned long long np = 0;
@@ -272,13 +273,33 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
         np += (unsigned long long)m * (m - 1);
         const int mp = round_up(m, 16);
         const int v8 = (int)(ldb >> 3);  // 16-byte chunks per row
-        for (int i = t; i < mp * v8; i += B) {
-            const int r = i / v8, e = i - r * v8;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (r < m) v = *reinterpret_cast<const uint4*>(Xb + (int64_t)tidx[s + r] * ldb + e * 8);
-            *reinterpret_cast<uint4*>(bsm16 + r * lds_ld + e * 8) = v;
+        // member ids and norms first, so that the row loads below do not wait on an id load each
+        for (int i = t; i < mp; i += B) {
+            const int32_t id = i < m ? tidx[s + i] : -1;
+            rid[i] = id;
+            nr[i] = id >= 0 ? nrm[id] : 0.f;
         }
-        for (int i = t; i < mp; i += B) nr[i] = i < m ? nrm[tidx[s + i]] : 0.f;
+        __syncthreads();
+        for (int i0 = 0; i0 < mp * v8; i0 += 8 * B) {  // eight 16-byte row loads in flight per thread
+            uint4 v[8];
+            bool pad[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = min(i0 + u * B + t, mp * v8 - 1);
+                const int r = i / v8, e = i - r * v8;
+                const int32_t id = rid[r];
+                pad[u] = id < 0;  // rows m..mp-1 are zero
+                v[u] = *reinterpret_cast<const uint4*>(Xb + (int64_t)(id < 0 ? 0 : id) * ldb + e * 8);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + u * B + t;
+                if (i < mp * v8) {
+                    const int r = i / v8, e = i - r * v8;
+                    *reinterpret_cast<uint4*>(bsm16 + r * lds_ld + e * 8) = pad[u] ? make_uint4(0, 0, 0, 0) : v[u];
+                }
+            }
+        }
         __syncthreads();
         const int nb = mp >> 4;
         const int T = nb * (nb + 1) / 2;
@@ -311,7 +332,7 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
 }
 
 size_t leaf_dist_mfma_lds(int64_t ldb, int mcap) {
-    return (size_t)round_up((int64_t)mcap, (int64_t)16) * (ldb + 8) * 2 + (size_t)round_up((int64_t)mcap, (int64_t)16) * 4;
+    return (size_t)round_up((int64_t)mcap, (int64_t)16) * (ldb + 8) * 2 + (size_t)round_up((int64_t)mcap, (int64_t)16) * 8;
 }
 
 void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,

@@ -115,6 +115,39 @@ __global__ __launch_bounds__(256) void k_leaf_init(int metric, const float* __re
 // to the members of its leaf; k_leaf_push gives each position one lane that applies its pushes in
 // leaf order with its heap in an LDS column and the root in a register, so rejected pushes cost
 // one compare.
+// Stages rows tidx[s..s+m) of X (stride ld) into LDS rows of stride ldl, rows m..mp-1 zero.  The member ids
+// go to rid first, so the row loads (eight float4 in flight per thread) do not each wait on an id load.
+// Ends with a barrier.
+__device__ __forceinline__ void stage_leaf_rows(const float* __restrict__ X, int64_t ld, const int32_t* __restrict__ tidx,
+                                                int64_t s, int m, int mp, float* rsm, int64_t ldl, int32_t* rid) {
+    const int B = blockDim.x, t = threadIdx.x;
+    for (int i = t; i < mp; i += B) rid[i] = i < m ? tidx[s + i] : -1;
+    __syncthreads();
+    const int vec = (int)(ld >> 2);
+    const int total = mp * vec;
+    for (int i0 = 0; i0 < total; i0 += 8 * B) {
+        float4 v[8];
+        bool pad[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = min(i0 + u * B + t, total - 1);
+            const int c = i / vec, e4 = i - c * vec;
+            const int32_t id = rid[c];
+            pad[u] = id < 0;
+            v[u] = *reinterpret_cast<const float4*>(X + (int64_t)(id < 0 ? 0 : id) * ld + e4 * 4);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * B + t;
+            if (i < total) {
+                const int c = i / vec, e4 = i - c * vec;
+                *reinterpret_cast<float4*>(rsm + c * ldl + e4 * 4) = pad[u] ? make_float4(0.f, 0.f, 0.f, 0.f) : v[u];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ void tri_pair(int q, int m, int& a, int& o) {
     // q-th pair (a < o) in row-major order: counted back from the last row, r = k(k+1)/2 + rest
     const int r = m * (m - 1) / 2 - 1 - q;
@@ -139,13 +172,7 @@ __global__ __launch_bounds__(256) void k_leaf_dist(int metric, const float* __re
         const int m = (int)(leaves[2 * leaf + 1] - s);
         if (m > mcap || m < 2) continue;  // uniform; larger leaves go to the untiled kernel
         np += (unsigned long long)m * (m - 1);
-        const int vec = (int)(ld >> 2);
-        for (int i = t; i < m * vec; i += B) {
-            const int c = i / vec, e4 = i - c * vec;
-            *reinterpret_cast<float4*>(rsm + c * ldl + e4 * 4) =
-                *reinterpret_cast<const float4*>(X + (int64_t)tidx[s + c] * ld + e4 * 4);
-        }
-        __syncthreads();
+        stage_leaf_rows(X, ld, tidx, s, m, m, rsm, ldl, (int32_t*)(rsm + (int64_t)round_up((int64_t)mcap, (int64_t)4) * ldl));
         // every metric here is bitwise symmetric, so one evaluation serves both members
         const int T = m * (m - 1) / 2;
         for (int q = t; q < T; q += B) {
@@ -179,15 +206,8 @@ __global__ __launch_bounds__(256) void k_leaf_dist_sq(const float* __restrict__ 
         const int m = (int)(leaves[2 * leaf + 1] - s);
         if (m > mcap || m < 2) continue;  // uniform; larger leaves go to the untiled kernel
         np += (unsigned long long)m * (m - 1);
-        const int vec = (int)(ld >> 2);
-        const int mp = round_up(m, 4);
-        for (int i = t; i < mp * vec; i += B) {  // rows m..mp-1 are zero padding for the edge tiles
-            const int c = i / vec, e4 = i - c * vec;
-            *reinterpret_cast<float4*>(rsm + c * ldl + e4 * 4) =
-                c < m ? *reinterpret_cast<const float4*>(X + (int64_t)tidx[s + c] * ld + e4 * 4)
-                      : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        __syncthreads();
+        const int mp = round_up(m, 4);  // rows m..mp-1 are zero padding for the edge tiles
+        stage_leaf_rows(X, ld, tidx, s, m, mp, rsm, ldl, (int32_t*)(rsm + (int64_t)round_up((int64_t)mcap, (int64_t)4) * ldl));
         const int nb = mp >> 2;
         const int T = nb * (nb + 1) / 2;
         for (int q = t; q < T; q += B) {
@@ -352,7 +372,7 @@ struct TilePlan {
 static TilePlan tile_plan(int64_t ld, int64_t leaf_size) {
     TilePlan P;
     const int mcap = (int)std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 256);
-    const size_t rows = (size_t)round_up((int64_t)mcap, (int64_t)4) * (ld + 4) * 4;
+    const size_t rows = (size_t)round_up((int64_t)mcap, (int64_t)4) * (ld + 4 + 1) * 4;  // rows + member ids
     if (rows > 150 * 1024) return P;
     P.mcap = mcap;
     P.Dl4 = round_up((int64_t)mcap, (int64_t)4);
