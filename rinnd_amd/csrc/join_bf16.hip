@@ -121,7 +121,7 @@ struct JoinBF16Args {
     unsigned long long* rescored;
 };
 
-template <int TA, int TB>
+template <int TA, int TB, bool EXACT>
 __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
     constexpr int W = 4;
     __shared__ int32_t ids[W][TB * 16];
@@ -148,36 +148,50 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
         const int64_t r0 = i - a.row_begin;
         JRec* U = a.out.U + a.out.rowoff[r0];
         unsigned run = 0;
+        // operand rows: absent candidates read row 0 and are zeroed after the load, so every load of a batch
+        // is unconditional and KB k-steps of fragments are in flight together
         const uint16_t* arow[TA];
         const uint16_t* brow[TB];
+        bool aval[TA], bval[TB];
 #pragma unroll
         for (int t = 0; t < TA; ++t) {
             const int32_t id = (16 * t + m < C) ? ids[w][16 * t + m] : -1;
-            arow[t] = id >= 0 ? a.Xb + (int64_t)id * a.ldb + kq : nullptr;
+            aval[t] = id >= 0;
+            arow[t] = a.Xb + (int64_t)(id >= 0 ? id : 0) * a.ldb + kq;
         }
 #pragma unroll
         for (int t = 0; t < TB; ++t) {
             const int32_t id = ids[w][16 * t + m];
-            brow[t] = id >= 0 ? a.Xb + (int64_t)id * a.ldb + kq : nullptr;
+            bval[t] = id >= 0;
+            brow[t] = a.Xb + (int64_t)(id >= 0 ? id : 0) * a.ldb + kq;
         }
         v4f acc[TA][TB];
 #pragma unroll
         for (int x = 0; x < TA; ++x)
 #pragma unroll
             for (int y = 0; y < TB; ++y) acc[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
-        for (int64_t k0 = 0; k0 < a.ldb; k0 += 16) {
-            v4s A[TA], B[TB];
+        constexpr int KB = 2;
+        const v4s z4 = {0, 0, 0, 0};
+        for (int64_t k0 = 0; k0 < a.ldb; k0 += 16 * KB) {
+            v4s A[KB][TA], B[KB][TB];
 #pragma unroll
-            for (int t = 0; t < TA; ++t)
-                A[t] = arow[t] ? *reinterpret_cast<const v4s*>(arow[t] + k0) : v4s{0, 0, 0, 0};
+            for (int kb = 0; kb < KB; ++kb) {
+                const int64_t ko = min(k0 + 16 * kb, a.ldb - 16);
 #pragma unroll
-            for (int t = 0; t < TB; ++t)
-                B[t] = brow[t] ? *reinterpret_cast<const v4s*>(brow[t] + k0) : v4s{0, 0, 0, 0};
+                for (int t = 0; t < TA; ++t) A[kb][t] = *reinterpret_cast<const v4s*>(arow[t] + ko);
 #pragma unroll
-            for (int x = 0; x < TA; ++x)
+                for (int t = 0; t < TB; ++t) B[kb][t] = *reinterpret_cast<const v4s*>(brow[t] + ko);
+            }
 #pragma unroll
-                for (int y = 0; y < TB; ++y)
-                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[x], B[y], acc[x][y], 0, 0, 0);
+            for (int kb = 0; kb < KB; ++kb) {
+                const bool live = k0 + 16 * kb < a.ldb;  // uniform: steps past ldb add zeros
+#pragma unroll
+                for (int x = 0; x < TA; ++x)
+#pragma unroll
+                    for (int y = 0; y < TB; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            aval[x] && live ? A[kb][x] : z4, bval[y] ? B[kb][y] : z4, acc[x][y], 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int x = 0; x < TA; ++x)
@@ -197,7 +211,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                         if (valid) {
                             const float thr = fmaxf(roots[w][j], roots[w][kk]);
                             const float dt = (nr[w][j] + nr[w][kk]) - 2.0f * acc[x][y][r];
-                            if (a.exact) {
+                            if (EXACT) {  // no re-evaluation code in this instance: fewer registers
                                 d = dt;
                                 keep = d <= thr;
                             } else {
@@ -376,6 +390,15 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
     HIP_TRY(hipGetLastError());
 }
 
+template <bool EXACT>
+static void launch_join_bf16_tiles(hipStream_t s, unsigned grid, int C, const JoinBF16Args& a) {
+    if (C <= 8) hipLaunchKernelGGL((k_join_bf16<1, 1, EXACT>), dim3(grid), dim3(256), 0, s, a);
+    else if (C <= 16) hipLaunchKernelGGL((k_join_bf16<1, 2, EXACT>), dim3(grid), dim3(256), 0, s, a);
+    else if (C <= 32) hipLaunchKernelGGL((k_join_bf16<2, 4, EXACT>), dim3(grid), dim3(256), 0, s, a);
+    else if (C <= 48) hipLaunchKernelGGL((k_join_bf16<3, 6, EXACT>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_join_bf16<4, 8, EXACT>), dim3(grid), dim3(256), 0, s, a);
+}
+
 void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const Bf16Rows& R,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
                       int64_t row_end, const float* heap_dist, int k, const JoinOut& out,
@@ -402,11 +425,8 @@ void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const 
     a.out = out;
     a.rescored = rescored;
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(row_end - row_begin, 4), 1 << 18);
-    if (C <= 8) hipLaunchKernelGGL((k_join_bf16<1, 1>), dim3(grid), dim3(256), 0, s, a);
-    else if (C <= 16) hipLaunchKernelGGL((k_join_bf16<1, 2>), dim3(grid), dim3(256), 0, s, a);
-    else if (C <= 32) hipLaunchKernelGGL((k_join_bf16<2, 4>), dim3(grid), dim3(256), 0, s, a);
-    else if (C <= 48) hipLaunchKernelGGL((k_join_bf16<3, 6>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_join_bf16<4, 8>), dim3(grid), dim3(256), 0, s, a);
+    if (a.exact) launch_join_bf16_tiles<true>(s, grid, C, a);
+    else launch_join_bf16_tiles<false>(s, grid, C, a);
 }
 
 }  // namespace rinnd
