@@ -1068,8 +1068,8 @@ __global__ __launch_bounds__(256) void k_heap_push_lanes(const int32_t* __restri
     const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
     unsigned long long ch = 0;
     for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
-        const int64_t e = e0 + c;
-        if (e < total) {
+        const int64_t e = total - 1 - (e0 + c);  // longest bins first: they start early instead of trailing
+        if (e >= 0) {
             const int64_t v = order[e];
             const uint64_t o0 = off[v];
             const int64_t L = (int64_t)(off[v + 1] - o0);
@@ -1297,8 +1297,8 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
     int32_t* oi = (int32_t*)(op + B * C);
     const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
     for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
-        const int64_t e = e0 + c;
-        if (e >= total) continue;
+        const int64_t e = total - 1 - (e0 + c);  // longest bins first: they start early instead of trailing
+        if (e < 0) continue;
         const int64_t v = order[e];
         const uint64_t o0 = off[v], o1 = off[v + 1];
         for (int j = 0; j < C; ++j) {
@@ -1702,8 +1702,8 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
     const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
     unsigned long long ch = 0;
     for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
-        const int64_t e = e0 + c;
-        if (e < total) {
+        const int64_t e = total - 1 - (e0 + c);  // longest bins first: they start early instead of trailing
+        if (e >= 0) {
             const int64_t v = order[e];
             const uint64_t o0 = off[v];
             const int64_t L = (int64_t)(off[v + 1] - o0);
