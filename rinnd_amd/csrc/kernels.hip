@@ -1311,9 +1311,15 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
             if ((int32_t)x < 0) cand_push_lane(np, ni, B, c, C, pri, (int32_t)(x & 0x7fffffffu));  // "new" edge
             else cand_push_lane(op, oi, B, c, C, pri, (int32_t)x);
         };
-        auto rev = [&](uint64_t a, uint64_t b) {
+        auto rev = [&](uint64_t a, uint64_t b) {  // a ring of four records in flight ahead of the push
+            if (a >= b) return;
+            uint64_t q0 = rv[a], q1 = rv[min(a + 1, b - 1)], q2 = rv[min(a + 2, b - 1)], q3 = rv[min(a + 3, b - 1)];
             for (uint64_t r = a; r < b; ++r) {
-                const uint64_t w = rv[r];
+                const uint64_t w = q0;
+                q0 = q1;
+                q1 = q2;
+                q2 = q3;
+                q3 = rv[min(r + 4, b - 1)];
                 push((uint32_t)w, __builtin_bit_cast(float, (uint32_t)(w >> 32)));
             }
         };
