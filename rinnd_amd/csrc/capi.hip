@@ -247,6 +247,8 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     const double t_start = wall_ms();
     hipEvent_t* ev = ctx->ev;
     rinnd_build_stats st{};
+    LaunchLog tlog;
+    tlog.pool = &ctx->ev_pool;
 
     HIP_TRY(hipEventRecord(ev[0], s));
     build_forest_dev(ctx, DX, Dld, n, dim, p.n_trees, p.leaf_size, p.angular, p.max_depth, p.rng_mode, main, F,
@@ -304,13 +306,19 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         WBuf<unsigned long long> lpairs(W, 13, 1);
         HIP_TRY(hipMemsetAsync(lpairs.p, 0, 8, s));
         const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, use_bf16 && B16.exact ? &B16 : nullptr};
-        const KTimer dist_timer{ev[8], ev[9], &st.timed[RINND_TIMED_LEAF_DIST]};
-        const KTimer push_timer{ev[10], ev[11], &st.timed[RINND_TIMED_LEAF_PUSH]};
+        // deferred timings: no wait per tree
+        const KTimer dist_timer{nullptr, nullptr, stats ? &st.timed[RINND_TIMED_LEAF_DIST] : nullptr, &tlog};
+        const KTimer push_timer{nullptr, nullptr, stats ? &st.timed[RINND_TIMED_LEAF_PUSH] : nullptr, &tlog};
         for (size_t t = 0; t < F.trees.size(); ++t) {
             launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
                              leaves.p, ltmp.p, lt_bytes);
             launch_leaf_init(s, metric, DXw, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
                              w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer);
+        }
+        if (stats) {  // per-pair bytes: D written (4 B per ordered pair), D and the partner id read (8 B)
+            const double pairs = (double)d2h_u64(s, lpairs.p);
+            st.timed[RINND_TIMED_LEAF_DIST].bytes += pairs * 4.0;
+            st.timed[RINND_TIMED_LEAF_PUSH].bytes += pairs * 8.0;
         }
         PL.init(n, 1);  // the update buffer is only used by the grouped leaf path below
     } else {
@@ -559,6 +567,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         for (auto& T : F.trees) launch_map_ids(s, T.indices.p, n, perm0.p);
         HIP_TRY(hipStreamSynchronize(s));  // the relabelled buffers die here
     }
+    tlog.flush();
     st.iterations = it;
     st.converged = converged ? 1 : 0;
     st.ms_forest = elapsed_ms(ev[0], ev[1]);
@@ -668,6 +677,7 @@ void rinnd_ctx_destroy(rinnd_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->ev_pool) (void)hipEventDestroy(e);
     ctx->jump_tables.release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx->ws;  // frees the build scratch with the device current

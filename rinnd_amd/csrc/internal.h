@@ -112,6 +112,7 @@ struct rinnd_ctx {
     std::mutex mu;
     rinnd::DBuf<uint64_t> jump_tables;  // 64 x 256 x 4 words
     hipEvent_t ev[12] = {};
+    std::vector<hipEvent_t> ev_pool;    // events of deferred launch timings (released by rinnd_ctx_destroy)
     rinnd::Workspace* ws = nullptr;     // build scratch kept between calls (released by rinnd_ctx_destroy)
 };
 
@@ -119,24 +120,67 @@ namespace rinnd {
 
 // Times one kernel launch with a pair of events on its stream and adds it to a rinnd_kernel_timing
 // (a null target makes it a no-op).  end() waits for the launch.
+// Launches whose durations are read once, after the build has synchronised anyway: event pairs come from a
+// context-owned pool and flush() adds every recorded launch to its rinnd_kernel_timing.
+struct LaunchLog {
+    std::vector<hipEvent_t>* pool = nullptr;
+    size_t used = 0;
+    struct Rec {
+        hipEvent_t a, b;
+        rinnd_kernel_timing* out;
+    };
+    std::vector<Rec> recs;
+    hipEvent_t event() {
+        if (used == pool->size()) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreate(&e));
+            pool->push_back(e);
+        }
+        return (*pool)[used++];
+    }
+    void flush() {
+        for (const Rec& r : recs) {
+            HIP_TRY(hipEventSynchronize(r.b));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+            r.out->ms += ms;
+            r.out->launches += 1;
+        }
+        recs.clear();
+        used = 0;
+    }
+};
+
 struct KTimer {
     hipEvent_t a = nullptr, b = nullptr;
     rinnd_kernel_timing* out = nullptr;
+    LaunchLog* log = nullptr;  // set: no wait per launch, durations are read at log->flush()
+    mutable hipEvent_t la = nullptr;
     void begin(hipStream_t s) const {
-        if (out) HIP_TRY(hipEventRecord(a, s));
+        if (!out) return;
+        if (log) la = log->event();
+        HIP_TRY(hipEventRecord(log ? la : a, s));
     }
     void stop(hipStream_t s) const {
-        if (out) HIP_TRY(hipEventRecord(b, s));
+        if (!out) return;
+        if (log) {
+            hipEvent_t lb = log->event();
+            HIP_TRY(hipEventRecord(lb, s));
+            log->recs.push_back(LaunchLog::Rec{la, lb, out});
+        } else {
+            HIP_TRY(hipEventRecord(b, s));
+        }
     }
-    // waits for the launch; bytes are the launch's algorithmic bytes
+    // bytes are the launch's algorithmic bytes; without a log this waits for the launch
     void collect(double bytes) const {
         if (!out) return;
+        out->bytes += bytes;
+        if (log) return;
         HIP_TRY(hipEventSynchronize(b));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, a, b));
         out->ms += ms;
         out->launches += 1;
-        out->bytes += bytes;
     }
 };
 

@@ -410,14 +410,10 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
     int PB = 256;
     while (PB > 64 && init_heap_bytes(PB, k) > 64 * 1024) PB -= 64;
     if (P.mcap > 0 && ws.D && init_heap_bytes(PB, k) <= 64 * 1024) {
-        unsigned long long p0 = 0, p1 = 0;
-        if (dist_timer.out) {
-            HIP_TRY(hipMemcpyAsync(&p0, ws.pairs, 8, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
-        }
+        const bool mfma = ws.exact_bf16 && metric == RINND_DIST_SQEUCLIDEAN &&
+                          leaf_dist_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024;
         dist_timer.begin(s);
-        if (ws.exact_bf16 && metric == RINND_DIST_SQEUCLIDEAN &&
-            leaf_dist_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024)
+        if (mfma)
             launch_leaf_dist_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, ws.D,
                                   P.Dl4, ws.pairs);
         else if (metric == RINND_DIST_SQEUCLIDEAN)
@@ -432,15 +428,13 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
                            dim3(PB), init_heap_bytes(PB, k) + (size_t)PB * kPushTL * 4, s, tree_indices, leaves,
                            ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
         push_timer.stop(s);
-        if (dist_timer.out || push_timer.out) {
-            HIP_TRY(hipMemcpyAsync(&p1, ws.pairs, 8, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
-            const double pairs = (double)(p1 - p0);  // ordered pairs (a, o), a != o
-            // dist: each member row once (n rows at most), ids, both halves of D written
-            dist_timer.collect((double)n * (dim * 4.0 + 4.0) + pairs * 4.0);
-            // push: D rows read, the ids of the partners, every member's heap read and written
-            push_timer.collect(pairs * 8.0 + (double)n * k * 9.0 * 2.0 + n * 40.0);
-        }
+        // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
+        // per-pair parts (4 B of D written per ordered pair; 4 B of D and 4 B of partner id read) once it
+        // has read the pair counter
+        // dist: each member's row (bf16 + norm on MFMA) and id once
+        dist_timer.collect((double)n * ((mfma ? ws.exact_bf16->ldb * 2.0 + 4.0 : dim * 4.0) + 4.0));
+        // push: every member's heap read and written, its leaf lookup
+        push_timer.collect((double)n * k * 9.0 * 2.0 + n * 40.0);
         untiled_cap = P.mcap;
     } else {
         const int64_t cap_rows = ((int64_t)kInitLdsBudget - (int64_t)heap) / (ldl * 4);
