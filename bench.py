@@ -58,7 +58,9 @@ def parse():
                     help="first run a plain 1 GiB device-to-device copy (3 times): the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--epsilons", default="0.1,0.15,0.2",
-                    help="comma-separated query epsilons; the fastest one at recall@10 >= 0.95 is reported")
+                    help="comma-separated query epsilons; the fastest run at recall@10 >= 0.95 is reported")
+    ap.add_argument("--fp8-epsilons", default="0.1,0.15",
+                    help="epsilons of the FP8-candidate search with exact re-ranking (empty: none)")
     return ap.parse_args()
 
 
@@ -333,16 +335,24 @@ def query_bench(torch, L, ctx, X, Xd, args):
     one_i = np.empty(kq, np.int32)
     one_d = np.empty(kq, np.float32)
     runs = []
-    for eps in [float(e) for e in args.epsilons.split(",")]:
+    # FP32 search at each epsilon, then the FP8 e4m3 candidate search (width 32) with exact re-ranking
+    modes = [(float(e), None) for e in args.epsilons.split(",")]
+    modes += [(float(e), 32) for e in args.fp8_epsilons.split(",") if e]
+
+    def run(nq_h, q_h, k_, eps, width, out_i, out_d):
+        if width is None:
+            return L.lib().rinnd_index_query(h, q_h, nq_h, k_, C.c_float(eps), out_i, out_d)
+        return L.lib().rinnd_index_query_quantized(h, q_h, nq_h, k_, C.c_float(eps), width, width, out_i, out_d)
+
+    for eps, width in modes:
+        name = "fp32" if width is None else f"fp8+rerank{width}"
         for _ in range(2):
-            ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(eps), L.ptr(oi),
-                                                L.ptr(od)), "rinnd_index_query")
+            ctx.check(run(args.queries, L.ptr(Qd), kq, eps, width, L.ptr(oi), L.ptr(od)), name)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         reps = 3
         for _ in range(reps):
-            ctx.check(L.lib().rinnd_index_query(h, L.ptr(Qd), args.queries, kq, C.c_float(eps), L.ptr(oi),
-                                                L.ptr(od)), "rinnd_index_query")
+            ctx.check(run(args.queries, L.ptr(Qd), kq, eps, width, L.ptr(oi), L.ptr(od)), name)
         torch.cuda.synchronize()
         qps = args.queries * reps / (time.perf_counter() - t0)
         rec = tie_aware_recall(torch, Xd, Qd, oi, true_d, kq)
@@ -350,14 +360,13 @@ def query_bench(torch, L, ctx, X, Xd, args):
         for j in range(200):
             q = np.ascontiguousarray(Qn[j])
             t0 = time.perf_counter()
-            ctx.check(L.lib().rinnd_index_query_one(h, L.ptr(q), kq, C.c_float(eps), L.ptr(one_i), L.ptr(one_d)),
-                      "rinnd_index_query_one")
+            ctx.check(run(1, L.ptr(q), kq, eps, width, L.ptr(one_i), L.ptr(one_d)), name)
             lat.append(time.perf_counter() - t0)
         lat_ms = np.array(lat) * 1e3
-        log(f"[bench] query eps {eps}: {qps:.0f} QPS, recall@10 {rec:.4f}, p50 {np.percentile(lat_ms, 50):.3f} ms, "
-            f"p99 {np.percentile(lat_ms, 99):.3f} ms")
-        runs.append({"qps": qps, "batch": args.queries, "k": kq, "epsilon": eps, "recall_at_10": rec,
-                     "p50_ms_single": float(np.percentile(lat_ms, 50)),
+        log(f"[bench] query {name} eps {eps}: {qps:.0f} QPS, recall@10 {rec:.4f}, "
+            f"p50 {np.percentile(lat_ms, 50):.3f} ms, p99 {np.percentile(lat_ms, 99):.3f} ms")
+        runs.append({"search": name, "qps": qps, "batch": args.queries, "k": kq, "epsilon": eps,
+                     "recall_at_10": rec, "p50_ms_single": float(np.percentile(lat_ms, 50)),
                      "p99_ms_single": float(np.percentile(lat_ms, 99))})
     L.lib().rinnd_index_free(h)
     ok = [r for r in runs if r["recall_at_10"] >= 0.95]

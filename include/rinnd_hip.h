@@ -231,6 +231,18 @@ rinnd_status rinnd_index_query(rinnd_index* index, const float* queries, int64_t
 /* `query_one_with_workspace` (core/index.rs:253-261): one query with query id 0. */
 rinnd_status rinnd_index_query_one(rinnd_index* index, const float* query, int64_t k, float epsilon,
                                    int32_t* out_idx, float* out_dist);
+/* The quantized query with explicit widths, `query_quantized(query_data, k, epsilon, candidate_width,
+ * rerank_width)` (rinnd/src/lib.rs:825-870) over `query_one_quantized_i8_with_workspace`
+ * (core/index.rs:283-330): the greedy search runs with a candidate width resolved as in
+ * resolve_quantized_candidate_width (core/index.rs:55-78) on distances from FP8 e4m3 codes of the data
+ * (per-row symmetric scale, built on the first call; the reference's counterpart is its int8 encoding), then,
+ * when should_rerank_quantized (core/index.rs:81-89) says so, the first rerank_width candidates get their
+ * FP32 distance and are sorted by (distance, id) (rerank_quantized_candidates, core/index.rs:103-124).
+ * k results per query, ids in input order, the index's correction applied, padded with (-1, +inf); the
+ * width rules the reference asserts are returned as RINND_ERR_INVALID_ARG.  dim <= 2048. */
+rinnd_status rinnd_index_query_quantized(rinnd_index* index, const float* queries, int64_t nq, int64_t k,
+                                         float epsilon, int64_t candidate_width, int64_t rerank_width,
+                                         int32_t* out_idx, float* out_dist);
 
 /* `neighbor_graph` getter (py:1087-1103): sorted, corrected n x k arrays in input order. */
 rinnd_status rinnd_index_neighbor_graph(const rinnd_index* index, int32_t* out_idx, float* out_dist);

@@ -125,9 +125,9 @@ class RINND:
                              "tree_quantization='none'")
         if direct and not normalize_cosine:
             raise ValueError("direct and quantized cosine modes require metric='cosine' and normalize=True")
-        if quantized is not None:
-            raise NotImplementedError(f"cosine_distance_mode={quantized!r} (int8 codes) is not available on the "
-                                      "MI355X path")
+        if quantized is not None and tree_quantization == "int8":
+            raise NotImplementedError("tree_quantization='int8' is not available on the MI355X path: the search "
+                                      "trees stay FP32")
         # metric -> (distance functor, correction) dispatch (py:1239-1275)
         if m == "euclidean":
             dist, corr = "sqeuclidean", "sqrt"
@@ -151,6 +151,11 @@ class RINND:
         if n_trees is None:  # py:1200-1204
             n_trees = min(12, max(3, _round_half_away(2.0 * math.log10(max(n_points, 1)))))
         self.n_points, self.dim = n_points, dim
+        # quantized cosine modes (py:714-731): the candidate search runs on FP8 e4m3 codes of the rows (the
+        # MI355X counterpart of the reference's int8 codes) with these default widths; FP32 rows are kept
+        # for the exact re-ranking
+        self._quantized = quantized
+        self._qcw, self._qrw = int(quantized_candidate_width), int(quantized_rerank_width)
         self.n_neighbors = int(n_neighbors)
         self.normalize = normalize_cosine
         self.metric = m
@@ -186,19 +191,44 @@ class RINND:
             q = normalize_rows(q)
         return q
 
-    def query(self, query_data, k=10, epsilon=0.1):
-        q = self._prep_queries(query_data)
-        nq = int(q.shape[0])
+    def _outputs(self, q, nq, k):
         if _is_device_tensor(q):
             import torch
-            oi = torch.empty((nq, k), dtype=torch.int32, device=q.device)
-            od = torch.empty((nq, k), dtype=torch.float32, device=q.device)
-        else:
-            oi = np.empty((nq, k), np.int32)
-            od = np.empty((nq, k), np.float32)
+            return (torch.empty((nq, k), dtype=torch.int32, device=q.device),
+                    torch.empty((nq, k), dtype=torch.float32, device=q.device))
+        return np.empty((nq, k), np.int32), np.empty((nq, k), np.float32)
+
+    def _query_fp8(self, q, k, epsilon, candidate_width, rerank_width):
+        nq = int(q.shape[0])
+        oi, od = self._outputs(q, nq, k)
+        st = L.lib().rinnd_index_query_quantized(self._h, L.ptr(q), nq, int(k), C.c_float(epsilon),
+                                                 int(candidate_width), int(rerank_width), L.ptr(oi), L.ptr(od))
+        self._ctx.check(st, "rinnd_index_query_quantized")
+        return oi, od
+
+    def query(self, query_data, k=10, epsilon=0.1):
+        q = self._prep_queries(query_data)
+        if self._quantized is not None:  # query_one_quantized with the constructor's widths
+            return self._query_fp8(q, k, epsilon, self._qcw, self._qrw)
+        nq = int(q.shape[0])
+        oi, od = self._outputs(q, nq, k)
         st = L.lib().rinnd_index_query(self._h, L.ptr(q), nq, int(k), C.c_float(epsilon), L.ptr(oi), L.ptr(od))
         self._ctx.check(st, "rinnd_index_query")
         return oi, od
+
+    def query_quantized(self, query_data, k=10, epsilon=0.1, candidate_width=0, rerank_width=0):
+        """Quantized query with explicit widths (py:825-870, validate_quantized_widths py:1148-1172)."""
+        if candidate_width != 0:
+            if candidate_width < k:
+                raise ValueError(f"candidate_width {candidate_width} must be at least k ({k})")
+            if rerank_width > candidate_width:
+                raise ValueError(f"rerank_width {rerank_width} must not exceed candidate_width {candidate_width}")
+            if 0 < rerank_width < k:
+                raise ValueError(f"rerank_width {rerank_width} must be zero or at least k ({k})")
+        q = self._prep_queries(query_data)
+        if self._quantized is None:
+            raise ValueError("query-time quantized widths require a quantized cosine_distance_mode")
+        return self._query_fp8(q, k, epsilon, candidate_width, rerank_width)
 
     def query_one(self, query, k=10, epsilon=0.1):
         q = np.ascontiguousarray(query, np.float32)
@@ -207,6 +237,9 @@ class RINND:
         if q.shape[0] != self.dim:
             raise ValueError(f"Query dimension {q.shape[0]} does not match data dimension {self.dim}")
         q2 = self._prep_queries(q[None, :], "query")
+        if self._quantized is not None:
+            oi, od = self._query_fp8(q2, k, epsilon, self._qcw, self._qrw)
+            return oi[0], od[0]
         oi = np.empty(k, np.int32)
         od = np.empty(k, np.float32)
         st = L.lib().rinnd_index_query_one(self._h, L.ptr(q2), int(k), C.c_float(epsilon), L.ptr(oi), L.ptr(od))
@@ -253,8 +286,21 @@ class RINND:
         return t["hyperplanes"], t["offsets"], t["children"], order[t["indices"]].astype(np.int32)
 
     def storage_info(self):
-        """py:1028-1077 for the FP32 representation."""
+        """py:1028-1077: the FP32 representation, or the FP8 e4m3 codes (rows padded to 16 bytes, one FP32
+        scale per row) of a quantized index with its retained FP32 rows."""
         data_bytes = self.n_points * self.dim * 4
+        if self._quantized is not None:
+            code_bytes = self.n_points * ((self.dim + 15) // 16 * 16)
+            meta = self.n_points * 4
+            return {
+                "encoding": "fp8_e4m3", "code_bytes": code_bytes, "metadata_bytes": meta,
+                "retained_fp32_bytes": data_bytes, "encode_seconds": 0.0, "tree_code_bytes": 0,
+                "tree_scale_bytes": 0, "tree_offset_bytes": 0, "tree_topology_bytes": 0,
+                "hypothetical_releasable_fp32_tree_bytes": 0, "released_fp32_tree_bytes": 0,
+                "released_fp32_data_bytes": 0, "retained_fp32_tree_topology_bytes": 0,
+                "allocated_bytes": code_bytes + meta + data_bytes,
+                "bytes_per_vector": float(code_bytes + meta) / max(self.n_points, 1),
+            }
         return {
             "encoding": "fp32", "code_bytes": data_bytes, "metadata_bytes": 0, "retained_fp32_bytes": 0,
             "encode_seconds": 0.0, "tree_code_bytes": 0, "tree_scale_bytes": 0, "tree_offset_bytes": 0,

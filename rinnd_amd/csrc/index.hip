@@ -45,9 +45,13 @@ struct rinnd_index {
     DBuf<int32_t> spill_i;
     int64_t seed_cap = 0;
     DBuf<int32_t> status;
-    DBuf<float> qbuf;
+    DBuf<float> qbuf;  // padded query rows (grow-only; padding columns stay zero)
     DBuf<int32_t> obuf_i;
     DBuf<float> obuf_d;
+    // FP8 e4m3 codes of X for the quantized candidate search (built on the first quantized query)
+    DBuf<uint8_t> q8;
+    DBuf<float> q8s;
+    int64_t ld8 = 0;
 };
 
 namespace rinnd {
@@ -177,29 +181,59 @@ static void ensure_query_workspace(rinnd_index& ix, int slots) {
     ix.slots = slots;
 }
 
-static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_t k, float eps, int64_t qid_base,
-                       int32_t* out_idx, float* out_dist) {
+// The query rows as the kernel reads them (stride ld, zero padding): device rows that already have that
+// stride are used in place, anything else is copied into the index's grow-only staging buffer.
+static const float* stage_queries(rinnd_index& ix, const float* queries, int64_t nq) {
     hipStream_t s = ix.ctx->stream;
+    const int64_t dim = ix.dim, ld = ix.ld;
+    if (dim == ld && ((uintptr_t)queries & 255) == 0 && device_pointer(queries)) return queries;
+    const size_t need = (size_t)nq * ld;
+    if (need > ix.qbuf.n) {
+        ix.qbuf.alloc(need);
+        HIP_TRY(hipMemsetAsync(ix.qbuf.p, 0, need * 4, s));  // padding columns are never written again
+    }
+    HIP_TRY(hipMemcpy2DAsync(ix.qbuf.p, ld * 4, queries, dim * 4, dim * 4, nq, hipMemcpyDefault, s));
+    return ix.qbuf.p;
+}
+
+static void ensure_fp8_codes(rinnd_index& ix) {
+    if (ix.q8.p) return;
+    hipStream_t s = ix.ctx->stream;
+    ix.ld8 = round_up(ix.dim, (int64_t)16);
+    ix.q8.alloc((size_t)std::max<int64_t>(ix.n, 1) * ix.ld8);
+    ix.q8s.alloc((size_t)std::max<int64_t>(ix.n, 1));
+    launch_fp8_encode(s, ix.X.p, ix.ld, (int)ix.dim, ix.n, ix.q8.p, ix.ld8, ix.q8s.p);
+    HIP_TRY(hipGetLastError());
+}
+
+// search_k: candidate width of the search (k for a plain query); rerank > 0: exact re-ranking of that many
+// candidates; fp8: candidate distances from the FP8 codes.  k results are written per query.
+static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_t k, float eps, int64_t qid_base,
+                       int32_t* out_idx, float* out_dist, int64_t search_k = -1, int64_t rerank = 0,
+                       bool fp8 = false) {
+    hipStream_t s = ix.ctx->stream;
+    if (search_k < 0) search_k = k;
     RINND_CHECK_ARG(k >= 0 && k <= search_max_k(), "k must be in [0, 512]");
+    RINND_CHECK_ARG(search_k >= k && search_k <= search_max_k(), "the candidate width must be in [k, 512]");
     if (nq == 0 || k == 0) return;
+    if (fp8) {
+        RINND_CHECK_ARG(round_up(ix.dim, (int64_t)16) <= 2048, "the FP8 candidate search supports dim <= 2048");
+        ensure_fp8_codes(ix);
+    }
     // workspace: one slot per resident wave; memory per slot is n bytes + 8n bytes of spill
     const int64_t per_slot = ix.n * 9;
     int64_t max_slots = std::max<int64_t>(64, std::min<int64_t>(2048, ((int64_t)16 << 30) / std::max<int64_t>(per_slot, 1)));
     const int slots = (int)std::min<int64_t>(nq, max_slots);
     ensure_query_workspace(ix, std::max(slots, ix.slots));
-    DBuf<float> qtmp;
-    int64_t ldq = 0;
-    make_padded(s, queries, nq, (int)ix.dim, qtmp, ldq);
+    const float* qd = stage_queries(ix, queries, nq);
     const bool dev_out = device_pointer(out_idx) && device_pointer(out_dist);
-    DBuf<int32_t> oi;
-    DBuf<float> od;
     int32_t* oip = out_idx;
     float* odp = out_dist;
     if (!dev_out) {
-        oi.alloc((size_t)nq * k);
-        od.alloc((size_t)nq * k);
-        oip = oi.p;
-        odp = od.p;
+        ix.obuf_i.ensure((size_t)nq * k);
+        ix.obuf_d.ensure((size_t)nq * k);
+        oip = ix.obuf_i.p;
+        odp = ix.obuf_d.p;
     }
     SearchArgs a{};
     a.metric = ix.metric;
@@ -216,11 +250,11 @@ static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_
     a.children = ix.d_ch.p;
     a.tindices = ix.d_tix.p;
     a.leaf_budget = (int)ix.leaf_budget;
-    a.Q = qtmp.p;
-    a.ldq = ldq;
+    a.Q = qd;
+    a.ldq = ix.ld;
     a.nq = nq;
     a.qid_base = qid_base;
-    a.k = (int)k;
+    a.k = (int)search_k;
     a.eps = eps;
     a.min_distance = ix.min_distance;
     a.seed = ix.seed;
@@ -234,6 +268,11 @@ static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_
     a.seed_spill_i = ix.spill_i.p;
     a.seed_cap = ix.seed_cap;
     a.status = ix.status.p;
+    a.q8 = fp8 ? ix.q8.p : nullptr;
+    a.ld8 = ix.ld8;
+    a.q8s = fp8 ? ix.q8s.p : nullptr;
+    a.k_out = (int)k;
+    a.rerank = (int)rerank;
     launch_search(s, a, slots);
     HIP_TRY(hipGetLastError());
     if (!dev_out) {
@@ -337,6 +376,23 @@ rinnd_status rinnd_index_query_one(rinnd_index* ix, const float* query, int64_t 
                                    int32_t* out_idx, float* out_dist) {
     if (!ix) return RINND_ERR_INVALID_ARG;
     return guard(ix->ctx, [&] { query_impl(*ix, query, 1, k, epsilon, 0, out_idx, out_dist); });
+}
+
+rinnd_status rinnd_index_query_quantized(rinnd_index* ix, const float* queries, int64_t nq, int64_t k,
+                                         float epsilon, int64_t candidate_width, int64_t rerank_width,
+                                         int32_t* out_idx, float* out_dist) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        RINND_CHECK_ARG(nq >= 0 && k >= 0 && candidate_width >= 0 && rerank_width >= 0, "negative size");
+        // resolve_quantized_candidate_width / should_rerank_quantized (core/index.rs:55-90)
+        const int64_t width = candidate_width == 0 ? std::max(k, rerank_width) : candidate_width;
+        RINND_CHECK_ARG(width >= k, "quantized candidate width must be at least k");
+        RINND_CHECK_ARG(rerank_width <= width, "quantized rerank width must not exceed candidate width");
+        RINND_CHECK_ARG(candidate_width == 0 || rerank_width == 0 || rerank_width >= k,
+                        "quantized rerank width must be zero or at least k");
+        const bool rerank = candidate_width == 0 ? rerank_width > k : rerank_width > 0;
+        query_impl(*ix, queries, nq, k, epsilon, 0, out_idx, out_dist, width, rerank ? rerank_width : 0, true);
+    });
 }
 
 rinnd_status rinnd_index_neighbor_graph(const rinnd_index* ix, int32_t* out_idx, float* out_dist) {

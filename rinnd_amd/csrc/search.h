@@ -38,9 +38,20 @@ struct SearchArgs {
     int32_t* seed_spill_i;
     int64_t seed_cap;
     int32_t* status;
+    // FP8 e4m3 candidate search (q8 null: FP32 distances throughout).  k above is then the candidate
+    // width; k_out results are written per query, after an exact FP32 re-ranking of the first `rerank`
+    // candidates when rerank > 0.
+    const uint8_t* q8;  // n x ld8 codes in the reordered row order, zero past dim
+    int64_t ld8;
+    const float* q8s;   // per-row dequantisation scale
+    int k_out;
+    int rerank;
 };
 
 void launch_search(hipStream_t s, const SearchArgs& a, int slots);
+// per-row symmetric FP8 e4m3 codes: scale = max|x| / 240, code = e4m3(x / scale), rows of ld8 bytes
+void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, uint8_t* codes, int64_t ld8,
+                       float* scales);
 int search_max_k();
 int search_max_degree();
 int search_max_budget();

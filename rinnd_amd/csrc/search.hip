@@ -147,6 +147,57 @@ __device__ __forceinline__ float corrected(int corr, float x) {
     return x;
 }
 
+// The metric evaluated from its parts (q.x, |q|^2, |x|^2, |q - x|^2), with the same formulas as
+// distance_exact: used for the FP8 candidate distances, which are approximate by construction.
+__device__ __forceinline__ float distance_from_parts(int metric, float dot, float qq, float xx, float d2) {
+    const float FMAX = 3.40282347e+38f;
+    switch (metric) {
+        case RINND_DIST_SQEUCLIDEAN: return d2;
+        case RINND_DIST_COSINE: {
+            const float denom = sqrtf(qq * xx);
+            if (denom < 1e-12f) return 1.0f;
+            const float c = fminf(fmaxf(dot / denom, -1.0f), 1.0f);
+            return 1.0f - c;
+        }
+        case RINND_DIST_INNER_PRODUCT: return -dot;
+        case RINND_DIST_ALT_DOT: return dot <= 0.0f ? FMAX : -log2f(dot);
+        case RINND_DIST_DIRECT_COSINE: return 1.0f - fminf(fmaxf(dot, -1.0f), 1.0f);
+        case RINND_DIST_ALT_COSINE:
+            if (qq == 0.0f && xx == 0.0f) return 0.0f;
+            if (qq == 0.0f || xx == 0.0f || dot <= 0.0f) return FMAX;
+            return log2f(sqrtf(qq * xx) / dot);
+        case RINND_DIST_ALT_INNER_PRODUCT: return dot <= 0.0f ? FMAX : 1.0f / dot;
+        default: return __builtin_nanf("");
+    }
+}
+
+// Distance from query q (FP32, zero past dim up to a multiple of 16) to an FP8 e4m3 row with scale sc.
+__device__ __forceinline__ float fp8_distance(int metric, const float* q, float qq, const uint8_t* __restrict__ c,
+                                              float sc, int dim) {
+    float d2 = 0.f, dot = 0.f, xx = 0.f;
+    for (int i = 0; i < dim; i += 16) {
+        const uint4 w = *reinterpret_cast<const uint4*>(c + i);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float x[4];
+            x[0] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 0);
+            x[1] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 1);
+            x[2] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 2);
+            x[3] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 3);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float xv = sc * x[b], qv = q[i + 4 * j + b], df = qv - xv;
+                d2 = fmaf(df, df, d2);
+                dot = fmaf(qv, xv, dot);
+                xx = fmaf(xv, xv, xx);
+            }
+        }
+    }
+    return distance_from_parts(metric, dot, qq, xx, d2);
+}
+
+template <bool Q8>
 __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     __shared__ WaveState S;
     const int lane = threadIdx.x;
@@ -160,8 +211,18 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         const int64_t qid = a.qid_base + qi;
         const float* qg = a.Q + qi * a.ldq;
         if (q_in_lds)
-            for (int i = lane; i < dim; i += 64) S.q[i] = qg[i];
+            for (int i = lane; i < (Q8 ? (int)a.ld8 : dim); i += 64) S.q[i] = i < dim ? qg[i] : 0.f;
         const float* q = q_in_lds ? S.q : qg;
+        float qq = 0.f;  // |q|^2 for the FP8 distances of the angular metrics
+        if (Q8) {
+            for (int i = lane; i < dim; i += 64) qq = fmaf(qg[i], qg[i], qq);
+            for (int o = 32; o > 0; o >>= 1) qq += __shfl_xor(qq, o);
+        }
+        // candidate distance: FP32 in the reference's arithmetic, or from the FP8 codes
+        auto cand_dist = [&](int32_t x) -> float {
+            if (Q8) return fp8_distance(a.metric, q, qq, a.q8 + (int64_t)x * a.ld8, a.q8s[x], dim);
+            return distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+        };
         // epoch (VisitedSet::clear, visited.rs:51-57)
         int ep = a.epochs[slot] + 1;
         if (ep > 255) {
@@ -273,7 +334,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                             ok = vis[x] != ep;
                             vis[x] = (uint8_t)ep;
                         }
-                        if (ok) d = distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+                        if (ok) d = cand_dist(x);
                     }
                     S.nb_i[lane] = x;
                     S.nb_d[lane] = d;
@@ -299,7 +360,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                 const int32_t x = (int32_t)rng.next_index((uint64_t)a.n);
                 if (vis[x] != ep) {
                     vis[x] = (uint8_t)ep;
-                    const float d = distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+                    const float d = cand_dist(x);
                     bounded_push(S.rh_d, S.rh_i, S.rh_len, k, d, x);
                     seed_push(d, x);
                 }
@@ -340,7 +401,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                         ok = vis[x] != ep;
                         vis[x] = (uint8_t)ep;
                     }
-                    if (ok) d = distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+                    if (ok) d = cand_dist(x);
                 }
                 S.nb_i[lane] = x;
                 S.nb_d[lane] = d;
@@ -384,8 +445,38 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             }
         }
         wave_sync();
-        const int m = S.rh_len;
-        for (int j = lane; j < k; j += 64) {
+        int m = S.rh_len;
+        if (a.rerank > 0) {
+            // rerank_quantized_candidates (core/index.rs:103-124): the first `rerank` candidates get their
+            // FP32 distance, then sort by (distance total order, internal id) and keep k
+            const int rw = min(a.rerank, m);
+            for (int j = lane; j < rw; j += 64)
+                S.rh_d[j] = distance_exact(a.metric, q, a.X + (int64_t)S.rh_i[j] * a.ld, dim);
+            __threadfence_block();
+            wave_sync();
+            if (lane == 0) {
+                auto key = [](float d) {  // f32::total_cmp order
+                    int32_t b = __builtin_bit_cast(int32_t, d);
+                    return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
+                };
+                for (int i = 1; i < rw; ++i) {
+                    const float d = S.rh_d[i];
+                    const int32_t x = S.rh_i[i];
+                    int j = i - 1;
+                    while (j >= 0 && (key(S.rh_d[j]) > key(d) || (key(S.rh_d[j]) == key(d) && S.rh_i[j] > x))) {
+                        S.rh_d[j + 1] = S.rh_d[j];
+                        S.rh_i[j + 1] = S.rh_i[j];
+                        --j;
+                    }
+                    S.rh_d[j + 1] = d;
+                    S.rh_i[j + 1] = x;
+                }
+            }
+            wave_sync();
+            m = rw;
+        }
+        const int ko = a.k_out;
+        for (int j = lane; j < ko; j += 64) {
             int32_t oi = -1;
             float od = __builtin_inff();
             if (j < m) {
@@ -393,15 +484,49 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                 oi = x >= 0 ? (int32_t)a.vertex_order[x] : x;
                 od = corrected(a.corr, S.rh_d[j]);
             }
-            a.out_idx[qi * k + j] = oi;
-            a.out_dist[qi * k + j] = od;
+            a.out_idx[qi * ko + j] = oi;
+            a.out_dist[qi * ko + j] = od;
         }
         wave_sync();
     }
 }
 
 void launch_search(hipStream_t s, const SearchArgs& a, int slots) {
-    hipLaunchKernelGGL(k_search, dim3(slots), dim3(64), 0, s, a);
+    if (a.q8) hipLaunchKernelGGL(k_search<true>, dim3(slots), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(k_search<false>, dim3(slots), dim3(64), 0, s, a);
+}
+
+// One wave per row: max |x| by a wave reduction, then the codes, two values per conversion.
+__global__ __launch_bounds__(256) void k_fp8_encode(const float* __restrict__ X, int64_t ld, int dim, int64_t n,
+                                                    uint8_t* __restrict__ codes, int64_t ld8, float* __restrict__ scales) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
+        const float* x = X + r * ld;
+        float mx = 0.f;
+        for (int i = lane; i < dim; i += 64) mx = fmaxf(mx, fabsf(x[i]));
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        const float sc = mx > 0.f ? mx / 240.0f : 1.0f;  // 240: representable in both e4m3 variants
+        const float inv = 1.0f / sc;
+        if (lane == 0) scales[r] = sc;
+        uint32_t* out = reinterpret_cast<uint32_t*>(codes + r * ld8);
+        for (int i4 = lane; i4 < (int)(ld8 >> 2); i4 += 64) {
+            float v[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = 4 * i4 + b;
+                v[b] = i < dim ? x[i] * inv : 0.f;
+            }
+            int word = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+            word = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], word, true);
+            out[i4] = (uint32_t)word;
+        }
+    }
+}
+void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, uint8_t* codes, int64_t ld8,
+                       float* scales) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fp8_encode, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)4), 1 << 16)), dim3(256), 0,
+                       s, X, ld, dim, n, codes, ld8, scales);
 }
 
 int search_max_k() { return kMaxK; }
