@@ -220,9 +220,11 @@ static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_
         RINND_CHECK_ARG(round_up(ix.dim, (int64_t)16) <= 2048, "the FP8 candidate search supports dim <= 2048");
         ensure_fp8_codes(ix);
     }
-    // workspace: one slot per resident wave; memory per slot is n bytes + 8n bytes of spill
+    // workspace: one slot per concurrent query (a wave; up to 16 are resident per CU); memory per slot is n
+    // bytes of visited epochs + 8n bytes of seed-heap spill (the heap never holds more than n entries), at
+    // most 32 GB of the 288 GB in all
     const int64_t per_slot = ix.n * 9;
-    int64_t max_slots = std::max<int64_t>(64, std::min<int64_t>(2048, ((int64_t)16 << 30) / std::max<int64_t>(per_slot, 1)));
+    int64_t max_slots = std::max<int64_t>(64, std::min<int64_t>(4096, ((int64_t)32 << 30) / std::max<int64_t>(per_slot, 1)));
     const int slots = (int)std::min<int64_t>(nq, max_slots);
     ensure_query_workspace(ix, std::max(slots, ix.slots));
     const float* qd = stage_queries(ix, queries, nq);

@@ -46,9 +46,11 @@ struct SearchArgs {
     const float* q8s;   // per-row dequantisation scale
     int k_out;
     int rerank;
+    int qcap;  // floats of the query row kept in LDS (0: read from memory); set by launch_search
+    int kcap;  // result-heap entries in LDS; set by launch_search
 };
 
-void launch_search(hipStream_t s, const SearchArgs& a, int slots);
+void launch_search(hipStream_t s, SearchArgs a, int slots);
 // per-row symmetric FP8 e4m3 codes: scale = max|x| / 240, code = e4m3(x / scale), rows of ld8 bytes
 void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, uint8_t* codes, int64_t ld8,
                        float* scales);

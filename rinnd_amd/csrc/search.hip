@@ -119,17 +119,23 @@ __device__ void mh_pop(float* hd, int32_t* hi, int64_t& len, float& d, int32_t& 
     hi[pos] = li;
 }
 
+// Per-wave LDS besides the dynamic part (the query row and the result heap, sized by dim and k).  The
+// best-first frontier of the tree descent and the neighbour batch of the seeding/traversal are never live
+// together, so they share storage.
 struct WaveState {
-    __align__(16) float q[kQLds];
-    float rh_d[kMaxK];
-    int32_t rh_i[kMaxK];
     float sh_d[kSeedLds];
     int32_t sh_i[kSeedLds];
-    int32_t nb_i[kMaxDeg];
-    float nb_d[kMaxDeg];
-    uint8_t nb_ok[kMaxDeg];
-    float fr_p[kFrontier];
-    int32_t fr_n[kFrontier];
+    union {
+        struct {
+            int32_t nb_i[kMaxDeg];
+            float nb_d[kMaxDeg];
+            uint8_t nb_ok[kMaxDeg];
+        };
+        struct {
+            float fr_p[kFrontier];
+            int32_t fr_n[kFrontier];
+        };
+    };
     int64_t leaves[64 * 2];
     int rh_len;
     int64_t sh_len;
@@ -197,22 +203,29 @@ __device__ __forceinline__ float fp8_distance(int metric, const float* q, float 
     return distance_from_parts(metric, dot, qq, xx, d2);
 }
 
-template <bool Q8>
+// One instance per metric (METRIC >= 0; the switch folds away, so only that metric's accumulators take
+// registers) and one for any metric (METRIC < 0).
+template <bool Q8, int METRIC>
 __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
+    const int metric = METRIC >= 0 ? METRIC : a.metric;
     __shared__ WaveState S;
+    extern __shared__ __align__(16) float sdyn[];
+    float* const sq = sdyn;                 // a.qcap floats (0: the query is read from memory)
+    float* const rh_d = sdyn + a.qcap;      // result heap, a.kcap entries
+    int32_t* const rh_i = (int32_t*)(rh_d + a.kcap);
     const int lane = threadIdx.x;
     const int slot = blockIdx.x;
     uint8_t* vis = a.visited + (int64_t)slot * a.n;
     float* gsd = a.seed_spill_d + (int64_t)slot * a.seed_cap;
     int32_t* gsi = a.seed_spill_i + (int64_t)slot * a.seed_cap;
     const int dim = a.dim;
-    const bool q_in_lds = dim <= kQLds;
+    const bool q_in_lds = a.qcap > 0;
     for (int64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
         const int64_t qid = a.qid_base + qi;
         const float* qg = a.Q + qi * a.ldq;
         if (q_in_lds)
-            for (int i = lane; i < (Q8 ? (int)a.ld8 : dim); i += 64) S.q[i] = i < dim ? qg[i] : 0.f;
-        const float* q = q_in_lds ? S.q : qg;
+            for (int i = lane; i < (Q8 ? (int)a.ld8 : dim); i += 64) sq[i] = i < dim ? qg[i] : 0.f;
+        const float* q = q_in_lds ? sq : qg;
         float qq = 0.f;  // |q|^2 for the FP8 distances of the angular metrics
         if (Q8) {
             for (int i = lane; i < dim; i += 64) qq = fmaf(qg[i], qg[i], qq);
@@ -220,8 +233,8 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         }
         // candidate distance: FP32 in the reference's arithmetic, or from the FP8 codes
         auto cand_dist = [&](int32_t x) -> float {
-            if (Q8) return fp8_distance(a.metric, q, qq, a.q8 + (int64_t)x * a.ld8, a.q8s[x], dim);
-            return distance_exact(a.metric, q, a.X + (int64_t)x * a.ld, dim);
+            if (Q8) return fp8_distance(metric, q, qq, a.q8 + (int64_t)x * a.ld8, a.q8s[x], dim);
+            return distance_exact(metric, q, a.X + (int64_t)x * a.ld, dim);
         };
         // epoch (VisitedSet::clear, visited.rs:51-57)
         int ep = a.epochs[slot] + 1;
@@ -345,7 +358,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                         const int cnt = (int)min((int64_t)64, s1 - c0);
                         for (int j = 0; j < cnt; ++j)
                             if (S.nb_ok[j]) {
-                                bounded_push(S.rh_d, S.rh_i, S.rh_len, k, S.nb_d[j], S.nb_i[j]);
+                                bounded_push(rh_d, rh_i, S.rh_len, k, S.nb_d[j], S.nb_i[j]);
                                 seed_push(S.nb_d[j], S.nb_i[j]);
                             }
                     }
@@ -361,11 +374,11 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                 if (vis[x] != ep) {
                     vis[x] = (uint8_t)ep;
                     const float d = cand_dist(x);
-                    bounded_push(S.rh_d, S.rh_i, S.rh_len, k, d, x);
+                    bounded_push(rh_d, rh_i, S.rh_len, k, d, x);
                     seed_push(d, x);
                 }
             }
-            const float maxd = S.rh_len > 0 ? S.rh_d[0] : __builtin_inff();
+            const float maxd = S.rh_len > 0 ? rh_d[0] : __builtin_inff();
             S.bound = maxd + a.eps * (maxd - a.min_distance);
         }
         __threadfence_block();
@@ -415,10 +428,10 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                         if (!S.nb_ok[j]) continue;
                         const float dj = S.nb_d[j];
                         if (dj < bound) {
-                            const bool ins = bounded_push(S.rh_d, S.rh_i, S.rh_len, k, dj, S.nb_i[j]);
+                            const bool ins = bounded_push(rh_d, rh_i, S.rh_len, k, dj, S.nb_i[j]);
                             seed_push(dj, S.nb_i[j]);
                             if (ins) {
-                                const float maxd = S.rh_d[0];
+                                const float maxd = rh_d[0];
                                 bound = maxd + a.eps * (maxd - a.min_distance);
                             }
                         }
@@ -432,16 +445,16 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         if (lane == 0) {
             const int m = S.rh_len;
             for (int i = 1; i < m; ++i) {  // stable insertion sort by distance
-                const float d = S.rh_d[i];
-                const int32_t x = S.rh_i[i];
+                const float d = rh_d[i];
+                const int32_t x = rh_i[i];
                 int j = i - 1;
-                while (j >= 0 && S.rh_d[j] > d) {
-                    S.rh_d[j + 1] = S.rh_d[j];
-                    S.rh_i[j + 1] = S.rh_i[j];
+                while (j >= 0 && rh_d[j] > d) {
+                    rh_d[j + 1] = rh_d[j];
+                    rh_i[j + 1] = rh_i[j];
                     --j;
                 }
-                S.rh_d[j + 1] = d;
-                S.rh_i[j + 1] = x;
+                rh_d[j + 1] = d;
+                rh_i[j + 1] = x;
             }
         }
         wave_sync();
@@ -451,7 +464,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             // FP32 distance, then sort by (distance total order, internal id) and keep k
             const int rw = min(a.rerank, m);
             for (int j = lane; j < rw; j += 64)
-                S.rh_d[j] = distance_exact(a.metric, q, a.X + (int64_t)S.rh_i[j] * a.ld, dim);
+                rh_d[j] = distance_exact(metric, q, a.X + (int64_t)rh_i[j] * a.ld, dim);
             __threadfence_block();
             wave_sync();
             if (lane == 0) {
@@ -460,16 +473,16 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                     return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
                 };
                 for (int i = 1; i < rw; ++i) {
-                    const float d = S.rh_d[i];
-                    const int32_t x = S.rh_i[i];
+                    const float d = rh_d[i];
+                    const int32_t x = rh_i[i];
                     int j = i - 1;
-                    while (j >= 0 && (key(S.rh_d[j]) > key(d) || (key(S.rh_d[j]) == key(d) && S.rh_i[j] > x))) {
-                        S.rh_d[j + 1] = S.rh_d[j];
-                        S.rh_i[j + 1] = S.rh_i[j];
+                    while (j >= 0 && (key(rh_d[j]) > key(d) || (key(rh_d[j]) == key(d) && rh_i[j] > x))) {
+                        rh_d[j + 1] = rh_d[j];
+                        rh_i[j + 1] = rh_i[j];
                         --j;
                     }
-                    S.rh_d[j + 1] = d;
-                    S.rh_i[j + 1] = x;
+                    rh_d[j + 1] = d;
+                    rh_i[j + 1] = x;
                 }
             }
             wave_sync();
@@ -480,9 +493,9 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             int32_t oi = -1;
             float od = __builtin_inff();
             if (j < m) {
-                const int32_t x = S.rh_i[j];
+                const int32_t x = rh_i[j];
                 oi = x >= 0 ? (int32_t)a.vertex_order[x] : x;
-                od = corrected(a.corr, S.rh_d[j]);
+                od = corrected(a.corr, rh_d[j]);
             }
             a.out_idx[qi * ko + j] = oi;
             a.out_dist[qi * ko + j] = od;
@@ -491,9 +504,28 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     }
 }
 
-void launch_search(hipStream_t s, const SearchArgs& a, int slots) {
-    if (a.q8) hipLaunchKernelGGL(k_search<true>, dim3(slots), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL(k_search<false>, dim3(slots), dim3(64), 0, s, a);
+template <bool Q8>
+static void launch_search_q(hipStream_t s, const SearchArgs& a, int slots, size_t lds) {
+    switch (a.metric) {
+        case RINND_DIST_SQEUCLIDEAN:
+            hipLaunchKernelGGL((k_search<Q8, RINND_DIST_SQEUCLIDEAN>), dim3(slots), dim3(64), lds, s, a);
+            break;
+        case RINND_DIST_ALT_DOT:
+            hipLaunchKernelGGL((k_search<Q8, RINND_DIST_ALT_DOT>), dim3(slots), dim3(64), lds, s, a);
+            break;
+        case RINND_DIST_DIRECT_COSINE:
+            hipLaunchKernelGGL((k_search<Q8, RINND_DIST_DIRECT_COSINE>), dim3(slots), dim3(64), lds, s, a);
+            break;
+        default: hipLaunchKernelGGL((k_search<Q8, -1>), dim3(slots), dim3(64), lds, s, a);
+    }
+}
+void launch_search(hipStream_t s, SearchArgs a, int slots) {
+    const int qlen = a.q8 ? (int)a.ld8 : a.dim;  // the FP8 path reads the zero padding up to ld8
+    a.qcap = qlen <= kQLds ? (int)round_up(qlen, 4) : 0;
+    a.kcap = (int)round_up(std::max(a.k, 1), 4);
+    const size_t lds = ((size_t)a.qcap + 2 * (size_t)a.kcap) * 4;
+    if (a.q8) launch_search_q<true>(s, a, slots, lds);
+    else launch_search_q<false>(s, a, slots, lds);
 }
 
 // One wave per row: max |x| by a wave reduction, then the codes, two values per conversion.
