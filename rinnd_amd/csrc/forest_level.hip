@@ -76,9 +76,8 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 
 __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* nodes, int64_t na,
                                                     const int32_t* const* perm, uint8_t* split, PInfo* pinfo,
-                                                    unsigned long long* active_pairs, unsigned* tree_children) {
+                                                    unsigned long long* active_pairs) {
     __shared__ float red[4];
-    if (blockIdx.x == 0 && threadIdx.x < a.n_trees) tree_children[threadIdx.x] = 0;  // counted by k_lvl_counts
     unsigned long long act = 0;  // members of split nodes: the (tree, point) pairs tested this level
     for (int64_t e = blockIdx.x; e < na; e += gridDim.x) {
         const LvlNode nd = nodes[e];
@@ -442,7 +441,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
 }
 
 __global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const uint32_t* prefix, uint8_t* split,
-                             uint32_t* nchild, uint32_t* leftsz, unsigned* tree_children) {
+                             uint32_t* nchild, uint32_t* leftsz) {
     if (blockIdx.x == 0 && threadIdx.x == 0) nchild[na] = 0;  // the scan's total
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
         const LvlNode nd = nodes[e];
@@ -461,7 +460,6 @@ __global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const 
                 a.offsets[nd.tree][nd.node] = 0.f;
             } else {
                 c = 2;
-                atomicAdd(&tree_children[nd.tree], 2u);
             }
         }
         nchild[e] = c;
@@ -470,16 +468,22 @@ __global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const 
 
 // Per tree: first child id of the next level (the tree's node count so far) and the exclusive prefix of
 // the children counts over the trees (the active list is grouped by tree).
-__global__ void k_lvl_tree_base(int n_trees, const unsigned* tree_children, int32_t* node_count, int32_t* base,
-                                uint32_t* cum) {
+// Tree t's entries of the active list are [start[t], start[t+1]) (the list is grouped by tree), so its
+// children count is a difference of the children scan - no atomics in k_lvl_counts.  cum (the next list's
+// start per tree, with the total at n_trees) becomes the next level's start.
+__global__ void k_lvl_tree_base(int n_trees, const uint32_t* start, const uint32_t* cscan, unsigned* tree_children,
+                                int32_t* node_count, int32_t* base, uint32_t* cum) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t acc = 0;
     for (int t = 0; t < n_trees; ++t) {
+        const uint32_t c = cscan[start[t + 1]] - cscan[start[t]];
+        tree_children[t] = c;
         base[t] = node_count[t];
         cum[t] = acc;
-        acc += tree_children[t];
-        node_count[t] += (int32_t)tree_children[t];
+        acc += c;
+        node_count[t] += (int32_t)c;
     }
+    cum[n_trees] = acc;
 }
 
 __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* split, const uint32_t* cscan,
@@ -632,7 +636,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     LvlNode* nxt = nxtb.p;
     unsigned long long pairs_before = 0;
     WBuf<int32_t> d_base(W, kWsForest + 22, T), d_count(W, kWsForest + 20, T);
-    WBuf<uint32_t> d_cum(W, kWsForest + 23, T);
+    WBuf<uint32_t> d_cum(W, kWsForest + 23, T + 1), d_start(W, kWsForest + 26, T + 1);
     size_t cscan_bytes = 0;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, cscan_bytes, nchild.p, cscan.p, (int)(max_nodes + 1)));
     WBuf<char> cscan_tmp(W, kWsForest + 24, std::max<size_t>(cscan_bytes, 16));
@@ -645,10 +649,15 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     std::vector<int64_t> node_count(T, 1);
     {
         const std::vector<int32_t> ones(T, 1);
+        std::vector<uint32_t> first(T + 1);  // level 0: root of tree t is entry t
+        for (int t = 0; t <= T; ++t) first[t] = (uint32_t)t;
         HIP_TRY(hipMemcpyAsync(d_count.p, ones.data(), T * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_start.p, first.data(), (T + 1) * 4, hipMemcpyHostToDevice, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
     std::vector<unsigned long long> lst(1 + (T + 1) / 2);
+    uint32_t* st_cur = d_start.p;  // first active entry of each tree (and the total), this level
+    uint32_t* st_next = d_cum.p;   // the same for the next level
     int64_t na = T;
     int phase = 0;
     // Side tests: point-major (every point's row read once for all trees) while most (tree, point) pairs
@@ -660,7 +669,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
         hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur, na,
-                           (const int32_t* const*)P, split.p, pinfo.p, active_pairs, tree_children);
+                           (const int32_t* const*)P, split.p, pinfo.p, active_pairs);
         margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
         if (sparse)
@@ -684,10 +693,11 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         size_t b = scan_bytes;  // flags[total] stays 0 from the initial clear: the scan's total
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
         hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, prefix.p, split.p,
-                           nchild.p, leftsz.p, tree_children);
+                           nchild.p, leftsz.p);
         size_t cb = cscan_bytes;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
-        hipLaunchKernelGGL(k_lvl_tree_base, dim3(1), dim3(64), 0, s, T, tree_children, d_count.p, d_base.p, d_cum.p);
+        hipLaunchKernelGGL(k_lvl_tree_base, dim3(1), dim3(64), 0, s, T, st_cur, cscan.p, tree_children, d_count.p,
+                           d_base.p, st_next);
         HIP_TRY(hipMemcpyAsync(lst.data(), lstat.p, lst.size() * 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         const unsigned long long pairs_now = lst[0];
@@ -711,13 +721,14 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         }
         if (over) return false;
         hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
-                           leftsz.p, d_base.p, d_cum.p, d_ch.p, nxt);
+                           leftsz.p, d_base.p, st_next, d_ch.p, nxt);
         hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur, T, n, split.p,
                            cscan.p, leftsz.p, prefix.p, flags.p, (const int32_t* const*)P, P + T,
                            P + 2 * T, P + 3 * T, nodePos.p, sparse_next ? 0 : 1);
         HIP_TRY(hipGetLastError());
         sparse = sparse_next;
         std::swap(cur, nxt);
+        std::swap(st_cur, st_next);
         phase ^= 1;
         na = next_na;
     }
