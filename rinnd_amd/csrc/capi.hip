@@ -313,7 +313,8 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
                              leaves.p, ltmp.p, lt_bytes);
             launch_leaf_init(s, metric, DXw, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
-                             w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer);
+                             w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer,
+                             t < 3 /* the first trees: most pushes are accepted */);
         }
         if (stats) {  // per-pair bytes: D written (4 B per ordered pair), D and the partner id read (8 B)
             const double pairs = (double)d2h_u64(s, lpairs.p);

@@ -248,7 +248,8 @@ int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size);
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
                       const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
                       int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws,
-                      const KTimer& dist_timer, const KTimer& push_timer);
+                      const KTimer& dist_timer, const KTimer& push_timer,
+                      bool early);  // early: one of the first trees of a build (narrower push tiles)
 
 // ordered-apply pipeline
 void launch_leaf_pairs(hipStream_t s, int metric, const float* X, int64_t ld, int dim,

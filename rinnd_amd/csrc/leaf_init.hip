@@ -270,11 +270,12 @@ __global__ __launch_bounds__(256) void k_leaf_dist_sq(const float* __restrict__ 
 
 // One lane per tree position: its heap in an LDS column, its root in a register (a rejected push costs
 // one compare).  A wave's 64 positions are consecutive, so their rows of D form one contiguous block:
-// the wave streams it through a private LDS tile in 32-column chunks, eight lanes per row segment
+// the wave streams it through a private LDS tile in COLS-column chunks, COLS/4 lanes per row segment
 // (coalesced, each byte read once), with the next chunk loaded into registers while the current one is
-// replayed.  Waves never wait for each other.
-constexpr int kPushCols = 32;
-constexpr int kPushTL = kPushCols + 1;  // padded tile row: lane-strided reads are conflict-free
+// replayed.  Waves never wait for each other.  16-column tiles leave room for a third workgroup per CU,
+// which hides the sift latency of the many accepted pushes of the first trees; 32-column tiles halve the
+// chunk overhead once most pushes are rejected.
+template <int COLS>
 __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
                                                    const int32_t* __restrict__ mark, const int64_t* __restrict__ rank,
                                                    int64_t n, int mcap, const float* __restrict__ D, int64_t Dl4,
@@ -285,6 +286,7 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
     float* hd = psm;
     int32_t* hi = (int32_t*)(hd + B * k);
     uint8_t* hf = (uint8_t*)(hi + B * k);
+    constexpr int kPushCols = COLS, kPushTL = COLS + 1;  // padded tile row: lane-strided reads conflict-free
     float* tile = psm + round_up((int64_t)B * k * 2 + (B * k + 3) / 4, 4) + w * 64 * kPushTL;
     constexpr int LPR = kPushCols / 4;  // loader lanes per row (one float4 each)
     constexpr int RPI = 64 / LPR;       // rows per load instruction
@@ -398,7 +400,7 @@ int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size) {
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
                       const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
                       int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws,
-                      const KTimer& dist_timer, const KTimer& push_timer) {
+                      const KTimer& dist_timer, const KTimer& push_timer, bool early) {
     if (max_leaves <= 0) return;
     const int B = init_block(leaf_size);
     const size_t heap = init_heap_bytes(B, k);
@@ -424,9 +426,13 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
                                ld, dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
         dist_timer.stop(s);
         push_timer.begin(s);
-        hipLaunchKernelGGL(k_leaf_push, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16)),
-                           dim3(PB), init_heap_bytes(PB, k) + (size_t)PB * kPushTL * 4, s, tree_indices, leaves,
-                           ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+        const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
+        if (early)
+            hipLaunchKernelGGL(k_leaf_push<16>, dim3(pgrid), dim3(PB), init_heap_bytes(PB, k) + (size_t)PB * 17 * 4, s,
+                               tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+        else
+            hipLaunchKernelGGL(k_leaf_push<32>, dim3(pgrid), dim3(PB), init_heap_bytes(PB, k) + (size_t)PB * 33 * 4, s,
+                               tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
         push_timer.stop(s);
         // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
         // per-pair parts (4 B of D written per ordered pair; 4 B of D and 4 B of partner id read) once it
