@@ -43,6 +43,59 @@ __device__ inline void wave_sync() {
 
 __device__ inline int lane_id() { return threadIdx.x & (RINND_WAVE - 1); }
 
+// Moves the k-entry neighbour-heap rows of a wave's 64 lanes between global memory (row r's slot j at
+// r * k + j) and LDS columns (slot j of column c at j * S + c).  Each row is handled by 16 lanes, so
+// every global access covers consecutive slots of four rows instead of one slot of 64 scattered rows;
+// the four rows of an access are 16 columns apart, which keeps the LDS side conflict-free for odd S.
+// row: this lane's global row, or -1 for none; col0: the wave's first column.  Every lane of the wave
+// calls it; LOAD ends, and the store starts, with a wave sync.
+template <bool LOAD>
+__device__ __forceinline__ void wave_heap_rows(int32_t row, int k, float* dist, int32_t* idx, uint8_t* flags,
+                                               float* hd, int32_t* hi, uint8_t* hf, int S, int col0) {
+    const int lane = threadIdx.x & 63, sub = lane & 15, g = lane >> 4;
+    if (!LOAD) wave_sync();
+    if (k <= 16) {
+#pragma unroll 4
+        for (int q = 0; q < 16; ++q) {
+            const int rr = q + 16 * g;
+            const int32_t rid = __shfl(row, rr);
+            if (rid >= 0 && sub < k) {
+                const int64_t gi = (int64_t)rid * k + sub;
+                const int li = sub * S + col0 + rr;
+                if (LOAD) {
+                    hd[li] = dist[gi];
+                    hi[li] = idx[gi];
+                    hf[li] = flags[gi];
+                } else {
+                    dist[gi] = hd[li];
+                    idx[gi] = hi[li];
+                    flags[gi] = hf[li];
+                }
+            }
+        }
+    } else {
+        for (int q = 0; q < 16; ++q) {
+            const int rr = q + 16 * g;
+            const int32_t rid = __shfl(row, rr);
+            if (rid < 0) continue;
+            for (int j = sub; j < k; j += 16) {
+                const int64_t gi = (int64_t)rid * k + j;
+                const int li = j * S + col0 + rr;
+                if (LOAD) {
+                    hd[li] = dist[gi];
+                    hi[li] = idx[gi];
+                    hf[li] = flags[gi];
+                } else {
+                    dist[gi] = hd[li];
+                    idx[gi] = hi[li];
+                    flags[gi] = hf[li];
+                }
+            }
+        }
+    }
+    if (LOAD) wave_sync();
+}
+
 // operand and accumulator vectors of the 16x16x16 bf16 MFMA
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));

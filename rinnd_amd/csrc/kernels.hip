@@ -1702,23 +1702,19 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
                                                        unsigned long long* changes) {
     extern __shared__ __align__(16) float hsm2[];
     const int B = blockDim.x, c = threadIdx.x;
+    const int S = B + 1;  // heap column stride (odd: the row staging is conflict-free)
     float* hd = hsm2;
-    int32_t* hi = (int32_t*)(hd + B * k);
-    uint8_t* hf = (uint8_t*)(hi + B * k);
+    int32_t* hi = (int32_t*)(hd + S * k);
+    uint8_t* hf = (uint8_t*)(hi + S * k);
     const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
     unsigned long long ch = 0;
     for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
         const int64_t e = total - 1 - (e0 + c);  // longest bins first: they start early instead of trailing
+        const int32_t v = e >= 0 ? order[e] : -1;
+        wave_heap_rows<true>(v, k, dist, idx, flags, hd, hi, hf, S, c & ~63);
         if (e >= 0) {
-            const int64_t v = order[e];
             const uint64_t o0 = off[v];
             const int64_t L = (int64_t)(off[v + 1] - o0);
-            const int64_t hb = v * k;
-            for (int j = 0; j < k; ++j) {
-                hd[j * B + c] = dist[hb + j];
-                hi[j * B + c] = idx[hb + j];
-                hf[j * B + c] = flags[hb + j];
-            }
             const uint64_t* R = rv + o0;
             float root = hd[c];
             int64_t r = 0;
@@ -1729,7 +1725,7 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const float d = __builtin_bit_cast(float, (uint32_t)(q[u] >> 32));
-                    if (d < root && heap_push_lane(hd, hi, hf, B, c, k, (int32_t)(uint32_t)q[u], d)) {
+                    if (d < root && heap_push_lane(hd, hi, hf, S, c, k, (int32_t)(uint32_t)q[u], d)) {
                         ++ch;
                         root = hd[c];
                     }
@@ -1738,17 +1734,13 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
             for (; r < L; ++r) {
                 const uint64_t q = R[r];
                 const float d = __builtin_bit_cast(float, (uint32_t)(q >> 32));
-                if (d < root && heap_push_lane(hd, hi, hf, B, c, k, (int32_t)(uint32_t)q, d)) {
+                if (d < root && heap_push_lane(hd, hi, hf, S, c, k, (int32_t)(uint32_t)q, d)) {
                     ++ch;
                     root = hd[c];
                 }
             }
-            for (int j = 0; j < k; ++j) {
-                dist[hb + j] = hd[j * B + c];
-                idx[hb + j] = hi[j * B + c];
-                flags[hb + j] = hf[j * B + c];
-            }
         }
+        wave_heap_rows<false>(v, k, dist, idx, flags, hd, hi, hf, S, c & ~63);
     }
     for (int o = 32; o > 0; o >>= 1) ch += __shfl_xor(ch, o);
     if ((threadIdx.x & 63) == 0 && ch) atomicAdd(changes, ch);
@@ -1778,7 +1770,7 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
     const int B = lanes_block(a.k, 9, 256);
-    const size_t lds = (size_t)round_up((int64_t)B * a.k * 2 + (B * a.k + 3) / 4, 4) * 4;
+    const size_t lds = (size_t)round_up((int64_t)(B + 1) * a.k * 2 + ((B + 1) * a.k + 3) / 4, 4) * 4;
     hipLaunchKernelGGL(k_heap_push_rec, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins, a.off, a.rval2,
                        a.idx, a.dist, a.flags, a.k, a.changes);
 }

@@ -282,19 +282,20 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
                                                    int32_t* idx, float* dist, uint8_t* flags, int k) {
     extern __shared__ __align__(16) float psm[];
     const int B = blockDim.x, c = threadIdx.x;
+    const int S = B + 1;  // heap column stride (odd: the row staging is conflict-free)
     const int lane = c & 63, w = c >> 6;
     float* hd = psm;
-    int32_t* hi = (int32_t*)(hd + B * k);
-    uint8_t* hf = (uint8_t*)(hi + B * k);
+    int32_t* hi = (int32_t*)(hd + S * k);
+    uint8_t* hf = (uint8_t*)(hi + S * k);
     constexpr int kPushCols = COLS, kPushTL = COLS + 1;  // padded tile row: lane-strided reads conflict-free
-    float* tile = psm + round_up((int64_t)B * k * 2 + (B * k + 3) / 4, 4) + w * 64 * kPushTL;
+    float* tile = psm + round_up((int64_t)S * k * 2 + (S * k + 3) / 4, 4) + w * 64 * kPushTL;
     constexpr int LPR = kPushCols / 4;  // loader lanes per row (one float4 each)
     constexpr int RPI = 64 / LPR;       // rows per load instruction
     const int rl = lane / LPR, q4 = (lane % LPR) * 4;  // loader role: rows rl + RPI i, columns q4..q4+3
     for (int64_t base = ((int64_t)blockIdx.x * B) + w * 64; base < n; base += (int64_t)gridDim.x * B) {
         const int64_t pos = base + lane;
         int m = 0, a = 0;
-        int64_t s = 0, hb = 0;
+        int64_t s = 0;
         if (pos < n) {
             const int64_t l = rank[pos] + mark[pos] - 1;  // leaf holding this position
             s = leaves[2 * l];
@@ -302,14 +303,8 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
             if (m > mcap || m < 2) m = 0;  // not a two-pass leaf
             a = (int)(pos - s);
         }
-        if (m) {
-            hb = (int64_t)tidx[pos] * k;
-            for (int j = 0; j < k; ++j) {
-                hd[j * B + c] = dist[hb + j];
-                hi[j * B + c] = idx[hb + j];
-                hf[j * B + c] = flags[hb + j];
-            }
-        }
+        const int32_t hrow = m ? tidx[pos] : -1;
+        wave_heap_rows<true>(hrow, k, dist, idx, flags, hd, hi, hf, S, w * 64);
         float root = m ? hd[c] : 0.f;
         int mw = m;  // the wave's longest row
         for (int o = 32; o > 0; o >>= 1) mw = max(mw, __shfl_xor(mw, o));
@@ -351,18 +346,13 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
                     msk &= msk - 1;
                     const float d = tile[lane * kPushTL + j];
                     if (d < root) {
-                        heap_push_col(hd, hi, hf, B, c, k, tidx[s + c0 + j], d);
+                        heap_push_col(hd, hi, hf, S, c, k, tidx[s + c0 + j], d);
                         root = hd[c];
                     }
                 }
             }
         }
-        if (m)
-            for (int j = 0; j < k; ++j) {
-                dist[hb + j] = hd[j * B + c];
-                idx[hb + j] = hi[j * B + c];
-                flags[hb + j] = hf[j * B + c];
-            }
+        wave_heap_rows<false>(hrow, k, dist, idx, flags, hd, hi, hf, S, w * 64);
     }
 }
 
@@ -410,8 +400,8 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
     const TilePlan P = tile_plan(ld, leaf_size);
     int untiled_cap = 0;  // leaves with m <= untiled_cap are done by the two-pass kernels
     int PB = 256;
-    while (PB > 64 && init_heap_bytes(PB, k) > 64 * 1024) PB -= 64;
-    if (P.mcap > 0 && ws.D && init_heap_bytes(PB, k) <= 64 * 1024) {
+    while (PB > 64 && init_heap_bytes(PB + 1, k) > 64 * 1024) PB -= 64;
+    if (P.mcap > 0 && ws.D && init_heap_bytes(PB + 1, k) <= 64 * 1024) {
         const bool mfma = ws.exact_bf16 && metric == RINND_DIST_SQEUCLIDEAN &&
                           leaf_dist_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024;
         dist_timer.begin(s);
@@ -428,10 +418,10 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
         push_timer.begin(s);
         const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
         if (early)
-            hipLaunchKernelGGL(k_leaf_push<16>, dim3(pgrid), dim3(PB), init_heap_bytes(PB, k) + (size_t)PB * 17 * 4, s,
+            hipLaunchKernelGGL(k_leaf_push<16>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 17 * 4, s,
                                tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
         else
-            hipLaunchKernelGGL(k_leaf_push<32>, dim3(pgrid), dim3(PB), init_heap_bytes(PB, k) + (size_t)PB * 33 * 4, s,
+            hipLaunchKernelGGL(k_leaf_push<32>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 33 * 4, s,
                                tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
         push_timer.stop(s);
         // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
