@@ -96,6 +96,24 @@ __device__ __forceinline__ void wave_heap_rows(int32_t row, int k, float* dist, 
     if (LOAD) wave_sync();
 }
 
+// The same for one array of rows with k entries (g: global rows, l: LDS columns).
+template <bool LOAD, typename T>
+__device__ __forceinline__ void wave_rows(int32_t row, int k, T* g, T* l, int S, int col0) {
+    const int lane = threadIdx.x & 63, sub = lane & 15, grp = lane >> 4;
+    if (!LOAD) wave_sync();
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+        const int rr = q + 16 * grp;
+        const int32_t rid = __shfl(row, rr);
+        if (rid < 0) continue;
+        for (int j = sub; j < k; j += 16) {
+            if (LOAD) l[j * S + col0 + rr] = g[(int64_t)rid * k + j];
+            else g[(int64_t)rid * k + j] = l[j * S + col0 + rr];
+        }
+    }
+    if (LOAD) wave_sync();
+}
+
 // operand and accumulator vectors of the 16x16x16 bf16 MFMA
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));

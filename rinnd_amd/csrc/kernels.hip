@@ -1291,65 +1291,80 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
                                                        int32_t* new_idx, int32_t* old_idx, int C) {
     extern __shared__ __align__(16) float csm[];
     const int B = blockDim.x, c = threadIdx.x;
+    const int S = B + 1;  // column stride (odd: the row stores are conflict-free)
     float* np = csm;
-    int32_t* ni = (int32_t*)(np + B * C);
-    float* op = (float*)(ni + B * C);
-    int32_t* oi = (int32_t*)(op + B * C);
+    int32_t* ni = (int32_t*)(np + S * C);
+    float* op = (float*)(ni + S * C);
+    int32_t* oi = (int32_t*)(op + S * C);
     const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
     for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
         const int64_t e = total - 1 - (e0 + c);  // longest bins first: they start early instead of trailing
-        if (e < 0) continue;
-        const int64_t v = order[e];
-        const uint64_t o0 = off[v], o1 = off[v + 1];
-        for (int j = 0; j < C; ++j) {
-            np[j * B + c] = __builtin_inff();
-            op[j * B + c] = __builtin_inff();
-            ni[j * B + c] = -1;
-            oi[j * B + c] = -1;
-        }
-        auto push = [&](uint32_t x, float pri) {
-            if ((int32_t)x < 0) cand_push_lane(np, ni, B, c, C, pri, (int32_t)(x & 0x7fffffffu));  // "new" edge
-            else cand_push_lane(op, oi, B, c, C, pri, (int32_t)x);
-        };
-        auto rev = [&](uint64_t a, uint64_t b) {  // a ring of four records in flight ahead of the push
-            if (a >= b) return;
-            uint64_t q0 = rv[a], q1 = rv[min(a + 1, b - 1)], q2 = rv[min(a + 2, b - 1)], q3 = rv[min(a + 3, b - 1)];
-            for (uint64_t r = a; r < b; ++r) {
-                const uint64_t w = q0;
-                q0 = q1;
-                q1 = q2;
-                q2 = q3;
-                q3 = rv[min(r + 4, b - 1)];
-                push((uint32_t)w, __builtin_bit_cast(float, (uint32_t)(w >> 32)));
+        const int32_t v = e >= 0 ? order[e] : -1;
+        if (e >= 0) {
+            const uint64_t o0 = off[v], o1 = off[v + 1];
+            for (int j = 0; j < C; ++j) {
+                np[j * S + c] = __builtin_inff();
+                op[j * S + c] = __builtin_inff();
+                ni[j * S + c] = -1;
+                oi[j * S + c] = -1;
             }
-        };
-        auto fwd = [&]() {
-            for (int j = 0; j < k; ++j) {
-                const int64_t f = v * k + j;
-                const int32_t nb = idx[f];
-                if (nb >= 0) push((uint32_t)nb | (flags[f] ? 0x80000000u : 0u), fpr[f]);
+            auto push = [&](uint32_t x, float pri) {
+                if ((int32_t)x < 0) cand_push_lane(np, ni, S, c, C, pri, (int32_t)(x & 0x7fffffffu));  // "new" edge
+                else cand_push_lane(op, oi, S, c, C, pri, (int32_t)x);
+            };
+            auto rev = [&](uint64_t a, uint64_t b) {  // a ring of four records in flight ahead of the push
+                if (a >= b) return;
+                uint64_t q0 = rv[a], q1 = rv[min(a + 1, b - 1)], q2 = rv[min(a + 2, b - 1)], q3 = rv[min(a + 3, b - 1)];
+                for (uint64_t r = a; r < b; ++r) {
+                    const uint64_t w = q0;
+                    q0 = q1;
+                    q1 = q2;
+                    q2 = q3;
+                    q3 = rv[min(r + 4, b - 1)];
+                    push((uint32_t)w, __builtin_bit_cast(float, (uint32_t)(w >> 32)));
+                }
+            };
+            auto fwd = [&]() {  // the row's own edges, with a ring of four entries in flight ahead of the push
+                const int64_t f0 = (int64_t)v * k;
+                int32_t i0 = idx[f0], i1 = idx[f0 + min(1, k - 1)], i2 = idx[f0 + min(2, k - 1)],
+                        i3 = idx[f0 + min(3, k - 1)];
+                uint8_t g0 = flags[f0], g1 = flags[f0 + min(1, k - 1)], g2 = flags[f0 + min(2, k - 1)],
+                        g3 = flags[f0 + min(3, k - 1)];
+                float p0 = fpr[f0], p1 = fpr[f0 + min(1, k - 1)], p2 = fpr[f0 + min(2, k - 1)],
+                      p3 = fpr[f0 + min(3, k - 1)];
+                for (int j = 0; j < k; ++j) {
+                    const int32_t nb = i0;
+                    const uint8_t fl = g0;
+                    const float pr = p0;
+                    i0 = i1, i1 = i2, i2 = i3;
+                    g0 = g1, g1 = g2, g2 = g3;
+                    p0 = p1, p1 = p2, p2 = p3;
+                    const int64_t fn = f0 + min(j + 4, k - 1);
+                    i3 = idx[fn];
+                    g3 = flags[fn];
+                    p3 = fpr[fn];
+                    if (nb >= 0) push((uint32_t)nb | (fl ? 0x80000000u : 0u), pr);
+                }
+            };
+            if (mode != 1) {  // key order = source row order; v's own row sits between sources < v and > v
+                const uint64_t pv = src_lower_bound(rv, o0, o1, v + 1);
+                rev(o0, pv);
+                fwd();
+                rev(pv, o1);
+            } else {  // own row block first (merged with v's row), then the other blocks ascending
+                const int64_t lo = (v / block_rows) * block_rows, hi = lo + block_rows;
+                const uint64_t pa = src_lower_bound(rv, o0, o1, lo);
+                const uint64_t pb = src_lower_bound(rv, pa, o1, hi);
+                const uint64_t pv = src_lower_bound(rv, pa, pb, v + 1);
+                rev(pa, pv);
+                fwd();
+                rev(pv, pb);
+                rev(o0, pa);
+                rev(pb, o1);
             }
-        };
-        if (mode != 1) {  // key order = source row order; v's own row sits between sources < v and > v
-            const uint64_t pv = src_lower_bound(rv, o0, o1, v + 1);
-            rev(o0, pv);
-            fwd();
-            rev(pv, o1);
-        } else {  // own row block first (merged with v's row), then the other blocks ascending
-            const int64_t lo = (v / block_rows) * block_rows, hi = lo + block_rows;
-            const uint64_t pa = src_lower_bound(rv, o0, o1, lo);
-            const uint64_t pb = src_lower_bound(rv, pa, o1, hi);
-            const uint64_t pv = src_lower_bound(rv, pa, pb, v + 1);
-            rev(pa, pv);
-            fwd();
-            rev(pv, pb);
-            rev(o0, pa);
-            rev(pb, o1);
         }
-        for (int j = 0; j < C; ++j) {
-            new_idx[v * C + j] = ni[j * B + c];
-            old_idx[v * C + j] = oi[j * B + c];
-        }
+        wave_rows<false>(v, C, new_idx, ni, S, c & ~63);
+        wave_rows<false>(v, C, old_idx, oi, S, c & ~63);
     }
 }
 
@@ -1413,7 +1428,7 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
     hipLaunchKernelGGL(k_bin_scatter2, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins,
                        a.order);
     const int B = lanes_block(a.C, 16, 128);
-    hipLaunchKernelGGL(k_cand_push_rev, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)B * a.C * 16, s, a.order,
+    hipLaunchKernelGGL(k_cand_push_rev, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)(B + 1) * a.C * 16, s, a.order,
                        a.bins, a.off, a.rval2, a.idx, a.flags, a.fpr, a.k, a.mode, a.block_rows, a.new_idx, a.old_idx,
                        a.C);
 }

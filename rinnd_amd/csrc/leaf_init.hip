@@ -341,14 +341,20 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
                     const int o = c0 + j;
                     if (o < m && o != a && tile[lane * kPushTL + j] < root) msk |= 1u << j;
                 }
-                while (msk) {
-                    const int j = __ffs((int)msk) - 1;
+                // the next survivor's id is loaded while the current one is pushed
+                int j = msk ? __ffs((int)msk) - 1 : -1;
+                int32_t xj = j >= 0 ? tidx[s + c0 + j] : 0;
+                while (j >= 0) {
                     msk &= msk - 1;
+                    const int jn = msk ? __ffs((int)msk) - 1 : -1;
+                    const int32_t xn = jn >= 0 ? tidx[s + c0 + jn] : 0;
                     const float d = tile[lane * kPushTL + j];
                     if (d < root) {
-                        heap_push_col(hd, hi, hf, S, c, k, tidx[s + c0 + j], d);
+                        heap_push_col(hd, hi, hf, S, c, k, xj, d);
                         root = hd[c];
                     }
+                    j = jn;
+                    xj = xn;
                 }
             }
         }
