@@ -3,18 +3,24 @@
 // Same tree algorithm as TreeBuilder (core/tree/builder.rs:172-348): a node is a leaf when
 // m <= leaf_size or depth >= max_depth; otherwise it splits by the bisector of two random members
 // (euclidean) or by the normalised difference of their directions (angular), a point goes left iff
-// its margin is < 0 (ties |margin| < 1e-8 decided by a random bit), and a degenerate split (all
+// its margin is < 0 (ties |margin| < 1e-8 decided by a random bit per point), and a degenerate split (all
 // points on one side) makes the node a leaf.  The unit normal is rounded to bf16 (the stored f32
 // hyperplane holds exactly those values, and a bf16 copy halves the bytes the side tests read);
 // the plane passes through the midpoint of the two members.  The random draws come from mix64(tree key, node id,
 // position) instead of a depth-first FastRng stream, so every node of a level is independent and
 // one level is processed for all trees at once:
 //   k_lvl_planes   one workgroup per active node: split points, hyperplane, offset (or leaf)
-//   k_lvl_margins  16 lanes per point, all trees: side of each tree's current hyperplane
-//   scan           exclusive prefix of the left flags over all (tree, position)
+//   k_lvl_margins  16 lanes per point, all trees: the point's entry in each tree advanced from the last
+//                  level's side and the children table, then the side of the tree's current hyperplane,
+//                  kept as one bit per (tree, point) in per-point 16-bit masks
+//   scan           exclusive prefix of the left flags over all (tree, position), the flags gathered
+//                  from the masks through each tree's order
 //   k_lvl_counts   left size per node; degenerate splits become leaves; children counted per tree
-//   k_lvl_assign   child node ids (breadth-first per tree) and the next level's active list
+//   k_lvl_assign   child node ids (breadth-first per tree), the next level's active list and each
+//                  entry's first child entry
 //   k_lvl_scatter  stable partition of every split node's positions
+// Dense levels write nothing indexed by position from the point-major side tests (no scattered writes);
+// sparse levels test position-major and write position-ordered flags.
 // Node ids are breadth-first per tree (root 0); leaves hold their [start, end) range of
 // tree.indices exactly as in the depth-first layout.
 #include <hipcub/hipcub.hpp>
@@ -157,41 +163,61 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
     if (threadIdx.x == 0 && act) atomicAdd(active_pairs, act);
 }
 
-// Left flag of every (tree, point) whose node splits at this level, written at the point's position
-// (flags of other positions are never read: a split node's range holds only its own members).
-// Point-major: 16 lanes per point.  First the lanes fetch, in parallel, the point's node, split flag,
-// offset and position for every tree (lane l owns trees l, l+16, ...); then the point's row, held in
+__device__ __forceinline__ uint32_t side_of(float margin, uint64_t key, int32_t node, int64_t point) {
+    if (fabsf(margin) < 1e-8f)  // ties: a random bit per (tree, node, point)
+        return (uint32_t)(mix64(key ^ mix64(((uint64_t)(uint32_t)node << 32) ^ (uint64_t)point)) & 1);
+    return margin < 0.0f ? 1u : 0u;
+}
+
+// The entry of point i in tree t at this level: its entry one level up (nodeOf), moved to the left or right
+// child entry by the side bit the last level left in the point's mask word (child_prev: first child entry
+// of every entry of the last level, -1 where the node did not split).  child_prev is null at level 0.
+// nodeOf is rewritten in place (the lane that owns (t, i) is its only reader and writer).
+__device__ __forceinline__ int32_t lvl_entry(int64_t n, const int32_t* __restrict__ child_prev, int32_t* nodeOf,
+                                             const uint16_t* mask16, int t, int64_t i) {
+    const int64_t ti = (int64_t)t * n + i;
+    int32_t e = nodeOf[ti];
+    if (child_prev && e >= 0) {
+        const int32_t c = child_prev[e];
+        const bool left = (mask16[(int64_t)(t >> 4) * n + i] >> (t & 15)) & 1u;
+        e = c >= 0 ? c + (left ? 0 : 1) : -1;
+        nodeOf[ti] = e;
+    }
+    return e;
+}
+
+// Side tests, point-major, 16 lanes per point (any dim).  First the lanes fetch, in parallel, the point's
+// entry, node and offset for every tree (lane l owns trees l, l+16, ...); then the point's row, held in
 // registers (CH float4 per lane, dim <= 64 * CH), is tested against the split trees' hyperplanes, four
-// trees per batch so their hyperplane loads are in flight together.
+// trees per batch so their hyperplane loads are in flight together.  The left bits of trees 16 j .. 16 j + 15
+// go to the point's mask word j.
 template <int CH>
-__global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* nodes, int n_trees,
-                                                     const int2* __restrict__ nodePos, const uint8_t* split,
-                                                     uint32_t* flags) {
+__global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
+                                                     const int32_t* __restrict__ child_prev, int32_t* nodeOf,
+                                                     uint16_t* mask16) {
     constexpr int kMaxT = 64;  // trees per point handled by the lane tables (4 per lane)
     const int gl = threadIdx.x & 15;
     const int gbase = threadIdx.x & 48;  // first lane of this 16-lane group within the wave
     const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 4;
     for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4; i < a.n; i += groups) {
         // per-lane tables for trees t = gl + 16 j
-        int32_t tnode[kMaxT / 16], tpos[kMaxT / 16];
+        int32_t tnode[kMaxT / 16];
         float toff[kMaxT / 16];
         uint32_t mask = 0;  // bit j: tree gl + 16 j splits this point's node
 #pragma unroll
         for (int j = 0; j < kMaxT / 16; ++j) {
             const int t = gl + 16 * j;
             tnode[j] = 0;
-            tpos[j] = 0;
             toff[j] = 0.f;
             if (t < n_trees) {
-                const int64_t ti = (int64_t)t * a.n + i;
-                const int2 np = nodePos[ti];
-                const int32_t e = np.x;
-                if (e >= 0 && split[e]) {
-                    const int32_t node = nodes[e].node;
-                    tnode[j] = node;
-                    tpos[j] = np.y;
-                    toff[j] = a.offsets[t][node];
-                    mask |= 1u << j;
+                const int32_t e = lvl_entry(a.n, child_prev, nodeOf, mask16, t, i);
+                if (e >= 0) {
+                    const PInfo pi = pinfo[e];
+                    if (pi.node >= 0) {
+                        tnode[j] = pi.node;
+                        toff[j] = pi.off;
+                        mask |= 1u << j;
+                    }
                 }
             }
         }
@@ -204,6 +230,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* n
                 xr[c] = d0 < a.dim ? *reinterpret_cast<const float4*>(x + d0) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
+        uint32_t lbits = 0;  // bit j: tree gl + 16 j sends the point left
         // walk the split trees of this point in batches of four
         for (int j = 0; j < kMaxT / 16; ++j) {
             // trees t = l + 16 j over the group's lanes l whose bit j is set
@@ -211,14 +238,10 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* n
             uint32_t gm = (uint32_t)((wm >> gbase) & 0xffffu);
             while (gm) {
                 int tl[4];
-                int cnt = 0;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     tl[u] = gm ? __ffs((int)gm) - 1 : -1;
-                    if (gm) {
-                        gm &= gm - 1;
-                        ++cnt;
-                    }
+                    if (gm) gm &= gm - 1;
                 }
                 float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -246,27 +269,18 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const LvlNode* n
                     for (int o = 8; o > 0; o >>= 1) acc[u] += __shfl_xor(acc[u], o);  // inside the 16-lane group
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (tl[u] != gl) continue;  // the lane that owns tree t writes its flag
+                    if (tl[u] != gl) continue;  // the lane that owns tree t keeps its side
                     const int t = gl + 16 * j;
-                    const float margin = acc[u] + toff[j];
-                    uint32_t left;
-                    if (fabsf(margin) < 1e-8f)
-                        left = (uint32_t)(mix64(a.keys[t] ^ mix64(((uint64_t)(uint32_t)tnode[j] << 32) ^
-                                                                  (uint64_t)tpos[j])) & 1);
-                    else
-                        left = margin < 0.0f ? 1u : 0u;
-                    flags[(int64_t)t * a.n + tpos[j]] = left;
+                    lbits |= side_of(acc[u] + toff[j], a.keys[t], tnode[j], i) << j;
                 }
-                (void)cnt;
             }
         }
+        for (int j = 0; 16 * j < n_trees; ++j) {  // words of batches where some tree splits this point's node
+            const uint32_t any = (uint32_t)((__ballot((mask >> j) & 1u) >> gbase) & 0xffffu);
+            const uint32_t bits = (uint32_t)((__ballot((lbits >> j) & 1u) >> gbase) & 0xffffu);
+            if (gl == 0 && any) mask16[(int64_t)j * a.n + i] = (uint16_t)bits;
+        }
     }
-}
-
-__device__ __forceinline__ uint32_t side_of(float margin, uint64_t key, int32_t node, int32_t pos) {
-    if (fabsf(margin) < 1e-8f)  // ties: a random bit per (tree, node, position)
-        return (uint32_t)(mix64(key ^ mix64(((uint64_t)(uint32_t)node << 32) ^ (uint64_t)(uint32_t)pos)) & 1);
-    return margin < 0.0f ? 1u : 0u;
 }
 
 template <int CTRL>
@@ -308,14 +322,15 @@ __device__ __forceinline__ float row16_sum(float v) {
 }
 
 // Dense levels, point-major: a 16-lane group (one DPP row) per point holds the point's row in registers
-// (lane l: dims 8 (l + 16 r) .. +8).  Trees go in batches of 16: lane u fetches tree u's nodePos word and
-// PInfo, the group broadcasts each tree's node from its lane (row_newbcast), reads that tree's bf16 plane
-// with one coalesced 16-byte load per lane, and reduces the 16 partial dot products with row rotations;
-// lane u keeps tree u's margin and writes its flag.  Needs dim % 8 == 0 and dim <= 128 R.
+// (lane l: dims 8 (l + 16 r) .. +8).  Trees go in batches of 16: lane u advances tree u's entry and
+// fetches its PInfo, the group broadcasts each tree's node from its lane (row_newbcast), reads that tree's
+// bf16 plane with one coalesced 16-byte load per lane, and reduces the 16 partial dot products with row
+// rotations; lane u keeps tree u's margin, and the batch's 16 sides become the point's mask word.
+// Needs dim % 8 == 0 and dim <= 128 R.
 template <int R>
 __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
-                                                        const int2* __restrict__ nodePos,
-                                                        uint32_t* __restrict__ flags) {
+                                                        const int32_t* __restrict__ child_prev, int32_t* nodeOf,
+                                                        uint16_t* mask16) {
     const int gl = threadIdx.x & 15;
     const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 4;
     const int64_t tstride = a.cap * a.dim;
@@ -334,18 +349,18 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
         }
         for (int tb = 0; tb < n_trees; tb += 16) {
             const int tl = tb + gl;
-            int32_t nodeL = -1, posL = 0;
+            int32_t nodeL = -1;
             float offL = 0.f;
             if (tl < n_trees) {
-                const int2 np = nodePos[(int64_t)tl * a.n + i];
-                if (np.x >= 0) {
-                    const PInfo pi = pinfo[np.x];
+                const int32_t e = lvl_entry(a.n, child_prev, nodeOf, mask16, tl, i);
+                if (e >= 0) {
+                    const PInfo pi = pinfo[e];
                     nodeL = pi.node;
                     offL = pi.off;
-                    posL = np.y;
                 }
             }
-            if (!((__ballot(nodeL >= 0) >> (threadIdx.x & 48)) & 0xffffull)) continue;  // group-uniform
+            // group-uniform: no tree of the batch splits this point's node (its mask word is not read again)
+            if (!((__ballot(nodeL >= 0) >> (threadIdx.x & 48)) & 0xffffull)) continue;
             float mine = 0.f;
             constexpr int UB = R >= 4 ? 2 : 8 / R;  // planes in flight per batch (8 for dim <= 128)
 #pragma unroll
@@ -383,14 +398,16 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
                     if (gl == ub + u) mine = acc;
                 }
             }
-            if (nodeL >= 0) flags[(int64_t)tl * a.n + posL] = side_of(mine + offL, a.keys[tl], nodeL, posL);
+            const uint32_t left = nodeL >= 0 ? side_of(mine + offL, a.keys[tl], nodeL, i) : 0u;
+            const uint64_t bits = __ballot(left != 0);
+            if (gl == 0) mask16[(int64_t)(tb >> 4) * a.n + i] = (uint16_t)(bits >> (threadIdx.x & 48));
         }
     }
 }
 
 // Sparse levels, position-major: a wave reads 64 consecutive (tree, position) items of the act table,
 // and its four 16-lane groups test the active ones four at a time: the member's row is gathered through
-// perm and the 16 lanes split the dimensions.  Needs dim % 4 == 0; nodePos is not used.
+// perm and the 16 lanes split the dimensions; flags are written at the positions.  Needs dim % 4 == 0.
 __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
                                                          const int32_t* const* act, const int32_t* const* perm,
                                                          uint32_t* __restrict__ flags) {
@@ -422,8 +439,10 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
             const int32_t nd = __shfl(node, sl), tt = __shfl(t, sl), ps = __shfl(pos, sl);
             const float of = __shfl(off, sl);
             float acc = 0.f;
+            int32_t pt = 0;
             if (src >= 0) {
-                const float* x = a.X + (int64_t)perm[tt][ps] * a.ld;
+                pt = perm[tt][ps];
+                const float* x = a.X + (int64_t)pt * a.ld;
                 const uint16_t* hb = a.hyper_bf[tt] + (int64_t)nd * a.dim;
                 for (int d0 = 4 * gl; d0 < a.dim; d0 += 64) {
                     const float4 xv = *reinterpret_cast<const float4*>(x + d0);
@@ -435,7 +454,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
                 }
             }
             for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // inside the 16-lane group
-            if (src >= 0 && gl == 0) flags[(int64_t)tt * a.n + ps] = side_of(acc + of, a.keys[tt], nd, ps);
+            if (src >= 0 && gl == 0) flags[(int64_t)tt * a.n + ps] = side_of(acc + of, a.keys[tt], nd, pt);
         }
     }
 }
@@ -486,10 +505,12 @@ __global__ void k_lvl_tree_base(int n_trees, const uint32_t* start, const uint32
     cum[n_trees] = acc;
 }
 
+// child_first[e]: the next level's entry of e's left child (the right one follows), -1 if e did not split
 __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* split, const uint32_t* cscan,
                              const uint32_t* leftsz, const int32_t* tree_base_id, const uint32_t* tree_cum,
-                             int32_t** children, LvlNode* next) {
+                             int32_t** children, LvlNode* next, int32_t* child_first) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
+        child_first[e] = split[e] ? (int32_t)cscan[e] : -1;
         if (!split[e]) continue;
         const LvlNode nd = nodes[e];
         const uint32_t s = cscan[e];  // the active list stays grouped by tree, so s - tree_cum is per tree
@@ -504,10 +525,12 @@ __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* sp
 
 // act values: >= 0 active entry; -2 became a leaf one level ago (the perm buffers already agree, the
 // act buffers are synced now); -1 settled in both buffers (never touched again).
+// The side of a position comes from its point's mask word (dense levels) or from the position-ordered
+// flags (sparse levels, mask16 null).
 __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, const uint8_t* split,
                               const uint32_t* cscan, const uint32_t* leftsz, const uint32_t* prefix,
-                              const uint32_t* flags, const int32_t* const* perm, int32_t* const* perm2,
-                              int32_t* const* act, int32_t* const* act2, int2* nodePos, int write_np) {
+                              const uint32_t* flags, const uint16_t* mask16, const int32_t* const* perm,
+                              int32_t* const* perm2, int32_t* const* act, int32_t* const* act2) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
          item += (int64_t)gridDim.x * blockDim.x) {
@@ -521,26 +544,36 @@ __global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, cons
             continue;
         }
         if (!split[e]) {  // became a leaf at this level
-            const int32_t pt = perm[t][pos];
-            perm2[t][pos] = pt;
+            perm2[t][pos] = perm[t][pos];
             act2[t][pos] = -2;
-            if (write_np) nodePos[(int64_t)t * n + pt] = make_int2(-1, (int32_t)pos);
             continue;
         }
         const LvlNode nd = nodes[e];
         const int64_t off = (int64_t)t * n;
         const uint32_t lr = prefix[off + pos] - prefix[off + nd.start];  // left members before pos
-        const bool left = flags[item] != 0;
-        const int64_t np = left ? nd.start + lr : nd.start + leftsz[e] + (pos - nd.start - lr);
         const int32_t pt = perm[t][pos];
+        const bool left = mask16 ? ((mask16[(int64_t)(t >> 4) * n + pt] >> (t & 15)) & 1u) != 0 : flags[item] != 0;
+        const int64_t np = left ? nd.start + lr : nd.start + leftsz[e] + (pos - nd.start - lr);
         const int32_t ne = (int32_t)cscan[e] + (left ? 0 : 1);
         perm2[t][np] = pt;
         act2[t][np] = ne;
-        if (write_np) nodePos[(int64_t)t * n + pt] = make_int2(ne, (int32_t)np);
     }
 }
 
-__global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act, int2* nodePos) {
+// Left flag of item (tree t, position pos) of a dense level: bit t % 16 of the mask word of the point at
+// that position; the item past the last is 0 (the scan's total).
+struct MaskFlag {
+    const uint16_t* mask16;
+    const int32_t* const* perm;
+    uint32_t n, total;
+    __host__ __device__ uint32_t operator()(uint32_t item) const {
+        if (item >= total) return 0u;
+        const uint32_t t = item / n, pos = item - t * n;
+        return (mask16[(size_t)(t >> 4) * n + (uint32_t)perm[t][pos]] >> (t & 15)) & 1u;
+    }
+};
+
+__global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act, int32_t* nodeOf) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
          item += (int64_t)gridDim.x * blockDim.x) {
@@ -548,21 +581,21 @@ __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t
         const int64_t pos = item - (int64_t)t * n;
         perm[t][pos] = (int32_t)pos;
         act[t][pos] = t;  // level 0: the root of tree t is active entry t
-        nodePos[item] = make_int2(t, (int32_t)pos);
+        nodeOf[item] = t;
     }
 }
 
-static void launch_margins_pm(hipStream_t s, const LvlArgs& a, const PInfo* pinfo, int T, const int2* nodePos,
-                              uint32_t* flags) {
+static void launch_margins_pm(hipStream_t s, const LvlArgs& a, const PInfo* pinfo, int T, const int32_t* child_prev,
+                              int32_t* nodeOf, uint16_t* mask16) {
     const unsigned grid = lgrid(a.n * 16, 256, 1 << 16);
     if (a.dim <= 128)
-        hipLaunchKernelGGL(k_lvl_margins_pm<1>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
+        hipLaunchKernelGGL(k_lvl_margins_pm<1>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
     else if (a.dim <= 256)
-        hipLaunchKernelGGL(k_lvl_margins_pm<2>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
+        hipLaunchKernelGGL(k_lvl_margins_pm<2>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
     else if (a.dim <= 512)
-        hipLaunchKernelGGL(k_lvl_margins_pm<4>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
+        hipLaunchKernelGGL(k_lvl_margins_pm<4>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
     else
-        hipLaunchKernelGGL(k_lvl_margins_pm<8>, dim3(grid), dim3(256), 0, s, a, pinfo, T, nodePos, flags);
+        hipLaunchKernelGGL(k_lvl_margins_pm<8>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
 }
 
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
@@ -577,7 +610,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     std::vector<int32_t*> perm_h(T), perm2_h(T), act_h(T), act2_h(T), ch_h(T);
     std::vector<float*> hp_h(T), of_h(T);
     WBuf<int32_t> perm2(W, kWsForest + 0, total), act(W, kWsForest + 1, total), act2(W, kWsForest + 2, total);
-    WBuf<int2> nodePos(W, kWsForest + 3, total);
+    WBuf<int32_t> nodeOf(W, kWsForest + 3, total);  // dense levels: each (tree, point)'s entry
+    const int NB = (T + 15) / 16;                      // mask words per point
+    WBuf<uint16_t> mask16(W, kWsForest + 27, (size_t)NB * n);
     WBuf<uint16_t> hyper_bf(W, kWsForest + 4, (size_t)T * cap * dim);
     for (int t = 0; t < T; ++t) {
         perm_h[t] = F.trees[t].indices.p;
@@ -617,8 +652,13 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
 
     WBuf<uint32_t> flags(W, kWsForest + 11, total + 1), prefix(W, kWsForest + 12, total + 1);
     HIP_TRY(hipMemsetAsync(flags.p, 0, (total + 1) * 4, s));  // only split nodes' positions are rewritten
-    size_t scan_bytes = 0;
+    using MaskIt = hipcub::TransformInputIterator<uint32_t, MaskFlag, hipcub::CountingInputIterator<uint32_t>>;
+    size_t scan_bytes = 0, mscan_bytes = 0;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags.p, prefix.p, (int)(total + 1)));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, mscan_bytes,
+                                             MaskIt(hipcub::CountingInputIterator<uint32_t>(0u), MaskFlag{}),
+                                             prefix.p, (int)(total + 1)));
+    scan_bytes = std::max(scan_bytes, mscan_bytes);
     WBuf<char> scan_tmp(W, kWsForest + 13, std::max<size_t>(scan_bytes, 16));
     // a level holds at most n nodes per tree (every active node has at least one member)
     const int64_t max_nodes = std::max<int64_t>(1, std::min<int64_t>(cap, n)) * (int64_t)T;
@@ -627,6 +667,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<PInfo> pinfo(W, kWsForest + 25, max_nodes + 1);
     WBuf<uint32_t> nchild(W, kWsForest + 17, max_nodes + 1), cscan(W, kWsForest + 18, max_nodes + 1),
         leftsz(W, kWsForest + 19, max_nodes + 1);
+    WBuf<int32_t> child_first(W, kWsForest + 28, max_nodes + 1);
     // level status read back once per level: [tested pairs so far (u64) | children per tree (u32 x T)]
     WBuf<unsigned long long> lstat(W, kWsForest + 21, 1 + (T + 1) / 2);
     unsigned long long* active_pairs = lstat.p;
@@ -645,7 +686,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     for (int t = 0; t < T; ++t) roots[t] = LvlNode{t, 0, 0, (int32_t)n, 0};
     HIP_TRY(hipMemcpyAsync(cur, roots.data(), T * sizeof(LvlNode), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T,
-                       nodePos.p);
+                       nodeOf.p);
     std::vector<int64_t> node_count(T, 1);
     {
         const std::vector<int32_t> ones(T, 1);
@@ -656,6 +697,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         HIP_TRY(hipStreamSynchronize(s));
     }
     std::vector<unsigned long long> lst(1 + (T + 1) / 2);
+    int level = 0;
     uint32_t* st_cur = d_start.p;  // first active entry of each tree (and the total), this level
     uint32_t* st_next = d_cum.p;   // the same for the next level
     int64_t na = T;
@@ -672,26 +714,34 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
                            (const int32_t* const*)P, split.p, pinfo.p, active_pairs);
         margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
+        const int32_t* child_prev = level > 0 ? child_first.p : nullptr;
         if (sparse)
             hipLaunchKernelGGL(k_lvl_margins_pos, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, a, pinfo.p, T,
                                (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags.p);
         else if (point_major_dpp)
-            launch_margins_pm(s, a, pinfo.p, T, nodePos.p, flags.p);
+            launch_margins_pm(s, a, pinfo.p, T, child_prev, nodeOf.p, mask16.p);
         else if (dim % 4 == 0 && dim <= 64)
-            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
-                               flags.p);
+            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+                               mask16.p);
         else if (dim % 4 == 0 && dim <= 128)
-            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
-                               flags.p);
+            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+                               mask16.p);
         else if (dim % 4 == 0 && dim <= 256)
-            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
-                               flags.p);
+            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+                               mask16.p);
         else
-            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, cur, T, nodePos.p, split.p,
-                               flags.p);
+            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+                               mask16.p);
         margins_timer.stop(s);
-        size_t b = scan_bytes;  // flags[total] stays 0 from the initial clear: the scan's total
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
+        size_t b = scan_bytes;
+        if (sparse)  // flags[total] stays 0 from the initial clear: the scan's total
+            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
+        else
+            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(
+                scan_tmp.p, b,
+                MaskIt(hipcub::CountingInputIterator<uint32_t>(0u),
+                       MaskFlag{mask16.p, (const int32_t* const*)P, (uint32_t)n, (uint32_t)total}),
+                prefix.p, (int)(total + 1), s));
         hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, prefix.p, split.p,
                            nchild.p, leftsz.p);
         size_t cb = cscan_bytes;
@@ -721,16 +771,17 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         }
         if (over) return false;
         hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
-                           leftsz.p, d_base.p, st_next, d_ch.p, nxt);
+                           leftsz.p, d_base.p, st_next, d_ch.p, nxt, child_first.p);
         hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur, T, n, split.p,
-                           cscan.p, leftsz.p, prefix.p, flags.p, (const int32_t* const*)P, P + T,
-                           P + 2 * T, P + 3 * T, nodePos.p, sparse_next ? 0 : 1);
+                           cscan.p, leftsz.p, prefix.p, flags.p, sparse ? nullptr : mask16.p,
+                           (const int32_t* const*)P, P + T, P + 2 * T, P + 3 * T);
         HIP_TRY(hipGetLastError());
         sparse = sparse_next;
         std::swap(cur, nxt);
         std::swap(st_cur, st_next);
         phase ^= 1;
         na = next_na;
+        ++level;
     }
     // after an odd number of levels the final order sits in the second buffer
     for (int t = 0; t < T; ++t) {

@@ -244,6 +244,33 @@ def test_nn_descent_cfg1_bit_exact(ctx):
         np.testing.assert_array_equal(gf, r["flags"])
 
 
+def test_counter_forest_large(ctx):
+    """A 70,000-point counter forest (dense point-major levels, then sparse ones) is a partition in the
+    caller's ids and deterministic, and the graph built on it finds the true neighbours about as well as
+    the reference algorithm, with the distances of the returned ids."""
+    n, d = 70000, 16
+    X = uniform(n, d, seed=9)
+    F = gpu_forest_counter(ctx, X, 5, 60, 13)
+    G = gpu_forest_counter(ctx, X, 5, 60, 13)
+    for t in (0, 4):
+        tr = F.tree(t, n)
+        assert check_tree_structure(X, tr, 60) == 0
+        np.testing.assert_array_equal(tr["indices"], G.tree(t, n)["indices"])
+    gi, gd, _, _, _, Fg = gpu_nn_descent(ctx, X, 10, 5, 60, 10, 8, rng_mode="counter")
+    assert check_tree_structure(X, Fg.tree(0, n), 60) == 0
+    rows = np.arange(0, n, 211)
+    X64 = X.astype(np.float64)
+    D = (X64[rows] ** 2).sum(1)[:, None] + (X64 ** 2).sum(1)[None, :] - 2.0 * X64[rows] @ X64.T
+    D[np.arange(len(rows)), rows] = np.inf
+    true = np.argsort(D, axis=1)[:, :10]
+    def recall(ids):
+        return np.mean([len(set(ids[r].tolist()) & set(true[i].tolist())) / 10 for i, r in enumerate(rows)])
+    ref = O.nn_descent(X, 10, 5, 60, 10, 8, 0.001, threads=8)["indices"]  # the reference algorithm's quality
+    assert recall(gi) >= recall(ref) - 0.03, (recall(gi), recall(ref))
+    dd = ((X64[rows, None, :] - X64[gi[rows]]) ** 2).sum(-1)
+    np.testing.assert_allclose(gd[rows], dd, rtol=1e-4, atol=1e-6)
+
+
 def test_counter_mode_quality(ctx):
     X = uniform(3000, 16, seed=5)
     gi, gd, gf, gs, _, _ = gpu_nn_descent(ctx, X, 10, 4, 30, 10, 8, rng_mode="counter")
