@@ -13,8 +13,9 @@
 //   k_lvl_margins  16 lanes per point, all trees: the point's entry in each tree advanced from the last
 //                  level's side and the children table, then the side of the tree's current hyperplane,
 //                  kept as one bit per (tree, point) in per-point 16-bit masks
-//   scan           exclusive prefix of the left flags over all (tree, position), the flags gathered
-//                  from the masks through each tree's order
+//   flag chunks    left flags in (tree, position) order as bytes (dense levels: gathered from the masks
+//                  through each tree's order) and their count per chunk of 256; an exclusive scan of the
+//                  chunk counts gives the prefix of any item with at most 256 bytes read
 //   k_lvl_counts   left size per node; degenerate splits become leaves; children counted per tree
 //   k_lvl_assign   child node ids (breadth-first per tree), the next level's active list and each
 //                  entry's first child entry
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
 // perm and the 16 lanes split the dimensions; flags are written at the positions.  Needs dim % 4 == 0.
 __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
                                                          const int32_t* const* act, const int32_t* const* perm,
-                                                         uint32_t* __restrict__ flags) {
+                                                         uint8_t* __restrict__ flags8) {
     const int lane = threadIdx.x & 63, gl = lane & 15, g = lane >> 4;
     const int64_t total = (int64_t)n_trees * a.n;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -454,22 +455,127 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
                 }
             }
             for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // inside the 16-lane group
-            if (src >= 0 && gl == 0) flags[(int64_t)tt * a.n + ps] = side_of(acc + of, a.keys[tt], nd, pt);
+            if (src >= 0 && gl == 0) flags8[(int64_t)tt * a.n + ps] = (uint8_t)side_of(acc + of, a.keys[tt], nd, pt);
         }
     }
 }
 
-__global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const uint32_t* prefix, uint8_t* split,
-                             uint32_t* nchild, uint32_t* leftsz) {
+constexpr int kChunk = 256;  // items per flag chunk: one wave, four consecutive items per lane
+
+// The four items item0 .. item0 + 3 of a lane as (tree, position); VEC4 (n % 4 == 0): one tree, and the
+// per-tree arrays are 16-byte aligned at pos.
+struct Item4 {
+    int t[4];
+    int64_t pos[4];
+    bool ok[4];
+};
+template <bool VEC4>
+__device__ __forceinline__ Item4 item4(int64_t item0, int64_t n, int64_t total) {
+    Item4 r;
+    if (VEC4) {
+        const int t = (int)(item0 / n);
+        const int64_t p = item0 - (int64_t)t * n;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            r.t[k] = t;
+            r.pos[k] = p + k;
+            r.ok[k] = item0 < total;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t it = item0 + k;
+            r.ok[k] = it < total;
+            r.t[k] = r.ok[k] ? (int)(it / n) : 0;
+            r.pos[k] = r.ok[k] ? it - (int64_t)r.t[k] * n : 0;
+        }
+    }
+    return r;
+}
+
+// Left flags (bytes 0/1, one per (tree, position) item) of one level and their count per chunk.  Dense
+// levels gather each item's side from the mask word of the point at that position; sparse levels find the
+// flags already written by the side tests (mask16 null).  Flags of positions outside split nodes are
+// stale, which is harmless: every prefix difference that is used spans one split node's items only.
+template <bool VEC4>
+__global__ __launch_bounds__(256) void k_lvl_flag_chunks(int n_trees, int64_t n, const uint16_t* __restrict__ mask16,
+                                                         const int32_t* const* perm, uint8_t* flags8,
+                                                         uint32_t* chunk_cnt) {
+    const int64_t total = (int64_t)n_trees * n;
+    const int64_t nchunks = ceil_div(total, (int64_t)kChunk);
+    const int lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) chunk_cnt[nchunks] = 0;  // the scan's total
+    for (int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunks; c += (int64_t)gridDim.x * 4) {
+        const int64_t item0 = c * kChunk + 4 * lane;
+        uint32_t f[4] = {0u, 0u, 0u, 0u};
+        if (mask16) {
+            const Item4 it = item4<VEC4>(item0, n, total);
+            int32_t pt[4] = {0, 0, 0, 0};
+            if (VEC4) {
+                if (it.ok[0]) {
+                    const int4 v = *reinterpret_cast<const int4*>(perm[it.t[0]] + it.pos[0]);
+                    pt[0] = v.x, pt[1] = v.y, pt[2] = v.z, pt[3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (it.ok[k]) pt[k] = perm[it.t[k]][it.pos[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (it.ok[k]) f[k] = (mask16[(int64_t)(it.t[k] >> 4) * n + pt[k]] >> (it.t[k] & 15)) & 1u;
+            if (item0 + 3 < total) {
+                *reinterpret_cast<uint32_t*>(flags8 + item0) = f[0] | f[1] << 8 | f[2] << 16 | f[3] << 24;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (item0 + k < total) flags8[item0 + k] = (uint8_t)f[k];
+            }
+        } else if (item0 + 3 < total) {
+            const uint32_t v = *reinterpret_cast<const uint32_t*>(flags8 + item0);
+            f[0] = v & 0xff, f[1] = (v >> 8) & 0xff, f[2] = (v >> 16) & 0xff, f[3] = v >> 24;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (item0 + k < total) f[k] = flags8[item0 + k];
+        }
+        uint32_t cnt = f[0] + f[1] + f[2] + f[3];
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        if (lane == 0) chunk_cnt[c] = cnt;
+    }
+}
+
+// Number of left flags among items [0, x): the chunk's offset plus the flags of the chunk before x.
+__device__ __forceinline__ uint32_t lvl_prefix(int64_t x, const uint32_t* __restrict__ chunk_off,
+                                               const uint8_t* __restrict__ flags8) {
+    const int64_t c = x / kChunk;
+    const int r = (int)(x - c * kChunk);
+    uint32_t p = chunk_off[c];
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(flags8 + c * kChunk);
+    int k = 0;
+    for (; k + 4 <= (r >> 2); k += 4) {  // bytes are 0 or 1: one set bit per left flag
+        const uint4 v = *reinterpret_cast<const uint4*>(w + k);
+        p += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+    }
+    for (; k < (r >> 2); ++k) p += __popc(w[k]);
+    if (r & 3) p += __popc(w[r >> 2] & ((1u << (8 * (r & 3))) - 1u));
+    return p;
+}
+
+__global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const uint32_t* chunk_off,
+                             const uint8_t* flags8, uint8_t* split, uint32_t* nchild, uint32_t* leftsz,
+                             uint32_t* lstart) {
     if (blockIdx.x == 0 && threadIdx.x == 0) nchild[na] = 0;  // the scan's total
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
         const LvlNode nd = nodes[e];
         uint32_t c = 0;
         if (split[e]) {
             const int64_t off = (int64_t)nd.tree * a.n;
-            const uint32_t L = prefix[off + nd.end] - prefix[off + nd.start];
+            const uint32_t p0 = lvl_prefix(off + nd.start, chunk_off, flags8);
+            const uint32_t L = lvl_prefix(off + nd.end, chunk_off, flags8) - p0;
             const uint32_t m = (uint32_t)(nd.end - nd.start);
             leftsz[e] = L;
+            lstart[e] = p0;
             if (L == 0 || L == m) {  // degenerate split -> leaf with a zero hyperplane (builder.rs:198-210)
                 split[e] = 0;
                 a.children[nd.tree][2 * nd.node] = -nd.start;
@@ -525,53 +631,75 @@ __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* sp
 
 // act values: >= 0 active entry; -2 became a leaf one level ago (the perm buffers already agree, the
 // act buffers are synced now); -1 settled in both buffers (never touched again).
-// The side of a position comes from its point's mask word (dense levels) or from the position-ordered
-// flags (sparse levels, mask16 null).
-__global__ void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, const uint8_t* split,
-                              const uint32_t* cscan, const uint32_t* leftsz, const uint32_t* prefix,
-                              const uint32_t* flags, const uint16_t* mask16, const int32_t* const* perm,
-                              int32_t* const* perm2, int32_t* const* act, int32_t* const* act2) {
+// One wave per chunk, four consecutive items per lane: the prefix of an item is its chunk's offset plus a
+// wave scan of the lanes' flag counts; lstart holds the prefix at each split node's first item.
+template <bool VEC4>
+__global__ __launch_bounds__(256) void k_lvl_scatter(const LvlNode* nodes, int n_trees, int64_t n, const uint8_t* split,
+                                                     const uint32_t* cscan, const uint32_t* leftsz,
+                                                     const uint32_t* lstart, const uint32_t* chunk_off,
+                                                     const uint8_t* flags8, const int32_t* const* perm,
+                                                     int32_t* const* perm2, int32_t* const* act,
+                                                     int32_t* const* act2) {
     const int64_t total = (int64_t)n_trees * n;
-    for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
-         item += (int64_t)gridDim.x * blockDim.x) {
-        const int t = (int)(item / n);
-        const int64_t pos = item - (int64_t)t * n;
-        const int32_t e = act[t][pos];
-        if (e == -1) continue;
-        if (e == -2) {
-            act[t][pos] = -1;
-            act2[t][pos] = -1;
-            continue;
+    const int64_t nchunks = ceil_div(total, (int64_t)kChunk);
+    const int lane = threadIdx.x & 63;
+    for (int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunks; c += (int64_t)gridDim.x * 4) {
+        const int64_t item0 = c * kChunk + 4 * lane;
+        const Item4 it = item4<VEC4>(item0, n, total);
+        uint32_t f[4] = {0u, 0u, 0u, 0u};
+        int32_t ev[4] = {-1, -1, -1, -1}, pv[4] = {0, 0, 0, 0};
+        if (VEC4) {
+            if (it.ok[0]) {
+                const uint32_t fv = *reinterpret_cast<const uint32_t*>(flags8 + item0);
+                const int4 av = *reinterpret_cast<const int4*>(act[it.t[0]] + it.pos[0]);
+                const int4 p4 = *reinterpret_cast<const int4*>(perm[it.t[0]] + it.pos[0]);
+                f[0] = fv & 0xff, f[1] = (fv >> 8) & 0xff, f[2] = (fv >> 16) & 0xff, f[3] = fv >> 24;
+                ev[0] = av.x, ev[1] = av.y, ev[2] = av.z, ev[3] = av.w;
+                pv[0] = p4.x, pv[1] = p4.y, pv[2] = p4.z, pv[3] = p4.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (it.ok[k]) {
+                    f[k] = flags8[item0 + k];
+                    ev[k] = act[it.t[k]][it.pos[k]];
+                    pv[k] = perm[it.t[k]][it.pos[k]];
+                }
         }
-        if (!split[e]) {  // became a leaf at this level
-            perm2[t][pos] = perm[t][pos];
-            act2[t][pos] = -2;
-            continue;
+        const uint32_t cnt = f[0] + f[1] + f[2] + f[3];
+        uint32_t incl = cnt;  // inclusive scan over the wave's lanes
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
         }
-        const LvlNode nd = nodes[e];
-        const int64_t off = (int64_t)t * n;
-        const uint32_t lr = prefix[off + pos] - prefix[off + nd.start];  // left members before pos
-        const int32_t pt = perm[t][pos];
-        const bool left = mask16 ? ((mask16[(int64_t)(t >> 4) * n + pt] >> (t & 15)) & 1u) != 0 : flags[item] != 0;
-        const int64_t np = left ? nd.start + lr : nd.start + leftsz[e] + (pos - nd.start - lr);
-        const int32_t ne = (int32_t)cscan[e] + (left ? 0 : 1);
-        perm2[t][np] = pt;
-        act2[t][np] = ne;
+        uint32_t before = chunk_off[c] + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t e = ev[k];
+            const int t = it.t[k];
+            const int64_t pos = it.pos[k];
+            const uint32_t fk = f[k];
+            const uint32_t b = before;
+            before += fk;
+            if (e == -1) continue;  // also the items past total
+            if (e == -2) {
+                act[t][pos] = -1;
+                act2[t][pos] = -1;
+                continue;
+            }
+            if (!split[e]) {  // became a leaf at this level
+                perm2[t][pos] = pv[k];
+                act2[t][pos] = -2;
+                continue;
+            }
+            const LvlNode nd = nodes[e];
+            const uint32_t lr = b - lstart[e];  // left members before pos
+            const int64_t np = fk ? nd.start + lr : nd.start + leftsz[e] + (pos - nd.start - lr);
+            perm2[t][np] = pv[k];
+            act2[t][np] = (int32_t)cscan[e] + (fk ? 0 : 1);
+        }
     }
 }
-
-// Left flag of item (tree t, position pos) of a dense level: bit t % 16 of the mask word of the point at
-// that position; the item past the last is 0 (the scan's total).
-struct MaskFlag {
-    const uint16_t* mask16;
-    const int32_t* const* perm;
-    uint32_t n, total;
-    __host__ __device__ uint32_t operator()(uint32_t item) const {
-        if (item >= total) return 0u;
-        const uint32_t t = item / n, pos = item - t * n;
-        return (mask16[(size_t)(t >> 4) * n + (uint32_t)perm[t][pos]] >> (t & 15)) & 1u;
-    }
-};
 
 __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act, int32_t* nodeOf) {
     const int64_t total = (int64_t)n_trees * n;
@@ -650,15 +778,15 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     const LvlArgs a{X, ld, dim, n, std::max<int64_t>(leaf_size, 1), max_depth, angular,
                     d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p, T, hyper_bf.p, cap};
 
-    WBuf<uint32_t> flags(W, kWsForest + 11, total + 1), prefix(W, kWsForest + 12, total + 1);
-    HIP_TRY(hipMemsetAsync(flags.p, 0, (total + 1) * 4, s));  // only split nodes' positions are rewritten
-    using MaskIt = hipcub::TransformInputIterator<uint32_t, MaskFlag, hipcub::CountingInputIterator<uint32_t>>;
-    size_t scan_bytes = 0, mscan_bytes = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags.p, prefix.p, (int)(total + 1)));
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, mscan_bytes,
-                                             MaskIt(hipcub::CountingInputIterator<uint32_t>(0u), MaskFlag{}),
-                                             prefix.p, (int)(total + 1)));
-    scan_bytes = std::max(scan_bytes, mscan_bytes);
+    const int64_t nchunks = ceil_div(total, (int64_t)kChunk);
+    WBuf<uint8_t> flags8(W, kWsForest + 11, (size_t)nchunks * kChunk);
+    // four consecutive items per lane in one tree, 16-byte aligned in every per-tree array
+    bool vec4 = n % 4 == 0 && ((uintptr_t)perm2.p & 15) == 0 && ((uintptr_t)act.p & 15) == 0 &&
+                ((uintptr_t)act2.p & 15) == 0;
+    for (int t = 0; t < T; ++t) vec4 = vec4 && ((uintptr_t)perm_h[t] & 15) == 0;
+    WBuf<uint32_t> chunk_cnt(W, kWsForest + 12, nchunks + 1), chunk_off(W, kWsForest + 29, nchunks + 1);
+    size_t scan_bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, chunk_cnt.p, chunk_off.p, (int)(nchunks + 1)));
     WBuf<char> scan_tmp(W, kWsForest + 13, std::max<size_t>(scan_bytes, 16));
     // a level holds at most n nodes per tree (every active node has at least one member)
     const int64_t max_nodes = std::max<int64_t>(1, std::min<int64_t>(cap, n)) * (int64_t)T;
@@ -668,6 +796,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<uint32_t> nchild(W, kWsForest + 17, max_nodes + 1), cscan(W, kWsForest + 18, max_nodes + 1),
         leftsz(W, kWsForest + 19, max_nodes + 1);
     WBuf<int32_t> child_first(W, kWsForest + 28, max_nodes + 1);
+    WBuf<uint32_t> lstart(W, kWsForest + 30, max_nodes + 1);
     // level status read back once per level: [tested pairs so far (u64) | children per tree (u32 x T)]
     WBuf<unsigned long long> lstat(W, kWsForest + 21, 1 + (T + 1) / 2);
     unsigned long long* active_pairs = lstat.p;
@@ -717,7 +846,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         const int32_t* child_prev = level > 0 ? child_first.p : nullptr;
         if (sparse)
             hipLaunchKernelGGL(k_lvl_margins_pos, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, a, pinfo.p, T,
-                               (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags.p);
+                               (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags8.p);
         else if (point_major_dpp)
             launch_margins_pm(s, a, pinfo.p, T, child_prev, nodeOf.p, mask16.p);
         else if (dim % 4 == 0 && dim <= 64)
@@ -733,17 +862,17 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
             hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
                                mask16.p);
         margins_timer.stop(s);
-        size_t b = scan_bytes;
-        if (sparse)  // flags[total] stays 0 from the initial clear: the scan's total
-            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, flags.p, prefix.p, (int)(total + 1), s));
+        const unsigned cgrid = lgrid(nchunks, 4, 1 << 16);  // four chunks (waves) per workgroup
+        if (vec4)
+            hipLaunchKernelGGL(k_lvl_flag_chunks<true>, dim3(cgrid), dim3(256), 0, s, T, n,
+                               sparse ? nullptr : mask16.p, (const int32_t* const*)P, flags8.p, chunk_cnt.p);
         else
-            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(
-                scan_tmp.p, b,
-                MaskIt(hipcub::CountingInputIterator<uint32_t>(0u),
-                       MaskFlag{mask16.p, (const int32_t* const*)P, (uint32_t)n, (uint32_t)total}),
-                prefix.p, (int)(total + 1), s));
-        hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, prefix.p, split.p,
-                           nchild.p, leftsz.p);
+            hipLaunchKernelGGL(k_lvl_flag_chunks<false>, dim3(cgrid), dim3(256), 0, s, T, n,
+                               sparse ? nullptr : mask16.p, (const int32_t* const*)P, flags8.p, chunk_cnt.p);
+        size_t b = scan_bytes;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, chunk_cnt.p, chunk_off.p, (int)(nchunks + 1), s));
+        hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, chunk_off.p, flags8.p,
+                           split.p, nchild.p, leftsz.p, lstart.p);
         size_t cb = cscan_bytes;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
         hipLaunchKernelGGL(k_lvl_tree_base, dim3(1), dim3(64), 0, s, T, st_cur, cscan.p, tree_children, d_count.p,
@@ -772,9 +901,14 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         if (over) return false;
         hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
                            leftsz.p, d_base.p, st_next, d_ch.p, nxt, child_first.p);
-        hipLaunchKernelGGL(k_lvl_scatter, dim3(lgrid(total, 256)), dim3(256), 0, s, cur, T, n, split.p,
-                           cscan.p, leftsz.p, prefix.p, flags.p, sparse ? nullptr : mask16.p,
-                           (const int32_t* const*)P, P + T, P + 2 * T, P + 3 * T);
+        if (vec4)
+            hipLaunchKernelGGL(k_lvl_scatter<true>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
+                               leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
+                               P + 3 * T);
+        else
+            hipLaunchKernelGGL(k_lvl_scatter<false>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
+                               leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
+                               P + 3 * T);
         HIP_TRY(hipGetLastError());
         sparse = sparse_next;
         std::swap(cur, nxt);

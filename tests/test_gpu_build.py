@@ -140,8 +140,8 @@ def check_tree_structure(X, tr, leaf, max_depth=200):
 
 
 @pytest.mark.parametrize("angular", [False, True])
-@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (3000, 36, 30), (20000, 128, 60), (6000, 200, 40),
-                                      (3000, 600, 50)])
+@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (3000, 36, 30), (4001, 24, 25), (20000, 128, 60),
+                                      (6000, 200, 40), (3000, 600, 50)])
 def test_counter_forest_structure(ctx, n, d, leaf, angular):
     X = uniform(n, d, seed=n + d + 1)
     if angular:
