@@ -303,9 +303,11 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         size_t lt_bytes = std::max(scan_u64_tmp_bytes(n + 2), (size_t)16);
         WBuf<char> ltmp(W, 11, lt_bytes);
         WBuf<float> dscr(W, 12, leaf_init_scratch_floats(n, Dld, p.leaf_size));
-        WBuf<unsigned long long> lpairs(W, 13, 1);
-        HIP_TRY(hipMemsetAsync(lpairs.p, 0, 8, s));
-        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, use_bf16 && B16.exact ? &B16 : nullptr};
+        WBuf<unsigned long long> lpairs(W, 13, 2);
+        HIP_TRY(hipMemsetAsync(lpairs.p, 0, 16, s));
+        const bool exact16 = use_bf16 && B16.exact;
+        WBuf<uint32_t> smask(W, 39, exact16 ? (size_t)n * 8 : 1);
+        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, exact16 ? &B16 : nullptr, exact16 ? smask.p : nullptr};
         // deferred timings: no wait per tree
         const KTimer dist_timer{nullptr, nullptr, stats ? &st.timed[RINND_TIMED_LEAF_DIST] : nullptr, &tlog};
         const KTimer push_timer{nullptr, nullptr, stats ? &st.timed[RINND_TIMED_LEAF_PUSH] : nullptr, &tlog};
@@ -316,8 +318,11 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
                              w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer,
                              t < 3 /* the first trees: most pushes are accepted */);
         }
-        if (stats) {  // per-pair bytes: D written (4 B per ordered pair), D and the partner id read (8 B)
-            const double pairs = (double)d2h_u64(s, lpairs.p);
+        if (stats) {  // per-pair bytes: D written (4 B per ordered pair or survivor), D and the partner id read
+            unsigned long long lp[2];
+            HIP_TRY(hipMemcpyAsync(lp, lpairs.p, 16, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            const double pairs = (double)(lp[0] + lp[1]);
             st.timed[RINND_TIMED_LEAF_DIST].bytes += pairs * 4.0;
             st.timed[RINND_TIMED_LEAF_PUSH].bytes += pairs * 8.0;
         }

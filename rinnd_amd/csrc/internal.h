@@ -240,8 +240,9 @@ struct LeafInitWs {
     const int32_t* mark;
     const int64_t* rank;
     float* D;
-    unsigned long long* pairs;  // 1 counter: ordered pairs m(m-1) of the two-pass leaves (accumulated)
+    unsigned long long* pairs;  // 2 counters: ordered pairs m(m-1) written whole, survivor pairs (accumulated)
     const Bf16Rows* exact_bf16; // exact-mode bf16 rows: pair distances on MFMA (null: FP32 kernels)
+    uint32_t* surv_mask;        // n x 8 survivor column masks (null: every tree writes whole leaves)
 };
 bool leaf_init_supported(int k, int64_t leaf_size);
 int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size);
@@ -378,6 +379,12 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
 
 // leaf pair distances on MFMA (exact mode only), same D layout as the FP32 two-pass kernel
 size_t leaf_dist_mfma_lds(int64_t ldb, int mcap);
+size_t leaf_surv_mfma_lds(int64_t ldb, int mcap);
+// survivor pairs only (pairs below the destination's heap root at the start of the tree): per position a
+// column mask (8 words) and the surviving distances in column order at the start of its D row
+void launch_leaf_surv_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
+                           const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, const float* hdist, int k,
+                           float* D, int64_t Dl4, uint32_t* Msk, unsigned long long* surv);
 void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
                            const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, float* D, int64_t Dl4,
                            unsigned long long* pairs);

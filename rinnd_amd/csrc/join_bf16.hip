@@ -345,6 +345,158 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
     if (t == 0 && np) atomicAdd(pairs, np);
 }
 
+// The same pair distances for the trees after the first few, writing only the pairs that can enter a heap:
+// a push of d onto p's heap changes it only if d < p's root, and the roots only fall while a tree's pushes
+// are applied, so the pairs with d >= p's root at the start of the tree are dropped for p.  Every position
+// gets a mask of its surviving columns (Msk, 8 words, bit o = member o) and its surviving distances in
+// column order at the start of its D row.  A wave owns 16 rows at a time, keeps their bf16 operands in
+// registers and walks the column blocks in order, so each survivor's rank is a running count (the full
+// square is computed: in exact mode G(a, o) and G(o, a) are the same exact integer).  surv counts the
+// survivors.
+constexpr int kSurvMaxK = 16;  // 16-wide k steps held in registers (ldb <= 256)
+template <int NK>
+__global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restrict__ Xb, int64_t ldb,
+                                                        const float* __restrict__ nrm, const int32_t* __restrict__ tidx,
+                                                        const int64_t* __restrict__ leaves,
+                                                        const int64_t* __restrict__ n_leaves_dev, int mcap,
+                                                        const float* __restrict__ hdist, int k, float* __restrict__ D,
+                                                        int64_t Dl4, uint32_t* __restrict__ Msk,
+                                                        unsigned long long* surv) {
+    extern __shared__ __align__(16) uint16_t ssm16[];
+    const int B = blockDim.x, t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63, nw = B >> 6;
+    const int lds_ld = 16 * NK + 8;  // bf16 elements per staged row (16-byte aligned, skewed banks)
+    const int mp16 = (int)round_up((int64_t)mcap, (int64_t)16);
+    float* nr = (float*)(ssm16 + (int64_t)mp16 * lds_ld);
+    float* rt = nr + mp16;  // each member's heap root
+    int32_t* rid = (int32_t*)(rt + mp16);
+    const int64_t n_leaves = *n_leaves_dev;
+    const int mrow = lane & 15, kq = (lane >> 4) * 4, kqi = lane >> 4;
+    unsigned long long ns = 0;
+    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
+        const int64_t s = leaves[2 * leaf];
+        const int m = (int)(leaves[2 * leaf + 1] - s);
+        if (m > mcap || m < 2) continue;  // uniform
+        const int mp = round_up(m, 16);
+        constexpr int v8 = 2 * NK;  // 16-byte chunks per row
+        for (int i = t; i < mp; i += B) {
+            const int32_t id = i < m ? tidx[s + i] : -1;
+            rid[i] = id;
+            nr[i] = id >= 0 ? nrm[id] : 0.f;
+            rt[i] = id >= 0 ? hdist[(int64_t)id * k] : 0.f;
+        }
+        __syncthreads();
+        for (int i0 = 0; i0 < mp * v8; i0 += 8 * B) {  // eight 16-byte row loads in flight per thread
+            uint4 v[8];
+            bool pad[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = min(i0 + u * B + t, mp * v8 - 1);
+                const int r = i / v8, e = i - r * v8;
+                const int32_t id = rid[r];
+                pad[u] = id < 0;  // rows m..mp-1 are zero
+                v[u] = *reinterpret_cast<const uint4*>(Xb + (int64_t)(id < 0 ? 0 : id) * ldb + e * 8);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + u * B + t;
+                if (i < mp * v8) {
+                    const int r = i / v8, e = i - r * v8;
+                    *reinterpret_cast<uint4*>(ssm16 + r * lds_ld + e * 8) = pad[u] ? make_uint4(0, 0, 0, 0) : v[u];
+                }
+            }
+        }
+        __syncthreads();
+        const int nb = mp >> 4;
+        for (int R = w; R < nb; R += nw) {  // wave-uniform: rows 16 R .. 16 R + 15
+            v4s areg[NK];
+            const uint16_t* ra = ssm16 + (16 * R + mrow) * lds_ld + kq;
+#pragma unroll
+            for (int kk = 0; kk < NK; ++kk) areg[kk] = *reinterpret_cast<const v4s*>(ra + 16 * kk);
+            float nx[4], rx[4];
+            uint32_t cnt[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                nx[r] = nr[16 * R + kq + r];
+                rx[r] = rt[16 * R + kq + r];
+            }
+            for (int C = 0; C < nb; C += 2) {  // two column blocks per step: independent MFMA chains
+                const int C1 = C + 1 < nb ? C + 1 : C;  // an odd last block repeats C; its results are unused
+                const uint16_t* rb0 = ssm16 + (16 * C + mrow) * lds_ld + kq;
+                const uint16_t* rb1 = ssm16 + (16 * C1 + mrow) * lds_ld + kq;
+                v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kk = 0; kk < NK; ++kk) {
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(areg[kk], *reinterpret_cast<const v4s*>(rb0 + 16 * kk),
+                                                                     acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(areg[kk], *reinterpret_cast<const v4s*>(rb1 + 16 * kk),
+                                                                     acc1, 0, 0, 0);
+                }
+                uint32_t mw[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {  // blocks C, C + 1 in order: ranks are running counts
+                    const int Ch = C + h;
+                    if (Ch >= nb) break;  // wave-uniform
+                    const int o = 16 * Ch + mrow;  // D[row][col] layout: row 16 R + kq + r, column o
+                    const float no = nr[o];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int x = 16 * R + kq + r;
+                        const float d = (nx[r] + no) - 2.0f * (h ? acc1[r] : acc0[r]);
+                        const bool sv = x < m && o < m && o != x && d < rx[r];
+                        const uint32_t half = (uint32_t)(__ballot(sv) >> (16 * kqi)) & 0xffffu;
+                        if (sv) D[(s + x) * Dl4 + cnt[r] + __popc(half & ((1u << mrow) - 1u))] = d;
+                        cnt[r] += __popc(half);
+                        mw[r] |= half << (16 * h);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {  // mask word C / 2 of row x
+                    const int x = 16 * R + kq + r;
+                    if (mrow == 0 && x < m) Msk[(s + x) * 8 + (C >> 1)] = mw[r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = 16 * R + kq + r;
+                if (mrow == 0 && x < m) {
+                    for (int j = (nb + 1) >> 1; j < 8; ++j) Msk[(s + x) * 8 + j] = 0u;  // columns past the leaf
+                    ns += cnt[r];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o);
+    if (lane == 0 && ns) atomicAdd(surv, ns);
+}
+
+size_t leaf_surv_mfma_lds(int64_t ldb, int mcap) {
+    if (ldb > 16 * kSurvMaxK) return SIZE_MAX;  // operands held in registers
+    const size_t mp16 = (size_t)round_up((int64_t)mcap, (int64_t)16);
+    return mp16 * (ldb + 8) * 2 + mp16 * 12;
+}
+
+void launch_leaf_surv_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
+                           const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, const float* hdist, int k,
+                           float* D, int64_t Dl4, uint32_t* Msk, unsigned long long* surv) {
+    const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(max_leaves, 1), 4096);
+    const size_t lds = leaf_surv_mfma_lds(R.ldb, mcap);
+#define RINND_SURV_CASE(NKV)                                                                                     \
+    case NKV:                                                                                                    \
+        hipLaunchKernelGGL(k_leaf_surv_mfma<NKV>, dim3(grid), dim3(256), lds, s, R.Xb.p, R.ldb, R.nrm.p,          \
+                           tree_indices, leaves, n_leaves_dev, mcap, hdist, k, D, Dl4, Msk, surv);               \
+        break;
+    switch (R.ldb / 16) {
+        RINND_SURV_CASE(1) RINND_SURV_CASE(2) RINND_SURV_CASE(3) RINND_SURV_CASE(4) RINND_SURV_CASE(5)
+        RINND_SURV_CASE(6) RINND_SURV_CASE(7) RINND_SURV_CASE(8) RINND_SURV_CASE(9) RINND_SURV_CASE(10)
+        RINND_SURV_CASE(11) RINND_SURV_CASE(12) RINND_SURV_CASE(13) RINND_SURV_CASE(14) RINND_SURV_CASE(15)
+        RINND_SURV_CASE(16)
+        default: throw Error{RINND_ERR_INVALID_ARG, "survivor leaf kernel: row width above 256"};
+    }
+#undef RINND_SURV_CASE
+}
+
 size_t leaf_dist_mfma_lds(int64_t ldb, int mcap) {
     return (size_t)round_up((int64_t)mcap, (int64_t)16) * (ldb + 8) * 2 + (size_t)round_up((int64_t)mcap, (int64_t)16) * 8;
 }

@@ -362,6 +362,68 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
     }
 }
 
+// The pushes of a survivor tree: each position's column mask and its surviving distances (in column
+// order at the start of its D row) from k_leaf_surv_mfma; the lane walks them in leaf order, the next
+// survivor's distance and partner id in flight while the current one is pushed.
+__global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restrict__ tidx,
+                                                        const int64_t* __restrict__ leaves,
+                                                        const int32_t* __restrict__ mark,
+                                                        const int64_t* __restrict__ rank, int64_t n, int mcap,
+                                                        const float* __restrict__ D, int64_t Dl4,
+                                                        const uint32_t* __restrict__ Msk, int32_t* idx, float* dist,
+                                                        uint8_t* flags, int k) {
+    extern __shared__ __align__(16) float qsm[];
+    const int B = blockDim.x, c = threadIdx.x;
+    const int S = B + 1;  // heap column stride (odd: the row staging is conflict-free)
+    const int lane = c & 63, w = c >> 6;
+    float* hd = qsm;
+    int32_t* hi = (int32_t*)(hd + S * k);
+    uint8_t* hf = (uint8_t*)(hi + S * k);
+    for (int64_t base = ((int64_t)blockIdx.x * B) + w * 64; base < n; base += (int64_t)gridDim.x * B) {
+        const int64_t pos = base + lane;
+        int m = 0;
+        int64_t s = 0;
+        if (pos < n) {
+            const int64_t l = rank[pos] + mark[pos] - 1;  // leaf holding this position
+            s = leaves[2 * l];
+            m = (int)(leaves[2 * l + 1] - s);
+            if (m > mcap || m < 2) m = 0;  // not a two-pass leaf
+        }
+        const int32_t hrow = m ? tidx[pos] : -1;
+        wave_heap_rows<true>(hrow, k, dist, idx, flags, hd, hi, hf, S, w * 64);
+        if (m) {
+            float root = hd[c];
+            const uint4 m0 = *reinterpret_cast<const uint4*>(Msk + pos * 8);
+            const uint4 m1 = *reinterpret_cast<const uint4*>(Msk + pos * 8 + 4);
+            const uint32_t words[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+            const float* Dr = D + pos * Dl4;
+            int r = 0;
+            float dn = Dr[0];  // the next survivor's distance (a valid read even with no survivor)
+#pragma unroll
+            for (int wi = 0; wi < 8; ++wi) {
+                uint32_t bits = words[wi];
+                int j = bits ? __ffs((int)bits) - 1 : -1;
+                int32_t xj = j >= 0 ? tidx[s + 32 * wi + j] : 0;
+                while (j >= 0) {
+                    bits &= bits - 1;
+                    const int jn = bits ? __ffs((int)bits) - 1 : -1;
+                    const int32_t xn = jn >= 0 ? tidx[s + 32 * wi + jn] : 0;
+                    const float d = dn;
+                    ++r;
+                    dn = Dr[r < Dl4 ? r : 0];
+                    if (d < root) {
+                        heap_push_col(hd, hi, hf, S, c, k, xj, d);
+                        root = hd[c];
+                    }
+                    j = jn;
+                    xj = xn;
+                }
+            }
+        }
+        wave_heap_rows<false>(hrow, k, dist, idx, flags, hd, hi, hf, S, w * 64);
+    }
+}
+
 struct TilePlan {
     int mcap = 0;
     int64_t Dl4 = 0;
@@ -410,34 +472,53 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
     if (P.mcap > 0 && ws.D && init_heap_bytes(PB + 1, k) <= 64 * 1024) {
         const bool mfma = ws.exact_bf16 && metric == RINND_DIST_SQEUCLIDEAN &&
                           leaf_dist_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024;
-        dist_timer.begin(s);
-        if (mfma)
-            launch_leaf_dist_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, ws.D,
-                                  P.Dl4, ws.pairs);
-        else if (metric == RINND_DIST_SQEUCLIDEAN)
-            hipLaunchKernelGGL(k_leaf_dist_sq, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, X, ld,
-                               dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
-        else
-            hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X,
-                               ld, dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
-        dist_timer.stop(s);
-        push_timer.begin(s);
-        const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
-        if (early)
-            hipLaunchKernelGGL(k_leaf_push<16>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 17 * 4, s,
-                               tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
-        else
-            hipLaunchKernelGGL(k_leaf_push<32>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 33 * 4, s,
-                               tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
-        push_timer.stop(s);
-        // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
-        // per-pair parts (4 B of D written per ordered pair; 4 B of D and 4 B of partner id read) once it
-        // has read the pair counter
-        // dist: each member's row (bf16 + norm on MFMA) and id once
-        dist_timer.collect((double)n * ((mfma ? ws.exact_bf16->ldb * 2.0 + 4.0 : dim * 4.0) + 4.0));
-        // push: every member's heap read and written, its leaf lookup
-        push_timer.collect((double)n * k * 9.0 * 2.0 + n * 40.0);
-        untiled_cap = P.mcap;
+        // after the first trees most pairs cannot enter a heap: survivors only
+        const bool surv = mfma && !early && ws.surv_mask && leaf_surv_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024;
+        if (surv) {
+            dist_timer.begin(s);
+            launch_leaf_surv_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, dist, k,
+                                  ws.D, P.Dl4, ws.surv_mask, ws.pairs + 1);
+            dist_timer.stop(s);
+            push_timer.begin(s);
+            const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
+            hipLaunchKernelGGL(k_leaf_push_surv, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s, tree_indices,
+                               leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, ws.surv_mask, idx, dist, flags, k);
+            push_timer.stop(s);
+            // per row: bf16 row + norm + root + id read, mask written (dist); heap, leaf lookup, mask (push);
+            // the per-survivor bytes are added by the caller from the survivor counter
+            dist_timer.collect((double)n * (ws.exact_bf16->ldb * 2.0 + 12.0 + 32.0));
+            push_timer.collect((double)n * k * 9.0 * 2.0 + n * (40.0 + 32.0));
+            untiled_cap = P.mcap;
+        } else {
+            dist_timer.begin(s);
+            if (mfma)
+                launch_leaf_dist_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, ws.D,
+                                      P.Dl4, ws.pairs);
+            else if (metric == RINND_DIST_SQEUCLIDEAN)
+                hipLaunchKernelGGL(k_leaf_dist_sq, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, X, ld,
+                                   dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
+            else
+                hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X,
+                                   ld, dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
+            dist_timer.stop(s);
+            push_timer.begin(s);
+            const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
+            if (early)
+                hipLaunchKernelGGL(k_leaf_push<16>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 17 * 4, s,
+                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+            else
+                hipLaunchKernelGGL(k_leaf_push<32>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 33 * 4, s,
+                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+            push_timer.stop(s);
+            // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
+            // per-pair parts (4 B of D written per ordered pair; 4 B of D and 4 B of partner id read) once it
+            // has read the pair counter
+            // dist: each member's row (bf16 + norm on MFMA) and id once
+            dist_timer.collect((double)n * ((mfma ? ws.exact_bf16->ldb * 2.0 + 4.0 : dim * 4.0) + 4.0));
+            // push: every member's heap read and written, its leaf lookup
+            push_timer.collect((double)n * k * 9.0 * 2.0 + n * 40.0);
+            untiled_cap = P.mcap;
+        }
     } else {
         const int64_t cap_rows = ((int64_t)kInitLdsBudget - (int64_t)heap) / (ldl * 4);
         untiled_cap = (int)std::max<int64_t>(0, std::min<int64_t>(cap_rows, std::max<int64_t>(leaf_size, 1)));

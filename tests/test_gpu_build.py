@@ -317,15 +317,16 @@ def _build(ctx, X, k, trees, leaf, C_, iters, rng_mode, precision):
 @pytest.mark.parametrize("rng_mode", ["reference", "counter"])
 @pytest.mark.parametrize("leaf,k", [(256, 15), (100, 30), (17, 8)])
 def test_bf16_exact_leaf_init_equals_fp32(ctx, rng_mode, leaf, k):
-    """Integer data (exact in bf16): the MFMA leaf distances and the BF16 join give the FP32 path's heaps,
-    flags and change counts bit for bit, for leaves of up to 256 members."""
+    """Integer data (exact in bf16): the MFMA leaf distances (whole leaves for the first trees, survivor
+    pairs only after them) and the BF16 join give the FP32 path's heaps, flags and change counts bit for
+    bit, for leaves of up to 256 members."""
     rng = np.random.default_rng(leaf)
     n, d = 6000, 128
     centres = rng.random((30, d)) * 100
     X = np.clip(np.floor(centres[rng.integers(0, 30, n)] + 6 * rng.standard_normal((n, d)) + 0.5), 0, 255)
     X = X.astype(np.float32)
-    a = _build(ctx, X, k, 4, leaf, k, 4, rng_mode, "fp32")
-    b = _build(ctx, X, k, 4, leaf, k, 4, rng_mode, "bf16")
+    a = _build(ctx, X, k, 7, leaf, k, 4, rng_mode, "fp32")
+    b = _build(ctx, X, k, 7, leaf, k, 4, rng_mode, "bf16")
     assert b[3]["changes"] == a[3]["changes"]
     np.testing.assert_array_equal(b[0], a[0])
     np.testing.assert_array_equal(b[1].view(np.uint32), a[1].view(np.uint32))
