@@ -379,30 +379,31 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
         if (m > mcap || m < 2) continue;  // uniform
         const int mp = round_up(m, 16);
         constexpr int v8 = 2 * NK;  // 16-byte chunks per row
-        for (int i = t; i < mp; i += B) {
-            const int32_t id = i < m ? tidx[s + i] : -1;
-            rid[i] = id;
-            nr[i] = id >= 0 ? nrm[id] : 0.f;
-            rt[i] = id >= 0 ? hdist[(int64_t)id * k] : 0.f;
-        }
+        for (int i = t; i < mp; i += B) rid[i] = i < m ? tidx[s + i] : -1;
         __syncthreads();
-        for (int i0 = 0; i0 < mp * v8; i0 += 8 * B) {  // eight 16-byte row loads in flight per thread
-            uint4 v[8];
-            bool pad[8];
+        // then the rows (sixteen 16-byte loads in flight per thread), norms and roots in one round trip
+        for (int i0 = 0; i0 < mp * v8; i0 += 16 * B) {
+            uint4 v[16];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 16; ++u) {
                 const int i = min(i0 + u * B + t, mp * v8 - 1);
                 const int r = i / v8, e = i - r * v8;
                 const int32_t id = rid[r];
-                pad[u] = id < 0;  // rows m..mp-1 are zero
                 v[u] = *reinterpret_cast<const uint4*>(Xb + (int64_t)(id < 0 ? 0 : id) * ldb + e * 8);
             }
+            if (i0 == 0)
+                for (int i = t; i < mp; i += B) {
+                    const int32_t id = rid[i];
+                    nr[i] = id >= 0 ? nrm[id] : 0.f;
+                    rt[i] = id >= 0 ? hdist[(int64_t)id * k] : 0.f;
+                }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 16; ++u) {
                 const int i = i0 + u * B + t;
                 if (i < mp * v8) {
                     const int r = i / v8, e = i - r * v8;
-                    *reinterpret_cast<uint4*>(ssm16 + r * lds_ld + e * 8) = pad[u] ? make_uint4(0, 0, 0, 0) : v[u];
+                    *reinterpret_cast<uint4*>(ssm16 + r * lds_ld + e * 8) =
+                        rid[r] < 0 ? make_uint4(0, 0, 0, 0) : v[u];  // rows m..mp-1 are zero
                 }
             }
         }
