@@ -479,14 +479,21 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             ca.old_idx = old_idx.p;
             launch_cand_build(s, ca);
         }
+        // update_iteration: generation into per-row regions, grouping by destination, ordered apply.
+        // The pair count sizes the join's region buffer; it is read while the flag marking and the
+        // candidate statistics run.
+        launch_row_pairs(s, new_idx.p, old_idx.p, 0, n, C, rpc.p, rowoff.p, PL.scan_tmp.p, PL.scan_bytes);
+        uint64_t* hst = (uint64_t*)W.status.ensure(16);
+        const hipEvent_t hev = W.status_ev.get();
+        HIP_TRY(hipMemcpyAsync(hst, rowoff.p + n, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipEventRecord(hev, s));
         launch_mark_old(s, w_idx, w_flags, new_idx.p, n, k, C);
         HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
         launch_cand_stats(s, new_idx.p, old_idx.p, n, C, cstat.p);
-        // update_iteration: generation into per-row regions, grouping by destination, ordered apply
         HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
         HIP_TRY(hipMemsetAsync(rescored.p, 0, 8, s));
-        launch_row_pairs(s, new_idx.p, old_idx.p, 0, n, C, rpc.p, rowoff.p, PL.scan_tmp.p, PL.scan_bytes);
-        JU = WBuf<JRec>(W, 33, std::max<uint64_t>(d2h_u64(s, rowoff.p + n), 1));
+        HIP_TRY(hipEventSynchronize(hev));
+        JU = WBuf<JRec>(W, 33, std::max<uint64_t>(hst[0], 1));
         const JoinOut jo{rowoff.p, JU.p, kc.p, PL.cnt.p};
         PL.begin(s);
         HIP_TRY(hipEventRecord(ev[4], s));
@@ -497,7 +504,19 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         HIP_TRY(hipEventRecord(ev[5], s));
         scan_u64_exclusive(s, PL.cnt.p, PL.off.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
         scan_u64_exclusive(s, kc.p, koff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
-        const uint64_t n_rec = d2h_u64(s, PL.off.p + n);
+        // the record count sizes the sort; it is read while the destinations are binned
+        HIP_TRY(hipMemcpyAsync(hst + 1, PL.off.p + n, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipEventRecord(hev, s));
+        {
+            JoinApplyArgs jb{};
+            jb.n = n;
+            jb.off = PL.off.p;
+            jb.order = PL.big_list.p;
+            jb.bins = PL.bins.p;
+            launch_join_bins(s, jb);
+        }
+        HIP_TRY(hipEventSynchronize(hev));
+        const uint64_t n_rec = hst[1];
         {
             const uint64_t items = std::max<uint64_t>(n_rec, 1);
             jr_key = WBuf<uint32_t>(W, 34, items);

@@ -825,7 +825,10 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         HIP_TRY(hipMemcpyAsync(d_start.p, first.data(), (T + 1) * 4, hipMemcpyHostToDevice, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
-    std::vector<unsigned long long> lst(1 + (T + 1) / 2);
+    // the level status is copied to pinned memory and read while assign and scatter run
+    const size_t lst_bytes = (1 + (T + 1) / 2) * sizeof(unsigned long long);
+    unsigned long long* lst = (unsigned long long*)W.status.ensure(lst_bytes);
+    const hipEvent_t lst_ev = W.status_ev.get();
     int level = 0;
     uint32_t* st_cur = d_start.p;  // first active entry of each tree (and the total), this level
     uint32_t* st_next = d_cum.p;   // the same for the next level
@@ -877,10 +880,23 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
         hipLaunchKernelGGL(k_lvl_tree_base, dim3(1), dim3(64), 0, s, T, st_cur, cscan.p, tree_children, d_count.p,
                            d_base.p, st_next);
-        HIP_TRY(hipMemcpyAsync(lst.data(), lstat.p, lst.size() * 8, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpyAsync(lst, lstat.p, lst_bytes, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipEventRecord(lst_ev, s));
+        // children ids, the next active list and the partition need nothing from the host
+        hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
+                           leftsz.p, d_base.p, st_next, d_ch.p, nxt, child_first.p);
+        if (vec4)
+            hipLaunchKernelGGL(k_lvl_scatter<true>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
+                               leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
+                               P + 3 * T);
+        else
+            hipLaunchKernelGGL(k_lvl_scatter<false>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
+                               leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
+                               P + 3 * T);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(lst_ev));
         const unsigned long long pairs_now = lst[0];
-        const unsigned* tc = reinterpret_cast<const unsigned*>(lst.data() + 1);
+        const unsigned* tc = reinterpret_cast<const unsigned*>(lst + 1);
         bool sparse_next = sparse;
         {
             // algorithmic bytes: every point row once, each split node's hyperplane once, and per tested
@@ -898,18 +914,10 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
             next_na += tc[t];
             if (node_count[t] > cap) over = true;
         }
-        if (over) return false;
-        hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
-                           leftsz.p, d_base.p, st_next, d_ch.p, nxt, child_first.p);
-        if (vec4)
-            hipLaunchKernelGGL(k_lvl_scatter<true>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
-                               leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
-                               P + 3 * T);
-        else
-            hipLaunchKernelGGL(k_lvl_scatter<false>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
-                               leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
-                               P + 3 * T);
-        HIP_TRY(hipGetLastError());
+        if (over) {  // the next level's nodes would not fit: the kernels above stayed inside this level's
+            HIP_TRY(hipStreamSynchronize(s));
+            return false;
+        }
         sparse = sparse_next;
         std::swap(cur, nxt);
         std::swap(st_cur, st_next);

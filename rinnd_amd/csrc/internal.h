@@ -363,7 +363,8 @@ struct JoinApplyArgs {
     unsigned long long* changes;
 };
 size_t join_sort_tmp_bytes(int64_t items, int end_bit);
-void launch_join_apply(hipStream_t s, const JoinApplyArgs& a);
+void launch_join_bins(hipStream_t s, const JoinApplyArgs& a);   // needs off, n, bins, order only
+void launch_join_apply(hipStream_t s, const JoinApplyArgs& a);  // after launch_join_bins
 
 // local join, BF16 rows on MFMA with FP32 re-evaluation (join_bf16.hip)
 struct Bf16Rows {
@@ -465,10 +466,48 @@ struct Pipeline {
 // Grow-only device buffers a context keeps between builds: allocating and releasing the multi-GB
 // scratch of a build would otherwise stall every call.  Each use site owns a fixed slot; contents are
 // never assumed to survive from one build to the next.
+// Pinned host memory for small device -> host status reads that run while later kernels are queued.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* ensure(size_t b) {
+        if (b > bytes) {
+            if (p) HIP_TRY(hipHostFree(p));
+            p = nullptr;
+            HIP_TRY(hipHostMalloc(&p, b, hipHostMallocDefault));
+            bytes = b;
+        }
+        return p;
+    }
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+// An event that is created on first use and destroyed with its owner.
+struct OwnedEvent {
+    hipEvent_t e = nullptr;
+    hipEvent_t get() {
+        if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    }
+    OwnedEvent() = default;
+    OwnedEvent(const OwnedEvent&) = delete;
+    OwnedEvent& operator=(const OwnedEvent&) = delete;
+    ~OwnedEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
 struct Workspace {
     std::vector<DBuf<char>> slots;
     Bf16Rows b16;
     std::unique_ptr<Pipeline> pipeline;
+    PinnedBuf status;      // device -> host status words
+    OwnedEvent status_ev;  // marks the status copy
     void* get(int slot, size_t bytes) {
         if ((int)slots.size() <= slot) slots.resize(slot + 1);
         slots[slot].ensure(std::max<size_t>(bytes, 16));

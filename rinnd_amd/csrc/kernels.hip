@@ -1768,6 +1768,15 @@ size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
     return b;
 }
 
+// Destinations binned by record count, longest first (needs only the per-destination offsets).
+void launch_join_bins(hipStream_t s, const JoinApplyArgs& a) {
+    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.n, a.bins);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
+}
+
+// The records of the kept pairs, sorted by destination, applied in generation order (after launch_join_bins).
 void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     const int C2 = 2 * a.C, S = a.C * C2, W = (S + 63) >> 6;
     const size_t per_wave = (size_t)round_up(W * 8 + S * 4 + C2 * 4, 16);
@@ -1780,10 +1789,6 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
         HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2,
                                                    (int)a.n_records, 0, end_bit, s));
     }
-    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.n, a.bins);
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
     const int B = lanes_block(a.k, 9, 256);
     const size_t lds = (size_t)round_up((int64_t)(B + 1) * a.k * 2 + ((B + 1) * a.k + 3) / 4, 4) * 4;
     hipLaunchKernelGGL(k_heap_push_rec, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins, a.off, a.rval2,
