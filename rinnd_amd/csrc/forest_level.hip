@@ -31,13 +31,22 @@
 namespace rinnd {
 
 struct LvlNode {
-    int32_t tree, node, start, end, depth;
+    int32_t tree, node, start, end, depth, parent;  // parent: the entry one level up (-1 for the roots)
 };
 
 // What the side tests of an active entry read: its node id (-1 if it does not split) and offset.
 struct PInfo {
     int32_t node;
     float off;
+};
+// The same for the two children of an entry of the last level (left child first), with the child's entry
+// (-1 for both when the parent did not split), so that a point moves to its next entry and finds its next
+// node with one 16-byte load.
+struct CInfo {
+    int32_t node;
+    float off;
+    int32_t entry;
+    int32_t pad;
 };
 
 struct LvlArgs {
@@ -83,7 +92,7 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
 
 __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* nodes, int64_t na,
                                                     const int32_t* const* perm, uint8_t* split, PInfo* pinfo,
-                                                    unsigned long long* active_pairs) {
+                                                    CInfo* cinfo, unsigned long long* active_pairs) {
     __shared__ float red[4];
     unsigned long long act = 0;  // members of split nodes: the (tree, point) pairs tested this level
     for (int64_t e = blockIdx.x; e < na; e += gridDim.x) {
@@ -96,6 +105,7 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
                 ch[2 * nd.node + 1] = -nd.end;
                 split[e] = 0;
                 pinfo[e] = PInfo{-1, 0.f};
+                if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{-1, 0.f, (int32_t)e, 0};
             }
             continue;  // uniform across the block
         }
@@ -131,6 +141,7 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
             if (threadIdx.x == 0) {
                 a.offsets[nd.tree][nd.node] = scale ? -dm : 0.f;
                 pinfo[e] = PInfo{nd.node, scale ? -dm : 0.f};
+                if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{nd.node, scale ? -dm : 0.f, (int32_t)e, 0};
             }
         } else {
             float n1 = 0.f, n2 = 0.f;
@@ -156,6 +167,7 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
             if (threadIdx.x == 0) {
                 a.offsets[nd.tree][nd.node] = 0.f;
                 pinfo[e] = PInfo{nd.node, 0.f};
+                if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{nd.node, 0.f, (int32_t)e, 0};
             }
         }
         if (threadIdx.x == 0) split[e] = 1;
@@ -170,21 +182,21 @@ __device__ __forceinline__ uint32_t side_of(float margin, uint64_t key, int32_t 
     return margin < 0.0f ? 1u : 0u;
 }
 
-// The entry of point i in tree t at this level: its entry one level up (nodeOf), moved to the left or right
-// child entry by the side bit the last level left in the point's mask word (child_prev: first child entry
-// of every entry of the last level, -1 where the node did not split).  child_prev is null at level 0.
-// nodeOf is rewritten in place (the lane that owns (t, i) is its only reader and writer).
-__device__ __forceinline__ int32_t lvl_entry(int64_t n, const int32_t* __restrict__ child_prev, int32_t* nodeOf,
-                                             const uint16_t* mask16, int t, int64_t i) {
+// The side-test info of point i in tree t at this level.  At level 0 (cinfo null) its entry is tree t's root
+// and the info that entry's PInfo.  Later, its entry one level up (nodeOf) and the side bit the last level
+// left in the point's mask word choose one of that entry's two CInfo (children entries are consecutive, the
+// left one first); the point's entry moves to the child's.  nodeOf is rewritten in place (the lane that
+// owns (t, i) is its only reader and writer).
+__device__ __forceinline__ PInfo lvl_info(int64_t n, const PInfo* __restrict__ pinfo, const CInfo* __restrict__ cinfo,
+                                          int32_t* nodeOf, const uint16_t* mask16, int t, int64_t i) {
     const int64_t ti = (int64_t)t * n + i;
-    int32_t e = nodeOf[ti];
-    if (child_prev && e >= 0) {
-        const int32_t c = child_prev[e];
-        const bool left = (mask16[(int64_t)(t >> 4) * n + i] >> (t & 15)) & 1u;
-        e = c >= 0 ? c + (left ? 0 : 1) : -1;
-        nodeOf[ti] = e;
-    }
-    return e;
+    const int32_t e = nodeOf[ti];
+    if (e < 0) return PInfo{-1, 0.f};
+    if (!cinfo) return pinfo[e];
+    const bool left = (mask16[(int64_t)(t >> 4) * n + i] >> (t & 15)) & 1u;
+    const CInfo c = cinfo[2 * (int64_t)e + (left ? 0 : 1)];
+    nodeOf[ti] = c.entry;
+    return PInfo{c.entry >= 0 ? c.node : -1, c.off};
 }
 
 // Side tests, point-major, 16 lanes per point (any dim).  First the lanes fetch, in parallel, the point's
@@ -194,7 +206,7 @@ __device__ __forceinline__ int32_t lvl_entry(int64_t n, const int32_t* __restric
 // go to the point's mask word j.
 template <int CH>
 __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
-                                                     const int32_t* __restrict__ child_prev, int32_t* nodeOf,
+                                                     const CInfo* __restrict__ cinfo, int32_t* nodeOf,
                                                      uint16_t* mask16) {
     constexpr int kMaxT = 64;  // trees per point handled by the lane tables (4 per lane)
     const int gl = threadIdx.x & 15;
@@ -211,14 +223,11 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const PInfo* __r
             tnode[j] = 0;
             toff[j] = 0.f;
             if (t < n_trees) {
-                const int32_t e = lvl_entry(a.n, child_prev, nodeOf, mask16, t, i);
-                if (e >= 0) {
-                    const PInfo pi = pinfo[e];
-                    if (pi.node >= 0) {
-                        tnode[j] = pi.node;
-                        toff[j] = pi.off;
-                        mask |= 1u << j;
-                    }
+                const PInfo pi = lvl_info(a.n, pinfo, cinfo, nodeOf, mask16, t, i);
+                if (pi.node >= 0) {
+                    tnode[j] = pi.node;
+                    toff[j] = pi.off;
+                    mask |= 1u << j;
                 }
             }
         }
@@ -330,7 +339,7 @@ __device__ __forceinline__ float row16_sum(float v) {
 // Needs dim % 8 == 0 and dim <= 128 R.
 template <int R>
 __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
-                                                        const int32_t* __restrict__ child_prev, int32_t* nodeOf,
+                                                        const CInfo* __restrict__ cinfo, int32_t* nodeOf,
                                                         uint16_t* mask16) {
     const int gl = threadIdx.x & 15;
     const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 4;
@@ -353,12 +362,9 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
             int32_t nodeL = -1;
             float offL = 0.f;
             if (tl < n_trees) {
-                const int32_t e = lvl_entry(a.n, child_prev, nodeOf, mask16, tl, i);
-                if (e >= 0) {
-                    const PInfo pi = pinfo[e];
-                    nodeL = pi.node;
-                    offL = pi.off;
-                }
+                const PInfo pi = lvl_info(a.n, pinfo, cinfo, nodeOf, mask16, tl, i);
+                nodeL = pi.node;
+                offL = pi.off;
             }
             // group-uniform: no tree of the batch splits this point's node (its mask word is not read again)
             if (!((__ballot(nodeL >= 0) >> (threadIdx.x & 48)) & 0xffffull)) continue;
@@ -611,21 +617,25 @@ __global__ void k_lvl_tree_base(int n_trees, const uint32_t* start, const uint32
     cum[n_trees] = acc;
 }
 
-// child_first[e]: the next level's entry of e's left child (the right one follows), -1 if e did not split
+// An entry that does not split gets its two CInfo marked (no child entry); the children of the others
+// fill theirs in the next level's k_lvl_planes.
 __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* split, const uint32_t* cscan,
                              const uint32_t* leftsz, const int32_t* tree_base_id, const uint32_t* tree_cum,
-                             int32_t** children, LvlNode* next, int32_t* child_first) {
+                             int32_t** children, LvlNode* next, CInfo* cinfo) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
-        child_first[e] = split[e] ? (int32_t)cscan[e] : -1;
-        if (!split[e]) continue;
+        if (!split[e]) {
+            cinfo[2 * e] = CInfo{-1, 0.f, -1, 0};
+            cinfo[2 * e + 1] = CInfo{-1, 0.f, -1, 0};
+            continue;
+        }
         const LvlNode nd = nodes[e];
         const uint32_t s = cscan[e];  // the active list stays grouped by tree, so s - tree_cum is per tree
         const int32_t left = tree_base_id[nd.tree] + (int32_t)(s - tree_cum[nd.tree]);
         children[nd.tree][2 * nd.node] = left;
         children[nd.tree][2 * nd.node + 1] = left + 1;
         const int32_t mid = nd.start + (int32_t)leftsz[e];
-        next[s] = LvlNode{nd.tree, left, nd.start, mid, nd.depth + 1};
-        next[s + 1] = LvlNode{nd.tree, left + 1, mid, nd.end, nd.depth + 1};
+        next[s] = LvlNode{nd.tree, left, nd.start, mid, nd.depth + 1, (int32_t)e};
+        next[s + 1] = LvlNode{nd.tree, left + 1, mid, nd.end, nd.depth + 1, (int32_t)e};
     }
 }
 
@@ -713,17 +723,17 @@ __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t
     }
 }
 
-static void launch_margins_pm(hipStream_t s, const LvlArgs& a, const PInfo* pinfo, int T, const int32_t* child_prev,
+static void launch_margins_pm(hipStream_t s, const LvlArgs& a, const PInfo* pinfo, int T, const CInfo* cinfo,
                               int32_t* nodeOf, uint16_t* mask16) {
     const unsigned grid = lgrid(a.n * 16, 256, 1 << 16);
     if (a.dim <= 128)
-        hipLaunchKernelGGL(k_lvl_margins_pm<1>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
+        hipLaunchKernelGGL(k_lvl_margins_pm<1>, dim3(grid), dim3(256), 0, s, a, pinfo, T, cinfo, nodeOf, mask16);
     else if (a.dim <= 256)
-        hipLaunchKernelGGL(k_lvl_margins_pm<2>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
+        hipLaunchKernelGGL(k_lvl_margins_pm<2>, dim3(grid), dim3(256), 0, s, a, pinfo, T, cinfo, nodeOf, mask16);
     else if (a.dim <= 512)
-        hipLaunchKernelGGL(k_lvl_margins_pm<4>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
+        hipLaunchKernelGGL(k_lvl_margins_pm<4>, dim3(grid), dim3(256), 0, s, a, pinfo, T, cinfo, nodeOf, mask16);
     else
-        hipLaunchKernelGGL(k_lvl_margins_pm<8>, dim3(grid), dim3(256), 0, s, a, pinfo, T, child_prev, nodeOf, mask16);
+        hipLaunchKernelGGL(k_lvl_margins_pm<8>, dim3(grid), dim3(256), 0, s, a, pinfo, T, cinfo, nodeOf, mask16);
 }
 
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
@@ -795,7 +805,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<PInfo> pinfo(W, kWsForest + 25, max_nodes + 1);
     WBuf<uint32_t> nchild(W, kWsForest + 17, max_nodes + 1), cscan(W, kWsForest + 18, max_nodes + 1),
         leftsz(W, kWsForest + 19, max_nodes + 1);
-    WBuf<int32_t> child_first(W, kWsForest + 28, max_nodes + 1);
+    WBuf<CInfo> cinfo(W, kWsForest + 28, 2 * (size_t)(max_nodes + 1));
     WBuf<uint32_t> lstart(W, kWsForest + 30, max_nodes + 1);
     // level status read back once per level: [tested pairs so far (u64) | children per tree (u32 x T)]
     WBuf<unsigned long long> lstat(W, kWsForest + 21, 1 + (T + 1) / 2);
@@ -812,7 +822,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<char> cscan_tmp(W, kWsForest + 24, std::max<size_t>(cscan_bytes, 16));
 
     std::vector<LvlNode> roots(T);
-    for (int t = 0; t < T; ++t) roots[t] = LvlNode{t, 0, 0, (int32_t)n, 0};
+    for (int t = 0; t < T; ++t) roots[t] = LvlNode{t, 0, 0, (int32_t)n, 0, -1};
     HIP_TRY(hipMemcpyAsync(cur, roots.data(), T * sizeof(LvlNode), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_lvl_init, dim3(lgrid(total, 256)), dim3(256), 0, s, T, n, d_tab.p, d_tab.p + 2 * T,
                        nodeOf.p);
@@ -843,26 +853,26 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
         hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur, na,
-                           (const int32_t* const*)P, split.p, pinfo.p, active_pairs);
+                           (const int32_t* const*)P, split.p, pinfo.p, cinfo.p, active_pairs);
         margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
-        const int32_t* child_prev = level > 0 ? child_first.p : nullptr;
+        const CInfo* cinfo_prev = level > 0 ? cinfo.p : nullptr;
         if (sparse)
             hipLaunchKernelGGL(k_lvl_margins_pos, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, a, pinfo.p, T,
                                (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags8.p);
         else if (point_major_dpp)
-            launch_margins_pm(s, a, pinfo.p, T, child_prev, nodeOf.p, mask16.p);
+            launch_margins_pm(s, a, pinfo.p, T, cinfo_prev, nodeOf.p, mask16.p);
         else if (dim % 4 == 0 && dim <= 64)
-            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+            hipLaunchKernelGGL(k_lvl_margins<1>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, cinfo_prev, nodeOf.p,
                                mask16.p);
         else if (dim % 4 == 0 && dim <= 128)
-            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+            hipLaunchKernelGGL(k_lvl_margins<2>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, cinfo_prev, nodeOf.p,
                                mask16.p);
         else if (dim % 4 == 0 && dim <= 256)
-            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+            hipLaunchKernelGGL(k_lvl_margins<4>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, cinfo_prev, nodeOf.p,
                                mask16.p);
         else
-            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, child_prev, nodeOf.p,
+            hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, cinfo_prev, nodeOf.p,
                                mask16.p);
         margins_timer.stop(s);
         const unsigned cgrid = lgrid(nchunks, 4, 1 << 16);  // four chunks (waves) per workgroup
@@ -884,7 +894,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         HIP_TRY(hipEventRecord(lst_ev, s));
         // children ids, the next active list and the partition need nothing from the host
         hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
-                           leftsz.p, d_base.p, st_next, d_ch.p, nxt, child_first.p);
+                           leftsz.p, d_base.p, st_next, d_ch.p, nxt, cinfo.p);
         if (vec4)
             hipLaunchKernelGGL(k_lvl_scatter<true>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
                                leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
