@@ -449,6 +449,8 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         }
         launch_fill_i32(s, new_idx.p, -1, (int64_t)n * C);
         launch_fill_i32(s, old_idx.p, -1, (int64_t)n * C);
+        HIP_TRY(hipMemsetAsync(rpc.p, 0, (n + 1) * sizeof(unsigned long long), s));  // rows without candidates
+        HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
         {
             CandBuildArgs ca{};
             ca.idx = w_idx;
@@ -477,19 +479,18 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             ca.scan_tmp_bytes = PL.scan_bytes;
             ca.new_idx = new_idx.p;
             ca.old_idx = old_idx.p;
+            ca.row_pairs = rpc.p;
+            ca.counts = cstat.p;
             launch_cand_build(s, ca);
         }
         // update_iteration: generation into per-row regions, grouping by destination, ordered apply.
-        // The pair count sizes the join's region buffer; it is read while the flag marking and the
-        // candidate statistics run.
-        launch_row_pairs(s, new_idx.p, old_idx.p, 0, n, C, rpc.p, rowoff.p, PL.scan_tmp.p, PL.scan_bytes);
+        // The pair count sizes the join's region buffer; it is read while the flags are marked.
+        scan_u64_exclusive(s, rpc.p, rowoff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);  // each row's pair region
         uint64_t* hst = (uint64_t*)W.status.ensure(16);
         const hipEvent_t hev = W.status_ev.get();
         HIP_TRY(hipMemcpyAsync(hst, rowoff.p + n, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(hev, s));
         launch_mark_old(s, w_idx, w_flags, new_idx.p, n, k, C);
-        HIP_TRY(hipMemsetAsync(cstat.p, 0, 16, s));
-        launch_cand_stats(s, new_idx.p, old_idx.p, n, C, cstat.p);
         HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
         HIP_TRY(hipMemsetAsync(rescored.p, 0, 8, s));
         HIP_TRY(hipEventSynchronize(hev));

@@ -303,13 +303,13 @@ struct CandBuildArgs {
     void* scan_tmp;
     size_t scan_tmp_bytes;
     int32_t *new_idx, *old_idx;
+    // per row: pairs the join evaluates, nn(nn-1)/2 + nn*no (rows without candidates keep the caller's
+    // zero); counts[0] += valid new+old candidates, counts[1] += pairs
+    unsigned long long* row_pairs;
+    unsigned long long* counts;
 };
 size_t cand_sort_tmp_bytes(int64_t items, int end_bit);
 void launch_cand_build(hipStream_t s, const CandBuildArgs& a);
-// counts[0] += valid new+old candidates, counts[1] += pairs the join evaluates (per row:
-// nn(nn-1)/2 + nn*no)
-void launch_cand_stats(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t n, int C,
-                       unsigned long long* counts);
 void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const int32_t* new_idx,
                      int64_t n, int k, int C);
 
@@ -327,10 +327,6 @@ struct JoinOut {
     unsigned long long* kc;  // kept pairs per row
     unsigned long long* cnt;
 };
-// rowoff[r] = exclusive prefix of nn(nn-1)/2 + nn*no over the rows (rowoff[n_rows] = total)
-void launch_row_pairs(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t row_begin,
-                      int64_t row_end, int C, unsigned long long* counts, uint64_t* rowoff, void* scan_tmp,
-                      size_t scan_tmp_bytes);
 void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
                       int64_t row_end, const float* heap_dist, int k, const JoinOut& out);

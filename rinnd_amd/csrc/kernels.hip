@@ -1288,7 +1288,8 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
                                                        const uint64_t* __restrict__ off, const uint64_t* __restrict__ rv,
                                                        const int32_t* __restrict__ idx, const uint8_t* __restrict__ flags,
                                                        const float* __restrict__ fpr, int k, int mode, int64_t block_rows,
-                                                       int32_t* new_idx, int32_t* old_idx, int C) {
+                                                       int32_t* new_idx, int32_t* old_idx, int C,
+                                                       unsigned long long* row_pairs, unsigned long long* counts) {
     extern __shared__ __align__(16) float csm[];
     const int B = blockDim.x, c = threadIdx.x;
     const int S = B + 1;  // column stride (odd: the row stores are conflict-free)
@@ -1297,6 +1298,7 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
     float* op = (float*)(ni + S * C);
     int32_t* oi = (int32_t*)(op + S * C);
     const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
+    unsigned long long st_rows = 0, st_pairs = 0;
     for (int64_t e0 = (int64_t)blockIdx.x * B; e0 < total; e0 += (int64_t)gridDim.x * B) {
         const int64_t e = total - 1 - (e0 + c);  // longest bins first: they start early instead of trailing
         const int32_t v = e >= 0 ? order[e] : -1;
@@ -1362,9 +1364,26 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
                 rev(o0, pa);
                 rev(pb, o1);
             }
+            unsigned long long nn = 0, no = 0;
+            for (int j = 0; j < C; ++j) {
+                nn += ni[j * S + c] >= 0;
+                no += oi[j * S + c] >= 0;
+            }
+            const unsigned long long pr = nn * (nn - (nn > 0)) / 2 + nn * no;
+            row_pairs[v] = pr;
+            st_rows += nn + no;
+            st_pairs += pr;
         }
         wave_rows<false>(v, C, new_idx, ni, S, c & ~63);
         wave_rows<false>(v, C, old_idx, oi, S, c & ~63);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        st_rows += __shfl_xor(st_rows, o);
+        st_pairs += __shfl_xor(st_pairs, o);
+    }
+    if ((threadIdx.x & 63) == 0 && st_rows) {
+        atomicAdd(&counts[0], st_rows);
+        atomicAdd(&counts[1], st_pairs);
     }
 }
 
@@ -1430,34 +1449,7 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
     const int B = lanes_block(a.C, 16, 128);
     hipLaunchKernelGGL(k_cand_push_rev, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)(B + 1) * a.C * 16, s, a.order,
                        a.bins, a.off, a.rval2, a.idx, a.flags, a.fpr, a.k, a.mode, a.block_rows, a.new_idx, a.old_idx,
-                       a.C);
-}
-
-__global__ void k_cand_stats(const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx,
-                             int64_t n, int C, unsigned long long* counts) {
-    unsigned long long rows = 0, pairs = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        unsigned long long nn = 0, no = 0;
-        for (int c = 0; c < C; ++c) {
-            nn += new_idx[i * C + c] >= 0;
-            no += old_idx[i * C + c] >= 0;
-        }
-        rows += nn + no;
-        pairs += nn * (nn - (nn ? 1 : 0)) / 2 + nn * no;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        rows += __shfl_xor(rows, o);
-        pairs += __shfl_xor(pairs, o);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&counts[0], rows);
-        atomicAdd(&counts[1], pairs);
-    }
-}
-void launch_cand_stats(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t n, int C,
-                       unsigned long long* counts) {
-    hipLaunchKernelGGL(k_cand_stats, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, new_idx, old_idx, n, C, counts);
+                       a.C, a.row_pairs, a.counts);
 }
 
 // mark_old_flags (candidates.rs:354-417)
@@ -1489,33 +1481,6 @@ void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const in
 // (no global counter); per-candidate counts are summed in LDS and added to cnt once per row.
 constexpr int kJoinThreads = 256;
 constexpr int kJoinLdsBytes = 64 * 1024;
-
-__global__ void k_row_pairs(const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx,
-                            int64_t row_begin, int64_t row_end, int C, unsigned long long* out) {
-    const int64_t rows = row_end - row_begin;
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= rows;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        if (r == rows) {
-            out[r] = 0;
-            continue;
-        }
-        const int64_t i = row_begin + r;
-        unsigned long long nn = 0, no = 0;
-        for (int c = 0; c < C; ++c) {
-            nn += new_idx[i * C + c] >= 0;
-            no += old_idx[i * C + c] >= 0;
-        }
-        out[r] = nn * (nn - (nn > 0)) / 2 + nn * no;
-    }
-}
-void launch_row_pairs(hipStream_t s, const int32_t* new_idx, const int32_t* old_idx, int64_t row_begin,
-                      int64_t row_end, int C, unsigned long long* counts, uint64_t* rowoff, void* scan_tmp,
-                      size_t scan_tmp_bytes) {
-    const int64_t rows = std::max<int64_t>(row_end - row_begin, 0);
-    hipLaunchKernelGGL(k_row_pairs, dim3(grid_for(rows + 1, 256)), dim3(256), 0, s, new_idx, old_idx, row_begin,
-                       row_end, C, counts);
-    scan_u64_exclusive(s, counts, rowoff, rows + 1, scan_tmp, scan_tmp_bytes);
-}
 
 template <bool STAGED>
 __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
