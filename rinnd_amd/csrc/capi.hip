@@ -484,13 +484,12 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             launch_cand_build(s, ca);
         }
         // update_iteration: generation into per-row regions, grouping by destination, ordered apply.
-        // The pair count sizes the join's region buffer; it is read while the flags are marked.
+        // The pair count sizes the join's region buffer.
         scan_u64_exclusive(s, rpc.p, rowoff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);  // each row's pair region
         uint64_t* hst = (uint64_t*)W.status.ensure(16);
         const hipEvent_t hev = W.status_ev.get();
         HIP_TRY(hipMemcpyAsync(hst, rowoff.p + n, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(hev, s));
-        launch_mark_old(s, w_idx, w_flags, new_idx.p, n, k, C);
         HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
         HIP_TRY(hipMemsetAsync(rescored.p, 0, 8, s));
         HIP_TRY(hipEventSynchronize(hev));

@@ -283,7 +283,7 @@ void launch_rng_streams(hipStream_t s, const uint64_t* jump_tables, const RngStr
 // bins (128), sort/scan temporaries.  The forward draws need C <= 64 lanes of LDS per destination.
 struct CandBuildArgs {
     const int32_t* idx;
-    const uint8_t* flags;
+    uint8_t* flags;  // the row's new entries that became new candidates are flagged old here
     int64_t n;
     int k, C, mode;
     const uint64_t* rvp;  // exclusive prefix of valid edges per row (n+1)
@@ -310,8 +310,6 @@ struct CandBuildArgs {
 };
 size_t cand_sort_tmp_bytes(int64_t items, int end_bit);
 void launch_cand_build(hipStream_t s, const CandBuildArgs& a);
-void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const int32_t* new_idx,
-                     int64_t n, int k, int C);
 
 // local join.  Row r = i - row_begin owns the region U[rowoff[r], rowoff[r+1]) sized by its valid
 // pair count; the join writes its kept pairs there as (slot t = j*2C + kk, d) in any order, the kept

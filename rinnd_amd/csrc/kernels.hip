@@ -1286,7 +1286,7 @@ __device__ __forceinline__ uint64_t src_lower_bound(const uint64_t* rv, uint64_t
 
 __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict__ order, const uint32_t* bins,
                                                        const uint64_t* __restrict__ off, const uint64_t* __restrict__ rv,
-                                                       const int32_t* __restrict__ idx, const uint8_t* __restrict__ flags,
+                                                       const int32_t* __restrict__ idx, uint8_t* flags,
                                                        const float* __restrict__ fpr, int k, int mode, int64_t block_rows,
                                                        int32_t* new_idx, int32_t* old_idx, int C,
                                                        unsigned long long* row_pairs, unsigned long long* counts) {
@@ -1373,6 +1373,18 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
             row_pairs[v] = pr;
             st_rows += nn + no;
             st_pairs += pr;
+            // mark_old_flags (candidates.rs:354-417): the row's entries that became new candidates are
+            // flagged old.  Only this lane reads or writes row v's flags in this kernel.
+            const int64_t f0 = (int64_t)v * k;
+            for (int j = 0; j < k; ++j) {
+                const int32_t nb = idx[f0 + j];
+                if (nb < 0 || !flags[f0 + j]) continue;
+                for (int cc = 0; cc < C; ++cc)
+                    if (ni[cc * S + c] == nb) {
+                        flags[f0 + j] = 0;
+                        break;
+                    }
+            }
         }
         wave_rows<false>(v, C, new_idx, ni, S, c & ~63);
         wave_rows<false>(v, C, old_idx, oi, S, c & ~63);
@@ -1452,27 +1464,6 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
                        a.C, a.row_pairs, a.counts);
 }
 
-// mark_old_flags (candidates.rs:354-417)
-__global__ void k_mark_old(const int32_t* __restrict__ idx, uint8_t* flags,
-                           const int32_t* __restrict__ new_idx, int64_t n, int k, int C) {
-    const int64_t total = n * k;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t nb = idx[e];
-        if (nb < 0 || flags[e] == 0) continue;
-        const int64_t i = e / k;
-        for (int c = 0; c < C; ++c)
-            if (new_idx[i * C + c] == nb) {
-                flags[e] = 0;
-                break;
-            }
-    }
-}
-void launch_mark_old(hipStream_t s, const int32_t* idx, uint8_t* flags, const int32_t* new_idx,
-                     int64_t n, int k, int C) {
-    hipLaunchKernelGGL(k_mark_old, dim3(grid_for(n * k, 256)), dim3(256), 0, s, idx, flags, new_idx, n,
-                       k, C);
-}
 
 // ---- local join, FP32 (update_iteration generation, mod.rs:383-447) --------------------------------
 // One workgroup per row i: the row's new (C) and old (C) candidate rows are staged in LDS, every
