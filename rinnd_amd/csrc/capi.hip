@@ -286,7 +286,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     // BF16 rows (join tiles; leaf tiles too when the data are exact in BF16)
     const bool use_bf16 = p.join_precision == RINND_JOIN_BF16;
     Bf16Rows& B16 = W.b16;
-    WBuf<unsigned long long> rescored(W, 6, 1);
+    WBuf<unsigned long long> rescored(W, 6, kKeptSlots);
     if (use_bf16) {
         if (metric != RINND_DIST_SQEUCLIDEAN)
             throw Error{RINND_ERR_UNSUPPORTED, "the BF16 join is implemented for squared Euclidean"};
@@ -378,8 +378,8 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     WBuf<char> c_sort_tmp(W, 27, std::max<size_t>(cand_sort_tmp_bytes((int64_t)n * k, c_end_bit), 16));
     WBuf<unsigned long long> rpc(W, 28, n + 1);
     WBuf<uint64_t> rowoff(W, 29, n + 1);
-    WBuf<unsigned long long> kc(W, 30, n + 1);
-    HIP_TRY(hipMemsetAsync(kc.p, 0, (n + 1) * 8, s));  // kc[n] stays 0: the scan's total slot
+    WBuf<unsigned long long> kc(W, 30, n + 1), krec(W, 40, n + 1), kept(W, 41, kKeptSlots);
+    HIP_TRY(hipMemsetAsync(krec.p, 0, (n + 1) * 8, s));  // krec[n] stays 0: the scan's total slot
     WBuf<uint64_t> koff(W, 31, n + 1);
     WBuf<uint32_t> jr_key, jr_key2;
     WBuf<uint64_t> jr_val, jr_val2;
@@ -491,10 +491,11 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         HIP_TRY(hipMemcpyAsync(hst, rowoff.p + n, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(hev, s));
         HIP_TRY(hipMemsetAsync(PL.changes.p, 0, 8, s));
-        HIP_TRY(hipMemsetAsync(rescored.p, 0, 8, s));
+        HIP_TRY(hipMemsetAsync(rescored.p, 0, kKeptSlots * 8, s));
+        HIP_TRY(hipMemsetAsync(kept.p, 0, kKeptSlots * 8, s));
         HIP_TRY(hipEventSynchronize(hev));
         JU = WBuf<JRec>(W, 33, std::max<uint64_t>(hst[0], 1));
-        const JoinOut jo{rowoff.p, JU.p, kc.p, PL.cnt.p};
+        const JoinOut jo{rowoff.p, JU.p, kc.p, PL.cnt.p, krec.p, kept.p};
         PL.begin(s);
         HIP_TRY(hipEventRecord(ev[4], s));
         if (use_bf16)
@@ -503,7 +504,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             launch_join_fp32(s, metric, DXw, Dld, dim, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo);
         HIP_TRY(hipEventRecord(ev[5], s));
         scan_u64_exclusive(s, PL.cnt.p, PL.off.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
-        scan_u64_exclusive(s, kc.p, koff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
+        scan_u64_exclusive(s, krec.p, koff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
         // the record count sizes the sort; it is read while the destinations are binned
         HIP_TRY(hipMemcpyAsync(hst + 1, PL.off.p + n, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(hev, s));
@@ -553,12 +554,16 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             launch_join_apply(s, ja);
         }
         HIP_TRY(hipEventRecord(ev[6], s));
-        uint64_t ch = 0, cs[2] = {0, 0}, nres = 0;
-        const uint64_t nup = n_rec / 2;
+        uint64_t ch = 0, cs[2] = {0, 0}, nres = 0, nup = 0, kslots[kKeptSlots], rslots[kKeptSlots];
         HIP_TRY(hipMemcpyAsync(cs, cstat.p, 16, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&nres, rescored.p, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(rslots, rescored.p, sizeof(rslots), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(kslots, kept.p, sizeof(kslots), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(&ch, PL.changes.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        for (int j = 0; j < kKeptSlots; ++j) {
+            nup += kslots[j];
+            nres += rslots[j];
+        }
         ms_cand += elapsed_ms(ev[3], ev[4]);  // includes the small counter resets
         const double jm = elapsed_ms(ev[4], ev[5]);
         ms_join += jm;

@@ -314,17 +314,24 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a);
 // local join.  Row r = i - row_begin owns the region U[rowoff[r], rowoff[r+1]) sized by its valid
 // pair count; the join writes its kept pairs there as (slot t = j*2C + kk, d) in any order, the kept
 // count to kc[r], and adds each candidate's share to cnt[id] (one atomic per candidate and row).
+// A kept pair of a row's join: slot t = j * 2C + kk of the row's candidate pair (j, kk), with flags for the
+// endpoints whose heaps it can still enter (d below that endpoint's root when the pair was evaluated; a
+// push at or above the root is rejected, and the roots only fall until the records are applied).
 struct JRec {
     uint32_t t;
     float d;
 };
 static_assert(sizeof(JRec) == 8, "JRec layout");
+constexpr uint32_t kEmitP = 1u << 31, kEmitQ = 1u << 30, kSlotMask = kEmitQ - 1u;
 struct JoinOut {
     const uint64_t* rowoff;
     JRec* U;
-    unsigned long long* kc;  // kept pairs per row
-    unsigned long long* cnt;
+    unsigned long long* kc;    // kept pairs per row
+    unsigned long long* cnt;   // records per destination
+    unsigned long long* krec;  // records per row
+    unsigned long long* kept;  // kept pairs in all, spread over kKeptSlots counters (summed by the caller)
 };
+constexpr int kKeptSlots = 256;
 void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
                       int64_t row_end, const float* heap_dist, int k, const JoinOut& out);
@@ -386,7 +393,7 @@ void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree
 void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const Bf16Rows& R,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
                       int64_t row_end, const float* heap_dist, int k, const JoinOut& out,
-                      unsigned long long* rescored);
+                      unsigned long long* rescored);  // kKeptSlots counters
 
 // relabelling: Xr[r] = X[perm[r]], inv[perm[r]] = r; ids[i] = map[ids[i]] (negative ids kept);
 // heap row r -> caller row perm[r] with ids mapped through perm

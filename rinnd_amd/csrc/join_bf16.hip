@@ -118,7 +118,7 @@ struct JoinBF16Args {
     int exact;
     float g_d, g_r;
     JoinOut out;
-    unsigned long long* rescored;
+    unsigned long long* rescored;  // kKeptSlots counters
 };
 
 template <int TA, int TB, bool EXACT>
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
     const int C = a.C, C2 = 2 * C;
     const float u = 1.0f / 256.0f, u24 = 1.0f / 16777216.0f;
     const int m = lane & 15, kq = (lane >> 4) * 4;
-    unsigned long long resc = 0;
+    unsigned long long resc = 0, kept = 0;
     for (int64_t i = a.row_begin + (int64_t)blockIdx.x * W + w; i < a.row_end; i += (int64_t)gridDim.x * W) {
         for (int c = lane; c < TB * 16; c += 64) {
             int32_t id = -1;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
         wave_sync();
         const int64_t r0 = i - a.row_begin;
         JRec* U = a.out.U + a.out.rowoff[r0];
-        unsigned run = 0;
+        unsigned run = 0, recs = 0;
         // operand rows: absent candidates read row 0 and are zeroed after the load, so every load of a batch
         // is unconditional and KB k-steps of fragments are in flight together
         const uint16_t* arow[TA];
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                 for (int r = 0; r < 4; ++r) {
                     const int j = 16 * x + (lane >> 4) * 4 + r;  // D[row][col] layout of 16x16 MFMA
                     const int kk = 16 * y + m;
-                    bool keep = false;
+                    bool keep = false, ep = false, eq = false;
                     int32_t p = 0, q = 0;
                     float d = 0.f;
                     if (j < C && kk < C2) {
@@ -213,7 +213,9 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                             const float dt = (nr[w][j] + nr[w][kk]) - 2.0f * acc[x][y][r];
                             if (EXACT) {  // no re-evaluation code in this instance: fewer registers
                                 d = dt;
-                                keep = d <= thr;
+                                ep = d < roots[w][j];
+                                eq = d < roots[w][kk];
+                                keep = ep || eq;
                             } else {
                                 bool rescore;
                                 if (!(thr < __builtin_inff())) {
@@ -229,7 +231,9 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                                 if (rescore) {
                                     ++resc;
                                     d = d_sqeuclidean(a.X + (int64_t)p * a.ld, a.X + (int64_t)q * a.ld, a.dim);
-                                    keep = d <= thr;
+                                    ep = d < roots[w][j];
+                                    eq = d < roots[w][kk];
+                                    keep = ep || eq;
                                 }
                             }
                         }
@@ -237,20 +241,28 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                     // kept pairs go to the row's own region in any order (the slot carries the key)
                     const unsigned long long bal = __ballot(keep);
                     if (keep) {
-                        U[run + __popcll(bal & lt)] = JRec{(uint32_t)(j * C2 + kk), d};
-                        atomicAdd(&cc[w][j], 1u);
-                        atomicAdd(&cc[w][kk], 1u);
+                        U[run + __popcll(bal & lt)] =
+                            JRec{(uint32_t)(j * C2 + kk) | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
+                        if (ep) atomicAdd(&cc[w][j], 1u);
+                        if (eq) atomicAdd(&cc[w][kk], 1u);
+                        recs += (unsigned)ep + (unsigned)eq;
                     }
                     run += (unsigned)__popcll(bal);
                 }
         wave_sync();
-        if (lane == 0) a.out.kc[r0] = (unsigned long long)run;
+        for (int o = 32; o > 0; o >>= 1) recs += __shfl_xor(recs, o);
+        if (lane == 0) {
+            a.out.kc[r0] = (unsigned long long)run;
+            a.out.krec[r0] = (unsigned long long)recs;
+            kept += run;
+        }
         for (int c = lane; c < C2; c += 64)
             if (cc[w][c]) atomicAdd(&a.out.cnt[ids[w][c]], (unsigned long long)cc[w][c]);
         wave_sync();
     }
     for (int o = 32; o > 0; o >>= 1) resc += __shfl_xor(resc, o);
-    if (lane == 0 && resc) atomicAdd(a.rescored, resc);
+    if (lane == 0 && resc) atomicAdd(&a.rescored[(blockIdx.x * W + w) % kKeptSlots], resc);
+    if (lane == 0 && kept) atomicAdd(&a.out.kept[(blockIdx.x * W + w) % kKeptSlots], kept);
 }
 
 // Leaf pair distances on MFMA for exact-mode data (initialize_from_leaves, core/nndescent/mod.rs:203-300):

@@ -1482,11 +1482,12 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
     __shared__ int32_t ids[128];
     __shared__ float roots[128];
     __shared__ unsigned cc[128];
-    __shared__ unsigned run;
+    __shared__ unsigned run, nrec;
     const int C2 = 2 * C;
     const int64_t ldl = ld + 4;  // padded LDS row stride
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    unsigned long long kept = 0;  // thread 0: kept pairs of this workgroup's rows
     for (int64_t i = row_begin + blockIdx.x; i < row_end; i += gridDim.x) {
         const int64_t r = i - row_begin;
         for (int c = threadIdx.x; c < C2; c += blockDim.x) {
@@ -1495,7 +1496,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
             roots[c] = id >= 0 ? hd[(int64_t)id * k] : 0.f;
             cc[c] = 0;
         }
-        if (threadIdx.x == 0) run = 0;
+        if (threadIdx.x == 0) run = nrec = 0;
         __syncthreads();
         if (STAGED) {
             const int vec = (int)(ld >> 2);
@@ -1512,7 +1513,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
         const int total = C * C2;
         for (int t0 = 0; t0 < total; t0 += blockDim.x) {
             const int t = t0 + threadIdx.x;
-            bool keep = false;
+            bool keep = false, ep = false, eq = false;
             float d = 0.f;
             int j = 0, kk = 0;
             if (t < total) {
@@ -1524,28 +1525,39 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
                     const float* xp = STAGED ? jsm + j * ldl : X + (int64_t)p * ld;
                     const float* xq = STAGED ? jsm + kk * ldl : X + (int64_t)q * ld;
                     d = distance_exact(metric, xp, xq, dim);
-                    keep = d <= fmaxf(roots[j], roots[kk]);
+                    ep = d < roots[j];
+                    eq = d < roots[kk];
+                    keep = ep || eq;
                 }
             }
             const unsigned long long bal = __ballot(keep);
+            const unsigned nr = (unsigned)(__popcll(__ballot(ep)) + __popcll(__ballot(eq)));
             if (bal) {
                 const int leader = __ffsll((long long)bal) - 1;
                 unsigned base = 0;
-                if (lane == leader) base = atomicAdd(&run, (unsigned)__popcll(bal));
+                if (lane == leader) {
+                    base = atomicAdd(&run, (unsigned)__popcll(bal));
+                    atomicAdd(&nrec, nr);
+                }
                 base = __shfl(base, leader);
                 if (keep) {
-                    U[base + __popcll(bal & lt)] = JRec{(uint32_t)t, d};
-                    atomicAdd(&cc[j], 1u);
-                    atomicAdd(&cc[kk], 1u);
+                    U[base + __popcll(bal & lt)] = JRec{(uint32_t)t | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
+                    if (ep) atomicAdd(&cc[j], 1u);
+                    if (eq) atomicAdd(&cc[kk], 1u);
                 }
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) out.kc[r] = (unsigned long long)run;
+        if (threadIdx.x == 0) {
+            out.kc[r] = (unsigned long long)run;
+            out.krec[r] = (unsigned long long)nrec;
+            kept += run;
+        }
         for (int c = threadIdx.x; c < C2; c += blockDim.x)
             if (cc[c]) atomicAdd(&out.cnt[ids[c]], (unsigned long long)cc[c]);
         __syncthreads();
     }
+    if (threadIdx.x == 0 && kept) atomicAdd(&out.kept[blockIdx.x % kKeptSlots], kept);
 }
 void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
@@ -1620,34 +1632,37 @@ __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict_
     extern __shared__ __align__(16) char jrm[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int C2 = 2 * C, S = C * C2, W = (S + 63) >> 6;
-    const int per_wave = round_up(W * 8 + S * 4 + C2 * 4, 16);
-    unsigned long long* mask = (unsigned long long*)(jrm + w * per_wave);
-    float* dv = (float*)(mask + W);
+    const int per_wave = round_up(2 * W * 8 + S * 4 + C2 * 4, 16);
+    unsigned long long* mp = (unsigned long long*)(jrm + w * per_wave);  // slots with a record for p
+    unsigned long long* mq = mp + W;                                      // slots with a record for q
+    float* dv = (float*)(mq + W);
     int32_t* ids = (int32_t*)(dv + S);
     for (int64_t i = (int64_t)blockIdx.x * 4 + w; i < n; i += (int64_t)gridDim.x * 4) {
         for (int c = lane; c < C2; c += 64) ids[c] = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
-        for (int q = lane; q < W; q += 64) mask[q] = 0ull;
+        for (int q = lane; q < 2 * W; q += 64) mp[q] = 0ull;
         wave_sync();
         const uint32_t nk = (uint32_t)out.kc[i];
         const JRec* E = out.U + out.rowoff[i];
         for (uint32_t e = lane; e < nk; e += 64) {
             const JRec q = E[e];
-            dv[q.t] = q.d;
-            atomicOr(&mask[q.t >> 6], 1ull << (q.t & 63));
+            const uint32_t t = q.t & kSlotMask;
+            dv[t] = q.d;
+            if (q.t & kEmitP) atomicOr(&mp[t >> 6], 1ull << (t & 63));
+            if (q.t & kEmitQ) atomicOr(&mq[t >> 6], 1ull << (t & 63));
         }
         wave_sync();
-        uint64_t carry = koff[i];
+        uint64_t carry = koff[i];  // records are emitted in slot order, p's before q's within a slot
         for (int q0 = 0; q0 < W; q0 += 64) {
             const int q = q0 + lane;
-            const unsigned long long bits = q < W ? mask[q] : 0ull;
-            const uint32_t pc = (uint32_t)__popcll(bits);
+            const unsigned long long bp = q < W ? mp[q] : 0ull, bq = q < W ? mq[q] : 0ull;
+            const uint32_t pc = (uint32_t)(__popcll(bp) + __popcll(bq));
             uint32_t incl = pc;
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t v = __shfl_up(incl, o);
                 if (lane >= o) incl += v;
             }
             uint64_t rank = carry + (incl - pc);
-            unsigned long long b = bits;
+            unsigned long long b = bp | bq;
             while (b) {
                 const int bit = __ffsll((long long)b) - 1;
                 b &= b - 1;
@@ -1655,11 +1670,16 @@ __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict_
                 const int j = t / C2, kk = t - j * C2;
                 const int32_t p = ids[j], qq = ids[kk];
                 const uint32_t dbits = __builtin_bit_cast(uint32_t, dv[t]);
-                rkey[2 * rank] = (uint32_t)p;
-                rval[2 * rank] = ((uint64_t)dbits << 32) | (uint32_t)qq;
-                rkey[2 * rank + 1] = (uint32_t)qq;
-                rval[2 * rank + 1] = ((uint64_t)dbits << 32) | (uint32_t)p;
-                ++rank;
+                if ((bp >> bit) & 1ull) {
+                    rkey[rank] = (uint32_t)p;
+                    rval[rank] = ((uint64_t)dbits << 32) | (uint32_t)qq;
+                    ++rank;
+                }
+                if ((bq >> bit) & 1ull) {
+                    rkey[rank] = (uint32_t)qq;
+                    rval[rank] = ((uint64_t)dbits << 32) | (uint32_t)p;
+                    ++rank;
+                }
             }
             carry += __shfl(incl, 63);
         }
@@ -1735,7 +1755,7 @@ void launch_join_bins(hipStream_t s, const JoinApplyArgs& a) {
 // The records of the kept pairs, sorted by destination, applied in generation order (after launch_join_bins).
 void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     const int C2 = 2 * a.C, S = a.C * C2, W = (S + 63) >> 6;
-    const size_t per_wave = (size_t)round_up(W * 8 + S * 4 + C2 * 4, 16);
+    const size_t per_wave = (size_t)round_up(2 * W * 8 + S * 4 + C2 * 4, 16);
     hipLaunchKernelGGL(k_join_records, dim3(grid_for(ceil_div(a.n, (int64_t)4), 1, 1 << 18)), dim3(256), 4 * per_wave,
                        s, a.new_idx, a.old_idx, a.C, a.n, a.out, a.koff, a.rkey, a.rval);
     int end_bit = 1;
