@@ -149,16 +149,13 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
         JRec* U = a.out.U + a.out.rowoff[r0];
         unsigned run = 0, recs = 0;
         // operand rows: absent candidates read row 0 and are zeroed after the load, so every load of a batch
-        // is unconditional and KB k-steps of fragments are in flight together
-        const uint16_t* arow[TA];
+        // is unconditional and KB k-steps of fragments are in flight together.  The new candidates (the A
+        // operand) are the first candidates of the B operand, so A's fragments are B's first TA tiles, zeroed
+        // past the new candidates.
         const uint16_t* brow[TB];
         bool aval[TA], bval[TB];
 #pragma unroll
-        for (int t = 0; t < TA; ++t) {
-            const int32_t id = (16 * t + m < C) ? ids[w][16 * t + m] : -1;
-            aval[t] = id >= 0;
-            arow[t] = a.Xb + (int64_t)(id >= 0 ? id : 0) * a.ldb + kq;
-        }
+        for (int t = 0; t < TA; ++t) aval[t] = 16 * t + m < C && ids[w][16 * t + m] >= 0;
 #pragma unroll
         for (int t = 0; t < TB; ++t) {
             const int32_t id = ids[w][16 * t + m];
@@ -173,12 +170,10 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
         constexpr int KB = 2;
         const v4s z4 = {0, 0, 0, 0};
         for (int64_t k0 = 0; k0 < a.ldb; k0 += 16 * KB) {
-            v4s A[KB][TA], B[KB][TB];
+            v4s B[KB][TB];
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
                 const int64_t ko = min(k0 + 16 * kb, a.ldb - 16);
-#pragma unroll
-                for (int t = 0; t < TA; ++t) A[kb][t] = *reinterpret_cast<const v4s*>(arow[t] + ko);
 #pragma unroll
                 for (int t = 0; t < TB; ++t) B[kb][t] = *reinterpret_cast<const v4s*>(brow[t] + ko);
             }
@@ -190,7 +185,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
 #pragma unroll
                     for (int y = 0; y < TB; ++y)
                         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                            aval[x] && live ? A[kb][x] : z4, bval[y] ? B[kb][y] : z4, acc[x][y], 0, 0, 0);
+                            aval[x] && live ? B[kb][x] : z4, bval[y] ? B[kb][y] : z4, acc[x][y], 0, 0, 0);
             }
         }
 #pragma unroll
