@@ -1274,6 +1274,41 @@ __global__ void k_cand_gen_rev(const int32_t* __restrict__ idx, const uint8_t* _
     }
 }
 
+// The same for counter draws, one thread per heap entry (coalesced): the draws depend on the entry index only,
+// and the destinations' segments are found in the sorted keys afterwards (no counting atomics).
+__global__ __launch_bounds__(256) void k_cand_gen_rev_ctr(const int32_t* __restrict__ idx,
+                                                          const uint8_t* __restrict__ flags, int64_t n, int k,
+                                                          uint64_t counter_key, float* fpr, uint32_t* rkey,
+                                                          uint64_t* rval) {
+    const int64_t total = n * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t nb = idx[e];
+        if (nb < 0) {
+            rkey[e] = (uint32_t)n;  // sorts after every destination
+            continue;
+        }
+        const int64_t i = e / k;
+        const uint32_t fbit = flags[e] ? 0x80000000u : 0u;
+        const uint64_t h1 = mix64(counter_key ^ (2 * (uint64_t)e));
+        const uint64_t h2 = mix64(counter_key ^ (2 * (uint64_t)e + 1));
+        fpr[e] = (float)(uint32_t)(h1 >> 40) * (1.0f / 16777216.0f);
+        rkey[e] = (uint32_t)nb;
+        rval[e] = ((uint64_t)__builtin_bit_cast(uint32_t, (float)(uint32_t)(h2 >> 40) * (1.0f / 16777216.0f)) << 32) |
+                  ((uint32_t)i | fbit);
+    }
+}
+
+// Segment starts of keys sorted ascending (keys in [0, n]): off[v] = the first position whose key is >= v,
+// for v in [0, n] (absent keys get the next present key's start; off[n] counts the keys below n).
+__global__ __launch_bounds__(256) void k_seg_starts(const uint32_t* __restrict__ keys, int64_t N, int64_t n,
+                                                    uint64_t* off) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p <= N; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t kprev = p > 0 ? (int64_t)keys[p - 1] : -1;
+        const int64_t kcur = p < N ? (int64_t)keys[p] : n + 1;
+        for (int64_t v = kprev + 1; v <= kcur && v <= n; ++v) off[v] = (uint64_t)p;
+    }
+}
+
 // first position in [lo, hi) whose record's source row is >= bound (records sorted by source)
 __device__ __forceinline__ uint64_t src_lower_bound(const uint64_t* rv, uint64_t lo, uint64_t hi, int64_t bound) {
     while (lo < hi) {
@@ -1444,15 +1479,23 @@ size_t cand_sort_tmp_bytes(int64_t items, int end_bit) {
 
 void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
     const int64_t nk = a.n * a.k;
-    HIP_TRY(hipMemsetAsync(a.cnt, 0, (a.n + 1) * sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_cand_gen_rev, dim3(grid_for(a.n, 128)), dim3(128), 0, s, a.idx, a.flags, a.n, a.k, a.rvp,
-                       a.prio, a.mode, a.block_rows, a.block_prio_base, a.counter_key, a.fpr, a.rkey, a.rval, a.cnt);
+    if (a.mode == 2) {
+        hipLaunchKernelGGL(k_cand_gen_rev_ctr, dim3(grid_for(nk, 256, 1 << 16)), dim3(256), 0, s, a.idx, a.flags, a.n,
+                           a.k, a.counter_key, a.fpr, a.rkey, a.rval);
+    } else {
+        HIP_TRY(hipMemsetAsync(a.cnt, 0, (a.n + 1) * sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_cand_gen_rev, dim3(grid_for(a.n, 128)), dim3(128), 0, s, a.idx, a.flags, a.n, a.k, a.rvp,
+                           a.prio, a.mode, a.block_rows, a.block_prio_base, a.counter_key, a.fpr, a.rkey, a.rval, a.cnt);
+    }
     int end_bit = 1;
     while (((int64_t)1 << end_bit) <= a.n) ++end_bit;  // destinations 0..n (n = no edge)
     size_t tb = a.sort_tmp_bytes;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2, (int)nk, 0, end_bit,
                                                s));
-    scan_u64_exclusive(s, a.cnt, a.off, a.n + 1, a.scan_tmp, a.scan_tmp_bytes);
+    if (a.mode == 2)
+        hipLaunchKernelGGL(k_seg_starts, dim3(grid_for(nk + 1, 256)), dim3(256), 0, s, a.rkey2, nk, a.n, a.off);
+    else
+        scan_u64_exclusive(s, a.cnt, a.off, a.n + 1, a.scan_tmp, a.scan_tmp_bytes);
     HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_bin_count2, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
