@@ -495,27 +495,17 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         HIP_TRY(hipMemsetAsync(kept.p, 0, kKeptSlots * 8, s));
         HIP_TRY(hipEventSynchronize(hev));
         JU = WBuf<JRec>(W, 33, std::max<uint64_t>(hst[0], 1));
-        const JoinOut jo{rowoff.p, JU.p, kc.p, PL.cnt.p, krec.p, kept.p};
-        PL.begin(s);
+        const JoinOut jo{rowoff.p, JU.p, kc.p, krec.p, kept.p};
         HIP_TRY(hipEventRecord(ev[4], s));
         if (use_bf16)
             launch_join_bf16(s, DXw, Dld, dim, B16, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo, rescored.p);
         else
             launch_join_fp32(s, metric, DXw, Dld, dim, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo);
         HIP_TRY(hipEventRecord(ev[5], s));
-        scan_u64_exclusive(s, PL.cnt.p, PL.off.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
         scan_u64_exclusive(s, krec.p, koff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
-        // the record count sizes the sort; it is read while the destinations are binned
-        HIP_TRY(hipMemcpyAsync(hst + 1, PL.off.p + n, 8, hipMemcpyDeviceToHost, s));
+        // the record count sizes the sort
+        HIP_TRY(hipMemcpyAsync(hst + 1, koff.p + n, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(hev, s));
-        {
-            JoinApplyArgs jb{};
-            jb.n = n;
-            jb.off = PL.off.p;
-            jb.order = PL.big_list.p;
-            jb.bins = PL.bins.p;
-            launch_join_bins(s, jb);
-        }
         HIP_TRY(hipEventSynchronize(hev));
         const uint64_t n_rec = hst[1];
         {

@@ -327,7 +327,6 @@ struct JoinOut {
     const uint64_t* rowoff;
     JRec* U;
     unsigned long long* kc;    // kept pairs per row
-    unsigned long long* cnt;   // records per destination
     unsigned long long* krec;  // records per row
     unsigned long long* kept;  // kept pairs in all, spread over kKeptSlots counters (summed by the caller)
 };
@@ -346,11 +345,11 @@ struct JoinApplyArgs {
     int C;
     int64_t n;
     JoinOut out;
-    uint64_t* koff;          // n + 1
-    uint64_t n_records;      // 2 * koff[n]
+    uint64_t* koff;          // n + 1: exclusive prefix of the records per row
+    uint64_t n_records;      // koff[n]
     uint32_t *rkey, *rkey2;  // n_records each
     uint64_t *rval, *rval2;
-    const uint64_t* off;     // n + 1, prefix of cnt
+    uint64_t* off;           // n + 1: each destination's segment of the sorted records (written here)
     int32_t* order;          // n
     uint32_t* bins;          // 128
     void* sort_tmp;
@@ -364,8 +363,9 @@ struct JoinApplyArgs {
     unsigned long long* changes;
 };
 size_t join_sort_tmp_bytes(int64_t items, int end_bit);
-void launch_join_bins(hipStream_t s, const JoinApplyArgs& a);   // needs off, n, bins, order only
-void launch_join_apply(hipStream_t s, const JoinApplyArgs& a);  // after launch_join_bins
+// records of the kept pairs, sorted by destination (off: each destination's segment, found in the sorted
+// keys), applied in generation order
+void launch_join_apply(hipStream_t s, const JoinApplyArgs& a);
 
 // local join, BF16 rows on MFMA with FP32 re-evaluation (join_bf16.hip)
 struct Bf16Rows {

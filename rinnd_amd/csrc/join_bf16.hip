@@ -127,7 +127,6 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
     __shared__ int32_t ids[W][TB * 16];
     __shared__ float roots[W][TB * 16];
     __shared__ float nr[W][TB * 16];
-    __shared__ unsigned cc[W][TB * 16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     const int C = a.C, C2 = 2 * C;
@@ -142,7 +141,6 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
             ids[w][c] = id;
             roots[w][c] = id >= 0 ? a.hd[(int64_t)id * a.k] : 0.f;
             nr[w][c] = id >= 0 ? a.nrm[id] : 0.f;
-            cc[w][c] = 0;
         }
         wave_sync();
         const int64_t r0 = i - a.row_begin;
@@ -238,8 +236,6 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                     if (keep) {
                         U[run + __popcll(bal & lt)] =
                             JRec{(uint32_t)(j * C2 + kk) | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
-                        if (ep) atomicAdd(&cc[w][j], 1u);
-                        if (eq) atomicAdd(&cc[w][kk], 1u);
                         recs += (unsigned)ep + (unsigned)eq;
                     }
                     run += (unsigned)__popcll(bal);
@@ -251,8 +247,6 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
             a.out.krec[r0] = (unsigned long long)recs;
             kept += run;
         }
-        for (int c = lane; c < C2; c += 64)
-            if (cc[w][c]) atomicAdd(&a.out.cnt[ids[w][c]], (unsigned long long)cc[w][c]);
         wave_sync();
     }
     for (int o = 32; o > 0; o >>= 1) resc += __shfl_xor(resc, o);

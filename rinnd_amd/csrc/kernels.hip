@@ -1511,8 +1511,8 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
 // ---- local join, FP32 (update_iteration generation, mod.rs:383-447) --------------------------------
 // One workgroup per row i: the row's new (C) and old (C) candidate rows are staged in LDS, every
 // pair (j, kk) with kk in (j+1..C) over new and C+(0..C) over old is evaluated with the reference's
-// arithmetic and kept if d <= max(root_p, root_q).  Kept pairs go to the row's own region of U
-// (no global counter); per-candidate counts are summed in LDS and added to cnt once per row.
+// arithmetic and kept if d is below the root of p or of q (the flags say which).  Kept pairs go to the
+// row's own region of U (no global counter); the row's record count goes to krec.
 constexpr int kJoinThreads = 256;
 constexpr int kJoinLdsBytes = 64 * 1024;
 
@@ -1524,7 +1524,6 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
     extern __shared__ __align__(16) float jsm[];
     __shared__ int32_t ids[128];
     __shared__ float roots[128];
-    __shared__ unsigned cc[128];
     __shared__ unsigned run, nrec;
     const int C2 = 2 * C;
     const int64_t ldl = ld + 4;  // padded LDS row stride
@@ -1537,7 +1536,6 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
             const int32_t id = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
             ids[c] = id;
             roots[c] = id >= 0 ? hd[(int64_t)id * k] : 0.f;
-            cc[c] = 0;
         }
         if (threadIdx.x == 0) run = nrec = 0;
         __syncthreads();
@@ -1583,11 +1581,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
                     atomicAdd(&nrec, nr);
                 }
                 base = __shfl(base, leader);
-                if (keep) {
-                    U[base + __popcll(bal & lt)] = JRec{(uint32_t)t | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
-                    if (ep) atomicAdd(&cc[j], 1u);
-                    if (eq) atomicAdd(&cc[kk], 1u);
-                }
+                if (keep) U[base + __popcll(bal & lt)] = JRec{(uint32_t)t | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
             }
         }
         __syncthreads();
@@ -1596,8 +1590,6 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
             out.krec[r] = (unsigned long long)nrec;
             kept += run;
         }
-        for (int c = threadIdx.x; c < C2; c += blockDim.x)
-            if (cc[c]) atomicAdd(&out.cnt[ids[c]], (unsigned long long)cc[c]);
         __syncthreads();
     }
     if (threadIdx.x == 0 && kept) atomicAdd(&out.kept[blockIdx.x % kKeptSlots], kept);
@@ -1787,15 +1779,6 @@ size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
     return b;
 }
 
-// Destinations binned by record count, longest first (needs only the per-destination offsets).
-void launch_join_bins(hipStream_t s, const JoinApplyArgs& a) {
-    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.n, a.bins);
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
-}
-
-// The records of the kept pairs, sorted by destination, applied in generation order (after launch_join_bins).
 void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     const int C2 = 2 * a.C, S = a.C * C2, W = (S + 63) >> 6;
     const size_t per_wave = (size_t)round_up(2 * W * 8 + S * 4 + C2 * 4, 16);
@@ -1808,6 +1791,13 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
         HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2,
                                                    (int)a.n_records, 0, end_bit, s));
     }
+    hipLaunchKernelGGL(k_seg_starts, dim3(grid_for((int64_t)a.n_records + 1, 256)), dim3(256), 0, s, a.rkey2,
+                       (int64_t)a.n_records, a.n, a.off);
+    // destinations binned by record count, longest first
+    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.n, a.bins);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
     const int B = lanes_block(a.k, 9, 256);
     const size_t lds = (size_t)round_up((int64_t)(B + 1) * a.k * 2 + ((B + 1) * a.k + 3) / 4, 4) * 4;
     hipLaunchKernelGGL(k_heap_push_rec, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins, a.off, a.rval2,
