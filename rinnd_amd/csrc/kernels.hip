@@ -1660,6 +1660,9 @@ void launch_unpermute_heap(hipStream_t s, const int32_t* idx, const float* dist,
 // One wave per row.  The row's kept pairs sit in its region in any order; their slots t are unique, so a
 // bitmask over the C x 2C slots plus a distance per slot puts them back in loop order, and a wave prefix
 // of the mask's popcounts gives each pair its rank.
+// The records of each row's kept pairs in generation order (slot order; p's record before q's within a slot):
+// a wave per row marks the slots of its kept pairs, counts the records before every 64-slot word, and then
+// every kept pair writes its records at their ranks.
 __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict__ new_idx,
                                                       const int32_t* __restrict__ old_idx, int C, int64_t n,
                                                       JoinOut out, const uint64_t* __restrict__ koff, uint32_t* rkey,
@@ -1667,11 +1670,11 @@ __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict_
     extern __shared__ __align__(16) char jrm[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int C2 = 2 * C, S = C * C2, W = (S + 63) >> 6;
-    const int per_wave = round_up(2 * W * 8 + S * 4 + C2 * 4, 16);
+    const int per_wave = round_up(2 * W * 8 + W * 4 + C2 * 4, 16);
     unsigned long long* mp = (unsigned long long*)(jrm + w * per_wave);  // slots with a record for p
     unsigned long long* mq = mp + W;                                      // slots with a record for q
-    float* dv = (float*)(mq + W);
-    int32_t* ids = (int32_t*)(dv + S);
+    uint32_t* pw = (uint32_t*)(mq + W);                                   // records before each word
+    int32_t* ids = (int32_t*)(pw + W);
     for (int64_t i = (int64_t)blockIdx.x * 4 + w; i < n; i += (int64_t)gridDim.x * 4) {
         for (int c = lane; c < C2; c += 64) ids[c] = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
         for (int q = lane; q < 2 * W; q += 64) mp[q] = 0ull;
@@ -1679,44 +1682,44 @@ __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict_
         const uint32_t nk = (uint32_t)out.kc[i];
         const JRec* E = out.U + out.rowoff[i];
         for (uint32_t e = lane; e < nk; e += 64) {
-            const JRec q = E[e];
-            const uint32_t t = q.t & kSlotMask;
-            dv[t] = q.d;
-            if (q.t & kEmitP) atomicOr(&mp[t >> 6], 1ull << (t & 63));
-            if (q.t & kEmitQ) atomicOr(&mq[t >> 6], 1ull << (t & 63));
+            const uint32_t t = E[e].t;
+            const uint32_t s0 = t & kSlotMask;
+            if (t & kEmitP) atomicOr(&mp[s0 >> 6], 1ull << (s0 & 63));
+            if (t & kEmitQ) atomicOr(&mq[s0 >> 6], 1ull << (s0 & 63));
         }
         wave_sync();
-        uint64_t carry = koff[i];  // records are emitted in slot order, p's before q's within a slot
+        uint32_t carry = 0;
         for (int q0 = 0; q0 < W; q0 += 64) {
             const int q = q0 + lane;
-            const unsigned long long bp = q < W ? mp[q] : 0ull, bq = q < W ? mq[q] : 0ull;
-            const uint32_t pc = (uint32_t)(__popcll(bp) + __popcll(bq));
+            const uint32_t pc = q < W ? (uint32_t)(__popcll(mp[q]) + __popcll(mq[q])) : 0u;
             uint32_t incl = pc;
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t v = __shfl_up(incl, o);
                 if (lane >= o) incl += v;
             }
-            uint64_t rank = carry + (incl - pc);
-            unsigned long long b = bp | bq;
-            while (b) {
-                const int bit = __ffsll((long long)b) - 1;
-                b &= b - 1;
-                const int t = q * 64 + bit;
-                const int j = t / C2, kk = t - j * C2;
-                const int32_t p = ids[j], qq = ids[kk];
-                const uint32_t dbits = __builtin_bit_cast(uint32_t, dv[t]);
-                if ((bp >> bit) & 1ull) {
-                    rkey[rank] = (uint32_t)p;
-                    rval[rank] = ((uint64_t)dbits << 32) | (uint32_t)qq;
-                    ++rank;
-                }
-                if ((bq >> bit) & 1ull) {
-                    rkey[rank] = (uint32_t)qq;
-                    rval[rank] = ((uint64_t)dbits << 32) | (uint32_t)p;
-                    ++rank;
-                }
-            }
+            if (q < W) pw[q] = carry + incl - pc;
             carry += __shfl(incl, 63);
+        }
+        wave_sync();
+        const uint64_t base = koff[i];
+        for (uint32_t e = lane; e < nk; e += 64) {
+            const JRec rec = E[e];
+            const uint32_t s0 = rec.t & kSlotMask;
+            const int q = (int)(s0 >> 6);
+            const unsigned long long below = (1ull << (s0 & 63)) - 1ull;
+            uint64_t rank = base + pw[q] + (uint64_t)(__popcll(mp[q] & below) + __popcll(mq[q] & below));
+            const int j = (int)(s0 / (uint32_t)C2), kk = (int)(s0 - (uint32_t)j * C2);
+            const int32_t p = ids[j], qq = ids[kk];
+            const uint32_t dbits = __builtin_bit_cast(uint32_t, rec.d);
+            if (rec.t & kEmitP) {
+                rkey[rank] = (uint32_t)p;
+                rval[rank] = ((uint64_t)dbits << 32) | (uint32_t)qq;
+                ++rank;
+            }
+            if (rec.t & kEmitQ) {
+                rkey[rank] = (uint32_t)qq;
+                rval[rank] = ((uint64_t)dbits << 32) | (uint32_t)p;
+            }
         }
         wave_sync();
     }
@@ -1781,7 +1784,7 @@ size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
 
 void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     const int C2 = 2 * a.C, S = a.C * C2, W = (S + 63) >> 6;
-    const size_t per_wave = (size_t)round_up(2 * W * 8 + S * 4 + C2 * 4, 16);
+    const size_t per_wave = (size_t)round_up(2 * W * 8 + W * 4 + C2 * 4, 16);
     hipLaunchKernelGGL(k_join_records, dim3(grid_for(ceil_div(a.n, (int64_t)4), 1, 1 << 18)), dim3(256), 4 * per_wave,
                        s, a.new_idx, a.old_idx, a.C, a.n, a.out, a.koff, a.rkey, a.rval);
     int end_bit = 1;
