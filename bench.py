@@ -248,7 +248,8 @@ def main():
 # timed family -> kernel (per join precision; the synthetic data are integers, exact in bf16, so the BF16
 # build's leaf distances run on MFMA)
 KERNEL_NAMES = {"join": {"fp32": "k_join_fp32", "bf16": "k_join_bf16"}, "margins": "k_lvl_margins",
-                "leaf_dist": {"fp32": "k_leaf_dist_sq", "bf16": "k_leaf_dist_mfma"}, "leaf_push": "k_leaf_push"}
+                "leaf_dist": {"fp32": "k_leaf_dist_sq", "bf16": "k_leaf_dist_mfma/k_leaf_surv_mfma"},
+                "leaf_push": "k_leaf_push"}
 
 
 def kernel_line(stats, fam, head):

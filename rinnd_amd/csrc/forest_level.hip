@@ -6,9 +6,9 @@
 // its margin is < 0 (ties |margin| < 1e-8 decided by a random bit per point), and a degenerate split (all
 // points on one side) makes the node a leaf.  The unit normal is rounded to bf16 (the stored f32
 // hyperplane holds exactly those values, and a bf16 copy halves the bytes the side tests read);
-// the plane passes through the midpoint of the two members.  The random draws come from mix64(tree key, node id,
-// position) instead of a depth-first FastRng stream, so every node of a level is independent and
-// one level is processed for all trees at once:
+// the plane passes through the midpoint of the two members.  The random draws come from mix64 of the tree key
+// and node id (the two members) or of the tree key, node id and point (a tie) instead of a depth-first
+// FastRng stream, so every node of a level is independent and one level is processed for all trees at once:
 //   k_lvl_planes   one workgroup per active node: split points, hyperplane, offset (or leaf)
 //   k_lvl_margins  16 lanes per point, all trees: the point's entry in each tree advanced from the last
 //                  level's side and the children table, then the side of the tree's current hyperplane,

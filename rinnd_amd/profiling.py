@@ -18,8 +18,8 @@ import sys
 FAMILIES = {
     "margins": ("k_lvl_margins", r"k_lvl_margins"),
     "leaf_dist_sq": ("k_leaf_dist_sq", r"k_leaf_dist_sq"),
-    "leaf_dist_mfma": ("k_leaf_dist_mfma", r"k_leaf_dist_mfma"),
-    "leaf_push": ("k_leaf_push", r"k_leaf_push"),
+    "leaf_dist_mfma": ("k_leaf_dist_mfma/k_leaf_surv_mfma", r"k_leaf_dist_mfma|k_leaf_surv_mfma"),
+    "leaf_push": ("k_leaf_push", r"k_leaf_push"),  # k_leaf_push<16|32> and k_leaf_push_surv
     "join_fp32": ("k_join_fp32", r"k_join_fp32"),
     "join_bf16": ("k_join_bf16", r"k_join_bf16"),
 }
