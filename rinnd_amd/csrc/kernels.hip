@@ -17,6 +17,7 @@
 // update therefore carries its generation key; a destination sorts its updates by key and replays
 // checked_flagged_push sequentially, which gives the reference's heaps and change counts exactly.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "internal.h"
 
@@ -1470,10 +1471,39 @@ __global__ __launch_bounds__(256) void k_bin_scatter2(const uint64_t* __restrict
     }
 }
 
+// Stable sort of (u32 destination, u64 record) pairs on the key's low end_bit bits.  rocprim's gfx950
+// onesweep default for this pair type ranks 8-bit digits, three scatter passes for the 17..24-bit
+// destinations of a 65k..16M-point graph; here the digit is widened to at most 11 bits so that keys of up
+// to 22 bits take two passes (each pass reads and writes 12 B per record).  Block shape as the default.
+template <unsigned BITS>
+using PairSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>, BITS,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
+template <unsigned BITS>
+static hipError_t sort_pairs_bits(void* tmp, size_t& bytes, const uint32_t* kin, uint32_t* kout,
+                                  const uint64_t* vin, uint64_t* vout, int64_t items, int end_bit, hipStream_t s) {
+    return rocprim::radix_sort_pairs<PairSortConfig<BITS>>(tmp, bytes, kin, kout, vin, vout, (size_t)items, 0u,
+                                                           (unsigned)end_bit, s);
+}
+
+static hipError_t sort_pairs_u32_u64(void* tmp, size_t& bytes, const uint32_t* kin, uint32_t* kout,
+                                     const uint64_t* vin, uint64_t* vout, int64_t items, int end_bit,
+                                     hipStream_t s) {
+    const int passes = (end_bit + 10) / 11;
+    const int bits = passes > 0 ? (end_bit + passes - 1) / passes : 8;  // <= 11
+    switch (bits) {
+        case 9: return sort_pairs_bits<9>(tmp, bytes, kin, kout, vin, vout, items, end_bit, s);
+        case 10: return sort_pairs_bits<10>(tmp, bytes, kin, kout, vin, vout, items, end_bit, s);
+        case 11: return sort_pairs_bits<11>(tmp, bytes, kin, kout, vin, vout, items, end_bit, s);
+        default: return sort_pairs_bits<8>(tmp, bytes, kin, kout, vin, vout, items, end_bit, s);
+    }
+}
+
 size_t cand_sort_tmp_bytes(int64_t items, int end_bit) {
     size_t b = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                               (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)items, 0, end_bit));
+    HIP_TRY(sort_pairs_u32_u64(nullptr, b, nullptr, nullptr, nullptr, nullptr, items, end_bit, nullptr));
     return b;
 }
 
@@ -1490,8 +1520,7 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
     int end_bit = 1;
     while (((int64_t)1 << end_bit) <= a.n) ++end_bit;  // destinations 0..n (n = no edge)
     size_t tb = a.sort_tmp_bytes;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2, (int)nk, 0, end_bit,
-                                               s));
+    HIP_TRY(sort_pairs_u32_u64(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2, nk, end_bit, s));
     if (a.mode == 2)
         hipLaunchKernelGGL(k_seg_starts, dim3(grid_for(nk + 1, 256)), dim3(256), 0, s, a.rkey2, nk, a.n, a.off);
     else
@@ -1777,8 +1806,7 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
 
 size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
     size_t b = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                               (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)items, 0, end_bit));
+    HIP_TRY(sort_pairs_u32_u64(nullptr, b, nullptr, nullptr, nullptr, nullptr, items, end_bit, nullptr));
     return b;
 }
 
@@ -1791,8 +1819,8 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     while (((int64_t)1 << end_bit) < a.n) ++end_bit;
     if (a.n_records > 0) {
         size_t tb = a.sort_tmp_bytes;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2,
-                                                   (int)a.n_records, 0, end_bit, s));
+        HIP_TRY(sort_pairs_u32_u64(a.sort_tmp, tb, a.rkey, a.rkey2, a.rval, a.rval2, (int64_t)a.n_records, end_bit,
+                                   s));
     }
     hipLaunchKernelGGL(k_seg_starts, dim3(grid_for((int64_t)a.n_records + 1, 256)), dim3(256), 0, s, a.rkey2,
                        (int64_t)a.n_records, a.n, a.off);
