@@ -1112,15 +1112,17 @@ __global__ __launch_bounds__(256) void k_heap_push_lanes(const int32_t* __restri
     if ((threadIdx.x & 63) == 0 && ch) atomicAdd(changes, ch);
 }
 
-// checked_heap_push_flat (candidates.rs:433-484) on an LDS column: swap sift toward the larger child.
+// checked_heap_push_flat (candidates.rs:433-484) on an LDS column: swap sift toward the larger child.  The
+// caller keeps the root priority rt in a register and calls this only when priority < rt (the heap's first
+// test); rt is updated to the new root.
 __device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, int c, int n, float priority,
-                                               int32_t index) {
-    if (priority >= pri[c]) return;
+                                               int32_t index, float& rt) {
     bool dup = false;  // no early exit: the slot loads issue together
 #pragma unroll 8
     for (int i = 0; i < n; ++i) dup |= ix[i * B + c] == index;
     if (dup) return;
     int pos = 0;
+    float top = priority;  // the root after the sift: the larger of the value and the root's children
     for (;;) {  // both children's slots load together
         const int l = 2 * pos + 1, r = 2 * pos + 2;
         if (l >= n) break;
@@ -1145,6 +1147,7 @@ __device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, i
             bp = pr;
             bi = ir;
         }
+        if (pos == 0) top = bp;
         if (best == pos) break;
         pri[pos * B + c] = bp;
         ix[pos * B + c] = bi;
@@ -1152,6 +1155,7 @@ __device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, i
     }
     pri[pos * B + c] = priority;
     ix[pos * B + c] = index;
+    rt = top;
 }
 
 
@@ -1346,9 +1350,17 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
                 ni[j * S + c] = -1;
                 oi[j * S + c] = -1;
             }
+            // One push path for both heaps (a lane picks its heap by an LDS offset), so a wave whose lanes
+            // push into different heaps runs the sift once, not once per heap; both roots stay in registers.
+            float rt_new = __builtin_inff(), rt_old = __builtin_inff();
             auto push = [&](uint32_t x, float pri) {
-                if ((int32_t)x < 0) cand_push_lane(np, ni, S, c, C, pri, (int32_t)(x & 0x7fffffffu));  // "new" edge
-                else cand_push_lane(op, oi, S, c, C, pri, (int32_t)x);
+                const bool isnew = (int32_t)x < 0;  // "new" edge
+                float rt = isnew ? rt_new : rt_old;
+                if (pri >= rt) return;
+                const int h = isnew ? 0 : 2 * S * C;
+                cand_push_lane(np + h, ni + h, S, c, C, pri, (int32_t)(x & 0x7fffffffu), rt);
+                if (isnew) rt_new = rt;
+                else rt_old = rt;
             };
             auto rev = [&](uint64_t a, uint64_t b) {  // a ring of four records in flight ahead of the push
                 if (a >= b) return;
