@@ -1114,13 +1114,16 @@ __global__ __launch_bounds__(256) void k_heap_push_lanes(const int32_t* __restri
 
 // checked_heap_push_flat (candidates.rs:433-484) on an LDS column: swap sift toward the larger child.  The
 // caller keeps the root priority rt in a register and calls this only when priority < rt (the heap's first
-// test); rt is updated to the new root.
-__device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, int c, int n, float priority,
-                                               int32_t index, float& rt) {
-    bool dup = false;  // no early exit: the slot loads issue together
+// test); rt is updated to the new root.  The duplicate scan runs only when dupchk is set (the caller knows
+// when index cannot be in the heap).  Returns whether index was inserted.
+__device__ __forceinline__ bool cand_push_lane(float* pri, int32_t* ix, int B, int c, int n, float priority,
+                                               int32_t index, float& rt, bool dupchk) {
+    if (dupchk) {
+        bool dup = false;  // no early exit: the slot loads issue together
 #pragma unroll 8
-    for (int i = 0; i < n; ++i) dup |= ix[i * B + c] == index;
-    if (dup) return;
+        for (int i = 0; i < n; ++i) dup |= ix[i * B + c] == index;
+        if (dup) return false;
+    }
     int pos = 0;
     float top = priority;  // the root after the sift: the larger of the value and the root's children
     for (;;) {  // both children's slots load together
@@ -1156,6 +1159,7 @@ __device__ __forceinline__ void cand_push_lane(float* pri, int32_t* ix, int B, i
     pri[pos * B + c] = priority;
     ix[pos * B + c] = index;
     rt = top;
+    return true;
 }
 
 
@@ -1324,6 +1328,13 @@ __device__ __forceinline__ uint64_t src_lower_bound(const uint64_t* rv, uint64_t
     return lo;
 }
 
+// FIDS (k <= 16 and ids below 2^26): the row's own ids are kept in registers.  A destination's reverse
+// sources are distinct and differ from v, and v's row holds distinct ids, so an id can reach the same heap
+// twice only if it is both a reverse source and in v's row: only then does a push scan the heap for a
+// duplicate (pushed_* records the row slots whose id entered that heap).  A new-heap entry whose id is
+// row slot j carries j + 1 in bits 26..30, which flags slot j old at the end (mark_old_flags) without
+// searching the heap for every slot.
+template <bool FIDS>
 __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict__ order, const uint32_t* bins,
                                                        const uint64_t* __restrict__ off, const uint64_t* __restrict__ rv,
                                                        const int32_t* __restrict__ idx, uint8_t* flags,
@@ -1353,14 +1364,36 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
             // One push path for both heaps (a lane picks its heap by an LDS offset), so a wave whose lanes
             // push into different heaps runs the sift once, not once per heap; both roots stay in registers.
             float rt_new = __builtin_inff(), rt_old = __builtin_inff();
+            int32_t fid[16];
+            uint32_t pushed_new = 0, pushed_old = 0;
+            if (FIDS) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) fid[j] = j < k ? idx[(int64_t)v * k + j] : -1;
+            }
             auto push = [&](uint32_t x, float pri) {
                 const bool isnew = (int32_t)x < 0;  // "new" edge
-                float rt = isnew ? rt_new : rt_old;
+                const float rn = rt_new, ro = rt_old;
+                float rt = isnew ? rn : ro;
                 if (pri >= rt) return;
                 const int h = isnew ? 0 : 2 * S * C;
-                cand_push_lane(np + h, ni + h, S, c, C, pri, (int32_t)(x & 0x7fffffffu), rt);
-                if (isnew) rt_new = rt;
-                else rt_old = rt;
+                int32_t id = (int32_t)(x & 0x7fffffffu);
+                bool dupchk = true;
+                int m = 0;  // 1 + the row slot holding id, or 0
+                if (FIDS) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) m = fid[j] == id ? j + 1 : m;
+                    const uint32_t pn = pushed_new, po = pushed_old;  // values: no select of addresses
+                    dupchk = m && (((isnew ? pn : po) >> (m - 1)) & 1u);
+                    if (isnew) id |= m << 26;
+                }
+                const bool ins = cand_push_lane(np + h, ni + h, S, c, C, pri, id, rt, dupchk);
+                if (FIDS) {  // selects, not branches: the pairs stay in registers
+                    const uint32_t bit = ins && m ? 1u << (m - 1) : 0u;
+                    pushed_new |= isnew ? bit : 0u;
+                    pushed_old |= isnew ? 0u : bit;
+                }
+                rt_new = isnew ? rt : rt_new;
+                rt_old = isnew ? rt_old : rt;
             };
             auto rev = [&](uint64_t a, uint64_t b) {  // a ring of four records in flight ahead of the push
                 if (a >= b) return;
@@ -1414,17 +1447,23 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
             }
             unsigned long long nn = 0, no = 0;
             for (int j = 0; j < C; ++j) {
-                nn += ni[j * S + c] >= 0;
+                const int32_t e = ni[j * S + c];
+                nn += e >= 0;
                 no += oi[j * S + c] >= 0;
+                if (FIDS && e >= (1 << 26)) {  // row slot (e >> 26) - 1 became a new candidate: flag it old
+                    flags[(int64_t)v * k + (e >> 26) - 1] = 0;
+                    ni[j * S + c] = e & ((1 << 26) - 1);
+                }
             }
             const unsigned long long pr = nn * (nn - (nn > 0)) / 2 + nn * no;
             row_pairs[v] = pr;
             st_rows += nn + no;
             st_pairs += pr;
             // mark_old_flags (candidates.rs:354-417): the row's entries that became new candidates are
-            // flagged old.  Only this lane reads or writes row v's flags in this kernel.
+            // flagged old (FIDS: from the tags, in the loop above).  Only this lane reads or writes row v's
+            // flags in this kernel.
             const int64_t f0 = (int64_t)v * k;
-            for (int j = 0; j < k; ++j) {
+            for (int j = 0; j < (FIDS ? 0 : k); ++j) {
                 const int32_t nb = idx[f0 + j];
                 if (nb < 0 || !flags[f0 + j]) continue;
                 for (int cc = 0; cc < C; ++cc)
@@ -1543,9 +1582,14 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
     hipLaunchKernelGGL(k_bin_scatter2, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins,
                        a.order);
     const int B = lanes_block(a.C, 16, 128);
-    hipLaunchKernelGGL(k_cand_push_rev, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)(B + 1) * a.C * 16, s, a.order,
-                       a.bins, a.off, a.rval2, a.idx, a.flags, a.fpr, a.k, a.mode, a.block_rows, a.new_idx, a.old_idx,
-                       a.C, a.row_pairs, a.counts);
+    if (a.k <= 16 && a.n < ((int64_t)1 << 26))
+        hipLaunchKernelGGL(k_cand_push_rev<true>, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)(B + 1) * a.C * 16, s,
+                           a.order, a.bins, a.off, a.rval2, a.idx, a.flags, a.fpr, a.k, a.mode, a.block_rows, a.new_idx,
+                           a.old_idx, a.C, a.row_pairs, a.counts);
+    else
+        hipLaunchKernelGGL(k_cand_push_rev<false>, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)(B + 1) * a.C * 16, s,
+                           a.order, a.bins, a.off, a.rval2, a.idx, a.flags, a.fpr, a.k, a.mode, a.block_rows, a.new_idx,
+                           a.old_idx, a.C, a.row_pairs, a.counts);
 }
 
 

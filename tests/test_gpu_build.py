@@ -198,8 +198,9 @@ def gpu_nn_descent(ctx, X, k, n_trees, leaf, C_, iters, delta=0.001, metric="sqe
 
 
 @pytest.mark.parametrize("threads", [1, 8])
-@pytest.mark.parametrize("n,d,k,leaf", [(600, 16, 10, 20), (2500, 32, 15, 40)])
+@pytest.mark.parametrize("n,d,k,leaf", [(600, 16, 10, 20), (2500, 32, 15, 40), (1500, 24, 20, 40)])
 def test_nn_descent_bit_exact(ctx, n, d, k, leaf, threads):
+    """k <= 16 keeps the candidate rows' own ids in registers; k = 20 takes the general duplicate scan."""
     X = uniform(n, d, seed=n)
     C_ = min(k, 60)
     gi, gd, gf, gs, gst, _ = gpu_nn_descent(ctx, X, k, 4, leaf, C_, 6, threads=threads)
