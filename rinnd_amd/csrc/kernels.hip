@@ -1742,31 +1742,38 @@ void launch_unpermute_heap(hipStream_t s, const int32_t* idx, const float* dist,
 }
 
 // ---- join records in generation order + stable destination sort + lane apply ----------------------
-// One wave per row.  The row's kept pairs sit in its region in any order; their slots t are unique, so a
-// bitmask over the C x 2C slots plus a distance per slot puts them back in loop order, and a wave prefix
-// of the mask's popcounts gives each pair its rank.
-// The records of each row's kept pairs in generation order (slot order; p's record before q's within a slot):
-// a wave per row marks the slots of its kept pairs, counts the records before every 64-slot word, and then
-// every kept pair writes its records at their ranks.
+// The records of each row's kept pairs in generation order (slot order; p's record before q's within a
+// slot).  The row's kept pairs sit in its region in any order and their slots are unique: a group of G
+// lanes per row marks the slots in two bitmasks over the C x 2C slots (records for p, records for q),
+// scans the record counts of the 64-slot words, and then every kept pair writes its records at their ranks.
+template <int G>
 __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict__ new_idx,
                                                       const int32_t* __restrict__ old_idx, int C, int64_t n,
                                                       JoinOut out, const uint64_t* __restrict__ koff, uint32_t* rkey,
                                                       uint64_t* rval) {
+    // G lanes per row (64 / G rows per wave): a row keeps about as many pairs as it has candidates, so 16
+    // lanes cover it in one pass when its slot masks fit in 16 words
     extern __shared__ __align__(16) char jrm[];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, gl = lane & (G - 1);
+    const int grp = threadIdx.x / G;  // group within the block
     const int C2 = 2 * C, S = C * C2, W = (S + 63) >> 6;
-    const int per_wave = round_up(2 * W * 8 + W * 4 + C2 * 4, 16);
-    unsigned long long* mp = (unsigned long long*)(jrm + w * per_wave);  // slots with a record for p
-    unsigned long long* mq = mp + W;                                      // slots with a record for q
-    uint32_t* pw = (uint32_t*)(mq + W);                                   // records before each word
+    const int per_group = round_up(2 * W * 8 + W * 4 + C2 * 4, 16);
+    unsigned long long* mp = (unsigned long long*)(jrm + grp * per_group);  // slots with a record for p
+    unsigned long long* mq = mp + W;                                        // slots with a record for q
+    uint32_t* pw = (uint32_t*)(mq + W);                                     // records before each word
     int32_t* ids = (int32_t*)(pw + W);
-    for (int64_t i = (int64_t)blockIdx.x * 4 + w; i < n; i += (int64_t)gridDim.x * 4) {
-        for (int c = lane; c < C2; c += 64) ids[c] = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
-        for (int q = lane; q < 2 * W; q += 64) mp[q] = 0ull;
+    constexpr int RPW = 64 / G;  // rows per wave
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * RPW; r0 < n;
+         r0 += waves * RPW) {  // wave-uniform
+        const int64_t i = r0 + lane / G;
+        const bool valid = i < n;
+        for (int c = gl; c < C2; c += G) ids[c] = !valid ? -1 : c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
+        for (int q = gl; q < 2 * W; q += G) mp[q] = 0ull;
         wave_sync();
-        const uint32_t nk = (uint32_t)out.kc[i];
-        const JRec* E = out.U + out.rowoff[i];
-        for (uint32_t e = lane; e < nk; e += 64) {
+        const uint32_t nk = valid ? (uint32_t)out.kc[i] : 0u;
+        const JRec* E = out.U + (valid ? out.rowoff[i] : 0);
+        for (uint32_t e = gl; e < nk; e += G) {
             const uint32_t t = E[e].t;
             const uint32_t s0 = t & kSlotMask;
             if (t & kEmitP) atomicOr(&mp[s0 >> 6], 1ull << (s0 & 63));
@@ -1774,20 +1781,20 @@ __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict_
         }
         wave_sync();
         uint32_t carry = 0;
-        for (int q0 = 0; q0 < W; q0 += 64) {
-            const int q = q0 + lane;
+        for (int q0 = 0; q0 < W; q0 += G) {  // exclusive scan of the words' record counts
+            const int q = q0 + gl;
             const uint32_t pc = q < W ? (uint32_t)(__popcll(mp[q]) + __popcll(mq[q])) : 0u;
             uint32_t incl = pc;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
+            for (int o = 1; o < G; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o, G);
+                if (gl >= o) incl += v;
             }
             if (q < W) pw[q] = carry + incl - pc;
-            carry += __shfl(incl, 63);
+            carry += __shfl(incl, G - 1, G);
         }
         wave_sync();
-        const uint64_t base = koff[i];
-        for (uint32_t e = lane; e < nk; e += 64) {
+        const uint64_t base = valid ? koff[i] : 0;
+        for (uint32_t e = gl; e < nk; e += G) {
             const JRec rec = E[e];
             const uint32_t s0 = rec.t & kSlotMask;
             const int q = (int)(s0 >> 6);
@@ -1868,9 +1875,13 @@ size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
 
 void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     const int C2 = 2 * a.C, S = a.C * C2, W = (S + 63) >> 6;
-    const size_t per_wave = (size_t)round_up(2 * W * 8 + W * 4 + C2 * 4, 16);
-    hipLaunchKernelGGL(k_join_records, dim3(grid_for(ceil_div(a.n, (int64_t)4), 1, 1 << 18)), dim3(256), 4 * per_wave,
-                       s, a.new_idx, a.old_idx, a.C, a.n, a.out, a.koff, a.rkey, a.rval);
+    const size_t per_group = (size_t)round_up(2 * W * 8 + W * 4 + C2 * 4, 16);
+    if (W <= 16)
+        hipLaunchKernelGGL(k_join_records<16>, dim3(grid_for(ceil_div(a.n, (int64_t)16), 1, 1 << 18)), dim3(256),
+                           16 * per_group, s, a.new_idx, a.old_idx, a.C, a.n, a.out, a.koff, a.rkey, a.rval);
+    else
+        hipLaunchKernelGGL(k_join_records<64>, dim3(grid_for(ceil_div(a.n, (int64_t)4), 1, 1 << 18)), dim3(256),
+                           4 * per_group, s, a.new_idx, a.old_idx, a.C, a.n, a.out, a.koff, a.rkey, a.rval);
     int end_bit = 1;
     while (((int64_t)1 << end_bit) < a.n) ++end_bit;
     if (a.n_records > 0) {
