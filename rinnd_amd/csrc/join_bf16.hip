@@ -49,29 +49,62 @@ __global__ void k_col_mean(const double* sums, int64_t n, int dim, float* mu) {
         mu[c] = (float)(sums[c] / (double)n);
 }
 
-// exactness test: flag[0] = 1 if some coordinate is not an integer in [-256, 256];
-// maxnorm = max over rows of |x|^2 (as ordered bits of a non-negative double)
-// exactness test: flag bit 0 = some coordinate is not an integer in [-256, 256], bit 1 = some
-// coordinate is negative; maxnorm = max over rows of |x|^2 (as ordered bits of a non-negative double)
-__global__ void k_exact_check(const float* __restrict__ X, int64_t ld, int64_t n, int dim, int* flag,
-                              unsigned long long* maxnorm) {
-    const int lane = threadIdx.x & 63;
+// One pass over the rows for the BF16 join: the exactness test (flag bit 0: a value that is not an integer
+// in [-256, 256]; bit 1: a negative value; the largest squared row norm in FP64) and the uncentred bf16 rows with the FP32 norms of the rounded rows, which are the rows exact mode uses
+// (other data are converted again after centring).  16 lanes per row, 8 consecutive dims per lane.  Exact
+// data have integer squared norms below 2^24, so the FP32 norms do not depend on the summation order.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_check_to_bf16(const float* __restrict__ X, int64_t ld, int64_t n, int dim,
+                                                       uint16_t* __restrict__ Xb, int64_t ldb,
+                                                       float* __restrict__ nrm, int* flag,
+                                                       unsigned long long* maxnorm) {
+    const int lane = threadIdx.x & 63, gl = lane & 15;
     const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     double mx = 0.0;
     int bad = 0;
-    for (int64_t r = wid; r < n; r += nw) {
+    for (int64_t r0 = wid * 4; r0 < n; r0 += nw * 4) {  // wave-uniform: four rows per wave
+        const int64_t r = r0 + (lane >> 4);
+        const bool valid = r < n;
         double s = 0.0;
-        for (int c = lane; c < dim; c += 64) {
-            const float v = X[r * ld + c];
-            bad |= !(v == rintf(v) && fabsf(v) <= 256.0f) ? 1 : 0;
-            bad |= v < 0.0f ? 2 : 0;
-            s += (double)v * (double)v;
+        float sf = 0.f;
+        for (int64_t c0 = 8 * gl; c0 < ldb; c0 += 128) {
+            float v[8];
+            if (VEC && valid && c0 + 8 <= dim) {
+                const float4 p = *reinterpret_cast<const float4*>(X + r * ld + c0);
+                const float4 q = *reinterpret_cast<const float4*>(X + r * ld + c0 + 4);
+                v[0] = p.x, v[1] = p.y, v[2] = p.z, v[3] = p.w, v[4] = q.x, v[5] = q.y, v[6] = q.z, v[7] = q.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = valid && c0 + e < dim ? X[r * ld + c0 + e] : 0.f;  // pads are 0
+            }
+            uint32_t w[4];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                bad |= !(v[e] == rintf(v[e]) && fabsf(v[e]) <= 256.0f) ? 1 : 0;
+                bad |= v[e] < 0.0f ? 2 : 0;
+                s += (double)v[e] * (double)v[e];
+                const uint16_t h = f2bf(v[e]);
+                const float f = bf2f(h);
+                sf = __builtin_fmaf(f, f, sf);
+                if (e & 1) w[e >> 1] |= (uint32_t)h << 16;
+                else w[e >> 1] = h;
+            }
+            if (valid) *reinterpret_cast<uint4*>(Xb + r * ldb + c0) = make_uint4(w[0], w[1], w[2], w[3]);
         }
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        mx = fmax(mx, s);
+        for (int o = 8; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o, 16);
+            sf += __shfl_xor(sf, o, 16);
+        }
+        if (valid) {
+            mx = fmax(mx, s);
+            if (gl == 0) nrm[r] = sf;
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o);
+    for (int o = 32; o > 0; o >>= 1) {
+        bad |= __shfl_xor(bad, o);
+        mx = fmax(mx, __shfl_xor(mx, o));
+    }
     if (lane == 0) {  // one pair of atomics per wave
         if (bad) atomicOr(flag, bad);
         atomicMax(maxnorm, (unsigned long long)__double_as_longlong(mx));
@@ -517,7 +550,17 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
     unsigned long long* mx = (unsigned long long*)(R.aux.p + 1);
     double* sums = R.aux.p + 2;
     HIP_TRY(hipMemsetAsync(R.aux.p, 0, 16, s));
-    hipLaunchKernelGGL(k_exact_check, dim3(1024), dim3(256), 0, s, X, ld, n, dim, flag, mx);
+    R.ldb = round_up(dim, 16);
+    R.Xb.ensure((size_t)std::max<int64_t>(n, 1) * R.ldb);
+    R.nrm.ensure(std::max<int64_t>(n, 1));
+    // the exactness test and the uncentred rows in one pass (kept when the data are exact)
+    const unsigned cg = (unsigned)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 16), 1), 8192);
+    if (((uintptr_t)X & 15) == 0 && ld % 4 == 0)
+        hipLaunchKernelGGL(k_check_to_bf16<true>, dim3(cg), dim3(256), 0, s, X, ld, n, dim, R.Xb.p, R.ldb, R.nrm.p,
+                           flag, mx);
+    else
+        hipLaunchKernelGGL(k_check_to_bf16<false>, dim3(cg), dim3(256), 0, s, X, ld, n, dim, R.Xb.p, R.ldb, R.nrm.p,
+                           flag, mx);
     int f = 0;
     unsigned long long mb = 0;
     HIP_TRY(hipMemcpyAsync(&f, flag, 4, hipMemcpyDeviceToHost, s));
@@ -527,20 +570,15 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
     std::memcpy(&maxnorm, &mb, 8);
     // integers in [-256, 256] with |x|^2 <= 2^22, or non-negative with |x|^2 <= 2^23
     R.exact = ((f & 1) == 0 && (maxnorm <= 4194304.0 || ((f & 2) == 0 && maxnorm <= 8388608.0))) ? 1 : 0;
-    R.ldb = round_up(dim, 16);
-    R.Xb.ensure((size_t)std::max<int64_t>(n, 1) * R.ldb);
-    R.nrm.ensure(std::max<int64_t>(n, 1));
-    const float* mu = nullptr;
-    if (!R.exact) {
+    if (!R.exact) {  // centred rows
         HIP_TRY(hipMemsetAsync(sums, 0, dim * 8, s));
         R.mu.ensure(dim);
         hipLaunchKernelGGL(k_col_sums, dim3(1024), dim3(256), 0, s, X, ld, n, dim, sums);
         hipLaunchKernelGGL(k_col_mean, dim3(ceil_div(dim, 256)), dim3(256), 0, s, sums, n, dim, R.mu.p);
-        mu = R.mu.p;
         HIP_TRY(hipStreamSynchronize(s));
+        hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)std::min<int64_t>(ceil_div(n, 4), 1 << 16)), dim3(256), 0, s, X,
+                           ld, n, dim, R.mu.p, R.Xb.p, R.ldb, R.nrm.p);
     }
-    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)std::min<int64_t>(ceil_div(n, 4), 1 << 16)), dim3(256), 0, s, X, ld,
-                       n, dim, mu, R.Xb.p, R.ldb, R.nrm.p);
     HIP_TRY(hipGetLastError());
 }
 
