@@ -54,6 +54,66 @@ __device__ __forceinline__ void heap_push_col(float* hd, int32_t* hi, uint8_t* h
     hf[pos * B + c] = 1;
 }
 
+// The same push once the caller has shown that it is accepted: d is below the root (kept in a register by
+// the caller) and x is not in the heap.  Returns the new root distance; evicted receives the id of the old
+// root, which leaves the heap.
+__device__ __forceinline__ float heap_push_col_new(float* hd, int32_t* hi, uint8_t* hf, int B, int c, int k, int32_t x,
+                                                   float d, int32_t& evicted) {
+    evicted = hi[c];
+    int pos = 0;
+    float top = d;
+    for (;;) {
+        const int l = 2 * pos + 1, r = l + 1;
+        if (l >= k) break;
+        const float dl = hd[l * B + c];
+        const int32_t il = hi[l * B + c];
+        const uint8_t fl = hf[l * B + c];
+        float dr = dl;
+        int32_t ir = il;
+        uint8_t fr = fl;
+        if (r < k) {
+            dr = hd[r * B + c];
+            ir = hi[r * B + c];
+            fr = hf[r * B + c];
+        }
+        const bool right = r < k && dl < dr;  // the right child only if strictly larger
+        const float dc = right ? dr : dl;
+        if (d >= dc) break;
+        if (pos == 0) top = dc;
+        hd[pos * B + c] = dc;
+        hi[pos * B + c] = right ? ir : il;
+        hf[pos * B + c] = right ? fr : fl;
+        pos = right ? r : l;
+    }
+    hd[pos * B + c] = d;
+    hi[pos * B + c] = x;
+    hf[pos * B + c] = 1;
+    return top;
+}
+
+// A tree offers each partner id to a row at most once (leaf members are distinct), so an offered id can
+// already be in the row's heap only if it was there when the tree's pass began and has not been evicted
+// since.  REG (k <= 16): those ids are kept in registers (an evicted id is cleared), which replaces the
+// heap's duplicate scan by register compares.
+template <bool REG>
+__device__ __forceinline__ void leaf_offer(float* hd, int32_t* hi, uint8_t* hf, int S, int c, int k, int32_t x, float d,
+                                           float& root, int32_t (&sid)[16]) {
+    if (d >= root) return;
+    if (REG) {
+        bool dup = false;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) dup |= sid[j] == x;
+        if (dup) return;
+        int32_t ev;
+        root = heap_push_col_new(hd, hi, hf, S, c, k, x, d, ev);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) sid[j] = sid[j] == ev ? -1 : sid[j];
+    } else {
+        heap_push_col(hd, hi, hf, S, c, k, x, d);
+        root = hd[c];
+    }
+}
+
 template <bool STAGED>
 __global__ __launch_bounds__(256) void k_leaf_init(int metric, const float* __restrict__ X, int64_t ld, int dim,
                                                    const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
@@ -275,7 +335,7 @@ __global__ __launch_bounds__(256) void k_leaf_dist_sq(const float* __restrict__ 
 // replayed.  Waves never wait for each other.  16-column tiles leave room for a third workgroup per CU,
 // which hides the sift latency of the many accepted pushes of the first trees; 32-column tiles halve the
 // chunk overhead once most pushes are rejected.
-template <int COLS>
+template <int COLS, bool REG>
 __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ tidx, const int64_t* __restrict__ leaves,
                                                    const int32_t* __restrict__ mark, const int64_t* __restrict__ rank,
                                                    int64_t n, int mcap, const float* __restrict__ D, int64_t Dl4,
@@ -306,6 +366,9 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
         const int32_t hrow = m ? tidx[pos] : -1;
         wave_heap_rows<true>(hrow, k, dist, idx, flags, hd, hi, hf, S, w * 64);
         float root = m ? hd[c] : 0.f;
+        int32_t sid[16];  // REG: the row's ids at the start of the pass that are still in the heap
+#pragma unroll
+        for (int j = 0; j < 16; ++j) sid[j] = REG && m && j < k ? hi[j * S + c] : -1;
         int mw = m;  // the wave's longest row
         for (int o = 32; o > 0; o >>= 1) mw = max(mw, __shfl_xor(mw, o));
         const int64_t rows = min((int64_t)64, n - base);
@@ -349,10 +412,7 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
                     const int jn = msk ? __ffs((int)msk) - 1 : -1;
                     const int32_t xn = jn >= 0 ? tidx[s + c0 + jn] : 0;
                     const float d = tile[lane * kPushTL + j];
-                    if (d < root) {
-                        heap_push_col(hd, hi, hf, S, c, k, xj, d);
-                        root = hd[c];
-                    }
+                    leaf_offer<REG>(hd, hi, hf, S, c, k, xj, d, root, sid);
                     j = jn;
                     xj = xn;
                 }
@@ -365,6 +425,7 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
 // The pushes of a survivor tree: each position's column mask and its surviving distances (in column
 // order at the start of its D row) from k_leaf_surv_mfma; the lane walks them in leaf order, the next
 // survivor's distance and partner id in flight while the current one is pushed.
+template <bool REG>
 __global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restrict__ tidx,
                                                         const int64_t* __restrict__ leaves,
                                                         const int32_t* __restrict__ mark,
@@ -393,6 +454,9 @@ __global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restric
         wave_heap_rows<true>(hrow, k, dist, idx, flags, hd, hi, hf, S, w * 64);
         if (m) {
             float root = hd[c];
+            int32_t sid[16];  // REG: the row's ids at the start of the pass that are still in the heap
+#pragma unroll
+            for (int j = 0; j < 16; ++j) sid[j] = REG && j < k ? hi[j * S + c] : -1;
             const uint4 m0 = *reinterpret_cast<const uint4*>(Msk + pos * 8);
             const uint4 m1 = *reinterpret_cast<const uint4*>(Msk + pos * 8 + 4);
             const uint32_t words[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
@@ -411,10 +475,7 @@ __global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restric
                     const float d = dn;
                     ++r;
                     dn = Dr[r < Dl4 ? r : 0];
-                    if (d < root) {
-                        heap_push_col(hd, hi, hf, S, c, k, xj, d);
-                        root = hd[c];
-                    }
+                    leaf_offer<REG>(hd, hi, hf, S, c, k, xj, d, root, sid);
                     j = jn;
                     xj = xn;
                 }
@@ -455,6 +516,18 @@ int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size) {
     return P.mcap ? std::max<int64_t>(n, 1) * P.Dl4 : 0;
 }
 
+template <int COLS>
+static void launch_leaf_push(hipStream_t s, unsigned grid, int B, size_t lds, const int32_t* tree_indices,
+                             const int64_t* leaves, const LeafInitWs& ws, int64_t n, const TilePlan& P, int32_t* idx,
+                             float* dist, uint8_t* flags, int k) {
+    if (k <= 16)
+        hipLaunchKernelGGL((k_leaf_push<COLS, true>), dim3(grid), dim3(B), lds, s, tree_indices, leaves, ws.mark, ws.rank,
+                           n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+    else
+        hipLaunchKernelGGL((k_leaf_push<COLS, false>), dim3(grid), dim3(B), lds, s, tree_indices, leaves, ws.mark,
+                           ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+}
+
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
                       const int64_t* leaves, const int64_t* n_leaves_dev, int64_t max_leaves, int64_t leaf_size,
                       int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws,
@@ -481,8 +554,14 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
             dist_timer.stop(s);
             push_timer.begin(s);
             const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
-            hipLaunchKernelGGL(k_leaf_push_surv, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s, tree_indices,
-                               leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, ws.surv_mask, idx, dist, flags, k);
+            if (k <= 16)
+                hipLaunchKernelGGL(k_leaf_push_surv<true>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s,
+                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, ws.surv_mask, idx,
+                                   dist, flags, k);
+            else
+                hipLaunchKernelGGL(k_leaf_push_surv<false>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s,
+                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, ws.surv_mask, idx,
+                                   dist, flags, k);
             push_timer.stop(s);
             // per row: bf16 row + norm + root + id read, mask written (dist); heap, leaf lookup, mask (push);
             // the per-survivor bytes are added by the caller from the survivor counter
@@ -504,11 +583,11 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
             push_timer.begin(s);
             const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
             if (early)
-                hipLaunchKernelGGL(k_leaf_push<16>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 17 * 4, s,
-                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+                launch_leaf_push<16>(s, pgrid, PB, init_heap_bytes(PB + 1, k) + (size_t)PB * 17 * 4, tree_indices, leaves,
+                                     ws, n, P, idx, dist, flags, k);
             else
-                hipLaunchKernelGGL(k_leaf_push<32>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k) + (size_t)PB * 33 * 4, s,
-                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, idx, dist, flags, k);
+                launch_leaf_push<32>(s, pgrid, PB, init_heap_bytes(PB + 1, k) + (size_t)PB * 33 * 4, tree_indices, leaves,
+                                     ws, n, P, idx, dist, flags, k);
             push_timer.stop(s);
             // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
             // per-pair parts (4 B of D written per ordered pair; 4 B of D and 4 B of partner id read) once it
