@@ -114,6 +114,44 @@ __device__ __forceinline__ void wave_rows(int32_t row, int k, T* g, T* l, int S,
     if (LOAD) wave_sync();
 }
 
+// A neighbour-heap push (neighbor_heap.rs:207-243) on an LDS column once the caller has shown that it is
+// accepted: d is below the root (kept in a register by
+// the caller) and x is not in the heap.  Returns the new root distance; evicted receives the id of the old
+// root, which leaves the heap.
+__device__ __forceinline__ float heap_push_col_new(float* hd, int32_t* hi, uint8_t* hf, int B, int c, int k, int32_t x,
+                                                   float d, int32_t& evicted) {
+    evicted = hi[c];
+    int pos = 0;
+    float top = d;
+    for (;;) {
+        const int l = 2 * pos + 1, r = l + 1;
+        if (l >= k) break;
+        const float dl = hd[l * B + c];
+        const int32_t il = hi[l * B + c];
+        const uint8_t fl = hf[l * B + c];
+        float dr = dl;
+        int32_t ir = il;
+        uint8_t fr = fl;
+        if (r < k) {
+            dr = hd[r * B + c];
+            ir = hi[r * B + c];
+            fr = hf[r * B + c];
+        }
+        const bool right = r < k && dl < dr;  // the right child only if strictly larger
+        const float dc = right ? dr : dl;
+        if (d >= dc) break;
+        if (pos == 0) top = dc;
+        hd[pos * B + c] = dc;
+        hi[pos * B + c] = right ? ir : il;
+        hf[pos * B + c] = right ? fr : fl;
+        pos = right ? r : l;
+    }
+    hd[pos * B + c] = d;
+    hi[pos * B + c] = x;
+    hf[pos * B + c] = 1;
+    return top;
+}
+
 // operand and accumulator vectors of the 16x16x16 bf16 MFMA
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));

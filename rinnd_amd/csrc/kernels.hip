@@ -1817,6 +1817,10 @@ __global__ __launch_bounds__(256) void k_join_records(const int32_t* __restrict_
     }
 }
 
+// REG (k <= 16): the lane mirrors its heap's ids in registers (one slot per entry, unordered), so the
+// duplicate test of a push below the root is a register compare; an accepted push replaces one occurrence
+// of the evicted id by the new one.
+template <bool REG>
 __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict__ order, const uint32_t* bins,
                                                        const uint64_t* __restrict__ off, const uint64_t* __restrict__ rv,
                                                        int32_t* idx, float* dist, uint8_t* flags, int k,
@@ -1838,28 +1842,42 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
             const int64_t L = (int64_t)(off[v + 1] - o0);
             const uint64_t* R = rv + o0;
             float root = hd[c];
+            int32_t hid[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) hid[j] = REG && j < k ? hi[j * S + c] : -1;
+            auto offer = [&](uint64_t q) {
+                const float d = __builtin_bit_cast(float, (uint32_t)(q >> 32));
+                const int32_t x = (int32_t)(uint32_t)q;
+                if (d >= root) return;
+                if (REG) {
+                    bool dup = false;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) dup |= hid[j] == x;
+                    if (dup) return;
+                    int32_t ev;
+                    root = heap_push_col_new(hd, hi, hf, S, c, k, x, d, ev);
+                    bool done = false;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const bool hit = !done && j < k && hid[j] == ev;
+                        hid[j] = hit ? x : hid[j];
+                        done |= hit;
+                    }
+                    ++ch;
+                } else if (heap_push_lane(hd, hi, hf, S, c, k, x, d)) {
+                    ++ch;
+                    root = hd[c];
+                }
+            };
             int64_t r = 0;
             for (; r + 8 <= L; r += 8) {  // eight records in flight per batch
                 uint64_t q[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) q[u] = R[r + u];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const float d = __builtin_bit_cast(float, (uint32_t)(q[u] >> 32));
-                    if (d < root && heap_push_lane(hd, hi, hf, S, c, k, (int32_t)(uint32_t)q[u], d)) {
-                        ++ch;
-                        root = hd[c];
-                    }
-                }
+                for (int u = 0; u < 8; ++u) offer(q[u]);
             }
-            for (; r < L; ++r) {
-                const uint64_t q = R[r];
-                const float d = __builtin_bit_cast(float, (uint32_t)(q >> 32));
-                if (d < root && heap_push_lane(hd, hi, hf, S, c, k, (int32_t)(uint32_t)q, d)) {
-                    ++ch;
-                    root = hd[c];
-                }
-            }
+            for (; r < L; ++r) offer(R[r]);
         }
         wave_heap_rows<false>(v, k, dist, idx, flags, hd, hi, hf, S, c & ~63);
     }
@@ -1898,8 +1916,12 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
     const int B = lanes_block(a.k, 9, 256);
     const size_t lds = (size_t)round_up((int64_t)(B + 1) * a.k * 2 + ((B + 1) * a.k + 3) / 4, 4) * 4;
-    hipLaunchKernelGGL(k_heap_push_rec, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins, a.off, a.rval2,
-                       a.idx, a.dist, a.flags, a.k, a.changes);
+    if (a.k <= 16)
+        hipLaunchKernelGGL(k_heap_push_rec<true>, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins,
+                           a.off, a.rval2, a.idx, a.dist, a.flags, a.k, a.changes);
+    else
+        hipLaunchKernelGGL(k_heap_push_rec<false>, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins,
+                           a.off, a.rval2, a.idx, a.dist, a.flags, a.k, a.changes);
 }
 
 // ---- sort_all (neighbor_heap.rs:293-320): stable ascending sort per row -------------------------

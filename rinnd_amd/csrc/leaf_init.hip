@@ -54,43 +54,6 @@ __device__ __forceinline__ void heap_push_col(float* hd, int32_t* hi, uint8_t* h
     hf[pos * B + c] = 1;
 }
 
-// The same push once the caller has shown that it is accepted: d is below the root (kept in a register by
-// the caller) and x is not in the heap.  Returns the new root distance; evicted receives the id of the old
-// root, which leaves the heap.
-__device__ __forceinline__ float heap_push_col_new(float* hd, int32_t* hi, uint8_t* hf, int B, int c, int k, int32_t x,
-                                                   float d, int32_t& evicted) {
-    evicted = hi[c];
-    int pos = 0;
-    float top = d;
-    for (;;) {
-        const int l = 2 * pos + 1, r = l + 1;
-        if (l >= k) break;
-        const float dl = hd[l * B + c];
-        const int32_t il = hi[l * B + c];
-        const uint8_t fl = hf[l * B + c];
-        float dr = dl;
-        int32_t ir = il;
-        uint8_t fr = fl;
-        if (r < k) {
-            dr = hd[r * B + c];
-            ir = hi[r * B + c];
-            fr = hf[r * B + c];
-        }
-        const bool right = r < k && dl < dr;  // the right child only if strictly larger
-        const float dc = right ? dr : dl;
-        if (d >= dc) break;
-        if (pos == 0) top = dc;
-        hd[pos * B + c] = dc;
-        hi[pos * B + c] = right ? ir : il;
-        hf[pos * B + c] = right ? fr : fl;
-        pos = right ? r : l;
-    }
-    hd[pos * B + c] = d;
-    hi[pos * B + c] = x;
-    hf[pos * B + c] = 1;
-    return top;
-}
-
 // A tree offers each partner id to a row at most once (leaf members are distinct), so an offered id can
 // already be in the row's heap only if it was there when the tree's pass began and has not been evicted
 // since.  REG (k <= 16): those ids are kept in registers (an evicted id is cleared), which replaces the
