@@ -105,7 +105,19 @@ __global__ __launch_bounds__(256) void k_check_to_bf16(const float* __restrict__
         bad |= __shfl_xor(bad, o);
         mx = fmax(mx, __shfl_xor(mx, o));
     }
-    if (lane == 0) {  // one pair of atomics per wave
+    // one pair of atomics per workgroup: atomics on one address serialise in the L2
+    __shared__ double wmx[4];
+    __shared__ int wbad[4];
+    if (lane == 0) {
+        wmx[threadIdx.x >> 6] = mx;
+        wbad[threadIdx.x >> 6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+            mx = fmax(mx, wmx[i]);
+            bad |= wbad[i];
+        }
         if (bad) atomicOr(flag, bad);
         atomicMax(maxnorm, (unsigned long long)__double_as_longlong(mx));
     }
@@ -554,7 +566,7 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
     R.Xb.ensure((size_t)std::max<int64_t>(n, 1) * R.ldb);
     R.nrm.ensure(std::max<int64_t>(n, 1));
     // the exactness test and the uncentred rows in one pass (kept when the data are exact)
-    const unsigned cg = (unsigned)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 16), 1), 8192);
+    const unsigned cg = (unsigned)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 16), 1), 1024);
     if (((uintptr_t)X & 15) == 0 && ld % 4 == 0)
         hipLaunchKernelGGL(k_check_to_bf16<true>, dim3(cg), dim3(256), 0, s, X, ld, n, dim, R.Xb.p, R.ldb, R.nrm.p,
                            flag, mx);
