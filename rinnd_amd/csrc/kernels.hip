@@ -1480,9 +1480,22 @@ __global__ __launch_bounds__(128) void k_cand_push_rev(const int32_t* __restrict
         st_rows += __shfl_xor(st_rows, o);
         st_pairs += __shfl_xor(st_pairs, o);
     }
-    if ((threadIdx.x & 63) == 0 && st_rows) {
-        atomicAdd(&counts[0], st_rows);
-        atomicAdd(&counts[1], st_pairs);
+    // one pair of atomics per workgroup: atomics on one address serialise in the L2
+    __shared__ unsigned long long wst[2][16];
+    if ((threadIdx.x & 63) == 0) {
+        wst[0][threadIdx.x >> 6] = st_rows;
+        wst[1][threadIdx.x >> 6] = st_pairs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+            st_rows += wst[0][i];
+            st_pairs += wst[1][i];
+        }
+        if (st_rows) {
+            atomicAdd(&counts[0], st_rows);
+            atomicAdd(&counts[1], st_pairs);
+        }
     }
 }
 
@@ -1882,7 +1895,14 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
         wave_heap_rows<false>(v, k, dist, idx, flags, hd, hi, hf, S, c & ~63);
     }
     for (int o = 32; o > 0; o >>= 1) ch += __shfl_xor(ch, o);
-    if ((threadIdx.x & 63) == 0 && ch) atomicAdd(changes, ch);
+    // one atomic per workgroup: atomics on one address serialise in the L2
+    __shared__ unsigned long long wch[16];
+    if ((threadIdx.x & 63) == 0) wch[threadIdx.x >> 6] = ch;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) ch += wch[i];
+        if (ch) atomicAdd(changes, ch);
+    }
 }
 
 size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
