@@ -261,6 +261,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     const bool reorder = p.rng_mode == RINND_RNG_COUNTER && !F.trees.empty() && n > 1;
     Workspace& W = *ctx->ws;
     WBuf<int32_t> perm0, inv0, r_idx;
+    WBuf<int32_t*> tree_ix;  // device table of the trees' index arrays (relabelling)
     WBuf<float> Xr, r_dist;
     WBuf<uint8_t> r_flags;
     const float* DXw = DX;
@@ -276,7 +277,13 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         r_flags = WBuf<uint8_t>(W, 5, (size_t)n * k);
         HIP_TRY(hipMemcpyAsync(perm0.p, F.trees[0].indices.p, n * 4, hipMemcpyDeviceToDevice, s));
         launch_relabel_points(s, DX, Dld, n, perm0.p, inv0.p, Xr.p);
-        for (auto& T : F.trees) launch_map_ids(s, T.indices.p, n, inv0.p);
+        // a device table of the trees' index arrays: one launch maps them all (here and at the end)
+        std::vector<int32_t*> tix;
+        for (auto& T : F.trees) tix.push_back(T.indices.p);
+        tree_ix = WBuf<int32_t*>(W, 42, std::max<size_t>(tix.size(), 1));
+        if (!tix.empty())
+            HIP_TRY(hipMemcpyAsync(tree_ix.p, tix.data(), tix.size() * sizeof(int32_t*), hipMemcpyHostToDevice, s));
+        launch_map_ids_trees(s, tree_ix.p, (int)tix.size(), n, inv0.p);
         DXw = Xr.p;
         w_idx = r_idx.p;
         w_dist = r_dist.p;
@@ -584,7 +591,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     }
     if (reorder) {
         launch_unpermute_heap(s, w_idx, w_dist, w_flags, perm0.p, n, k, h_idx, h_dist, h_flags);
-        for (auto& T : F.trees) launch_map_ids(s, T.indices.p, n, perm0.p);
+        launch_map_ids_trees(s, tree_ix.p, (int)F.trees.size(), n, perm0.p);
         HIP_TRY(hipStreamSynchronize(s));  // the relabelled buffers die here
     }
     tlog.flush();

@@ -399,7 +399,7 @@ void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const 
 // heap row r -> caller row perm[r] with ids mapped through perm
 void launch_relabel_points(hipStream_t s, const float* X, int64_t ld, int64_t n, const int32_t* perm, int32_t* inv,
                            float* Xr);
-void launch_map_ids(hipStream_t s, int32_t* ids, int64_t count, const int32_t* map);
+void launch_map_ids_trees(hipStream_t s, int32_t* const* ids_dev, int n_trees, int64_t count, const int32_t* map);
 void launch_unpermute_heap(hipStream_t s, const int32_t* idx, const float* dist, const uint8_t* flags,
                            const int32_t* perm, int64_t n, int k, int32_t* oidx, float* odist, uint8_t* oflags);
 

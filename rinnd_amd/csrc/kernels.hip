@@ -1725,14 +1725,18 @@ void launch_relabel_points(hipStream_t s, const float* X, int64_t ld, int64_t n,
                            float* Xr) {
     hipLaunchKernelGGL(k_relabel_rows, dim3(grid_for(n * (ld >> 2), 256)), dim3(256), 0, s, X, ld, n, perm, inv, Xr);
 }
-__global__ void k_map_ids(int32_t* ids, int64_t count, const int32_t* __restrict__ map) {
+// every tree's indices through the same map in one launch (blockIdx.y: the tree)
+__global__ void k_map_ids_trees(int32_t* const* ids, int64_t count, const int32_t* __restrict__ map) {
+    int32_t* p = ids[blockIdx.y];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t v = ids[i];
-        if (v >= 0) ids[i] = map[v];
+        const int32_t v = p[i];
+        if (v >= 0) p[i] = map[v];
     }
 }
-void launch_map_ids(hipStream_t s, int32_t* ids, int64_t count, const int32_t* map) {
-    if (count > 0) hipLaunchKernelGGL(k_map_ids, dim3(grid_for(count, 256)), dim3(256), 0, s, ids, count, map);
+void launch_map_ids_trees(hipStream_t s, int32_t* const* ids_dev, int n_trees, int64_t count, const int32_t* map) {
+    if (count > 0 && n_trees > 0)
+        hipLaunchKernelGGL(k_map_ids_trees, dim3(grid_for(count, 256, 2048), n_trees), dim3(256), 0, s, ids_dev, count,
+                           map);
 }
 // row r of the relabelled heap is row perm[r] of the caller's, and its ids map through perm
 __global__ void k_unpermute_heap(const int32_t* __restrict__ idx, const float* __restrict__ dist,
