@@ -82,6 +82,8 @@ def lib():
             "orc_index_n_trees": (_i64, [_p]),
             "orc_index_forest": (_p, [_p]),
             "orc_index_trace": (None, [_p, _p, _p, _p]),
+            "orc_diversify": (_i64, [_p, _p, _p, _i64, C.c_int, _p, _i64, C.c_int, _i64, _p, _p]),
+            "orc_greedy": (_i64, [_p, _p, _i64, _i64, C.c_int, _p, _p, _p, _i64, _i64, _f, _f, _p, _p, _p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -314,6 +316,36 @@ def search_graph(sorted_idx, sorted_dist, data, metric="sqeuclidean", prob=1.0, 
                                  METRICS[metric], C.c_float(prob), C.c_float(mult), threads, ptr(indptr), ptr(indices),
                                  cap, ptr(md))
     return indptr, indices[:nnz].copy(), float(md[0])
+
+
+def diversify(row_idx, row_dist, data, max_degree, order=None, stop_at_neg=True, metric="sqeuclidean"):
+    """diversify_row (stop_at_neg, positions in row order) or diversify_sorted_csr (positions in `order`,
+    -1 entries skipped) on one row; returns the retained (ids, distances)."""
+    ri = np.ascontiguousarray(row_idx, np.int32)
+    rd = np.ascontiguousarray(row_dist, np.float32)
+    data = np.ascontiguousarray(data, np.float32)
+    order = np.arange(len(ri), dtype=np.int64) if order is None else np.ascontiguousarray(order, np.int64)
+    oi = np.zeros(max(len(ri), 1), np.int32)
+    od = np.zeros(max(len(ri), 1), np.float32)
+    m = int(lib().orc_diversify(ptr(ri), ptr(rd), ptr(order), len(order), int(stop_at_neg), ptr(data), data.shape[1],
+                                METRICS[metric], max_degree, ptr(oi), ptr(od)))
+    return oi[:m].copy(), od[:m].copy()
+
+
+def greedy(q, data, indptr, indices, seeds, k, epsilon, min_distance, st, metric="sqeuclidean"):
+    """Greedy search from explicit seed ids over a CSR graph; `st` (FastRng state) is advanced in place.
+    Returns (ids, distances) padded to k with (-1, +inf)."""
+    q = np.ascontiguousarray(q, np.float32)
+    data = np.ascontiguousarray(data, np.float32)
+    indptr = np.ascontiguousarray(indptr, np.int32)
+    indices = np.ascontiguousarray(indices if len(indices) else [0], np.int32)
+    seeds = np.ascontiguousarray(seeds if len(seeds) else [-1], np.int32)
+    oi = np.zeros(k, np.int32)
+    od = np.zeros(k, np.float32)
+    lib().orc_greedy(ptr(q), ptr(data), data.shape[0], data.shape[1], METRICS[metric], ptr(indptr), ptr(indices),
+                     ptr(seeds), len(seeds), k, C.c_float(epsilon), C.c_float(min_distance), ptr(st), ptr(oi),
+                     ptr(od))
+    return oi, od
 
 
 def weighted_union(forward, reverse, max_degree):

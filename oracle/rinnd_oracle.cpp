@@ -1067,39 +1067,35 @@ Index build_index(const float* data, size_t n, size_t dim, int metric, int corr,
     return ix;
 }
 
-// process_query_with_workspace (core/index.rs:157-202) around search (greedy.rs:203-241).
-void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* out_idx,
-                      float* out_dist) const {
-    FastRng rng(rng_seed + (uint64_t)qid);
-    std::vector<uint8_t> visited(n_points, 0);
+// The body of search (greedy.rs:203-241) after the tree descent: the seed ids are the leaf points in
+// the order initialize_fp32 (greedy.rs:70-99) visits them; then fill_random (greedy.rs:148-163) and
+// traverse (greedy.rs:165-201); the result in into_sorted order (candidate_heap.rs:140-148).
+void greedy_from_seeds(const float* q, const float* data, size_t dim, int metric, size_t n_vertices,
+                       const int32_t* indptr, const int32_t* indices, const std::vector<int32_t>& seed_ids,
+                       size_t k, float eps, float min_distance, FastRng& rng, std::vector<int32_t>& ri,
+                       std::vector<float>& rd) {
+    std::vector<uint8_t> visited(n_vertices, 0);
     BoundedHeap result(k);
     using E = std::pair<float, int32_t>;
     auto gt = [](const E& a, const E& b) { return ord_cmp(a, b) > 0; };
     std::priority_queue<E, std::vector<E>, decltype(gt)> seeds(gt);  // min-heap on (d, idx)
-    auto dist_to = [&](int32_t i) { return distance(metric, q, data.data() + (size_t)i * dim, dim); };
+    auto dist_to = [&](int32_t i) { return distance(metric, q, data + (size_t)i * dim, dim); };
     auto check_and_mark = [&](int32_t i) {
         bool was = visited[i] != 0;
         visited[i] = 1;
         return was;
     };
     // initialize_fp32 (greedy.rs:70-99)
-    std::vector<std::pair<size_t, size_t>> leaves;
-    for (const auto& t : search_trees) {
-        t.search_leaves(q, rng, search_tree_leaf_budget, leaves);
-        for (auto [s, e] : leaves)
-            for (size_t z = s; z < e; ++z) {
-                const int32_t idx = t.indices[z];
-                if (idx >= 0 && !check_and_mark(idx)) {
-                    float d = dist_to(idx);
-                    result.push(d, idx);
-                    seeds.push({d, idx});
-                }
-            }
-    }
+    for (int32_t idx : seed_ids)
+        if (idx >= 0 && !check_and_mark(idx)) {
+            float d = dist_to(idx);
+            result.push(d, idx);
+            seeds.push({d, idx});
+        }
     // fill_random (greedy.rs:148-163)
     const size_t nfill = k > seeds.size() ? k - seeds.size() : 0;
     for (size_t r = 0; r < nfill; ++r) {
-        int32_t idx = (int32_t)rng.next_index(search_graph.n_vertices);
+        int32_t idx = (int32_t)rng.next_index(n_vertices);
         if (!check_and_mark(idx)) {
             float d = dist_to(idx);
             result.push(d, idx);
@@ -1114,8 +1110,8 @@ void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* 
         seeds.pop();
         if (result.full() && top.first >= bound) break;
         const int32_t v = top.second;
-        for (int32_t z = search_graph.indptr[v]; z < search_graph.indptr[v + 1]; ++z) {
-            const int32_t nb = search_graph.indices[z];
+        for (int32_t z = indptr[v]; z < indptr[v + 1]; ++z) {
+            const int32_t nb = indices[z];
             if (nb < 0 || check_and_mark(nb)) continue;
             float d = dist_to(nb);
             if (d < bound) {
@@ -1128,9 +1124,25 @@ void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* 
             }
         }
     }
+    result.into_sorted(ri, rd);
+}
+
+// process_query_with_workspace (core/index.rs:157-202) around search (greedy.rs:203-241).
+void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* out_idx,
+                      float* out_dist) const {
+    FastRng rng(rng_seed + (uint64_t)qid);
+    // initialize_fp32 (greedy.rs:70-99): every tree's leaves, in order; the descent draws from rng first
+    std::vector<int32_t> seed_ids;
+    std::vector<std::pair<size_t, size_t>> leaves;
+    for (const auto& t : search_trees) {
+        t.search_leaves(q, rng, search_tree_leaf_budget, leaves);
+        for (auto [s, e] : leaves)
+            for (size_t z = s; z < e; ++z) seed_ids.push_back(t.indices[z]);
+    }
     std::vector<int32_t> ri;
     std::vector<float> rd;
-    result.into_sorted(ri, rd);
+    greedy_from_seeds(q, data.data(), dim, metric, search_graph.n_vertices, search_graph.indptr.data(),
+                      search_graph.indices.data(), seed_ids, k, eps, min_distance, rng, ri, rd);
     for (size_t j = 0; j < k; ++j) {
         if (j < ri.size()) {
             out_idx[j] = ri[j] >= 0 ? (int32_t)vertex_order[ri[j]] : ri[j];
@@ -1507,6 +1519,39 @@ void orc_index_trace(void* h, int64_t* changes, int64_t* n_iters, double* ms4) {
     ms4[1] = tr.ms_init;
     ms4[2] = tr.ms_candidates;
     ms4[3] = tr.ms_updates;
+}
+
+// diversify_row (stop_at_neg = 1) / diversify_sorted_csr (stop_at_neg = 0) on explicit inputs; prob >= 1
+// (the reference's default, no random pruning).  Returns the retained count.
+int64_t orc_diversify(const int32_t* ri, const float* rd, const int64_t* order, int64_t n_order, int stop_at_neg,
+                      const float* data, int64_t dim, int metric, int64_t max_degree, int32_t* out_idx,
+                      float* out_dist) {
+    std::vector<size_t> ord(order, order + n_order);
+    FastRng prng(0);
+    auto kept = diversify(ri, rd, ord, stop_at_neg != 0, data, (size_t)dim, metric, 1.0f, (size_t)max_degree, prng);
+    for (size_t i = 0; i < kept.size(); ++i) {
+        out_idx[i] = kept[i].first;
+        out_dist[i] = kept[i].second;
+    }
+    return (int64_t)kept.size();
+}
+
+// The greedy search from explicit seed ids over a caller-given CSR graph (greedy_from_seeds); st is the
+// query's FastRng state, advanced by fill_random.  Writes k results padded with (-1, +inf), returns the count.
+int64_t orc_greedy(const float* q, const float* data, int64_t n, int64_t dim, int metric, const int32_t* indptr,
+                   const int32_t* indices, const int32_t* seed_ids, int64_t n_seeds, int64_t k, float eps,
+                   float min_distance, uint64_t st[4], int32_t* out_idx, float* out_dist) {
+    FastRng rng = rng_from(st);
+    std::vector<int32_t> seeds(seed_ids, seed_ids + n_seeds), ri;
+    std::vector<float> rd;
+    greedy_from_seeds(q, data, (size_t)dim, metric, (size_t)n, indptr, indices, seeds, (size_t)k, eps,
+                      min_distance, rng, ri, rd);
+    std::memcpy(st, rng.s, 32);
+    for (int64_t j = 0; j < k; ++j) {
+        out_idx[j] = j < (int64_t)ri.size() ? ri[j] : -1;
+        out_dist[j] = j < (int64_t)ri.size() ? rd[j] : std::numeric_limits<float>::infinity();
+    }
+    return (int64_t)ri.size();
 }
 
 }  // extern "C"

@@ -145,6 +145,11 @@ struct Index {
 Index build_index(const float* data, size_t n, size_t dim, int metric, int corr,
                   const IndexParams& p, int threads);
 
+void greedy_from_seeds(const float* q, const float* data, size_t dim, int metric, size_t n_vertices,
+                       const int32_t* indptr, const int32_t* indices, const std::vector<int32_t>& seed_ids,
+                       size_t k, float eps, float min_distance, FastRng& rng, std::vector<int32_t>& ri,
+                       std::vector<float>& rd);
+
 // std::collections::BinaryHeap<(OrderedFloat<f32>, i32)> as used by BoundedHeap
 // (core/heap/candidate_heap.rs:78-163); the layout decides tie order in into_sorted.
 struct BoundedHeap {

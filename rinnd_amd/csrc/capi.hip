@@ -262,6 +262,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     Workspace& W = *ctx->ws;
     WBuf<int32_t> perm0, inv0, r_idx;
     WBuf<int32_t*> tree_ix;  // device table of the trees' index arrays (relabelling)
+    std::vector<int32_t*> tix;  // its host copy: lives until the stream synchronises at the end of the build
     WBuf<float> Xr, r_dist;
     WBuf<uint8_t> r_flags;
     const float* DXw = DX;
@@ -278,7 +279,6 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         HIP_TRY(hipMemcpyAsync(perm0.p, F.trees[0].indices.p, n * 4, hipMemcpyDeviceToDevice, s));
         launch_relabel_points(s, DX, Dld, n, perm0.p, inv0.p, Xr.p);
         // a device table of the trees' index arrays: one launch maps them all (here and at the end)
-        std::vector<int32_t*> tix;
         for (auto& T : F.trees) tix.push_back(T.indices.p);
         tree_ix = WBuf<int32_t*>(W, 42, std::max<size_t>(tix.size(), 1));
         if (!tix.empty())

@@ -1178,10 +1178,13 @@ static void sort_and_bin(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec
                        big_list);
 }
 
-static int lanes_block(int slots, int bytes_per_slot, int cap) {
+// Lanes per workgroup (a multiple of 64, at most cap) for kernels that keep one heap column of `slots`
+// entries per lane in LDS with a column stride of B + pad; 0 if 64 lanes do not fit in 64 KiB.
+static int lanes_block(int slots, int bytes_per_slot, int cap, int pad = 0) {
+    auto bytes = [&](int B) { return (int64_t)(B + pad) * slots * bytes_per_slot; };
     int B = cap;
-    while (B > 64 && (int64_t)B * slots * bytes_per_slot > 64 * 1024) B -= 64;
-    return (int64_t)B * slots * bytes_per_slot <= 64 * 1024 ? B : 0;
+    while (B > 64 && bytes(B) > 64 * 1024) B -= 64;
+    return bytes(B) <= 64 * 1024 ? B : 0;
 }
 
 void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* seg2,
@@ -1594,7 +1597,7 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
     hipLaunchKernelGGL(k_bin_scatter2, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins,
                        a.order);
-    const int B = lanes_block(a.C, 16, 128);
+    const int B = lanes_block(a.C, 16, 128, 1);  // C <= 60: 64 lanes always fit
     if (a.k <= 16 && a.n < ((int64_t)1 << 26))
         hipLaunchKernelGGL(k_cand_push_rev<true>, dim3(grid_for(a.n, B, 8192)), dim3(B), (size_t)(B + 1) * a.C * 16, s,
                            a.order, a.bins, a.off, a.rval2, a.idx, a.flags, a.fpr, a.k, a.mode, a.block_rows, a.new_idx,
@@ -1909,6 +1912,57 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
     }
 }
 
+// The same ordered apply for heaps too large for a lane's LDS column (64 columns of k entries do not fit):
+// one wave per destination, its heap row contiguous in LDS.  The wave tests each record's id against
+// the whole row at once (a ballot) and lane 0 sifts, so the pushes stay in the records' order.
+constexpr int kWaveApplyWaves = 4;
+__global__ __launch_bounds__(kWaveApplyWaves * RINND_WAVE) void k_heap_push_rec_wave(
+    const int32_t* __restrict__ order, const uint32_t* bins, const uint64_t* __restrict__ off,
+    const uint64_t* __restrict__ rv, int32_t* idx, float* dist, uint8_t* flags, int k, unsigned long long* changes) {
+    extern __shared__ __align__(16) float wsm[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* hd = wsm + (size_t)w * k;
+    int32_t* hi = (int32_t*)(wsm + (size_t)kWaveApplyWaves * k) + (size_t)w * k;
+    uint8_t* hf = (uint8_t*)(wsm + (size_t)2 * kWaveApplyWaves * k) + (size_t)w * k;
+    const int64_t total = (int64_t)bins[kBins + kBins - 1] + bins[kBins - 1];
+    unsigned long long ch = 0;
+    for (int64_t e = (int64_t)blockIdx.x * kWaveApplyWaves + w; e < total; e += (int64_t)gridDim.x * kWaveApplyWaves) {
+        const int32_t v = order[e];
+        const int64_t row = (int64_t)v * k;
+        for (int j = lane; j < k; j += RINND_WAVE) {
+            hd[j] = dist[row + j];
+            hi[j] = idx[row + j];
+            hf[j] = flags[row + j];
+        }
+        wave_sync();
+        const uint64_t o0 = off[v];
+        const int64_t L = (int64_t)(off[v + 1] - o0);
+        for (int64_t r = 0; r < L; ++r) {
+            const uint64_t q = rv[o0 + r];
+            const float d = __builtin_bit_cast(float, (uint32_t)(q >> 32));
+            const int32_t x = (int32_t)(uint32_t)q;
+            if (d >= hd[0]) continue;  // uniform: every lane reads the same root
+            bool dup = false;
+            for (int j = lane; j < k; j += RINND_WAVE) dup |= hi[j] == x;
+            if (__ballot(dup)) continue;
+            if (lane == 0) {
+                int32_t ev;
+                (void)heap_push_col_new(hd, hi, hf, 1, 0, k, x, d, ev);
+                ++ch;
+            }
+            wave_sync();
+        }
+        wave_sync();
+        for (int j = lane; j < k; j += RINND_WAVE) {
+            dist[row + j] = hd[j];
+            idx[row + j] = hi[j];
+            flags[row + j] = hf[j];
+        }
+        wave_sync();
+    }
+    if (lane == 0 && ch) atomicAdd(changes, ch);
+}
+
 size_t join_sort_tmp_bytes(int64_t items, int end_bit) {
     size_t b = 0;
     HIP_TRY(sort_pairs_u32_u64(nullptr, b, nullptr, nullptr, nullptr, nullptr, items, end_bit, nullptr));
@@ -1938,7 +1992,14 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.n, a.bins);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
-    const int B = lanes_block(a.k, 9, 256);
+    const int B = lanes_block(a.k, 9, 256, 1);
+    if (B == 0) {  // large k: a wave per destination
+        const size_t lds = (size_t)kWaveApplyWaves * round_up((int64_t)a.k * 9, 16);
+        hipLaunchKernelGGL(k_heap_push_rec_wave, dim3(grid_for(a.n, kWaveApplyWaves, 8192)),
+                           dim3(kWaveApplyWaves * RINND_WAVE), lds, s, a.order, a.bins, a.off, a.rval2, a.idx, a.dist,
+                           a.flags, a.k, a.changes);
+        return;
+    }
     const size_t lds = (size_t)round_up((int64_t)(B + 1) * a.k * 2 + ((B + 1) * a.k + 3) / 4, 4) * 4;
     if (a.k <= 16)
         hipLaunchKernelGGL(k_heap_push_rec<true>, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins,

@@ -50,11 +50,24 @@ def test_index_matches_reference(threads):
     vo = r.vertex_order
     inv = np.empty_like(vo)
     inv[vo] = np.arange(len(vo))
-    for o in range(0, len(X), 97):
-        row_g = indices[indptr[o]:indptr[o + 1]]
+    # search_graph_original_order (core/index.rs:137-141): every row, in the reference's entry order
+    want_ptr = np.zeros(len(X) + 1, np.int64)
+    want = []
+    for o in range(len(X)):
         ir = inv[o]
-        row_r = vo[rc[rp[ir]:rp[ir + 1]]]
-        np.testing.assert_array_equal(np.sort(row_g), np.sort(row_r))
+        row = vo[rc[rp[ir]:rp[ir + 1]]]
+        want.append(row)
+        want_ptr[o + 1] = want_ptr[o] + len(row)
+    np.testing.assert_array_equal(indptr, want_ptr)
+    np.testing.assert_array_equal(indices, np.concatenate(want))
+    # export_search_tree (core/index.rs:143-154): the first retained tree with leaf ids in input order
+    hp, off, ch, leaf_ids = g.export_search_tree()
+    t = r.forest().tree(0)
+    np.testing.assert_array_equal(hp.view(np.uint32), r.forest().hyperplanes(0, X.shape[1]).view(np.uint32))
+    np.testing.assert_array_equal(off.view(np.uint32), t["offsets"].view(np.uint32))
+    np.testing.assert_array_equal(ch, t["children"])
+    np.testing.assert_array_equal(leaf_ids, vo[t["indices"]])
+    assert sorted(leaf_ids.tolist()) == list(range(len(X)))
     Q = uniform(200, 32, seed=12)
     qi, qd = g.query(Q, k=10, epsilon=0.1)
     oi, od = r.query(Q, k=10, epsilon=0.1)
