@@ -244,6 +244,19 @@ rinnd_status rinnd_index_query_quantized(rinnd_index* index, const float* querie
                                          float epsilon, int64_t candidate_width, int64_t rerank_width,
                                          int32_t* out_idx, float* out_dist);
 
+/* The query of rinnd_index_query without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  queries (nq x dim), out_idx and out_dist (nq x k) must be device pointers; results are
+ * identical to rinnd_index_query's.  The search state lives in one fixed 128 MiB device array per GPU
+ * (allocated when the library's code is loaded on that GPU, held until the process ends): per concurrent
+ * query n + 8*min(n, 65536) + 4 bytes, plus nq * round_up(dim, 8) * 4 bytes for padded query rows when dim is
+ * not a multiple of 8 or the rows are not 256-byte aligned.  A call whose smallest layout does not fit returns
+ * RINND_ERR_UNSUPPORTED (split the batch); a query whose frontier exceeds 65,536 entries returns
+ * RINND_ERR_UNSUPPORTED.  All calls on one GPU share the array: the library serialises them (a per-device lock
+ * held until the call's work has finished); kernels of overlapping calls would race on it. */
+rinnd_status rinnd_index_query_noalloc(rinnd_index* index, const float* queries, int64_t nq, int64_t k,
+                                       float epsilon, int32_t* out_idx, float* out_dist);
+
 /* `neighbor_graph` getter (py:1087-1103): sorted, corrected n x k arrays in input order. */
 rinnd_status rinnd_index_neighbor_graph(const rinnd_index* index, int32_t* out_idx, float* out_dist);
 /* Search graph CSR sizes and `search_graph_original_order` (core/index.rs:137-141, py:1013-1025). */

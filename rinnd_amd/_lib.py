@@ -31,7 +31,7 @@ EXPORTS = [
     "rinnd_rng_jump", "rinnd_pair_distances", "rinnd_nn_descent", "rinnd_build_forest",
     "rinnd_forest_n_trees", "rinnd_forest_tree_info", "rinnd_forest_tree_copy", "rinnd_forest_free",
     "rinnd_sort_rows", "rinnd_search_graph_build", "rinnd_index_build", "rinnd_index_free",
-    "rinnd_index_query", "rinnd_index_query_one", "rinnd_index_query_quantized", "rinnd_index_neighbor_graph",
+    "rinnd_index_query", "rinnd_index_query_noalloc", "rinnd_index_query_one", "rinnd_index_query_quantized", "rinnd_index_neighbor_graph",
     "rinnd_index_search_graph_info", "rinnd_index_search_graph", "rinnd_index_vertex_order",
     "rinnd_index_trees",
 ]
@@ -128,6 +128,7 @@ def lib():
                                                 C.POINTER(_p), C.POINTER(BuildStats)]),
                 "rinnd_index_free": (None, [_p]),
                 "rinnd_index_query": (C.c_int, [_p, _p, _i64, _i64, C.c_float, _p, _p]),
+                "rinnd_index_query_noalloc": (C.c_int, [_p, _p, _i64, _i64, C.c_float, _p, _p]),
                 "rinnd_index_query_one": (C.c_int, [_p, _p, _i64, C.c_float, _p, _p]),
                 "rinnd_index_query_quantized": (C.c_int, [_p, _p, _i64, _i64, C.c_float, _i64, _i64, _p, _p]),
                 "rinnd_index_neighbor_graph": (C.c_int, [_p, _p, _p]),

@@ -609,6 +609,9 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     if (stats) *stats = st;
 }
 
+// index.hip: looks up (once per device, with it current) the device address of the no-allocation query's array
+hipError_t noalloc_scratch_init(int device);
+
 void make_padded(hipStream_t s, const float* data, int64_t n, int dim, DBuf<float>& X, int64_t& ld) {
     DataDev D;
     make_data(s, data, n, dim, D);
@@ -685,6 +688,10 @@ rinnd_status rinnd_ctx_create(int device, rinnd_ctx** out) {
     }
     if (device < 0 || device >= count) return RINND_ERR_INVALID_ARG;
     if (hipSetDevice(device) != hipSuccess) return RINND_ERR_HIP;
+    if (rinnd::noalloc_scratch_init(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return RINND_ERR_HIP;
+    }
     auto* c = new rinnd_ctx;
     c->ws = new rinnd::Workspace();
     c->device = device;

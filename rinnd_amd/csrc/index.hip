@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "graph.h"
@@ -208,6 +209,45 @@ static void ensure_fp8_codes(rinnd_index& ix) {
 
 // search_k: candidate width of the search (k for a plain query); rerank > 0: exact re-ranking of that many
 // candidates; fp8: candidate distances from the FP8 codes.  k results are written per query.
+// The index's part of a launch of the greedy search: data, graph, trees and the query parameters.
+static SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq, int64_t k, float eps,
+                                    int64_t qid_base, int64_t search_k, int64_t rerank, bool fp8, int32_t* oip,
+                                    float* odp) {
+    SearchArgs a{};
+    a.metric = ix.metric;
+    a.X = ix.X.p;
+    a.ld = ix.ld;
+    a.dim = (int)ix.dim;
+    a.n = ix.n;
+    a.indptr = ix.indptr.p;
+    a.indices = ix.indices.p;
+    a.n_trees = (int)ix.trees.trees.size();
+    a.hyper = ix.d_hyper.p;
+    a.hdim = ix.ld;
+    a.offsets = ix.d_off.p;
+    a.children = ix.d_ch.p;
+    a.tindices = ix.d_tix.p;
+    a.leaf_budget = (int)ix.leaf_budget;
+    a.Q = qd;
+    a.ldq = ix.ld;
+    a.nq = nq;
+    a.qid_base = qid_base;
+    a.k = (int)search_k;
+    a.eps = eps;
+    a.min_distance = ix.min_distance;
+    a.seed = ix.seed;
+    a.vertex_order = ix.vertex_order.p;
+    a.corr = ix.corr;
+    a.out_idx = oip;
+    a.out_dist = odp;
+    a.q8 = fp8 ? ix.q8.p : nullptr;
+    a.ld8 = ix.ld8;
+    a.q8s = fp8 ? ix.q8s.p : nullptr;
+    a.k_out = (int)k;
+    a.rerank = (int)rerank;
+    return a;
+}
+
 static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_t k, float eps, int64_t qid_base,
                        int32_t* out_idx, float* out_dist, int64_t search_k = -1, int64_t rerank = 0,
                        bool fp8 = false) {
@@ -237,44 +277,13 @@ static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_
         oip = ix.obuf_i.p;
         odp = ix.obuf_d.p;
     }
-    SearchArgs a{};
-    a.metric = ix.metric;
-    a.X = ix.X.p;
-    a.ld = ix.ld;
-    a.dim = (int)ix.dim;
-    a.n = ix.n;
-    a.indptr = ix.indptr.p;
-    a.indices = ix.indices.p;
-    a.n_trees = (int)ix.trees.trees.size();
-    a.hyper = ix.d_hyper.p;
-    a.hdim = ix.ld;
-    a.offsets = ix.d_off.p;
-    a.children = ix.d_ch.p;
-    a.tindices = ix.d_tix.p;
-    a.leaf_budget = (int)ix.leaf_budget;
-    a.Q = qd;
-    a.ldq = ix.ld;
-    a.nq = nq;
-    a.qid_base = qid_base;
-    a.k = (int)search_k;
-    a.eps = eps;
-    a.min_distance = ix.min_distance;
-    a.seed = ix.seed;
-    a.vertex_order = ix.vertex_order.p;
-    a.corr = ix.corr;
-    a.out_idx = oip;
-    a.out_dist = odp;
+    SearchArgs a = index_search_args(ix, qd, nq, k, eps, qid_base, search_k, rerank, fp8, oip, odp);
     a.visited = ix.visited.p;
     a.epochs = ix.epochs.p;
     a.seed_spill_d = ix.spill_d.p;
     a.seed_spill_i = ix.spill_i.p;
     a.seed_cap = ix.seed_cap;
     a.status = ix.status.p;
-    a.q8 = fp8 ? ix.q8.p : nullptr;
-    a.ld8 = ix.ld8;
-    a.q8s = fp8 ? ix.q8s.p : nullptr;
-    a.k_out = (int)k;
-    a.rerank = (int)rerank;
     launch_search(s, a, slots);
     HIP_TRY(hipGetLastError());
     if (!dev_out) {
@@ -288,6 +297,115 @@ static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_
         HIP_TRY(hipMemsetAsync(ix.status.p, 0, 4, s));
         throw Error{RINND_ERR_INTERNAL, "search workspace capacity exceeded"};
     }
+}
+
+// ---- the query that allocates nothing ---------------------------------------------------------------
+// A fixed device array holds the search's per-slot state (status word, visited epochs, visited bytes, seed-heap
+// spill) and, when the query rows need padding, the padded rows.  Its device address is looked up once per
+// device (noalloc_scratch_init, called when a context is created) and passed to the kernels; the array's name is
+// never used as a device address in host code.  Every call on a device shares that device's array: calls are
+// serialised by a per-device lock held until the call's stream work has finished (kernels of overlapping calls
+// would race on it), and each call clears what must start at zero.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;  // 128 MiB per GPU, held until the process ends
+constexpr int64_t kNoAllocSeedCap = 1 << 16;               // seed-heap entries per query slot
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_init_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu[kMaxDevices];
+
+hipError_t noalloc_scratch_init(int device) {  // with `device` current
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+__global__ void k_zero_words(uint32_t* p, int64_t words) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+static void zero_bytes(hipStream_t s, void* p, size_t bytes) {  // p and bytes multiples of 4; a kernel, no memset
+    const int64_t words = (int64_t)(bytes / 4);
+    if (words == 0) return;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(ceil_div(words, (int64_t)256), 8192)),
+                       dim3(256), 0, s, (uint32_t*)p, words);
+}
+
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static void query_noalloc_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_t k, float eps,
+                               int32_t* out_idx, float* out_dist) {
+    RINND_CHECK_ARG(nq >= 0, "negative query count");
+    RINND_CHECK_ARG(k >= 0 && k <= search_max_k(), "k must be in [0, 512]");
+    if (nq == 0 || k == 0) return;
+    RINND_CHECK_ARG(device_pointer(queries) && device_pointer(out_idx) && device_pointer(out_dist),
+                    "the no-allocation query takes device pointers for the queries and both outputs");
+    const int dev = ix.ctx->device;
+    RINND_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device id out of range");
+    if (!g_scratch_ptr[dev]) HIP_TRY(noalloc_scratch_init(dev));
+    char* base = (char*)g_scratch_ptr[dev];
+    hipStream_t s = ix.ctx->stream;
+    // layout: status | epochs | padded query rows (if needed) | visited bytes | seed spill (d, i)
+    const bool stage = !(ix.dim == ix.ld && ((uintptr_t)queries & 255) == 0);
+    const size_t q_bytes = stage ? al256((size_t)nq * ix.ld * 4) : 0;
+    const int64_t seed_cap = std::max<int64_t>(1, std::min<int64_t>(ix.n, kNoAllocSeedCap));
+    const size_t per_slot = 4 + (size_t)ix.n + 8 * (size_t)seed_cap;
+    const size_t fixed = 256 + q_bytes + 4 * 256;  // the status block, the query rows, rounding of four regions
+    if (fixed + per_slot > kNoAllocScratchBytes) {
+        throw Error{RINND_ERR_UNSUPPORTED,
+                    "the no-allocation query needs " + std::to_string(fixed + per_slot) + " bytes for " +
+                        std::to_string(nq) + " queries over " + std::to_string(ix.n) +
+                        " points; its device array holds " + std::to_string(kNoAllocScratchBytes) +
+                        " bytes (split the batch, or use rinnd_index_query)"};
+    }
+    const int slots = (int)std::min<int64_t>(std::min<int64_t>(nq, 4096),
+                                              (int64_t)((kNoAllocScratchBytes - fixed) / per_slot));
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base + off;
+        off += al256(bytes);
+        return p;
+    };
+    int32_t* status = (int32_t*)take(256);
+    int32_t* epochs = (int32_t*)(base + 4);  // inside the status block when slots <= 63
+    if (slots > 63) epochs = (int32_t*)take((size_t)slots * 4);
+    float* qrows = stage ? (float*)take(q_bytes) : nullptr;
+    uint8_t* visited = (uint8_t*)take((size_t)slots * ix.n);
+    float* spd = (float*)take((size_t)slots * seed_cap * 4);
+    int32_t* spi = (int32_t*)take((size_t)slots * seed_cap * 4);
+    if (off > kNoAllocScratchBytes) throw Error{RINND_ERR_INTERNAL, "no-allocation scratch layout overflow"};
+    std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+    // what must start at zero: the status word, the epochs and the visited bytes (an earlier call, maybe on
+    // another index, left its own epochs there)
+    zero_bytes(s, status, 256);
+    if (slots > 63) zero_bytes(s, epochs, al256((size_t)slots * 4));
+    zero_bytes(s, visited, al256((size_t)slots * ix.n));
+    const float* qd = queries;
+    if (stage) {
+        launch_pad_rows(s, queries, nq, (int)ix.dim, qrows, ix.ld);  // zero padding columns, every call
+        qd = qrows;
+    }
+    SearchArgs a = index_search_args(ix, qd, nq, k, eps, 0, k, 0, false, out_idx, out_dist);
+    a.visited = visited;
+    a.epochs = epochs;
+    a.seed_spill_d = spd;
+    a.seed_spill_i = spi;
+    a.seed_cap = seed_cap;
+    a.status = status;
+    launch_search(s, a, slots);
+    HIP_TRY(hipGetLastError());
+    int32_t st = 0;
+    HIP_TRY(hipMemcpyAsync(&st, status, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (st == 1)
+        throw Error{RINND_ERR_UNSUPPORTED, "a query's frontier exceeded the no-allocation limit of " +
+                                               std::to_string(kNoAllocSeedCap) + " entries (use rinnd_index_query)"};
+    if (st != 0) throw Error{RINND_ERR_INTERNAL, "search workspace capacity exceeded"};
 }
 
 template <class F>
@@ -372,6 +490,12 @@ rinnd_status rinnd_index_query(rinnd_index* ix, const float* queries, int64_t nq
         RINND_CHECK_ARG(nq >= 0, "negative query count");
         query_impl(*ix, queries, nq, k, epsilon, 0, out_idx, out_dist);
     });
+}
+
+rinnd_status rinnd_index_query_noalloc(rinnd_index* ix, const float* queries, int64_t nq, int64_t k, float epsilon,
+                                       int32_t* out_idx, float* out_dist) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] { query_noalloc_impl(*ix, queries, nq, k, epsilon, out_idx, out_dist); });
 }
 
 rinnd_status rinnd_index_query_one(rinnd_index* ix, const float* query, int64_t k, float epsilon,
