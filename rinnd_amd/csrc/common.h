@@ -152,6 +152,15 @@ __device__ __forceinline__ float heap_push_col_new(float* hd, int32_t* hi, uint8
     return top;
 }
 
+// A workgroup barrier that orders LDS only: unlike __syncthreads(), whose workgroup fence also waits for every
+// outstanding global access, it leaves global loads in flight (prefetches issued before it keep going).  For
+// kernels whose waves communicate through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // operand and accumulator vectors of the 16x16x16 bf16 MFMA
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));

@@ -10,6 +10,17 @@
 
 namespace rinnd {
 
+// Compute units of the current device (the persistent kernels size their grids by it).
+inline int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return cus;
+}
+
 // ---- device memory -------------------------------------------------------------------------------
 template <class T>
 struct DBuf {

@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
 
 // Leaf pair distances on MFMA for exact-mode data (initialize_from_leaves, core/nndescent/mod.rs:203-300):
 // the leaf's bf16 rows are staged in LDS, one wave per 16 x 16 block (A <= O) of the leaf's Gram matrix
-// over 16x16x16 bf16 MFMAs, d = |a|^2 + |o|^2 - 2 G.  In exact mode this is the reference's FP32 value
+// over 16x16x32 bf16 MFMAs, d = |a|^2 + |o|^2 - 2 G.  In exact mode this is the reference's FP32 value
 // bit for bit (see the header), so the scratch D gets exactly what k_leaf_dist would write.
 __device__ __forceinline__ void tri_pair16(int q, int m, int& a, int& o) {
     const int r = m * (m - 1) / 2 - 1 - q;
@@ -310,6 +310,23 @@ __device__ __forceinline__ void tri_pair16(int q, int m, int& a, int& o) {
     while (k * (k + 1) / 2 > r) --k;
     a = m - 2 - k;
     o = m - 1 - (r - k * (k + 1) / 2);
+}
+
+// One 16 x 16 block of a Gram matrix on gfx950's 16x16x32 bf16 MFMA (K = ldb, a multiple of 16; an odd
+// 16-column tail takes one 16x16x16 step).  ra / ro point at this lane's A row (block row + lane % 16) and
+// O row in LDS; lane group g = lane / 16 holds k = 8 g .. 8 g + 7 of each 32-wide step for both operands, so
+// every product pairs the same k.  Result: acc[r] = G(16 A + 4 g + r, 16 O + lane % 16).
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ v4f gram_block16(const uint16_t* ra, const uint16_t* ro, int ldb, int g) {
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    int k0 = 0;
+    for (; k0 + 32 <= ldb; k0 += 32)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const v8bf*>(ra + k0 + 8 * g),
+                                                      *reinterpret_cast<const v8bf*>(ro + k0 + 8 * g), acc, 0, 0, 0);
+    if (k0 < ldb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const v4s*>(ra + k0 + 4 * g),
+                                                        *reinterpret_cast<const v4s*>(ro + k0 + 4 * g), acc, 0, 0, 0);
+    return acc;
 }
 
 __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restrict__ Xb, int64_t ldb,
@@ -367,14 +384,8 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
             int A, O;
             tri_pair16(q, nb + 1, A, O);  // A < O' over nb + 1 blocks -> A <= O = O' - 1
             O -= 1;
-            const uint16_t* ra = bsm16 + (16 * A + mrow) * lds_ld + kq;
-            const uint16_t* ro = bsm16 + (16 * O + mrow) * lds_ld + kq;
-            v4f acc = {0.f, 0.f, 0.f, 0.f};
-            for (int64_t k0 = 0; k0 < ldb; k0 += 16) {
-                const v4s av = *reinterpret_cast<const v4s*>(ra + k0);
-                const v4s bv = *reinterpret_cast<const v4s*>(ro + k0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-            }
+            const v4f acc = gram_block16(bsm16 + (16 * A + mrow) * lds_ld, bsm16 + (16 * O + mrow) * lds_ld, (int)ldb,
+                                         lane >> 4);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int a = 16 * A + kq + r;  // D[row][col] layout of the 16x16 MFMA
@@ -393,145 +404,262 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
 
 // The same pair distances for the trees after the first few, writing only the pairs that can enter a heap:
 // a push of d onto p's heap changes it only if d < p's root, and the roots only fall while a tree's pushes
-// are applied, so the pairs with d >= p's root at the start of the tree are dropped for p.  Every position
-// gets a mask of its surviving columns (Msk, 8 words, bit o = member o) and its surviving distances in
-// column order at the start of its D row.  A wave owns 16 rows at a time, keeps their bf16 operands in
-// registers and walks the column blocks in order, so each survivor's rank is a running count (the full
-// square is computed: in exact mode G(a, o) and G(o, a) are the same exact integer).  surv counts the
-// survivors.
-constexpr int kSurvMaxK = 16;  // 16-wide k steps held in registers (ldb <= 256)
-template <int NK>
-__global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restrict__ Xb, int64_t ldb,
+// are applied, so the pairs with d >= p's root at the start of the tree are dropped for p.  Each unordered
+// pair is evaluated once (16 x 16 blocks A <= O of the leaf's Gram matrix, the upper triangle inside a diagonal
+// block; exact-mode distances are symmetric) and tested against both endpoints' roots.  A survivor is written
+// at its column in the member's D row, and the member's mask (Msk, 8 words, bit o = member o) gets its bit; the
+// masks are gathered in LDS with atomic ORs and written once per leaf.  surv counts the survivors.
+// Launched with 256 threads (four waves) and mcap <= 256; NK16 = ldb / 16.  Wave w owns the row-block pairs
+// (P, nb - 1 - P), P = w, w + 4, ... (nb + 1 block-pair columns each, balanced), keeps a row block's operands
+// and its rows' norms and roots in registers and walks the column blocks O >= A two at a time (two independent
+// MFMA chains).  PRE (rows of at most 128 bf16): the leaves are software-pipelined - wave w stages rows
+// 64 w .. 64 w + 63 of a leaf, 16 lanes per row, taking the row ids from its own lanes by shuffle; while a leaf
+// is evaluated, the next leaf's rows (ids loaded one leaf earlier) are already in flight in registers.
+template <int NK16, bool PRE>
+__global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restrict__ Xb,
                                                         const float* __restrict__ nrm, const int32_t* __restrict__ tidx,
                                                         const int64_t* __restrict__ leaves,
                                                         const int64_t* __restrict__ n_leaves_dev, int mcap,
                                                         const float* __restrict__ hdist, int k, float* __restrict__ D,
                                                         int64_t Dl4, uint32_t* __restrict__ Msk,
                                                         unsigned long long* surv) {
+    constexpr int ldb = 16 * NK16, NK32 = NK16 / 2, v8 = ldb / 8;
     extern __shared__ __align__(16) uint16_t ssm16[];
     const int B = blockDim.x, t = threadIdx.x;
     const int w = t >> 6, lane = t & 63, nw = B >> 6;
-    const int lds_ld = 16 * NK + 8;  // bf16 elements per staged row (16-byte aligned, skewed banks)
+    constexpr int lds_ld = ldb + 8;  // bf16 elements per staged row (16-byte aligned, skewed banks)
     const int mp16 = (int)round_up((int64_t)mcap, (int64_t)16);
     float* nr = (float*)(ssm16 + (int64_t)mp16 * lds_ld);
     float* rt = nr + mp16;  // each member's heap root
     int32_t* rid = (int32_t*)(rt + mp16);
+    uint32_t* lmask = (uint32_t*)(rid + mp16);  // mp16 x 8 survivor mask words
     const int64_t n_leaves = *n_leaves_dev;
-    const int mrow = lane & 15, kq = (lane >> 4) * 4, kqi = lane >> 4;
-    unsigned long long ns = 0;
-    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
-        const int64_t s = leaves[2 * leaf];
-        const int m = (int)(leaves[2 * leaf + 1] - s);
-        if (m > mcap || m < 2) continue;  // uniform
-        const int mp = round_up(m, 16);
-        constexpr int v8 = 2 * NK;  // 16-byte chunks per row
-        for (int i = t; i < mp; i += B) rid[i] = i < m ? tidx[s + i] : -1;
-        __syncthreads();
-        // then the rows (sixteen 16-byte loads in flight per thread), norms and roots in one round trip
-        for (int i0 = 0; i0 < mp * v8; i0 += 16 * B) {
-            uint4 v[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int i = min(i0 + u * B + t, mp * v8 - 1);
-                const int r = i / v8, e = i - r * v8;
-                const int32_t id = rid[r];
-                v[u] = *reinterpret_cast<const uint4*>(Xb + (int64_t)(id < 0 ? 0 : id) * ldb + e * 8);
-            }
-            if (i0 == 0)
-                for (int i = t; i < mp; i += B) {
-                    const int32_t id = rid[i];
-                    nr[i] = id >= 0 ? nrm[id] : 0.f;
-                    rt[i] = id >= 0 ? hdist[(int64_t)id * k] : 0.f;
-                }
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int i = i0 + u * B + t;
-                if (i < mp * v8) {
-                    const int r = i / v8, e = i - r * v8;
-                    *reinterpret_cast<uint4*>(ssm16 + r * lds_ld + e * 8) =
-                        rid[r] < 0 ? make_uint4(0, 0, 0, 0) : v[u];  // rows m..mp-1 are zero
-                }
-            }
+    const int64_t G = gridDim.x;
+    const int mrow = lane & 15, kqi = lane >> 4, kq = kqi * 4;
+    const int dl = (int)Dl4;
+    auto geom = [&](int64_t L, int64_t& s, int& m) {  // wave-uniform; m = 0 for a leaf this kernel skips
+        s = 0;
+        m = 0;
+        if (L < n_leaves) {
+            s = leaves[2 * L];
+            m = (int)(leaves[2 * L + 1] - s);
+            if (m > mcap || m < 2) m = 0;
         }
-        __syncthreads();
-        const int nb = mp >> 4;
-        for (int R = w; R < nb; R += nw) {  // wave-uniform: rows 16 R .. 16 R + 15
-            v4s areg[NK];
-            const uint16_t* ra = ssm16 + (16 * R + mrow) * lds_ld + kq;
+    };
+    auto own_id = [&](int64_t s, int m) -> int32_t {  // the id of row 64 w + lane
+        const int r = 64 * w + lane;
+        return r < m ? tidx[s + r] : -1;
+    };
+    // PRE: the staged leaf's row 64 w + lane (id, norm, root) and chunk lane % 16 of rows 64 w + 4 u + lane / 16
+    int32_t pid = -1;
+    float pnr = 0.f, prt = 0.f;
+    uint4 pv[16];
+    auto issue = [&](int32_t id) {
+        pid = id;
+        pnr = id >= 0 ? nrm[id] : 0.f;
+        prt = id >= 0 ? hdist[(int64_t)id * k] : 0.f;
 #pragma unroll
-            for (int kk = 0; kk < NK; ++kk) areg[kk] = *reinterpret_cast<const v4s*>(ra + 16 * kk);
-            float nx[4], rx[4];
-            uint32_t cnt[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                nx[r] = nr[16 * R + kq + r];
-                rx[r] = rt[16 * R + kq + r];
-            }
-            for (int C = 0; C < nb; C += 2) {  // two column blocks per step: independent MFMA chains
-                const int C1 = C + 1 < nb ? C + 1 : C;  // an odd last block repeats C; its results are unused
-                const uint16_t* rb0 = ssm16 + (16 * C + mrow) * lds_ld + kq;
-                const uint16_t* rb1 = ssm16 + (16 * C1 + mrow) * lds_ld + kq;
-                v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kk = 0; kk < NK; ++kk) {
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(areg[kk], *reinterpret_cast<const v4s*>(rb0 + 16 * kk),
-                                                                     acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(areg[kk], *reinterpret_cast<const v4s*>(rb1 + 16 * kk),
-                                                                     acc1, 0, 0, 0);
+        for (int u = 0; u < 16; ++u) {
+            const int32_t ru = __shfl(id, 4 * u + (lane >> 4));
+            pv[u] = (ru >= 0 && mrow < v8) ? *reinterpret_cast<const uint4*>(Xb + (int64_t)ru * ldb + mrow * 8)
+                                           : make_uint4(0, 0, 0, 0);
+        }
+    };
+    int64_t s0, s1;
+    int m0, m1;
+    geom(blockIdx.x, s0, m0);
+    geom(blockIdx.x + G, s1, m1);
+    int32_t id1 = -1;
+    if (PRE) {
+        id1 = own_id(s1, m1);
+        issue(own_id(s0, m0));
+    }
+    unsigned long long ns = 0;
+    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += G) {
+        const int64_t s = s0;
+        const int m = m0;
+        const int mp = round_up(m, 16);
+        if (m) {
+            for (int i = t; i < mp * 8; i += B) lmask[i] = 0u;
+            if (PRE) {
+                const int r = 64 * w + lane;
+                if (r < mp) {
+                    rid[r] = pid;
+                    nr[r] = pnr;
+                    rt[r] = prt;
                 }
-                uint32_t mw[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {  // blocks C, C + 1 in order: ranks are running counts
-                    const int Ch = C + h;
-                    if (Ch >= nb) break;  // wave-uniform
-                    const int o = 16 * Ch + mrow;  // D[row][col] layout: row 16 R + kq + r, column o
-                    const float no = nr[o];
+                for (int u = 0; u < 16; ++u) {
+                    const int row = 64 * w + 4 * u + (lane >> 4);
+                    if (row < mp && mrow < v8) *reinterpret_cast<uint4*>(ssm16 + row * lds_ld + mrow * 8) = pv[u];
+                }
+            } else {
+                for (int i = t; i < mp; i += B) rid[i] = i < m ? tidx[s + i] : -1;
+                lds_barrier();
+                // then the rows (eight 16-byte loads in flight per thread), norms and roots in one round trip
+                for (int i0 = 0; i0 < mp * v8; i0 += 8 * B) {
+                    uint4 v[8];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int x = 16 * R + kq + r;
-                        const float d = (nx[r] + no) - 2.0f * (h ? acc1[r] : acc0[r]);
-                        const bool sv = x < m && o < m && o != x && d < rx[r];
-                        const uint32_t half = (uint32_t)(__ballot(sv) >> (16 * kqi)) & 0xffffu;
-                        if (sv) D[(s + x) * Dl4 + cnt[r] + __popc(half & ((1u << mrow) - 1u))] = d;
-                        cnt[r] += __popc(half);
-                        mw[r] |= half << (16 * h);
+                    for (int u = 0; u < 8; ++u) {
+                        const int i = min(i0 + u * B + t, mp * v8 - 1);
+                        const int r = i / v8, e = i - r * v8;
+                        const int32_t id = rid[r];
+                        v[u] = *reinterpret_cast<const uint4*>(Xb + (int64_t)(id < 0 ? 0 : id) * ldb + e * 8);
+                    }
+                    if (i0 == 0)
+                        for (int i = t; i < mp; i += B) {
+                            const int32_t id = rid[i];
+                            nr[i] = id >= 0 ? nrm[id] : 0.f;
+                            rt[i] = id >= 0 ? hdist[(int64_t)id * k] : 0.f;
+                        }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int i = i0 + u * B + t;
+                        if (i < mp * v8) {
+                            const int r = i / v8, e = i - r * v8;
+                            *reinterpret_cast<uint4*>(ssm16 + r * lds_ld + e * 8) =
+                                rid[r] < 0 ? make_uint4(0, 0, 0, 0) : v[u];  // rows m..mp-1 are zero
+                        }
                     }
                 }
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {  // mask word C / 2 of row x
-                    const int x = 16 * R + kq + r;
-                    if (mrow == 0 && x < m) Msk[(s + x) * 8 + (C >> 1)] = mw[r];
-                }
             }
+            lds_barrier();
+        }
+        // the next leaf's rows go in flight now (their ids were loaded one leaf ago), and the ids of the one after
+        int64_t s2;
+        int m2;
+        geom(leaf + 2 * G, s2, m2);
+        if (PRE) {
+            issue(id1);
+            id1 = own_id(s2, m2);
+        }
+        s0 = s1;
+        m0 = m1;
+        s1 = s2;
+        m1 = m2;
+        if (!m) continue;  // uniform
+        float* const Dl = D + s * Dl4;  // the leaf's D rows; offsets below stay 32-bit
+        const int nb = mp >> 4;
+        for (int P = w; P < ((nb + 1) >> 1); P += nw) {
+            for (int h = 0; h < 2; ++h) {
+                const int A = h ? nb - 1 - P : P;  // wave-uniform row block
+                if (h && A == P) break;            // the middle block of an odd count
+                v8bf a32[NK32 > 0 ? NK32 : 1];
+                v4s a16;
+                {
+                    const uint16_t* ra = ssm16 + (16 * A + mrow) * lds_ld;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = 16 * R + kq + r;
-                if (mrow == 0 && x < m) {
-                    for (int j = (nb + 1) >> 1; j < 8; ++j) Msk[(s + x) * 8 + j] = 0u;  // columns past the leaf
-                    ns += cnt[r];
+                    for (int kk = 0; kk < NK32; ++kk) a32[kk] = *reinterpret_cast<const v8bf*>(ra + 32 * kk + 8 * kqi);
+                    if (NK16 & 1) a16 = *reinterpret_cast<const v4s*>(ra + 32 * NK32 + 4 * kqi);
+                }
+                const int x0 = 16 * A + kq;  // this lane's rows x0 .. x0 + 3
+                const float4 nx4 = *reinterpret_cast<const float4*>(nr + x0);
+                const float4 rx4 = *reinterpret_cast<const float4*>(rt + x0);
+                const float nx[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, rx[4] = {rx4.x, rx4.y, rx4.z, rx4.w};
+                // B operands of column blocks (O, O + 1), the next pair's loaded while this pair's results are
+                // tested (an odd last block repeats O; its results are unused)
+                v8bf b0[NK32 > 0 ? NK32 : 1], b1[NK32 > 0 ? NK32 : 1];
+                v4s t0, t1;
+                float cn[2], cr[2];  // the two columns' norms and roots
+                auto load_b = [&](int O) {
+                    const int O1 = O + 1 < nb ? O + 1 : O;
+                    cn[0] = nr[16 * O + mrow];
+                    cr[0] = rt[16 * O + mrow];
+                    cn[1] = nr[16 * O1 + mrow];
+                    cr[1] = rt[16 * O1 + mrow];
+                    const uint16_t* rb0 = ssm16 + (16 * O + mrow) * lds_ld;
+                    const uint16_t* rb1 = ssm16 + (16 * O1 + mrow) * lds_ld;
+#pragma unroll
+                    for (int kk = 0; kk < NK32; ++kk) {
+                        b0[kk] = *reinterpret_cast<const v8bf*>(rb0 + 32 * kk + 8 * kqi);
+                        b1[kk] = *reinterpret_cast<const v8bf*>(rb1 + 32 * kk + 8 * kqi);
+                    }
+                    if (NK16 & 1) {
+                        t0 = *reinterpret_cast<const v4s*>(rb0 + 32 * NK32 + 4 * kqi);
+                        t1 = *reinterpret_cast<const v4s*>(rb1 + 32 * NK32 + 4 * kqi);
+                    }
+                };
+                load_b(A);
+                for (int O = A; O < nb; O += 2) {  // two column blocks per step: independent MFMA chains
+                    v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kk = 0; kk < NK32; ++kk) {
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a32[kk], b0[kk], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a32[kk], b1[kk], acc1, 0, 0, 0);
+                    }
+                    if (NK16 & 1) {
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a16, t0, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a16, t1, acc1, 0, 0, 0);
+                    }
+                    const float cnow_n[2] = {cn[0], cn[1]}, cnow_r[2] = {cr[0], cr[1]};
+                    if (O + 2 < nb) load_b(O + 2);  // wave-uniform
+#pragma unroll
+                    for (int hb = 0; hb < 2; ++hb) {
+                        const int Oh = O + hb;
+                        if (Oh >= nb) break;  // wave-uniform
+                        const v4f acc = hb ? acc1 : acc0;
+                        const int o = 16 * Oh + mrow;  // this lane's column
+                        const float no = cnow_n[hb], ro = cnow_r[hb];
+                        const bool col_ok = o < m;
+                        bool sx[4], so[4];
+                        float d[4];
+                        uint64_t bx[4];
+                        uint64_t any_so = 0;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            d[r] = (nx[r] + no) - 2.0f * acc[r];
+                            const bool pair = col_ok & (x0 + r < o);  // each unordered pair once, both directions
+                            sx[r] = pair & (d[r] < rx[r]);            // o enters x's heap
+                            so[r] = pair & (d[r] < ro);               // x enters o's heap
+                            bx[r] = __ballot(sx[r]);
+                            const uint64_t bo = __ballot(so[r]);
+                            any_so |= bo;
+                            ns += (unsigned)(__popcll(bx[r]) + __popcll(bo));  // wave-uniform count
+                        }
+                        uint32_t nib = 0u;  // rows x that survive for column o's member (o's mask bits)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int x = x0 + r;
+                            if (bx[r]) {  // wave-uniform
+                                if (sx[r]) Dl[x * dl + o] = d[r];
+                                const uint32_t half = (uint32_t)(bx[r] >> (16 * kqi)) & 0xffffu;
+                                if ((mrow == 0) & (half != 0u))
+                                    atomicOr(&lmask[x * 8 + (Oh >> 1)], half << (16 * (Oh & 1)));
+                            }
+                            if (any_so && so[r]) Dl[o * dl + x] = d[r];
+                            nib |= (uint32_t)so[r] << r;
+                        }
+                        if (!any_so) continue;  // wave-uniform: no mask bits for any column this block
+                        // row o, columns 16 A + kq .. + 3: each lane group ORs its nibble (no return, no wait)
+                        if (nib) atomicOr(&lmask[o * 8 + (A >> 1)], nib << (kq + 16 * (A & 1)));
+                    }
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
+        for (int i = t; i < m * 8; i += B) Msk[(s + (i >> 3)) * 8 + (i & 7)] = lmask[i];
+        lds_barrier();
     }
-    for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o);
-    if (lane == 0 && ns) atomicAdd(surv, ns);
+    if (lane == 0 && ns) atomicAdd(surv, ns);  // ns is the wave's count (the same in every lane)
 }
 
 size_t leaf_surv_mfma_lds(int64_t ldb, int mcap) {
-    if (ldb > 16 * kSurvMaxK) return SIZE_MAX;  // operands held in registers
+    if (ldb > 256 || mcap > 256) return SIZE_MAX;  // instances cover rows of up to 256 bf16, leaves of up to 256
     const size_t mp16 = (size_t)round_up((int64_t)mcap, (int64_t)16);
-    return mp16 * (ldb + 8) * 2 + mp16 * 12;
+    return mp16 * (ldb + 8) * 2 + mp16 * 12 + mp16 * 32;
 }
 
 void launch_leaf_surv_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
                            const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, const float* hdist, int k,
                            float* D, int64_t Dl4, uint32_t* Msk, unsigned long long* surv) {
-    const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(max_leaves, 1), 4096);
+    if (mcap > 256) throw Error{RINND_ERR_INVALID_ARG, "survivor leaf kernel: leaves above 256 members"};
+    // rows of at most 128 bf16: persistent, two workgroups per CU (the LDS limit), each pipelining its leaves
+    const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(max_leaves, 1),
+                                                      R.ldb <= 128 ? 2 * (int64_t)device_cus() : 4096);
     const size_t lds = leaf_surv_mfma_lds(R.ldb, mcap);
 #define RINND_SURV_CASE(NKV)                                                                                     \
     case NKV:                                                                                                    \
-        hipLaunchKernelGGL(k_leaf_surv_mfma<NKV>, dim3(grid), dim3(256), lds, s, R.Xb.p, R.ldb, R.nrm.p,          \
+        hipLaunchKernelGGL((k_leaf_surv_mfma<NKV, (NKV <= 8)>), dim3(grid), dim3(256), lds, s, R.Xb.p, R.nrm.p,   \
                            tree_indices, leaves, n_leaves_dev, mcap, hdist, k, D, Dl4, Msk, surv);               \
         break;
     switch (R.ldb / 16) {

@@ -385,9 +385,9 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
     }
 }
 
-// The pushes of a survivor tree: each position's column mask and its surviving distances (in column
-// order at the start of its D row) from k_leaf_surv_mfma; the lane walks them in leaf order, the next
-// survivor's distance and partner id in flight while the current one is pushed.
+// The pushes of a survivor tree: each position's column mask and its surviving distances (survivor o at
+// column o of its D row) from k_leaf_surv_mfma; the lane walks them in leaf order, the next survivor's
+// distance and partner id in flight while the current one is pushed.
 template <bool REG>
 __global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restrict__ tidx,
                                                         const int64_t* __restrict__ leaves,
@@ -423,24 +423,22 @@ __global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restric
             const uint4 m0 = *reinterpret_cast<const uint4*>(Msk + pos * 8);
             const uint4 m1 = *reinterpret_cast<const uint4*>(Msk + pos * 8 + 4);
             const uint32_t words[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
-            const float* Dr = D + pos * Dl4;
-            int r = 0;
-            float dn = Dr[0];  // the next survivor's distance (a valid read even with no survivor)
+            const float* Dr = D + pos * Dl4;  // survivor o's distance at column o
 #pragma unroll
             for (int wi = 0; wi < 8; ++wi) {
                 uint32_t bits = words[wi];
                 int j = bits ? __ffs((int)bits) - 1 : -1;
                 int32_t xj = j >= 0 ? tidx[s + 32 * wi + j] : 0;
-                while (j >= 0) {
+                float dj = j >= 0 ? Dr[32 * wi + j] : 0.f;
+                while (j >= 0) {  // the next survivor's id and distance in flight
                     bits &= bits - 1;
                     const int jn = bits ? __ffs((int)bits) - 1 : -1;
                     const int32_t xn = jn >= 0 ? tidx[s + 32 * wi + jn] : 0;
-                    const float d = dn;
-                    ++r;
-                    dn = Dr[r < Dl4 ? r : 0];
-                    leaf_offer<REG>(hd, hi, hf, S, c, k, xj, d, root, sid);
+                    const float dnx = jn >= 0 ? Dr[32 * wi + jn] : 0.f;
+                    leaf_offer<REG>(hd, hi, hf, S, c, k, xj, dj, root, sid);
                     j = jn;
                     xj = xn;
+                    dj = dnx;
                 }
             }
         }
