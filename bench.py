@@ -61,6 +61,10 @@ def parse():
                     help="comma-separated query epsilons; the fastest run at recall@10 >= 0.95 is reported")
     ap.add_argument("--fp8-epsilons", default="0.1,0.15",
                     help="epsilons of the FP8-candidate search with exact re-ranking (empty: none)")
+    ap.add_argument("--no-precision-form", action="store_true",
+                    help="skip the 1,000,000 x 960, k=30 precision-form builds (BF16 join vs FP32)")
+    ap.add_argument("--pf-n", type=int, default=1_000_000, help="rows of the precision-form workload")
+    ap.add_argument("--pf-steps", type=int, default=2, help="timed builds per precision of the precision form")
     return ap.parse_args()
 
 
@@ -237,6 +241,11 @@ def main():
             line["query"] = query_bench(torch, L, ctx, X, Xd, args)
         except Exception as e:
             log(f"[bench] query bench failed: {e}")
+    if not args.profile and not args.no_precision_form:
+        try:
+            line["precision_form"] = precision_form(torch, ctx, L, args)
+        except Exception as e:
+            log(f"[bench] precision form failed: {e}")
     if not args.profile and not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(X, args)
@@ -292,6 +301,78 @@ def roofline_of(stats, head):
     return {"bound": "hbm", "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": k["frac_hbm"], "traffic": measured_traffic(k["kernel"]), "kernel": k["kernel"],
             "avg_launch_ms": k["avg_launch_ms"], "bytes_per_launch": k["bytes_per_launch"]}
+
+
+def precision_form(torch, ctx, L, args):
+    """1,000,000 x 960 Gaussian mixture (100 centres U[0,1]^960, sigma 0.1, clipped to [0,1]), squared Euclidean,
+    k=30 (max_candidates 30, leaf 150, 12 trees, <= 20 iterations), counter draws: the BF16-MFMA join with FP32
+    re-evaluation inside the proven margin against the FP32 join.  Reports points/s of both, the fraction of
+    pairs re-evaluated, recall@30 of both against an FP64 brute force (tie-aware) and of the BF16 graph against
+    the FP32 graph, and the largest |stored - FP64| distance of the sampled rows."""
+    from rinnd_amd import datasets
+    n, d, k = args.pf_n, 960, 30
+    log(f"[bench] precision form: generating {n}x{d} Gaussian mixture")
+    X = datasets.cfg3(n=n, d=d)
+    Xd = torch.from_numpy(X).cuda()
+    del X
+    pa = argparse.Namespace(**vars(args))
+    pa.k, pa.trees, pa.leaf, pa.iters, pa.delta, pa.rng = k, 12, 150, 20, 0.001, "counter"
+    outs = (torch.empty((n, k), dtype=torch.int32, device="cuda"),
+            torch.empty((n, k), dtype=torch.float32, device="cuda"),
+            torch.empty((n, k), dtype=torch.uint8, device="cuda"))
+    res = {}
+    for prec in ("fp32", "bf16"):
+        gpu_build(ctx, L, Xd, n, d, pa, prec, outs)  # warm-up
+        torch.cuda.synchronize()
+        times, stats = [], []
+        for s_ in range(args.pf_steps):
+            t0 = time.perf_counter()
+            st = gpu_build(ctx, L, Xd, n, d, pa, prec, outs)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            stats.append(st)
+            log(f"[bench] precision form {prec} step {s_ + 1}: {times[-1] * 1e3:.1f} ms, iters {st['iterations']}, "
+                f"forest {st['ms_forest']:.1f} init {st['ms_init']:.1f} cand {st['ms_candidates']:.1f} "
+                f"join {st['ms_join']:.1f} apply {st['ms_apply']:.1f} ms, rescored {st['join_rescored']}")
+        res[prec] = {"times": times, "stats": stats, "idx": outs[0].clone(), "dist": outs[1].clone()}
+    rows = np.random.default_rng(3).choice(n, size=min(args.recall_rows, n), replace=False)
+    rt = torch.as_tensor(rows, device="cuda")
+    true_d, _ = exact_knn_f64(torch, Xd, rows, k, chunk=500)
+    out = {"workload": f"{n}x{d} Gaussian mixture (100 centres, sigma 0.1, clipped to [0,1]), sqeuclidean, k={k}, "
+                       f"max_candidates {k}, leaf 150, 12 trees, <= 20 iterations, counter draws",
+           "dtype": "bf16-join/f32"}
+    X64 = Xd.double()
+    for prec, r in res.items():
+        st = r["stats"][-1]
+        it = st["iterations"]
+        pairs = sum(st["join_pairs"][:it])
+        resc = sum(st["join_rescored"][:it])
+        ids = r["idx"][rt]
+        f = ids.long().clamp(min=0)
+        d64 = ((X64[f] - X64[rt][:, None, :]) ** 2).sum(-1)
+        dd = (r["dist"][rt].double() - d64).abs()
+        dd[ids < 0] = 0
+        out[prec] = {"points_per_s": n / (sum(r["times"]) / len(r["times"])),
+                     "ms_per_build": 1e3 * sum(r["times"]) / len(r["times"]), "iterations": it,
+                     "phase_ms": {x: st[x] for x in ("ms_forest", "ms_init", "ms_candidates", "ms_join", "ms_apply")},
+                     "rescored_fraction": (resc / pairs) if (prec == "bf16" and pairs) else None,
+                     "recall_at_k_vs_fp64": tie_aware_recall(torch, Xd, Xd[rt], ids, true_d, k),
+                     "max_abs_dist_err_vs_fp64": float(dd.max()),
+                     "max_rel_dist_err_vs_fp64": float((dd / d64.clamp(min=1e-30)).max())}
+    a, b = res["fp32"]["idx"][rt], res["bf16"]["idx"][rt]
+    inter = (a[:, :, None] == b[:, None, :]).any(-1).double().sum(1) / k
+    out["bf16"]["recall_at_k_vs_fp32_path"] = float(inter.mean())
+    out["bf16_graph_equals_fp32_graph"] = bool(torch.equal(res["fp32"]["idx"], res["bf16"]["idx"]) and
+                                               torch.equal(res["fp32"]["dist"].view(torch.int32),
+                                                           res["bf16"]["dist"].view(torch.int32)))
+    out["bf16_speedup"] = out["bf16"]["points_per_s"] / out["fp32"]["points_per_s"]
+    log(f"[bench] precision form: fp32 {out['fp32']['ms_per_build']:.1f} ms, bf16 {out['bf16']['ms_per_build']:.1f} ms "
+        f"({out['bf16_speedup']:.2f}x), rescored {out['bf16']['rescored_fraction']}, recall fp32 "
+        f"{out['fp32']['recall_at_k_vs_fp64']:.4f} bf16 {out['bf16']['recall_at_k_vs_fp64']:.4f}, identical "
+        f"{out['bf16_graph_equals_fp32_graph']}")
+    del X64, Xd
+    torch.cuda.empty_cache()
+    return out
 
 
 def reference_build(torch, ctx, L, Xd, n, dim, args, outs, rows, true_d):

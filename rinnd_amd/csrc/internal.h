@@ -382,6 +382,7 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a);
 struct Bf16Rows {
     DBuf<uint16_t> Xb;  // n x ldb bf16 (centred unless exact)
     DBuf<float> nrm;    // FP32 squared norms of the rounded rows
+    DBuf<float> err;    // per row, an upper bound of |bf16(x - mu) - (x - mu)| (non-exact rows)
     DBuf<float> mu;     // column means used for centring
     DBuf<double> aux;   // [flag word | max norm | column sums] of the preparation pass
     int64_t ldb = 0;

@@ -5,17 +5,20 @@
 // G = A B^T of the bf16-rounded candidate rows on v_mfma_f32_16x16x16bf16 (FP32 accumulation) and
 // the screened distance  dt = |a|^2 + |b|^2 - 2 G,  where |a|^2 are FP32 norms of the rounded rows.
 //
-// Error bound.  With x^ = bf16(x) (|x^ - x| <= u|x|, u = 2^-8), D = |x - y|^2, N = |x|^2 + |y|^2:
-//   | |x^ - y^|^2 - D |  <=  2u sqrt(2 D N) + 2u^2 N                      (rounding of the rows)
-//   | dt - |x^ - y^|^2 | <=  (2 g_d + 4 u24) N^                         (FP32 norms, Gram, final ops)
-//   | D_ref - D |        <=  g_r D                                       (the reference's FP32 sum)
-// with u24 = 2^-24, g_d = d u24 / (1 - d u24), g_r = 1.01 (d + 8) u24 and N^ <= (1 + 4.1u) N.
-// A pair can pass the reference filter D_ref <= thr only if D <= thr (1 + g_r) =: Db, so every
-// such pair has dt <= thr + E with
-//   E = 1.05 (2u sqrt(2 Db N') + 2u^2 N' + (2 g_d + 4 u24) N' + g_r Db),  N' = (1 + 4.1u) N^.
-// Those pairs are re-evaluated with the reference-order FP32 distance and filtered exactly, so
-// the emitted updates - and the final heaps - equal the FP32 path's.  Rows are centred by the
-// column mean first (distances are translation invariant; N shrinks).
+// Error bound.  Rows are centred by the column means mu first (distances are translation invariant; the
+// norms shrink): c = fl(x - mu), x^ = bf16(c).  The preparation pass stores, per row, an upper bound e_x of
+// |x^ - (x - mu)| (the rounding x^ - c is exact in FP32, its squared norm is summed with relative error g_d,
+// and the centring adds at most 1.01 u24 |c|).  With a = x - y, a^ = x^ - y^ and e = e_x + e_y, the triangle
+// inequality gives |a^ - a| <= e, so |a^|^2 <= (|a| + e)^2.  The screened value dt = (n_x + n_y) - 2 G, from the
+// FP32 norms n of the rounded rows and the MFMA Gram G with FP32 accumulation, is within
+// F = (2 g_d + 4 u24) (n_x + n_y) (1 + 2 g_d) of |a^|^2, and the reference's FP32 sum D_ref is within g_r |a|^2
+// of |a|^2 (u24 = 2^-24, g_d = d u24 / (1 - d u24), g_r = 1.01 (d + 8) u24).  A pair can pass the reference
+// filter D_ref < thr only if |a|^2 < thr / (1 - g_r) =: Db, and then
+//   dt < (sqrt(Db) + e)^2 + F = Db + 2 e sqrt(Db) + e^2 + F.
+// Every pair with dt <= Db + 1.05 (2 e sqrt(Db) + e^2 + F) is therefore re-evaluated with the reference-order
+// FP32 distance and filtered exactly; no other pair can be kept, so the emitted updates - and the final heaps -
+// equal the FP32 path's.  The pairs to re-evaluate are queued in LDS by the screen and evaluated afterwards
+// with every lane busy.
 //
 // Exact mode.  If every coordinate is an integer with |x| <= 256 (exact in bf16) and either every
 // |x|^2 <= 2^22, or no coordinate is negative and every |x|^2 <= 2^23, then all products, partial sums,
@@ -123,15 +126,18 @@ __global__ __launch_bounds__(256) void k_check_to_bf16(const float* __restrict__
     }
 }
 
-// bf16 rows (optionally centred) and FP32 norms of the rounded rows; one wave per row
+// bf16 rows (optionally centred) and FP32 norms of the rounded rows; one wave per row.  err (if not null) gets an
+// upper bound of |x^ - (x - mu)|: the rounding x^ - c of each centred coordinate c is exact in FP32, the sum of
+// their squares carries a relative error of at most g_d, and the centring c = fl(x - mu) adds at most
+// 1.01 u24 |c| <= 1.01 u24 (|x^| + |x^ - c|).
 __global__ void k_to_bf16(const float* __restrict__ X, int64_t ld, int64_t n, int dim,
                           const float* __restrict__ mu, uint16_t* __restrict__ Xb, int64_t ldb,
-                          float* __restrict__ nrm) {
+                          float* __restrict__ nrm, float* __restrict__ err, float g_d) {
     const int lane = threadIdx.x & 63;
     const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t r = wid; r < n; r += nw) {
-        float s = 0.f;
+        float s = 0.f, se = 0.f;
         for (int c = lane; c < ldb; c += 64) {
             uint16_t h = 0;
             if (c < dim) {
@@ -139,11 +145,23 @@ __global__ void k_to_bf16(const float* __restrict__ X, int64_t ld, int64_t n, in
                 h = f2bf(v);
                 const float f = bf2f(h);
                 s = __builtin_fmaf(f, f, s);
+                const float dv = f - v;  // exact
+                se = __builtin_fmaf(dv, dv, se);
             }
             Xb[r * ldb + c] = h;
         }
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (lane == 0) nrm[r] = s;
+        for (int o = 32; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o);
+            se += __shfl_xor(se, o);
+        }
+        if (lane == 0) {
+            nrm[r] = s;
+            if (err) {
+                const float eb = sqrtf(se * (1.0f + g_d)) * (1.0f + 2.4e-7f);
+                const float en = sqrtf(s * (1.0f + g_d)) * (1.0f + 2.4e-7f);
+                err[r] = (eb + 1.01f * 5.9604645e-8f * (en + eb)) * (1.0f + 4.8e-7f);
+            }
+        }
     }
 }
 
@@ -154,6 +172,7 @@ struct JoinBF16Args {
     const uint16_t* Xb;  // bf16 rows
     int64_t ldb;
     const float* nrm;
+    const float* err;  // per-row rounding error bound (non-exact rows)
     const int32_t* new_idx;
     const int32_t* old_idx;
     int C;
@@ -166,16 +185,19 @@ struct JoinBF16Args {
     unsigned long long* rescored;  // kKeptSlots counters
 };
 
+constexpr int kJoinQueue = 1024;  // re-evaluation queue entries per wave (flushed when 256 more might not fit)
 template <int TA, int TB, bool EXACT>
 __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
     constexpr int W = 4;
     __shared__ int32_t ids[W][TB * 16];
     __shared__ float roots[W][TB * 16];
     __shared__ float nr[W][TB * 16];
+    __shared__ float er[W][EXACT ? 1 : TB * 16];
+    __shared__ uint16_t rq[W][EXACT ? 1 : kJoinQueue];  // queued pair slots j * 2C + kk
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     const int C = a.C, C2 = 2 * C;
-    const float u = 1.0f / 256.0f, u24 = 1.0f / 16777216.0f;
+    const float u24 = 1.0f / 16777216.0f;
     const int m = lane & 15, kq = (lane >> 4) * 4;
     unsigned long long resc = 0, kept = 0;
     for (int64_t i = a.row_begin + (int64_t)blockIdx.x * W + w; i < a.row_end; i += (int64_t)gridDim.x * W) {
@@ -186,6 +208,7 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
             ids[w][c] = id;
             roots[w][c] = id >= 0 ? a.hd[(int64_t)id * a.k] : 0.f;
             nr[w][c] = id >= 0 ? a.nrm[id] : 0.f;
+            if (!EXACT) er[w][c] = id >= 0 ? a.err[id] : 0.f;
         }
         wave_sync();
         const int64_t r0 = i - a.row_begin;
@@ -231,23 +254,52 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                             aval[x] && live ? B[kb][x] : z4, bval[y] ? B[kb][y] : z4, acc[x][y], 0, 0, 0);
             }
         }
+        // kept pairs go to the row's own region in any order (the slot carries the key)
+        auto emit = [&](bool keep, bool ep, bool eq, int slot, float d) {
+            const unsigned long long bal = __ballot(keep);
+            if (keep) {
+                U[run + __popcll(bal & lt)] = JRec{(uint32_t)slot | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
+                recs += (unsigned)ep + (unsigned)eq;
+            }
+            run += (unsigned)__popcll(bal);
+        };
+        // non-exact rows: the pairs inside the margin are queued, then re-evaluated with every lane busy
+        unsigned nq = 0;
+        auto flush = [&]() {
+            wave_sync();
+            for (unsigned e0 = 0; e0 < nq; e0 += 64) {
+                const unsigned e = e0 + lane;
+                bool keep = false, ep = false, eq = false;
+                int slot = 0;
+                float d = 0.f;
+                if (e < nq) {
+                    slot = rq[w][e];
+                    const int j = slot / C2, kk = slot - j * C2;
+                    d = d_sqeuclidean(a.X + (int64_t)ids[w][j] * a.ld, a.X + (int64_t)ids[w][kk] * a.ld, a.dim);
+                    ep = d < roots[w][j];
+                    eq = d < roots[w][kk];
+                    keep = ep || eq;
+                }
+                emit(keep, ep, eq, slot, d);
+            }
+            if (lane == 0) resc += nq;  // nq is wave-uniform: counted once (the lanes' counts are summed below)
+            nq = 0;
+            wave_sync();
+        };
 #pragma unroll
         for (int x = 0; x < TA; ++x)
 #pragma unroll
-            for (int y = 0; y < TB; ++y)
+            for (int y = 0; y < TB; ++y) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int j = 16 * x + (lane >> 4) * 4 + r;  // D[row][col] layout of 16x16 MFMA
                     const int kk = 16 * y + m;
-                    bool keep = false, ep = false, eq = false;
-                    int32_t p = 0, q = 0;
+                    bool keep = false, ep = false, eq = false, need = false;
                     float d = 0.f;
                     if (j < C && kk < C2) {
-                        p = ids[w][j];
-                        q = ids[w][kk];
+                        const int32_t p = ids[w][j], q = ids[w][kk];
                         const bool valid = p >= 0 && q >= 0 && (kk < C ? kk > j : p != q);
                         if (valid) {
-                            const float thr = fmaxf(roots[w][j], roots[w][kk]);
                             const float dt = (nr[w][j] + nr[w][kk]) - 2.0f * acc[x][y][r];
                             if (EXACT) {  // no re-evaluation code in this instance: fewer registers
                                 d = dt;
@@ -255,36 +307,30 @@ __global__ __launch_bounds__(256) void k_join_bf16(JoinBF16Args a) {
                                 eq = d < roots[w][kk];
                                 keep = ep || eq;
                             } else {
-                                bool rescore;
+                                const float thr = fmaxf(roots[w][j], roots[w][kk]);
                                 if (!(thr < __builtin_inff())) {
-                                    rescore = true;
+                                    need = true;
                                 } else {
-                                    const float Np = (nr[w][j] + nr[w][kk]) * (1.0f + 4.1f * u);
-                                    const float Db = thr * (1.0f + a.g_r);
-                                    const float E = 1.05f * (2.0f * u * sqrtf(2.0f * fmaxf(Db, 0.f) * Np) +
-                                                             2.0f * u * u * Np + (2.0f * a.g_d + 4.0f * u24) * Np +
-                                                             a.g_r * fabsf(Db)) + 1e-30f;
-                                    rescore = dt <= thr + E;
-                                }
-                                if (rescore) {
-                                    ++resc;
-                                    d = d_sqeuclidean(a.X + (int64_t)p * a.ld, a.X + (int64_t)q * a.ld, a.dim);
-                                    ep = d < roots[w][j];
-                                    eq = d < roots[w][kk];
-                                    keep = ep || eq;
+                                    const float Db = thr / (1.0f - a.g_r);
+                                    const float e = er[w][j] + er[w][kk];
+                                    const float F = (2.0f * a.g_d + 4.0f * u24) * (nr[w][j] + nr[w][kk]) *
+                                                    (1.0f + 2.0f * a.g_d);
+                                    need = dt <= Db + 1.05f * (2.0f * e * sqrtf(Db) + e * e + F) + 1e-30f;
                                 }
                             }
                         }
                     }
-                    // kept pairs go to the row's own region in any order (the slot carries the key)
-                    const unsigned long long bal = __ballot(keep);
-                    if (keep) {
-                        U[run + __popcll(bal & lt)] =
-                            JRec{(uint32_t)(j * C2 + kk) | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
-                        recs += (unsigned)ep + (unsigned)eq;
+                    if (EXACT) {
+                        emit(keep, ep, eq, j * C2 + kk, d);
+                    } else {
+                        const unsigned long long bq = __ballot(need);
+                        if (need) rq[w][nq + __popcll(bq & lt)] = (uint16_t)(j * C2 + kk);
+                        nq += (unsigned)__popcll(bq);
                     }
-                    run += (unsigned)__popcll(bal);
                 }
+                if (!EXACT && nq > kJoinQueue - 256) flush();  // wave-uniform
+            }
+        if (!EXACT && nq) flush();
         wave_sync();
         for (int o = 32; o > 0; o >>= 1) recs += __shfl_xor(recs, o);
         if (lane == 0) {
@@ -716,8 +762,11 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
         hipLaunchKernelGGL(k_col_sums, dim3(1024), dim3(256), 0, s, X, ld, n, dim, sums);
         hipLaunchKernelGGL(k_col_mean, dim3(ceil_div(dim, 256)), dim3(256), 0, s, sums, n, dim, R.mu.p);
         HIP_TRY(hipStreamSynchronize(s));
+        R.err.ensure(std::max<int64_t>(n, 1));
+        const double u24 = 1.0 / 16777216.0;
         hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)std::min<int64_t>(ceil_div(n, 4), 1 << 16)), dim3(256), 0, s, X,
-                           ld, n, dim, R.mu.p, R.Xb.p, R.ldb, R.nrm.p);
+                           ld, n, dim, R.mu.p, R.Xb.p, R.ldb, R.nrm.p, R.err.p,
+                           (float)(1.01 * dim * u24 / (1.0 - dim * u24)));
     }
     HIP_TRY(hipGetLastError());
 }
@@ -743,6 +792,7 @@ void launch_join_bf16(hipStream_t s, const float* X, int64_t ld, int dim, const 
     a.Xb = R.Xb.p;
     a.ldb = R.ldb;
     a.nrm = R.nrm.p;
+    a.err = R.exact ? nullptr : R.err.p;
     a.new_idx = new_idx;
     a.old_idx = old_idx;
     a.C = C;
