@@ -65,6 +65,11 @@ def parse():
                     help="skip the 1,000,000 x 960, k=30 precision-form builds (BF16 join vs FP32)")
     ap.add_argument("--pf-n", type=int, default=1_000_000, help="rows of the precision-form workload")
     ap.add_argument("--pf-steps", type=int, default=2, help="timed builds per precision of the precision form")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving form (2,000,000 x 96 resident index, Poisson query stream)")
+    ap.add_argument("--sv-n", type=int, default=2_000_000, help="rows of the serving-form index")
+    ap.add_argument("--sv-queries", type=int, default=4_000_000, help="queries of the serving-form stream")
+    ap.add_argument("--sv-load", type=float, default=0.7, help="offered Poisson rate as a fraction of the saturated QPS")
     return ap.parse_args()
 
 
@@ -246,6 +251,11 @@ def main():
             line["precision_form"] = precision_form(torch, ctx, L, args)
         except Exception as e:
             log(f"[bench] precision form failed: {e}")
+    if not args.profile and not args.no_serving:
+        try:
+            line["serving_form"] = serving_form(torch, ctx, L, args)
+        except Exception as e:
+            log(f"[bench] serving form failed: {e}")
     if not args.profile and not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(X, args)
@@ -371,6 +381,149 @@ def precision_form(torch, ctx, L, args):
         f"{out['fp32']['recall_at_k_vs_fp64']:.4f} bf16 {out['bf16']['recall_at_k_vs_fp64']:.4f}, identical "
         f"{out['bf16_graph_equals_fp32_graph']}")
     del X64, Xd
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving_form(torch, ctx, L, args):
+    """Resident 2,000,000 x 96 unit-norm index (500 centres N(0, I), sigma 0.2, rows normalised), cosine on
+    normalised rows (DirectNormalizedCosine), graph k=20 (search degree <= 30); a stream of 4,000,000 queries from the
+    same mixture, in arrival batches of 1-4,096 queries (log-uniform), k in {10, 100} (4:1), epsilon in {0.05, 0.2}
+    (1:1), through the query service (rinnd_server_*): first saturated (4,096-query batches back to back), then an
+    open-loop Poisson stream at a fraction of that rate, for the FP32 search and for FP8 e4m3 candidates on MFMA
+    with FP32 re-ranking.  Latency of a batch = completion - scheduled arrival."""
+    import threading
+    from rinnd_amd import datasets
+    sys.setswitchinterval(1e-4)  # the waiter thread takes the GIL promptly when a request completes
+    n, d, nq = args.sv_n, 96, args.sv_queries
+    log(f"[bench] serving form: generating {n}x{d} index rows and {nq} queries")
+    X, centres = datasets.cfg5(n=n, d=d)
+    Q = datasets.cfg5_queries(centres, nq)
+    Xd = torch.from_numpy(X).cuda()
+    Qd = torch.from_numpy(Q).cuda()
+    del X
+    p = L.IndexParams(20, 12, 1, 1, -1, -1, -1, 0.001, 1.0, 1.5, 1, 42, 0, args.threads, L.RNG["counter"],
+                      L.JOIN["fp32"])
+    h = C.c_void_p()
+    stats = L.BuildStats()
+    t0 = time.perf_counter()
+    ctx.check(L.lib().rinnd_index_build(ctx.h, L.ptr(Xd), n, d, L.DIST["direct_cosine"], L.CORR["none"], C.byref(p),
+                                        C.byref(h), C.byref(stats)), "rinnd_index_build")
+    t_index = time.perf_counter() - t0
+    log(f"[bench] serving form: index built in {t_index:.2f} s ({stats.iterations} iterations)")
+    # FP64 ground truth of a sample of the stream (cosine on unit rows: largest dot products)
+    sample = np.arange(0, nq, max(1, nq // 10_000))
+    X64 = Xd.double()
+    true_dot = []
+    for s0 in range(0, len(sample), 1000):
+        qs = Qd[torch.as_tensor(sample[s0:s0 + 1000], device="cuda")].double()
+        true_dot.append(torch.topk(qs @ X64.T, 10, dim=1).values)
+    true_dot = torch.cat(true_dot)
+    del X64
+    rng = np.random.default_rng(9)
+    sizes = []
+    tot = 0
+    while tot < nq:  # log-uniform arrival batches of 1..4096 queries
+        b = int(min(nq - tot, max(1, round(float(np.exp(rng.uniform(0.0, np.log(4096.0))))))))
+        sizes.append(b)
+        tot += b
+    ks = np.where(rng.random(len(sizes)) < 0.8, 10, 100)
+    eps = np.where(rng.random(len(sizes)) < 0.5, 0.05, 0.2)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    Oi = torch.empty(nq * 100, dtype=torch.int32, device="cuda")
+    Od = torch.empty(nq * 100, dtype=torch.float32, device="cuda")
+    qbase, ibase, dbase = Qd.data_ptr(), Oi.data_ptr(), Od.data_ptr()
+    out = {"workload": f"{n}x{d} unit-norm Gaussian mixture (500 centres N(0, I), sigma 0.2, normalised), "
+                       f"DirectNormalizedCosine, graph k=20 "
+                       f"(search degree <= 30), {nq} queries, arrival batches 1-4096 (log-uniform), k in {{10, 100}} "
+                       f"(4:1), epsilon in {{0.05, 0.2}} (1:1)", "index_build_s": t_index, "batches": len(sizes)}
+
+    def widths(k, fp8):
+        return (64 if k == 10 else 128, 64 if k == 10 else 128) if fp8 else (None, None)
+
+    def run(srv, fp8, arrivals):
+        """Submits batch b at arrivals[b] (seconds after the start, None: at once); returns the wall time, each
+        batch's latency and the per-batch output offsets.  A second thread waits the tickets in order."""
+        tickets = [None] * len(sizes)
+        done_t = [0.0] * len(sizes)
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for b in range(len(sizes)):
+                ready.acquire()
+                srv.wait(tickets[b])
+                done_t[b] = time.perf_counter()
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_start = time.perf_counter()
+        sched = [0.0] * len(sizes)
+        for b, (st, sz) in enumerate(zip(starts, sizes)):
+            if arrivals is not None:
+                sched[b] = t_start + arrivals[b]
+                gap = sched[b] - time.perf_counter()
+                if gap > 3e-4:  # sleep (releasing the GIL for the waiter thread), then spin the last 0.3 ms
+                    time.sleep(gap - 3e-4)
+                while time.perf_counter() < sched[b]:
+                    pass
+            else:
+                sched[b] = time.perf_counter()
+            k = int(ks[b])
+            cw, rw = widths(k, fp8)
+            tickets[b] = srv.submit(qbase + int(st) * d * 4, sz, k, float(eps[b]), ibase + int(st) * 100 * 4,
+                                    dbase + int(st) * 100 * 4, candidate_width=cw, rerank_width=rw)
+            ready.release()
+        th.join()
+        wall = max(done_t) - t_start
+        lat = np.array([done_t[b] - sched[b] for b in range(len(sizes))]) * 1e3
+        return wall, lat
+
+    def recall():
+        """recall@10 of the sampled queries by (k, epsilon) class: a hit if its FP64 dot is >= the 10th true dot."""
+        bidx = np.searchsorted(starts, sample, side="right") - 1
+        off = torch.as_tensor((starts[bidx] * 100 + (sample - starts[bidx]) * ks[bidx]).astype(np.int64), device="cuda")
+        ids = torch.stack([Oi[off + j] for j in range(10)], 1).long()
+        qs = Qd[torch.as_tensor(sample, device="cuda")].double()
+        dots = (Xd[ids.clamp(min=0)].double() * qs[:, None, :]).sum(-1)
+        hit = ((dots >= true_dot[:, 9:10] - 1e-9) & (ids >= 0)).double().mean(1).cpu().numpy()
+        res = {}
+        for k in (10, 100):
+            for e in (0.05, 0.2):
+                m = (ks[bidx] == k) & (eps[bidx] == e)
+                if m.any():
+                    res[f"k{k}_eps{e}"] = float(hit[m].mean())
+        res["all"] = float(hit.mean())
+        return res
+
+    for mode in ("fp32", "fp8"):
+        fp8 = mode == "fp8"
+        srv = L.Server(h, max_batch=4096, max_inflight=4, max_k=128, visited_bytes=8 << 30, quantized=fp8,
+                       target_batch=1024)
+        wall, lat = run(srv, fp8, None)  # saturated
+        qps_max = nq / wall
+        rec = recall()
+        log(f"[bench] serving {mode}: saturated {qps_max:.0f} QPS, recall@10 {rec}")
+        rate_q = args.sv_load * qps_max  # queries per second offered
+        gaps = rng.exponential(1.0, len(sizes)) * (np.array(sizes) / rate_q)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        wall_p, lat_p = run(srv, fp8, arrivals)
+        st = srv.stats()
+        srv.close()
+        w = np.array(sizes, np.float64)
+        order = np.argsort(lat_p)
+        cw_ = np.cumsum(w[order]) / w.sum()  # query-weighted latency percentiles
+        pq = lambda q: float(lat_p[order][np.searchsorted(cw_, q)])  # noqa: E731
+        out[mode] = {"saturated_qps": qps_max, "recall_at_10": rec,
+                     "poisson": {"offered_qps": rate_q, "achieved_qps": nq / wall_p,
+                                 "p50_ms": pq(0.5), "p99_ms": pq(0.99), "batch_p50_ms": float(np.percentile(lat_p, 50)),
+                                 "batch_p99_ms": float(np.percentile(lat_p, 99))},
+                     "launches": st["launches"], "mean_launch_queries": st["queries"] / max(1, st["launches"]),
+                     "gpu_ms": st["gpu_ms"]}
+        if fp8:
+            out[mode]["widths"] = "candidate/rerank 64 for k=10, 128 for k=100"
+        log(f"[bench] serving {mode}: Poisson {nq / wall_p:.0f} QPS (offered {rate_q:.0f}), p50 {pq(0.5):.2f} ms, "
+            f"p99 {pq(0.99):.2f} ms, {st['launches']} launches")
+    L.lib().rinnd_index_free(h)
+    del Xd, Qd, Oi, Od
     torch.cuda.empty_cache()
     return out
 

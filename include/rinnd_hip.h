@@ -257,6 +257,56 @@ rinnd_status rinnd_index_query_quantized(rinnd_index* index, const float* querie
 rinnd_status rinnd_index_query_noalloc(rinnd_index* index, const float* queries, int64_t nq, int64_t k,
                                        float epsilon, int32_t* out_idx, float* out_dist);
 
+/* ---- query service (asynchronous batch submission) ---------------------------------------------------
+ * The reference answers each `query` call on the caller's thread (core/index.rs:406-439, a Rayon task per query)
+ * and serialises `query_one` on a Mutex<SearchWorkspace> (py:571-586).  A server keeps one index resident and
+ * answers requests submitted from any number of host threads: a dispatcher thread gathers the pending requests
+ * into one launch (arrival batching; one launch mixes requests of different k and epsilon), a completion thread
+ * retires the launches in order.  Each request's results equal those of rinnd_index_query (or, for a quantized
+ * server, rinnd_index_query_quantized with the server's widths) on the same queries: query ids 0..nq-1 within the
+ * request seed its random fill.
+ * Admission by memory: a request is admitted into a launch when the launch has room for its queries (at most
+ * max_batch per launch; launch buffers are fixed, max_inflight launches at a time, each on its own stream) and its
+ * search width fits the beam memory (max_k result-heap entries per query); the concurrent searches share a fixed
+ * pool of visited sets (n bytes + an 8-byte entry per frontier vertex, at most 65,536 frontier entries per query)
+ * sized by visited_bytes and split evenly between the launch slots.  Requests that do not fit wait in submission
+ * order. */
+typedef struct rinnd_server rinnd_server;
+typedef struct rinnd_server_params {
+    int64_t max_batch;      /* queries per launch (and per request), e.g. 4096 */
+    int32_t max_inflight;   /* launches prepared or running at a time (each owns its staging buffers), e.g. 3 */
+    int64_t max_k;          /* largest search width a request may use (beam memory per query) */
+    int64_t visited_bytes;  /* memory of the visited-set pool; fixes the number of concurrent searches */
+    int32_t quantized;      /* 1: FP8 e4m3 candidate distances on MFMA, then FP32 re-ranking */
+    int64_t candidate_width;/* quantized: as rinnd_index_query_quantized's candidate_width */
+    int64_t rerank_width;   /* quantized: as rinnd_index_query_quantized's rerank_width */
+    int64_t target_batch;   /* a launch closes once it holds this many queries (0: min(1024, max_batch)) */
+} rinnd_server_params;
+typedef struct rinnd_server_stats {
+    int64_t requests, queries, launches, max_launch_queries;
+    double gpu_ms;          /* summed launch durations (HIP events on the server's stream) */
+} rinnd_server_stats;
+/* The index must outlive the server.  The server owns a HIP stream and two host threads. */
+rinnd_status rinnd_server_create(rinnd_index* index, const rinnd_server_params* params, rinnd_server** out);
+/* Queues nq queries (nq x dim, host or device memory) with k results each (out_idx / out_dist nq x k, host or
+ * device memory); the buffers must stay valid until the request completes.  Returns at once with a ticket. */
+rinnd_status rinnd_server_submit(rinnd_server* server, const float* queries, int64_t nq, int64_t k, float epsilon,
+                                 int32_t* out_idx, float* out_dist, uint64_t* ticket);
+/* The same with the quantized widths of this request, as query_quantized takes them per call (py:825-870); the
+ * server must be quantized.  candidate_width / rerank_width resolve as in rinnd_index_query_quantized. */
+rinnd_status rinnd_server_submit_quantized(rinnd_server* server, const float* queries, int64_t nq, int64_t k,
+                                           float epsilon, int64_t candidate_width, int64_t rerank_width,
+                                           int32_t* out_idx, float* out_dist, uint64_t* ticket);
+/* *done = 1 once the request has finished (its results are in the caller's buffers); the ticket is then released
+ * and the request's status returned. */
+rinnd_status rinnd_server_poll(rinnd_server* server, uint64_t ticket, int32_t* done);
+/* Blocks until the request has finished; releases the ticket and returns the request's status. */
+rinnd_status rinnd_server_wait(rinnd_server* server, uint64_t ticket);
+rinnd_status rinnd_server_get_stats(const rinnd_server* server, rinnd_server_stats* out);
+const char* rinnd_server_last_error(const rinnd_server* server);
+/* Finishes the queued requests, then stops the threads and frees the server. */
+void rinnd_server_destroy(rinnd_server* server);
+
 /* `neighbor_graph` getter (py:1087-1103): sorted, corrected n x k arrays in input order. */
 rinnd_status rinnd_index_neighbor_graph(const rinnd_index* index, int32_t* out_idx, float* out_dist);
 /* Search graph CSR sizes and `search_graph_original_order` (core/index.rs:137-141, py:1013-1025). */

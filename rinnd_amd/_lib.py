@@ -33,7 +33,8 @@ EXPORTS = [
     "rinnd_sort_rows", "rinnd_search_graph_build", "rinnd_index_build", "rinnd_index_free",
     "rinnd_index_query", "rinnd_index_query_noalloc", "rinnd_index_query_one", "rinnd_index_query_quantized", "rinnd_index_neighbor_graph",
     "rinnd_index_search_graph_info", "rinnd_index_search_graph", "rinnd_index_vertex_order",
-    "rinnd_index_trees",
+    "rinnd_index_trees", "rinnd_server_create", "rinnd_server_submit", "rinnd_server_submit_quantized", "rinnd_server_poll", "rinnd_server_wait",
+    "rinnd_server_get_stats", "rinnd_server_last_error", "rinnd_server_destroy",
 ]
 
 
@@ -91,6 +92,20 @@ class IndexParams(C.Structure):
     ]
 
 
+class ServerParams(C.Structure):
+    """rinnd_server_params (include/rinnd_hip.h)."""
+    _fields_ = [
+        ("max_batch", C.c_int64), ("max_inflight", C.c_int32), ("max_k", C.c_int64), ("visited_bytes", C.c_int64),
+        ("quantized", C.c_int32), ("candidate_width", C.c_int64), ("rerank_width", C.c_int64),
+        ("target_batch", C.c_int64),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_int64), ("queries", C.c_int64), ("launches", C.c_int64),
+                ("max_launch_queries", C.c_int64), ("gpu_ms", C.c_double)]
+
+
 _lib = None
 _lock = threading.Lock()
 _p = C.c_void_p
@@ -136,6 +151,15 @@ def lib():
                 "rinnd_index_search_graph": (C.c_int, [_p, _p, _p]),
                 "rinnd_index_vertex_order": (C.c_int, [_p, _p]),
                 "rinnd_index_trees": (_p, [_p]),
+                "rinnd_server_create": (C.c_int, [_p, C.POINTER(ServerParams), C.POINTER(_p)]),
+                "rinnd_server_submit": (C.c_int, [_p, _p, _i64, _i64, C.c_float, _p, _p, C.POINTER(C.c_uint64)]),
+                "rinnd_server_submit_quantized": (C.c_int, [_p, _p, _i64, _i64, C.c_float, _i64, _i64, _p, _p,
+                                                            C.POINTER(C.c_uint64)]),
+                "rinnd_server_poll": (C.c_int, [_p, C.c_uint64, C.POINTER(C.c_int32)]),
+                "rinnd_server_wait": (C.c_int, [_p, C.c_uint64]),
+                "rinnd_server_get_stats": (C.c_int, [_p, C.POINTER(ServerStats)]),
+                "rinnd_server_last_error": (C.c_char_p, [_p]),
+                "rinnd_server_destroy": (None, [_p]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(L, name)
@@ -242,3 +266,61 @@ class Forest:
         if st != OK:
             raise RinndError("rinnd_forest_tree_copy failed")
         return {"hyperplanes": hp, "offsets": off, "children": ch, "indices": idx, "n_nodes": nn}
+
+
+class Server:
+    """rinnd_server: asynchronous batched queries over a resident index (handle of rinnd_index_build)."""
+
+    def __init__(self, index_handle, max_batch=4096, max_inflight=3, max_k=128, visited_bytes=8 << 30,
+                 quantized=False, candidate_width=0, rerank_width=0, target_batch=0):
+        p = ServerParams(int(max_batch), int(max_inflight), int(max_k), int(visited_bytes), int(bool(quantized)),
+                         int(candidate_width), int(rerank_width), int(target_batch))
+        h = C.c_void_p()
+        st = lib().rinnd_server_create(index_handle, C.byref(p), C.byref(h))
+        if st != OK:
+            raise RinndError(f"rinnd_server_create failed ({STATUS.get(st, st)})")
+        self.h = h
+
+    def _check(self, st, what):
+        if st != OK:
+            msg = lib().rinnd_server_last_error(self.h)
+            msg = msg.decode() if msg else ""
+            if st == 1:
+                raise ValueError(f"{what}: {msg}")
+            raise RinndError(f"{what} failed ({STATUS.get(st, st)}): {msg}")
+
+    def submit(self, queries, nq, k, epsilon, out_idx, out_dist, candidate_width=None, rerank_width=None):
+        """Queues a request; queries / out_idx / out_dist are arrays, tensors or raw addresses (int) and must stay
+        valid until the request completes.  Widths (quantized servers) default to the server's."""
+        t = C.c_uint64()
+        addr = lambda a: C.c_void_p(a) if isinstance(a, int) else ptr(a)  # noqa: E731
+        if candidate_width is None and rerank_width is None:
+            st = lib().rinnd_server_submit(self.h, addr(queries), int(nq), int(k), C.c_float(epsilon), addr(out_idx),
+                                           addr(out_dist), C.byref(t))
+        else:
+            st = lib().rinnd_server_submit_quantized(self.h, addr(queries), int(nq), int(k), C.c_float(epsilon),
+                                                     int(candidate_width or 0), int(rerank_width or 0), addr(out_idx),
+                                                     addr(out_dist), C.byref(t))
+        self._check(st, "rinnd_server_submit")
+        return t.value
+
+    def poll(self, ticket):
+        d = C.c_int32()
+        self._check(lib().rinnd_server_poll(self.h, C.c_uint64(ticket), C.byref(d)), "rinnd_server_poll")
+        return bool(d.value)
+
+    def wait(self, ticket):
+        self._check(lib().rinnd_server_wait(self.h, C.c_uint64(ticket)), "rinnd_server_wait")
+
+    def stats(self):
+        s = ServerStats()
+        lib().rinnd_server_get_stats(self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.rinnd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -286,11 +286,11 @@ class RINND:
         return t["hyperplanes"], t["offsets"], t["children"], order[t["indices"]].astype(np.int32)
 
     def storage_info(self):
-        """py:1028-1077: the FP32 representation, or the FP8 e4m3 codes (rows padded to 16 bytes, one FP32
+        """py:1028-1077: the FP32 representation, or the FP8 e4m3 codes (rows padded to 32 bytes, one FP32
         scale per row) of a quantized index with its retained FP32 rows."""
         data_bytes = self.n_points * self.dim * 4
         if self._quantized is not None:
-            code_bytes = self.n_points * ((self.dim + 15) // 16 * 16)
+            code_bytes = self.n_points * ((self.dim + 31) // 32 * 32)
             meta = self.n_points * 4
             return {
                 "encoding": "fp8_e4m3", "code_bytes": code_bytes, "metadata_bytes": meta,

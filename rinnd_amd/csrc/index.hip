@@ -8,7 +8,7 @@
 #include <vector>
 
 #include "graph.h"
-#include "search.h"
+#include "index.h"
 
 namespace rinnd {
 // defined in capi.hip
@@ -21,39 +21,6 @@ bool device_pointer(const void* p);
 
 using namespace rinnd;
 
-struct rinnd_index {
-    rinnd_ctx* ctx = nullptr;
-    int64_t n = 0, dim = 0, k = 0, ld = 0;
-    int metric = 0, corr = 0;
-    DBuf<float> X;  // reordered, padded
-    DBuf<int32_t> indptr, indices;
-    int64_t nnz = 0;
-    float min_distance = 0.f;
-    rinnd_forest trees;  // retained search trees, ids in the reordered space
-    std::vector<DBuf<float>> hyper_padded;
-    DBuf<const float*> d_hyper, d_off;
-    DBuf<const int32_t*> d_ch, d_tix;
-    DBuf<int64_t> vertex_order, inv;
-    DBuf<int32_t> nb_idx;
-    DBuf<float> nb_dist;
-    int64_t leaf_budget = 1;
-    uint64_t seed = 42;
-    // query workspace (grown on demand)
-    int slots = 0;
-    DBuf<uint8_t> visited;
-    DBuf<int32_t> epochs;
-    DBuf<float> spill_d;
-    DBuf<int32_t> spill_i;
-    int64_t seed_cap = 0;
-    DBuf<int32_t> status;
-    DBuf<float> qbuf;  // padded query rows (grow-only; padding columns stay zero)
-    DBuf<int32_t> obuf_i;
-    DBuf<float> obuf_d;
-    // FP8 e4m3 codes of X for the quantized candidate search (built on the first quantized query)
-    DBuf<uint8_t> q8;
-    DBuf<float> q8s;
-    int64_t ld8 = 0;
-};
 
 namespace rinnd {
 
@@ -197,20 +164,21 @@ static const float* stage_queries(rinnd_index& ix, const float* queries, int64_t
     return ix.qbuf.p;
 }
 
-static void ensure_fp8_codes(rinnd_index& ix) {
+void ensure_fp8_codes(rinnd_index& ix) {
     if (ix.q8.p) return;
     hipStream_t s = ix.ctx->stream;
-    ix.ld8 = round_up(ix.dim, (int64_t)16);
+    ix.ld8 = round_up(ix.dim, (int64_t)32);
     ix.q8.alloc((size_t)std::max<int64_t>(ix.n, 1) * ix.ld8);
     ix.q8s.alloc((size_t)std::max<int64_t>(ix.n, 1));
-    launch_fp8_encode(s, ix.X.p, ix.ld, (int)ix.dim, ix.n, ix.q8.p, ix.ld8, ix.q8s.p);
+    ix.q8n.alloc((size_t)std::max<int64_t>(ix.n, 1));
+    launch_fp8_encode(s, ix.X.p, ix.ld, (int)ix.dim, ix.n, ix.q8.p, ix.ld8, ix.q8s.p, ix.q8n.p);
     HIP_TRY(hipGetLastError());
 }
 
 // search_k: candidate width of the search (k for a plain query); rerank > 0: exact re-ranking of that many
 // candidates; fp8: candidate distances from the FP8 codes.  k results are written per query.
 // The index's part of a launch of the greedy search: data, graph, trees and the query parameters.
-static SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq, int64_t k, float eps,
+SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq, int64_t k, float eps,
                                     int64_t qid_base, int64_t search_k, int64_t rerank, bool fp8, int32_t* oip,
                                     float* odp) {
     SearchArgs a{};
@@ -243,6 +211,7 @@ static SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int6
     a.q8 = fp8 ? ix.q8.p : nullptr;
     a.ld8 = ix.ld8;
     a.q8s = fp8 ? ix.q8s.p : nullptr;
+    a.q8n = fp8 ? ix.q8n.p : nullptr;
     a.k_out = (int)k;
     a.rerank = (int)rerank;
     return a;
@@ -257,7 +226,7 @@ static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_
     RINND_CHECK_ARG(search_k >= k && search_k <= search_max_k(), "the candidate width must be in [k, 512]");
     if (nq == 0 || k == 0) return;
     if (fp8) {
-        RINND_CHECK_ARG(round_up(ix.dim, (int64_t)16) <= 2048, "the FP8 candidate search supports dim <= 2048");
+        RINND_CHECK_ARG(round_up(ix.dim, (int64_t)32) <= 2048, "the FP8 candidate search supports dim <= 2048");
         ensure_fp8_codes(ix);
     }
     // workspace: one slot per concurrent query (a wave; up to 16 are resident per CU); memory per slot is n

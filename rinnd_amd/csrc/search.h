@@ -46,14 +46,26 @@ struct SearchArgs {
     const float* q8s;   // per-row dequantisation scale
     int k_out;
     int rerank;
+    const float* q8n;   // squared norms of the dequantized FP8 rows (FP8 search)
+    // Per-query parameters (null: the scalar fields above for every query).  k above is then the largest search
+    // width of the launch (it sizes the result heap in LDS).
+    const int32_t* q_k;       // search width
+    const int32_t* q_kout;    // results written
+    const int32_t* q_rerank;  // candidates re-ranked in FP32 (FP8 search)
+    const float* q_eps;
+    const int64_t* q_qid;     // query id of the RNG seed, FastRng::new(seed + id)
+    int32_t* const* q_oi;     // where the query's k_out results go
+    float* const* q_od;
     int qcap;  // floats of the query row kept in LDS (0: read from memory); set by launch_search
     int kcap;  // result-heap entries in LDS; set by launch_search
+    int c8cap; // bytes of the query's FP8 codes in LDS; set by launch_search
 };
 
 void launch_search(hipStream_t s, SearchArgs a, int slots);
-// per-row symmetric FP8 e4m3 codes: scale = max|x| / 240, code = e4m3(x / scale), rows of ld8 bytes
+// per-row symmetric FP8 e4m3 codes: scale = max|x| / 240, code = e4m3(x / scale), rows of ld8 bytes (a multiple
+// of 32: whole MFMA k-steps), and the squared norm of each dequantized row
 void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, uint8_t* codes, int64_t ld8,
-                       float* scales);
+                       float* scales, float* norms);
 int search_max_k();
 int search_max_degree();
 int search_max_budget();

@@ -177,30 +177,49 @@ __device__ __forceinline__ float distance_from_parts(int metric, float dot, floa
     }
 }
 
-// Distance from query q (FP32, zero past dim up to a multiple of 16) to an FP8 e4m3 row with scale sc.
-__device__ __forceinline__ float fp8_distance(int metric, const float* q, float qq, const uint8_t* __restrict__ c,
-                                              float sc, int dim) {
-    float d2 = 0.f, dot = 0.f, xx = 0.f;
-    for (int i = 0; i < dim; i += 16) {
-        const uint4 w = *reinterpret_cast<const uint4*>(c + i);
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float x[4];
-            x[0] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 0);
-            x[1] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 1);
-            x[2] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 2);
-            x[3] = __builtin_amdgcn_cvt_f32_fp8(ws[j], 3);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const float xv = sc * x[b], qv = q[i + 4 * j + b], df = qv - xv;
-                d2 = fmaf(df, df, d2);
-                dot = fmaf(qv, xv, dot);
-                xx = fmaf(xv, xv, xx);
-            }
+// FP8 candidate distances on the matrix cores.  Both the rows and the query are e4m3 codes with one scale each
+// (the query's codes sit in LDS); the dot products of a batch of up to 64 neighbours (lane t: neighbour x, valid
+// ok) are 16 x 16 x 32 FP8 MFMAs with FP32 accumulation: 16 neighbours' code rows form the A operand, the query's
+// codes the B operand (the same in all 16 columns), and lane t takes its neighbour's row of the result.  Every
+// lane of the wave must call it.  Returns the dot product of the codes (scale it by both rows' scales).
+typedef long fp8x8;
+__device__ __forceinline__ float fp8_wave_dot(const uint8_t* __restrict__ q8, int64_t ld8, const uint8_t* qc8,
+                                              int32_t x, bool ok, int lane) {
+    const unsigned long long live = __ballot(ok);
+    const int g = lane >> 4, m = lane & 15;
+    float mine = 0.f;
+    for (int tile = 0; tile < 4; ++tile) {
+        if (!((live >> (16 * tile)) & 0xffffull)) continue;  // wave-uniform
+        const int src = 16 * tile + m;
+        const int32_t xr = __shfl(x, src);
+        const bool okr = (live >> src) & 1ull;
+        const uint8_t* row = q8 + (int64_t)(okr ? xr : 0) * ld8 + 8 * g;
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        for (int64_t k0 = 0; k0 < ld8; k0 += 32) {
+            const fp8x8 av = okr ? *reinterpret_cast<const fp8x8*>(row + k0) : (fp8x8)0;
+            const fp8x8 bv = *reinterpret_cast<const fp8x8*>(qc8 + k0 + 8 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, bv, acc, 0, 0, 0);
         }
+        // acc[r] = dot(tile row 4 g + r, q) in every column: lane 16 tile + i takes row i from lane group i / 4
+        const int i = lane - 16 * tile;
+        const int from = ((i >> 2) & 3) * 16;
+        const float v0 = __shfl(acc[0], from), v1 = __shfl(acc[1], from), v2 = __shfl(acc[2], from),
+                    v3 = __shfl(acc[3], from);
+        if (i >= 0 && i < 16) mine = (i & 3) == 0 ? v0 : (i & 3) == 1 ? v1 : (i & 3) == 2 ? v2 : v3;
     }
-    return distance_from_parts(metric, dot, qq, xx, d2);
+    return mine;
+}
+// The same dot product by one lane (the random fill of the seeds).
+__device__ __forceinline__ float fp8_lane_dot(const uint8_t* __restrict__ c, const uint8_t* qc8, int64_t ld8) {
+    float dot = 0.f;
+    for (int64_t i = 0; i < ld8; i += 4) {
+        const uint32_t wx = *reinterpret_cast<const uint32_t*>(c + i), wq = *reinterpret_cast<const uint32_t*>(qc8 + i);
+        dot = fmaf(__builtin_amdgcn_cvt_f32_fp8(wx, 0), __builtin_amdgcn_cvt_f32_fp8(wq, 0), dot);
+        dot = fmaf(__builtin_amdgcn_cvt_f32_fp8(wx, 1), __builtin_amdgcn_cvt_f32_fp8(wq, 1), dot);
+        dot = fmaf(__builtin_amdgcn_cvt_f32_fp8(wx, 2), __builtin_amdgcn_cvt_f32_fp8(wq, 2), dot);
+        dot = fmaf(__builtin_amdgcn_cvt_f32_fp8(wx, 3), __builtin_amdgcn_cvt_f32_fp8(wq, 3), dot);
+    }
+    return dot;
 }
 
 // One instance per metric (METRIC >= 0; the switch folds away, so only that metric's accumulators take
@@ -213,6 +232,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     float* const sq = sdyn;                 // a.qcap floats (0: the query is read from memory)
     float* const rh_d = sdyn + a.qcap;      // result heap, a.kcap entries
     int32_t* const rh_i = (int32_t*)(rh_d + a.kcap);
+    uint8_t* const qc8 = (uint8_t*)(rh_i + a.kcap);  // FP8 search: the query's codes, a.c8cap bytes
     const int lane = threadIdx.x;
     const int slot = blockIdx.x;
     uint8_t* vis = a.visited + (int64_t)slot * a.n;
@@ -221,20 +241,53 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     const int dim = a.dim;
     const bool q_in_lds = a.qcap > 0;
     for (int64_t qi = blockIdx.x; qi < a.nq; qi += gridDim.x) {
-        const int64_t qid = a.qid_base + qi;
+        const int64_t qid = a.q_qid ? a.q_qid[qi] : a.qid_base + qi;
+        const float eps = a.q_eps ? a.q_eps[qi] : a.eps;
         const float* qg = a.Q + qi * a.ldq;
         if (q_in_lds)
-            for (int i = lane; i < (Q8 ? (int)a.ld8 : dim); i += 64) sq[i] = i < dim ? qg[i] : 0.f;
+            for (int i = lane; i < dim; i += 64) sq[i] = qg[i];
         const float* q = q_in_lds ? sq : qg;
-        float qq = 0.f;  // |q|^2 for the FP8 distances of the angular metrics
+        // FP8 search: the query's e4m3 codes (scale max|q| / 240) in LDS and the squared norm of its dequantized form
+        float qsc = 0.f, qq = 0.f;
         if (Q8) {
-            for (int i = lane; i < dim; i += 64) qq = fmaf(qg[i], qg[i], qq);
+            float mx = 0.f;
+            for (int i = lane; i < dim; i += 64) mx = fmaxf(mx, fabsf(qg[i]));
+            for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+            qsc = mx > 0.f ? mx / 240.0f : 1.0f;
+            const float inv = 1.0f / qsc;
+            for (int i4 = lane; i4 < (int)(a.ld8 >> 2); i4 += 64) {
+                float v[4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) v[b] = 4 * i4 + b < dim ? qg[4 * i4 + b] * inv : 0.f;
+                int word = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+                word = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], word, true);
+                reinterpret_cast<uint32_t*>(qc8)[i4] = (uint32_t)word;
+                const float f[4] = {qsc * __builtin_amdgcn_cvt_f32_fp8(word, 0), qsc * __builtin_amdgcn_cvt_f32_fp8(word, 1),
+                                    qsc * __builtin_amdgcn_cvt_f32_fp8(word, 2), qsc * __builtin_amdgcn_cvt_f32_fp8(word, 3)};
+#pragma unroll
+                for (int b = 0; b < 4; ++b) qq = fmaf(f[b], f[b], qq);
+            }
             for (int o = 32; o > 0; o >>= 1) qq += __shfl_xor(qq, o);
         }
-        // candidate distance: FP32 in the reference's arithmetic, or from the FP8 codes
+        __threadfence_block();
+        wave_sync();
+        auto fp8_dist = [&](int32_t x, float code_dot) -> float {
+            const float dot = code_dot * a.q8s[x] * qsc;
+            const float xx = a.q8n[x];
+            return distance_from_parts(metric, dot, qq, xx, fmaxf(qq + xx - 2.0f * dot, 0.0f));
+        };
+        // candidate distance of one lane: FP32 in the reference's arithmetic, or from the FP8 codes
         auto cand_dist = [&](int32_t x) -> float {
-            if (Q8) return fp8_distance(metric, q, qq, a.q8 + (int64_t)x * a.ld8, a.q8s[x], dim);
+            if (Q8) return fp8_dist(x, fp8_lane_dot(a.q8 + (int64_t)x * a.ld8, qc8, a.ld8));
             return distance_exact(metric, q, a.X + (int64_t)x * a.ld, dim);
+        };
+        // distances of a batch of neighbours (every lane calls it; lane t: neighbour x, valid ok)
+        auto batch_dist = [&](int32_t x, bool ok) -> float {
+            if (Q8) {
+                const float cd = fp8_wave_dot(a.q8, a.ld8, qc8, x, ok, lane);
+                return ok ? fp8_dist(x, cd) : 0.f;
+            }
+            return ok ? distance_exact(metric, q, a.X + (int64_t)x * a.ld, dim) : 0.f;
         };
         // epoch (VisitedSet::clear, visited.rs:51-57)
         int ep = a.epochs[slot] + 1;
@@ -251,7 +304,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         __threadfence_block();
         wave_sync();
         Xoshiro rng = xoshiro_seed(a.seed + (uint64_t)qid);  // FastRng::new(seed + query_id)
-        const int k = a.k;
+        const int k = a.q_k ? a.q_k[qi] : a.k;
 
         // seed heap storage (LDS, or the per-slot global region after an overflow)
         auto seed_push = [&](float d, int32_t x) {  // lane 0 only
@@ -347,8 +400,8 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                             ok = vis[x] != ep;
                             vis[x] = (uint8_t)ep;
                         }
-                        if (ok) d = cand_dist(x);
                     }
+                    d = batch_dist(x, ok);
                     S.nb_i[lane] = x;
                     S.nb_d[lane] = d;
                     S.nb_ok[lane] = ok;
@@ -379,7 +432,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                 }
             }
             const float maxd = S.rh_len > 0 ? rh_d[0] : __builtin_inff();
-            S.bound = maxd + a.eps * (maxd - a.min_distance);
+            S.bound = maxd + eps * (maxd - a.min_distance);
         }
         __threadfence_block();
         wave_sync();
@@ -414,8 +467,8 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                         ok = vis[x] != ep;
                         vis[x] = (uint8_t)ep;
                     }
-                    if (ok) d = cand_dist(x);
                 }
+                d = batch_dist(x, ok);
                 S.nb_i[lane] = x;
                 S.nb_d[lane] = d;
                 S.nb_ok[lane] = ok;
@@ -432,7 +485,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                             seed_push(dj, S.nb_i[j]);
                             if (ins) {
                                 const float maxd = rh_d[0];
-                                bound = maxd + a.eps * (maxd - a.min_distance);
+                                bound = maxd + eps * (maxd - a.min_distance);
                             }
                         }
                     }
@@ -459,10 +512,11 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         }
         wave_sync();
         int m = S.rh_len;
-        if (a.rerank > 0) {
+        const int rerank = a.q_rerank ? a.q_rerank[qi] : a.rerank;
+        if (rerank > 0) {
             // rerank_quantized_candidates (core/index.rs:103-124): the first `rerank` candidates get their
             // FP32 distance, then sort by (distance total order, internal id) and keep k
-            const int rw = min(a.rerank, m);
+            const int rw = min(rerank, m);
             for (int j = lane; j < rw; j += 64)
                 rh_d[j] = distance_exact(metric, q, a.X + (int64_t)rh_i[j] * a.ld, dim);
             __threadfence_block();
@@ -488,7 +542,9 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             wave_sync();
             m = rw;
         }
-        const int ko = a.k_out;
+        const int ko = a.q_kout ? a.q_kout[qi] : a.k_out;
+        int32_t* const oip = a.q_oi ? a.q_oi[qi] : a.out_idx + qi * ko;
+        float* const odp = a.q_od ? a.q_od[qi] : a.out_dist + qi * ko;
         for (int j = lane; j < ko; j += 64) {
             int32_t oi = -1;
             float od = __builtin_inff();
@@ -497,8 +553,8 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                 oi = x >= 0 ? (int32_t)a.vertex_order[x] : x;
                 od = corrected(a.corr, rh_d[j]);
             }
-            a.out_idx[qi * ko + j] = oi;
-            a.out_dist[qi * ko + j] = od;
+            oip[j] = oi;
+            odp[j] = od;
         }
         wave_sync();
     }
@@ -520,17 +576,18 @@ static void launch_search_q(hipStream_t s, const SearchArgs& a, int slots, size_
     }
 }
 void launch_search(hipStream_t s, SearchArgs a, int slots) {
-    const int qlen = a.q8 ? (int)a.ld8 : a.dim;  // the FP8 path reads the zero padding up to ld8
-    a.qcap = qlen <= kQLds ? (int)round_up(qlen, 4) : 0;
+    a.qcap = a.dim <= kQLds ? (int)round_up(a.dim, 4) : 0;
     a.kcap = (int)round_up(std::max(a.k, 1), 4);
-    const size_t lds = ((size_t)a.qcap + 2 * (size_t)a.kcap) * 4;
+    a.c8cap = a.q8 ? (int)round_up(a.ld8, (int64_t)16) : 0;
+    const size_t lds = ((size_t)a.qcap + 2 * (size_t)a.kcap) * 4 + (size_t)a.c8cap;
     if (a.q8) launch_search_q<true>(s, a, slots, lds);
     else launch_search_q<false>(s, a, slots, lds);
 }
 
 // One wave per row: max |x| by a wave reduction, then the codes, two values per conversion.
 __global__ __launch_bounds__(256) void k_fp8_encode(const float* __restrict__ X, int64_t ld, int dim, int64_t n,
-                                                    uint8_t* __restrict__ codes, int64_t ld8, float* __restrict__ scales) {
+                                                    uint8_t* __restrict__ codes, int64_t ld8, float* __restrict__ scales,
+                                                    float* __restrict__ norms) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
         const float* x = X + r * ld;
@@ -541,6 +598,7 @@ __global__ __launch_bounds__(256) void k_fp8_encode(const float* __restrict__ X,
         const float inv = 1.0f / sc;
         if (lane == 0) scales[r] = sc;
         uint32_t* out = reinterpret_cast<uint32_t*>(codes + r * ld8);
+        float xx = 0.f;
         for (int i4 = lane; i4 < (int)(ld8 >> 2); i4 += 64) {
             float v[4];
 #pragma unroll
@@ -551,14 +609,20 @@ __global__ __launch_bounds__(256) void k_fp8_encode(const float* __restrict__ X,
             int word = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
             word = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], word, true);
             out[i4] = (uint32_t)word;
+            const float f[4] = {sc * __builtin_amdgcn_cvt_f32_fp8(word, 0), sc * __builtin_amdgcn_cvt_f32_fp8(word, 1),
+                                sc * __builtin_amdgcn_cvt_f32_fp8(word, 2), sc * __builtin_amdgcn_cvt_f32_fp8(word, 3)};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) xx = fmaf(f[b], f[b], xx);
         }
+        for (int o = 32; o > 0; o >>= 1) xx += __shfl_xor(xx, o);
+        if (lane == 0) norms[r] = xx;
     }
 }
 void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, uint8_t* codes, int64_t ld8,
-                       float* scales) {
+                       float* scales, float* norms) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_fp8_encode, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)4), 1 << 16)), dim3(256), 0,
-                       s, X, ld, dim, n, codes, ld8, scales);
+                       s, X, ld, dim, n, codes, ld8, scales, norms);
 }
 
 int search_max_k() { return kMaxK; }

@@ -5,8 +5,8 @@ cfg2  1,000,000 x 128: 1,000-centre Gaussian mixture, centres U[0,100)^128, sigm
       away from zero and clipped to 0..255, stored as float32              (seed 2)
 cfg3  1,000,000 x 960: 100-centre mixture, centres U[0,1]^960, sigma 0.1, clipped to [0,1]  (seed 4)
 cfg4  512 indexes, n log-uniform in [2,000, 100,000], dim 64, N(0,1) rows L2-normalised  (seed 5)
-cfg5  2,000,000 x 96 unit-norm rows, 500-centre mixture with sigma 0.2 before normalisation;
-      queries drawn from the same mixture                                   (seed 6)
+cfg5  2,000,000 x 96 unit-norm rows: 500 centres N(0, I), x = centre + 0.2 N(0, I) (sigma relative to the
+      centres' unit per-coordinate spread), then normalised; queries from the same mixture  (seed 6)
 """
 from __future__ import annotations
 
@@ -62,7 +62,6 @@ def cfg4_index(n, d=64, seed=5, i=0):
 def _mixture_unit(n, d, centers, sigma, rng, c=None, chunk=250_000):
     if c is None:
         c = rng.standard_normal((centers, d)).astype(np.float32)
-        c /= np.linalg.norm(c, axis=1, keepdims=True)
     labels = rng.integers(0, centers, n)
     out = np.empty((n, d), np.float32)
     for s in range(0, n, chunk):

@@ -72,19 +72,25 @@ def test_recall_close_to_the_fp32_search(index):
 
 
 def test_approximate_distances_within_fp8_rounding(index):
-    """Without re-ranking the distances come from the codes: per coordinate the e4m3 rounding error is at most
-    2^-4 of the value (3 mantissa bits, round to nearest) for values in the normal range, so
-    |d~ - d| <= 2 sqrt(d) |x - x~| + |x - x~|^2 with |x - x~| <= |x| / 16, plus subnormal steps."""
+    """Without re-ranking the distances come from e4m3 codes of both the row and the query (one scale each): per
+    coordinate the rounding error is at most 2^-4 of the value (3 mantissa bits, round to nearest) in the normal
+    range, plus a subnormal step of 2^-9 of the scale below it.  With e = |x - x~| + |q - q~| (triangle inequality),
+    |d~ - d| <= 2 sqrt(d) e + e^2, plus the FP32 rounding of the parts (|q~|^2 + |x~|^2 - 2 q~.x~)."""
     X, Q, g = index
     k = 10
     st, oi, od = quantized(g, Q, k, 0.1, 24, 0)
     assert st == 0
+
+    def err(v):
+        scale = np.abs(v).max(-1, keepdims=True) / 240.0
+        return np.sqrt(((np.abs(v) / 16.0 + scale * 2.0 ** -9) ** 2).sum(-1))
     for q in range(0, len(Q), 11):
-        xs = X[oi[q]]
-        exact = ((Q[q][None, :].astype(np.float64) - xs) ** 2).sum(-1)
-        scale = np.abs(xs).max(1, keepdims=True) / 240.0
-        err = np.sqrt(((np.abs(xs) / 16.0 + scale * 2.0 ** -9) ** 2).sum(-1))
-        bound = 2 * np.sqrt(exact) * err + err ** 2 + 1e-5 * (1 + exact)
+        xs = X[oi[q]].astype(np.float64)
+        qv = Q[q].astype(np.float64)
+        exact = ((qv[None, :] - xs) ** 2).sum(-1)
+        e = err(xs) + err(qv[None, :])
+        parts = (qv * qv).sum() + (xs * xs).sum(-1)
+        bound = 2 * np.sqrt(exact) * e + e ** 2 + 1e-6 * parts + 1e-6
         assert np.all(np.abs(od[q] - exact) <= bound), (od[q], exact, bound)
 
 
