@@ -162,6 +162,41 @@ rinnd_status rinnd_nn_descent(rinnd_ctx* ctx, const float* data, int64_t n, int6
                               int32_t* out_idx, float* out_dist, uint8_t* out_flags,
                               rinnd_forest** out_forest, rinnd_build_stats* stats);
 
+/* Counters and phase timings of a batched build (HIP events on the context stream). */
+typedef struct rinnd_batch_stats {
+    int32_t passes;                 /* passes over the batch: the largest per-index iteration count */
+    int32_t graph_replays;          /* passes whose candidate phase and local join replayed the captured graph */
+    int32_t graph_memsets_replaced; /* memset nodes of the captured graph replaced by fill kernels */
+    int32_t join_threads_per_row;   /* FP32 join tile shape: threads per row ... */
+    int32_t join_rows_per_group;    /* ... and rows per workgroup (tuned on the first pass) */
+    int64_t changes[64];            /* heap changes per pass, all indexes */
+    int64_t join_pairs[64];         /* candidate pairs evaluated per pass, all indexes */
+    double ms_forest;               /* every index's RP forest (one work-queue launch) */
+    double ms_init;                 /* leaf initialisation */
+    double ms_candidates_join;      /* candidate phase + local join, all passes */
+    double ms_apply;                /* ordered heap updates + convergence tests, all passes */
+    double ms_total;                /* whole call */
+} rinnd_batch_stats;
+
+/* rinnd_nn_descent_batch flags */
+#define RINND_BATCH_NO_GRAPH 1 /* run every pass eagerly instead of replaying the captured graph */
+
+/* Many independent builds in one call.  The reference builds one index per call (core/index.rs:601-733 calls
+ * nn_descent, core/nndescent/mod.rs:79-197, once per index); this runs n_indexes of those builds together.
+ * Index s owns rows offsets[s] .. offsets[s+1] of `data` (n_total x dim, n_total = offsets[n_indexes],
+ * offsets[0] = 0, non-decreasing) and rows of the same numbers of the outputs (n_total x k, ids in the
+ * index's own numbering 0 .. n_s - 1).  Per index: n_trees[s] and n_iters[s] (NULL: params->n_trees /
+ * params->n_iters for all), its FastRng state rng_states[4s .. 4s+3] (consumed and advanced), and in
+ * out_iterations / out_converged (may be NULL) its iteration count and whether the delta test stopped it.
+ * Shared: k, leaf_size, max_candidates, delta, angular, max_depth.  Each index's outputs equal those of
+ * rinnd_nn_descent on its rows alone with rng_mode RINND_RNG_REFERENCE and n_threads 1, which the batch
+ * requires (and the FP32 join); other settings return RINND_ERR_UNSUPPORTED. */
+rinnd_status rinnd_nn_descent_batch(rinnd_ctx* ctx, const float* data, const int64_t* offsets, int64_t n_indexes,
+                                    int64_t dim, rinnd_distance dist, const rinnd_params* params,
+                                    const int64_t* n_trees, const int64_t* n_iters, uint64_t* rng_states,
+                                    int32_t* out_idx, float* out_dist, uint8_t* out_flags, int32_t* out_iterations,
+                                    int32_t* out_converged, int32_t flags, rinnd_batch_stats* stats);
+
 /* RP forest built alone: `build_rp_forest` (core/tree/builder.rs:89-118). */
 rinnd_status rinnd_build_forest(rinnd_ctx* ctx, const float* data, int64_t n, int64_t dim,
                                 int64_t n_trees, int64_t leaf_size, int32_t angular,

@@ -28,7 +28,7 @@ JOIN = {"fp32": 0, "bf16": 1}
 # every symbol include/rinnd_hip.h declares
 EXPORTS = [
     "rinnd_ctx_create", "rinnd_ctx_destroy", "rinnd_last_error", "rinnd_version", "rinnd_rng_seed",
-    "rinnd_rng_jump", "rinnd_pair_distances", "rinnd_nn_descent", "rinnd_build_forest",
+    "rinnd_rng_jump", "rinnd_pair_distances", "rinnd_nn_descent", "rinnd_nn_descent_batch", "rinnd_build_forest",
     "rinnd_forest_n_trees", "rinnd_forest_tree_info", "rinnd_forest_tree_copy", "rinnd_forest_free",
     "rinnd_sort_rows", "rinnd_search_graph_build", "rinnd_index_build", "rinnd_index_free",
     "rinnd_index_query", "rinnd_index_query_noalloc", "rinnd_index_query_one", "rinnd_index_query_quantized", "rinnd_index_neighbor_graph",
@@ -54,6 +54,27 @@ class KernelTiming(C.Structure):
 
 # rinnd_timed_kernel order, with the kernel each entry times
 TIMED_KERNELS = ("join", "margins", "leaf_dist", "leaf_push")
+
+
+class BatchStats(C.Structure):
+    """rinnd_batch_stats."""
+    _fields_ = [
+        ("passes", C.c_int32), ("graph_replays", C.c_int32), ("graph_memsets_replaced", C.c_int32),
+        ("join_threads_per_row", C.c_int32), ("join_rows_per_group", C.c_int32),
+        ("changes", C.c_int64 * 64), ("join_pairs", C.c_int64 * 64),
+        ("ms_forest", C.c_double), ("ms_init", C.c_double), ("ms_candidates_join", C.c_double),
+        ("ms_apply", C.c_double), ("ms_total", C.c_double),
+    ]
+
+    def to_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("changes", "join_pairs")}
+        p = self.passes
+        d["changes"] = list(self.changes[:min(p, 64)])
+        d["join_pairs"] = list(self.join_pairs[:min(p, 64)])
+        return d
+
+
+BATCH_NO_GRAPH = 1
 
 
 class BuildStats(C.Structure):
@@ -130,6 +151,8 @@ def lib():
                 "rinnd_pair_distances": (C.c_int, [_p, C.c_int, _p, _i64, _i64, _p, _p, _i64, _p]),
                 "rinnd_nn_descent": (C.c_int, [_p, _p, _i64, _i64, C.c_int, C.POINTER(Params), _p, _p, _p,
                                                _p, C.POINTER(_p), C.POINTER(BuildStats)]),
+                "rinnd_nn_descent_batch": (C.c_int, [_p, _p, _p, _i64, _i64, C.c_int, C.POINTER(Params), _p, _p,
+                                                     _p, _p, _p, _p, _p, _p, C.c_int32, C.POINTER(BatchStats)]),
                 "rinnd_build_forest": (C.c_int, [_p, _p, _i64, _i64, _i64, _i64, C.c_int32, _i64, C.c_int32,
                                                  _p, C.POINTER(_p)]),
                 "rinnd_forest_n_trees": (_i64, [_p]),
@@ -233,6 +256,33 @@ def make_params(n_neighbors, n_trees, leaf_size, max_candidates, n_iters, delta=
     return Params(int(n_neighbors), int(n_trees), int(leaf_size), int(max_candidates), int(n_iters),
                   float(delta), int(bool(angular)), int(max_depth), int(bool(verbose)), int(n_threads),
                   RNG[rng_mode], JOIN[join_precision])
+
+
+def nn_descent_batch(ctx, data, offsets, dist, params, seeds, n_trees=None, n_iters=None, graph=True):
+    """rinnd_nn_descent_batch over host arrays: data (n_total x dim float32), offsets (n_indexes + 1), one FastRng
+    seed per index.  Returns (indices, distances, flags, iterations, converged, rng_states, stats dict); the heap
+    rows of index s are rows offsets[s]..offsets[s+1], ids in its own numbering."""
+    X = np.ascontiguousarray(data, np.float32)
+    off = np.ascontiguousarray(offsets, np.int64)
+    m = len(off) - 1
+    n, d = X.shape
+    k = int(params.n_neighbors)
+    states = np.stack([rng_seed(int(sd)) for sd in seeds]) if m else np.zeros((0, 4), np.uint64)
+    states = np.ascontiguousarray(states, np.uint64)
+    nt = None if n_trees is None else np.ascontiguousarray(n_trees, np.int64)
+    ni = None if n_iters is None else np.ascontiguousarray(n_iters, np.int64)
+    idx = np.zeros((n, k), np.int32)
+    dst = np.zeros((n, k), np.float32)
+    flg = np.zeros((n, k), np.uint8)
+    its = np.zeros(m, np.int32)
+    conv = np.zeros(m, np.int32)
+    st = BatchStats()
+    rc = lib().rinnd_nn_descent_batch(ctx.h, ptr(X), ptr(off), m, d, DIST[dist] if isinstance(dist, str) else dist,
+                                      C.byref(params), ptr(nt) if nt is not None else None,
+                                      ptr(ni) if ni is not None else None, ptr(states), ptr(idx), ptr(dst), ptr(flg),
+                                      ptr(its), ptr(conv), 0 if graph else BATCH_NO_GRAPH, C.byref(st))
+    ctx.check(rc, "rinnd_nn_descent_batch")
+    return idx, dst, flg, its, conv, states, st.to_dict()
 
 
 class Forest:

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librinnd_hip.so")
-SOURCES = ["kernels.hip", "forest_level.hip", "leaf_init.hip", "join_bf16.hip", "search.hip", "graph.hip", "capi.hip", "index.hip", "serve.hip"]
+SOURCES = ["kernels.hip", "forest_level.hip", "leaf_init.hip", "join_bf16.hip", "search.hip", "graph.hip", "capi.hip", "index.hip", "serve.hip", "batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RINND_ARCH", "gfx950")

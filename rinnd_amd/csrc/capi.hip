@@ -231,6 +231,21 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
 
 // ---- NN-Descent ---------------------------------------------------------------------------------
 
+// The FP32 join with the context's tile shape for (metric, row stride, C), tuned on the first call.
+void join_fp32_cached(rinnd_ctx* ctx, hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t n, const float* heap_dist, int k,
+                      const JoinOut& out) {
+    const uint64_t key = ((uint64_t)metric << 48) ^ ((uint64_t)ld << 16) ^ (uint64_t)C;
+    const auto it = ctx->join_shapes.find(key);
+    if (it != ctx->join_shapes.end()) {
+        launch_join_fp32(s, metric, X, ld, dim, new_idx, old_idx, C, 0, n, heap_dist, k, out, it->second);
+        return;
+    }
+    bool tuned = false;
+    const int shape = launch_join_fp32_tuned(s, metric, X, ld, dim, new_idx, old_idx, C, n, heap_dist, k, out, &tuned);
+    if (tuned) ctx->join_shapes[key] = shape;
+}
+
 static uint64_t d2h_u64(hipStream_t s, const void* p) {
     uint64_t v = 0;
     HIP_TRY(hipMemcpyAsync(&v, p, 8, hipMemcpyDeviceToHost, s));
@@ -507,7 +522,7 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         if (use_bf16)
             launch_join_bf16(s, DXw, Dld, dim, B16, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo, rescored.p);
         else
-            launch_join_fp32(s, metric, DXw, Dld, dim, new_idx.p, old_idx.p, C, 0, n, w_dist, k, jo);
+            join_fp32_cached(ctx, s, metric, DXw, Dld, dim, new_idx.p, old_idx.p, C, n, w_dist, k, jo);
         HIP_TRY(hipEventRecord(ev[5], s));
         scan_u64_exclusive(s, krec.p, koff.p, n + 1, PL.scan_tmp.p, PL.scan_bytes);
         // the record count sizes the sort
@@ -836,6 +851,83 @@ rinnd_status rinnd_nn_descent(rinnd_ctx* ctx, const float* data, int64_t n, int6
         HIP_TRY(hipStreamSynchronize(s));
         for (int w = 0; w < 4; ++w) rng_state[w] = main.s[w];
         if (out_forest) *out_forest = F.release();
+    });
+}
+
+rinnd_status rinnd_nn_descent_batch(rinnd_ctx* ctx, const float* data, const int64_t* offsets, int64_t n_indexes,
+                                    int64_t dim, rinnd_distance dist, const rinnd_params* params,
+                                    const int64_t* n_trees, const int64_t* n_iters, uint64_t* rng_states,
+                                    int32_t* out_idx, float* out_dist, uint8_t* out_flags, int32_t* out_iterations,
+                                    int32_t* out_converged, int32_t flags, rinnd_batch_stats* stats) {
+    return guarded(ctx, [&] {
+        const double t0 = wall_ms();
+        RINND_CHECK_ARG(n_indexes >= 0 && n_indexes <= 65535, "n_indexes must be in [0, 65535]");
+        RINND_CHECK_ARG(n_indexes == 0 || (offsets && rng_states), "offsets or rng_states is NULL");
+        RINND_CHECK_ARG((int)dist >= 0 && (int)dist <= 6, "unknown distance");
+        std::vector<int64_t> off(n_indexes + 1, 0), nt(n_indexes), ni(n_indexes);
+        for (int64_t i = 0; i <= n_indexes; ++i) off[i] = n_indexes ? offsets[i] : 0;
+        RINND_CHECK_ARG(off[0] == 0, "offsets[0] must be 0");
+        for (int64_t i = 0; i < n_indexes; ++i) {
+            RINND_CHECK_ARG(off[i + 1] >= off[i], "offsets must be non-decreasing");
+            nt[i] = n_trees ? n_trees[i] : params ? params->n_trees : 0;
+            ni[i] = n_iters ? n_iters[i] : params ? params->n_iters : 0;
+            RINND_CHECK_ARG(nt[i] >= 0 && nt[i] <= 256, "n_trees must be in [0, 256]");
+            RINND_CHECK_ARG(ni[i] >= 0, "n_iters must be non-negative");
+        }
+        const int64_t N = off[n_indexes];
+        validate_params(params, N, dim);
+        if (params->rng_mode != RINND_RNG_REFERENCE || params->n_threads > 1)
+            throw Error{RINND_ERR_UNSUPPORTED, "the batched build reproduces reference draws with one candidate thread"};
+        if (params->join_precision != RINND_JOIN_FP32)
+            throw Error{RINND_ERR_UNSUPPORTED, "the batched build runs the FP32 join"};
+        if (!leaf_init_supported((int)params->n_neighbors, params->leaf_size))
+            throw Error{RINND_ERR_UNSUPPORTED, "the batched build needs the fused leaf initialisation (k, leaf_size)"};
+        if (stats) *stats = rinnd_batch_stats{};
+        if (N == 0) {
+            for (int64_t i = 0; i < n_indexes; ++i) {
+                if (out_iterations) out_iterations[i] = 0;
+                if (out_converged) out_converged[i] = 0;
+            }
+            return;
+        }
+        hipStream_t s = ctx->stream;
+        std::vector<Xoshiro> mains(n_indexes);
+        for (int64_t i = 0; i < n_indexes; ++i)
+            for (int w = 0; w < 4; ++w) mains[i].s[w] = rng_states[4 * i + w];
+        DataDev D;
+        make_data(s, data, N, (int)dim, D);
+        const int64_t k = params->n_neighbors;
+        const size_t cnt = (size_t)N * k;
+        const bool direct = is_device_ptr(out_idx) && is_device_ptr(out_dist) && is_device_ptr(out_flags);
+        Workspace& W = *ctx->ws;
+        WBuf<int32_t> hi;
+        WBuf<float> hd;
+        WBuf<uint8_t> hf;
+        if (!direct) {
+            hi = WBuf<int32_t>(W, kWsHeap + 0, cnt);
+            hd = WBuf<float>(W, kWsHeap + 1, cnt);
+            hf = WBuf<uint8_t>(W, kWsHeap + 2, cnt);
+        }
+        int32_t* h_idx = direct ? out_idx : hi.p;
+        float* h_dist = direct ? out_dist : hd.p;
+        uint8_t* h_flags = direct ? out_flags : hf.p;
+        std::vector<int32_t> its, conv;
+        rinnd_batch_stats st{};
+        nn_descent_batch_dev(ctx, D.X.p, D.ld, off, (int)dim, (int)dist, *params, nt, ni, mains, h_idx, h_dist,
+                             h_flags, (flags & RINND_BATCH_NO_GRAPH) == 0, its, conv, st);
+        if (!direct) {
+            copy_out(s, out_idx, h_idx, cnt);
+            copy_out(s, out_dist, h_dist, cnt);
+            copy_out(s, out_flags, h_flags, cnt);
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n_indexes; ++i) {
+            for (int w = 0; w < 4; ++w) rng_states[4 * i + w] = mains[i].s[w];
+            if (out_iterations) out_iterations[i] = its[i];
+            if (out_converged) out_converged[i] = conv[i];
+        }
+        st.ms_total = wall_ms() - t0;
+        if (stats) *stats = st;
     });
 }
 

@@ -34,6 +34,13 @@ struct Error {
 // ---- small helpers -----------------------------------------------------------------------------
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 __host__ __device__ inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+// workgroups for `work` items of `block` each: at least 1, at most cap (the kernels loop over the rest)
+inline unsigned grid_for(int64_t work, int block, int64_t cap = 1 << 20) {
+    int64_t g = ceil_div(work, block);
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
 
 // Wave-local LDS ordering: wait for this wave's LDS traffic, then re-converge the wave.
 __device__ inline void wave_sync() {

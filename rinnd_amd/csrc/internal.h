@@ -4,6 +4,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "common.h"
@@ -125,6 +126,7 @@ struct rinnd_ctx {
     hipEvent_t ev[12] = {};
     std::vector<hipEvent_t> ev_pool;    // events of deferred launch timings (released by rinnd_ctx_destroy)
     rinnd::Workspace* ws = nullptr;     // build scratch kept between calls (released by rinnd_ctx_destroy)
+    std::map<uint64_t, int> join_shapes;  // FP32 join tile shape picked by the tuner, per (metric, row stride, C)
 };
 
 namespace rinnd {
@@ -199,6 +201,8 @@ struct KTimer {
 void launch_pad_rows(hipStream_t s, const float* src, int64_t n, int dim, float* dst, int64_t ld);
 void launch_fill_heap(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t count);
 void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t count);
+// zero `bytes` bytes with a kernel (not a memset node, so the call can sit inside a captured graph)
+void launch_zero(hipStream_t s, void* p, size_t bytes);
 void launch_pair_distances(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                            const int32_t* a, const int32_t* b, int64_t n_pairs, float* out);
 
@@ -225,6 +229,16 @@ struct ForestDFSArgs {
     int32_t* rankR;
     int32_t* PQ;
     int32_t* tmp;
+    // Many independent point sets (queue non-null): tree t covers rows tree_base[t] .. tree_base[t] + tree_n[t]
+    // of X (its indices are local to that range), and `grid` persistent workgroups pull the trees from a work
+    // queue (*queue, zeroed) in the order order[] (largest first).  Workgroup g uses the scratch slot starting at
+    // element g * slot_stride of side / rankR / tmp (2 x that of PQ); slot_stride > the largest tree_n.
+    const int64_t* tree_base;
+    const int64_t* tree_n;
+    const int32_t* order;
+    int32_t* queue;
+    int64_t slot_stride;
+    int grid;
 };
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a);
 
@@ -241,6 +255,19 @@ void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, i
                       int32_t* leaf_mark /* n+1 scratch */, int64_t* leaf_rank /* n+1 */,
                       int32_t* leaf_end_at /* n */, int64_t* leaves /* 2 per leaf */,
                       void* scan_tmp, size_t scan_tmp_bytes);
+
+// The same for a virtual tree of a batch: one entry per index that has tree number t, its positions placed
+// at pbase .. pbase + n of a combined position space of n positions (vidx: the combined tree indices, ids
+// offset by the index's first row).  max_span bounds every entry's max(n_nodes, n).
+struct VTreeEnt {
+    const int32_t* children;
+    int64_t n_nodes;
+    const int32_t* indices;
+    int64_t pbase, base, n;
+};
+void launch_leaf_list_batch(hipStream_t s, const VTreeEnt* ents_dev, int n_ents, int64_t max_span, int64_t n,
+                            int32_t* leaf_mark, int64_t* leaf_rank, int32_t* leaf_end_at, int64_t* leaves,
+                            int32_t* vidx, void* scan_tmp, size_t scan_tmp_bytes);
 
 // initialize_from_leaves fused per leaf (leaf_init.hip); n_leaves is read on the device,
 // max_leaves bounds the grid.  Supported when a 64..256-thread block's heaps fit in LDS.
@@ -280,7 +307,8 @@ void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* 
                        unsigned long long* changes, int32_t* big_list, int32_t* big_count, uint32_t* bins);
 
 // candidates
-void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt);
+void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt,
+                             const uint8_t* row_on = nullptr);
 struct RngStream {
     uint64_t s[4];
     int64_t count;
@@ -318,6 +346,7 @@ struct CandBuildArgs {
     // zero); counts[0] += valid new+old candidates, counts[1] += pairs
     unsigned long long* row_pairs;
     unsigned long long* counts;
+    const uint8_t* row_on;  // optional: rows whose byte is 0 take no part (finished indexes of a batch)
 };
 size_t cand_sort_tmp_bytes(int64_t items, int end_bit);
 void launch_cand_build(hipStream_t s, const CandBuildArgs& a);
@@ -342,9 +371,26 @@ struct JoinOut {
     unsigned long long* kept;  // kept pairs in all, spread over kKeptSlots counters (summed by the caller)
 };
 constexpr int kKeptSlots = 256;
+// shape: a tile shape (threads per row, rows per workgroup) of join_fp32_shape_info; 0 is the default
 void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
                       const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out);
+                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out, int shape = 0);
+int join_fp32_shape_count();
+void join_fp32_shape_info(int shape, int* tpr, int* rpg);
+// rows [0, n): times the shapes on slices of the rows, joins the rest with the fastest and returns it
+// Many independent builds in one pass over the device (batch.hip): index s owns rows off[s] .. off[s+1] of DX
+// and the heap rows of the same numbers; its heap ids come back in its own numbering.
+void nn_descent_batch_dev(rinnd_ctx* ctx, const float* DX, int64_t ld, const std::vector<int64_t>& off, int dim,
+                          int metric, const rinnd_params& p, const std::vector<int64_t>& n_trees,
+                          const std::vector<int64_t>& n_iters, std::vector<Xoshiro>& mains, int32_t* idx,
+                          float* dist, uint8_t* flags, bool use_graph, std::vector<int32_t>& iterations,
+                          std::vector<int32_t>& converged, rinnd_batch_stats& st);
+void join_fp32_cached(rinnd_ctx* ctx, hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t n, const float* heap_dist, int k,
+                      const JoinOut& out);
+int launch_join_fp32_tuned(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* new_idx,
+                           const int32_t* old_idx, int C, int64_t n, const float* heap_dist, int k, const JoinOut& out,
+                           bool* tuned);
 // Kept pairs -> destination-grouped (id, distance) records for the ordered heap update.  Each row's
 // kept pairs are laid out in loop order (slot t ascending) as two records each - (p; q, d) and
 // (q; p, d) - at 2 * koff[r] (koff: exclusive prefix of kc), then one stable radix sort by destination
@@ -372,6 +418,7 @@ struct JoinApplyArgs {
     uint8_t* flags;
     int k;
     unsigned long long* changes;
+    uint32_t* row_changes;  // optional: accepted pushes per destination (rows without records keep their value)
 };
 size_t join_sort_tmp_bytes(int64_t items, int end_bit);
 // records of the kept pairs, sorted by destination (off: each destination's segment, found in the sorted

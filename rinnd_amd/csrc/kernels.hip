@@ -23,12 +23,7 @@
 
 namespace rinnd {
 
-static inline unsigned grid_for(int64_t work, int block, int64_t cap = 1 << 20) {
-    int64_t g = ceil_div(work, block);
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (unsigned)g;
-}
+
 
 // ---- fills / copies ------------------------------------------------------------------------------
 __global__ void k_pad_rows(const float* __restrict__ src, int64_t n, int dim,
@@ -68,6 +63,18 @@ void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t count) {
     if (count == 0) return;
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256)), dim3(256), 0, s, p, v, count);
 }
+__global__ void k_zero_u8(uint8_t* p, int64_t count) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = 0;
+}
+void launch_zero(hipStream_t s, void* p, size_t bytes) {
+    if (bytes == 0) return;
+    if (((uintptr_t)p & 3) == 0 && (bytes & 3) == 0)
+        launch_fill_i32(s, (int32_t*)p, 0, (int64_t)(bytes >> 2));
+    else
+        hipLaunchKernelGGL(k_zero_u8, dim3(grid_for((int64_t)bytes, 256)), dim3(256), 0, s, (uint8_t*)p,
+                           (int64_t)bytes);
+}
 
 __global__ void k_pair_distances(int metric, const float* __restrict__ X, int64_t ld, int dim,
                                  const int32_t* __restrict__ a, const int32_t* __restrict__ b,
@@ -99,7 +106,7 @@ struct StackEnt {
     int32_t start, end, depth, pslot;
 };
 
-__global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) {
+__device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64_t so) {
     extern __shared__ float fsm[];  // hp[dim], p1[dim], p2[dim]
     __shared__ StackEnt stack[kForestStack];
     __shared__ uint64_t rs[4];
@@ -109,10 +116,10 @@ __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) 
     __shared__ int wave_tot[kForestThreads / RINND_WAVE];
     __shared__ int sh_running;
 
-    const int t = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
-    const int64_t n = a.n;
+    const int64_t n = a.tree_n ? a.tree_n[t] : a.n;
+    const float* const X = a.X + (a.tree_base ? a.tree_base[t] : 0) * a.ld;  // this tree's point set
     const int dim = a.dim;
     float* hp = fsm;
     float* p1s = fsm + dim;
@@ -121,11 +128,11 @@ __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) 
     float* offs = a.offsets[t];
     int32_t* ch = a.children[t];
     int32_t* idx = a.indices[t];
-    uint8_t* side = a.side + (int64_t)t * (n + 1);
-    int32_t* rankR = a.rankR + (int64_t)t * (n + 1);
-    int32_t* P = a.PQ + (int64_t)t * 2 * (n + 1);
+    uint8_t* side = a.side + so;
+    int32_t* rankR = a.rankR + so;
+    int32_t* P = a.PQ + 2 * so;
     int32_t* Q = P + (n + 1);
-    int32_t* tmp = a.tmp + (int64_t)t * (n + 1);
+    int32_t* tmp = a.tmp + so;
     const int64_t cap = a.cap_nodes[t];
 
     for (int64_t i = tid; i < n; i += blockDim.x) idx[i] = (int32_t)i;
@@ -176,8 +183,8 @@ __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) 
             }
             continue;  // next iteration re-reads sh_sp after the barrier below
         }
-        const float* x1 = a.X + (int64_t)idx[e.start + sh_i1] * a.ld;
-        const float* x2 = a.X + (int64_t)idx[e.start + sh_i2] * a.ld;
+        const float* x1 = X + (int64_t)idx[e.start + sh_i1] * a.ld;
+        const float* x2 = X + (int64_t)idx[e.start + sh_i2] * a.ld;
         for (int i = tid; i < dim; i += blockDim.x) {
             p1s[i] = x1[i];
             p2s[i] = x2[i];
@@ -245,7 +252,7 @@ __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) 
                 const int pos = base + g;
                 const bool act = pos < e.end;
                 float acc = 0.f;
-                const float* x = a.X + (int64_t)(act ? idx[pos] : 0) * a.ld;
+                const float* x = X + (int64_t)(act ? idx[pos] : 0) * a.ld;
                 if (act)
                     for (int c = 0; c < chunks; ++c) acc = __builtin_fmaf(x[c * 8 + l], hp[c * 8 + l], acc);
                 const float s4 = acc + __shfl_xor(acc, 4);
@@ -366,11 +373,30 @@ __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) 
         a.n_nodes_out[t] = sh_ctr;
         a.status[t] = sh_abort;
     }
+    __syncthreads();  // the workgroup's shared state is reused by its next tree
+}
+
+// One workgroup per tree, or (a.queue) persistent workgroups pulling trees from a work queue, largest first.
+__global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) {
+    if (!a.queue) {
+        forest_tree_dfs(a, blockIdx.x, (int64_t)blockIdx.x * (a.n + 1));
+        return;
+    }
+    __shared__ int sh_item;
+    for (;;) {
+        if (threadIdx.x == 0) sh_item = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = sh_item;
+        __syncthreads();
+        if (item >= a.n_trees) break;  // uniform: every workgroup ends once the queue is drained
+        forest_tree_dfs(a, a.order[item], (int64_t)blockIdx.x * a.slot_stride);
+    }
 }
 
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a) {
     const size_t lds = (size_t)3 * a.dim * sizeof(float);
-    hipLaunchKernelGGL(k_forest_dfs, dim3(a.n_trees), dim3(kForestThreads), lds, s, a);
+    const unsigned grid = a.queue ? (unsigned)std::max(1, std::min(a.grid, a.n_trees)) : (unsigned)a.n_trees;
+    hipLaunchKernelGGL(k_forest_dfs, dim3(grid), dim3(kForestThreads), lds, s, a);
 }
 
 // ---- leaves in depth-first order -------------------------------------------------------------------
@@ -417,6 +443,41 @@ void launch_leaf_list(hipStream_t s, const int32_t* children, int64_t n_nodes, i
                                              (int)(n + 1), s));
     hipLaunchKernelGGL(k_leaf_emit, dim3(grid_for(n, 256)), dim3(256), 0, s, leaf_mark,
                        (const uint64_t*)leaf_rank, leaf_end_at, n, leaves);
+}
+
+// Leaves of a virtual tree: tree number t of every index of a batch that has one, laid side by side.  Entry e
+// places its tree's positions at pbase .. pbase + n (ids offset by the index's first row, base) and its leaves
+// (start, end) at those positions; each position is in exactly one leaf of its own index's tree.
+__global__ void k_vtree_build(const VTreeEnt* __restrict__ ents, int32_t* mark, int32_t* end_at, int32_t* vidx) {
+    const VTreeEnt e = ents[blockIdx.y];
+    const int64_t span = e.n_nodes > e.n ? e.n_nodes : e.n;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < span; v += (int64_t)gridDim.x * blockDim.x) {
+        if (v < e.n_nodes) {
+            const int32_t c0 = e.children[2 * v];
+            if (c0 <= 0) {
+                const int32_t st = -c0, en = -e.children[2 * v + 1];
+                if (en > st) {
+                    mark[e.pbase + st] = 1;
+                    end_at[e.pbase + st] = (int32_t)(e.pbase + en);
+                }
+            }
+        }
+        if (v < e.n) vidx[e.pbase + v] = (int32_t)(e.base + e.indices[v]);
+    }
+}
+void launch_leaf_list_batch(hipStream_t s, const VTreeEnt* ents_dev, int n_ents, int64_t max_span, int64_t n,
+                            int32_t* leaf_mark, int64_t* leaf_rank, int32_t* leaf_end_at, int64_t* leaves,
+                            int32_t* vidx, void* scan_tmp, size_t scan_tmp_bytes) {
+    launch_zero(s, leaf_mark, (size_t)(n + 1) * sizeof(int32_t));
+    if (n_ents > 0)
+        hipLaunchKernelGGL(k_vtree_build, dim3(grid_for(max_span, 256, 512), n_ents), dim3(256), 0, s, ents_dev,
+                           leaf_mark, leaf_end_at, vidx);
+    unsigned long long* wide = (unsigned long long*)(leaf_rank + (n + 1));
+    hipLaunchKernelGGL(k_i32_to_u64, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, leaf_mark, wide, n + 1);
+    size_t bytes = scan_tmp_bytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, wide, (unsigned long long*)leaf_rank, (int)(n + 1), s));
+    hipLaunchKernelGGL(k_leaf_emit, dim3(grid_for(n, 256)), dim3(256), 0, s, leaf_mark, (const uint64_t*)leaf_rank,
+                       leaf_end_at, n, leaves);
 }
 
 __global__ void k_leaf_pair_counts(const int64_t* leaves, int64_t n_leaves, uint64_t* out) {
@@ -1202,18 +1263,20 @@ void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* 
 }
 
 // ---- candidates -----------------------------------------------------------------------------------
-__global__ void k_row_valid(const int32_t* __restrict__ idx, int64_t n, int k,
+// row_on (optional): rows whose byte is 0 (indexes of a batch that have finished) count no edges
+__global__ void k_row_valid(const int32_t* __restrict__ idx, int64_t n, int k, const uint8_t* __restrict__ row_on,
                             unsigned long long* cnt) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n;
          i += (int64_t)gridDim.x * blockDim.x) {
         unsigned long long c = 0;
-        if (i < n)
+        if (i < n && (!row_on || row_on[i]))
             for (int j = 0; j < k; ++j) c += idx[i * k + j] >= 0;
         cnt[i] = c;
     }
 }
-void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt) {
-    hipLaunchKernelGGL(k_row_valid, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, idx, n, k,
+void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt,
+                             const uint8_t* row_on) {
+    hipLaunchKernelGGL(k_row_valid, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, idx, n, k, row_on,
                        (unsigned long long*)cnt);
 }
 
@@ -1252,14 +1315,16 @@ void launch_rng_streams(hipStream_t s, const uint64_t* jump_tables, const RngStr
 __global__ void k_cand_gen_rev(const int32_t* __restrict__ idx, const uint8_t* __restrict__ flags, int64_t n, int k,
                                const uint64_t* __restrict__ rvp, const float* __restrict__ prio, int mode,
                                int64_t block_rows, const int64_t* __restrict__ block_prio_base, uint64_t counter_key,
-                               float* fpr, uint32_t* rkey, uint64_t* rval, unsigned long long* cnt) {
+                               const uint8_t* __restrict__ row_on, float* fpr, uint32_t* rkey, uint64_t* rval,
+                               unsigned long long* cnt) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint64_t r = rvp[i];
         const int64_t blk = mode == 1 ? i / block_rows : 0;
         const uint64_t blk_first = mode == 1 ? rvp[blk * block_rows] : 0;
+        const bool on = !row_on || row_on[i];  // a finished index's rows send no reverse edges
         for (int j = 0; j < k; ++j) {
             const uint64_t e = (uint64_t)i * k + j;
-            const int32_t nb = idx[e];
+            const int32_t nb = on ? idx[e] : -1;
             if (nb < 0) {
                 rkey[e] = (uint32_t)n;  // sorts after every destination
                 continue;
@@ -1580,9 +1645,10 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
         hipLaunchKernelGGL(k_cand_gen_rev_ctr, dim3(grid_for(nk, 256, 1 << 16)), dim3(256), 0, s, a.idx, a.flags, a.n,
                            a.k, a.counter_key, a.fpr, a.rkey, a.rval);
     } else {
-        HIP_TRY(hipMemsetAsync(a.cnt, 0, (a.n + 1) * sizeof(unsigned long long), s));
+        launch_zero(s, a.cnt, (a.n + 1) * sizeof(unsigned long long));
         hipLaunchKernelGGL(k_cand_gen_rev, dim3(grid_for(a.n, 128)), dim3(128), 0, s, a.idx, a.flags, a.n, a.k, a.rvp,
-                           a.prio, a.mode, a.block_rows, a.block_prio_base, a.counter_key, a.fpr, a.rkey, a.rval, a.cnt);
+                           a.prio, a.mode, a.block_rows, a.block_prio_base, a.counter_key, a.row_on, a.fpr, a.rkey,
+                           a.rval, a.cnt);
     }
     int end_bit = 1;
     while (((int64_t)1 << end_bit) <= a.n) ++end_bit;  // destinations 0..n (n = no edge)
@@ -1592,7 +1658,7 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
         hipLaunchKernelGGL(k_seg_starts, dim3(grid_for(nk + 1, 256)), dim3(256), 0, s, a.rkey2, nk, a.n, a.off);
     else
         scan_u64_exclusive(s, a.cnt, a.off, a.n + 1, a.scan_tmp, a.scan_tmp_bytes);
-    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
+    launch_zero(s, a.bins, 3 * kBins * sizeof(uint32_t));
     hipLaunchKernelGGL(k_bin_count2, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
     hipLaunchKernelGGL(k_bin_scatter2, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.rvp, a.n, a.bins,
@@ -1610,65 +1676,72 @@ void launch_cand_build(hipStream_t s, const CandBuildArgs& a) {
 
 
 // ---- local join, FP32 (update_iteration generation, mod.rs:383-447) --------------------------------
-// One workgroup per row i: the row's new (C) and old (C) candidate rows are staged in LDS, every
-// pair (j, kk) with kk in (j+1..C) over new and C+(0..C) over old is evaluated with the reference's
-// arithmetic and kept if d is below the root of p or of q (the flags say which).  Kept pairs go to the
-// row's own region of U (no global counter); the row's record count goes to krec.
-constexpr int kJoinThreads = 256;
+// A workgroup takes RPG rows at a time with TPR threads per row (TPR >= 64: a wave never spans two rows).
+// Row i's new (C) and old (C) candidate rows are staged in LDS; every pair (j, kk) with kk in (j+1..C) over
+// new and C+(0..C) over old is evaluated with the reference's arithmetic and kept if d is below the root
+// of p or of q (the flags say which).  Kept pairs go to the row's own region of U (no global counter);
+// the row's record count goes to krec.  The tile shape changes only the schedule, never a result.
 constexpr int kJoinLdsBytes = 64 * 1024;
 
-template <bool STAGED>
-__global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
+template <bool STAGED, int TPR, int RPG>
+__global__ __launch_bounds__(TPR * RPG) void k_join_fp32(
     int metric, const float* __restrict__ X, int64_t ld, int dim,
     const int32_t* __restrict__ new_idx, const int32_t* __restrict__ old_idx, int C,
     int64_t row_begin, int64_t row_end, const float* __restrict__ hd, int k, JoinOut out) {
+    static_assert(TPR % RINND_WAVE == 0, "a wave must not span two rows");
     extern __shared__ __align__(16) float jsm[];
-    __shared__ int32_t ids[128];
-    __shared__ float roots[128];
-    __shared__ unsigned run, nrec;
+    __shared__ int32_t ids[RPG][128];
+    __shared__ float roots[RPG][128];
+    __shared__ unsigned run[RPG], nrec[RPG];
     const int C2 = 2 * C;
     const int64_t ldl = ld + 4;  // padded LDS row stride
     const int lane = threadIdx.x & 63;
+    const int g = threadIdx.x / TPR, gt = threadIdx.x - g * TPR;  // the thread's row in the group, its rank there
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    unsigned long long kept = 0;  // thread 0: kept pairs of this workgroup's rows
-    for (int64_t i = row_begin + blockIdx.x; i < row_end; i += gridDim.x) {
-        const int64_t r = i - row_begin;
-        for (int c = threadIdx.x; c < C2; c += blockDim.x) {
-            const int32_t id = c < C ? new_idx[i * C + c] : old_idx[i * C + (c - C)];
-            ids[c] = id;
-            roots[c] = id >= 0 ? hd[(int64_t)id * k] : 0.f;
+    unsigned long long kept = 0;  // threads gt == 0: kept pairs of their rows
+    for (int64_t i0 = row_begin + (int64_t)blockIdx.x * RPG; i0 < row_end; i0 += (int64_t)gridDim.x * RPG) {
+        for (int c = threadIdx.x; c < RPG * C2; c += blockDim.x) {
+            const int gg = c / C2, cc = c - gg * C2;
+            const int64_t i = i0 + gg;
+            int32_t id = -1;
+            if (i < row_end) id = cc < C ? new_idx[i * C + cc] : old_idx[i * C + (cc - C)];
+            ids[gg][cc] = id;
+            roots[gg][cc] = id >= 0 ? hd[(int64_t)id * k] : 0.f;
         }
-        if (threadIdx.x == 0) run = nrec = 0;
+        if (threadIdx.x < RPG) run[threadIdx.x] = nrec[threadIdx.x] = 0;
         __syncthreads();
         if (STAGED) {
             const int vec = (int)(ld >> 2);
-            for (int t = threadIdx.x; t < C2 * vec; t += blockDim.x) {
+            for (int t = threadIdx.x; t < RPG * C2 * vec; t += blockDim.x) {
                 const int c = t / vec, e4 = t - c * vec;
-                const int32_t id = ids[c];
+                const int gg = c / C2;
+                const int32_t id = ids[gg][c - gg * C2];
                 float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (id >= 0) v = *reinterpret_cast<const float4*>(X + (int64_t)id * ld + e4 * 4);
                 *reinterpret_cast<float4*>(jsm + c * ldl + e4 * 4) = v;
             }
             __syncthreads();
         }
-        JRec* U = out.U + out.rowoff[r];
+        const int64_t i = i0 + g;
+        const bool row_ok = i < row_end;
+        const int64_t r = i - row_begin;
+        JRec* U = out.U + (row_ok ? out.rowoff[r] : 0);
+        const float* rows = jsm + (int64_t)g * C2 * ldl;
         const int total = C * C2;
-        for (int t0 = 0; t0 < total; t0 += blockDim.x) {
-            const int t = t0 + threadIdx.x;
+        for (int t0 = 0; t0 < total; t0 += TPR) {
+            const int t = t0 + gt;
             bool keep = false, ep = false, eq = false;
             float d = 0.f;
-            int j = 0, kk = 0;
             if (t < total) {
-                j = t / C2;
-                kk = t - j * C2;
-                const int32_t p = ids[j], q = ids[kk];
+                const int j = t / C2, kk = t - j * C2;
+                const int32_t p = ids[g][j], q = ids[g][kk];
                 const bool valid = p >= 0 && q >= 0 && (kk < C ? kk > j : p != q);
                 if (valid) {
-                    const float* xp = STAGED ? jsm + j * ldl : X + (int64_t)p * ld;
-                    const float* xq = STAGED ? jsm + kk * ldl : X + (int64_t)q * ld;
+                    const float* xp = STAGED ? rows + j * ldl : X + (int64_t)p * ld;
+                    const float* xq = STAGED ? rows + kk * ldl : X + (int64_t)q * ld;
                     d = distance_exact(metric, xp, xq, dim);
-                    ep = d < roots[j];
-                    eq = d < roots[kk];
+                    ep = d < roots[g][j];
+                    eq = d < roots[g][kk];
                     keep = ep || eq;
                 }
             }
@@ -1678,37 +1751,124 @@ __global__ __launch_bounds__(kJoinThreads) void k_join_fp32(
                 const int leader = __ffsll((long long)bal) - 1;
                 unsigned base = 0;
                 if (lane == leader) {
-                    base = atomicAdd(&run, (unsigned)__popcll(bal));
-                    atomicAdd(&nrec, nr);
+                    base = atomicAdd(&run[g], (unsigned)__popcll(bal));
+                    atomicAdd(&nrec[g], nr);
                 }
                 base = __shfl(base, leader);
                 if (keep) U[base + __popcll(bal & lt)] = JRec{(uint32_t)t | (ep ? kEmitP : 0u) | (eq ? kEmitQ : 0u), d};
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            out.kc[r] = (unsigned long long)run;
-            out.krec[r] = (unsigned long long)nrec;
-            kept += run;
+        if (gt == 0 && row_ok) {
+            out.kc[r] = (unsigned long long)run[g];
+            out.krec[r] = (unsigned long long)nrec[g];
+            kept += run[g];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && kept) atomicAdd(&out.kept[blockIdx.x % kKeptSlots], kept);
+    if (gt == 0 && kept) atomicAdd(&out.kept[(blockIdx.x * RPG + g) % kKeptSlots], kept);
 }
-void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
-                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
-                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out) {
-    if (row_end <= row_begin) return;
+
+// Tile shapes (threads per row, rows per workgroup); shape 0 is the default.
+struct JoinShape {
+    int tpr, rpg;
+};
+static constexpr JoinShape kJoinShapes[] = {{256, 1}, {128, 1}, {128, 2}, {64, 2}, {64, 4}};
+constexpr int kJoinShapeCount = (int)(sizeof(kJoinShapes) / sizeof(kJoinShapes[0]));
+
+static bool join_shape_staged(int shape, int C, int64_t ld) {
+    return (int64_t)kJoinShapes[shape].rpg * 2 * C * (ld + 4) * 4 <= kJoinLdsBytes;
+}
+
+template <int TPR, int RPG>
+static void launch_join_fp32_shape(hipStream_t s, bool staged, int metric, const float* X, int64_t ld, int dim,
+                                   const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                                   int64_t row_end, const float* heap_dist, int k, const JoinOut& out) {
     const int64_t ldl = ld + 4;
-    const bool staged = (int64_t)2 * C * ldl * 4 <= kJoinLdsBytes;
-    const size_t lds = staged ? (size_t)2 * C * ldl * 4 : 16;
-    const dim3 grid(grid_for(row_end - row_begin, 1, 1 << 18));
+    const size_t lds = staged ? (size_t)RPG * 2 * C * ldl * 4 : 16;
+    const dim3 grid(grid_for(ceil_div(row_end - row_begin, (int64_t)RPG), 1, 1 << 18));
     if (staged)
-        hipLaunchKernelGGL(k_join_fp32<true>, grid, dim3(kJoinThreads), lds, s, metric, X, ld, dim, new_idx,
+        hipLaunchKernelGGL((k_join_fp32<true, TPR, RPG>), grid, dim3(TPR * RPG), lds, s, metric, X, ld, dim, new_idx,
                            old_idx, C, row_begin, row_end, heap_dist, k, out);
     else
-        hipLaunchKernelGGL(k_join_fp32<false>, grid, dim3(kJoinThreads), lds, s, metric, X, ld, dim, new_idx,
+        hipLaunchKernelGGL((k_join_fp32<false, TPR, RPG>), grid, dim3(TPR * RPG), lds, s, metric, X, ld, dim, new_idx,
                            old_idx, C, row_begin, row_end, heap_dist, k, out);
+}
+
+void launch_join_fp32(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
+                      const int32_t* new_idx, const int32_t* old_idx, int C, int64_t row_begin,
+                      int64_t row_end, const float* heap_dist, int k, const JoinOut& out, int shape) {
+    if (row_end <= row_begin) return;
+    if (shape < 0 || shape >= kJoinShapeCount) shape = 0;
+    // a multi-row shape is used only where all its rows' candidates fit in LDS
+    if (shape != 0 && !join_shape_staged(shape, C, ld)) shape = 0;
+    const bool staged = join_shape_staged(shape, C, ld);
+    switch (shape) {
+        case 1: launch_join_fp32_shape<128, 1>(s, staged, metric, X, ld, dim, new_idx, old_idx, C, row_begin, row_end, heap_dist, k, out); break;
+        case 2: launch_join_fp32_shape<128, 2>(s, staged, metric, X, ld, dim, new_idx, old_idx, C, row_begin, row_end, heap_dist, k, out); break;
+        case 3: launch_join_fp32_shape<64, 2>(s, staged, metric, X, ld, dim, new_idx, old_idx, C, row_begin, row_end, heap_dist, k, out); break;
+        case 4: launch_join_fp32_shape<64, 4>(s, staged, metric, X, ld, dim, new_idx, old_idx, C, row_begin, row_end, heap_dist, k, out); break;
+        default: launch_join_fp32_shape<256, 1>(s, staged, metric, X, ld, dim, new_idx, old_idx, C, row_begin, row_end, heap_dist, k, out); break;
+    }
+}
+
+int join_fp32_shape_count() { return kJoinShapeCount; }
+void join_fp32_shape_info(int shape, int* tpr, int* rpg) {
+    if (shape < 0 || shape >= kJoinShapeCount) shape = 0;
+    *tpr = kJoinShapes[shape].tpr;
+    *rpg = kJoinShapes[shape].rpg;
+}
+
+// The join over rows [0, n) with the tile-shape tuner: every eligible shape runs, timed, on two slices of the
+// rows (shapes in order, then in reverse order, so drift over the launch sequence cancels), and the rest of the
+// rows go to the fastest per row.  Each row is joined once, whichever shape takes it: the results are those of
+// any single shape.  Returns the fastest shape.
+int launch_join_fp32_tuned(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* new_idx,
+                           const int32_t* old_idx, int C, int64_t n, const float* heap_dist, int k, const JoinOut& out,
+                           bool* tuned) {
+    std::vector<int> shapes;
+    for (int sh = 0; sh < kJoinShapeCount; ++sh)
+        if (sh == 0 || join_shape_staged(sh, C, ld)) shapes.push_back(sh);
+    const int V = (int)shapes.size();
+    const int64_t slice = n / (4 * (int64_t)V);
+    *tuned = false;
+    if (V < 2 || slice < 16384) {  // too few rows to time: the default shape
+        launch_join_fp32(s, metric, X, ld, dim, new_idx, old_idx, C, 0, n, heap_dist, k, out, 0);
+        return 0;
+    }
+    *tuned = true;
+    std::vector<hipEvent_t> ev(2 * V + 1);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    auto sub = [&](int64_t b) {  // the join's outputs for rows starting at b
+        JoinOut o = out;
+        o.rowoff += b;
+        o.kc += b;
+        o.krec += b;
+        return o;
+    };
+    int64_t row = 0;
+    HIP_TRY(hipEventRecord(ev[0], s));
+    for (int pass = 0; pass < 2; ++pass)
+        for (int v = 0; v < V; ++v) {
+            const int sh = shapes[pass == 0 ? v : V - 1 - v];
+            launch_join_fp32(s, metric, X, ld, dim, new_idx, old_idx, C, row, row + slice, heap_dist, k, sub(row), sh);
+            row += slice;
+            HIP_TRY(hipEventRecord(ev[1 + pass * V + v], s));
+        }
+    HIP_TRY(hipEventSynchronize(ev[2 * V]));
+    std::vector<double> ms(V, 0.0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int v = 0; v < V; ++v) {
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, ev[pass * V + v], ev[1 + pass * V + v]));
+            ms[pass == 0 ? v : V - 1 - v] += t;
+        }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    int best = 0;
+    for (int v = 1; v < V; ++v)
+        if (ms[v] < ms[best]) best = v;
+    launch_join_fp32(s, metric, X, ld, dim, new_idx, old_idx, C, row, n, heap_dist, k, sub(row), shapes[best]);
+    return shapes[best];
 }
 
 
@@ -1844,7 +2004,7 @@ template <bool REG>
 __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict__ order, const uint32_t* bins,
                                                        const uint64_t* __restrict__ off, const uint64_t* __restrict__ rv,
                                                        int32_t* idx, float* dist, uint8_t* flags, int k,
-                                                       unsigned long long* changes) {
+                                                       unsigned long long* changes, uint32_t* row_changes) {
     extern __shared__ __align__(16) float hsm2[];
     const int B = blockDim.x, c = threadIdx.x;
     const int S = B + 1;  // heap column stride (odd: the row staging is conflict-free)
@@ -1858,6 +2018,7 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
         const int32_t v = e >= 0 ? order[e] : -1;
         wave_heap_rows<true>(v, k, dist, idx, flags, hd, hi, hf, S, c & ~63);
         if (e >= 0) {
+            const unsigned long long ch0 = ch;
             const uint64_t o0 = off[v];
             const int64_t L = (int64_t)(off[v + 1] - o0);
             const uint64_t* R = rv + o0;
@@ -1898,6 +2059,7 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
                 for (int u = 0; u < 8; ++u) offer(q[u]);
             }
             for (; r < L; ++r) offer(R[r]);
+            if (row_changes) row_changes[v] = (uint32_t)(ch - ch0);
         }
         wave_heap_rows<false>(v, k, dist, idx, flags, hd, hi, hf, S, c & ~63);
     }
@@ -1918,7 +2080,8 @@ __global__ __launch_bounds__(256) void k_heap_push_rec(const int32_t* __restrict
 constexpr int kWaveApplyWaves = 4;
 __global__ __launch_bounds__(kWaveApplyWaves * RINND_WAVE) void k_heap_push_rec_wave(
     const int32_t* __restrict__ order, const uint32_t* bins, const uint64_t* __restrict__ off,
-    const uint64_t* __restrict__ rv, int32_t* idx, float* dist, uint8_t* flags, int k, unsigned long long* changes) {
+    const uint64_t* __restrict__ rv, int32_t* idx, float* dist, uint8_t* flags, int k, unsigned long long* changes,
+    uint32_t* row_changes) {
     extern __shared__ __align__(16) float wsm[];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* hd = wsm + (size_t)w * k;
@@ -1935,6 +2098,7 @@ __global__ __launch_bounds__(kWaveApplyWaves * RINND_WAVE) void k_heap_push_rec_
             hf[j] = flags[row + j];
         }
         wave_sync();
+        const unsigned long long ch0 = ch;
         const uint64_t o0 = off[v];
         const int64_t L = (int64_t)(off[v + 1] - o0);
         for (int64_t r = 0; r < L; ++r) {
@@ -1958,6 +2122,7 @@ __global__ __launch_bounds__(kWaveApplyWaves * RINND_WAVE) void k_heap_push_rec_
             idx[row + j] = hi[j];
             flags[row + j] = hf[j];
         }
+        if (row_changes && lane == 0) row_changes[v] = (uint32_t)(ch - ch0);
         wave_sync();
     }
     if (lane == 0 && ch) atomicAdd(changes, ch);
@@ -1988,7 +2153,7 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
     hipLaunchKernelGGL(k_seg_starts, dim3(grid_for((int64_t)a.n_records + 1, 256)), dim3(256), 0, s, a.rkey2,
                        (int64_t)a.n_records, a.n, a.off);
     // destinations binned by record count, longest first
-    HIP_TRY(hipMemsetAsync(a.bins, 0, 3 * kBins * sizeof(uint32_t), s));
+    launch_zero(s, a.bins, 3 * kBins * sizeof(uint32_t));
     hipLaunchKernelGGL(k_bin_count, dim3(grid_for(a.n, 256, 2048)), dim3(256), 0, s, a.off, a.n, a.bins);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, a.bins);
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(a.n, 256, 4096)), dim3(256), 0, s, a.off, a.n, a.bins, a.order);
@@ -1997,16 +2162,16 @@ void launch_join_apply(hipStream_t s, const JoinApplyArgs& a) {
         const size_t lds = (size_t)kWaveApplyWaves * round_up((int64_t)a.k * 9, 16);
         hipLaunchKernelGGL(k_heap_push_rec_wave, dim3(grid_for(a.n, kWaveApplyWaves, 8192)),
                            dim3(kWaveApplyWaves * RINND_WAVE), lds, s, a.order, a.bins, a.off, a.rval2, a.idx, a.dist,
-                           a.flags, a.k, a.changes);
+                           a.flags, a.k, a.changes, a.row_changes);
         return;
     }
     const size_t lds = (size_t)round_up((int64_t)(B + 1) * a.k * 2 + ((B + 1) * a.k + 3) / 4, 4) * 4;
     if (a.k <= 16)
         hipLaunchKernelGGL(k_heap_push_rec<true>, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins,
-                           a.off, a.rval2, a.idx, a.dist, a.flags, a.k, a.changes);
+                           a.off, a.rval2, a.idx, a.dist, a.flags, a.k, a.changes, a.row_changes);
     else
         hipLaunchKernelGGL(k_heap_push_rec<false>, dim3(grid_for(a.n, B, 8192)), dim3(B), lds, s, a.order, a.bins,
-                           a.off, a.rval2, a.idx, a.dist, a.flags, a.k, a.changes);
+                           a.off, a.rval2, a.idx, a.dist, a.flags, a.k, a.changes, a.row_changes);
 }
 
 // ---- sort_all (neighbor_heap.rs:293-320): stable ascending sort per row -------------------------
