@@ -65,6 +65,13 @@ def parse():
                     help="skip the 1,000,000 x 960, k=30 precision-form builds (BF16 join vs FP32)")
     ap.add_argument("--pf-n", type=int, default=1_000_000, help="rows of the precision-form workload")
     ap.add_argument("--pf-steps", type=int, default=2, help="timed builds per precision of the precision form")
+    ap.add_argument("--leg", choices=["batching", "precision", "serving"],
+                    help="run only this form's leg and print its JSON (profiling runs)")
+    ap.add_argument("--no-batching", action="store_true", help="skip the batching-form leg (512 independent indexes)")
+    ap.add_argument("--bf-count", type=int, default=512, help="indexes of the batching form")
+    ap.add_argument("--bf-steps", type=int, default=2, help="timed batched builds of the batching form")
+    ap.add_argument("--bf-no-sequential", action="store_true",
+                    help="skip the sequential builds of the batching form (profiling runs)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving form (2,000,000 x 96 resident index, Poisson query stream)")
     ap.add_argument("--sv-n", type=int, default=2_000_000, help="rows of the serving-form index")
@@ -155,6 +162,10 @@ def main():
     from rinnd_amd import _lib as L
     from rinnd_amd import datasets
 
+    if args.leg:
+        fn = {"batching": batching_form, "precision": precision_form, "serving": serving_form}[args.leg]
+        print(json.dumps({args.leg + "_form": fn(torch, L.Context.default(0), L, args)}), flush=True)
+        return
     n, dim, k = args.n, args.dim, args.k
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
@@ -251,6 +262,11 @@ def main():
             line["precision_form"] = precision_form(torch, ctx, L, args)
         except Exception as e:
             log(f"[bench] precision form failed: {e}")
+    if not args.profile and not args.no_batching:
+        try:
+            line["batching_form"] = batching_form(torch, ctx, L, args)
+        except Exception as e:
+            log(f"[bench] batching form failed: {e}")
     if not args.profile and not args.no_serving:
         try:
             line["serving_form"] = serving_form(torch, ctx, L, args)
@@ -381,6 +397,127 @@ def precision_form(torch, ctx, L, args):
         f"{out['fp32']['recall_at_k_vs_fp64']:.4f} bf16 {out['bf16']['recall_at_k_vs_fp64']:.4f}, identical "
         f"{out['bf16_graph_equals_fp32_graph']}")
     del X64, Xd
+    torch.cuda.empty_cache()
+    return out
+
+
+def batching_form(torch, ctx, L, args):
+    """512 independent indexes, n log-uniform in [2,000, 100,000] (numpy seed 5), d=64 standard-Gaussian rows
+    L2-normalised, Cosine, k=10 (max_candidates 10, leaf 60, delta 0.001; per index n_trees = clamp(round(2 log10 n),
+    3, 12) and n_iters = max(ceil(log2 n), 5)), reference draws with one candidate thread.  One rinnd_nn_descent_batch
+    call (work-queue forest, hipGraph-replayed passes, tuned join tiles) against the same indexes built by 512
+    sequential rinnd_nn_descent calls; both from device-resident rows to row-sorted device heaps (sort_all
+    included).  The two sets of heaps must be identical; recall@10 is sampled against an FP64 brute force."""
+    import math
+    from rinnd_amd import datasets
+    m, d, k = args.bf_count, 64, 10
+    sizes = datasets.cfg4_sizes(m)
+    off = np.zeros(m + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    N = int(off[-1])
+    log(f"[bench] batching form: generating {m} indexes, {N} rows x {d}")
+    X = np.empty((N, d), np.float32)
+    for i, n in enumerate(sizes):
+        X[off[i]:off[i + 1]] = datasets.cfg4_index(n, d, seed=5, i=i)
+    Xd = torch.from_numpy(X).cuda()
+    del X
+    nt = np.array([min(12, max(3, int(round(2 * math.log10(n))))) for n in sizes], np.int64)
+    ni = np.array([max(int(math.ceil(math.log2(n))), 5) for n in sizes], np.int64)
+    seeds = [1000 + i for i in range(m)]
+    p = L.make_params(k, 0, 60, k, 0, 0.001, angular=True, n_threads=1, rng_mode="reference")
+    outs_b = (torch.empty((N, k), dtype=torch.int32, device="cuda"), torch.empty((N, k), dtype=torch.float32, device="cuda"),
+              torch.empty((N, k), dtype=torch.uint8, device="cuda"))
+    outs_s = tuple(torch.empty_like(t) for t in outs_b)
+    its = np.zeros(m, np.int32)
+    conv = np.zeros(m, np.int32)
+
+    def batched():
+        states = np.ascontiguousarray(np.stack([L.rng_seed(sd) for sd in seeds]), np.uint64)
+        st = L.BatchStats()
+        t0 = time.perf_counter()
+        rc = L.lib().rinnd_nn_descent_batch(ctx.h, L.ptr(Xd), L.ptr(off), m, d, L.DIST["cosine"], C.byref(p),
+                                            L.ptr(nt), L.ptr(ni), L.ptr(states), L.ptr(outs_b[0]), L.ptr(outs_b[1]),
+                                            L.ptr(outs_b[2]), L.ptr(its), L.ptr(conv), 0, C.byref(st))
+        ctx.check(rc, "rinnd_nn_descent_batch")
+        ctx.check(L.lib().rinnd_sort_rows(ctx.h, L.ptr(outs_b[0]), L.ptr(outs_b[1]), L.ptr(outs_b[2]), N, k),
+                  "rinnd_sort_rows")
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, st.to_dict()
+
+    batched()  # warm-up (the join tile shape is tuned here and kept by the context)
+    times, stats = [], []
+    for s_ in range(args.bf_steps):
+        t, st = batched()
+        times.append(t)
+        stats.append(st)
+        log(f"[bench] batching form batched step {s_ + 1}: {t * 1e3:.1f} ms, passes {st['passes']}, graph replays "
+            f"{st['graph_replays']}, forest {st['ms_forest']:.1f} init {st['ms_init']:.1f} cand+join "
+            f"{st['ms_candidates_join']:.1f} apply {st['ms_apply']:.1f} ms, join shape "
+            f"{st['join_threads_per_row']}x{st['join_rows_per_group']}")
+    t_b = float(np.median(times))
+    # the same indexes one rinnd_nn_descent call each
+    row_b = d * 4
+
+    def sequential():
+        t0 = time.perf_counter()
+        for i in range(m):
+            a, n = int(off[i]), int(sizes[i])
+            pi = L.make_params(k, int(nt[i]), 60, k, int(ni[i]), 0.001, angular=True, n_threads=1, rng_mode="reference")
+            st = L.rng_seed(seeds[i])
+            h = C.c_void_p()
+            oi = C.c_void_p(outs_s[0].data_ptr() + a * k * 4)
+            od = C.c_void_p(outs_s[1].data_ptr() + a * k * 4)
+            of = C.c_void_p(outs_s[2].data_ptr() + a * k)
+            rc = L.lib().rinnd_nn_descent(ctx.h, C.c_void_p(Xd.data_ptr() + a * row_b), n, d, L.DIST["cosine"],
+                                          C.byref(pi), L.ptr(st), oi, od, of, C.byref(h), None)
+            ctx.check(rc, "rinnd_nn_descent")
+            L.lib().rinnd_forest_free(h)
+            ctx.check(L.lib().rinnd_sort_rows(ctx.h, oi, od, of, n, k), "rinnd_sort_rows")
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    if args.bf_no_sequential:
+        t_s, same = float("nan"), None
+    else:
+        t_s = sequential()
+        log(f"[bench] batching form sequential: {t_s * 1e3:.1f} ms for {m} builds")
+        same = bool(torch.equal(outs_b[0], outs_s[0]) and
+                    torch.equal(outs_b[1].view(torch.int32), outs_s[1].view(torch.int32)) and
+                    torch.equal(outs_b[2], outs_s[2]))
+    # recall@10 on 100 rows of each of 16 indexes, FP64 brute force within the index (1 - cos on unit rows)
+    rng = np.random.default_rng(3)
+    hits, total = 0.0, 0
+    for i in rng.choice(m, size=min(16, m), replace=False):
+        a, b = int(off[i]), int(off[i + 1])
+        Xi = Xd[a:b].double()
+        rows = torch.as_tensor(rng.choice(b - a, size=min(100, b - a), replace=False), device="cuda")
+        D = 1.0 - Xi[rows] @ Xi.T
+        D[torch.arange(len(rows), device="cuda"), rows] = float("inf")
+        kth = torch.topk(D, k, dim=1, largest=False).values[:, k - 1:k]
+        f = outs_b[0][a:b][rows].long()
+        fd = torch.gather(D, 1, f.clamp(min=0))
+        fd[f < 0] = float("inf")
+        hits += float((fd <= kth + 1e-12).double().sum())
+        total += len(rows) * k
+    st = stats[-1]
+    out = {"workload": f"{m} independent indexes, n log-uniform in [2000, 100000] (sum {N}), {d}-dim N(0,1) rows "
+                       f"L2-normalised, Cosine, k={k}, max_candidates {k}, leaf 60, per-index n_trees clamp(round(2 log10 n), "
+                       f"3, 12) and n_iters max(ceil(log2 n), 5), reference draws, one candidate thread",
+           "dtype": "f32", "rows": N,
+           "batched": {"points_per_s": N / t_b, "ms": 1e3 * t_b, "passes": st["passes"],
+                       "graph_replays": st["graph_replays"], "graph_memsets_replaced": st["graph_memsets_replaced"],
+                       "join_tile": {"threads_per_row": st["join_threads_per_row"],
+                                     "rows_per_workgroup": st["join_rows_per_group"]},
+                       "phase_ms": {x: st[x] for x in ("ms_forest", "ms_init", "ms_candidates_join", "ms_apply")}},
+           "sequential": {"points_per_s": N / t_s, "ms": 1e3 * t_s, "builds": m},
+           "speedup_vs_sequential": t_s / t_b,
+           "heaps_identical_to_sequential": same,
+           "iterations_range": [int(its.min()), int(its.max())], "converged": int(conv.sum()),
+           "recall_at_10_sampled": hits / max(total, 1)}
+    log(f"[bench] batching form: batched {t_b * 1e3:.1f} ms ({N / t_b / 1e6:.2f} M pts/s) vs sequential "
+        f"{t_s * 1e3:.1f} ms ({N / t_s / 1e6:.2f} M pts/s): {t_s / t_b:.1f}x, identical {same}, recall "
+        f"{out['recall_at_10_sampled']:.4f}")
+    del Xd, outs_b, outs_s
     torch.cuda.empty_cache()
     return out
 

@@ -207,7 +207,7 @@ static void build_forest_batch(hipStream_t s, const float* DX, int64_t ld, int d
     std::vector<int32_t> order(nt);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return trees[a].n > trees[b].n; });
-    const int grid = std::min(nt, 2 * device_cus());
+    const int grid = std::min(nt, forest_queue_grid(dim));
     const int64_t stride = max_n + 1;
     DBuf<uint8_t> side((size_t)grid * stride);
     DBuf<int32_t> rankR((size_t)grid * stride), PQ((size_t)grid * 2 * stride), tmp((size_t)grid * stride);

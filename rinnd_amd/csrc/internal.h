@@ -241,6 +241,8 @@ struct ForestDFSArgs {
     int grid;
 };
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a);
+// resident workgroups of the work-queue forest kernel on this device (the grid for a.queue launches)
+int forest_queue_grid(int dim);
 
 // forest, counter streams (level-synchronous across trees, forest_level.hip).  Trees must be
 // allocated with cap_nodes each and zeroed hyperplanes/offsets; returns false when a tree needs

@@ -106,14 +106,22 @@ struct StackEnt {
     int32_t start, end, depth, pslot;
 };
 
+// In the work-queue form (many trees in flight), subtrees of at most kWaveSub points are walked by one wave
+// with their positions in LDS (no workgroup barriers): the nodes, their order and the draws are those of the
+// workgroup walk.
+constexpr int kWaveSub = 1024;
+
+template <int THREADS, bool WAVE_SUB>
 __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64_t so) {
     extern __shared__ float fsm[];  // hp[dim], p1[dim], p2[dim]
     __shared__ StackEnt stack[kForestStack];
+    __shared__ int32_t wv_idx[kWaveSub], wv_tmp[kWaveSub], wv_rank[kWaveSub + 1], wv_P[kWaveSub], wv_Q[kWaveSub];
+    __shared__ uint8_t wv_side[kWaveSub];
     __shared__ uint64_t rs[4];
     __shared__ int sh_sp, sh_ctr, sh_abort, sh_tie, sh_node, sh_split, sh_leaf, sh_scale;
     __shared__ int sh_i1, sh_i2;
     __shared__ float sh_off, sh_inv, sh_n1, sh_n2;
-    __shared__ int wave_tot[kForestThreads / RINND_WAVE];
+    __shared__ int wave_tot[THREADS / RINND_WAVE];
     __shared__ int sh_running;
 
     const int tid = threadIdx.x;
@@ -145,11 +153,247 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
     }
     __syncthreads();
 
+    // The subtree rooted at the stack's top entry (at most kWaveSub points, not a leaf), walked by wave 0 alone:
+    // the same pops, node numbers, draws, hyperplanes and partitions as the workgroup walk below, with the
+    // subtree's positions held in LDS.  Returns with the entry and everything it pushed popped.
+    auto subtree_wave = [&]() {
+        int sp = sh_sp;
+        const int sp_end = sp - 1;
+        const StackEnt root = stack[sp - 1];
+        const int o0 = root.start, M = root.end - root.start;
+        for (int i = lane; i < M; i += RINND_WAVE) wv_idx[i] = idx[o0 + i];
+        wave_sync();
+        const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        while (sp > sp_end) {
+            const StackEnt c = stack[sp - 1];
+            sp -= 1;
+            const int cm = c.end - c.start, lo = c.start - o0;
+            int node = 0, leaf = 0, abort = 0, i1 = 0, i2 = 0;
+            if (lane == 0) {
+                node = sh_ctr;
+                if (node >= cap) {
+                    abort = 1;
+                    sh_abort = 1;
+                } else {
+                    sh_ctr = node + 1;
+                    if (c.pslot >= 0) ch[c.pslot] = node;
+                }
+                leaf = (cm <= a.leaf_size || c.depth >= a.max_depth) ? 1 : 0;
+                if (!leaf && !abort) {  // make_split (builder.rs:228-247)
+                    Xoshiro r;
+                    for (int w = 0; w < 4; ++w) r.s[w] = rs[w];
+                    const uint64_t j1 = r.next_index((uint64_t)cm);
+                    uint64_t j2 = r.next_index((uint64_t)cm);
+                    while (j2 == j1 && cm > 1) j2 = r.next_index((uint64_t)cm);
+                    for (int w = 0; w < 4; ++w) rs[w] = r.s[w];
+                    i1 = (int)j1;
+                    i2 = (int)j2;
+                }
+            }
+            abort = __shfl(abort, 0);
+            if (abort) break;
+            node = __shfl(node, 0);
+            leaf = __shfl(leaf, 0);
+            if (leaf) {
+                if (lane == 0) {
+                    ch[2 * node] = -c.start;
+                    ch[2 * node + 1] = -c.end;
+                }
+                continue;
+            }
+            i1 = __shfl(i1, 0);
+            i2 = __shfl(i2, 0);
+            const float* x1 = X + (int64_t)wv_idx[lo + i1] * a.ld;
+            const float* x2 = X + (int64_t)wv_idx[lo + i2] * a.ld;
+            for (int i = lane; i < dim; i += RINND_WAVE) {
+                p1s[i] = x1[i];
+                p2s[i] = x2[i];
+            }
+            wave_sync();
+            float off = 0.f;
+            if (!a.angular) {  // make_euclidean_split (builder.rs:249-279)
+                for (int i = lane; i < dim; i += RINND_WAVE) hp[i] = p2s[i] - p1s[i];
+                wave_sync();
+                float inv = 0.f;
+                int scale = 0;
+                if (lane == 0) {
+                    float norm_sq = 0.f, dot_nm = 0.f;
+                    for (int i = 0; i < dim; ++i) {
+                        const float h = hp[i];
+                        const float mid = (p1s[i] + p2s[i]) * 0.5f;
+                        norm_sq += h * h;
+                        dot_nm += h * mid;
+                    }
+                    const float norm = sqrtf(norm_sq);
+                    if (norm > 1e-8f) {
+                        inv = 1.0f / norm;
+                        off = -dot_nm * inv;
+                        scale = 1;
+                    }
+                }
+                scale = __shfl(scale, 0);
+                inv = __shfl(inv, 0);
+                off = __shfl(off, 0);
+                if (scale)
+                    for (int i = lane; i < dim; i += RINND_WAVE) hp[i] *= inv;
+            } else {  // make_angular_split (builder.rs:281-308)
+                float n1 = 0.f, n2 = 0.f;
+                if (lane == 0) {
+                    for (int i = 0; i < dim; ++i) n1 += p1s[i] * p1s[i];
+                    for (int i = 0; i < dim; ++i) n2 += p2s[i] * p2s[i];
+                    n1 = sqrtf(n1);
+                    n2 = sqrtf(n2);
+                }
+                n1 = __shfl(n1, 0);
+                n2 = __shfl(n2, 0);
+                const bool deg = n1 < 1e-8f || n2 < 1e-8f;
+                for (int i = lane; i < dim; i += RINND_WAVE) hp[i] = deg ? 0.f : p2s[i] / n2 - p1s[i] / n1;
+                wave_sync();
+                if (!deg) {
+                    float nn = 0.f;
+                    if (lane == 0) {
+                        for (int i = 0; i < dim; ++i) nn += hp[i] * hp[i];
+                        nn = sqrtf(nn);
+                    }
+                    nn = __shfl(nn, 0);
+                    if (nn > 1e-8f)
+                        for (int i = lane; i < dim; i += RINND_WAVE) hp[i] /= nn;
+                }
+            }
+            wave_sync();
+            // margins, one lane per point: the eight AVX2 accumulators and their reduction order
+            // (dot_product_avx2, builder.rs:29-60)
+            const int chunks = dim >> 3;
+            bool tie = false;
+            for (int b = 0; b < cm; b += RINND_WAVE) {
+                const int p = b + lane;
+                if (p < cm) {
+                    const float* x = X + (int64_t)wv_idx[lo + p] * a.ld;
+                    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    for (int cc = 0; cc < chunks; ++cc) {
+                        const float4 u = *reinterpret_cast<const float4*>(x + cc * 8);
+                        const float4 v = *reinterpret_cast<const float4*>(x + cc * 8 + 4);
+                        acc[0] = __builtin_fmaf(u.x, hp[cc * 8 + 0], acc[0]);
+                        acc[1] = __builtin_fmaf(u.y, hp[cc * 8 + 1], acc[1]);
+                        acc[2] = __builtin_fmaf(u.z, hp[cc * 8 + 2], acc[2]);
+                        acc[3] = __builtin_fmaf(u.w, hp[cc * 8 + 3], acc[3]);
+                        acc[4] = __builtin_fmaf(v.x, hp[cc * 8 + 4], acc[4]);
+                        acc[5] = __builtin_fmaf(v.y, hp[cc * 8 + 5], acc[5]);
+                        acc[6] = __builtin_fmaf(v.z, hp[cc * 8 + 6], acc[6]);
+                        acc[7] = __builtin_fmaf(v.w, hp[cc * 8 + 7], acc[7]);
+                    }
+                    float r = ((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]));
+                    for (int i = chunks * 8; i < dim; ++i) r += x[i] * hp[i];
+                    const float margin = r + off;
+                    uint8_t sd;
+                    if (fabsf(margin) < 1e-8f) {
+                        sd = 2;
+                        tie = true;
+                    } else {
+                        sd = margin < 0.0f ? 0 : 1;
+                    }
+                    wv_side[lo + p] = sd;
+                }
+            }
+            tie = __ballot(tie) != 0ull;
+            wave_sync();
+            int split = 0;
+            if (tie) {
+                if (lane == 0) {  // partition_inplace verbatim (builder.rs:319-347)
+                    Xoshiro r;
+                    for (int w = 0; w < 4; ++w) r.s[w] = rs[w];
+                    int left = 0, right = cm;
+                    while (left < right) {
+                        const uint8_t sd = wv_side[lo + left];
+                        const bool goes_left = sd == 2 ? !r.next_bool() : sd == 0;
+                        if (goes_left) {
+                            ++left;
+                        } else {
+                            --right;
+                            const int32_t ti = wv_idx[lo + left];
+                            wv_idx[lo + left] = wv_idx[lo + right];
+                            wv_idx[lo + right] = ti;
+                            const uint8_t ts = wv_side[lo + left];
+                            wv_side[lo + left] = wv_side[lo + right];
+                            wv_side[lo + right] = ts;
+                        }
+                    }
+                    for (int w = 0; w < 4; ++w) rs[w] = r.s[w];
+                    split = left;
+                }
+                split = __shfl(split, 0);
+                wave_sync();
+            } else {  // the closed form of the two-pointer partition, as in the workgroup walk
+                int running = 0;
+                for (int b = 0; b < cm; b += RINND_WAVE) {
+                    const int p = b + lane;
+                    const bool isR = p < cm && wv_side[lo + p] == 1;
+                    const unsigned long long bal = __ballot(isR);
+                    if (p < cm) wv_rank[p] = running + __popcll(bal & lt);
+                    running += __popcll(bal);
+                }
+                const int totR = running;
+                split = cm - totR;
+                if (lane == 0) wv_rank[cm] = totR;
+                wave_sync();
+                for (int p = lane; p < cm; p += RINND_WAVE) {
+                    const uint8_t sd = wv_side[lo + p];
+                    if (sd == 1 && p < split) wv_P[wv_rank[p]] = p;
+                    if (sd == 0 && p >= split) wv_Q[(cm - p - 1) - (totR - wv_rank[p + 1])] = p;
+                }
+                wave_sync();
+                for (int p = lane; p < cm; p += RINND_WAVE) {
+                    const uint8_t sd = wv_side[lo + p];
+                    int dest;
+                    if (sd == 0) {
+                        dest = p < split ? p : wv_P[(cm - p - 1) - (totR - wv_rank[p + 1])];
+                    } else {
+                        const int rr = wv_rank[p];
+                        dest = p <= split ? ((rr == 0 ? cm : wv_Q[rr - 1]) - 1) : p - 1;
+                    }
+                    wv_tmp[dest] = wv_idx[lo + p];
+                }
+                wave_sync();
+                for (int p = lane; p < cm; p += RINND_WAVE) wv_idx[lo + p] = wv_tmp[p];
+                wave_sync();
+            }
+            if (split == 0 || split == cm) {  // degenerate split -> leaf (builder.rs:198-210)
+                if (lane == 0) {
+                    ch[2 * node] = -c.start;
+                    ch[2 * node + 1] = -c.end;
+                }
+            } else {
+                for (int i = lane; i < dim; i += RINND_WAVE) hyper[(int64_t)node * dim + i] = hp[i];
+                if (sp + 2 > kForestStack) {
+                    if (lane == 0) sh_abort = 2;
+                    break;
+                }
+                if (lane == 0) {
+                    offs[node] = off;
+                    stack[sp] = StackEnt{c.start + split, c.end, c.depth + 1, 2 * node + 1};
+                    stack[sp + 1] = StackEnt{c.start, c.start + split, c.depth + 1, 2 * node};
+                }
+                sp += 2;
+            }
+            wave_sync();
+        }
+        wave_sync();
+        for (int i = lane; i < M; i += RINND_WAVE) idx[o0 + i] = wv_idx[i];
+        if (lane == 0) sh_sp = sp_end;
+    };
+
     for (;;) {
         if (sh_sp == 0 || sh_abort) break;
         const StackEnt e = stack[sh_sp - 1];
         __syncthreads();
         const int m = e.end - e.start;
+        if constexpr (WAVE_SUB) {
+            if (m <= kWaveSub && m > a.leaf_size && e.depth < a.max_depth) {  // uniform
+                if (wid == 0) subtree_wave();
+                __syncthreads();
+                continue;
+            }
+        }
         if (tid == 0) {
             sh_sp -= 1;
             const int node = sh_ctr;
@@ -376,12 +620,14 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
     __syncthreads();  // the workgroup's shared state is reused by its next tree
 }
 
-// One workgroup per tree, or (a.queue) persistent workgroups pulling trees from a work queue, largest first.
+// One workgroup of 1024 threads per tree.
 __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) {
-    if (!a.queue) {
-        forest_tree_dfs(a, blockIdx.x, (int64_t)blockIdx.x * (a.n + 1));
-        return;
-    }
+    forest_tree_dfs<kForestThreads, false>(a, blockIdx.x, (int64_t)blockIdx.x * (a.n + 1));
+}
+// Persistent workgroups of 256 threads pulling trees from a work queue, largest first (many trees: more of them
+// in flight per CU).
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_forest_dfs_queue(ForestDFSArgs a) {
     __shared__ int sh_item;
     for (;;) {
         if (threadIdx.x == 0) sh_item = atomicAdd(a.queue, 1);
@@ -389,14 +635,30 @@ __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) 
         const int item = sh_item;
         __syncthreads();
         if (item >= a.n_trees) break;  // uniform: every workgroup ends once the queue is drained
-        forest_tree_dfs(a, a.order[item], (int64_t)blockIdx.x * a.slot_stride);
+        forest_tree_dfs<THREADS, true>(a, a.order[item], (int64_t)blockIdx.x * a.slot_stride);
     }
+}
+constexpr int kForestQueueThreads = 256;
+
+int forest_queue_grid(int dim) {
+    int per_cu = 0;
+    const size_t lds = (size_t)3 * dim * sizeof(float);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_forest_dfs_queue<kForestQueueThreads>,
+                                                     kForestQueueThreads, lds) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    return per_cu * device_cus();
 }
 
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a) {
     const size_t lds = (size_t)3 * a.dim * sizeof(float);
-    const unsigned grid = a.queue ? (unsigned)std::max(1, std::min(a.grid, a.n_trees)) : (unsigned)a.n_trees;
-    hipLaunchKernelGGL(k_forest_dfs, dim3(grid), dim3(kForestThreads), lds, s, a);
+    if (a.queue) {
+        const unsigned grid = (unsigned)std::max(1, std::min(a.grid, a.n_trees));
+        hipLaunchKernelGGL(k_forest_dfs_queue<kForestQueueThreads>, dim3(grid), dim3(kForestQueueThreads), lds, s, a);
+    } else {
+        hipLaunchKernelGGL(k_forest_dfs, dim3(a.n_trees), dim3(kForestThreads), lds, s, a);
+    }
 }
 
 // ---- leaves in depth-first order -------------------------------------------------------------------
