@@ -2,6 +2,11 @@
 
     python -m rinnd_amd.profiling stats <run_kernel_stats.csv>
     python -m rinnd_amd.profiling traffic <fetch run_counter_collection.csv> <write run_counter_collection.csv>
+    python -m rinnd_amd.profiling counters <run_counter_collection.csv>
+
+`counters` prints, per timed kernel family, the mean of every collected counter per launch, and for the MFMA
+counters the MFMA flops per launch (SQ_INSTS_VALU_MFMA_MOPS_* count units of 512 flops); divided by the mean
+launch duration of `stats` they give the MFMA rate against the dense peak.
 
 `stats` prints the kernels with at least 0.1 % of the kernel time.  `traffic` prints, per timed kernel
 family, the mean HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the
@@ -77,7 +82,33 @@ def traffic(fetch_csv: str, write_csv: str) -> list[str]:
     return out
 
 
+def counters(path: str) -> list[str]:
+    per: dict[int, dict] = {}
+    for r in csv.DictReader(open(path)):
+        d = per.setdefault(int(r["Dispatch_Id"]), {"name": r["Kernel_Name"]})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    fams = dict(FAMILIES)
+    fams["search"] = ("k_search", r"k_search")
+    out = []
+    for fam, (name, pat) in fams.items():
+        ds = [d for d in per.values() if re.search(pat, d["name"])]
+        if not ds:
+            continue
+        keys = sorted({k for d in ds for k in d if k != "name"})
+        mean = {k: sum(d.get(k, 0.0) for d in ds) / len(ds) for k in keys}
+        line = f"counters kernel={name} launches={len(ds)} " + " ".join(f"{k}={v:.4e}" for k, v in mean.items())
+        for kind in ("BF16", "F8"):
+            mops = mean.get(f"SQ_INSTS_VALU_MFMA_MOPS_{kind}")
+            if mops:
+                line += f" mfma_{kind.lower()}_flops={mops * 512:.4e}"
+        out.append(line)
+    return out
+
+
 def main(argv: list[str]) -> int:
+    if len(argv) == 2 and argv[0] == "counters":
+        print("\n".join(counters(argv[1])))
+        return 0
     if len(argv) == 2 and argv[0] == "stats":
         print("\n".join(kernel_stats(argv[1])))
         return 0
