@@ -210,6 +210,21 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
         a.rankR = rankR.p;
         a.PQ = PQ.p;
         a.tmp = tmp.p;
+        // Several workgroups per tree when every tree's group fits on the device at once: the walking workgroup
+        // hands its large nodes' side tests and partitions to the group.
+        const int G = forest_coop_groups(dim, (int)n_trees);
+        const int64_t coop_min = std::max<int64_t>(2048, 128 * (int64_t)G);
+        DBuf<CoopCtl> coop_ctl;
+        DBuf<float> coop_hp;
+        if (G >= 2 && n >= 2 * coop_min) {
+            coop_ctl.alloc(n_trees);
+            coop_hp.alloc((size_t)n_trees * dim);
+            launch_zero(s, coop_ctl.p, n_trees * sizeof(CoopCtl));
+            a.coop = coop_ctl.p;
+            a.coop_hp = coop_hp.p;
+            a.coop_groups = G;
+            a.coop_min = coop_min;
+        }
         launch_forest_dfs(s, a);
         HIP_TRY(hipGetLastError());
         std::vector<int64_t> nn(n_trees);
@@ -217,6 +232,14 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
         HIP_TRY(hipMemcpyAsync(nn.data(), d_nn.p, n_trees * 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(st.data(), d_status.p, n_trees * 4, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        if (a.coop && std::find(st.begin(), st.end(), 3) != st.end()) {  // a group did not come together: one each
+            a.coop = nullptr;
+            launch_forest_dfs(s, a);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(nn.data(), d_nn.p, n_trees * 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(st.data(), d_status.p, n_trees * 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
         bool overflow = false;
         for (int64_t t = 0; t < n_trees; ++t) {
             if (st[t] == 2) throw Error{RINND_ERR_INTERNAL, "RP tree stack overflow"};

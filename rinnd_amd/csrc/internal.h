@@ -239,8 +239,27 @@ struct ForestDFSArgs {
     int32_t* queue;
     int64_t slot_stride;
     int grid;
+    // Cooperative form (coop non-null): coop_groups workgroups per tree; the tree's first walks it and the others
+    // join every node of at least coop_min points (margins and partition split into contiguous slices).
+    struct CoopCtl* coop;   // one per tree, zeroed
+    float* coop_hp;         // dim floats per tree: the node's hyperplane, published by the walking workgroup
+    int coop_groups;
+    int64_t coop_min;
+};
+// The control block of one tree's cooperating workgroups.  Every wait on it is bounded: after about four
+// seconds without progress a workgroup sets `abort` and the tree reports status 3 (the caller reruns it alone).
+constexpr int kCoopMaxGroups = 32;
+struct CoopCtl {
+    int seq, done, abort, bar;
+    int start, end;
+    float off;
+    int pad;
+    int cnt[kCoopMaxGroups];
+    int tie[kCoopMaxGroups];
 };
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a);
+// workgroups per tree for the cooperative forest kernel with n_trees trees (0: not all of them fit at once)
+int forest_coop_groups(int dim, int n_trees);
 // resident workgroups of the work-queue forest kernel on this device (the grid for a.queue launches)
 int forest_queue_grid(int dim);
 

@@ -111,9 +111,289 @@ struct StackEnt {
 // workgroup walk.
 constexpr int kWaveSub = 1024;
 
-template <int THREADS, bool WAVE_SUB>
+// ---- partition_inplace with margin ties (builder.rs:319-347) on one wave ------------------------------------
+// The reference's two pointers examine the node's elements in an order fixed by the decisions: the first
+// element from the front; after an element goes left the next front element, after one goes right the next
+// element from the back.  A left-goer lands at (lefts so far), a right-goer at m - 1 - (rights so far), and a
+// tie takes the next draw (go left iff !next_bool()) in that order.  Wave 0 replays the order with 64-element
+// windows of both ends in registers, writes every element's final position to dst[] and advances rs; returns
+// the number of left-goers.  Every lane runs the same (uniform) steps; lane 0 writes.
+__device__ int tie_partition_wave(const uint8_t* __restrict__ side, int m, uint64_t* rs, int32_t* dst) {
+    const int lane = threadIdx.x & 63;
+    Xoshiro r;
+    for (int w = 0; w < 4; ++w) r.s[w] = rs[w];
+    int L = 0, R = 0, fi = 0, bj = 0;
+    bool fromF = true;
+    int fv = lane < m ? side[lane] : 0;            // front window: side[fi + lane]
+    int bv = m - 1 - lane >= 0 ? side[m - 1 - lane] : 0;  // back window: side[m - 1 - (bj + lane)]
+    for (int k = 0; k < m; ++k) {
+        int sd, p;
+        if (fromF) {
+            if (L - fi >= 64) {
+                fi += 64;
+                fv = fi + lane < m ? side[fi + lane] : 0;
+            }
+            sd = __builtin_amdgcn_readlane(fv, __builtin_amdgcn_readfirstlane(L - fi));
+            p = L;
+        } else {
+            const int j = R - 1;
+            if (j - bj >= 64) {
+                bj += 64;
+                bv = m - 1 - (bj + lane) >= 0 ? side[m - 1 - (bj + lane)] : 0;
+            }
+            sd = __builtin_amdgcn_readlane(bv, __builtin_amdgcn_readfirstlane(j - bj));
+            p = m - 1 - j;
+        }
+        const bool left = sd == 2 ? !r.next_bool() : sd == 0;
+        if (lane == 0) dst[p] = left ? L : m - 1 - R;
+        if (left) {
+            ++L;
+            fromF = true;
+        } else {
+            ++R;
+            fromF = false;
+        }
+    }
+    if (lane == 0)
+        for (int w = 0; w < 4; ++w) rs[w] = r.s[w];
+    return L;
+}
+
+// The same with few ties (thread 0): only the ties need the examination order.  With every decision before a
+// tie known, the step at which the two pointers reach it has a closed form: the front element at position x
+// (r right-goers before it in front order) is examined after the first r back runs, i.e. after the back
+// elements up to the r-th left-goer from the end; the back element at position t (l left-goers behind it) after
+// the front elements up to the (l+1)-th right-goer.  The undecided ties always form a contiguous range of the
+// sorted tie positions (the next one examined is the first or the last of them): both candidates' steps are
+// computed (a candidate is blocked when an undecided tie would be examined on the way), the earlier takes the
+// next draw.  cntR[p] = right-goers (side 1) before p, p in [0, m]; ties[] ascending; dec[i] = 0 left / 1 right.
+constexpr int kMaxTies = 64;
+__device__ void tie_resolve(const int32_t* __restrict__ cntR, int m, const int* ties, int T, int8_t* dec,
+                            uint64_t* rs) {
+    for (int i = 0; i < T; ++i) dec[i] = -1;
+    auto cntT = [&](int p) {  // ties before p
+        int c = 0;
+        for (int i = 0; i < T; ++i) c += ties[i] < p;
+        return c;
+    };
+    auto cntL = [&](int p) { return p - cntR[p] - cntT(p); };  // side-0 elements before p
+    const int totL = cntL(m);
+    auto g = [&](int q) {  // known left-goers in [q, m)
+        int c = totL - cntL(q);
+        for (int i = 0; i < T; ++i) c += dec[i] == 0 && ties[i] >= q;
+        return c;
+    };
+    auto h = [&](int p) {  // known right-goers in [0, p)
+        int c = cntR[p];
+        for (int i = 0; i < T; ++i) c += dec[i] == 1 && ties[i] < p;
+        return c;
+    };
+    Xoshiro r;
+    for (int w = 0; w < 4; ++w) r.s[w] = rs[w];
+    int lo_i = 0, hi_i = T - 1;
+    while (lo_i <= hi_i) {
+        const int tlo = ties[lo_i], thi = ties[hi_i];
+        int fpos = -1, bpos = -1;
+        {  // tlo from the front
+            const int rr = h(tlo);
+            if (rr == 0) {
+                fpos = tlo;
+            } else if (g(0) >= rr) {
+                int lo = 0, hi = m;  // the rr-th left-goer from the end: max { q : g(q) >= rr }
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (g(mid) >= rr) lo = mid;
+                    else hi = mid - 1;
+                }
+                if (thi <= lo) fpos = tlo + (m - lo);
+            }
+        }
+        {  // thi from the back
+            const int need = g(thi + 1) + 1;
+            if (h(m) >= need) {
+                int lo = 1, hi = m;  // the need-th right-goer from the front sits at min { p : h(p) >= need } - 1
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (h(mid) >= need) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int ps = lo - 1;
+                if (tlo >= ps) bpos = (m - 1 - thi) + ps + 1;
+            }
+        }
+        const int d = r.next_bool() ? 1 : 0;  // goes left iff !next_bool()
+        if (fpos >= 0 && (bpos < 0 || fpos < bpos)) dec[lo_i++] = (int8_t)d;
+        else dec[hi_i--] = (int8_t)d;
+    }
+    for (int w = 0; w < 4; ++w) rs[w] = r.s[w];
+}
+
+// ---- cooperating workgroups of one tree ----------------------------------------------------------------
+// Waits are bounded by the constant-rate wall clock (wall_clock64, 100 MHz on this hardware): about four
+// seconds without progress sets the tree's abort flag, which every waiting workgroup sees and leaves on.
+constexpr uint64_t kCoopTimeoutTicks = 400000000ull;
+
+// thread 0 only: wait until *p >= target; false on abort or timeout.  The polls are relaxed (an acquire per
+// poll would invalidate the caches the other workgroups are using); one acquire fence follows success.
+__device__ bool coop_wait(int* p, int target, int* abortp) {
+    const uint64_t t0 = wall_clock64();
+    for (;;) {
+        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return true;
+        }
+        if (__hip_atomic_load(abortp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (wall_clock64() - t0 > kCoopTimeoutTicks) {
+            __hip_atomic_store(abortp, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+}
+
+// All G workgroups of the tree: every wave waits for its own stores, then thread 0 releases the workgroup's
+// writes at device scope (one L2 write-back per workgroup) and counts in; false (in every thread) on abort.
+__device__ bool coop_barrier(CoopCtl* c, int G, int* sh_nb, int* sh_ok) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(&c->bar, 1);
+        const int nb = *sh_nb + 1;
+        *sh_nb = nb;
+        *sh_ok = coop_wait(&c->bar, G * nb, &c->abort) ? 1 : 0;
+    }
+    __syncthreads();
+    return *sh_ok != 0;
+}
+
+// One node's side tests and partition by workgroup g of G (contiguous slices of the node's m positions), the
+// same closed form as the single-workgroup walk: returns the split, -1 if some margin is a tie (side[] then
+// holds every position's side and the walking workgroup partitions alone), -2 on abort.
+template <int THREADS>
+__device__ int coop_node(const float* X, int64_t ld, int dim, const float* hp, float off, int start, int m, int g,
+                         int G, CoopCtl* c, int32_t* idx, uint8_t* side, int32_t* rankR, int32_t* P, int32_t* Q,
+                         int32_t* tmp, int* sh_nb) {
+    __shared__ int cw_tot[THREADS / RINND_WAVE];
+    __shared__ int cw_R, cw_tie, cw_ok, cw_base, cw_totR;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int slice = (int)round_up(ceil_div(m, G), 64);
+    const int s0 = min(m, g * slice), s1 = min(m, s0 + slice);
+    if (tid == 0) cw_R = cw_tie = 0;
+    __syncthreads();
+    {  // margins: two lanes per point (as the single-workgroup walk)
+        const int pg = tid >> 1, h = tid & 1;
+        const int chunks = dim >> 3;
+        int myR = 0, mytie = 0;
+        for (int base = s0; base < s1; base += THREADS / 2) {
+            const int pos = base + pg;
+            const bool act = pos < s1;
+            const float* x = X + (int64_t)(act ? idx[start + pos] : 0) * ld;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+            if (act)
+                for (int cc = 0; cc < chunks; ++cc) {
+                    const float4 u = *reinterpret_cast<const float4*>(x + cc * 8 + 4 * h);
+                    const float4 w = *reinterpret_cast<const float4*>(hp + cc * 8 + 4 * h);
+                    a0 = __builtin_fmaf(u.x, w.x, a0);
+                    a1 = __builtin_fmaf(u.y, w.y, a1);
+                    a2 = __builtin_fmaf(u.z, w.z, a2);
+                    a3 = __builtin_fmaf(u.w, w.w, a3);
+                }
+            const float b0 = __shfl_xor(a0, 1), b1 = __shfl_xor(a1, 1);
+            const float b2 = __shfl_xor(a2, 1), b3 = __shfl_xor(a3, 1);
+            if (act && h == 0) {
+                float r = ((a0 + b0) + (a1 + b1)) + ((a2 + b2) + (a3 + b3));
+                for (int i = chunks * 8; i < dim; ++i) r += x[i] * hp[i];
+                const float margin = r + off;
+                uint8_t sd;
+                if (fabsf(margin) < 1e-8f) {
+                    sd = 2;
+                    mytie = 1;
+                } else {
+                    sd = margin < 0.0f ? 0 : 1;
+                }
+                side[pos] = sd;
+                myR += sd == 1;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) myR += __shfl_xor(myR, o);
+        const unsigned long long anyt = __ballot(mytie != 0);
+        if (lane == 0) {
+            atomicAdd(&cw_R, myR);
+            if (anyt) atomicOr(&cw_tie, 1);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            c->cnt[g] = cw_R;
+            c->tie[g] = cw_tie;
+        }
+    }
+    if (!coop_barrier(c, G, sh_nb, &cw_ok)) return -2;
+    if (tid == 0) {
+        int tie = 0, base = 0, tot = 0;
+        for (int q = 0; q < G; ++q) {
+            const int v = __hip_atomic_load(&c->cnt[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            tie |= __hip_atomic_load(&c->tie[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (q < g) base += v;
+            tot += v;
+        }
+        cw_tie = tie;
+        cw_base = base;
+        cw_totR = tot;
+    }
+    __syncthreads();
+    if (cw_tie) {  // every workgroup has read the flags before the walking one moves on and rewrites them
+        if (!coop_barrier(c, G, sh_nb, &cw_ok)) return -2;
+        return -1;
+    }
+    const int totR = cw_totR, split = m - totR;
+    {  // rankR (rights before p) and the wrong-side lists P / Q of the closed form
+        int running = cw_base;
+        for (int cb = s0; cb < s1; cb += THREADS) {  // uniform
+            const int p = cb + tid;
+            const uint8_t sd = p < s1 ? side[p] : 0;
+            const bool isR = sd == 1;
+            const unsigned long long bal = __ballot(isR);
+            if (lane == 0) cw_tot[wid] = __popcll(bal);
+            __syncthreads();
+            int wpre = 0, tot = 0;
+            for (int w = 0; w < THREADS / RINND_WAVE; ++w) {
+                wpre += w < wid ? cw_tot[w] : 0;
+                tot += cw_tot[w];
+            }
+            if (p < s1) {
+                const int r = running + wpre + __popcll(bal & lt);
+                rankR[p] = r;
+                const int rn = r + (isR ? 1 : 0);
+                if (sd == 1 && p < split) P[r] = p;
+                if (sd == 0 && p >= split) Q[(m - p - 1) - (totR - rn)] = p;
+            }
+            __syncthreads();
+            running += tot;
+        }
+    }
+    if (!coop_barrier(c, G, sh_nb, &cw_ok)) return -2;
+    for (int p = s0 + tid; p < s1; p += THREADS) {  // destinations
+        const uint8_t sd = side[p];
+        const int r = rankR[p];
+        int dest;
+        if (sd == 0) {
+            dest = p < split ? p : P[(m - p - 1) - (totR - r)];
+        } else {
+            dest = p <= split ? ((r == 0 ? m : Q[r - 1]) - 1) : p - 1;
+        }
+        tmp[dest] = idx[start + p];
+    }
+    if (!coop_barrier(c, G, sh_nb, &cw_ok)) return -2;
+    for (int p = s0 + tid; p < s1; p += THREADS) idx[start + p] = tmp[p];
+    if (!coop_barrier(c, G, sh_nb, &cw_ok)) return -2;
+    return split;
+}
+
+template <int THREADS, bool WAVE_SUB, bool COOP = false>
 __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64_t so) {
-    extern __shared__ float fsm[];  // hp[dim], p1[dim], p2[dim]
+    extern __shared__ __align__(16) float fsm[];  // hp[dim], p1[dim], p2[dim]
     __shared__ StackEnt stack[kForestStack];
     __shared__ int32_t wv_idx[kWaveSub], wv_tmp[kWaveSub], wv_rank[kWaveSub + 1], wv_P[kWaveSub], wv_Q[kWaveSub];
     __shared__ uint8_t wv_side[kWaveSub];
@@ -123,6 +403,9 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
     __shared__ float sh_off, sh_inv, sh_n1, sh_n2;
     __shared__ int wave_tot[THREADS / RINND_WAVE];
     __shared__ int sh_running;
+    __shared__ int sh_seq, sh_nb;  // cooperative form: nodes published, barriers passed
+    __shared__ int sh_ntie, sh_ties[kMaxTies];
+    __shared__ int8_t sh_dec[kMaxTies];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
@@ -150,6 +433,8 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
         sh_sp = 1;
         sh_ctr = 0;
         sh_abort = 0;
+        sh_seq = 0;
+        sh_nb = 0;
     }
     __syncthreads();
 
@@ -486,24 +771,67 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
             }
         }
         __syncthreads();
-        // margins: 8 lanes per point in the AVX2 lane order (dot_product_avx2, builder.rs:29-60)
-        {
+        bool coop_done = false;
+        if constexpr (COOP) {
+            if (m >= a.coop_min) {  // uniform: the tree's other workgroups join this node
+                CoopCtl* cc = a.coop + t;
+                float* chp = a.coop_hp + (int64_t)t * dim;
+                for (int i = tid; i < dim; i += blockDim.x) chp[i] = hp[i];
+                if (tid == 0) {
+                    cc->start = e.start;
+                    cc->end = e.end;
+                    cc->off = sh_off;
+                }
+                __threadfence();
+                __syncthreads();
+                if (tid == 0) {
+                    sh_seq += 1;
+                    __hip_atomic_store(&cc->seq, sh_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                const int r = coop_node<THREADS>(X, a.ld, dim, hp, sh_off, e.start, m, 0, a.coop_groups, cc, idx, side,
+                                                 rankR, P, Q, tmp, &sh_nb);
+                if (r == -2) {  // uniform
+                    if (tid == 0) sh_abort = 3;
+                    __syncthreads();
+                    break;
+                }
+                if (tid == 0) {
+                    sh_tie = r == -1 ? 1 : 0;
+                    sh_split = r;
+                }
+                __syncthreads();
+                coop_done = true;
+            }
+        }
+        // margins: two lanes per point, lane h holding the AVX2 accumulators 4h .. 4h+3 of the point's 8-wide
+        // blocks (16-byte loads, a point's whole row in flight), combined in the AVX2 reduction order
+        // (dot_product_avx2, builder.rs:29-60)
+        if (!coop_done) {
             const float off = sh_off;
-            const int g = tid >> 3, l = tid & 7;
-            const int groups = blockDim.x >> 3;
+            const int g = tid >> 1, h = tid & 1;
+            const int groups = blockDim.x >> 1;
             const int chunks = dim >> 3;
+            int32_t nxt = e.start + g < e.end ? idx[e.start + g] : 0;  // the next point's id, one step ahead
             for (int base = e.start; base < e.end; base += groups) {
                 const int pos = base + g;
                 const bool act = pos < e.end;
-                float acc = 0.f;
-                const float* x = X + (int64_t)(act ? idx[pos] : 0) * a.ld;
+                const int32_t cur = nxt;
+                nxt = pos + groups < e.end ? idx[pos + groups] : 0;
+                const float* x = X + (int64_t)cur * a.ld;
+                float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
                 if (act)
-                    for (int c = 0; c < chunks; ++c) acc = __builtin_fmaf(x[c * 8 + l], hp[c * 8 + l], acc);
-                const float s4 = acc + __shfl_xor(acc, 4);
-                const float s1 = s4 + __shfl_xor(s4, 1);
-                const float tot = s1 + __shfl_xor(s1, 2);
-                if (act && l == 0) {
-                    float r = tot;
+                    for (int c = 0; c < chunks; ++c) {
+                        const float4 u = *reinterpret_cast<const float4*>(x + c * 8 + 4 * h);
+                        const float4 w = *reinterpret_cast<const float4*>(hp + c * 8 + 4 * h);
+                        a0 = __builtin_fmaf(u.x, w.x, a0);
+                        a1 = __builtin_fmaf(u.y, w.y, a1);
+                        a2 = __builtin_fmaf(u.z, w.z, a2);
+                        a3 = __builtin_fmaf(u.w, w.w, a3);
+                    }
+                const float b0 = __shfl_xor(a0, 1), b1 = __shfl_xor(a1, 1);
+                const float b2 = __shfl_xor(a2, 1), b3 = __shfl_xor(a3, 1);
+                if (act && h == 0) {
+                    float r = ((a0 + b0) + (a1 + b1)) + ((a2 + b2) + (a3 + b3));
                     for (int i = chunks * 8; i < dim; ++i) r += x[i] * hp[i];
                     const float margin = r + off;
                     uint8_t sd;
@@ -518,32 +846,8 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
             }
         }
         __syncthreads();
-        if (sh_tie) {
-            if (tid == 0) {  // partition_inplace verbatim (builder.rs:319-347)
-                Xoshiro r;
-                for (int w = 0; w < 4; ++w) r.s[w] = rs[w];
-                int left = 0, right = m;
-                while (left < right) {
-                    const uint8_t sd = side[left];
-                    const bool goes_left = sd == 2 ? !r.next_bool() : sd == 0;
-                    if (goes_left) {
-                        ++left;
-                    } else {
-                        --right;
-                        const int32_t ti = idx[e.start + left];
-                        idx[e.start + left] = idx[e.start + right];
-                        idx[e.start + right] = ti;
-                        const uint8_t ts = side[left];
-                        side[left] = side[right];
-                        side[right] = ts;
-                    }
-                }
-                for (int w = 0; w < 4; ++w) rs[w] = r.s[w];
-                sh_split = left;
-            }
-            __syncthreads();
-        } else {
-            // rankR[p] = number of right elements before p (exclusive scan of 1-bit flags)
+        // rankR[p] = number of right elements (side 1) before p, p in [0, m] (exclusive scan of 1-bit flags)
+        auto rank_pass = [&]() {
             if (tid == 0) sh_running = 0;
             __syncthreads();
             for (int base = 0; base < m; base += blockDim.x) {
@@ -565,10 +869,14 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
                 }
                 __syncthreads();
             }
+            if (tid == 0) rankR[m] = sh_running;
+            __syncthreads();
+        };
+        // the two-pointer partition of sides 0 / 1 in closed form
+        auto closed_form = [&]() {
+            rank_pass();
             const int totR = sh_running;
             const int split = m - totR;
-            if (tid == 0) rankR[m] = totR;
-            __syncthreads();
             for (int p = tid; p < m; p += blockDim.x) {
                 const uint8_t sd = side[p];
                 if (sd == 1 && p < split) P[rankR[p]] = p;
@@ -590,6 +898,44 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
             for (int p = tid; p < m; p += blockDim.x) idx[e.start + p] = tmp[p];
             if (tid == 0) sh_split = split;
             __syncthreads();
+        };
+        if (sh_tie) {  // partition_inplace with margin ties (builder.rs:319-347)
+            if (tid == 0) sh_ntie = 0;
+            __syncthreads();
+            for (int p = tid; p < m; p += blockDim.x)
+                if (side[p] == 2) {
+                    const int slot = atomicAdd(&sh_ntie, 1);
+                    if (slot < kMaxTies) sh_ties[slot] = p;
+                }
+            __syncthreads();
+            if (sh_ntie <= kMaxTies) {  // few ties: decide them in examination order, then the closed form
+                rank_pass();
+                if (tid == 0) {
+                    const int T = sh_ntie;
+                    for (int i = 1; i < T; ++i)  // ascending positions
+                        for (int j = i; j > 0 && sh_ties[j - 1] > sh_ties[j]; --j) {
+                            const int x = sh_ties[j];
+                            sh_ties[j] = sh_ties[j - 1];
+                            sh_ties[j - 1] = x;
+                        }
+                    tie_resolve(rankR, m, sh_ties, T, sh_dec, rs);
+                    for (int i = 0; i < T; ++i) side[sh_ties[i]] = (uint8_t)sh_dec[i];
+                }
+                __syncthreads();
+                closed_form();
+            } else {  // many ties: replay the examination order element by element on wave 0
+                if (wid == 0) {
+                    const int left = tie_partition_wave(side, m, rs, rankR);
+                    if (tid == 0) sh_split = left;
+                }
+                __syncthreads();
+                for (int p = tid; p < m; p += blockDim.x) tmp[rankR[p]] = idx[e.start + p];
+                __syncthreads();
+                for (int p = tid; p < m; p += blockDim.x) idx[e.start + p] = tmp[p];
+                __syncthreads();
+            }
+        } else if (!coop_done) {
+            closed_form();
         }
         const int split = sh_split;
         if (split == 0 || split == m) {  // degenerate split -> leaf (builder.rs:198-210)
@@ -616,13 +962,84 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
     if (tid == 0) {
         a.n_nodes_out[t] = sh_ctr;
         a.status[t] = sh_abort;
+        if constexpr (COOP) {  // the walk is over: the other workgroups leave
+            __threadfence();
+            __hip_atomic_store(&a.coop[t].done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     __syncthreads();  // the workgroup's shared state is reused by its next tree
+}
+
+// Workgroup g >= 1 of a tree in the cooperative form: joins each node the walking workgroup publishes.
+template <int THREADS>
+__device__ void forest_coop_helper(const ForestDFSArgs& a, const int t, const int g) {
+    extern __shared__ __align__(16) float fsm[];
+    __shared__ int hs_go, hs_nb, hs_start, hs_end;
+    __shared__ float hs_off;
+    const int tid = threadIdx.x;
+    const int64_t n = a.n;
+    const int dim = a.dim;
+    CoopCtl* cc = a.coop + t;
+    const float* chp = a.coop_hp + (int64_t)t * dim;
+    const int64_t so = (int64_t)t * (n + 1);
+    int32_t* idx = a.indices[t];
+    uint8_t* side = a.side + so;
+    int32_t* rankR = a.rankR + so;
+    int32_t* P = a.PQ + 2 * so;
+    int32_t* Q = P + (n + 1);
+    int32_t* tmp = a.tmp + so;
+    float* hp = fsm;
+    if (tid == 0) hs_nb = 0;
+    int last = 0;
+    for (;;) {
+        if (tid == 0) {  // the next published node, or the end of the walk
+            const uint64_t t0 = wall_clock64();
+            int go = 0;
+            for (;;) {  // relaxed polls, one acquire fence on success (as coop_wait)
+                if (__hip_atomic_load(&cc->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > last) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    go = 1;
+                    break;
+                }
+                if (__hip_atomic_load(&cc->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    __hip_atomic_load(&cc->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+                if (wall_clock64() - t0 > kCoopTimeoutTicks) {
+                    __hip_atomic_store(&cc->abort, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+            if (go) {
+                hs_start = __hip_atomic_load(&cc->start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                hs_end = __hip_atomic_load(&cc->end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                hs_off = __hip_atomic_load(&cc->off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            hs_go = go;
+        }
+        __syncthreads();
+        if (!hs_go) break;  // uniform
+        ++last;
+        for (int i = tid; i < dim; i += blockDim.x) hp[i] = chp[i];
+        __syncthreads();
+        const int r = coop_node<THREADS>(a.X, a.ld, dim, hp, hs_off, hs_start, hs_end - hs_start, g, a.coop_groups, cc,
+                                         idx, side, rankR, P, Q, tmp, &hs_nb);
+        __syncthreads();  // hp and the node fields are rewritten for the next node
+        if (r == -2) break;
+    }
 }
 
 // One workgroup of 1024 threads per tree.
 __global__ __launch_bounds__(kForestThreads) void k_forest_dfs(ForestDFSArgs a) {
     forest_tree_dfs<kForestThreads, false>(a, blockIdx.x, (int64_t)blockIdx.x * (a.n + 1));
+}
+// Cooperative form: coop_groups workgroups per tree (the first walks the tree, the others join its large nodes).
+__global__ __launch_bounds__(kForestThreads) void k_forest_dfs_coop(ForestDFSArgs a) {
+    const int t = blockIdx.x / a.coop_groups, g = blockIdx.x - t * a.coop_groups;
+    if (g == 0)
+        forest_tree_dfs<kForestThreads, false, true>(a, t, (int64_t)t * (a.n + 1));
+    else
+        forest_coop_helper<kForestThreads>(a, t, g);
 }
 // Persistent workgroups of 256 threads pulling trees from a work queue, largest first (many trees: more of them
 // in flight per CU).
@@ -651,9 +1068,23 @@ int forest_queue_grid(int dim) {
     return per_cu * device_cus();
 }
 
+int forest_coop_groups(int dim, int n_trees) {
+    int per_cu = 0;
+    const size_t lds = (size_t)3 * dim * sizeof(float);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_forest_dfs_coop, kForestThreads, lds) != hipSuccess ||
+        per_cu < 1) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const int G = std::min(kCoopMaxGroups, per_cu * device_cus() / std::max(n_trees, 1));
+    return G >= 2 ? G : 0;
+}
+
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a) {
     const size_t lds = (size_t)3 * a.dim * sizeof(float);
-    if (a.queue) {
+    if (a.coop) {
+        hipLaunchKernelGGL(k_forest_dfs_coop, dim3(a.n_trees * a.coop_groups), dim3(kForestThreads), lds, s, a);
+    } else if (a.queue) {
         const unsigned grid = (unsigned)std::max(1, std::min(a.grid, a.n_trees));
         hipLaunchKernelGGL(k_forest_dfs_queue<kForestQueueThreads>, dim3(grid), dim3(kForestQueueThreads), lds, s, a);
     } else {

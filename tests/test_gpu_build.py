@@ -55,7 +55,9 @@ def gpu_forest(ctx, X, n_trees, leaf, seed, angular=False):
 
 
 @pytest.mark.parametrize("angular", [False, True])
-@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (2000, 32, 20), (3000, 37, 60)])
+@pytest.mark.parametrize("n,d,leaf", [(1, 8, 5), (50, 3, 4), (2000, 32, 20), (3000, 37, 60),
+                                      # large enough for several workgroups per tree on the large nodes
+                                      (12000, 33, 100), (40000, 64, 60)])
 def test_forest_matches_reference_streams(ctx, n, d, leaf, angular):
     X = uniform(n, d, seed=n + d)
     if angular:
@@ -75,16 +77,41 @@ def test_forest_matches_reference_streams(ctx, n, d, leaf, angular):
         np.testing.assert_array_equal(g["hyperplanes"].view(np.uint32), R.hyperplanes(t, d).view(np.uint32))
 
 
-def test_forest_handles_duplicate_points(ctx):
-    # all-equal rows: zero hyperplanes, margins tie -> next_bool() in examination order
-    X = np.ones((300, 16), np.float32)
-    X[150:] = 2.0
+@pytest.mark.parametrize("n,d,kind", [(30000, 16, "mixture"), (20000, 128, "mixture"), (20000, 8, "grid"),
+                                      (6000, 3, "grid")])
+def test_forest_integer_data_with_margin_ties(ctx, n, d, kind):
+    """Integer rows: points exactly on a bisector give |margin| < 1e-8 ties, which take draws in the two-pointer
+    examination order (builder.rs:319-347) - a few per node in the headline generator's data, many in coarse
+    integer grids (both ways of deciding them), in large nodes shared by several workgroups too."""
+    from rinnd_amd import datasets
+    if kind == "mixture":
+        X = datasets.cfg2(n=n, d=d)
+    else:
+        X = np.random.default_rng(d).integers(0, 6, (n, d)).astype(np.float32)
+    F, st_gpu = gpu_forest(ctx, X, 4, 60, 42)
+    st_ref = O.rng_state(42)
+    R = O.build_forest(X, 4, 60, st_ref, threads=4)
+    np.testing.assert_array_equal(st_gpu, st_ref)
+    for t in range(4):
+        g, r = F.tree(t, n), R.tree(t)
+        assert g["n_nodes"] == r["n_nodes"]
+        np.testing.assert_array_equal(g["indices"], r["indices"])
+        np.testing.assert_array_equal(g["children"], r["children"])
+        np.testing.assert_array_equal(g["offsets"].view(np.uint32), r["offsets"].view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [300, 10000])
+def test_forest_handles_duplicate_points(ctx, n):
+    # all-equal rows: zero hyperplanes, margins tie -> next_bool() in examination order (at n = 10000 the ties
+    # arise in nodes large enough to be shared by several workgroups)
+    X = np.ones((n, 16), np.float32)
+    X[n // 2:] = 2.0
     F, st_gpu = gpu_forest(ctx, X, 3, 10, 7)
     st_ref = O.rng_state(7)
     R = O.build_forest(X, 3, 10, st_ref)
     np.testing.assert_array_equal(st_gpu, st_ref)
     for t in range(3):
-        g, r = F.tree(t, 300), R.tree(t)
+        g, r = F.tree(t, n), R.tree(t)
         np.testing.assert_array_equal(g["indices"], r["indices"])
         np.testing.assert_array_equal(g["children"], r["children"])
 
