@@ -413,7 +413,6 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
     WBuf<RngStream> d_streams(W, 19, 64);
     WBuf<unsigned long long> cstat(W, 20, 2);
     WBuf<int64_t> d_bbase(W, 21, 64);
-    WBuf<uint64_t> d_jt;
     // candidate build scratch (forward priorities, reverse records before / after the destination sort)
     WBuf<float> c_fpr(W, 22, (size_t)n * k);
     WBuf<uint32_t> c_rkey(W, 23, (size_t)n * k), c_rkey2(W, 24, (size_t)n * k);
@@ -459,36 +458,42 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
             } else {
                 mode = 1;
                 block_rows = ceil_div(n, T_eff);
-                std::vector<uint64_t> hv(n + 1);
-                HIP_TRY(hipMemcpyAsync(hv.data(), rvp.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+                // the valid-edge ranks at the row blocks' boundaries only
+                WBuf<uint64_t> d_bnd(W, 43, 65);
+                launch_gather_strided(s, rvp.p, block_rows, n, T_eff + 1, d_bnd.p);
+                std::vector<uint64_t> hv(T_eff + 1);
+                HIP_TRY(hipMemcpyAsync(hv.data(), d_bnd.p, (T_eff + 1) * 8, hipMemcpyDeviceToHost, s));
                 HIP_TRY(hipStreamSynchronize(s));
                 std::vector<int64_t> bbase(T_eff);
                 int64_t acc = 0;
                 for (int t = 0; t < T_eff; ++t) {
                     const uint64_t seed = main.next_u64();  // candidates.rs:171
-                    const int64_t r0 = std::min<int64_t>(t * block_rows, n);
-                    const int64_t r1 = std::min<int64_t>((t + 1) * block_rows, n);
                     RngStream r{};
                     Xoshiro x = xoshiro_seed(seed);
                     for (int w = 0; w < 4; ++w) r.s[w] = x.s[w];
-                    r.count = (int64_t)(hv[r1] - hv[r0]);
+                    r.count = (int64_t)(hv[t + 1] - hv[t]);
                     r.out_offset = acc;
                     bbase[t] = acc;
                     acc += r.count;
                     streams.push_back(r);
                 }
-                HIP_TRY(hipMemcpyAsync(d_bbase.p, bbase.data(), T_eff * 8, hipMemcpyHostToDevice, s));
+                int64_t* pb = (int64_t*)((char*)W.stage.ensure(8192) + 4096);
+                std::copy(bbase.begin(), bbase.end(), pb);
+                HIP_TRY(hipMemcpyAsync(d_bbase.p, pb, T_eff * 8, hipMemcpyHostToDevice, s));
             }
-            if (!d_jt.p) {
-                d_jt = WBuf<uint64_t>(W, 32, (size_t)kJumpLevels * 1024);
-                HIP_TRY(hipMemcpyAsync(d_jt.p, host_jump_tables(), d_jt.bytes(), hipMemcpyHostToDevice, s));
+            if (!ctx->jump_tables.p) {
+                ctx->jump_tables.alloc((size_t)kJumpLevels * 1024);
+                HIP_TRY(hipMemcpyAsync(ctx->jump_tables.p, host_jump_tables(), ctx->jump_tables.bytes(),
+                                       hipMemcpyHostToDevice, s));
+                HIP_TRY(hipStreamSynchronize(s));
             }
             int64_t maxc = 0;
             for (auto& r : streams) maxc = std::max(maxc, r.count);
-            HIP_TRY(hipMemcpyAsync(d_streams.p, streams.data(), streams.size() * sizeof(RngStream),
-                                   hipMemcpyHostToDevice, s));
-            launch_rng_streams(s, d_jt.p, d_streams.p, (int)streams.size(), maxc, prio.p);
-            HIP_TRY(hipStreamSynchronize(s));  // host vectors above go out of scope
+            // staged in pinned memory: no wait here (the next write to it follows this iteration's later waits)
+            RngStream* ps = (RngStream*)W.stage.ensure(8192);
+            std::copy(streams.begin(), streams.end(), ps);
+            HIP_TRY(hipMemcpyAsync(d_streams.p, ps, streams.size() * sizeof(RngStream), hipMemcpyHostToDevice, s));
+            launch_rng_streams(s, ctx->jump_tables.p, d_streams.p, (int)streams.size(), maxc, prio.p);
         } else {
             counter_key = main.next_u64();
         }
@@ -739,6 +744,17 @@ rinnd_status rinnd_ctx_create(int device, rinnd_ctx** out) {
     }
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return RINND_ERR_HIP;
+    // the FastRng jump tables (built once per process) on the device, so that no build pays for them
+    try {
+        const uint64_t* jt = rinnd::host_jump_tables();
+        c->jump_tables.alloc((size_t)rinnd::kJumpLevels * 1024);
+        if (hipMemcpy(c->jump_tables.p, jt, c->jump_tables.bytes(), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            c->jump_tables.release();
+        }
+    } catch (...) {
+        c->jump_tables.release();
+    }
     *out = c;
     return RINND_OK;
 }

@@ -328,6 +328,7 @@ void launch_heap_apply(hipStream_t s, const uint64_t* off, SegRec* seg, SegRec* 
                        unsigned long long* changes, int32_t* big_list, int32_t* big_count, uint32_t* bins);
 
 // candidates
+void launch_gather_strided(hipStream_t s, const uint64_t* in, int64_t stride, int64_t n, int count, uint64_t* out);
 void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt,
                              const uint8_t* row_on = nullptr);
 struct RngStream {
@@ -588,6 +589,7 @@ struct Workspace {
     Bf16Rows b16;
     std::unique_ptr<Pipeline> pipeline;
     PinnedBuf status;      // device -> host status words
+    PinnedBuf stage;       // host -> device parameter staging (candidate streams, block bases)
     OwnedEvent status_ev;  // marks the status copy
     void* get(int slot, size_t bytes) {
         if ((int)slots.size() <= slot) slots.resize(slot + 1);

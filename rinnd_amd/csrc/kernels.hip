@@ -1967,6 +1967,14 @@ __global__ void k_row_valid(const int32_t* __restrict__ idx, int64_t n, int k, c
         cnt[i] = c;
     }
 }
+// out[t] = in[min(t * stride, n)] for t < count
+__global__ void k_gather_strided(const uint64_t* __restrict__ in, int64_t stride, int64_t n, int count, uint64_t* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = in[min((int64_t)t * stride, n)];
+}
+void launch_gather_strided(hipStream_t s, const uint64_t* in, int64_t stride, int64_t n, int count, uint64_t* out) {
+    hipLaunchKernelGGL(k_gather_strided, dim3(grid_for(count, 64)), dim3(64), 0, s, in, stride, n, count, out);
+}
 void launch_row_valid_counts(hipStream_t s, const int32_t* idx, int64_t n, int k, uint64_t* cnt,
                              const uint8_t* row_on) {
     hipLaunchKernelGGL(k_row_valid, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, idx, n, k, row_on,
