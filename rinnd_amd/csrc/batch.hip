@@ -354,7 +354,7 @@ void nn_descent_batch_dev(rinnd_ctx* ctx, const float* DX, int64_t ld, const std
         WBuf<unsigned long long> lpairs(W, kWsBatch + 7, 2);
         WBuf<VTreeEnt> d_ents(W, kWsBatch + 8, std::max(nseg, 1));
         launch_zero(s, lpairs.p, 16);
-        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, nullptr, nullptr};
+        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, nullptr};
         const KTimer none{nullptr, nullptr, nullptr, nullptr};
         std::vector<std::vector<VTreeEnt>> ents(max_t);
         std::vector<int64_t> Nt(max_t, 0), span(max_t, 0);
@@ -370,7 +370,7 @@ void nn_descent_batch_dev(rinnd_ctx* ctx, const float* DX, int64_t ld, const std
             launch_leaf_list_batch(s, d_ents.p, (int)ents[t].size(), span[t], Nt[t], mark.p, rank.p, end_at.p,
                                    leaves.p, vidx.p, ltmp.p, lt_bytes);
             launch_leaf_init(s, metric, DX, ld, dim, vidx.p, leaves.p, rank.p + Nt[t], Nt[t], p.leaf_size, w_idx, w_dist,
-                             w_flags, k, lws, none, none, t < 3);
+                             w_flags, k, lws, none, none, (int)t < kLeafFullTrees);
             HIP_TRY(hipStreamSynchronize(s));  // the entry table is rewritten for the next tree
         }
     }

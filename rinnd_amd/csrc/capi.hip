@@ -351,17 +351,45 @@ static void nn_descent_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64_t
         WBuf<unsigned long long> lpairs(W, 13, 2);
         HIP_TRY(hipMemsetAsync(lpairs.p, 0, 16, s));
         const bool exact16 = use_bf16 && B16.exact;
-        WBuf<uint32_t> smask(W, 39, exact16 ? (size_t)n * 8 : 1);
-        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, exact16 ? &B16 : nullptr, exact16 ? smask.p : nullptr};
+        const LeafInitWs lws{mark.p, rank.p, dscr.p, lpairs.p, exact16 ? &B16 : nullptr};
         // deferred timings: no wait per tree
         const KTimer dist_timer{nullptr, nullptr, stats ? &st.timed[RINND_TIMED_LEAF_DIST] : nullptr, &tlog};
         const KTimer push_timer{nullptr, nullptr, stats ? &st.timed[RINND_TIMED_LEAF_PUSH] : nullptr, &tlog};
-        for (size_t t = 0; t < F.trees.size(); ++t) {
+        const int n_trees = (int)F.trees.size();
+        // exact-mode BF16 rows: after the first trees only the pairs below an endpoint's root are written
+        // (k_leaf_surv_mfma), in groups of trees sharing one push pass
+        const int64_t sdl4 = leaf_surv_dl4(metric, exact16 ? &B16 : nullptr, Dld, p.leaf_size, k);
+        const int t_full = sdl4 > 0 ? std::min(n_trees, kLeafFullTrees) : n_trees;
+        for (int t = 0; t < t_full; ++t) {
             launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
                              leaves.p, ltmp.p, lt_bytes);
             launch_leaf_init(s, metric, DXw, Dld, dim, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size,
                              w_idx, w_dist, w_flags, k, lws, dist_timer, push_timer,
-                             t < 3 /* the first trees: most pushes are accepted */);
+                             t < kLeafFullTrees /* the first trees: most pushes are accepted */);
+        }
+        if (t_full < n_trees) {
+            const double per_tree = (double)n * (sdl4 * 4.0 + 32.0 + 8.0);
+            const int G = (int)std::max<double>(1.0, std::min<double>({(double)kSurvGroupMax, (double)(n_trees - t_full),
+                                                                       std::floor(surv_group_budget() / per_tree)}));
+            WBuf<float> gD(W, 44, (size_t)G * n * sdl4);
+            WBuf<uint32_t> gM(W, 39, (size_t)G * n * 8);
+            WBuf<int2> gI(W, 45, (size_t)G * n);
+            for (int t0 = t_full; t0 < n_trees; t0 += G) {
+                SurvGroup grp{};
+                grp.D = gD.p;
+                grp.Msk = gM.p;
+                grp.info = gI.p;
+                grp.G = std::min(G, n_trees - t0);
+                for (int g = 0; g < grp.G; ++g) {
+                    const int t = t0 + g;
+                    grp.tidx[g] = F.trees[t].indices.p;
+                    launch_leaf_list(s, F.trees[t].children.p, F.trees[t].n_nodes, n, mark.p, rank.p, end_at.p,
+                                     leaves.p, ltmp.p, lt_bytes);
+                    launch_leaf_surv_tree(s, B16, F.trees[t].indices.p, leaves.p, rank.p + n, n, p.leaf_size, w_dist, k,
+                                          grp, g, lpairs.p + 1, dist_timer);
+                }
+                launch_leaf_push_group(s, grp, DXw, Dld, dim, n, p.leaf_size, w_idx, w_dist, w_flags, k, push_timer);
+            }
         }
         if (stats) {  // per-pair bytes: D written (4 B per ordered pair or survivor), D and the partner id read
             unsigned long long lp[2];

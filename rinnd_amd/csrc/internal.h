@@ -301,8 +301,14 @@ struct LeafInitWs {
     float* D;
     unsigned long long* pairs;  // 2 counters: ordered pairs m(m-1) written whole, survivor pairs (accumulated)
     const Bf16Rows* exact_bf16; // exact-mode bf16 rows: pair distances on MFMA (null: FP32 kernels)
-    uint32_t* surv_mask;        // n x 8 survivor column masks (null: every tree writes whole leaves)
 };
+// Trees of a build whose leaf pairs are all written and pushed (the heaps start empty, so most pushes of the
+// first trees are accepted); later trees write only the pairs below an endpoint's root when the MFMA survivor
+// kernel applies.
+#ifndef RINND_LEAF_FULL_TREES
+#define RINND_LEAF_FULL_TREES 3
+#endif
+constexpr int kLeafFullTrees = RINND_LEAF_FULL_TREES;
 bool leaf_init_supported(int k, int64_t leaf_size);
 int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size);
 void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int dim, const int32_t* tree_indices,
@@ -310,6 +316,33 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
                       int32_t* idx, float* dist, uint8_t* flags, int k, const LeafInitWs& ws,
                       const KTimer& dist_timer, const KTimer& push_timer,
                       bool early);  // early: one of the first trees of a build (narrower push tiles)
+
+// Survivor trees in groups (exact-mode BF16 rows, squared Euclidean): every tree of a group writes its survivors
+// against the roots at the group's start (a superset of the pushes the heaps accept: roots only fall), point-major;
+// then one pass per point replays the group's trees in order.  Up to kSurvGroupMax trees per group: a larger group
+// saves push passes but its later trees test against staler roots and write more survivors (at 1M x 128, leaf
+// 256, 16 trees the 13 survivor trees' leaf phase takes 10.8 ms in groups of 2-4, 11.2 ms one by one, 11.6 ms in
+// one group).
+#ifndef RINND_SURV_GROUP_MAX
+#define RINND_SURV_GROUP_MAX 4
+#endif
+constexpr int kSurvGroupMax = RINND_SURV_GROUP_MAX;
+// Device memory the scratch of one group may take: an eighth of the device's memory, at most 40 GiB.
+double surv_group_budget();
+struct SurvGroup {
+    const int32_t* tidx[kSurvGroupMax];  // tree g's indices
+    float* D;                            // tree g: D + g * n * Dl4 (n x Dl4)
+    uint32_t* Msk;                       // tree g: Msk + g * n * 8
+    int2* info;                          // tree g: info + g * n
+    int G;
+};
+// 0 if the survivor path does not apply, else the D row length (leaf columns) it uses
+int64_t leaf_surv_dl4(int metric, const Bf16Rows* exact_bf16, int64_t ld, int64_t leaf_size, int k);
+void launch_leaf_surv_tree(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
+                           const int64_t* n_leaves_dev, int64_t n, int64_t leaf_size, const float* hdist, int k,
+                           const SurvGroup& grp, int g, unsigned long long* surv, const KTimer& timer);
+void launch_leaf_push_group(hipStream_t s, const SurvGroup& grp, const float* X, int64_t ld, int dim, int64_t n,
+                            int64_t leaf_size, int32_t* idx, float* dist, uint8_t* flags, int k, const KTimer& timer);
 
 // ordered-apply pipeline
 void launch_leaf_pairs(hipStream_t s, int metric, const float* X, int64_t ld, int dim,
@@ -463,11 +496,12 @@ void prepare_bf16_rows(hipStream_t s, const float* X, int64_t ld, int64_t n, int
 // leaf pair distances on MFMA (exact mode only), same D layout as the FP32 two-pass kernel
 size_t leaf_dist_mfma_lds(int64_t ldb, int mcap);
 size_t leaf_surv_mfma_lds(int64_t ldb, int mcap);
-// survivor pairs only (pairs below the destination's heap root at the start of the tree): per position a
-// column mask (8 words) and the surviving distances in column order at the start of its D row
+// survivor pairs only (pairs below an endpoint's heap root when the kernel runs), point-major: per point a
+// column mask (8 words), the surviving distances at their leaf columns of its D row, and its leaf's
+// (start, size) in info
 void launch_leaf_surv_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
                            const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, const float* hdist, int k,
-                           float* D, int64_t Dl4, uint32_t* Msk, unsigned long long* surv);
+                           float* D, int64_t Dl4, uint32_t* Msk, int2* info, unsigned long long* surv);
 void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
                            const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, float* D, int64_t Dl4,
                            unsigned long long* pairs);

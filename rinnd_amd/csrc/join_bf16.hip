@@ -449,25 +449,29 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
 }
 
 // The same pair distances for the trees after the first few, writing only the pairs that can enter a heap:
-// a push of d onto p's heap changes it only if d < p's root, and the roots only fall while a tree's pushes
-// are applied, so the pairs with d >= p's root at the start of the tree are dropped for p.  Each unordered
-// pair is evaluated once (16 x 16 blocks A <= O of the leaf's Gram matrix, the upper triangle inside a diagonal
-// block; exact-mode distances are symmetric) and tested against both endpoints' roots.  A survivor is written
-// at its column in the member's D row, and the member's mask (Msk, 8 words, bit o = member o) gets its bit; the
-// masks are gathered in LDS with atomic ORs and written once per leaf.  surv counts the survivors.
+// a push of d onto p's heap changes it only if d < p's root, and the roots only fall while pushes are applied,
+// so the pairs with d >= p's root at the time this kernel reads it (before any push of this tree) are dropped
+// for p.  Each unordered pair is evaluated once (16 x 16 blocks A <= O of the leaf's Gram matrix, the upper
+// triangle inside a diagonal block; exact-mode distances are symmetric) and tested against both endpoints'
+// roots.  Everything is written point-major, so that one push pass can replay several trees per point: a
+// survivor goes to its leaf column in the member's D row (row = the member's id), the member's mask (Msk, 8
+// words, bit o = leaf member o) gets its bit (the masks are gathered in LDS with atomic ORs and written once
+// per leaf), and info[id] = (leaf start, leaf size) for every member of every leaf, including the leaves this
+// kernel does not evaluate (one member, or more than mcap: the push pass evaluates those itself).
+// surv counts the survivors.
 // Launched with 256 threads (four waves) and mcap <= 256; NK16 = ldb / 16.  Wave w owns the row-block pairs
 // (P, nb - 1 - P), P = w, w + 4, ... (nb + 1 block-pair columns each, balanced), keeps a row block's operands
-// and its rows' norms and roots in registers and walks the column blocks O >= A two at a time (two independent
-// MFMA chains).  PRE (rows of at most 128 bf16): the leaves are software-pipelined - wave w stages rows
-// 64 w .. 64 w + 63 of a leaf, 16 lanes per row, taking the row ids from its own lanes by shuffle; while a leaf
-// is evaluated, the next leaf's rows (ids loaded one leaf earlier) are already in flight in registers.
+// and its rows' norms, roots and ids in registers and walks the column blocks O >= A two at a time (two
+// independent MFMA chains).  PRE (rows of at most 128 bf16): the leaves are software-pipelined - wave w stages
+// rows 64 w .. 64 w + 63 of a leaf, 16 lanes per row, taking the row ids from its own lanes by shuffle; while a
+// leaf is evaluated, the next leaf's rows (ids loaded one leaf earlier) are already in flight in registers.
 template <int NK16, bool PRE>
 __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restrict__ Xb,
                                                         const float* __restrict__ nrm, const int32_t* __restrict__ tidx,
                                                         const int64_t* __restrict__ leaves,
                                                         const int64_t* __restrict__ n_leaves_dev, int mcap,
                                                         const float* __restrict__ hdist, int k, float* __restrict__ D,
-                                                        int64_t Dl4, uint32_t* __restrict__ Msk,
+                                                        int64_t Dl4, uint32_t* __restrict__ Msk, int2* __restrict__ info,
                                                         unsigned long long* surv) {
     constexpr int ldb = 16 * NK16, NK32 = NK16 / 2, v8 = ldb / 8;
     extern __shared__ __align__(16) uint16_t ssm16[];
@@ -482,7 +486,6 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
     const int64_t n_leaves = *n_leaves_dev;
     const int64_t G = gridDim.x;
     const int mrow = lane & 15, kqi = lane >> 4, kq = kqi * 4;
-    const int dl = (int)Dl4;
     auto geom = [&](int64_t L, int64_t& s, int& m) {  // wave-uniform; m = 0 for a leaf this kernel skips
         s = 0;
         m = 0;
@@ -583,8 +586,14 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
         m0 = m1;
         s1 = s2;
         m1 = m2;
-        if (!m) continue;  // uniform
-        float* const Dl = D + s * Dl4;  // the leaf's D rows; offsets below stay 32-bit
+        if (!m) {  // uniform: a leaf of one member or above mcap, evaluated by the push pass; its members' info
+            if (leaf < n_leaves) {
+                const int64_t ls = leaves[2 * leaf];
+                const int64_t lm = leaves[2 * leaf + 1] - ls;
+                for (int64_t i = t; i < lm; i += B) info[tidx[ls + i]] = make_int2((int)ls, (int)lm);
+            }
+            continue;
+        }
         const int nb = mp >> 4;
         for (int P = w; P < ((nb + 1) >> 1); P += nw) {
             for (int h = 0; h < 2; ++h) {
@@ -601,18 +610,23 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
                 const int x0 = 16 * A + kq;  // this lane's rows x0 .. x0 + 3
                 const float4 nx4 = *reinterpret_cast<const float4*>(nr + x0);
                 const float4 rx4 = *reinterpret_cast<const float4*>(rt + x0);
+                const int4 ix4 = *reinterpret_cast<const int4*>(rid + x0);
                 const float nx[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, rx[4] = {rx4.x, rx4.y, rx4.z, rx4.w};
+                const int32_t ix[4] = {ix4.x, ix4.y, ix4.z, ix4.w};  // the rows' ids: their D rows
                 // B operands of column blocks (O, O + 1), the next pair's loaded while this pair's results are
                 // tested (an odd last block repeats O; its results are unused)
                 v8bf b0[NK32 > 0 ? NK32 : 1], b1[NK32 > 0 ? NK32 : 1];
                 v4s t0, t1;
                 float cn[2], cr[2];  // the two columns' norms and roots
+                int32_t ci[2];       // and ids
                 auto load_b = [&](int O) {
                     const int O1 = O + 1 < nb ? O + 1 : O;
                     cn[0] = nr[16 * O + mrow];
                     cr[0] = rt[16 * O + mrow];
+                    ci[0] = rid[16 * O + mrow];
                     cn[1] = nr[16 * O1 + mrow];
                     cr[1] = rt[16 * O1 + mrow];
+                    ci[1] = rid[16 * O1 + mrow];
                     const uint16_t* rb0 = ssm16 + (16 * O + mrow) * lds_ld;
                     const uint16_t* rb1 = ssm16 + (16 * O1 + mrow) * lds_ld;
 #pragma unroll
@@ -638,6 +652,7 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
                         acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a16, t1, acc1, 0, 0, 0);
                     }
                     const float cnow_n[2] = {cn[0], cn[1]}, cnow_r[2] = {cr[0], cr[1]};
+                    const int32_t cnow_i[2] = {ci[0], ci[1]};
                     if (O + 2 < nb) load_b(O + 2);  // wave-uniform
 #pragma unroll
                     for (int hb = 0; hb < 2; ++hb) {
@@ -646,6 +661,7 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
                         const v4f acc = hb ? acc1 : acc0;
                         const int o = 16 * Oh + mrow;  // this lane's column
                         const float no = cnow_n[hb], ro = cnow_r[hb];
+                        float* const Do = D + (int64_t)cnow_i[hb] * Dl4;  // column o's D row
                         const bool col_ok = o < m;
                         bool sx[4], so[4];
                         float d[4];
@@ -667,12 +683,12 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
                         for (int r = 0; r < 4; ++r) {
                             const int x = x0 + r;
                             if (bx[r]) {  // wave-uniform
-                                if (sx[r]) Dl[x * dl + o] = d[r];
+                                if (sx[r]) D[(int64_t)ix[r] * Dl4 + o] = d[r];
                                 const uint32_t half = (uint32_t)(bx[r] >> (16 * kqi)) & 0xffffu;
                                 if ((mrow == 0) & (half != 0u))
                                     atomicOr(&lmask[x * 8 + (Oh >> 1)], half << (16 * (Oh & 1)));
                             }
-                            if (any_so && so[r]) Dl[o * dl + x] = d[r];
+                            if (any_so && so[r]) Do[x] = d[r];
                             nib |= (uint32_t)so[r] << r;
                         }
                         if (!any_so) continue;  // wave-uniform: no mask bits for any column this block
@@ -683,7 +699,8 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
             }
         }
         lds_barrier();
-        for (int i = t; i < m * 8; i += B) Msk[(s + (i >> 3)) * 8 + (i & 7)] = lmask[i];
+        for (int i = t; i < m * 8; i += B) Msk[(int64_t)rid[i >> 3] * 8 + (i & 7)] = lmask[i];
+        for (int i = t; i < m; i += B) info[rid[i]] = make_int2((int)s, m);
         lds_barrier();
     }
     if (lane == 0 && ns) atomicAdd(surv, ns);  // ns is the wave's count (the same in every lane)
@@ -697,7 +714,7 @@ size_t leaf_surv_mfma_lds(int64_t ldb, int mcap) {
 
 void launch_leaf_surv_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
                            const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, const float* hdist, int k,
-                           float* D, int64_t Dl4, uint32_t* Msk, unsigned long long* surv) {
+                           float* D, int64_t Dl4, uint32_t* Msk, int2* info, unsigned long long* surv) {
     if (mcap > 256) throw Error{RINND_ERR_INVALID_ARG, "survivor leaf kernel: leaves above 256 members"};
     // rows of at most 128 bf16: persistent, two workgroups per CU (the LDS limit), each pipelining its leaves
     const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(max_leaves, 1),
@@ -706,7 +723,7 @@ void launch_leaf_surv_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree
 #define RINND_SURV_CASE(NKV)                                                                                     \
     case NKV:                                                                                                    \
         hipLaunchKernelGGL((k_leaf_surv_mfma<NKV, (NKV <= 8)>), dim3(grid), dim3(256), lds, s, R.Xb.p, R.nrm.p,   \
-                           tree_indices, leaves, n_leaves_dev, mcap, hdist, k, D, Dl4, Msk, surv);               \
+                           tree_indices, leaves, n_leaves_dev, mcap, hdist, k, D, Dl4, Msk, info, surv);         \
         break;
     switch (R.ldb / 16) {
         RINND_SURV_CASE(1) RINND_SURV_CASE(2) RINND_SURV_CASE(3) RINND_SURV_CASE(4) RINND_SURV_CASE(5)

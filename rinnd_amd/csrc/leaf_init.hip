@@ -385,17 +385,18 @@ __global__ __launch_bounds__(256) void k_leaf_push(const int32_t* __restrict__ t
     }
 }
 
-// The pushes of a survivor tree: each position's column mask and its surviving distances (survivor o at
-// column o of its D row) from k_leaf_surv_mfma; the lane walks them in leaf order, the next survivor's
-// distance and partner id in flight while the current one is pushed.
+// The pushes of a group of survivor trees (k_leaf_surv_mfma, point-major): one lane per point, its heap in an
+// LDS column (64 consecutive rows staged per wave, coalesced), the root in a register.  For each tree of the
+// group in order: the point's leaf (info); for a leaf that k_leaf_surv_mfma evaluated, its survivors in leaf
+// order (mask bits, distance at the leaf column of its D row, partner id from the tree's indices), the next
+// survivor's distance and partner id in flight while the current one is pushed; for a leaf above mcap, every
+// other member in leaf order with the distance in the reference arithmetic (all metrics here are bitwise
+// symmetric).  A point's pushes come only from its own leaves, so replaying its trees one after another is the
+// reference's order for its heap.
 template <bool REG>
-__global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restrict__ tidx,
-                                                        const int64_t* __restrict__ leaves,
-                                                        const int32_t* __restrict__ mark,
-                                                        const int64_t* __restrict__ rank, int64_t n, int mcap,
-                                                        const float* __restrict__ D, int64_t Dl4,
-                                                        const uint32_t* __restrict__ Msk, int32_t* idx, float* dist,
-                                                        uint8_t* flags, int k) {
+__global__ __launch_bounds__(256) void k_leaf_push_group(SurvGroup grp, const float* __restrict__ X, int64_t ld,
+                                                         int dim, int64_t n, int mcap, int64_t Dl4, int32_t* idx,
+                                                         float* dist, uint8_t* flags, int k) {
     extern __shared__ __align__(16) float qsm[];
     const int B = blockDim.x, c = threadIdx.x;
     const int S = B + 1;  // heap column stride (odd: the row staging is conflict-free)
@@ -404,41 +405,51 @@ __global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restric
     int32_t* hi = (int32_t*)(hd + S * k);
     uint8_t* hf = (uint8_t*)(hi + S * k);
     for (int64_t base = ((int64_t)blockIdx.x * B) + w * 64; base < n; base += (int64_t)gridDim.x * B) {
-        const int64_t pos = base + lane;
-        int m = 0;
-        int64_t s = 0;
-        if (pos < n) {
-            const int64_t l = rank[pos] + mark[pos] - 1;  // leaf holding this position
-            s = leaves[2 * l];
-            m = (int)(leaves[2 * l + 1] - s);
-            if (m > mcap || m < 2) m = 0;  // not a two-pass leaf
-        }
-        const int32_t hrow = m ? tidx[pos] : -1;
+        const int64_t p = base + lane;
+        const int32_t hrow = p < n ? (int32_t)p : -1;
         wave_heap_rows<true>(hrow, k, dist, idx, flags, hd, hi, hf, S, w * 64);
-        if (m) {
+        if (p < n) {
             float root = hd[c];
-            int32_t sid[16];  // REG: the row's ids at the start of the pass that are still in the heap
+            int32_t sid[16];  // REG: the row's ids at the start of the tree's pass that are still in the heap
+            for (int g = 0; g < grp.G; ++g) {
+                const int2 inf = grp.info[(int64_t)g * n + p];
+                const int64_t s = inf.x;
+                const int m = inf.y;
+                if (m < 2) continue;
+                const int32_t* tidx = grp.tidx[g];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) sid[j] = REG && j < k ? hi[j * S + c] : -1;
-            const uint4 m0 = *reinterpret_cast<const uint4*>(Msk + pos * 8);
-            const uint4 m1 = *reinterpret_cast<const uint4*>(Msk + pos * 8 + 4);
-            const uint32_t words[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
-            const float* Dr = D + pos * Dl4;  // survivor o's distance at column o
+                for (int j = 0; j < 16; ++j) sid[j] = REG && j < k ? hi[j * S + c] : -1;
+                if (m <= mcap) {
+                    const uint32_t* mw = grp.Msk + ((int64_t)g * n + p) * 8;
+                    const uint4 m0 = *reinterpret_cast<const uint4*>(mw);
+                    const uint4 m1 = *reinterpret_cast<const uint4*>(mw + 4);
+                    const uint32_t words[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+                    const float* Dr = grp.D + ((int64_t)g * n + p) * Dl4;  // survivor o's distance at column o
 #pragma unroll
-            for (int wi = 0; wi < 8; ++wi) {
-                uint32_t bits = words[wi];
-                int j = bits ? __ffs((int)bits) - 1 : -1;
-                int32_t xj = j >= 0 ? tidx[s + 32 * wi + j] : 0;
-                float dj = j >= 0 ? Dr[32 * wi + j] : 0.f;
-                while (j >= 0) {  // the next survivor's id and distance in flight
-                    bits &= bits - 1;
-                    const int jn = bits ? __ffs((int)bits) - 1 : -1;
-                    const int32_t xn = jn >= 0 ? tidx[s + 32 * wi + jn] : 0;
-                    const float dnx = jn >= 0 ? Dr[32 * wi + jn] : 0.f;
-                    leaf_offer<REG>(hd, hi, hf, S, c, k, xj, dj, root, sid);
-                    j = jn;
-                    xj = xn;
-                    dj = dnx;
+                    for (int wi = 0; wi < 8; ++wi) {
+                        uint32_t bits = words[wi];
+                        int j = bits ? __ffs((int)bits) - 1 : -1;
+                        int32_t xj = j >= 0 ? tidx[s + 32 * wi + j] : 0;
+                        float dj = j >= 0 ? Dr[32 * wi + j] : 0.f;
+                        while (j >= 0) {  // the next survivor's id and distance in flight
+                            bits &= bits - 1;
+                            const int jn = bits ? __ffs((int)bits) - 1 : -1;
+                            const int32_t xn = jn >= 0 ? tidx[s + 32 * wi + jn] : 0;
+                            const float dnx = jn >= 0 ? Dr[32 * wi + jn] : 0.f;
+                            leaf_offer<REG>(hd, hi, hf, S, c, k, xj, dj, root, sid);
+                            j = jn;
+                            xj = xn;
+                            dj = dnx;
+                        }
+                    }
+                } else {
+                    const float* xp = X + p * ld;
+                    for (int o = 0; o < m; ++o) {
+                        const int32_t q = tidx[s + o];
+                        if (q == (int32_t)p) continue;
+                        const float d = distance_exact(RINND_DIST_SQEUCLIDEAN, xp, X + (int64_t)q * ld, dim);
+                        leaf_offer<REG>(hd, hi, hf, S, c, k, q, d, root, sid);
+                    }
                 }
             }
         }
@@ -446,19 +457,135 @@ __global__ __launch_bounds__(256) void k_leaf_push_surv(const int32_t* __restric
     }
 }
 
+// The same 4 x 4 register tiles for leaves whose rows do not fit in LDS together (wide rows): the leaf's rows
+// are staged KC dimensions at a time for all members, and every tile's 8 lane accumulators stay in registers
+// across the chunks, so each pair still sees the reference's arithmetic (chunks in order, then the hsum and the
+// scalar tail).  512 threads; a leaf with more tiles than threads takes several passes over its rows.
+template <int KC>
+__global__ __launch_bounds__(512) void k_leaf_dist_sq_kc(const float* __restrict__ X, int64_t ld, int dim,
+                                                         const int32_t* __restrict__ tidx,
+                                                         const int64_t* __restrict__ leaves,
+                                                         const int64_t* __restrict__ n_leaves_dev, int mcap,
+                                                         float* __restrict__ D, int64_t Dl4,
+                                                         unsigned long long* pairs) {
+    constexpr int LDK = KC + 4;  // staged row stride (floats): 16-byte aligned, rows start on different banks
+    extern __shared__ __align__(16) float csm[];
+    const int B = blockDim.x, t = threadIdx.x;
+    int32_t* rid = (int32_t*)(csm + (int64_t)round_up((int64_t)mcap, (int64_t)4) * LDK);
+    const int64_t n_leaves = *n_leaves_dev;
+    const int chunks = dim >> 3;
+    const int dim8 = chunks * 8;
+    unsigned long long np = 0;
+    for (int64_t leaf = blockIdx.x; leaf < n_leaves; leaf += gridDim.x) {
+        const int64_t s = leaves[2 * leaf];
+        const int m = (int)(leaves[2 * leaf + 1] - s);
+        if (m > mcap || m < 2) continue;  // uniform; larger leaves go to the untiled kernel
+        np += (unsigned long long)m * (m - 1);
+        const int mp = round_up(m, 4);  // rows m..mp-1 are zero padding for the edge tiles
+        __syncthreads();                // the previous leaf is done with rid
+        for (int i = t; i < mp; i += B) rid[i] = i < m ? tidx[s + i] : -1;
+        const int nb = mp >> 2;
+        const int T = nb * (nb + 1) / 2;
+        for (int q0 = 0; q0 < T; q0 += B) {  // uniform
+            const int q = q0 + t;
+            const bool act = q < T;
+            int A = 0, O = 0;
+            if (act) {
+                tri_pair(q, nb + 1, A, O);  // pairs A < O' over nb + 1 blocks -> A <= O = O' - 1
+                O -= 1;
+            }
+            float acc[4][4][8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int l = 0; l < 8; ++l) acc[i][j][l] = 0.f;
+            for (int c0 = 0; c0 < dim8; c0 += KC) {
+                const int kc = min(KC, dim8 - c0);  // a multiple of 8
+                const int v4 = kc >> 2;
+                __syncthreads();  // the last chunk is consumed (and rid is written)
+                for (int i = t; i < mp * v4; i += B) {
+                    const int r = i / v4, e = i - r * v4;
+                    const int32_t id = rid[r];
+                    *reinterpret_cast<float4*>(csm + r * LDK + 4 * e) =
+                        id >= 0 ? *reinterpret_cast<const float4*>(X + (int64_t)id * ld + c0 + 4 * e)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+                __syncthreads();
+                if (act) {
+                    const float* ra = csm + (4 * A) * LDK;
+                    const float* ro = csm + (4 * O) * LDK;
+                    for (int c = 0; c < kc; c += 8) {
+                        float4 xa[4][2], xo[4][2];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            xa[i][0] = *reinterpret_cast<const float4*>(ra + i * LDK + c);
+                            xa[i][1] = *reinterpret_cast<const float4*>(ra + i * LDK + c + 4);
+                            xo[i][0] = *reinterpret_cast<const float4*>(ro + i * LDK + c);
+                            xo[i][1] = *reinterpret_cast<const float4*>(ro + i * LDK + c + 4);
+                        }
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                float d;
+                                d = xa[i][0].x - xo[j][0].x; acc[i][j][0] = __builtin_fmaf(d, d, acc[i][j][0]);
+                                d = xa[i][0].y - xo[j][0].y; acc[i][j][1] = __builtin_fmaf(d, d, acc[i][j][1]);
+                                d = xa[i][0].z - xo[j][0].z; acc[i][j][2] = __builtin_fmaf(d, d, acc[i][j][2]);
+                                d = xa[i][0].w - xo[j][0].w; acc[i][j][3] = __builtin_fmaf(d, d, acc[i][j][3]);
+                                d = xa[i][1].x - xo[j][1].x; acc[i][j][4] = __builtin_fmaf(d, d, acc[i][j][4]);
+                                d = xa[i][1].y - xo[j][1].y; acc[i][j][5] = __builtin_fmaf(d, d, acc[i][j][5]);
+                                d = xa[i][1].z - xo[j][1].z; acc[i][j][6] = __builtin_fmaf(d, d, acc[i][j][6]);
+                                d = xa[i][1].w - xo[j][1].w; acc[i][j][7] = __builtin_fmaf(d, d, acc[i][j][7]);
+                            }
+                    }
+                }
+            }
+            if (act) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int a = 4 * A + i, o = 4 * O + j;
+                        if (a < o && o < m) {
+                            float r = hsum8(acc[i][j]);
+                            const float* xa = X + (int64_t)rid[a] * ld;
+                            const float* xo = X + (int64_t)rid[o] * ld;
+                            for (int e = dim8; e < dim; ++e) {  // scalar tail, no fma
+                                const float df = xa[e] - xo[e];
+                                r += df * df;
+                            }
+                            D[(s + a) * Dl4 + o] = r;
+                            D[(s + o) * Dl4 + a] = r;
+                        }
+                    }
+            }
+        }
+    }
+    if (t == 0 && np) atomicAdd(pairs, np);
+}
+
 struct TilePlan {
     int mcap = 0;
     int64_t Dl4 = 0;
     size_t rows_lds = 0;
+    bool chunked = false;  // rows staged kLeafKC dimensions at a time (squared Euclidean only)
 };
-static TilePlan tile_plan(int64_t ld, int64_t leaf_size) {
+constexpr int kLeafKC = 64;
+static TilePlan tile_plan(int64_t ld, int64_t leaf_size, int metric = RINND_DIST_SQEUCLIDEAN) {
     TilePlan P;
     const int mcap = (int)std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 256);
     const size_t rows = (size_t)round_up((int64_t)mcap, (int64_t)4) * (ld + 4 + 1) * 4;  // rows + member ids
-    if (rows > 150 * 1024) return P;
+    if (rows > 150 * 1024) {
+        if (metric != RINND_DIST_SQEUCLIDEAN) return P;
+        P.chunked = true;
+        P.rows_lds = (size_t)round_up((int64_t)mcap, (int64_t)4) * (kLeafKC + 4 + 1) * 4;
+    } else {
+        P.rows_lds = rows;
+    }
     P.mcap = mcap;
     P.Dl4 = round_up((int64_t)mcap, (int64_t)4);
-    P.rows_lds = rows;
     return P;
 }
 
@@ -469,11 +596,18 @@ static size_t init_heap_bytes(int B, int k) {
     return (size_t)round_up((int64_t)B * k * 2 + (B * k + 3) / 4, 4) * 4;
 }
 
+// Lanes per workgroup of the push kernels: one LDS heap column per lane (plus one for the odd stride) in 64 KiB.
+static int push_block(int k) {
+    int PB = 256;
+    while (PB > 64 && init_heap_bytes(PB + 1, k) > 64 * 1024) PB -= 64;
+    return PB;
+}
+
 bool leaf_init_supported(int k, int64_t leaf_size) {
     return init_heap_bytes(init_block(leaf_size), k) <= (size_t)kInitLdsBudget;
 }
 int64_t leaf_init_scratch_floats(int64_t n, int64_t ld, int64_t leaf_size) {
-    const TilePlan P = tile_plan(ld, leaf_size);
+    const TilePlan P = tile_plan(ld, leaf_size);  // the widest plan (squared Euclidean)
     return P.mcap ? std::max<int64_t>(n, 1) * P.Dl4 : 0;
 }
 
@@ -499,66 +633,43 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
     const int64_t ldl = ld + 4;
     const int64_t n = max_leaves;  // positions in the tree == points
     const unsigned grid = (unsigned)std::min<int64_t>(max_leaves, 8192);  // persistent over the leaves
-    const TilePlan P = tile_plan(ld, leaf_size);
+    const TilePlan P = tile_plan(ld, leaf_size, metric);
     int untiled_cap = 0;  // leaves with m <= untiled_cap are done by the two-pass kernels
-    int PB = 256;
-    while (PB > 64 && init_heap_bytes(PB + 1, k) > 64 * 1024) PB -= 64;
+    const int PB = push_block(k);
     if (P.mcap > 0 && ws.D && init_heap_bytes(PB + 1, k) <= 64 * 1024) {
         const bool mfma = ws.exact_bf16 && metric == RINND_DIST_SQEUCLIDEAN &&
                           leaf_dist_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024;
-        // after the first trees most pairs cannot enter a heap: survivors only
-        const bool surv = mfma && !early && ws.surv_mask && leaf_surv_mfma_lds(ws.exact_bf16->ldb, P.mcap) <= 150 * 1024;
-        if (surv) {
-            dist_timer.begin(s);
-            launch_leaf_surv_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, dist, k,
-                                  ws.D, P.Dl4, ws.surv_mask, ws.pairs + 1);
-            dist_timer.stop(s);
-            push_timer.begin(s);
-            const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
-            if (k <= 16)
-                hipLaunchKernelGGL(k_leaf_push_surv<true>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s,
-                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, ws.surv_mask, idx,
-                                   dist, flags, k);
-            else
-                hipLaunchKernelGGL(k_leaf_push_surv<false>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s,
-                                   tree_indices, leaves, ws.mark, ws.rank, n, P.mcap, ws.D, P.Dl4, ws.surv_mask, idx,
-                                   dist, flags, k);
-            push_timer.stop(s);
-            // per row: bf16 row + norm + root + id read, mask written (dist); heap, leaf lookup, mask (push);
-            // the per-survivor bytes are added by the caller from the survivor counter
-            dist_timer.collect((double)n * (ws.exact_bf16->ldb * 2.0 + 12.0 + 32.0));
-            push_timer.collect((double)n * k * 9.0 * 2.0 + n * (40.0 + 32.0));
-            untiled_cap = P.mcap;
-        } else {
-            dist_timer.begin(s);
-            if (mfma)
-                launch_leaf_dist_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, ws.D,
-                                      P.Dl4, ws.pairs);
-            else if (metric == RINND_DIST_SQEUCLIDEAN)
-                hipLaunchKernelGGL(k_leaf_dist_sq, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, X, ld,
-                                   dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
-            else
-                hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X,
-                                   ld, dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
-            dist_timer.stop(s);
-            push_timer.begin(s);
-            const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
-            if (early)
-                launch_leaf_push<16>(s, pgrid, PB, init_heap_bytes(PB + 1, k) + (size_t)PB * 17 * 4, tree_indices, leaves,
-                                     ws, n, P, idx, dist, flags, k);
-            else
-                launch_leaf_push<32>(s, pgrid, PB, init_heap_bytes(PB + 1, k) + (size_t)PB * 33 * 4, tree_indices, leaves,
-                                     ws, n, P, idx, dist, flags, k);
-            push_timer.stop(s);
-            // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
-            // per-pair parts (4 B of D written per ordered pair; 4 B of D and 4 B of partner id read) once it
-            // has read the pair counter
-            // dist: each member's row (bf16 + norm on MFMA) and id once
-            dist_timer.collect((double)n * ((mfma ? ws.exact_bf16->ldb * 2.0 + 4.0 : dim * 4.0) + 4.0));
-            // push: every member's heap read and written, its leaf lookup
-            push_timer.collect((double)n * k * 9.0 * 2.0 + n * 40.0);
-            untiled_cap = P.mcap;
-        }
+        dist_timer.begin(s);
+        if (mfma)
+            launch_leaf_dist_mfma(s, *ws.exact_bf16, tree_indices, leaves, n_leaves_dev, max_leaves, P.mcap, ws.D,
+                                  P.Dl4, ws.pairs);
+        else if (P.chunked)
+            hipLaunchKernelGGL(k_leaf_dist_sq_kc<kLeafKC>, dim3(std::min<unsigned>(grid, 4096)), dim3(512), P.rows_lds,
+                               s, X, ld, dim, tree_indices, leaves, n_leaves_dev, P.mcap, ws.D, P.Dl4, ws.pairs);
+        else if (metric == RINND_DIST_SQEUCLIDEAN)
+            hipLaunchKernelGGL(k_leaf_dist_sq, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, X, ld,
+                               dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
+        else
+            hipLaunchKernelGGL(k_leaf_dist, dim3(std::min<unsigned>(grid, 4096)), dim3(256), P.rows_lds, s, metric, X,
+                               ld, dim, tree_indices, leaves, n_leaves_dev, P.mcap, ldl, ws.D, P.Dl4, ws.pairs);
+        dist_timer.stop(s);
+        push_timer.begin(s);
+        const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
+        if (early)
+            launch_leaf_push<16>(s, pgrid, PB, init_heap_bytes(PB + 1, k) + (size_t)PB * 17 * 4, tree_indices, leaves,
+                                 ws, n, P, idx, dist, flags, k);
+        else
+            launch_leaf_push<32>(s, pgrid, PB, init_heap_bytes(PB + 1, k) + (size_t)PB * 33 * 4, tree_indices, leaves,
+                                 ws, n, P, idx, dist, flags, k);
+        push_timer.stop(s);
+        // the parts of the algorithmic bytes that do not depend on the leaf sizes; the caller adds the
+        // per-pair parts (4 B of D written per ordered pair; 4 B of D and 4 B of partner id read) once it
+        // has read the pair counter
+        // dist: each member's row (bf16 + norm on MFMA) and id once
+        dist_timer.collect((double)n * ((mfma ? ws.exact_bf16->ldb * 2.0 + 4.0 : dim * 4.0) + 4.0));
+        // push: every member's heap read and written, its leaf lookup
+        push_timer.collect((double)n * k * 9.0 * 2.0 + n * 40.0);
+        untiled_cap = P.mcap;
     } else {
         const int64_t cap_rows = ((int64_t)kInitLdsBudget - (int64_t)heap) / (ldl * 4);
         untiled_cap = (int)std::max<int64_t>(0, std::min<int64_t>(cap_rows, std::max<int64_t>(leaf_size, 1)));
@@ -569,6 +680,57 @@ void launch_leaf_init(hipStream_t s, int metric, const float* X, int64_t ld, int
     hipLaunchKernelGGL(k_leaf_init<false>, dim3(std::min<unsigned>(grid, 4096)), dim3(B), heap, s, metric, X, ld,
                        dim, tree_indices, leaves, n_leaves_dev, untiled_cap, ldl, idx, dist, flags, k);
     HIP_TRY(hipGetLastError());
+}
+
+double surv_group_budget() {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        total_b = (size_t)64 << 30;
+    }
+    return std::min((double)total_b / 8.0, 40.0 * (double)(1ull << 30));
+}
+
+int64_t leaf_surv_dl4(int metric, const Bf16Rows* R, int64_t ld, int64_t leaf_size, int k) {
+    if (!R || !R->exact || metric != RINND_DIST_SQEUCLIDEAN) return 0;
+    const TilePlan P = tile_plan(ld, leaf_size);
+    if (P.mcap <= 0 || P.chunked || leaf_surv_mfma_lds(R->ldb, P.mcap) > 150 * 1024) return 0;
+    if (init_heap_bytes(push_block(k) + 1, k) > 64 * 1024) return 0;
+    return P.Dl4;
+}
+
+void launch_leaf_surv_tree(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
+                           const int64_t* n_leaves_dev, int64_t n, int64_t leaf_size, const float* hdist, int k,
+                           const SurvGroup& grp, int g, unsigned long long* surv, const KTimer& timer) {
+    if (n <= 0) return;
+    const int mcap = (int)std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 256);
+    const int64_t Dl4 = round_up((int64_t)mcap, (int64_t)4);
+    timer.begin(s);
+    launch_leaf_surv_mfma(s, R, tree_indices, leaves, n_leaves_dev, n, mcap, hdist, k, grp.D + (int64_t)g * n * Dl4,
+                          Dl4, grp.Msk + (int64_t)g * n * 8, grp.info + (int64_t)g * n, surv);
+    timer.stop(s);
+    // per point: bf16 row, norm, root and id read; mask and leaf info written (survivors added by the caller)
+    timer.collect((double)n * (R.ldb * 2.0 + 12.0 + 32.0 + 8.0));
+}
+
+void launch_leaf_push_group(hipStream_t s, const SurvGroup& grp, const float* X, int64_t ld, int dim, int64_t n,
+                            int64_t leaf_size, int32_t* idx, float* dist, uint8_t* flags, int k, const KTimer& timer) {
+    if (n <= 0 || grp.G <= 0) return;
+    const int mcap = (int)std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 256);
+    const int64_t Dl4 = round_up((int64_t)mcap, (int64_t)4);
+    const int PB = push_block(k);
+    const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(n, (int64_t)PB), 1 << 16);
+    timer.begin(s);
+    if (k <= 16)
+        hipLaunchKernelGGL(k_leaf_push_group<true>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s, grp, X, ld,
+                           dim, n, mcap, Dl4, idx, dist, flags, k);
+    else
+        hipLaunchKernelGGL(k_leaf_push_group<false>, dim3(pgrid), dim3(PB), init_heap_bytes(PB + 1, k), s, grp, X, ld,
+                           dim, n, mcap, Dl4, idx, dist, flags, k);
+    HIP_TRY(hipGetLastError());
+    timer.stop(s);
+    // per point: heap read and written once; per tree its leaf info and mask (survivors added by the caller)
+    timer.collect((double)n * k * 9.0 * 2.0 + (double)n * grp.G * (8.0 + 32.0));
 }
 
 }  // namespace rinnd

@@ -242,10 +242,12 @@ def test_nn_descent_bit_exact(ctx, n, d, k, leaf, threads):
     np.testing.assert_array_equal(gst, r["rng_state"])
 
 
-@pytest.mark.parametrize("leaf,d", [(200, 128), (140, 64), (40, 300), (150, 300)])
+@pytest.mark.parametrize("leaf,d", [(200, 128), (140, 64), (40, 300), (150, 300), (256, 520)])
 def test_leaf_init_paths(ctx, leaf, d):
-    """Leaf initialisation through the two-pass kernels (rows of a leaf fit in LDS) and the single-pass
-    kernels (150 rows of 300 dims do not: small leaves staged, large ones read from memory)."""
+    """Leaf initialisation through the two-pass kernels: rows of a leaf staged whole in LDS, or (150 rows of 300
+    dims, 256 rows of 520) staged 64 dimensions at a time with the pairs' accumulators kept in registers - a last
+    partial chunk, a scalar tail (300 = 37 x 8 + 4) and leaves with more 4 x 4 tiles than threads (several passes
+    over the rows)."""
     X = uniform(4000, d, seed=leaf)
     gi, gd, gf, gs, gst, _ = gpu_nn_descent(ctx, X, 15, 2, leaf, 15, 1, threads=4)
     r = O.nn_descent(X, 15, 2, leaf, 15, 1, threads=4)
@@ -330,10 +332,10 @@ def test_sort_rows_matches_sort_all(ctx):
     np.testing.assert_array_equal(gf, rf)
 
 
-def _build(ctx, X, k, trees, leaf, C_, iters, rng_mode, precision, seed=7, threads=4):
+def _build(ctx, X, k, trees, leaf, C_, iters, rng_mode, precision, seed=7, threads=4, max_depth=200):
     n, d = X.shape
     p = L.make_params(k, trees, leaf, C_, iters, 0.001, n_threads=threads, rng_mode=rng_mode,
-                      join_precision=precision)
+                      join_precision=precision, max_depth=max_depth)
     st = L.rng_seed(seed)
     idx = np.zeros((n, k), np.int32)
     dist = np.zeros((n, k), np.float32)
@@ -371,6 +373,30 @@ def test_bf16_exact_leaf_init_matches_oracle(ctx, leaf, k):
     _assert_same_build(b, r["indices"], r["distances"], r["flags"], r["changes"])
     np.testing.assert_array_equal(b[4], r["rng_state"])
     assert sum(b[3]["join_rescored"]) == 0
+
+
+@pytest.mark.parametrize("k,leaf,depth", [(10, 40, 3), (20, 256, 2), (15, 60, 5)])
+def test_bf16_exact_oversized_leaves(ctx, k, leaf, depth):
+    """A depth limit leaves leaves above leaf_size (and above the survivor kernel's 256 members): in the
+    survivor trees the push pass evaluates those leaves itself, in the reference's order, and the build still
+    equals the oracle's, duplicate rows included."""
+    rng = np.random.default_rng(k)
+    X = _integer_mixture(rng, 3000, 32, 12, 4)
+    X[rng.choice(3000, 300, replace=False)] = X[5]  # 300 copies of one row
+    b = _build(ctx, X, k, 7, leaf, k, 4, "reference", "bf16", max_depth=depth)
+    r = O.nn_descent(X, k, 7, leaf, k, 4, 0.001, max_depth=depth, seed=7, threads=4)
+    _assert_same_build(b, r["indices"], r["distances"], r["flags"], r["changes"])
+    np.testing.assert_array_equal(b[4], r["rng_state"])
+
+
+def test_bf16_exact_many_survivor_trees(ctx):
+    """Eighteen survivor trees: several groups (a full one and a partial last one), each replayed by one push
+    pass against the roots at its start."""
+    X = _integer_mixture(np.random.default_rng(3), 3000, 64, 20, 5)
+    b = _build(ctx, X, 12, 21, 30, 12, 3, "reference", "bf16")
+    r = O.nn_descent(X, 12, 21, 30, 12, 3, 0.001, seed=7, threads=4)
+    _assert_same_build(b, r["indices"], r["distances"], r["flags"], r["changes"])
+    np.testing.assert_array_equal(b[4], r["rng_state"])
 
 
 @pytest.mark.parametrize("leaf,k", [(256, 15), (100, 30), (17, 8)])
