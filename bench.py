@@ -81,6 +81,13 @@ def parse():
                     help="queries of the serving-form stream (the stated form has 4,000,000; the default run streams "
                          "1,000,000 of the same form to stay within a few minutes)")
     ap.add_argument("--sv-load", type=float, default=0.7, help="offered Poisson rate as a fraction of the saturated QPS")
+    ap.add_argument("--sv-modes", default="fp32,fp32_w32,fp8",
+                    help="searches of the serving form: fp32 (the reference's query), fp32_wN (search-width floor N), "
+                         "fp8 (FP8 candidates + FP32 re-ranking)")
+    ap.add_argument("--sv-visited-gb", type=float, default=24.0,
+                    help="visited-set pool of the query service (GiB; n bytes per concurrent search)")
+    ap.add_argument("--sv-inflight", type=int, default=4, help="launches in flight in the query service")
+    ap.add_argument("--sv-target", type=int, default=1024, help="queries at which a launch of the query service closes")
     return ap.parse_args()
 
 
@@ -599,7 +606,10 @@ def serving_form(torch, ctx, L, args):
     out = {"workload": f"{n}x{d} unit-norm Gaussian mixture (500 centres N(0, I), sigma 0.2, normalised), "
                        f"DirectNormalizedCosine, graph k=20 "
                        f"(search degree <= 30), {nq} queries, arrival batches 1-4096 (log-uniform), k in {{10, 100}} "
-                       f"(4:1), epsilon in {{0.05, 0.2}} (1:1)", "index_build_s": t_index, "batches": len(sizes)}
+                       f"(4:1), epsilon in {{0.05, 0.2}} (1:1)", "index_build_s": t_index, "batches": len(sizes),
+           "server": {"max_batch": 4096, "max_inflight": args.sv_inflight, "target_batch": args.sv_target,
+                      "visited_gib": args.sv_visited_gb,
+                      "concurrent_searches": int(min(16384, args.sv_visited_gb * (1 << 30) // (n + 8 * 65536 + 4)))}}
 
     def widths(k, fp8):
         return (64 if k == 10 else 128, 64 if k == 10 else 128) if fp8 else (None, None)
@@ -657,10 +667,12 @@ def serving_form(torch, ctx, L, args):
         res["all"] = float(hit.mean())
         return res
 
-    for mode in ("fp32", "fp8"):
+    for mode in args.sv_modes.split(","):
         fp8 = mode == "fp8"
-        srv = L.Server(h, max_batch=4096, max_inflight=4, max_k=128, visited_bytes=8 << 30, quantized=fp8,
-                       target_batch=1024)
+        floor = int(mode[len("fp32_w"):]) if mode.startswith("fp32_w") else 0
+        srv = L.Server(h, max_batch=4096, max_inflight=args.sv_inflight, max_k=128,
+                       visited_bytes=int(args.sv_visited_gb * (1 << 30)), quantized=fp8, target_batch=args.sv_target,
+                       min_width=floor)
         wall, lat = run(srv, fp8, None)  # saturated
         qps_max = nq / wall
         rec = recall()
@@ -683,6 +695,8 @@ def serving_form(torch, ctx, L, args):
                      "gpu_ms": st["gpu_ms"]}
         if fp8:
             out[mode]["widths"] = "candidate/rerank 64 for k=10, 128 for k=100"
+        if floor:
+            out[mode]["min_width"] = floor
         log(f"[bench] serving {mode}: Poisson {nq / wall_p:.0f} QPS (offered {rate_q:.0f}), p50 {pq(0.5):.2f} ms, "
             f"p99 {pq(0.99):.2f} ms, {st['launches']} launches")
     L.lib().rinnd_index_free(h)

@@ -304,8 +304,9 @@ rinnd_status rinnd_index_query_noalloc(rinnd_index* index, const float* queries,
  * max_batch per launch; launch buffers are fixed, max_inflight launches at a time, each on its own stream) and its
  * search width fits the beam memory (max_k result-heap entries per query); the concurrent searches share a fixed
  * pool of visited sets (n bytes + an 8-byte entry per frontier vertex, at most 65,536 frontier entries per query)
- * sized by visited_bytes and split evenly between the launch slots.  Requests that do not fit wait in submission
- * order. */
+ * sized by visited_bytes and split evenly between the launch slots (at most 16,384 concurrent searches).  Requests
+ * that do not fit wait in submission order.  A search-width floor (min_width) trades throughput for recall at small
+ * k: such a request's results are the first k of rinnd_index_query with k = max(k, min_width). */
 typedef struct rinnd_server rinnd_server;
 typedef struct rinnd_server_params {
     int64_t max_batch;      /* queries per launch (and per request), e.g. 4096 */
@@ -316,6 +317,8 @@ typedef struct rinnd_server_params {
     int64_t candidate_width;/* quantized: as rinnd_index_query_quantized's candidate_width */
     int64_t rerank_width;   /* quantized: as rinnd_index_query_quantized's rerank_width */
     int64_t target_batch;   /* a launch closes once it holds this many queries (0: min(1024, max_batch)) */
+    int64_t min_width;      /* FP32 server: a request of k results searches with width max(k, min_width) and
+                               returns the first k of those results (0: width k, the reference's query) */
 } rinnd_server_params;
 typedef struct rinnd_server_stats {
     int64_t requests, queries, launches, max_launch_queries;

@@ -118,7 +118,7 @@ class ServerParams(C.Structure):
     _fields_ = [
         ("max_batch", C.c_int64), ("max_inflight", C.c_int32), ("max_k", C.c_int64), ("visited_bytes", C.c_int64),
         ("quantized", C.c_int32), ("candidate_width", C.c_int64), ("rerank_width", C.c_int64),
-        ("target_batch", C.c_int64),
+        ("target_batch", C.c_int64), ("min_width", C.c_int64),
     ]
 
 
@@ -322,9 +322,9 @@ class Server:
     """rinnd_server: asynchronous batched queries over a resident index (handle of rinnd_index_build)."""
 
     def __init__(self, index_handle, max_batch=4096, max_inflight=3, max_k=128, visited_bytes=8 << 30,
-                 quantized=False, candidate_width=0, rerank_width=0, target_batch=0):
+                 quantized=False, candidate_width=0, rerank_width=0, target_batch=0, min_width=0):
         p = ServerParams(int(max_batch), int(max_inflight), int(max_k), int(visited_bytes), int(bool(quantized)),
-                         int(candidate_width), int(rerank_width), int(target_batch))
+                         int(candidate_width), int(rerank_width), int(target_batch), int(min_width))
         h = C.c_void_p()
         st = lib().rinnd_server_create(index_handle, C.byref(p), C.byref(h))
         if st != OK:

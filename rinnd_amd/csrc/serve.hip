@@ -294,7 +294,11 @@ rinnd_status rinnd_server_create(rinnd_index* ix, const rinnd_server_params* par
     try {
         if (p.max_batch > (1 << 20) || p.max_inflight > 64 || p.max_k > 512)
             throw Error{RINND_ERR_INVALID_ARG, "server limits: max_batch <= 2^20, max_inflight <= 64, max_k <= 512"};
-        if (p.candidate_width < 0 || p.rerank_width < 0) throw Error{RINND_ERR_INVALID_ARG, "negative width"};
+        if (p.candidate_width < 0 || p.rerank_width < 0 || p.min_width < 0)
+            throw Error{RINND_ERR_INVALID_ARG, "negative width"};
+        if (p.min_width > p.max_k) throw Error{RINND_ERR_INVALID_ARG, "min_width exceeds max_k"};
+        if (p.quantized && p.min_width > 0)
+            throw Error{RINND_ERR_INVALID_ARG, "min_width applies to FP32 servers (quantized servers take widths)"};
         HIP_TRY(hipSetDevice(S->device));
         if (p.quantized) {
             std::lock_guard<std::mutex> lk(ix->ctx->mu);  // the codes are built on the index's stream
@@ -305,7 +309,7 @@ rinnd_status rinnd_server_create(rinnd_index* ix, const rinnd_server_params* par
         S->seed_cap = std::max<int64_t>(1, std::min<int64_t>(ix->n, kServeSeedCap));
         const int64_t per_slot = ix->n + 8 * S->seed_cap + 4;
         // each launch slot owns an equal share of the visited sets (concurrent launches never share one)
-        const int64_t total_sets = std::max<int64_t>(p.max_inflight, std::min<int64_t>(4096, p.visited_bytes / per_slot));
+        const int64_t total_sets = std::max<int64_t>(p.max_inflight, std::min<int64_t>(16384, p.visited_bytes / per_slot));
         S->visit_slots = (int)(total_sets / p.max_inflight);
         const int64_t sets = (int64_t)S->visit_slots * p.max_inflight;
         S->visited.alloc((size_t)sets * std::max<int64_t>(ix->n, 1));
@@ -368,7 +372,7 @@ static rinnd_status submit(rinnd_server* S, const float* queries, int64_t nq, in
     r->eps = epsilon;
     r->out_i = out_idx;
     r->out_d = out_dist;
-    r->width = k;
+    r->width = std::max<int64_t>(k, S->p.min_width);  // search-width floor of an FP32 server
     if (S->p.quantized) {  // resolve_quantized_candidate_width / should_rerank_quantized (core/index.rs:55-90)
         if (cw < 0 || rw < 0) return fail(RINND_ERR_INVALID_ARG, "negative width");
         r->width = cw == 0 ? std::max(k, rw) : cw;

@@ -165,3 +165,26 @@ def test_admission_limits_and_tickets(setup):
     t0 = srv.submit(q, 0, 10, 0.1, oi, od)  # an empty request completes at once
     assert srv.poll(t0)
     srv.close()
+
+
+def test_search_width_floor(setup):
+    """min_width: a request of k results searches with width max(k, min_width) and returns the first k of those
+    results, i.e. the first k columns of rinnd_index_query at that width; requests wider than the floor search as
+    usual.  Limits: min_width <= max_k, and FP32 servers only."""
+    X, Q, g = setup
+    srv = L.Server(g._h, max_batch=512, max_inflight=2, max_k=64, min_width=40)
+    reqs = []
+    for i, (nq, k, eps) in enumerate([(300, 10, 0.05), (200, 40, 0.2), (100, 50, 0.1), (7, 1, 0.2)]):
+        q = Q[i * 500: i * 500 + nq]
+        oi, od = np.full((nq, k), -7, np.int32), np.full((nq, k), -7.0, np.float32)
+        reqs.append((srv.submit(q, nq, k, eps, oi, od), q, k, eps, oi, od))
+    for t, q, k, eps, oi, od in reqs:
+        srv.wait(t)
+        wi, wd = direct(g, q, max(k, 40), eps)
+        np.testing.assert_array_equal(oi, wi[:, :k])
+        np.testing.assert_array_equal(od.view(np.uint32), wd[:, :k].view(np.uint32))
+    srv.close()
+    with pytest.raises(Exception):
+        L.Server(g._h, max_batch=64, max_k=32, min_width=40)
+    with pytest.raises(Exception):
+        L.Server(g._h, max_batch=64, max_k=64, quantized=True, min_width=40)
