@@ -171,6 +171,7 @@ __device__ __forceinline__ void lds_barrier() {
 // operand and accumulator vectors of the 16x16x16 bf16 MFMA
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));  // A/B operand of a 16x16x32 bf16 MFMA
 
 // Order-preserving float -> u32 map; -0.0 is folded onto +0.0 so that ties compare equal like
 // Rust's partial_cmp (neighbor_heap.rs:308-310).

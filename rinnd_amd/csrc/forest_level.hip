@@ -412,6 +412,148 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
     }
 }
 
+// Shallow dense levels on MFMA: while the level's active entries (over all trees) number at most NE, a
+// workgroup stages every split entry's bf16 plane in LDS once, and each wave multiplies blocks of 16 points by
+// all of them: the point's f32 row split into bf16 hi + lo parts (x = hi + lo up to 2^-16 relative), 16x16x32
+// bf16 MFMAs with f32 accumulation, the 16 x na margins staged in LDS.  Then the wave's 64 lanes take
+// (point p = lane % 16, trees lane / 16 + 4 i): the point's entry advances as in lvl_info, its margin is read
+// at that entry, and the sides become the point's 16-bit mask words (assembled from ballots).  The margins are
+// the same sums as the VALU kernels' in another order; a point within rounding of a plane may take the other
+// side, which any rounding order allows.  Needs dim % 32 == 0, dim <= 256, n_trees <= 64.
+template <int KS>  // dim / 32
+__global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNode* __restrict__ cur, int na,
+                                                          const PInfo* __restrict__ pinfo, int n_trees,
+                                                          const CInfo* __restrict__ cinfo, int32_t* nodeOf,
+                                                          uint16_t* mask16) {
+    extern __shared__ __align__(16) uint16_t msm[];
+    const int na16 = (na + 15) & ~15;
+    const int pld = KS * 32 + 8;  // staged plane stride (bf16): 16-byte aligned, rows on different banks
+    float* eoff = (float*)(msm + (int64_t)na16 * pld);   // per entry: offset
+    int32_t* enode = (int32_t*)(eoff + na16);            // per entry: node id, -1 if it does not split
+    float* csm = (float*)(enode + na16);                 // per wave: na16 x 16 margins (entry-major)
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int mrow = lane & 15, kqi = lane >> 4;
+    // stage the planes of the split entries (zeros elsewhere)
+    const int v8 = KS * 4;  // 16-byte chunks per plane
+    for (int i = t; i < na16 * v8; i += blockDim.x) {
+        const int e = i / v8, c = i - e * v8;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (e < na) {
+            const PInfo pi = pinfo[e];
+            if (pi.node >= 0)
+                v = *reinterpret_cast<const uint4*>(a.hb0 + ((int64_t)cur[e].tree * a.cap + pi.node) * a.dim + 8 * c);
+        }
+        *reinterpret_cast<uint4*>(msm + e * pld + 8 * c) = v;
+    }
+    for (int e = t; e < na16; e += blockDim.x) {
+        const PInfo pi = e < na ? pinfo[e] : PInfo{-1, 0.f};
+        enode[e] = pi.node;
+        eoff[e] = pi.off;
+    }
+    __syncthreads();
+    float* cw = csm + (int64_t)w * na16 * 16;
+    const int64_t nblk = (a.n + 15) >> 4;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6);
+    // software pipeline: the next block's rows and (first 16 trees) entries and mask word are loaded while the
+    // current block is evaluated
+    float4 rf[KS][2];
+    int32_t pe0[4];
+    uint16_t pmw = 0;
+    auto prefetch = [&](int64_t blk) {
+        const int64_t pr0 = (blk << 4) + mrow;
+        const bool ok = blk < nblk && pr0 < a.n;
+        const int64_t pr = ok ? pr0 : 0;  // every address stays in bounds
+        const float* x = a.X + pr * a.ld;
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            rf[kk][0] = ok ? *reinterpret_cast<const float4*>(x + 32 * kk + 8 * kqi) : make_float4(0.f, 0.f, 0.f, 0.f);
+            rf[kk][1] = ok ? *reinterpret_cast<const float4*>(x + 32 * kk + 8 * kqi + 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) {
+            const int tr = kqi + 4 * ii;
+            pe0[ii] = ok && tr < n_trees ? nodeOf[(int64_t)(tr < n_trees ? tr : 0) * a.n + pr] : -1;
+        }
+        pmw = ok && cinfo ? mask16[pr] : (uint16_t)0;
+    };
+    int64_t blk = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
+    prefetch(blk);
+    for (; blk < nblk; blk += wstride) {  // wave-uniform
+        const int64_t p0 = blk << 4;
+        // A operands: row p0 + mrow, dims 32 kk + 8 kqi .. + 7, as bf16 hi and lo parts
+        v8bf ah[KS], al[KS];
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const float f[8] = {rf[kk][0].x, rf[kk][0].y, rf[kk][0].z, rf[kk][0].w,
+                                rf[kk][1].x, rf[kk][1].y, rf[kk][1].z, rf[kk][1].w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float h = bf16_round(f[j]);
+                ah[kk][j] = __builtin_bit_cast(__bf16, (uint16_t)(__builtin_bit_cast(uint32_t, h) >> 16));
+                const float l = bf16_round(f[j] - h);
+                al[kk][j] = __builtin_bit_cast(__bf16, (uint16_t)(__builtin_bit_cast(uint32_t, l) >> 16));
+            }
+        }
+        int32_t ce0[4];
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) ce0[ii] = pe0[ii];
+        const uint16_t cmw = pmw;
+        prefetch(blk + wstride);
+        wave_sync();  // the previous block's margins are read
+        for (int ct = 0; ct < (na16 >> 4); ++ct) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            const uint16_t* brow = msm + (16 * ct + mrow) * pld + 8 * kqi;
+#pragma unroll
+            for (int kk = 0; kk < KS; ++kk) {
+                const v8bf b = *reinterpret_cast<const v8bf*>(brow + 32 * kk);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kk], b, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kk], b, acc, 0, 0, 0);
+            }
+            // rows 4 kqi .. + 3 (points), column 16 ct + mrow (entry)
+            *reinterpret_cast<float4*>(cw + (16 * ct + mrow) * 16 + 4 * kqi) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        wave_sync();
+        // sides: lane (point p0 + mrow, trees kqi + 4 i); mask word j of the point holds trees 16 j .. 16 j + 15,
+        // tree 16 j + g + 4 ii being bit (16 g + mrow) of the ballot of i = 4 j + ii
+        const int64_t p = p0 + mrow;
+        const bool pv = p < a.n;
+        for (int j = 0; 16 * j < n_trees; ++j) {  // uniform
+            uint32_t wbits = 0u;
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) {
+                const int tr = 16 * j + kqi + 4 * ii;
+                uint32_t left = 0u;
+                if (pv && tr < n_trees) {
+                    const int64_t ti = (int64_t)tr * a.n + p;
+                    const int32_t e0 = j == 0 ? ce0[ii] : nodeOf[ti];
+                    int32_t e = -1, node = -1;
+                    float off = 0.f;
+                    if (e0 >= 0) {
+                        if (!cinfo) {
+                            e = e0;
+                            node = enode[e];
+                            off = eoff[e];
+                        } else {
+                            const uint32_t mw = j == 0 ? cmw : mask16[(int64_t)j * a.n + p];
+                            const bool l0 = (mw >> (tr & 15)) & 1u;
+                            const CInfo c = cinfo[2 * (int64_t)e0 + (l0 ? 0 : 1)];
+                            nodeOf[ti] = c.entry;
+                            e = c.entry;
+                            node = c.entry >= 0 ? c.node : -1;
+                            off = c.off;
+                        }
+                    }
+                    if (node >= 0) left = side_of(cw[e * 16 + mrow] + off, a.keys[tr], node, p);
+                }
+                const uint64_t bal = __ballot(left != 0u);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) wbits |= (uint32_t)((bal >> (16 * g + mrow)) & 1ull) << (g + 4 * ii);
+            }
+            if (kqi == 0 && pv) mask16[(int64_t)j * a.n + p] = (uint16_t)wbits;
+        }
+    }
+}
+
 // Sparse levels, position-major: a wave reads 64 consecutive (tree, position) items of the act table,
 // and its four 16-lane groups test the active ones four at a time: the member's row is gathered through
 // perm and the 16 lanes split the dimensions; flags are written at the positions.  Needs dim % 4 == 0.
@@ -736,6 +878,28 @@ static void launch_margins_pm(hipStream_t s, const LvlArgs& a, const PInfo* pinf
         hipLaunchKernelGGL(k_lvl_margins_pm<8>, dim3(grid), dim3(256), 0, s, a, pinfo, T, cinfo, nodeOf, mask16);
 }
 
+constexpr int kMfmaEntries = 128;  // the MFMA side tests stage at most this many planes per workgroup
+static bool mfma_levels_ok(int dim, int T) { return dim % 32 == 0 && dim <= 256 && T <= 64; }
+static void launch_margins_mfma(hipStream_t s, const LvlArgs& a, const LvlNode* cur, int na, const PInfo* pinfo, int T,
+                                const CInfo* cinfo, int32_t* nodeOf, uint16_t* mask16) {
+    const int na16 = (na + 15) & ~15;
+    const size_t lds = (size_t)na16 * (a.dim + 8) * 2 + (size_t)na16 * 8 + 4 * (size_t)na16 * 16 * 4;
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+    const int64_t nblk = (a.n + 15) >> 4;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nblk, (int64_t)4),
+                                                                           (int64_t)device_cus() * per_cu));
+    switch (a.dim / 32) {
+        case 1: hipLaunchKernelGGL(k_lvl_margins_mfma<1>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+        case 2: hipLaunchKernelGGL(k_lvl_margins_mfma<2>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+        case 3: hipLaunchKernelGGL(k_lvl_margins_mfma<3>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+        case 4: hipLaunchKernelGGL(k_lvl_margins_mfma<4>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+        case 5: hipLaunchKernelGGL(k_lvl_margins_mfma<5>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+        case 6: hipLaunchKernelGGL(k_lvl_margins_mfma<6>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+        case 7: hipLaunchKernelGGL(k_lvl_margins_mfma<7>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+        default: hipLaunchKernelGGL(k_lvl_margins_mfma<8>, dim3(grid), dim3(256), lds, s, a, cur, na, pinfo, T, cinfo, nodeOf, mask16); break;
+    }
+}
+
 bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int64_t leaf_size,
                         int64_t max_depth, int angular, const std::vector<uint64_t>& keys, rinnd_forest& F,
                         const KTimer& margins_timer, Workspace& W) {
@@ -849,6 +1013,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     // tested pairs of a level fall to kSparsePairs per point, which they never exceed again.
     constexpr int64_t kSparsePairs = 2;
     const bool point_major_dpp = dim % 8 == 0 && dim <= 1024;
+    const bool mfma_levels = mfma_levels_ok(dim, T);  // shallow dense levels: planes in LDS, MFMA side tests
     bool sparse = false;
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
@@ -860,6 +1025,8 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         if (sparse)
             hipLaunchKernelGGL(k_lvl_margins_pos, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, a, pinfo.p, T,
                                (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags8.p);
+        else if (mfma_levels && na <= kMfmaEntries)
+            launch_margins_mfma(s, a, cur, (int)na, pinfo.p, T, cinfo_prev, nodeOf.p, mask16.p);
         else if (point_major_dpp)
             launch_margins_pm(s, a, pinfo.p, T, cinfo_prev, nodeOf.p, mask16.p);
         else if (dim % 4 == 0 && dim <= 64)

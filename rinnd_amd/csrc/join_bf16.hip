@@ -362,7 +362,6 @@ __device__ __forceinline__ void tri_pair16(int q, int m, int& a, int& o) {
 // 16-column tail takes one 16x16x16 step).  ra / ro point at this lane's A row (block row + lane % 16) and
 // O row in LDS; lane group g = lane / 16 holds k = 8 g .. 8 g + 7 of each 32-wide step for both operands, so
 // every product pairs the same k.  Result: acc[r] = G(16 A + 4 g + r, 16 O + lane % 16).
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ v4f gram_block16(const uint16_t* ra, const uint16_t* ro, int ldb, int g) {
     v4f acc = {0.f, 0.f, 0.f, 0.f};
     int k0 = 0;
