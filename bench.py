@@ -84,7 +84,7 @@ def parse():
     ap.add_argument("--sv-modes", default="fp32,fp32_w32,fp8",
                     help="searches of the serving form: fp32 (the reference's query), fp32_wN (search-width floor N), "
                          "fp8 (FP8 candidates + FP32 re-ranking)")
-    ap.add_argument("--sv-visited-gb", type=float, default=24.0,
+    ap.add_argument("--sv-visited-gb", type=float, default=8.0,
                     help="visited-set pool of the query service (GiB; n bytes per concurrent search)")
     ap.add_argument("--sv-inflight", type=int, default=4, help="launches in flight in the query service")
     ap.add_argument("--sv-target", type=int, default=1024, help="queries at which a launch of the query service closes")
