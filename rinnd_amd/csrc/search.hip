@@ -16,7 +16,10 @@
 
 namespace rinnd {
 
-constexpr int kSeedLds = 768;
+#ifndef RINND_SEED_LDS
+#define RINND_SEED_LDS 2048
+#endif
+constexpr int kSeedLds = RINND_SEED_LDS;  // seed-heap entries a wave keeps in LDS before it spills to memory
 constexpr int kMaxK = 512;
 constexpr int kMaxDeg = 256;
 constexpr int kFrontier = 256;

@@ -143,3 +143,16 @@ def test_inner_product_and_budget_paths():
     qi, _ = g.query(Q, k=8, epsilon=0.2)
     oi, _ = r.query(Q, k=8, epsilon=0.2)
     np.testing.assert_array_equal(qi, oi)
+
+
+def test_wide_queries_with_spilled_frontier():
+    """A large epsilon and k widen the search until the frontier (the seed min-heap) outgrows the 2,048 entries a
+    wave keeps in LDS and moves to the per-slot memory region; results still equal the oracle's bit for bit."""
+    X = uniform(12000, 16, seed=5)
+    Q = uniform(40, 16, seed=6)
+    g, r = make_pair(X, n_neighbors=20, n_iters=6, n_trees=4, threads=4)
+    for k, eps in ((100, 3.0), (40, 8.0)):
+        qi, qd = g.query(Q, k=k, epsilon=eps)
+        oi, od = r.query(Q, k=k, epsilon=eps)
+        np.testing.assert_array_equal(qi, oi)
+        np.testing.assert_array_equal(qd.view(np.uint32), od.view(np.uint32))
