@@ -225,6 +225,38 @@ __device__ __forceinline__ float fp8_lane_dot(const uint8_t* __restrict__ c, con
     return dot;
 }
 
+// Sorts the first len entries of (d, id) by (d, id) ascending with a wave-wide bitonic network in LDS (padding to
+// a power of two with (+inf, INT32_MAX)); the sorted array is a valid min-heap.  len <= capacity, the padded
+// length must fit the arrays too.  Every lane of the wave calls it.
+__device__ void wave_sort_pairs(float* d, int32_t* id, int len, int lane) {
+    int P = 1;
+    while (P < len) P <<= 1;
+    for (int i = len + lane; i < P; i += 64) {
+        d[i] = __builtin_inff();
+        id[i] = 0x7fffffff;
+    }
+    wave_sync();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < P; i += 64) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const float di = d[i], dl = d[l];
+                    const int32_t ii = id[i], il = id[l];
+                    const bool gt = di > dl || (di == dl && ii > il);  // entry i after entry l
+                    if (gt == ((i & k) == 0)) {                       // ascending halves where (i & k) == 0
+                        d[i] = dl;
+                        d[l] = di;
+                        id[i] = il;
+                        id[l] = ii;
+                    }
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
 // One instance per metric (METRIC >= 0; the switch folds away, so only that metric's accumulators take
 // registers) and one for any metric (METRIC < 0).
 template <bool Q8, int METRIC>
@@ -329,6 +361,12 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             }
         };
 
+        // The seeds enter the frontier in bulk: a min-heap over unique (distance, id) keys pops in key order
+        // whatever its layout, so the seeds are appended in parallel and the array is sorted once (a sorted array
+        // is a heap); a frontier that would outgrow the LDS array is sorted at that point and takes the seeds
+        // after it one by one (seed_push, which spills to memory).  The result heap takes every push in the
+        // reference's order, its layout decides the order of ties in into_sorted.
+        bool bulk = true;
         // ---- initialize_fp32 (greedy.rs:70-99) ----
         for (int t = 0; t < a.n_trees; ++t) {
             const int32_t* ch = a.children[t];
@@ -408,6 +446,24 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                     S.nb_i[lane] = x;
                     S.nb_d[lane] = d;
                     S.nb_ok[lane] = ok;
+                    const uint64_t okm = __ballot(ok);
+                    const int64_t len0 = S.sh_len;  // uniform: lane 0 last wrote it before a wave sync
+                    bool appended = false;
+                    if (bulk) {
+                        if (len0 + __popcll(okm) <= kSeedLds) {
+                            if (ok) {
+                                const int64_t pos = len0 + __popcll(okm & ((1ull << lane) - 1ull));
+                                S.sh_d[pos] = d;
+                                S.sh_i[pos] = x;
+                            }
+                            appended = true;
+                        } else {
+                            __threadfence_block();
+                            wave_sync();
+                            wave_sort_pairs(S.sh_d, S.sh_i, (int)len0, lane);
+                            bulk = false;
+                        }
+                    }
                     __threadfence_block();
                     wave_sync();
                     if (lane == 0) {
@@ -415,8 +471,9 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                         for (int j = 0; j < cnt; ++j)
                             if (S.nb_ok[j]) {
                                 bounded_push(rh_d, rh_i, S.rh_len, k, S.nb_d[j], S.nb_i[j]);
-                                seed_push(S.nb_d[j], S.nb_i[j]);
+                                if (!appended) seed_push(S.nb_d[j], S.nb_i[j]);
                             }
+                        if (appended) S.sh_len = len0 + __popcll(okm);
                     }
                     wave_sync();
                 }
@@ -431,12 +488,21 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                     vis[x] = (uint8_t)ep;
                     const float d = cand_dist(x);
                     bounded_push(rh_d, rh_i, S.rh_len, k, d, x);
-                    seed_push(d, x);
+                    if (bulk) {  // fills only up to k < kSeedLds entries: the array has room
+                        S.sh_d[S.sh_len] = d;
+                        S.sh_i[S.sh_len] = x;
+                        ++S.sh_len;
+                    } else {
+                        seed_push(d, x);
+                    }
                 }
             }
             const float maxd = S.rh_len > 0 ? rh_d[0] : __builtin_inff();
             S.bound = maxd + eps * (maxd - a.min_distance);
         }
+        __threadfence_block();
+        wave_sync();
+        if (bulk) wave_sort_pairs(S.sh_d, S.sh_i, (int)S.sh_len, lane);  // uniform
         __threadfence_block();
         wave_sync();
         // ---- traverse (greedy.rs:165-201) ----
