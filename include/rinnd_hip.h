@@ -279,6 +279,14 @@ rinnd_status rinnd_index_query_quantized(rinnd_index* index, const float* querie
                                          float epsilon, int64_t candidate_width, int64_t rerank_width,
                                          int32_t* out_idx, float* out_dist);
 
+/* tree_quantization = "int8" (crates/rinnd/src/lib.rs:242-248, 681-705): replaces the routing of every later query
+ * on this index by int8 copies of its search trees (QuantizedFlatTree::from_flat_tree, core/tree/flat_tree.rs:45-92:
+ * per node scale max|h| / 127, codes round(h / scale) clamped to [-127, 127]); the descent takes the query's
+ * per-vector symmetric int8 codes (round(x * 127 / max|x|), scale max|x| / 127, lib.rs:303-347) and the margin
+ * i8 dot * node scale * query scale + offset (flat_tree.rs:95-164).  The FP32 hyperplanes are released, as the
+ * reference takes its FP32 search trees (lib.rs:228-249): rinnd_index_trees returns NULL afterwards.  Idempotent. */
+rinnd_status rinnd_index_quantize_trees(rinnd_index* index);
+
 /* The query of rinnd_index_query without any device allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  queries (nq x dim), out_idx and out_dist (nq x k) must be device pointers; results are
@@ -353,7 +361,8 @@ rinnd_status rinnd_index_search_graph_info(const rinnd_index* index, int64_t* n,
 rinnd_status rinnd_index_search_graph(const rinnd_index* index, int32_t* indptr, int32_t* indices);
 /* `vertex_order` (core/index.rs:47, 679-685): internal id -> input id. */
 rinnd_status rinnd_index_vertex_order(const rinnd_index* index, int64_t* out);
-/* Retained search trees (core/index.rs:679-680, 712-715) in the internal (reordered) id space. */
+/* Retained search trees (core/index.rs:679-680, 712-715) in the internal (reordered) id space; NULL once
+ * rinnd_index_quantize_trees has replaced them by int8 trees (lib.rs:1117-1120: no retained search tree). */
 const rinnd_forest* rinnd_index_trees(const rinnd_index* index);
 
 #ifdef __cplusplus

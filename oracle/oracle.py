@@ -84,6 +84,10 @@ def lib():
             "orc_index_trace": (None, [_p, _p, _p, _p]),
             "orc_diversify": (_i64, [_p, _p, _p, _i64, C.c_int, _p, _i64, C.c_int, _i64, _p, _p]),
             "orc_greedy": (_i64, [_p, _p, _i64, _i64, C.c_int, _p, _p, _p, _i64, _i64, _f, _f, _p, _p, _p]),
+            "orc_index_quantize_trees": (None, [_p]),
+            "orc_quantized_margin": (_f, [_p, _f, _i64, _p, _f, _p, _p]),
+            "orc_quantize_query_i8": (_f, [_p, _i64, _p]),
+            "orc_tree_search_leaves_q": (_i64, [_p, _i64, _p, _f, _p, _i64, _p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -230,6 +234,14 @@ class Forest:
         ev = np.zeros(1, np.int64)
         m = int(lib().orc_tree_search_leaves(self.h, t, ptr(point), ptr(st), budget, ptr(out), budget, ptr(ev)))
         return [tuple(map(int, r)) for r in out[:m]], int(ev[0])
+
+    def search_leaves_quantized(self, t, codes, query_scale, st, budget):
+        """QuantizedFlatTree::search_leaves of tree t's int8 copy (flat_tree.rs:103-164)."""
+        codes = np.ascontiguousarray(codes, np.int8)
+        out = np.zeros((max(budget, 1), 2), np.int64)
+        m = int(lib().orc_tree_search_leaves_q(self.h, t, ptr(codes), C.c_float(query_scale), ptr(st), budget,
+                                               ptr(out)))
+        return [tuple(map(int, r)) for r in out[:m]]
 
 
 def build_forest(data, n_trees, leaf_size, st, angular=False, max_depth=200, threads=1):
@@ -393,6 +405,10 @@ class Index:
             lib().orc_index_free(self.h)
             self.h = None
 
+    def quantize_trees(self):
+        """tree_quantization = "int8": the search descends int8 copies of the search trees (flat_tree.rs:35-183)."""
+        lib().orc_index_quantize_trees(self.h)
+
     def query(self, q, k=10, epsilon=0.1, threads=None):
         q = np.ascontiguousarray(q, np.float32)
         nq = q.shape[0]
@@ -437,3 +453,24 @@ class Index:
         ms = np.zeros(4, np.float64)
         lib().orc_index_trace(self.h, ptr(ch), ptr(it), ptr(ms))
         return ch[: int(it[0])].copy(), ms
+
+
+# ---- int8 routing trees (flat_tree.rs:35-183) -------------------------------------------------------
+def quantized_margin(hyperplane, offset, query_codes, query_scale):
+    """One node's int8 plane and scale (QuantizedFlatTree::from_flat_tree) and its reconstructed margin for an int8
+    query: (codes, scale, margin)."""
+    h = np.ascontiguousarray(hyperplane, np.float32)
+    qc = np.ascontiguousarray(query_codes, np.int8)
+    codes = np.zeros(len(h), np.int8)
+    sc = np.zeros(1, np.float32)
+    m = lib().orc_quantized_margin(ptr(h), C.c_float(offset), len(h), ptr(qc), C.c_float(query_scale), ptr(codes),
+                                   ptr(sc))
+    return codes, float(sc[0]), float(m)
+
+
+def quantize_query_i8(q):
+    """The per-vector symmetric int8 query (crates/rinnd/src/lib.rs:303-347): (codes, scale)."""
+    x = np.ascontiguousarray(q, np.float32)
+    codes = np.zeros(len(x), np.int8)
+    s = lib().orc_quantize_query_i8(ptr(x), len(x), ptr(codes))
+    return codes, float(s)

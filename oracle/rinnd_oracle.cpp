@@ -524,6 +524,100 @@ size_t FlatTree::search_leaves(const float* point, FastRng& rng, size_t budget,
     return evals;
 }
 
+// ---- int8 routing trees (flat_tree.rs:35-183, quantized.rs:26-40, crates/rinnd/src/lib.rs:303-347) ------------
+int32_t quantized_i8_dot(const int8_t* x, const int8_t* y, size_t n) {
+    int32_t s = 0;  // exact in i32 whatever the summation order
+    for (size_t i = 0; i < n; ++i) s += (int32_t)x[i] * (int32_t)y[i];
+    return s;
+}
+
+// quantize_hyperplanes (flat_tree.rs:46-63): scale = max|h| / 127 (0 for a zero plane), codes round(h / scale)
+// (half away from zero) clamped to [-127, 127]
+QuantizedFlatTree QuantizedFlatTree::from_flat_tree(const FlatTree& t) {
+    QuantizedFlatTree q;
+    q.dim = t.dim;
+    q.n_nodes = t.n_nodes;
+    q.offsets = t.offsets;
+    q.children = t.children;
+    q.indices = t.indices;
+    q.hyperplanes.resize(t.n_nodes * t.dim);
+    q.scales.resize(t.n_nodes);
+    for (size_t v = 0; v < t.n_nodes; ++v) {
+        const float* h = t.hyperplanes.data() + v * t.dim;
+        float max_abs = 0.0f;
+        for (size_t i = 0; i < t.dim; ++i) max_abs = std::max(max_abs, std::fabs(h[i]));
+        const float scale = max_abs > 0.0f ? max_abs / 127.0f : 0.0f;
+        q.scales[v] = scale;
+        for (size_t i = 0; i < t.dim; ++i) {
+            float c = 0.0f;
+            if (scale > 0.0f) c = std::min(127.0f, std::max(-127.0f, std::round(h[i] / scale)));
+            q.hyperplanes[v * t.dim + i] = (int8_t)c;
+        }
+    }
+    return q;
+}
+
+// margin (flat_tree.rs:95-101): i8 dot as f32, times the node's scale, times the query's scale, plus the offset
+float QuantizedFlatTree::margin(size_t node, const int8_t* query, float query_scale) const {
+    const float dot = (float)quantized_i8_dot(hyperplanes.data() + node * dim, query, dim);
+    return dot * scales[node] * query_scale + offsets[node];
+}
+
+// search_leaves (flat_tree.rs:103-164): FlatTree::search_leaves with the reconstructed margin
+size_t QuantizedFlatTree::search_leaves(const int8_t* query, float query_scale, FastRng& rng, size_t budget,
+                                        std::vector<std::pair<size_t, size_t>>& leaves) const {
+    leaves.clear();
+    if (children.empty() || budget == 0) return 0;
+    size_t evals = 0;
+    if (budget == 1) {
+        size_t node = 0;
+        while (children[2 * node] > 0) {
+            const float m = margin(node, query, query_scale);
+            ++evals;
+            const size_t side = std::fabs(m) < 1e-8f ? (size_t)rng.next_bool() : (size_t)(m > 0.0f);
+            node = (size_t)children[2 * node + side];
+        }
+        leaves.emplace_back((size_t)(-children[2 * node]), (size_t)(-children[2 * node + 1]));
+        return evals;
+    }
+    std::vector<std::pair<float, size_t>> frontier{{0.0f, 0}};
+    auto total_lt = [](float a, float b) {  // f32::total_cmp(a, b) == Less
+        int32_t ia, ib;
+        std::memcpy(&ia, &a, 4);
+        std::memcpy(&ib, &b, 4);
+        ia ^= (int32_t)(((uint32_t)(ia >> 31)) >> 1);
+        ib ^= (int32_t)(((uint32_t)(ib >> 31)) >> 1);
+        return ia < ib;
+    };
+    while (leaves.size() < budget && !frontier.empty()) {
+        size_t best = 0;
+        for (size_t i = 1; i < frontier.size(); ++i)
+            if (total_lt(frontier[i].first, frontier[best].first)) best = i;
+        auto [prio, node] = frontier[best];
+        frontier.erase(frontier.begin() + best);
+        if (children[2 * node] <= 0) {
+            leaves.emplace_back((size_t)(-children[2 * node]), (size_t)(-children[2 * node + 1]));
+            continue;
+        }
+        const float m = margin(node, query, query_scale);
+        ++evals;
+        const size_t side = std::fabs(m) < 1e-8f ? (size_t)rng.next_bool() : (size_t)(m > 0.0f);
+        frontier.emplace_back(prio, (size_t)children[2 * node + side]);
+        frontier.emplace_back(std::max(prio, std::fabs(m)), (size_t)children[2 * node + 1 - side]);
+    }
+    return evals;
+}
+
+// quantize_query / tree_query_scale of the per-vector encodings (crates/rinnd/src/lib.rs:303-347)
+float quantize_query_i8(const float* q, size_t dim, int8_t* codes) {
+    float max_abs = 0.0f;
+    for (size_t i = 0; i < dim; ++i) max_abs = std::max(max_abs, std::fabs(q[i]));
+    const float multiplier = max_abs > 0.0f ? 127.0f / max_abs : 0.0f;
+    for (size_t i = 0; i < dim; ++i)
+        codes[i] = (int8_t)std::min(127.0f, std::max(-127.0f, std::round(q[i] * multiplier)));
+    return max_abs > 0.0f ? max_abs / 127.0f : 0.0f;
+}
+
 // ---- NN-Descent -------------------------------------------------------------------------------
 
 // initialize_from_leaves (core/nndescent/mod.rs:203-300).
@@ -1134,10 +1228,20 @@ void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* 
     // initialize_fp32 (greedy.rs:70-99): every tree's leaves, in order; the descent draws from rng first
     std::vector<int32_t> seed_ids;
     std::vector<std::pair<size_t, size_t>> leaves;
-    for (const auto& t : search_trees) {
-        t.search_leaves(q, rng, search_tree_leaf_budget, leaves);
-        for (auto [s, e] : leaves)
-            for (size_t z = s; z < e; ++z) seed_ids.push_back(t.indices[z]);
+    if (!quantized_trees.empty()) {  // initialize_quantized (greedy.rs:101-146) with int8 routing trees
+        std::vector<int8_t> codes(dim);
+        const float qs = quantize_query_i8(q, dim, codes.data());
+        for (const auto& t : quantized_trees) {
+            t.search_leaves(codes.data(), qs, rng, search_tree_leaf_budget, leaves);
+            for (auto [s, e] : leaves)
+                for (size_t z = s; z < e; ++z) seed_ids.push_back(t.indices[z]);
+        }
+    } else {
+        for (const auto& t : search_trees) {
+            t.search_leaves(q, rng, search_tree_leaf_budget, leaves);
+            for (auto [s, e] : leaves)
+                for (size_t z = s; z < e; ++z) seed_ids.push_back(t.indices[z]);
+        }
     }
     std::vector<int32_t> ri;
     std::vector<float> rd;
@@ -1480,6 +1584,42 @@ void* orc_index_build(const float* data, int64_t n, int64_t dim, int metric, int
     return new Index(build_index(data, n, dim, metric, corr, ip, threads));
 }
 void orc_index_free(void* h) { delete (Index*)h; }
+// tree_quantization = "int8": the search descends int8 copies of the search trees from now on
+void orc_index_quantize_trees(void* h) {
+    Index& ix = *(Index*)h;
+    ix.quantized_trees.clear();
+    for (const auto& t : ix.search_trees) ix.quantized_trees.push_back(QuantizedFlatTree::from_flat_tree(t));
+}
+// one node's int8 plane, its scale and the reconstructed margin of an int8 query (tests of the formulas)
+float orc_quantized_margin(const float* hyperplane, float offset, int64_t dim, const int8_t* query, float query_scale,
+                           int8_t* codes_out, float* scale_out) {
+    FlatTree t;
+    t.dim = (size_t)dim;
+    t.n_nodes = 1;
+    t.hyperplanes.assign(hyperplane, hyperplane + dim);
+    t.offsets = {offset};
+    t.children = {0, 0};
+    const QuantizedFlatTree q = QuantizedFlatTree::from_flat_tree(t);
+    std::memcpy(codes_out, q.hyperplanes.data(), (size_t)dim);
+    *scale_out = q.scales[0];
+    return q.margin(0, query, query_scale);
+}
+float orc_quantize_query_i8(const float* q, int64_t dim, int8_t* codes) { return quantize_query_i8(q, (size_t)dim, codes); }
+// leaves of one quantized tree (t-th of a forest) for an int8 query
+int64_t orc_tree_search_leaves_q(void* h, int64_t t, const int8_t* query, float query_scale, uint64_t st[4],
+                                 int64_t budget, int64_t* out) {
+    FastRng r;
+    std::memcpy(r.s, st, 32);
+    const QuantizedFlatTree q = QuantizedFlatTree::from_flat_tree(((OrcForest*)h)->trees[t]);
+    std::vector<std::pair<size_t, size_t>> lv;
+    q.search_leaves(query, query_scale, r, (size_t)budget, lv);
+    std::memcpy(st, r.s, 32);
+    for (size_t i = 0; i < lv.size(); ++i) {
+        out[2 * i] = (int64_t)lv[i].first;
+        out[2 * i + 1] = (int64_t)lv[i].second;
+    }
+    return (int64_t)lv.size();
+}
 void orc_index_query(void* h, const float* q, int64_t nq, int64_t k, float eps, int threads,
                      int32_t* out_idx, float* out_dist) {
     ((Index*)h)->query(q, nq, k, eps, out_idx, out_dist, threads);

@@ -69,6 +69,24 @@ struct FlatTree {
                          std::vector<std::pair<size_t, size_t>>& leaves) const;
 };
 
+// QuantizedFlatTree (flat_tree.rs:35-183): int8 hyperplanes with one symmetric scale per node; offsets, topology and
+// leaves as in the FP32 tree.
+struct QuantizedFlatTree {
+    std::vector<int8_t> hyperplanes;  // n_nodes x dim
+    std::vector<float> scales, offsets;
+    std::vector<int32_t> children, indices;
+    size_t dim = 0, n_nodes = 0;
+    static QuantizedFlatTree from_flat_tree(const FlatTree& t);
+    float margin(size_t node, const int8_t* query, float query_scale) const;
+    size_t search_leaves(const int8_t* query, float query_scale, FastRng& rng, size_t leaf_budget,
+                         std::vector<std::pair<size_t, size_t>>& leaves) const;
+};
+// quantized_i8_dot (core/distance/quantized.rs:26-40)
+int32_t quantized_i8_dot(const int8_t* x, const int8_t* y, size_t n);
+// the per-vector symmetric int8 query of a quantized index (crates/rinnd/src/lib.rs:303-347): codes
+// round(x * 127 / max|x|) clamped to [-127, 127], returned scale max|x| / 127 (0 for a zero query)
+float quantize_query_i8(const float* q, size_t dim, int8_t* codes);
+
 FlatTree build_rp_tree(const float* data, size_t n, size_t dim, size_t leaf_size, FastRng& rng,
                        bool angular, size_t max_depth);
 std::vector<FlatTree> build_rp_forest(const float* data, size_t n, size_t dim, size_t n_trees,
@@ -131,6 +149,7 @@ struct Index {
     std::vector<float> neighbor_distances;
     SearchGraph search_graph;
     std::vector<FlatTree> search_trees;
+    std::vector<QuantizedFlatTree> quantized_trees;  // non-empty: the search descends them (tree_quantization int8)
     size_t search_tree_leaf_budget = 1;
     std::vector<size_t> vertex_order;
     float min_distance = 0;

@@ -125,9 +125,6 @@ class RINND:
                              "tree_quantization='none'")
         if direct and not normalize_cosine:
             raise ValueError("direct and quantized cosine modes require metric='cosine' and normalize=True")
-        if quantized is not None and tree_quantization == "int8":
-            raise NotImplementedError("tree_quantization='int8' is not available on the MI355X path: the search "
-                                      "trees stay FP32")
         # metric -> (distance functor, correction) dispatch (py:1239-1275)
         if m == "euclidean":
             dist, corr = "sqeuclidean", "sqrt"
@@ -173,6 +170,12 @@ class RINND:
                                        C.byref(p), C.byref(h), C.byref(self.build_stats))
         self._ctx.check(st, "rinnd_index_build")
         self._h = h
+        # search-tree node counts: the storage accounting of the trees (lib.rs:498-541, 228-240)
+        forest = L.Forest(L.lib().rinnd_index_trees(h), owned=False)
+        self._tree_nodes = [forest.n_nodes(t) for t in range(forest.n_trees)]
+        self._tree_i8 = tree_quantization == "int8"
+        if self._tree_i8:  # lib.rs:242-248: int8 routing trees replace the FP32 search trees
+            self._ctx.check(L.lib().rinnd_index_quantize_trees(h), "rinnd_index_quantize_trees")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -292,14 +295,27 @@ class RINND:
         if self._quantized is not None:
             code_bytes = self.n_points * ((self.dim + 31) // 32 * 32)
             meta = self.n_points * 4
+            # search trees: children 8 bytes and offsets 4 bytes a node, hyperplanes dim values a node, leaf
+            # indices 4 bytes a point (lib.rs:498-541, 228-240)
+            nodes = sum(self._tree_nodes)
+            topology = nodes * 8 + len(self._tree_nodes) * self.n_points * 4
+            if self._tree_i8:
+                trees = {"tree_code_bytes": nodes * self.dim, "tree_scale_bytes": nodes * 4,
+                         "tree_offset_bytes": nodes * 4, "tree_topology_bytes": topology,
+                         "hypothetical_releasable_fp32_tree_bytes": 0,
+                         "released_fp32_tree_bytes": nodes * self.dim * 4 + nodes * 4 + topology,
+                         "retained_fp32_tree_topology_bytes": 0}
+            else:
+                trees = {"tree_code_bytes": 0, "tree_scale_bytes": 0, "tree_offset_bytes": 0,
+                         "tree_topology_bytes": 0, "hypothetical_releasable_fp32_tree_bytes": 0,
+                         "released_fp32_tree_bytes": 0, "retained_fp32_tree_topology_bytes": topology}
+            allocated = (code_bytes + meta + data_bytes + trees["retained_fp32_tree_topology_bytes"]
+                         + trees["tree_code_bytes"] + trees["tree_scale_bytes"] + trees["tree_offset_bytes"]
+                         + trees["tree_topology_bytes"])
             return {
                 "encoding": "fp8_e4m3", "code_bytes": code_bytes, "metadata_bytes": meta,
-                "retained_fp32_bytes": data_bytes, "encode_seconds": 0.0, "tree_code_bytes": 0,
-                "tree_scale_bytes": 0, "tree_offset_bytes": 0, "tree_topology_bytes": 0,
-                "hypothetical_releasable_fp32_tree_bytes": 0, "released_fp32_tree_bytes": 0,
-                "released_fp32_data_bytes": 0, "retained_fp32_tree_topology_bytes": 0,
-                "allocated_bytes": code_bytes + meta + data_bytes,
-                "bytes_per_vector": float(code_bytes + meta) / max(self.n_points, 1),
+                "retained_fp32_bytes": data_bytes, "encode_seconds": 0.0, **trees, "released_fp32_data_bytes": 0,
+                "allocated_bytes": allocated, "bytes_per_vector": float(allocated) / max(self.n_points, 1),
             }
         return {
             "encoding": "fp32", "code_bytes": data_bytes, "metadata_bytes": 0, "retained_fp32_bytes": 0,

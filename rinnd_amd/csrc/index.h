@@ -19,6 +19,12 @@ struct rinnd_index {
     std::vector<DBuf<float>> hyper_padded;
     DBuf<const float*> d_hyper, d_off;
     DBuf<const int32_t*> d_ch, d_tix;
+    // int8 routing trees (rinnd_index_quantize_trees): per tree codes (n_nodes x ld) and scales, and their tables
+    std::vector<DBuf<int8_t>> tq_codes;
+    std::vector<DBuf<float>> tq_scales;
+    DBuf<const int8_t*> d_tq;
+    DBuf<const float*> d_tqs;
+    bool tree_i8 = false;
     DBuf<int64_t> vertex_order, inv;
     DBuf<int32_t> nb_idx;
     DBuf<float> nb_dist;

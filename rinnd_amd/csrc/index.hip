@@ -190,9 +190,11 @@ SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq,
     a.indptr = ix.indptr.p;
     a.indices = ix.indices.p;
     a.n_trees = (int)ix.trees.trees.size();
-    a.hyper = ix.d_hyper.p;
+    a.hyper = ix.tree_i8 ? nullptr : ix.d_hyper.p;
     a.hdim = ix.ld;
     a.offsets = ix.d_off.p;
+    a.hq = ix.tree_i8 ? ix.d_tq.p : nullptr;
+    a.hqs = ix.tree_i8 ? ix.d_tqs.p : nullptr;
     a.children = ix.d_ch.p;
     a.tindices = ix.d_tix.p;
     a.leaf_budget = (int)ix.leaf_budget;
@@ -490,6 +492,43 @@ rinnd_status rinnd_index_query_quantized(rinnd_index* ix, const float* queries, 
     });
 }
 
+rinnd_status rinnd_index_quantize_trees(rinnd_index* ix) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        if (ix->tree_i8) return;
+        hipStream_t s = ix->ctx->stream;
+        const size_t T = ix->trees.trees.size();
+        ix->tq_codes.resize(T);
+        ix->tq_scales.resize(T);
+        std::vector<const int8_t*> tq(T);
+        std::vector<const float*> tqs(T);
+        for (size_t t = 0; t < T; ++t) {
+            const int64_t nn = std::max<int64_t>(ix->trees.trees[t].n_nodes, 1);
+            ix->tq_codes[t].alloc((size_t)nn * ix->ld);
+            ix->tq_scales[t].alloc((size_t)nn);
+            launch_quantize_tree(s, ix->hyper_padded[t].p, ix->trees.trees[t].n_nodes, ix->ld, (int)ix->dim,
+                                 ix->tq_codes[t].p, ix->tq_scales[t].p);
+            tq[t] = ix->tq_codes[t].p;
+            tqs[t] = ix->tq_scales[t].p;
+        }
+        ix->d_tq.alloc(std::max<size_t>(T, 1));
+        ix->d_tqs.alloc(std::max<size_t>(T, 1));
+        if (T) {
+            HIP_TRY(hipMemcpyAsync(ix->d_tq.p, tq.data(), T * sizeof(void*), hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(ix->d_tqs.p, tqs.data(), T * sizeof(void*), hipMemcpyHostToDevice, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));  // the host tables above go out of scope
+        // the FP32 hyperplanes are released as the reference takes its FP32 search trees (lib.rs:228-249);
+        // offsets, children and leaf indices stay: they are the int8 trees' own
+        for (size_t t = 0; t < T; ++t) {
+            ix->hyper_padded[t].release();
+            ix->trees.trees[t].hyper.release();
+        }
+        ix->d_hyper.release();
+        ix->tree_i8 = true;
+    });
+}
+
 rinnd_status rinnd_index_neighbor_graph(const rinnd_index* ix, int32_t* out_idx, float* out_dist) {
     if (!ix) return RINND_ERR_INVALID_ARG;
     return guard(ix->ctx, [&] {
@@ -543,6 +582,6 @@ rinnd_status rinnd_index_vertex_order(const rinnd_index* ix, int64_t* out) {
     });
 }
 
-const rinnd_forest* rinnd_index_trees(const rinnd_index* ix) { return ix ? &ix->trees : nullptr; }
+const rinnd_forest* rinnd_index_trees(const rinnd_index* ix) { return ix && !ix->tree_i8 ? &ix->trees : nullptr; }
 
 }  // extern "C"

@@ -17,6 +17,10 @@ struct SearchArgs {
     const float* const* hyper;  // per tree, n_nodes x hdim (padded)
     int64_t hdim;
     const float* const* offsets;
+    // int8 routing trees (tree_quantization = int8; null: the FP32 hyperplanes above): per tree n_nodes x hdim
+    // codes and one scale per node; the descent uses the query's per-vector int8 codes
+    const int8_t* const* hq;
+    const float* const* hqs;
     const int32_t* const* children;
     const int32_t* const* tindices;
     int leaf_budget;
@@ -59,6 +63,7 @@ struct SearchArgs {
     int qcap;  // floats of the query row kept in LDS (0: read from memory); set by launch_search
     int kcap;  // result-heap entries in LDS; set by launch_search
     int c8cap; // bytes of the query's FP8 codes in LDS; set by launch_search
+    int i8cap; // bytes of the query's int8 routing codes in LDS; set by launch_search
 };
 
 void launch_search(hipStream_t s, SearchArgs a, int slots);
@@ -66,6 +71,10 @@ void launch_search(hipStream_t s, SearchArgs a, int slots);
 // of 32: whole MFMA k-steps), and the squared norm of each dequantized row
 void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, uint8_t* codes, int64_t ld8,
                        float* scales, float* norms);
+// int8 copy of a tree's hyperplanes (QuantizedFlatTree::from_flat_tree): per node scale max|h| / 127 and codes
+// round(h / scale) clamped to [-127, 127]
+void launch_quantize_tree(hipStream_t s, const float* hyper, int64_t n_nodes, int64_t hdim, int dim, int8_t* codes,
+                          float* scales);
 int search_max_k();
 int search_max_degree();
 int search_max_budget();

@@ -268,6 +268,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     float* const rh_d = sdyn + a.qcap;      // result heap, a.kcap entries
     int32_t* const rh_i = (int32_t*)(rh_d + a.kcap);
     uint8_t* const qc8 = (uint8_t*)(rh_i + a.kcap);  // FP8 search: the query's codes, a.c8cap bytes
+    int8_t* const qi8 = (int8_t*)(qc8 + a.c8cap);     // int8 routing: the query's codes, a.i8cap bytes
     const int lane = threadIdx.x;
     const int slot = blockIdx.x;
     uint8_t* vis = a.visited + (int64_t)slot * a.n;
@@ -304,8 +305,30 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             }
             for (int o = 32; o > 0; o >>= 1) qq += __shfl_xor(qq, o);
         }
+        // int8 routing trees: the query's per-vector symmetric codes (crates/rinnd/src/lib.rs:320-347) and its scale
+        // (tree_query_scale, lib.rs:303-313)
+        float tqs = 0.f;
+        if (a.hq) {
+            float mx = 0.f;
+            for (int i = lane; i < dim; i += 64) mx = fmaxf(mx, fabsf(q[i]));
+            for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+            const float mul = mx > 0.f ? 127.0f / mx : 0.f;
+            for (int i = lane; i < dim; i += 64) qi8[i] = (int8_t)fminf(127.f, fmaxf(-127.f, roundf(q[i] * mul)));
+            tqs = mx > 0.f ? mx / 127.0f : 0.f;
+        }
         __threadfence_block();
         wave_sync();
+        // a search tree node's margin: FP32 hyperplane (flat_tree.rs:253-273), or the int8 one reconstructed
+        // (flat_tree.rs:95-101: i8 dot as f32, times the node's scale, times the query's scale, plus the offset)
+        auto tree_margin = [&](int t, int64_t node) -> float {  // lane 0 only
+            if (a.hq) {
+                const int8_t* hc = a.hq[t] + node * a.hdim;
+                int32_t dot = 0;
+                for (int i = 0; i < dim; ++i) dot += (int32_t)hc[i] * (int32_t)qi8[i];
+                return (float)dot * a.hqs[t][node] * tqs + a.offsets[t][node];
+            }
+            return d_dot8(q, a.hyper[t] + node * a.hdim, dim) + a.offsets[t][node];
+        };
         auto fp8_dist = [&](int32_t x, float code_dot) -> float {
             const float dot = code_dot * a.q8s[x] * qsc;
             const float xx = a.q8n[x];
@@ -370,14 +393,12 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         // ---- initialize_fp32 (greedy.rs:70-99) ----
         for (int t = 0; t < a.n_trees; ++t) {
             const int32_t* ch = a.children[t];
-            const float* hp = a.hyper[t];
-            const float* off = a.offsets[t];
             if (lane == 0) {
                 int nl = 0;
                 if (a.leaf_budget == 1) {  // hard descent (flat_tree.rs:253-273)
                     int64_t node = 0;
                     while (ch[2 * node] > 0) {
-                        const float margin = d_dot8(q, hp + node * a.hdim, dim) + off[node];
+                        const float margin = tree_margin(t, node);
                         const int side = fabsf(margin) < 1e-8f ? (int)rng.next_bool() : (int)(margin > 0.0f);
                         node = ch[2 * node + side];
                     }
@@ -410,7 +431,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                             ++nl;
                             continue;
                         }
-                        const float margin = d_dot8(q, hp + (int64_t)node * a.hdim, dim) + off[node];
+                        const float margin = tree_margin(t, node);
                         const int side = fabsf(margin) < 1e-8f ? (int)rng.next_bool() : (int)(margin > 0.0f);
                         if (nf + 2 > kFrontier) {
                             *a.status = 2;
@@ -648,9 +669,35 @@ void launch_search(hipStream_t s, SearchArgs a, int slots) {
     a.qcap = a.dim <= kQLds ? (int)round_up(a.dim, 4) : 0;
     a.kcap = (int)round_up(std::max(a.k, 1), 4);
     a.c8cap = a.q8 ? (int)round_up(a.ld8, (int64_t)16) : 0;
-    const size_t lds = ((size_t)a.qcap + 2 * (size_t)a.kcap) * 4 + (size_t)a.c8cap;
+    a.i8cap = a.hq ? (int)round_up((int64_t)a.dim, (int64_t)16) : 0;
+    const size_t lds = ((size_t)a.qcap + 2 * (size_t)a.kcap) * 4 + (size_t)a.c8cap + (size_t)a.i8cap;
     if (a.q8) launch_search_q<true>(s, a, slots, lds);
     else launch_search_q<false>(s, a, slots, lds);
+}
+
+// One wave per node: max |h| over the plane by a wave reduction, the node's scale, then its codes (the padding
+// columns stay zero).
+__global__ __launch_bounds__(256) void k_quantize_tree(const float* __restrict__ hyper, int64_t n_nodes, int64_t hdim,
+                                                       int dim, int8_t* __restrict__ codes, float* __restrict__ scales) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t v = (int64_t)blockIdx.x * 4 + w; v < n_nodes; v += (int64_t)gridDim.x * 4) {
+        const float* h = hyper + v * hdim;
+        float mx = 0.f;
+        for (int i = lane; i < dim; i += 64) mx = fmaxf(mx, fabsf(h[i]));
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        const float sc = mx > 0.f ? mx / 127.0f : 0.f;
+        if (lane == 0) scales[v] = sc;
+        for (int64_t i = lane; i < hdim; i += 64)
+            codes[v * hdim + i] =
+                (int8_t)(i < dim && sc > 0.f ? fminf(127.f, fmaxf(-127.f, roundf(h[i] / sc))) : 0.f);
+    }
+}
+void launch_quantize_tree(hipStream_t s, const float* hyper, int64_t n_nodes, int64_t hdim, int dim, int8_t* codes,
+                          float* scales) {
+    if (n_nodes <= 0) return;
+    hipLaunchKernelGGL(k_quantize_tree, dim3(grid_for(n_nodes, 4, 4096)), dim3(256), 0, s, hyper, n_nodes, hdim, dim,
+                       codes, scales);
+    HIP_TRY(hipGetLastError());
 }
 
 // One wave per row: max |x| by a wave reduction, then the codes, two values per conversion.
