@@ -287,6 +287,24 @@ rinnd_status rinnd_index_query_quantized(rinnd_index* index, const float* querie
  * reference takes its FP32 search trees (lib.rs:228-249): rinnd_index_trees returns NULL afterwards.  Idempotent. */
 rinnd_status rinnd_index_quantize_trees(rinnd_index* index);
 
+/* cosine_distance_mode "int8" / "sq8u" / "sq8p" (QuantizedI8Index::new, crates/rinnd/src/lib.rs:158-224): int8
+ * codes of the index's rows for the quantized search.  encoding 0 (int8): codes round(x * 127 / max|row|) and the
+ * inverse norm of each row's codes; 1 (sq8u): one global multiplier 127 / max|data| and its dequantisation
+ * scale; 2 (sq8p): per-row multipliers and each row's 1 / multiplier.  Codes are clamped to [-127, 127].  The
+ * index must be a DirectNormalizedCosine one (dist RINND_DIST_DIRECT_COSINE over normalized rows). */
+rinnd_status rinnd_index_encode_i8(rinnd_index* index, int encoding);
+/* query_one_quantized_widths over the int8 codes (lib.rs:366-398 -> query_one_quantized_i8_with_workspace,
+ * core/index.rs:283-330; greedy.rs:365-408): the query's codes under the index's encoding, candidate distances
+ * 1 - clamp(i8 dot * query inverse norm * row inverse norm, -1, 1), widths as rinnd_index_query_quantized
+ * (index.rs:55-90) and the exact FP32 re-ranking of the first rerank_width candidates (index.rs:103-124).
+ * Query ids 0..nq-1; results in input ids, padded with -1 / +inf. */
+rinnd_status rinnd_index_query_i8(rinnd_index* index, const float* queries, int64_t nq, int64_t k, float epsilon,
+                                  int64_t candidate_width, int64_t rerank_width, int32_t* out_idx, float* out_dist);
+/* retain_fp32_data = False (lib.rs:250-257): frees the FP32 rows once the int8 codes exist.  Afterwards only
+ * rinnd_index_query_i8 without re-ranking runs; the others return RINND_ERR_INVALID_ARG ("exact reranking is
+ * unavailable because FP32 vector data was released", lib.rs:287-300). */
+rinnd_status rinnd_index_release_data(rinnd_index* index);
+
 /* The query of rinnd_index_query without any device allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  queries (nq x dim), out_idx and out_dist (nq x k) must be device pointers; results are

@@ -88,6 +88,13 @@ def lib():
             "orc_quantized_margin": (_f, [_p, _f, _i64, _p, _f, _p, _p]),
             "orc_quantize_query_i8": (_f, [_p, _i64, _p]),
             "orc_tree_search_leaves_q": (_i64, [_p, _i64, _p, _f, _p, _i64, _p]),
+            "orc_index_encode_i8": (_p, [_p, C.c_int]),
+            "orc_i8_free": (None, [_p]),
+            "orc_i8_codes": (None, [_p, _p, _p, _p]),
+            "orc_encode_i8_rows": (_p, [_p, _i64, _i64, C.c_int]),
+            "orc_quantize_query_codes": (_f, [_p, _p, _p]),
+            "orc_i8_cosine_distance": (_f, [_p, _f, _p, _f, _i64]),
+            "orc_index_query_i8": (None, [_p, _p, _p, _i64, _i64, _i64, _i64, _f, C.c_int, _p, _p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -417,6 +424,21 @@ class Index:
         lib().orc_index_query(self.h, ptr(q), nq, k, C.c_float(epsilon), threads or self.threads, ptr(oi), ptr(od))
         return oi, od
 
+    def encode_i8(self, encoding):
+        """The quantized index's codes of this index's rows (crates/rinnd/src/lib.rs:158-224); encoding "int8",
+        "sq8u" or "sq8p"."""
+        return I8Codes(lib().orc_index_encode_i8(self.h, I8_ENCODINGS[encoding]), self.n, self.dim)
+
+    def query_i8(self, codes, q, k=10, epsilon=0.1, candidate_width=0, rerank_width=0, threads=None):
+        """query_one_quantized_i8_with_workspace (core/index.rs:283-330) for query ids 0..nq-1."""
+        q = np.ascontiguousarray(q, np.float32)
+        nq = q.shape[0]
+        oi = np.zeros((nq, k), np.int32)
+        od = np.zeros((nq, k), np.float32)
+        lib().orc_index_query_i8(self.h, codes.h, ptr(q), nq, k, candidate_width, rerank_width, C.c_float(epsilon),
+                                 threads or self.threads, ptr(oi), ptr(od))
+        return oi, od
+
     def query_one(self, q, k=10, epsilon=0.1):
         q = np.ascontiguousarray(q, np.float32)
         oi = np.zeros(k, np.int32)
@@ -466,6 +488,49 @@ def quantized_margin(hyperplane, offset, query_codes, query_scale):
     m = lib().orc_quantized_margin(ptr(h), C.c_float(offset), len(h), ptr(qc), C.c_float(query_scale), ptr(codes),
                                    ptr(sc))
     return codes, float(sc[0]), float(m)
+
+
+I8_ENCODINGS = {"int8": 0, "sq8u": 1, "sq8p": 2}
+
+
+class I8Codes:
+    """Codes of the quantized index (QuantizedI8Index, crates/rinnd/src/lib.rs:133-260)."""
+
+    def __init__(self, h, n, dim):
+        self.h, self.n, self.dim = h, n, dim
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_i8_free(self.h)
+            self.h = None
+
+    def arrays(self):
+        codes = np.zeros((self.n, self.dim), np.int8)
+        inv = np.zeros(self.n, np.float32)
+        g = np.zeros(2, np.float32)
+        lib().orc_i8_codes(self.h, ptr(codes), ptr(inv), ptr(g))
+        return codes, inv, float(g[0]), float(g[1])
+
+    def quantize_query(self, q):
+        """quantize_query (lib.rs:325-358): the query's codes and its inverse norm / dequantisation scale."""
+        q = np.ascontiguousarray(q, np.float32)
+        codes = np.zeros(self.dim, np.int8)
+        s = lib().orc_quantize_query_codes(self.h, ptr(q), ptr(codes))
+        return codes, float(np.float32(s))
+
+
+def encode_i8_rows(X, encoding):
+    X = np.ascontiguousarray(X, np.float32)
+    return I8Codes(lib().orc_encode_i8_rows(ptr(X), X.shape[0], X.shape[1], I8_ENCODINGS[encoding]), X.shape[0],
+                   X.shape[1])
+
+
+def i8_cosine_distance(q_codes, q_inv, x_codes, x_inv):
+    """greedy.rs:381-387."""
+    a = np.ascontiguousarray(q_codes, np.int8)
+    b = np.ascontiguousarray(x_codes, np.int8)
+    return float(np.float32(lib().orc_i8_cosine_distance(ptr(a), C.c_float(q_inv), ptr(b), C.c_float(x_inv),
+                                                         len(a))))
 
 
 def quantize_query_i8(q):

@@ -1168,12 +1168,19 @@ void greedy_from_seeds(const float* q, const float* data, size_t dim, int metric
                        const int32_t* indptr, const int32_t* indices, const std::vector<int32_t>& seed_ids,
                        size_t k, float eps, float min_distance, FastRng& rng, std::vector<int32_t>& ri,
                        std::vector<float>& rd) {
+    greedy_core([&](int32_t i) { return distance(metric, q, data + (size_t)i * dim, dim); }, n_vertices, indptr,
+                indices, seed_ids, k, eps, min_distance, rng, ri, rd);
+}
+
+// The search of greedy.rs:70-241 with any candidate distance (greedy.rs:101-146, 365-408 take the int8 one).
+void greedy_core(const std::function<float(int32_t)>& dist_to, size_t n_vertices, const int32_t* indptr,
+                 const int32_t* indices, const std::vector<int32_t>& seed_ids, size_t k, float eps,
+                 float min_distance, FastRng& rng, std::vector<int32_t>& ri, std::vector<float>& rd) {
     std::vector<uint8_t> visited(n_vertices, 0);
     BoundedHeap result(k);
     using E = std::pair<float, int32_t>;
     auto gt = [](const E& a, const E& b) { return ord_cmp(a, b) > 0; };
     std::priority_queue<E, std::vector<E>, decltype(gt)> seeds(gt);  // min-heap on (d, idx)
-    auto dist_to = [&](int32_t i) { return distance(metric, q, data + (size_t)i * dim, dim); };
     auto check_and_mark = [&](int32_t i) {
         bool was = visited[i] != 0;
         visited[i] = 1;
@@ -1221,28 +1228,35 @@ void greedy_from_seeds(const float* q, const float* data, size_t dim, int metric
     result.into_sorted(ri, rd);
 }
 
-// process_query_with_workspace (core/index.rs:157-202) around search (greedy.rs:203-241).
-void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* out_idx,
-                      float* out_dist) const {
-    FastRng rng(rng_seed + (uint64_t)qid);
-    // initialize_fp32 (greedy.rs:70-99): every tree's leaves, in order; the descent draws from rng first
-    std::vector<int32_t> seed_ids;
+// The seed ids of a query: every tree's leaves, in order (initialize_fp32, greedy.rs:70-99), descending the int8
+// routing trees with the query's codes and scale when there are any (initialize_quantized, greedy.rs:101-146).
+// Distances draw nothing from rng, so the descents may all run before the first distance.
+void Index::seed_ids(const float* q, const int8_t* tree_codes, float tree_scale, FastRng& rng,
+                     std::vector<int32_t>& ids) const {
     std::vector<std::pair<size_t, size_t>> leaves;
-    if (!quantized_trees.empty()) {  // initialize_quantized (greedy.rs:101-146) with int8 routing trees
-        std::vector<int8_t> codes(dim);
-        const float qs = quantize_query_i8(q, dim, codes.data());
+    if (!quantized_trees.empty()) {
         for (const auto& t : quantized_trees) {
-            t.search_leaves(codes.data(), qs, rng, search_tree_leaf_budget, leaves);
+            t.search_leaves(tree_codes, tree_scale, rng, search_tree_leaf_budget, leaves);
             for (auto [s, e] : leaves)
-                for (size_t z = s; z < e; ++z) seed_ids.push_back(t.indices[z]);
+                for (size_t z = s; z < e; ++z) ids.push_back(t.indices[z]);
         }
     } else {
         for (const auto& t : search_trees) {
             t.search_leaves(q, rng, search_tree_leaf_budget, leaves);
             for (auto [s, e] : leaves)
-                for (size_t z = s; z < e; ++z) seed_ids.push_back(t.indices[z]);
+                for (size_t z = s; z < e; ++z) ids.push_back(t.indices[z]);
         }
     }
+}
+
+// process_query_with_workspace (core/index.rs:157-202) around search (greedy.rs:203-241).
+void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* out_idx,
+                      float* out_dist) const {
+    FastRng rng(rng_seed + (uint64_t)qid);
+    std::vector<int32_t> seed_ids;
+    std::vector<int8_t> codes(dim);
+    const float qs = quantized_trees.empty() ? 0.f : quantize_query_i8(q, dim, codes.data());
+    this->seed_ids(q, codes.data(), qs, rng, seed_ids);
     std::vector<int32_t> ri;
     std::vector<float> rd;
     greedy_from_seeds(q, data.data(), dim, metric, search_graph.n_vertices, search_graph.indptr.data(),
@@ -1251,6 +1265,129 @@ void Index::query_one(const float* q, size_t qid, size_t k, float eps, int32_t* 
         if (j < ri.size()) {
             out_idx[j] = ri[j] >= 0 ? (int32_t)vertex_order[ri[j]] : ri[j];
             out_dist[j] = correct(corr, rd[j]);
+        } else {
+            out_idx[j] = -1;
+            out_dist[j] = std::numeric_limits<float>::infinity();
+        }
+    }
+}
+
+// QuantizedI8Index::new (crates/rinnd/src/lib.rs:158-224): per-vector codes round(x * 127 / max|row|), or the
+// global multiplier 127 / max|data| for sq8u, clamped to [-127, 127]; per row the inverse norm of the codes
+// (int8), the global dequantisation scale 1 / multiplier (sq8u) or the row's 1 / multiplier (sq8p).
+I8Codes encode_i8(const float* data, size_t n, size_t dim, int encoding) {
+    I8Codes c;
+    c.encoding = encoding;
+    c.dim = dim;
+    c.codes.resize(n * dim);
+    c.inv_norms.resize(n);
+    float gmax = 0.f;
+    for (size_t i = 0; i < n * dim; ++i) gmax = std::max(gmax, std::fabs(data[i]));
+    c.global_multiplier = gmax > 0.f ? 127.0f / gmax : 0.f;
+    c.global_dequant_scale = c.global_multiplier > 0.f ? 1.0f / c.global_multiplier : 0.f;
+    for (size_t r = 0; r < n; ++r) {
+        const float* row = data + r * dim;
+        float mult = c.global_multiplier;
+        if (encoding != I8_GLOBAL_SYMMETRIC) {
+            float mx = 0.f;
+            for (size_t j = 0; j < dim; ++j) mx = std::max(mx, std::fabs(row[j]));
+            mult = mx > 0.f ? 127.0f / mx : 0.f;
+        }
+        int8_t* code = c.codes.data() + r * dim;
+        int32_t norm_sq = 0;
+        for (size_t j = 0; j < dim; ++j) {
+            code[j] = (int8_t)std::min(127.0f, std::max(-127.0f, std::round(row[j] * mult)));
+            norm_sq += (int32_t)code[j] * (int32_t)code[j];
+        }
+        if (encoding == I8_PER_VECTOR_COSINE)
+            c.inv_norms[r] = norm_sq > 0 ? 1.0f / std::sqrt((float)norm_sq) : 0.f;
+        else if (encoding == I8_PER_VECTOR_SYMMETRIC)
+            c.inv_norms[r] = mult > 0.f ? 1.0f / mult : 0.f;
+        else
+            c.inv_norms[r] = c.global_dequant_scale;
+    }
+    return c;
+}
+
+// QuantizedI8Index::quantize_query (lib.rs:325-358): the query's codes under the index's encoding and its inverse
+// norm (int8) or dequantisation scale (sq8u: the global one, sq8p: 1 / its multiplier).
+float quantize_query_codes(const I8Codes& c, const float* q, int8_t* codes) {
+    float mult = c.global_multiplier;
+    if (c.encoding != I8_GLOBAL_SYMMETRIC) {
+        float mx = 0.f;
+        for (size_t j = 0; j < c.dim; ++j) mx = std::max(mx, std::fabs(q[j]));
+        mult = mx > 0.f ? 127.0f / mx : 0.f;
+    }
+    int32_t norm_sq = 0;
+    for (size_t j = 0; j < c.dim; ++j) {
+        codes[j] = (int8_t)std::min(127.0f, std::max(-127.0f, std::round(q[j] * mult)));
+        norm_sq += (int32_t)codes[j] * (int32_t)codes[j];
+    }
+    if (c.encoding == I8_PER_VECTOR_COSINE) return norm_sq > 0 ? 1.0f / std::sqrt((float)norm_sq) : 0.f;
+    if (c.encoding == I8_GLOBAL_SYMMETRIC) return c.global_dequant_scale;
+    return mult > 0.f ? 1.0f / mult : 0.f;
+}
+
+// greedy.rs:381-387: 1 - clamp(i8 dot as f32 * query inverse norm * row inverse norm, -1, 1).
+float i8_cosine_distance(const int8_t* q, float q_inv_norm, const int8_t* x, float x_inv_norm, size_t dim) {
+    const float similarity = (float)quantized_i8_dot(q, x, dim) * q_inv_norm * x_inv_norm;
+    return 1.0f - std::min(1.0f, std::max(-1.0f, similarity));
+}
+
+// query_one_quantized_i8_with_workspace (core/index.rs:283-330): the search over the codes with candidate width
+// resolve_quantized_candidate_width (index.rs:55-77), then rerank_quantized_candidates (index.rs:103-124) of the
+// first rerank_width candidates in FP32 when should_rerank_quantized (index.rs:80-88) says so.  The trees
+// descend with the FP32 query, or with the query's codes (and its tree scale max|q| / 127, lib.rs:303-313)
+// when the index routes by int8 trees.
+void Index::query_one_quantized(const I8Codes& c, const float* q, size_t qid, size_t k, size_t candidate_width,
+                                size_t rerank_width, float eps, int32_t* out_idx, float* out_dist) const {
+    const size_t search_k = candidate_width == 0 ? std::max(k, rerank_width) : candidate_width;
+    const bool rerank = candidate_width == 0 ? rerank_width > k : rerank_width > 0;
+    std::vector<int8_t> qc(dim);
+    const float qinv = quantize_query_codes(c, q, qc.data());
+    float tree_scale = 0.f;
+    if (c.encoding == I8_GLOBAL_SYMMETRIC) {
+        tree_scale = c.global_dequant_scale;
+    } else {
+        float mx = 0.f;
+        for (size_t j = 0; j < dim; ++j) mx = std::max(mx, std::fabs(q[j]));
+        tree_scale = mx > 0.f ? mx / 127.0f : 0.f;
+    }
+    (void)qid;
+    FastRng rng(rng_seed);  // index.rs:298: every quantized query starts from the index's seed itself
+    std::vector<int32_t> seeds;
+    this->seed_ids(q, qc.data(), tree_scale, rng, seeds);
+    std::vector<int32_t> ri;
+    std::vector<float> rd;
+    greedy_core([&](int32_t i) {
+                    return i8_cosine_distance(qc.data(), qinv, c.codes.data() + (size_t)i * dim, c.inv_norms[i], dim);
+                },
+                search_graph.n_vertices, search_graph.indptr.data(), search_graph.indices.data(), seeds, search_k,
+                eps, min_distance, rng, ri, rd);
+    if (rerank) {
+        std::vector<std::pair<float, int32_t>> rr;
+        for (size_t j = 0; j < ri.size() && rr.size() < rerank_width; ++j)
+            if (ri[j] >= 0) rr.push_back({distance(metric, q, data.data() + (size_t)ri[j] * dim, dim), ri[j]});
+        auto key = [](float d) {  // f32::total_cmp
+            int32_t b;
+            std::memcpy(&b, &d, 4);
+            return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
+        };
+        std::sort(rr.begin(), rr.end(), [&](const auto& a, const auto& b) {
+            return key(a.first) != key(b.first) ? key(a.first) < key(b.first) : a.second < b.second;
+        });
+        if (rr.size() > k) rr.resize(k);
+        ri.clear();
+        rd.clear();
+        for (auto& [d, i] : rr) {
+            ri.push_back(i);
+            rd.push_back(d);
+        }
+    }
+    for (size_t j = 0; j < k; ++j) {
+        if (j < ri.size()) {
+            out_idx[j] = ri[j] >= 0 ? (int32_t)vertex_order[ri[j]] : ri[j];
+            out_dist[j] = rd[j];
         } else {
             out_idx[j] = -1;
             out_dist[j] = std::numeric_limits<float>::infinity();
@@ -1605,6 +1742,41 @@ float orc_quantized_margin(const float* hyperplane, float offset, int64_t dim, c
     return q.margin(0, query, query_scale);
 }
 float orc_quantize_query_i8(const float* q, int64_t dim, int8_t* codes) { return quantize_query_i8(q, (size_t)dim, codes); }
+// The quantized index's codes (QuantizedI8Index::new) of the index's (reordered) rows; freed by orc_i8_free.
+void* orc_index_encode_i8(void* h, int encoding) {
+    const Index& ix = *(const Index*)h;
+    return new I8Codes(encode_i8(ix.data.data(), ix.n_points, ix.dim, encoding));
+}
+void orc_i8_free(void* c) { delete (I8Codes*)c; }
+// codes (n x dim), inverse norms (n) and the global multiplier / dequantisation scale
+void orc_i8_codes(void* c, int8_t* codes, float* inv_norms, float* global2) {
+    const I8Codes& q = *(const I8Codes*)c;
+    std::memcpy(codes, q.codes.data(), q.codes.size());
+    std::memcpy(inv_norms, q.inv_norms.data(), q.inv_norms.size() * 4);
+    global2[0] = q.global_multiplier;
+    global2[1] = q.global_dequant_scale;
+}
+// standalone encoding of rows (no index) and of a query under it
+void* orc_encode_i8_rows(const float* data, int64_t n, int64_t dim, int encoding) {
+    return new I8Codes(encode_i8(data, (size_t)n, (size_t)dim, encoding));
+}
+float orc_quantize_query_codes(void* c, const float* q, int8_t* codes) {
+    return quantize_query_codes(*(const I8Codes*)c, q, codes);
+}
+float orc_i8_cosine_distance(const int8_t* q, float qinv, const int8_t* x, float xinv, int64_t dim) {
+    return i8_cosine_distance(q, qinv, x, xinv, (size_t)dim);
+}
+void orc_index_query_i8(void* h, void* c, const float* queries, int64_t nq, int64_t k, int64_t candidate_width,
+                        int64_t rerank_width, float eps, int threads, int32_t* oi, float* od) {
+    const Index& ix = *(const Index*)h;
+    const I8Codes& codes = *(const I8Codes*)c;
+    const int T = std::max(1, std::min<int>(threads, (int)std::max<int64_t>(nq, 1)));
+    par_for(T, [&](int t) {
+        for (int64_t i = t; i < nq; i += T)
+            ix.query_one_quantized(codes, queries + i * ix.dim, (size_t)i, (size_t)k, (size_t)candidate_width,
+                                   (size_t)rerank_width, eps, oi + i * k, od + i * k);
+    });
+}
 // leaves of one quantized tree (t-th of a forest) for an int8 query
 int64_t orc_tree_search_leaves_q(void* h, int64_t t, const int8_t* query, float query_scale, uint64_t st[4],
                                  int64_t budget, int64_t* out) {

@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <cstddef>
+#include <functional>
 #include <vector>
 #include <utility>
 
@@ -141,6 +142,20 @@ struct IndexParams {
     uint64_t random_seed = 42;
 };
 
+// ---- quantized search representations (crates/rinnd/src/lib.rs:133-398) ---------------------------
+// int8 (per-vector cosine), sq8u (global symmetric) and sq8p (per-vector symmetric) codes of the rows.
+enum I8Encoding : int { I8_PER_VECTOR_COSINE = 0, I8_GLOBAL_SYMMETRIC = 1, I8_PER_VECTOR_SYMMETRIC = 2 };
+struct I8Codes {
+    int encoding = I8_PER_VECTOR_COSINE;
+    size_t dim = 0;
+    std::vector<int8_t> codes;     // n x dim
+    std::vector<float> inv_norms;  // per row: 1/|codes| (cosine), the global or the row's dequantisation scale
+    float global_multiplier = 0, global_dequant_scale = 0;
+};
+I8Codes encode_i8(const float* data, size_t n, size_t dim, int encoding);
+float quantize_query_codes(const I8Codes& c, const float* q, int8_t* codes);
+float i8_cosine_distance(const int8_t* q, float q_inv_norm, const int8_t* x, float x_inv_norm, size_t dim);
+
 struct Index {
     std::vector<float> data;  // reordered
     size_t n_points = 0, dim = 0, n_neighbors = 0;
@@ -159,6 +174,11 @@ struct Index {
                float* out_dist, int threads) const;
     void query_one(const float* q, size_t query_id, size_t k, float eps, int32_t* out_idx,
                    float* out_dist) const;
+    // query_one_quantized_i8_with_workspace (core/index.rs:283-330) over the codes c of this index's rows
+    void query_one_quantized(const I8Codes& c, const float* q, size_t query_id, size_t k, size_t candidate_width,
+                             size_t rerank_width, float eps, int32_t* out_idx, float* out_dist) const;
+    void seed_ids(const float* q, const int8_t* tree_codes, float tree_scale, FastRng& rng,
+                  std::vector<int32_t>& ids) const;
 };
 
 Index build_index(const float* data, size_t n, size_t dim, int metric, int corr,
@@ -168,6 +188,9 @@ void greedy_from_seeds(const float* q, const float* data, size_t dim, int metric
                        const int32_t* indptr, const int32_t* indices, const std::vector<int32_t>& seed_ids,
                        size_t k, float eps, float min_distance, FastRng& rng, std::vector<int32_t>& ri,
                        std::vector<float>& rd);
+void greedy_core(const std::function<float(int32_t)>& dist_to, size_t n_vertices, const int32_t* indptr,
+                 const int32_t* indices, const std::vector<int32_t>& seed_ids, size_t k, float eps,
+                 float min_distance, FastRng& rng, std::vector<int32_t>& ri, std::vector<float>& rd);
 
 // std::collections::BinaryHeap<(OrderedFloat<f32>, i32)> as used by BoundedHeap
 // (core/heap/candidate_heap.rs:78-163); the layout decides tie order in into_sorted.

@@ -32,7 +32,8 @@ EXPORTS = [
     "rinnd_forest_n_trees", "rinnd_forest_tree_info", "rinnd_forest_tree_copy", "rinnd_forest_free",
     "rinnd_sort_rows", "rinnd_search_graph_build", "rinnd_index_build", "rinnd_index_free",
     "rinnd_index_query", "rinnd_index_query_noalloc", "rinnd_index_query_one", "rinnd_index_query_quantized",
-    "rinnd_index_quantize_trees", "rinnd_index_neighbor_graph",
+    "rinnd_index_quantize_trees", "rinnd_index_encode_i8", "rinnd_index_query_i8", "rinnd_index_release_data",
+    "rinnd_index_neighbor_graph",
     "rinnd_index_search_graph_info", "rinnd_index_search_graph", "rinnd_index_vertex_order",
     "rinnd_index_trees", "rinnd_server_create", "rinnd_server_submit", "rinnd_server_submit_quantized", "rinnd_server_poll", "rinnd_server_wait",
     "rinnd_server_get_stats", "rinnd_server_last_error", "rinnd_server_destroy",
@@ -171,6 +172,9 @@ def lib():
                 "rinnd_index_query_one": (C.c_int, [_p, _p, _i64, C.c_float, _p, _p]),
                 "rinnd_index_query_quantized": (C.c_int, [_p, _p, _i64, _i64, C.c_float, _i64, _i64, _p, _p]),
                 "rinnd_index_quantize_trees": (C.c_int, [_p]),
+                "rinnd_index_encode_i8": (C.c_int, [_p, C.c_int]),
+                "rinnd_index_query_i8": (C.c_int, [_p, _p, _i64, _i64, C.c_float, _i64, _i64, _p, _p]),
+                "rinnd_index_release_data": (C.c_int, [_p]),
                 "rinnd_index_neighbor_graph": (C.c_int, [_p, _p, _p]),
                 "rinnd_index_search_graph_info": (C.c_int, [_p, _p, _p, _p]),
                 "rinnd_index_search_graph": (C.c_int, [_p, _p, _p]),

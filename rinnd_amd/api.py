@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import time
 
 import numpy as np
 
@@ -30,6 +31,10 @@ _METRIC_NAMES = {
 }
 # metrics with angular RP trees (core/index.rs:606-614)
 _ANGULAR = {"cosine", "inner_product", "dot", "correlation", "true_angular", "tsss"}
+# QuantizedI8Encoding (crates/rinnd/src/lib.rs:151-156) and its storage_info names (lib.rs:475-479)
+_I8_ENCODINGS = {"int8": 0, "sq8u": 1, "sq8p": 2}
+_I8_ENCODING_NAMES = {"int8": "int8_per_vector_cosine", "sq8u": "sq8u_global_symmetric",
+                      "sq8p": "sq8p_per_vector_symmetric"}
 
 
 def version() -> str:
@@ -148,9 +153,9 @@ class RINND:
         if n_trees is None:  # py:1200-1204
             n_trees = min(12, max(3, _round_half_away(2.0 * math.log10(max(n_points, 1)))))
         self.n_points, self.dim = n_points, dim
-        # quantized cosine modes (py:714-731): the candidate search runs on FP8 e4m3 codes of the rows (the
-        # MI355X counterpart of the reference's int8 codes) with these default widths; FP32 rows are kept
-        # for the exact re-ranking
+        # quantized cosine modes (py:714-731): the candidate search runs on the int8 codes of the rows
+        # (int8 / sq8u / sq8p) with these default widths; FP32 rows are kept for the exact re-ranking unless
+        # released
         self._quantized = quantized
         self._qcw, self._qrw = int(quantized_candidate_width), int(quantized_rerank_width)
         self.n_neighbors = int(n_neighbors)
@@ -174,8 +179,19 @@ class RINND:
         forest = L.Forest(L.lib().rinnd_index_trees(h), owned=False)
         self._tree_nodes = [forest.n_nodes(t) for t in range(forest.n_trees)]
         self._tree_i8 = tree_quantization == "int8"
-        if self._tree_i8:  # lib.rs:242-248: int8 routing trees replace the FP32 search trees
-            self._ctx.check(L.lib().rinnd_index_quantize_trees(h), "rinnd_index_quantize_trees")
+        self._released = False
+        self._encode_seconds = 0.0
+        if quantized is not None:  # QuantizedI8Index::new (lib.rs:158-276)
+            t0 = time.perf_counter()
+            self._ctx.check(L.lib().rinnd_index_encode_i8(h, _I8_ENCODINGS[quantized]), "rinnd_index_encode_i8")
+            if self._tree_i8:  # lib.rs:242-248: int8 routing trees replace the FP32 search trees
+                self._ctx.check(L.lib().rinnd_index_quantize_trees(h), "rinnd_index_quantize_trees")
+            # lib.rs:714-718: by default FP32 rows go only with int8 trees and no constructor rerank width
+            retain = (not self._tree_i8 or self._qrw > 0) if retain_fp32_data is None else bool(retain_fp32_data)
+            if not retain:
+                self._ctx.check(L.lib().rinnd_index_release_data(h), "rinnd_index_release_data")
+                self._released = True
+            self._encode_seconds = time.perf_counter() - t0
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -201,18 +217,18 @@ class RINND:
                     torch.empty((nq, k), dtype=torch.float32, device=q.device))
         return np.empty((nq, k), np.int32), np.empty((nq, k), np.float32)
 
-    def _query_fp8(self, q, k, epsilon, candidate_width, rerank_width):
+    def _query_i8(self, q, k, epsilon, candidate_width, rerank_width):
         nq = int(q.shape[0])
         oi, od = self._outputs(q, nq, k)
-        st = L.lib().rinnd_index_query_quantized(self._h, L.ptr(q), nq, int(k), C.c_float(epsilon),
-                                                 int(candidate_width), int(rerank_width), L.ptr(oi), L.ptr(od))
-        self._ctx.check(st, "rinnd_index_query_quantized")
+        st = L.lib().rinnd_index_query_i8(self._h, L.ptr(q), nq, int(k), C.c_float(epsilon),
+                                          int(candidate_width), int(rerank_width), L.ptr(oi), L.ptr(od))
+        self._ctx.check(st, "rinnd_index_query_i8")
         return oi, od
 
     def query(self, query_data, k=10, epsilon=0.1):
         q = self._prep_queries(query_data)
         if self._quantized is not None:  # query_one_quantized with the constructor's widths
-            return self._query_fp8(q, k, epsilon, self._qcw, self._qrw)
+            return self._query_i8(q, k, epsilon, self._qcw, self._qrw)
         nq = int(q.shape[0])
         oi, od = self._outputs(q, nq, k)
         st = L.lib().rinnd_index_query(self._h, L.ptr(q), nq, int(k), C.c_float(epsilon), L.ptr(oi), L.ptr(od))
@@ -231,7 +247,7 @@ class RINND:
         q = self._prep_queries(query_data)
         if self._quantized is None:
             raise ValueError("query-time quantized widths require a quantized cosine_distance_mode")
-        return self._query_fp8(q, k, epsilon, candidate_width, rerank_width)
+        return self._query_i8(q, k, epsilon, candidate_width, rerank_width)
 
     def query_one(self, query, k=10, epsilon=0.1):
         q = np.ascontiguousarray(query, np.float32)
@@ -241,7 +257,7 @@ class RINND:
             raise ValueError(f"Query dimension {q.shape[0]} does not match data dimension {self.dim}")
         q2 = self._prep_queries(q[None, :], "query")
         if self._quantized is not None:
-            oi, od = self._query_fp8(q2, k, epsilon, self._qcw, self._qrw)
+            oi, od = self._query_i8(q2, k, epsilon, self._qcw, self._qrw)
             return oi[0], od[0]
         oi = np.empty(k, np.int32)
         od = np.empty(k, np.float32)
@@ -289,16 +305,19 @@ class RINND:
         return t["hyperplanes"], t["offsets"], t["children"], order[t["indices"]].astype(np.int32)
 
     def storage_info(self):
-        """py:1028-1077: the FP32 representation, or the FP8 e4m3 codes (rows padded to 32 bytes, one FP32
-        scale per row) of a quantized index with its retained FP32 rows."""
+        """py:1028-1077: the FP32 representation, or the int8 codes of a quantized index (one byte a value,
+        one FP32 inverse norm or scale a row), its retained FP32 arrays and its search trees."""
         data_bytes = self.n_points * self.dim * 4
         if self._quantized is not None:
-            code_bytes = self.n_points * ((self.dim + 31) // 32 * 32)
+            code_bytes = self.n_points * self.dim
             meta = self.n_points * 4
             # search trees: children 8 bytes and offsets 4 bytes a node, hyperplanes dim values a node, leaf
             # indices 4 bytes a point (lib.rs:498-541, 228-240)
             nodes = sum(self._tree_nodes)
             topology = nodes * 8 + len(self._tree_nodes) * self.n_points * 4
+            # retained_fp32_bytes (core/index.rs:92-101): rows, neighbour distances and FP32 tree planes and offsets
+            retained = ((0 if self._released else data_bytes) + self.n_points * self.n_neighbors * 4
+                        + (0 if self._tree_i8 else (nodes * self.dim + nodes) * 4))
             if self._tree_i8:
                 trees = {"tree_code_bytes": nodes * self.dim, "tree_scale_bytes": nodes * 4,
                          "tree_offset_bytes": nodes * 4, "tree_topology_bytes": topology,
@@ -309,12 +328,13 @@ class RINND:
                 trees = {"tree_code_bytes": 0, "tree_scale_bytes": 0, "tree_offset_bytes": 0,
                          "tree_topology_bytes": 0, "hypothetical_releasable_fp32_tree_bytes": 0,
                          "released_fp32_tree_bytes": 0, "retained_fp32_tree_topology_bytes": topology}
-            allocated = (code_bytes + meta + data_bytes + trees["retained_fp32_tree_topology_bytes"]
+            allocated = (code_bytes + meta + retained + trees["retained_fp32_tree_topology_bytes"]
                          + trees["tree_code_bytes"] + trees["tree_scale_bytes"] + trees["tree_offset_bytes"]
                          + trees["tree_topology_bytes"])
             return {
-                "encoding": "fp8_e4m3", "code_bytes": code_bytes, "metadata_bytes": meta,
-                "retained_fp32_bytes": data_bytes, "encode_seconds": 0.0, **trees, "released_fp32_data_bytes": 0,
+                "encoding": _I8_ENCODING_NAMES[self._quantized], "code_bytes": code_bytes, "metadata_bytes": meta,
+                "retained_fp32_bytes": retained, "encode_seconds": self._encode_seconds, **trees,
+                "released_fp32_data_bytes": data_bytes if self._released else 0,
                 "allocated_bytes": allocated, "bytes_per_vector": float(allocated) / max(self.n_points, 1),
             }
         return {

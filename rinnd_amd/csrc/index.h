@@ -46,12 +46,22 @@ struct rinnd_index {
     DBuf<float> q8s;
     DBuf<float> q8n;  // squared norms of the dequantized rows
     int64_t ld8 = 0;
+    // the quantized index's int8 codes (rinnd_index_encode_i8): n x ldi8 codes of X, per-row inverse norms or
+    // dequantisation scales, the encoding and sq8u's global multiplier and dequantisation scale
+    DBuf<int8_t> i8c;
+    DBuf<float> i8inv;
+    int64_t ldi8 = 0;
+    int i8enc = -1;
+    float i8gm = 0.f, i8gd = 0.f;
+    bool data_released = false;  // X freed (rinnd_index_release_data): only the int8 search without reranking
 };
 
 namespace rinnd {
+// candidate distances of a query: the FP32 rows, their FP8 e4m3 codes or the quantized index's int8 codes
+enum QueryRep : int { kQueryF32 = 0, kQueryFP8 = 1, kQueryI8 = 2 };
 // the index's part of a launch of the greedy search: data, graph, trees and the query parameters
 SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq, int64_t k, float eps, int64_t qid_base,
-                             int64_t search_k, int64_t rerank, bool fp8, int32_t* oip, float* odp);
+                             int64_t search_k, int64_t rerank, int rep, int32_t* oip, float* odp);
 // the FP8 e4m3 codes of the rows (built on first use)
 void ensure_fp8_codes(rinnd_index& ix);
 bool device_pointer(const void* p);

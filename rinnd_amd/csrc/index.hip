@@ -178,9 +178,23 @@ void ensure_fp8_codes(rinnd_index& ix) {
 // search_k: candidate width of the search (k for a plain query); rerank > 0: exact re-ranking of that many
 // candidates; fp8: candidate distances from the FP8 codes.  k results are written per query.
 // The index's part of a launch of the greedy search: data, graph, trees and the query parameters.
+// The searches the index can still run: without its FP32 rows only the int8 search without reranking
+// (QuantizedI8Index::ensure_rerank_available, crates/rinnd/src/lib.rs:287-300), and the int8 search only once
+// the codes exist.
+static void check_query_rep(const rinnd_index& ix, int rep, int64_t rerank) {
+    if (rep == kQueryI8 && !ix.i8c.p)
+        throw Error{RINND_ERR_INVALID_ARG, "the index has no int8 codes (rinnd_index_encode_i8)"};
+    if (ix.data_released && rep == kQueryI8 && rerank > 0)
+        throw Error{RINND_ERR_INVALID_ARG, "exact reranking is unavailable because FP32 vector data was released"};
+    if (ix.data_released && rep != kQueryI8)
+        throw Error{RINND_ERR_INVALID_ARG, "FP32 vector data was released: only the int8 search is available"};
+}
+
 SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq, int64_t k, float eps,
-                                    int64_t qid_base, int64_t search_k, int64_t rerank, bool fp8, int32_t* oip,
+                                    int64_t qid_base, int64_t search_k, int64_t rerank, int rep, int32_t* oip,
                                     float* odp) {
+    check_query_rep(ix, rep, rerank);
+    const bool fp8 = rep == kQueryFP8;
     SearchArgs a{};
     a.metric = ix.metric;
     a.X = ix.X.p;
@@ -214,6 +228,13 @@ SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq,
     a.ld8 = ix.ld8;
     a.q8s = fp8 ? ix.q8s.p : nullptr;
     a.q8n = fp8 ? ix.q8n.p : nullptr;
+    a.i8 = rep == kQueryI8 ? ix.i8c.p : nullptr;
+    a.ldi8 = ix.ldi8;
+    a.i8inv = rep == kQueryI8 ? ix.i8inv.p : nullptr;
+    a.i8enc = ix.i8enc;
+    a.i8gm = ix.i8gm;
+    a.i8gd = ix.i8gd;
+    a.seed_fixed = rep == kQueryI8;
     a.k_out = (int)k;
     a.rerank = (int)rerank;
     return a;
@@ -221,13 +242,14 @@ SearchArgs index_search_args(const rinnd_index& ix, const float* qd, int64_t nq,
 
 static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_t k, float eps, int64_t qid_base,
                        int32_t* out_idx, float* out_dist, int64_t search_k = -1, int64_t rerank = 0,
-                       bool fp8 = false) {
+                       int rep = kQueryF32) {
     hipStream_t s = ix.ctx->stream;
     if (search_k < 0) search_k = k;
     RINND_CHECK_ARG(k >= 0 && k <= search_max_k(), "k must be in [0, 512]");
     RINND_CHECK_ARG(search_k >= k && search_k <= search_max_k(), "the candidate width must be in [k, 512]");
+    check_query_rep(ix, rep, rerank);
     if (nq == 0 || k == 0) return;
-    if (fp8) {
+    if (rep == kQueryFP8) {
         RINND_CHECK_ARG(round_up(ix.dim, (int64_t)32) <= 2048, "the FP8 candidate search supports dim <= 2048");
         ensure_fp8_codes(ix);
     }
@@ -248,7 +270,7 @@ static void query_impl(rinnd_index& ix, const float* queries, int64_t nq, int64_
         oip = ix.obuf_i.p;
         odp = ix.obuf_d.p;
     }
-    SearchArgs a = index_search_args(ix, qd, nq, k, eps, qid_base, search_k, rerank, fp8, oip, odp);
+    SearchArgs a = index_search_args(ix, qd, nq, k, eps, qid_base, search_k, rerank, rep, oip, odp);
     a.visited = ix.visited.p;
     a.epochs = ix.epochs.p;
     a.seed_spill_d = ix.spill_d.p;
@@ -361,7 +383,7 @@ static void query_noalloc_impl(rinnd_index& ix, const float* queries, int64_t nq
         launch_pad_rows(s, queries, nq, (int)ix.dim, qrows, ix.ld);  // zero padding columns, every call
         qd = qrows;
     }
-    SearchArgs a = index_search_args(ix, qd, nq, k, eps, 0, k, 0, false, out_idx, out_dist);
+    SearchArgs a = index_search_args(ix, qd, nq, k, eps, 0, k, 0, kQueryF32, out_idx, out_dist);
     a.visited = visited;
     a.epochs = epochs;
     a.seed_spill_d = spd;
@@ -475,20 +497,80 @@ rinnd_status rinnd_index_query_one(rinnd_index* ix, const float* query, int64_t 
     return guard(ix->ctx, [&] { query_impl(*ix, query, 1, k, epsilon, 0, out_idx, out_dist); });
 }
 
+// resolve_quantized_candidate_width / should_rerank_quantized (core/index.rs:55-90) around a quantized search
+static void quantized_query(rinnd_index& ix, const float* queries, int64_t nq, int64_t k, float epsilon,
+                            int64_t candidate_width, int64_t rerank_width, int32_t* out_idx, float* out_dist, int rep) {
+    RINND_CHECK_ARG(nq >= 0 && k >= 0 && candidate_width >= 0 && rerank_width >= 0, "negative size");
+    const int64_t width = candidate_width == 0 ? std::max(k, rerank_width) : candidate_width;
+    RINND_CHECK_ARG(width >= k, "quantized candidate width must be at least k");
+    RINND_CHECK_ARG(rerank_width <= width, "quantized rerank width must not exceed candidate width");
+    RINND_CHECK_ARG(candidate_width == 0 || rerank_width == 0 || rerank_width >= k,
+                    "quantized rerank width must be zero or at least k");
+    const bool rerank = candidate_width == 0 ? rerank_width > k : rerank_width > 0;
+    query_impl(ix, queries, nq, k, epsilon, 0, out_idx, out_dist, width, rerank ? rerank_width : 0, rep);
+}
+
 rinnd_status rinnd_index_query_quantized(rinnd_index* ix, const float* queries, int64_t nq, int64_t k,
                                          float epsilon, int64_t candidate_width, int64_t rerank_width,
                                          int32_t* out_idx, float* out_dist) {
     if (!ix) return RINND_ERR_INVALID_ARG;
     return guard(ix->ctx, [&] {
-        RINND_CHECK_ARG(nq >= 0 && k >= 0 && candidate_width >= 0 && rerank_width >= 0, "negative size");
-        // resolve_quantized_candidate_width / should_rerank_quantized (core/index.rs:55-90)
-        const int64_t width = candidate_width == 0 ? std::max(k, rerank_width) : candidate_width;
-        RINND_CHECK_ARG(width >= k, "quantized candidate width must be at least k");
-        RINND_CHECK_ARG(rerank_width <= width, "quantized rerank width must not exceed candidate width");
-        RINND_CHECK_ARG(candidate_width == 0 || rerank_width == 0 || rerank_width >= k,
-                        "quantized rerank width must be zero or at least k");
-        const bool rerank = candidate_width == 0 ? rerank_width > k : rerank_width > 0;
-        query_impl(*ix, queries, nq, k, epsilon, 0, out_idx, out_dist, width, rerank ? rerank_width : 0, true);
+        quantized_query(*ix, queries, nq, k, epsilon, candidate_width, rerank_width, out_idx, out_dist, kQueryFP8);
+    });
+}
+
+rinnd_status rinnd_index_encode_i8(rinnd_index* ix, int encoding) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        RINND_CHECK_ARG(encoding >= 0 && encoding <= 2, "encoding must be 0 (int8), 1 (sq8u) or 2 (sq8p)");
+        RINND_CHECK_ARG(ix->metric == RINND_DIST_DIRECT_COSINE,
+                        "the quantized codes belong to a directly normalized cosine index");
+        RINND_CHECK_ARG(!ix->data_released, "FP32 vector data was released");
+        RINND_CHECK_ARG(round_up(ix->dim, (int64_t)32) <= 2048, "the int8 candidate search supports dim <= 2048");
+        hipStream_t s = ix->ctx->stream;
+        // sq8u's global multiplier 127 / max|data| and dequantisation scale (lib.rs:171-185), in host float
+        float gm = 0.f, gd = 0.f;
+        if (encoding == 1) {
+            DBuf<unsigned> mx(1);
+            HIP_TRY(hipMemsetAsync(mx.p, 0, 4, s));
+            launch_abs_max(s, ix->X.p, ix->ld, (int)ix->dim, ix->n, mx.p);
+            unsigned bits = 0;
+            HIP_TRY(hipMemcpyAsync(&bits, mx.p, 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            float gmax;
+            std::memcpy(&gmax, &bits, 4);
+            gm = gmax > 0.f ? 127.0f / gmax : 0.f;
+            gd = gm > 0.f ? 1.0f / gm : 0.f;
+        }
+        ix->ldi8 = round_up(ix->dim, (int64_t)32);
+        ix->i8c.alloc((size_t)std::max<int64_t>(ix->n, 1) * ix->ldi8);
+        ix->i8inv.alloc((size_t)std::max<int64_t>(ix->n, 1));
+        launch_i8_encode(s, ix->X.p, ix->ld, (int)ix->dim, ix->n, encoding, gm, gd, ix->i8c.p, ix->ldi8, ix->i8inv.p);
+        HIP_TRY(hipStreamSynchronize(s));
+        ix->i8enc = encoding;
+        ix->i8gm = gm;
+        ix->i8gd = gd;
+    });
+}
+
+rinnd_status rinnd_index_query_i8(rinnd_index* ix, const float* queries, int64_t nq, int64_t k, float epsilon,
+                                  int64_t candidate_width, int64_t rerank_width, int32_t* out_idx, float* out_dist) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        quantized_query(*ix, queries, nq, k, epsilon, candidate_width, rerank_width, out_idx, out_dist, kQueryI8);
+    });
+}
+
+rinnd_status rinnd_index_release_data(rinnd_index* ix) {
+    if (!ix) return RINND_ERR_INVALID_ARG;
+    return guard(ix->ctx, [&] {
+        RINND_CHECK_ARG(ix->i8c.p, "the index has no int8 codes (rinnd_index_encode_i8)");
+        HIP_TRY(hipStreamSynchronize(ix->ctx->stream));
+        ix->X.release();
+        ix->q8.release();
+        ix->q8s.release();
+        ix->q8n.release();
+        ix->data_released = true;
     });
 }
 

@@ -51,6 +51,16 @@ struct SearchArgs {
     int k_out;
     int rerank;
     const float* q8n;   // squared norms of the dequantized FP8 rows (FP8 search)
+    // int8 candidate search over the quantized index's codes (i8 null: none; alone, on a directly normalized
+    // cosine index): n x ldi8 codes (zero past dim), per-row inverse norm or dequantisation scale, the encoding
+    // (0 int8 per-vector cosine, 1 sq8u global symmetric, 2 sq8p per-vector symmetric) and sq8u's global
+    // multiplier and dequantisation scale
+    const int8_t* i8;
+    int64_t ldi8;
+    const float* i8inv;
+    int i8enc;
+    float i8gm, i8gd;
+    int seed_fixed;  // 1: every query's RNG starts from seed itself (the quantized query, core/index.rs:298)
     // Per-query parameters (null: the scalar fields above for every query).  k above is then the largest search
     // width of the launch (it sizes the result heap in LDS).
     const int32_t* q_k;       // search width
@@ -62,7 +72,7 @@ struct SearchArgs {
     float* const* q_od;
     int qcap;  // floats of the query row kept in LDS (0: read from memory); set by launch_search
     int kcap;  // result-heap entries in LDS; set by launch_search
-    int c8cap; // bytes of the query's FP8 codes in LDS; set by launch_search
+    int c8cap; // bytes of the query's FP8 or int8 codes in LDS; set by launch_search
     int i8cap; // bytes of the query's int8 routing codes in LDS; set by launch_search
 };
 
@@ -71,6 +81,11 @@ void launch_search(hipStream_t s, SearchArgs a, int slots);
 // of 32: whole MFMA k-steps), and the squared norm of each dequantized row
 void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, uint8_t* codes, int64_t ld8,
                        float* scales, float* norms);
+// the quantized index's int8 codes of the rows (encoding as SearchArgs::i8enc) and per-row inverse norms or scales
+void launch_i8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int enc, float gm, float gd,
+                      int8_t* codes, int64_t ldi8, float* inv);
+// max |x| over n rows of dim values (stride ld), as float bits into *out, which must start at zero
+void launch_abs_max(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, unsigned* out);
 // int8 copy of a tree's hyperplanes (QuantizedFlatTree::from_flat_tree): per node scale max|h| / 127 and codes
 // round(h / scale) clamped to [-127, 127]
 void launch_quantize_tree(hipStream_t s, const float* hyper, int64_t n_nodes, int64_t hdim, int dim, int8_t* codes,

@@ -225,6 +225,44 @@ __device__ __forceinline__ float fp8_lane_dot(const uint8_t* __restrict__ c, con
     return dot;
 }
 
+// int8 candidate distances (the quantized index's codes): exact integer dot products, so any order of the sums
+// gives the reference's i32.  A batch of up to 64 neighbours goes through 16 x 16 x 32 int8 MFMAs with i32
+// accumulation, laid out as fp8_wave_dot; every lane of the wave must call it.
+typedef int v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int32_t i8_wave_dot(const int8_t* __restrict__ c8, int64_t ldi8, const int8_t* qc,
+                                               int32_t x, bool ok, int lane) {
+    const unsigned long long live = __ballot(ok);
+    const int g = lane >> 4, m = lane & 15;
+    int32_t mine = 0;
+    for (int tile = 0; tile < 4; ++tile) {
+        if (!((live >> (16 * tile)) & 0xffffull)) continue;  // wave-uniform
+        const int src = 16 * tile + m;
+        const int32_t xr = __shfl(x, src);
+        const bool okr = (live >> src) & 1ull;
+        const int8_t* row = c8 + (int64_t)(okr ? xr : 0) * ldi8 + 8 * g;
+        v4i acc = {0, 0, 0, 0};
+        for (int64_t k0 = 0; k0 < ldi8; k0 += 32) {
+            const long av = okr ? *reinterpret_cast<const long*>(row + k0) : 0l;
+            const long bv = *reinterpret_cast<const long*>(qc + k0 + 8 * g);
+            acc = __builtin_amdgcn_mfma_i32_16x16x32_i8(av, bv, acc, 0, 0, 0);
+        }
+        const int i = lane - 16 * tile;
+        const int from = ((i >> 2) & 3) * 16;
+        const int32_t v0 = __shfl(acc[0], from), v1 = __shfl(acc[1], from), v2 = __shfl(acc[2], from),
+                      v3 = __shfl(acc[3], from);
+        if (i >= 0 && i < 16) mine = (i & 3) == 0 ? v0 : (i & 3) == 1 ? v1 : (i & 3) == 2 ? v2 : v3;
+    }
+    return mine;
+}
+// The same dot product by one lane, four codes per dot4.
+__device__ __forceinline__ int32_t i8_lane_dot(const int8_t* __restrict__ c, const int8_t* qc, int64_t ldi8) {
+    int32_t dot = 0;
+    for (int64_t i = 0; i < ldi8; i += 4)
+        dot = __builtin_amdgcn_sdot4(*reinterpret_cast<const int*>(c + i), *reinterpret_cast<const int*>(qc + i), dot,
+                                     false);
+    return dot;
+}
+
 // Sorts the first len entries of (d, id) by (d, id) ascending with a wave-wide bitonic network in LDS (padding to
 // a power of two with (+inf, INT32_MAX)); the sorted array is a valid min-heap.  len <= capacity, the padded
 // length must fit the arrays too.  Every lane of the wave calls it.
@@ -257,17 +295,22 @@ __device__ void wave_sort_pairs(float* d, int32_t* id, int len, int lane) {
     }
 }
 
+// Candidate distances: FP32 rows in the reference's arithmetic, FP8 e4m3 codes, or the quantized index's int8
+// codes.
+enum : int { kRepF32 = 0, kRepFP8 = 1, kRepI8 = 2 };
+
 // One instance per metric (METRIC >= 0; the switch folds away, so only that metric's accumulators take
 // registers) and one for any metric (METRIC < 0).
-template <bool Q8, int METRIC>
+template <int REP, int METRIC>
 __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
+    constexpr bool Q8 = REP == kRepFP8;
     const int metric = METRIC >= 0 ? METRIC : a.metric;
     __shared__ WaveState S;
     extern __shared__ __align__(16) float sdyn[];
     float* const sq = sdyn;                 // a.qcap floats (0: the query is read from memory)
     float* const rh_d = sdyn + a.qcap;      // result heap, a.kcap entries
     int32_t* const rh_i = (int32_t*)(rh_d + a.kcap);
-    uint8_t* const qc8 = (uint8_t*)(rh_i + a.kcap);  // FP8 search: the query's codes, a.c8cap bytes
+    uint8_t* const qc8 = (uint8_t*)(rh_i + a.kcap);  // FP8 / int8 search: the query's codes, a.c8cap bytes
     int8_t* const qi8 = (int8_t*)(qc8 + a.c8cap);     // int8 routing: the query's codes, a.i8cap bytes
     const int lane = threadIdx.x;
     const int slot = blockIdx.x;
@@ -305,6 +348,30 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             }
             for (int o = 32; o > 0; o >>= 1) qq += __shfl_xor(qq, o);
         }
+        // int8 search: the query's codes under the index's encoding (quantize_query, crates/rinnd/src/lib.rs:325-358),
+        // zero to ldi8, and its inverse norm (int8), the global dequantisation scale (sq8u) or 1 / its multiplier
+        // (sq8p)
+        float qinv = 0.f;
+        if (REP == kRepI8) {
+            float mul = a.i8gm;
+            if (a.i8enc != 1) {
+                float mx = 0.f;
+                for (int i = lane; i < dim; i += 64) mx = fmaxf(mx, fabsf(qg[i]));
+                for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+                mul = mx > 0.f ? 127.0f / mx : 0.f;
+            }
+            int8_t* const qc = (int8_t*)qc8;
+            int32_t ns = 0;
+            for (int i = lane; i < (int)a.ldi8; i += 64) {
+                const int8_t c = i < dim ? (int8_t)fminf(127.f, fmaxf(-127.f, roundf(qg[i] * mul))) : (int8_t)0;
+                qc[i] = c;
+                ns += (int32_t)c * (int32_t)c;
+            }
+            for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o);
+            qinv = a.i8enc == 0 ? (ns > 0 ? 1.0f / sqrtf((float)ns) : 0.f)
+                 : a.i8enc == 1 ? a.i8gd
+                                : (mul > 0.f ? 1.0f / mul : 0.f);
+        }
         // int8 routing trees: the query's per-vector symmetric codes (crates/rinnd/src/lib.rs:320-347) and its scale
         // (tree_query_scale, lib.rs:303-313)
         float tqs = 0.f;
@@ -334,9 +401,15 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             const float xx = a.q8n[x];
             return distance_from_parts(metric, dot, qq, xx, fmaxf(qq + xx - 2.0f * dot, 0.0f));
         };
-        // candidate distance of one lane: FP32 in the reference's arithmetic, or from the FP8 codes
+        // greedy.rs:381-387: 1 - clamp(i8 dot as f32 * query inverse norm * row inverse norm, -1, 1)
+        auto i8_dist = [&](int32_t x, int32_t dot) -> float {
+            const float sim = (float)dot * qinv * a.i8inv[x];
+            return 1.0f - fminf(fmaxf(sim, -1.0f), 1.0f);
+        };
+        // candidate distance of one lane: FP32 in the reference's arithmetic, or from the FP8 / int8 codes
         auto cand_dist = [&](int32_t x) -> float {
             if (Q8) return fp8_dist(x, fp8_lane_dot(a.q8 + (int64_t)x * a.ld8, qc8, a.ld8));
+            if (REP == kRepI8) return i8_dist(x, i8_lane_dot(a.i8 + (int64_t)x * a.ldi8, (const int8_t*)qc8, a.ldi8));
             return distance_exact(metric, q, a.X + (int64_t)x * a.ld, dim);
         };
         // distances of a batch of neighbours (every lane calls it; lane t: neighbour x, valid ok)
@@ -344,6 +417,10 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             if (Q8) {
                 const float cd = fp8_wave_dot(a.q8, a.ld8, qc8, x, ok, lane);
                 return ok ? fp8_dist(x, cd) : 0.f;
+            }
+            if (REP == kRepI8) {
+                const int32_t cd = i8_wave_dot(a.i8, a.ldi8, (const int8_t*)qc8, x, ok, lane);
+                return ok ? i8_dist(x, cd) : 0.f;
             }
             return ok ? distance_exact(metric, q, a.X + (int64_t)x * a.ld, dim) : 0.f;
         };
@@ -361,7 +438,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         }
         __threadfence_block();
         wave_sync();
-        Xoshiro rng = xoshiro_seed(a.seed + (uint64_t)qid);  // FastRng::new(seed + query_id)
+        Xoshiro rng = xoshiro_seed(a.seed + (a.seed_fixed ? 0ull : (uint64_t)qid));  // FastRng::new(seed + query_id)
         const int k = a.q_k ? a.q_k[qi] : a.k;
 
         // seed heap storage (LDS, or the per-slot global region after an overflow)
@@ -650,29 +727,32 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     }
 }
 
-template <bool Q8>
+template <int REP>
 static void launch_search_q(hipStream_t s, const SearchArgs& a, int slots, size_t lds) {
     switch (a.metric) {
         case RINND_DIST_SQEUCLIDEAN:
-            hipLaunchKernelGGL((k_search<Q8, RINND_DIST_SQEUCLIDEAN>), dim3(slots), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((k_search<REP, RINND_DIST_SQEUCLIDEAN>), dim3(slots), dim3(64), lds, s, a);
             break;
         case RINND_DIST_ALT_DOT:
-            hipLaunchKernelGGL((k_search<Q8, RINND_DIST_ALT_DOT>), dim3(slots), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((k_search<REP, RINND_DIST_ALT_DOT>), dim3(slots), dim3(64), lds, s, a);
             break;
         case RINND_DIST_DIRECT_COSINE:
-            hipLaunchKernelGGL((k_search<Q8, RINND_DIST_DIRECT_COSINE>), dim3(slots), dim3(64), lds, s, a);
+            hipLaunchKernelGGL((k_search<REP, RINND_DIST_DIRECT_COSINE>), dim3(slots), dim3(64), lds, s, a);
             break;
-        default: hipLaunchKernelGGL((k_search<Q8, -1>), dim3(slots), dim3(64), lds, s, a);
+        default: hipLaunchKernelGGL((k_search<REP, -1>), dim3(slots), dim3(64), lds, s, a);
     }
 }
 void launch_search(hipStream_t s, SearchArgs a, int slots) {
+    if (a.i8 && (a.q8 || a.metric != RINND_DIST_DIRECT_COSINE))
+        throw Error{RINND_ERR_INVALID_ARG, "int8 codes are searched alone, on a directly normalized cosine index"};
     a.qcap = a.dim <= kQLds ? (int)round_up(a.dim, 4) : 0;
     a.kcap = (int)round_up(std::max(a.k, 1), 4);
-    a.c8cap = a.q8 ? (int)round_up(a.ld8, (int64_t)16) : 0;
+    a.c8cap = a.q8 ? (int)round_up(a.ld8, (int64_t)16) : a.i8 ? (int)round_up(a.ldi8, (int64_t)16) : 0;
     a.i8cap = a.hq ? (int)round_up((int64_t)a.dim, (int64_t)16) : 0;
     const size_t lds = ((size_t)a.qcap + 2 * (size_t)a.kcap) * 4 + (size_t)a.c8cap + (size_t)a.i8cap;
-    if (a.q8) launch_search_q<true>(s, a, slots, lds);
-    else launch_search_q<false>(s, a, slots, lds);
+    if (a.q8) launch_search_q<kRepFP8>(s, a, slots, lds);
+    else if (a.i8) hipLaunchKernelGGL((k_search<kRepI8, RINND_DIST_DIRECT_COSINE>), dim3(slots), dim3(64), lds, s, a);
+    else launch_search_q<kRepF32>(s, a, slots, lds);
 }
 
 // One wave per node: max |h| over the plane by a wave reduction, the node's scale, then its codes (the padding
@@ -739,6 +819,61 @@ void launch_fp8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64
     if (n == 0) return;
     hipLaunchKernelGGL(k_fp8_encode, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)4), 1 << 16)), dim3(256), 0,
                        s, X, ld, dim, n, codes, ld8, scales, norms);
+}
+
+// The quantized index's codes, one wave per row (QuantizedI8Index::new, crates/rinnd/src/lib.rs:166-224): the
+// multiplier 127 / max|row| (or the global one, sq8u), codes round(x * multiplier) clamped to [-127, 127] and zero
+// to ldi8, and per row 1 / sqrt(sum of squared codes) (int8), the global dequantisation scale (sq8u) or
+// 1 / multiplier (sq8p).
+__global__ __launch_bounds__(256) void k_i8_encode(const float* __restrict__ X, int64_t ld, int dim, int64_t n,
+                                                   int enc, float gm, float gd, int8_t* __restrict__ codes,
+                                                   int64_t ldi8, float* __restrict__ inv) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
+        const float* x = X + r * ld;
+        float mul = gm;
+        if (enc != 1) {
+            float mx = 0.f;
+            for (int i = lane; i < dim; i += 64) mx = fmaxf(mx, fabsf(x[i]));
+            for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+            mul = mx > 0.f ? 127.0f / mx : 0.f;
+        }
+        int32_t ns = 0;
+        for (int64_t i = lane; i < ldi8; i += 64) {
+            const int8_t c = i < dim ? (int8_t)fminf(127.f, fmaxf(-127.f, roundf(x[i] * mul))) : (int8_t)0;
+            codes[r * ldi8 + i] = c;
+            ns += (int32_t)c * (int32_t)c;
+        }
+        for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o);
+        if (lane == 0)
+            inv[r] = enc == 0 ? (ns > 0 ? 1.0f / sqrtf((float)ns) : 0.f) : enc == 1 ? gd : (mul > 0.f ? 1.0f / mul : 0.f);
+    }
+}
+void launch_i8_encode(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, int enc, float gm, float gd,
+                      int8_t* codes, int64_t ldi8, float* inv) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_i8_encode, dim3((unsigned)std::min<int64_t>(ceil_div(n, (int64_t)4), 1 << 16)), dim3(256), 0,
+                       s, X, ld, dim, n, enc, gm, gd, codes, ldi8, inv);
+    HIP_TRY(hipGetLastError());
+}
+
+// max |x| over the n x dim block (rows of stride ld) into *out (which starts at zero): the bits of a non-negative
+// float order as the float does, so an unsigned atomic max keeps the largest.
+__global__ __launch_bounds__(256) void k_abs_max(const float* __restrict__ X, int64_t ld, int dim, int64_t n,
+                                                 unsigned* out) {
+    float mx = 0.f;
+    const int64_t total = n * (int64_t)dim;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256)
+        mx = fmaxf(mx, fabsf(X[(e / dim) * ld + e % dim]));
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(mx));
+}
+void launch_abs_max(hipStream_t s, const float* X, int64_t ld, int dim, int64_t n, unsigned* out) {
+    if (n <= 0 || dim <= 0) return;
+    const int64_t total = n * (int64_t)dim;
+    hipLaunchKernelGGL(k_abs_max, dim3((unsigned)std::min<int64_t>(ceil_div(total, (int64_t)256), 4096)), dim3(256), 0,
+                       s, X, ld, dim, n, out);
+    HIP_TRY(hipGetLastError());
 }
 
 int search_max_k() { return kMaxK; }

@@ -144,7 +144,8 @@ void launch_batch(rinnd_server* S, LaunchSlot& L, std::vector<std::shared_ptr<Re
     }
     HIP_TRY(hipMemcpyAsync(L.params.p, L.pinned, P.bytes, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(L.status.p, 0, 4, s));
-    SearchArgs a = index_search_args(ix, L.Q.p, row, 0, 0.f, 0, maxw, 0, S->p.quantized != 0, nullptr, nullptr);
+    SearchArgs a = index_search_args(ix, L.Q.p, row, 0, 0.f, 0, maxw, 0, S->p.quantized ? kQueryFP8 : kQueryF32,
+                                     nullptr, nullptr);
     a.q_qid = (const int64_t*)(L.params.p + P.qid);
     a.q_oi = (int32_t* const*)(L.params.p + P.oi);
     a.q_od = (float* const*)(L.params.p + P.od);

@@ -80,7 +80,7 @@ def test_quantized_cosine_index_with_int8_trees():
     X = (centres[rng.integers(0, 40, 6000)] + 0.08 * rng.standard_normal((6000, 64))).astype(np.float32)
     Q = (centres[rng.integers(0, 40, 300)] + 0.08 * rng.standard_normal((300, 64))).astype(np.float32)
     kw = dict(metric="cosine", normalize=True, cosine_distance_mode="int8", n_neighbors=15, random_state=7)
-    a = RINND(X, tree_quantization="int8", **kw)
+    a = RINND(X, tree_quantization="int8", retain_fp32_data=True, **kw)
     b = RINND(X, **kw)
     sa, sb = a.storage_info(), b.storage_info()
     nodes = sum(b._tree_nodes)

@@ -121,7 +121,7 @@ def test_python_quantized_cosine_mode():
     with pytest.raises(ValueError, match="zero or at least k"):
         g.query_quantized(X[:2], k=5, candidate_width=8, rerank_width=2)
     info = g.storage_info()
-    assert info["encoding"] == "fp8_e4m3" and info["code_bytes"] == 3000 * 32
+    assert info["encoding"] == "int8_per_vector_cosine" and info["code_bytes"] == 3000 * 32
     plain = RINND(X, n_neighbors=10, n_trees=4, n_iters=3)
     with pytest.raises(ValueError, match="quantized cosine_distance_mode"):
         plain.query_quantized(X[:2], k=5)
