@@ -81,9 +81,10 @@ def parse():
                     help="queries of the serving-form stream (the stated form has 4,000,000; the default run streams "
                          "1,000,000 of the same form to stay within a few minutes)")
     ap.add_argument("--sv-load", type=float, default=0.7, help="offered Poisson rate as a fraction of the saturated QPS")
-    ap.add_argument("--sv-modes", default="fp32,fp32_w32,fp8",
+    ap.add_argument("--sv-modes", default="fp32,fp32_w32,fp8,int8",
                     help="searches of the serving form: fp32 (the reference's query), fp32_wN (search-width floor N), "
-                         "fp8 (FP8 candidates + FP32 re-ranking)")
+                         "fp8 (FP8 candidates + FP32 re-ranking), int8 (the reference's quantized cosine index: int8 "
+                         "codes, default widths)")
     ap.add_argument("--sv-visited-gb", type=float, default=8.0,
                     help="visited-set pool of the query service (GiB; n bytes per concurrent search)")
     ap.add_argument("--sv-inflight", type=int, default=4, help="launches in flight in the query service")
@@ -560,8 +561,9 @@ def serving_form(torch, ctx, L, args):
     normalised rows (DirectNormalizedCosine), graph k=20 (search degree <= 30); a stream of 4,000,000 queries from the
     same mixture, in arrival batches of 1-4,096 queries (log-uniform), k in {10, 100} (4:1), epsilon in {0.05, 0.2}
     (1:1), through the query service (rinnd_server_*): first saturated (4,096-query batches back to back), then an
-    open-loop Poisson stream at a fraction of that rate, for the FP32 search and for FP8 e4m3 candidates on MFMA
-    with FP32 re-ranking.  Latency of a batch = completion - scheduled arrival."""
+    open-loop Poisson stream at a fraction of that rate, for the FP32 search, for FP8 e4m3 candidates on MFMA
+    with FP32 re-ranking and for the int8 codes of the reference's quantized cosine index.  Latency of a batch =
+    completion - scheduled arrival."""
     import threading
     from rinnd_amd import datasets
     sys.setswitchinterval(1e-4)  # the waiter thread takes the GIL promptly when a request completes
@@ -581,6 +583,10 @@ def serving_form(torch, ctx, L, args):
                                         C.byref(h), C.byref(stats)), "rinnd_index_build")
     t_index = time.perf_counter() - t0
     log(f"[bench] serving form: index built in {t_index:.2f} s ({stats.iterations} iterations)")
+    if "int8" in args.sv_modes.split(","):  # cosine_distance_mode="int8": per-vector codes of the rows
+        t0 = time.perf_counter()
+        ctx.check(L.lib().rinnd_index_encode_i8(h, 0), "rinnd_index_encode_i8")
+        log(f"[bench] serving form: int8 codes in {time.perf_counter() - t0:.3f} s")
     # FP64 ground truth of a sample of the stream (cosine on unit rows: largest dot products)
     sample = np.arange(0, nq, max(1, nq // 10_000))
     X64 = Xd.double()
@@ -671,8 +677,8 @@ def serving_form(torch, ctx, L, args):
         fp8 = mode == "fp8"
         floor = int(mode[len("fp32_w"):]) if mode.startswith("fp32_w") else 0
         srv = L.Server(h, max_batch=4096, max_inflight=args.sv_inflight, max_k=128,
-                       visited_bytes=int(args.sv_visited_gb * (1 << 30)), quantized=fp8, target_batch=args.sv_target,
-                       min_width=floor)
+                       visited_bytes=int(args.sv_visited_gb * (1 << 30)),
+                       quantized="int8" if mode == "int8" else fp8, target_batch=args.sv_target, min_width=floor)
         wall, lat = run(srv, fp8, None)  # saturated
         qps_max = nq / wall
         rec = recall()
@@ -695,6 +701,8 @@ def serving_form(torch, ctx, L, args):
                      "gpu_ms": st["gpu_ms"]}
         if fp8:
             out[mode]["widths"] = "candidate/rerank 64 for k=10, 128 for k=100"
+        if mode == "int8":
+            out[mode]["widths"] = "candidate width k, no re-ranking (the reference's default quantized query)"
         if floor:
             out[mode]["min_width"] = floor
         log(f"[bench] serving {mode}: Poisson {nq / wall_p:.0f} QPS (offered {rate_q:.0f}), p50 {pq(0.5):.2f} ms, "

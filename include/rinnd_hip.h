@@ -266,15 +266,16 @@ rinnd_status rinnd_index_query(rinnd_index* index, const float* queries, int64_t
 /* `query_one_with_workspace` (core/index.rs:253-261): one query with query id 0. */
 rinnd_status rinnd_index_query_one(rinnd_index* index, const float* query, int64_t k, float epsilon,
                                    int32_t* out_idx, float* out_dist);
-/* The quantized query with explicit widths, `query_quantized(query_data, k, epsilon, candidate_width,
- * rerank_width)` (rinnd/src/lib.rs:825-870) over `query_one_quantized_i8_with_workspace`
- * (core/index.rs:283-330): the greedy search runs with a candidate width resolved as in
- * resolve_quantized_candidate_width (core/index.rs:55-78) on distances from FP8 e4m3 codes of the data
- * (per-row symmetric scale, built on the first call; the reference's counterpart is its int8 encoding), then,
- * when should_rerank_quantized (core/index.rs:81-89) says so, the first rerank_width candidates get their
- * FP32 distance and are sorted by (distance, id) (rerank_quantized_candidates, core/index.rs:103-124).
- * k results per query, ids in input order, the index's correction applied, padded with (-1, +inf); the
- * width rules the reference asserts are returned as RINND_ERR_INVALID_ARG.  dim <= 2048. */
+/* FP8 pre-filter, with no counterpart in the reference: the quantized query's structure (query_quantized,
+ * rinnd/src/lib.rs:825-870, over query_one_quantized_i8_with_workspace, core/index.rs:283-330) on any index,
+ * with candidate distances from FP8 e4m3 codes of the data (per-row symmetric scale, built on the first call) on
+ * the matrix cores instead of the reference's int8 codes (rinnd_index_query_i8 has those): the greedy search runs
+ * with a candidate width resolved as in resolve_quantized_candidate_width (core/index.rs:55-78), then, when
+ * should_rerank_quantized (core/index.rs:81-89) says so, the first rerank_width candidates get their FP32
+ * distance and are sorted by (distance, id) (rerank_quantized_candidates, core/index.rs:103-124).  Query ids
+ * 0..nq-1 seed the random fill as in rinnd_index_query.  k results per query, ids in input order, the index's
+ * correction applied, padded with (-1, +inf); the width rules the reference asserts are returned as
+ * RINND_ERR_INVALID_ARG.  dim <= 2048. */
 rinnd_status rinnd_index_query_quantized(rinnd_index* index, const float* queries, int64_t nq, int64_t k,
                                          float epsilon, int64_t candidate_width, int64_t rerank_width,
                                          int32_t* out_idx, float* out_dist);
@@ -324,8 +325,8 @@ rinnd_status rinnd_index_query_noalloc(rinnd_index* index, const float* queries,
  * answers requests submitted from any number of host threads: a dispatcher thread gathers the pending requests
  * into one launch (arrival batching; one launch mixes requests of different k and epsilon), a completion thread
  * retires the launches in order.  Each request's results equal those of rinnd_index_query (or, for a quantized
- * server, rinnd_index_query_quantized with the server's widths) on the same queries: query ids 0..nq-1 within the
- * request seed its random fill.
+ * server, rinnd_index_query_quantized / rinnd_index_query_i8 with the request's widths) on the same queries: query
+ * ids 0..nq-1 within the request seed its random fill (the int8 search seeds every query alike).
  * Admission by memory: a request is admitted into a launch when the launch has room for its queries (at most
  * max_batch per launch; launch buffers are fixed, max_inflight launches at a time, each on its own stream) and its
  * search width fits the beam memory (max_k result-heap entries per query); the concurrent searches share a fixed
@@ -339,7 +340,8 @@ typedef struct rinnd_server_params {
     int32_t max_inflight;   /* launches prepared or running at a time (each owns its staging buffers), e.g. 3 */
     int64_t max_k;          /* largest search width a request may use (beam memory per query) */
     int64_t visited_bytes;  /* memory of the visited-set pool; fixes the number of concurrent searches */
-    int32_t quantized;      /* 1: FP8 e4m3 candidate distances on MFMA, then FP32 re-ranking */
+    int32_t quantized;      /* 1: FP8 e4m3 candidate distances on MFMA (rinnd_index_query_quantized); 2: the
+                               index's int8 codes (rinnd_index_query_i8); then FP32 re-ranking */
     int64_t candidate_width;/* quantized: as rinnd_index_query_quantized's candidate_width */
     int64_t rerank_width;   /* quantized: as rinnd_index_query_quantized's rerank_width */
     int64_t target_batch;   /* a launch closes once it holds this many queries (0: min(1024, max_batch)) */

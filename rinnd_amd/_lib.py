@@ -332,11 +332,15 @@ class Forest:
 
 
 class Server:
-    """rinnd_server: asynchronous batched queries over a resident index (handle of rinnd_index_build)."""
+    """rinnd_server: asynchronous batched queries over a resident index (handle of rinnd_index_build).
+    quantized: False (FP32), True or "fp8" (FP8 e4m3 candidates) or "int8" (the index's int8 codes)."""
 
     def __init__(self, index_handle, max_batch=4096, max_inflight=3, max_k=128, visited_bytes=8 << 30,
                  quantized=False, candidate_width=0, rerank_width=0, target_batch=0, min_width=0):
-        p = ServerParams(int(max_batch), int(max_inflight), int(max_k), int(visited_bytes), int(bool(quantized)),
+        q = {"fp8": 1, "int8": 2}.get(quantized, None) if isinstance(quantized, str) else int(bool(quantized))
+        if q is None:
+            raise ValueError(f"unknown quantized mode {quantized!r}")
+        p = ServerParams(int(max_batch), int(max_inflight), int(max_k), int(visited_bytes), q,
                          int(candidate_width), int(rerank_width), int(target_batch), int(min_width))
         h = C.c_void_p()
         st = lib().rinnd_server_create(index_handle, C.byref(p), C.byref(h))

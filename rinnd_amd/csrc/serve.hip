@@ -144,8 +144,9 @@ void launch_batch(rinnd_server* S, LaunchSlot& L, std::vector<std::shared_ptr<Re
     }
     HIP_TRY(hipMemcpyAsync(L.params.p, L.pinned, P.bytes, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(L.status.p, 0, 4, s));
-    SearchArgs a = index_search_args(ix, L.Q.p, row, 0, 0.f, 0, maxw, 0, S->p.quantized ? kQueryFP8 : kQueryF32,
-                                     nullptr, nullptr);
+    SearchArgs a = index_search_args(ix, L.Q.p, row, 0, 0.f, 0, maxw, 0,
+                                     S->p.quantized == 2 ? kQueryI8 : S->p.quantized ? kQueryFP8 : kQueryF32, nullptr,
+                                     nullptr);
     a.q_qid = (const int64_t*)(L.params.p + P.qid);
     a.q_oi = (int32_t* const*)(L.params.p + P.oi);
     a.q_od = (float* const*)(L.params.p + P.od);
@@ -300,8 +301,14 @@ rinnd_status rinnd_server_create(rinnd_index* ix, const rinnd_server_params* par
         if (p.min_width > p.max_k) throw Error{RINND_ERR_INVALID_ARG, "min_width exceeds max_k"};
         if (p.quantized && p.min_width > 0)
             throw Error{RINND_ERR_INVALID_ARG, "min_width applies to FP32 servers (quantized servers take widths)"};
+        if (p.quantized < 0 || p.quantized > 2)
+            throw Error{RINND_ERR_INVALID_ARG, "quantized must be 0 (FP32), 1 (FP8) or 2 (int8 codes)"};
+        if (p.quantized == 2 && !ix->i8c.p)
+            throw Error{RINND_ERR_INVALID_ARG, "the index has no int8 codes (rinnd_index_encode_i8)"};
+        if (p.quantized != 2 && ix->data_released)
+            throw Error{RINND_ERR_INVALID_ARG, "FP32 vector data was released: only the int8 search is available"};
         HIP_TRY(hipSetDevice(S->device));
-        if (p.quantized) {
+        if (p.quantized == 1) {
             std::lock_guard<std::mutex> lk(ix->ctx->mu);  // the codes are built on the index's stream
             ensure_fp8_codes(*ix);
             HIP_TRY(hipStreamSynchronize(ix->ctx->stream));
@@ -382,6 +389,8 @@ static rinnd_status submit(rinnd_server* S, const float* queries, int64_t nq, in
         if (cw != 0 && rw != 0 && rw < k)
             return fail(RINND_ERR_INVALID_ARG, "quantized rerank width must be zero or at least k");
         r->rerank = cw == 0 ? (rw > k ? rw : 0) : rw;
+        if (r->rerank > 0 && S->ix->data_released)
+            return fail(RINND_ERR_INVALID_ARG, "exact reranking is unavailable because FP32 vector data was released");
     }
     if (r->width > S->p.max_k)
         return fail(RINND_ERR_UNSUPPORTED, "the request's search width exceeds the server's beam memory (max_k)");

@@ -147,6 +147,49 @@ def test_quantized_server_equals_quantized_query():
     srv.close()
 
 
+def test_int8_server_equals_int8_query():
+    """A server over the index's int8 codes (the reference's quantized cosine index): each request, with the
+    server's or its own widths, equals rinnd_index_query_i8 on the same rows; a released index refuses re-ranking
+    requests and FP32 / FP8 servers."""
+    rng = np.random.default_rng(6)
+    c = rng.standard_normal((40, 32)).astype(np.float32)
+    X = c[rng.integers(0, 40, 6000)] + 0.3 * rng.standard_normal((6000, 32)).astype(np.float32)
+    Q = c[rng.integers(0, 40, 400)] + 0.3 * rng.standard_normal((400, 32)).astype(np.float32)
+    Q /= np.linalg.norm(Q, axis=1, keepdims=True)
+    g = RINND(X, metric="cosine", normalize=True, cosine_distance_mode="sq8p", n_neighbors=20, n_trees=4, n_iters=5,
+              random_state=4)
+    srv = L.Server(g._h, max_batch=256, quantized="int8", max_k=64)
+    reqs = []
+    for i, (nq, k, eps, widths) in enumerate([(50, 10, 0.05, None), (200, 10, 0.2, None), (7, 20, 0.2, (48, 40)),
+                                              (143, 10, 0.05, (0, 24))]):
+        q = np.ascontiguousarray(Q[i * 50:i * 50 + nq])
+        oi, od = np.zeros((nq, k), np.int32), np.zeros((nq, k), np.float32)
+        if widths is None:
+            t = srv.submit(q, nq, k, eps, oi, od)
+        else:
+            t = srv.submit(q, nq, k, eps, oi, od, candidate_width=widths[0], rerank_width=widths[1])
+        reqs.append((t, q, k, eps, oi, od, widths or (0, 0)))
+    for t, q, k, eps, oi, od, widths in reqs:
+        srv.wait(t)
+        wi, wd = np.zeros((len(q), k), np.int32), np.zeros((len(q), k), np.float32)
+        assert L.lib().rinnd_index_query_i8(g._h, L.ptr(q), len(q), k, C.c_float(eps), widths[0], widths[1],
+                                            L.ptr(wi), L.ptr(wd)) == 0
+        np.testing.assert_array_equal(oi, wi)
+        np.testing.assert_array_equal(od.view(np.uint32), wd.view(np.uint32))
+    srv.close()
+    assert L.lib().rinnd_index_release_data(g._h) == 0
+    srv = L.Server(g._h, max_batch=64, quantized="int8", max_k=64)
+    q = np.ascontiguousarray(Q[:5])
+    oi, od = np.zeros((5, 10), np.int32), np.zeros((5, 10), np.float32)
+    srv.wait(srv.submit(q, 5, 10, 0.1, oi, od))
+    with pytest.raises(ValueError, match="FP32 vector data was released"):
+        srv.submit(q, 5, 10, 0.1, oi, od, candidate_width=20, rerank_width=10)
+    srv.close()
+    for mode in (False, "fp8"):
+        with pytest.raises(L.RinndError):
+            L.Server(g._h, max_batch=64, quantized=mode)
+
+
 def test_admission_limits_and_tickets(setup):
     X, Q, g = setup
     srv = L.Server(g._h, max_batch=100, max_k=50)
