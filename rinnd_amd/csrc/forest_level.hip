@@ -21,7 +21,11 @@
 //                  entry's first child entry
 //   k_lvl_scatter  stable partition of every split node's positions
 // Dense levels write nothing indexed by position from the point-major side tests (no scattered writes);
-// sparse levels test position-major and write position-ordered flags.
+// sparse levels test position-major and write position-ordered flags.  Once the active entries average at
+// most 1,024 of the tree x point positions, flag chunks, their scan, k_lvl_counts and k_lvl_scatter give way
+// to one pass per entry (k_lvl_part_wave: a wave per entry of at most 1,024 members; k_lvl_part_entry: a
+// workgroup per larger one) that reads the entry's flags (sparse levels: tests its sides with the arithmetic
+// of k_lvl_margins_pos), counts them and partitions the entry's positions in the same stable order.
 // Node ids are breadth-first per tree (root 0); leaves hold their [start, end) range of
 // tree.indices exactly as in the depth-first layout.
 #include <hipcub/hipcub.hpp>
@@ -853,6 +857,279 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(const LvlNode* nodes, int n
     }
 }
 
+// ---- entry-parallel partition (the levels with many active entries) -------------------------------------
+// One workgroup per active entry: the entry's left flags (dense levels: the bit the side tests left in each
+// member's mask word; sparse levels: the side test itself, with the arithmetic of k_lvl_margins_pos), their
+// count (a degenerate split becomes a leaf, as in k_lvl_counts), then the stable partition of the entry's
+// positions into the other buffer - the same order k_lvl_scatter gives, without the per-position entry table,
+// the flag chunks or their scan.  An entry that does not split copies its positions (both buffers agree from
+// then on).  Entries of at most kEntryStage members keep their ids and flags in LDS between the passes; larger
+// ones keep the flags in flags8.
+constexpr int kEntryStage = 4096;
+constexpr int kWaveStage = 1024;  // entries of at most this many members: one wave each (k_lvl_part_wave)
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* red) {  // 256 threads
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+// exclusive prefix of v over the 256 threads in thread order; *total gets the sum
+__device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* red, uint32_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    __syncthreads();
+    if (lane == 63) red[4 + w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int i = 0; i < w; ++i) before += red[4 + i];
+    *total = red[4] + red[5] + red[6] + red[7];
+    return before + incl - v;
+}
+
+// k_lvl_margins_pos's margin by a group of four lanes, added in the order of its xor butterfly (offsets 8, 4,
+// 2, 1) over its 16 lanes' partial sums (lane l: dimensions 4 l .. 4 l + 3, then every 64 further): q = lane % 4 takes the partial sums q, q + 4, q + 8, q + 12; rows
+// are read 64 contiguous bytes per group and step); valid in lane q == 0.  Every lane of the wave calls it.
+__device__ __forceinline__ float margin_pos_quad(const float* __restrict__ x, const uint16_t* __restrict__ hb,
+                                                 int dim, bool ok, int q) {
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gl = q + 4 * r;
+            float acc = 0.f;
+            for (int d0 = 4 * gl; d0 < dim; d0 += 64) {
+                const float4 xv = *reinterpret_cast<const float4*>(x + d0);
+                const uint2 hv = *reinterpret_cast<const uint2*>(hb + d0);
+                acc = fmaf(xv.x, bf16_lo(hv.x), acc);
+                acc = fmaf(xv.y, bf16_hi(hv.x), acc);
+                acc = fmaf(xv.z, bf16_lo(hv.y), acc);
+                acc = fmaf(xv.w, bf16_hi(hv.y), acc);
+            }
+            v[r] = acc;
+        }
+    }
+    const float b = (v[0] + v[2]) + (v[1] + v[3]);  // (a_q = v_q + v_q+8) + (a_q+4 = v_q+4 + v_q+12)
+    const float c = b + __shfl_xor(b, 2);
+    return c + __shfl_xor(c, 1);
+}
+
+template <bool SPARSE>
+__global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode* __restrict__ nodes, int64_t na,
+                                                        const int32_t* const* perm, int32_t* const* perm2,
+                                                        const uint16_t* __restrict__ mask16,
+                                                        const PInfo* __restrict__ pinfo, uint8_t* split,
+                                                        uint32_t* nchild, uint32_t* leftsz, uint8_t* flags8) {
+    __shared__ int32_t s_pt[kEntryStage];
+    __shared__ uint8_t s_side[kEntryStage];
+    __shared__ uint32_t red[8];
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) nchild[na] = 0;  // the scan's total
+    for (int64_t e = blockIdx.x; e < na; e += gridDim.x) {
+        const LvlNode nd = nodes[e];
+        const int t = nd.tree;
+        const int m = nd.end - nd.start;
+        if (m <= kWaveStage) continue;  // uniform: k_lvl_part_wave's
+        const int32_t* P = perm[t] + nd.start;
+        int32_t* P2 = perm2[t] + nd.start;
+        if (!split[e]) {  // uniform: a leaf since k_lvl_planes
+            for (int i = tid; i < m; i += 256) P2[i] = P[i];
+            if (tid == 0) nchild[e] = 0;
+            continue;
+        }
+        const bool staged = m <= kEntryStage;
+        uint8_t* F = flags8 + (int64_t)t * a.n + nd.start;
+        uint32_t cnt = 0;
+        if (SPARSE) {  // a lane per member, k_lvl_margins_pos's arithmetic
+            const PInfo pi = pinfo[e];
+            const uint16_t* hb = a.hyper_bf[t] + (int64_t)pi.node * a.dim;
+            const uint64_t key = a.keys[t];
+            const int q = tid & 3, g = tid >> 2;
+            for (int i0 = 0; i0 < m; i0 += 64) {
+                const int i = i0 + g;
+                const int32_t pt = i < m ? P[i] : 0;
+                const float acc = margin_pos_quad(a.X + (int64_t)pt * a.ld, hb, a.dim, i < m, q);
+                if (i < m && q == 0) {
+                    const uint8_t sd = (uint8_t)side_of(acc + pi.off, key, pi.node, pt);
+                    if (staged) {
+                        s_pt[i] = pt;
+                        s_side[i] = sd;
+                    } else {
+                        F[i] = sd;
+                    }
+                    cnt += sd;
+                }
+            }
+        } else {
+            const uint16_t* mk = mask16 + (int64_t)(t >> 4) * a.n;
+            const int sh = t & 15;
+            for (int i = tid; i < m; i += 256) {
+                const int32_t pt = P[i];
+                const uint8_t sd = (uint8_t)((mk[pt] >> sh) & 1u);
+                if (staged) {
+                    s_pt[i] = pt;
+                    s_side[i] = sd;
+                } else {
+                    F[i] = sd;
+                }
+                cnt += sd;
+            }
+        }
+        const uint32_t L = block_sum_u32(cnt, red);  // also orders pass 1's writes before pass 2's reads
+        if (L == 0 || L == (uint32_t)m) {  // uniform: degenerate split -> leaf with a zero hyperplane
+            if (tid == 0) {
+                split[e] = 0;
+                nchild[e] = 0;
+                a.children[t][2 * nd.node] = -nd.start;
+                a.children[t][2 * nd.node + 1] = -nd.end;
+                a.offsets[t][nd.node] = 0.f;
+            }
+            float* hp = a.hyper[t] + (int64_t)nd.node * a.dim;
+            for (int i = tid; i < a.dim; i += 256) hp[i] = 0.f;
+            for (int i = tid; i < m; i += 256) P2[i] = P[i];
+            __syncthreads();  // the staged arrays are rewritten by the next entry
+            continue;
+        }
+        if (tid == 0) {
+            nchild[e] = 2;
+            leftsz[e] = L;
+        }
+        uint32_t lb = 0, rb = 0;  // left / right members placed by the earlier tiles
+        for (int base = 0; base < m; base += 1024) {
+            int32_t pv[4];
+            uint32_t f[4];
+            const int i0 = base + 4 * tid;
+            uint32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + k;
+                f[k] = 0u;
+                pv[k] = 0;
+                if (i < m) {
+                    pv[k] = staged ? s_pt[i] : P[i];
+                    f[k] = staged ? s_side[i] : F[i];
+                }
+                c += f[k];
+            }
+            uint32_t tileL = 0;
+            const uint32_t lx = block_excl_u32(c, red, &tileL);  // lefts of this tile before this thread
+            const int valid_before = max(0, min(4 * tid, m - base));
+            uint32_t lp = lb + lx, rp = rb + (uint32_t)valid_before - lx;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (i0 + k >= m) break;
+                if (f[k]) P2[lp++] = pv[k];
+                else P2[L + rp++] = pv[k];
+            }
+            const uint32_t tile_items = (uint32_t)min(1024, m - base);
+            lb += tileL;
+            rb += tile_items - tileL;
+        }
+        __syncthreads();  // the staged arrays are rewritten by the next entry
+    }
+}
+
+// The same for the entries of at most kWaveStage members, one wave each (four per workgroup): no workgroup
+// barriers; the stable partition takes 64 members a step, ranks from ballots.
+template <bool SPARSE>
+__global__ __launch_bounds__(256) void k_lvl_part_wave(LvlArgs a, const LvlNode* __restrict__ nodes, int64_t na,
+                                                       const int32_t* const* perm, int32_t* const* perm2,
+                                                       const uint16_t* __restrict__ mask16,
+                                                       const PInfo* __restrict__ pinfo, uint8_t* split,
+                                                       uint32_t* nchild, uint32_t* leftsz) {
+    __shared__ int32_t s_pt[4][kWaveStage];
+    __shared__ uint8_t s_side[4][kWaveStage];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int32_t* const spt = s_pt[w];
+    uint8_t* const ssd = s_side[w];
+    if (blockIdx.x == 0 && threadIdx.x == 0) nchild[na] = 0;  // the scan's total
+    for (int64_t e = (int64_t)blockIdx.x * 4 + w; e < na; e += (int64_t)gridDim.x * 4) {  // wave-uniform
+        const LvlNode nd = nodes[e];
+        const int t = nd.tree;
+        const int m = nd.end - nd.start;
+        if (m > kWaveStage) continue;  // k_lvl_part_entry's
+        const int32_t* P = perm[t] + nd.start;
+        int32_t* P2 = perm2[t] + nd.start;
+        if (!split[e]) {
+            for (int i = lane; i < m; i += 64) P2[i] = P[i];
+            if (lane == 0) nchild[e] = 0;
+            continue;
+        }
+        uint32_t cnt = 0;
+        if (SPARSE) {  // a lane per member, k_lvl_margins_pos's arithmetic
+            const PInfo pi = pinfo[e];
+            const uint16_t* hb = a.hyper_bf[t] + (int64_t)pi.node * a.dim;
+            const uint64_t key = a.keys[t];
+            const int q = lane & 3, g = lane >> 2;
+            for (int i0 = 0; i0 < m; i0 += 16) {
+                const int i = i0 + g;
+                const int32_t pt = i < m ? P[i] : 0;
+                const float acc = margin_pos_quad(a.X + (int64_t)pt * a.ld, hb, a.dim, i < m, q);
+                if (i < m && q == 0) {
+                    const uint8_t sd = (uint8_t)side_of(acc + pi.off, key, pi.node, pt);
+                    spt[i] = pt;
+                    ssd[i] = sd;
+                    cnt += sd;
+                }
+            }
+        } else {
+            const uint16_t* mk = mask16 + (int64_t)(t >> 4) * a.n;
+            const int sh = t & 15;
+            for (int i = lane; i < m; i += 64) {
+                const int32_t pt = P[i];
+                const uint8_t sd = (uint8_t)((mk[pt] >> sh) & 1u);
+                spt[i] = pt;
+                ssd[i] = sd;
+                cnt += sd;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        const uint32_t L = cnt;
+        __threadfence_block();
+        wave_sync();
+        if (L == 0 || L == (uint32_t)m) {  // wave-uniform: degenerate split -> leaf with a zero hyperplane
+            if (lane == 0) {
+                split[e] = 0;
+                nchild[e] = 0;
+                a.children[t][2 * nd.node] = -nd.start;
+                a.children[t][2 * nd.node + 1] = -nd.end;
+                a.offsets[t][nd.node] = 0.f;
+            }
+            float* hp = a.hyper[t] + (int64_t)nd.node * a.dim;
+            for (int i = lane; i < a.dim; i += 64) hp[i] = 0.f;
+            for (int i = lane; i < m; i += 64) P2[i] = spt[i];
+            wave_sync();
+            continue;
+        }
+        if (lane == 0) {
+            nchild[e] = 2;
+            leftsz[e] = L;
+        }
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t lb = 0, rb = 0;
+        for (int i0 = 0; i0 < m; i0 += 64) {
+            const int i = i0 + lane;
+            const bool ok = i < m;
+            const uint32_t sd = ok ? ssd[i] : 0u;
+            const int32_t pt = ok ? spt[i] : 0;
+            const uint64_t lm = __ballot(ok && sd), rm = __ballot(ok && !sd);
+            if (ok) {
+                if (sd) P2[lb + (uint32_t)__popcll(lm & below)] = pt;
+                else P2[L + rb + (uint32_t)__popcll(rm & below)] = pt;
+            }
+            lb += (uint32_t)__popcll(lm);
+            rb += (uint32_t)__popcll(rm);
+        }
+        wave_sync();  // the staged arrays are rewritten by the next entry
+    }
+}
+
 __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t* const* act, int32_t* nodeOf) {
     const int64_t total = (int64_t)n_trees * n;
     for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
@@ -1015,14 +1292,22 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     const bool point_major_dpp = dim % 8 == 0 && dim <= 1024;
     const bool mfma_levels = mfma_levels_ok(dim, T);  // shallow dense levels: planes in LDS, MFMA side tests
     bool sparse = false;
+    // Entry-parallel partition (k_lvl_part_wave / k_lvl_part_entry) from the first level whose entries average
+    // at most kEntryItems of the T x n positions on (below that the chunked scatter over all positions is
+    // faster): the per-position entry tables (act) are not kept from then on.
+    constexpr int64_t kEntryItems = 1024;
+    bool entry_mode = false;
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
         hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur, na,
                            (const int32_t* const*)P, split.p, pinfo.p, cinfo.p, active_pairs);
-        margins_timer.begin(s);
+        entry_mode = entry_mode || na * kEntryItems >= total;
+        const bool fused_sides = entry_mode && sparse;  // k_lvl_part_entry tests the sides itself
+        if (!fused_sides) margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
         const CInfo* cinfo_prev = level > 0 ? cinfo.p : nullptr;
-        if (sparse)
+        if (fused_sides) {
+        } else if (sparse)
             hipLaunchKernelGGL(k_lvl_margins_pos, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, a, pinfo.p, T,
                                (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags8.p);
         else if (mfma_levels && na <= kMfmaEntries)
@@ -1041,18 +1326,35 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         else
             hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, cinfo_prev, nodeOf.p,
                                mask16.p);
-        margins_timer.stop(s);
+        if (!fused_sides) margins_timer.stop(s);
         const unsigned cgrid = lgrid(nchunks, 4, 1 << 16);  // four chunks (waves) per workgroup
-        if (vec4)
-            hipLaunchKernelGGL(k_lvl_flag_chunks<true>, dim3(cgrid), dim3(256), 0, s, T, n,
-                               sparse ? nullptr : mask16.p, (const int32_t* const*)P, flags8.p, chunk_cnt.p);
-        else
-            hipLaunchKernelGGL(k_lvl_flag_chunks<false>, dim3(cgrid), dim3(256), 0, s, T, n,
-                               sparse ? nullptr : mask16.p, (const int32_t* const*)P, flags8.p, chunk_cnt.p);
-        size_t b = scan_bytes;
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, chunk_cnt.p, chunk_off.p, (int)(nchunks + 1), s));
-        hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, chunk_off.p, flags8.p,
-                           split.p, nchild.p, leftsz.p, lstart.p);
+        if (entry_mode) {
+            const unsigned wg = lgrid(na, 4, 1 << 16), eg = lgrid(na, 1, 2048);
+            if (sparse) {
+                hipLaunchKernelGGL(k_lvl_part_wave<true>, dim3(wg), dim3(256), 0, s, a, cur, na,
+                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p);
+                hipLaunchKernelGGL(k_lvl_part_entry<true>, dim3(eg), dim3(256), 0, s, a, cur, na,
+                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p,
+                                   flags8.p);
+            } else {
+                hipLaunchKernelGGL(k_lvl_part_wave<false>, dim3(wg), dim3(256), 0, s, a, cur, na,
+                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p);
+                hipLaunchKernelGGL(k_lvl_part_entry<false>, dim3(eg), dim3(256), 0, s, a, cur, na,
+                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p,
+                                   flags8.p);
+            }
+        } else {
+            if (vec4)
+                hipLaunchKernelGGL(k_lvl_flag_chunks<true>, dim3(cgrid), dim3(256), 0, s, T, n,
+                                   sparse ? nullptr : mask16.p, (const int32_t* const*)P, flags8.p, chunk_cnt.p);
+            else
+                hipLaunchKernelGGL(k_lvl_flag_chunks<false>, dim3(cgrid), dim3(256), 0, s, T, n,
+                                   sparse ? nullptr : mask16.p, (const int32_t* const*)P, flags8.p, chunk_cnt.p);
+            size_t b = scan_bytes;
+            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, b, chunk_cnt.p, chunk_off.p, (int)(nchunks + 1), s));
+            hipLaunchKernelGGL(k_lvl_counts, dim3(lgrid(na, 256)), dim3(256), 0, s, a, cur, na, chunk_off.p,
+                               flags8.p, split.p, nchild.p, leftsz.p, lstart.p);
+        }
         size_t cb = cscan_bytes;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cscan_tmp.p, cb, nchild.p, cscan.p, (int)(na + 1), s));
         hipLaunchKernelGGL(k_lvl_tree_base, dim3(1), dim3(64), 0, s, T, st_cur, cscan.p, tree_children, d_count.p,
@@ -1062,7 +1364,8 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         // children ids, the next active list and the partition need nothing from the host
         hipLaunchKernelGGL(k_lvl_assign, dim3(lgrid(na, 256)), dim3(256), 0, s, cur, na, split.p, cscan.p,
                            leftsz.p, d_base.p, st_next, d_ch.p, nxt, cinfo.p);
-        if (vec4)
+        if (entry_mode) {
+        } else if (vec4)
             hipLaunchKernelGGL(k_lvl_scatter<true>, dim3(cgrid), dim3(256), 0, s, cur, T, n, split.p, cscan.p,
                                leftsz.p, lstart.p, chunk_off.p, flags8.p, (const int32_t* const*)P, P + T, P + 2 * T,
                                P + 3 * T);
@@ -1080,8 +1383,9 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
             // (tree, point) its node and position reads and its flag write
             const double tested = (double)(pairs_now - pairs_before);
             pairs_before = pairs_now;
-            margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim * 4.0 +
-                                  tested * 12.0);
+            if (!fused_sides)
+                margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim * 4.0 +
+                                      tested * 12.0);
             sparse_next = sparse || (dim % 4 == 0 && tested <= (double)(kSparsePairs * n));
         }
         int64_t next_na = 0;
