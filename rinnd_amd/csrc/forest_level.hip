@@ -24,8 +24,8 @@
 // sparse levels test position-major and write position-ordered flags.  Once the active entries average at
 // most 1,024 of the tree x point positions, flag chunks, their scan, k_lvl_counts and k_lvl_scatter give way
 // to one pass per entry (k_lvl_part_wave: a wave per entry of at most 1,024 members; k_lvl_part_entry: a
-// workgroup per larger one) that reads the entry's flags (sparse levels: tests its sides with the arithmetic
-// of k_lvl_margins_pos), counts them and partitions the entry's positions in the same stable order.
+// workgroup per larger one) that reads the entry's flags, counts them and partitions the entry's positions
+// in the same stable order.
 // Node ids are breadth-first per tree (root 0); leaves hold their [start, end) range of
 // tree.indices exactly as in the depth-first layout.
 #include <hipcub/hipcub.hpp>
@@ -561,8 +561,11 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
 // Sparse levels, position-major: a wave reads 64 consecutive (tree, position) items of the act table,
 // and its four 16-lane groups test the active ones four at a time: the member's row is gathered through
 // perm and the 16 lanes split the dimensions; flags are written at the positions.  Needs dim % 4 == 0.
+// cscan_prev (after an entry-parallel level): act holds 2 e + side of the position's entry e one level up,
+// whose children are entries cscan_prev[e] (left) and cscan_prev[e] + 1.
 __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo* __restrict__ pinfo, int n_trees,
                                                          const int32_t* const* act, const int32_t* const* perm,
+                                                         const uint32_t* __restrict__ cscan_prev,
                                                          uint8_t* __restrict__ flags8) {
     const int lane = threadIdx.x & 63, gl = lane & 15, g = lane >> 4;
     const int64_t total = (int64_t)n_trees * a.n;
@@ -575,7 +578,8 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
         if (item < total) {
             t = (int32_t)(item / a.n);
             pos = (int32_t)(item - (int64_t)t * a.n);
-            const int32_t e = act[t][pos];
+            int32_t e = act[t][pos];
+            if (e >= 0 && cscan_prev) e = (int32_t)cscan_prev[e >> 1] + (e & 1);  // parent entry, side
             if (e >= 0) {
                 const PInfo pi = pinfo[e];
                 node = pi.node;
@@ -859,12 +863,13 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(const LvlNode* nodes, int n
 
 // ---- entry-parallel partition (the levels with many active entries) -------------------------------------
 // One workgroup per active entry: the entry's left flags (dense levels: the bit the side tests left in each
-// member's mask word; sparse levels: the side test itself, with the arithmetic of k_lvl_margins_pos), their
-// count (a degenerate split becomes a leaf, as in k_lvl_counts), then the stable partition of the entry's
-// positions into the other buffer - the same order k_lvl_scatter gives, without the per-position entry table,
-// the flag chunks or their scan.  An entry that does not split copies its positions (both buffers agree from
-// then on).  Entries of at most kEntryStage members keep their ids and flags in LDS between the passes; larger
-// ones keep the flags in flags8.
+// member's mask word; sparse levels: the flags k_lvl_margins_pos wrote at the positions), their count (a
+// degenerate split becomes a leaf, as in k_lvl_counts), then the stable partition of the entry's positions
+// into the other buffer - the same order k_lvl_scatter gives, without the flag chunks or their scan.  Moved
+// positions get 2 e + side in the other act buffer (their next entry is cscan[e] + side, known once the
+// children are scanned); an entry that does not split copies its positions and marks them settled (-1) in
+// both act buffers.  Entries of at most kEntryStage members keep their ids and flags in LDS between the
+// passes; larger ones read the flags from flags8 (dense levels write them there first).
 constexpr int kEntryStage = 4096;
 constexpr int kWaveStage = 1024;  // entries of at most this many members: one wave each (k_lvl_part_wave)
 
@@ -893,38 +898,11 @@ __device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* red, ui
     return before + incl - v;
 }
 
-// k_lvl_margins_pos's margin by a group of four lanes, added in the order of its xor butterfly (offsets 8, 4,
-// 2, 1) over its 16 lanes' partial sums (lane l: dimensions 4 l .. 4 l + 3, then every 64 further): q = lane % 4 takes the partial sums q, q + 4, q + 8, q + 12; rows
-// are read 64 contiguous bytes per group and step); valid in lane q == 0.  Every lane of the wave calls it.
-__device__ __forceinline__ float margin_pos_quad(const float* __restrict__ x, const uint16_t* __restrict__ hb,
-                                                 int dim, bool ok, int q) {
-    float v[4] = {0.f, 0.f, 0.f, 0.f};
-    if (ok) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int gl = q + 4 * r;
-            float acc = 0.f;
-            for (int d0 = 4 * gl; d0 < dim; d0 += 64) {
-                const float4 xv = *reinterpret_cast<const float4*>(x + d0);
-                const uint2 hv = *reinterpret_cast<const uint2*>(hb + d0);
-                acc = fmaf(xv.x, bf16_lo(hv.x), acc);
-                acc = fmaf(xv.y, bf16_hi(hv.x), acc);
-                acc = fmaf(xv.z, bf16_lo(hv.y), acc);
-                acc = fmaf(xv.w, bf16_hi(hv.y), acc);
-            }
-            v[r] = acc;
-        }
-    }
-    const float b = (v[0] + v[2]) + (v[1] + v[3]);  // (a_q = v_q + v_q+8) + (a_q+4 = v_q+4 + v_q+12)
-    const float c = b + __shfl_xor(b, 2);
-    return c + __shfl_xor(c, 1);
-}
-
-template <bool SPARSE>
+template <bool FLAGS>
 __global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode* __restrict__ nodes, int64_t na,
                                                         const int32_t* const* perm, int32_t* const* perm2,
-                                                        const uint16_t* __restrict__ mask16,
-                                                        const PInfo* __restrict__ pinfo, uint8_t* split,
+                                                        int32_t* const* act, int32_t* const* act2,
+                                                        const uint16_t* __restrict__ mask16, uint8_t* split,
                                                         uint32_t* nchild, uint32_t* leftsz, uint8_t* flags8) {
     __shared__ int32_t s_pt[kEntryStage];
     __shared__ uint8_t s_side[kEntryStage];
@@ -938,48 +916,32 @@ __global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode
         if (m <= kWaveStage) continue;  // uniform: k_lvl_part_wave's
         const int32_t* P = perm[t] + nd.start;
         int32_t* P2 = perm2[t] + nd.start;
-        if (!split[e]) {  // uniform: a leaf since k_lvl_planes
-            for (int i = tid; i < m; i += 256) P2[i] = P[i];
+        int32_t* A = act[t] + nd.start;
+        int32_t* A2 = act2[t] + nd.start;
+        if (!split[e]) {  // uniform: a leaf since k_lvl_planes; settled in both buffers
+            for (int i = tid; i < m; i += 256) {
+                P2[i] = P[i];
+                A[i] = -1;
+                A2[i] = -1;
+            }
             if (tid == 0) nchild[e] = 0;
             continue;
         }
         const bool staged = m <= kEntryStage;
         uint8_t* F = flags8 + (int64_t)t * a.n + nd.start;
         uint32_t cnt = 0;
-        if (SPARSE) {  // a lane per member, k_lvl_margins_pos's arithmetic
-            const PInfo pi = pinfo[e];
-            const uint16_t* hb = a.hyper_bf[t] + (int64_t)pi.node * a.dim;
-            const uint64_t key = a.keys[t];
-            const int q = tid & 3, g = tid >> 2;
-            for (int i0 = 0; i0 < m; i0 += 64) {
-                const int i = i0 + g;
-                const int32_t pt = i < m ? P[i] : 0;
-                const float acc = margin_pos_quad(a.X + (int64_t)pt * a.ld, hb, a.dim, i < m, q);
-                if (i < m && q == 0) {
-                    const uint8_t sd = (uint8_t)side_of(acc + pi.off, key, pi.node, pt);
-                    if (staged) {
-                        s_pt[i] = pt;
-                        s_side[i] = sd;
-                    } else {
-                        F[i] = sd;
-                    }
-                    cnt += sd;
-                }
+        const uint16_t* mk = mask16 + (int64_t)(t >> 4) * a.n;
+        const int sh = t & 15;
+        for (int i = tid; i < m; i += 256) {
+            const int32_t pt = P[i];
+            const uint8_t sd = FLAGS ? F[i] : (uint8_t)((mk[pt] >> sh) & 1u);
+            if (staged) {
+                s_pt[i] = pt;
+                s_side[i] = sd;
+            } else if (!FLAGS) {
+                F[i] = sd;
             }
-        } else {
-            const uint16_t* mk = mask16 + (int64_t)(t >> 4) * a.n;
-            const int sh = t & 15;
-            for (int i = tid; i < m; i += 256) {
-                const int32_t pt = P[i];
-                const uint8_t sd = (uint8_t)((mk[pt] >> sh) & 1u);
-                if (staged) {
-                    s_pt[i] = pt;
-                    s_side[i] = sd;
-                } else {
-                    F[i] = sd;
-                }
-                cnt += sd;
-            }
+            cnt += sd;
         }
         const uint32_t L = block_sum_u32(cnt, red);  // also orders pass 1's writes before pass 2's reads
         if (L == 0 || L == (uint32_t)m) {  // uniform: degenerate split -> leaf with a zero hyperplane
@@ -992,7 +954,11 @@ __global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode
             }
             float* hp = a.hyper[t] + (int64_t)nd.node * a.dim;
             for (int i = tid; i < a.dim; i += 256) hp[i] = 0.f;
-            for (int i = tid; i < m; i += 256) P2[i] = P[i];
+            for (int i = tid; i < m; i += 256) {
+                P2[i] = P[i];
+                A[i] = -1;
+                A2[i] = -1;
+            }
             __syncthreads();  // the staged arrays are rewritten by the next entry
             continue;
         }
@@ -1000,6 +966,7 @@ __global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode
             nchild[e] = 2;
             leftsz[e] = L;
         }
+        const int32_t code = (int32_t)(2 * e);  // the next level's entry: 2 e + side (left 0, right 1)
         uint32_t lb = 0, rb = 0;  // left / right members placed by the earlier tiles
         for (int base = 0; base < m; base += 1024) {
             int32_t pv[4];
@@ -1024,8 +991,13 @@ __global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (i0 + k >= m) break;
-                if (f[k]) P2[lp++] = pv[k];
-                else P2[L + rp++] = pv[k];
+                if (f[k]) {
+                    P2[lp] = pv[k];
+                    A2[lp++] = code;
+                } else {
+                    P2[L + rp] = pv[k];
+                    A2[L + rp++] = code + 1;
+                }
             }
             const uint32_t tile_items = (uint32_t)min(1024, m - base);
             lb += tileL;
@@ -1037,11 +1009,12 @@ __global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode
 
 // The same for the entries of at most kWaveStage members, one wave each (four per workgroup): no workgroup
 // barriers; the stable partition takes 64 members a step, ranks from ballots.
-template <bool SPARSE>
+template <bool FLAGS>
 __global__ __launch_bounds__(256) void k_lvl_part_wave(LvlArgs a, const LvlNode* __restrict__ nodes, int64_t na,
                                                        const int32_t* const* perm, int32_t* const* perm2,
+                                                       int32_t* const* act, int32_t* const* act2,
                                                        const uint16_t* __restrict__ mask16,
-                                                       const PInfo* __restrict__ pinfo, uint8_t* split,
+                                                       const uint8_t* __restrict__ flags8, uint8_t* split,
                                                        uint32_t* nchild, uint32_t* leftsz) {
     __shared__ int32_t s_pt[4][kWaveStage];
     __shared__ uint8_t s_side[4][kWaveStage];
@@ -1056,38 +1029,27 @@ __global__ __launch_bounds__(256) void k_lvl_part_wave(LvlArgs a, const LvlNode*
         if (m > kWaveStage) continue;  // k_lvl_part_entry's
         const int32_t* P = perm[t] + nd.start;
         int32_t* P2 = perm2[t] + nd.start;
-        if (!split[e]) {
-            for (int i = lane; i < m; i += 64) P2[i] = P[i];
+        int32_t* A = act[t] + nd.start;
+        int32_t* A2 = act2[t] + nd.start;
+        if (!split[e]) {  // a leaf since k_lvl_planes; settled in both buffers
+            for (int i = lane; i < m; i += 64) {
+                P2[i] = P[i];
+                A[i] = -1;
+                A2[i] = -1;
+            }
             if (lane == 0) nchild[e] = 0;
             continue;
         }
+        const uint8_t* F = flags8 + (int64_t)t * a.n + nd.start;
+        const uint16_t* mk = mask16 + (int64_t)(t >> 4) * a.n;
+        const int sh = t & 15;
         uint32_t cnt = 0;
-        if (SPARSE) {  // a lane per member, k_lvl_margins_pos's arithmetic
-            const PInfo pi = pinfo[e];
-            const uint16_t* hb = a.hyper_bf[t] + (int64_t)pi.node * a.dim;
-            const uint64_t key = a.keys[t];
-            const int q = lane & 3, g = lane >> 2;
-            for (int i0 = 0; i0 < m; i0 += 16) {
-                const int i = i0 + g;
-                const int32_t pt = i < m ? P[i] : 0;
-                const float acc = margin_pos_quad(a.X + (int64_t)pt * a.ld, hb, a.dim, i < m, q);
-                if (i < m && q == 0) {
-                    const uint8_t sd = (uint8_t)side_of(acc + pi.off, key, pi.node, pt);
-                    spt[i] = pt;
-                    ssd[i] = sd;
-                    cnt += sd;
-                }
-            }
-        } else {
-            const uint16_t* mk = mask16 + (int64_t)(t >> 4) * a.n;
-            const int sh = t & 15;
-            for (int i = lane; i < m; i += 64) {
-                const int32_t pt = P[i];
-                const uint8_t sd = (uint8_t)((mk[pt] >> sh) & 1u);
-                spt[i] = pt;
-                ssd[i] = sd;
-                cnt += sd;
-            }
+        for (int i = lane; i < m; i += 64) {
+            const int32_t pt = P[i];
+            const uint8_t sd = FLAGS ? F[i] : (uint8_t)((mk[pt] >> sh) & 1u);
+            spt[i] = pt;
+            ssd[i] = sd;
+            cnt += sd;
         }
         for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
         const uint32_t L = cnt;
@@ -1103,7 +1065,11 @@ __global__ __launch_bounds__(256) void k_lvl_part_wave(LvlArgs a, const LvlNode*
             }
             float* hp = a.hyper[t] + (int64_t)nd.node * a.dim;
             for (int i = lane; i < a.dim; i += 64) hp[i] = 0.f;
-            for (int i = lane; i < m; i += 64) P2[i] = spt[i];
+            for (int i = lane; i < m; i += 64) {
+                P2[i] = spt[i];
+                A[i] = -1;
+                A2[i] = -1;
+            }
             wave_sync();
             continue;
         }
@@ -1111,6 +1077,7 @@ __global__ __launch_bounds__(256) void k_lvl_part_wave(LvlArgs a, const LvlNode*
             nchild[e] = 2;
             leftsz[e] = L;
         }
+        const int32_t code = (int32_t)(2 * e);  // the next level's entry: 2 e + side (left 0, right 1)
         const uint64_t below = (1ull << lane) - 1ull;
         uint32_t lb = 0, rb = 0;
         for (int i0 = 0; i0 < m; i0 += 64) {
@@ -1120,13 +1087,35 @@ __global__ __launch_bounds__(256) void k_lvl_part_wave(LvlArgs a, const LvlNode*
             const int32_t pt = ok ? spt[i] : 0;
             const uint64_t lm = __ballot(ok && sd), rm = __ballot(ok && !sd);
             if (ok) {
-                if (sd) P2[lb + (uint32_t)__popcll(lm & below)] = pt;
-                else P2[L + rb + (uint32_t)__popcll(rm & below)] = pt;
+                if (sd) {
+                    const uint32_t d = lb + (uint32_t)__popcll(lm & below);
+                    P2[d] = pt;
+                    A2[d] = code;
+                } else {
+                    const uint32_t d = L + rb + (uint32_t)__popcll(rm & below);
+                    P2[d] = pt;
+                    A2[d] = code + 1;
+                }
             }
             lb += (uint32_t)__popcll(lm);
             rb += (uint32_t)__popcll(rm);
         }
         wave_sync();  // the staged arrays are rewritten by the next entry
+    }
+}
+
+// The first entry-parallel level: positions that became leaves one level earlier (act -2, see k_lvl_scatter)
+// are settled in both buffers, since the entry-parallel levels never visit them.
+__global__ void k_lvl_act_settle(int n_trees, int64_t n, int32_t* const* act, int32_t* const* act2) {
+    const int64_t total = (int64_t)n_trees * n;
+    for (int64_t item = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; item < total;
+         item += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(item / n);
+        const int64_t pos = item - (int64_t)t * n;
+        if (act[t][pos] == -2) {
+            act[t][pos] = -1;
+            act2[t][pos] = -1;
+        }
     }
 }
 
@@ -1294,22 +1283,25 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     bool sparse = false;
     // Entry-parallel partition (k_lvl_part_wave / k_lvl_part_entry) from the first level whose entries average
     // at most kEntryItems of the T x n positions on (below that the chunked scatter over all positions is
-    // faster): the per-position entry tables (act) are not kept from then on.
+    // faster); act then holds each position's 2 e + side relative to the last level's entries.
     constexpr int64_t kEntryItems = 1024;
     bool entry_mode = false;
     while (na > 0) {
         int32_t* const* P = d_tab.p + (size_t)phase * 4 * T;  // perm, perm2, act, act2
         hipLaunchKernelGGL(k_lvl_planes, dim3(lgrid(na, 1, 4096)), dim3(256), 0, s, a, cur, na,
                            (const int32_t* const*)P, split.p, pinfo.p, cinfo.p, active_pairs);
+        const bool prev_entry = entry_mode;  // act holds 2 e + side codes of the last level's entries
         entry_mode = entry_mode || na * kEntryItems >= total;
-        const bool fused_sides = entry_mode && sparse;  // k_lvl_part_entry tests the sides itself
-        if (!fused_sides) margins_timer.begin(s);
+        if (entry_mode && !prev_entry)
+            hipLaunchKernelGGL(k_lvl_act_settle, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, T, n, P + 2 * T,
+                               P + 3 * T);
+        margins_timer.begin(s);
         const unsigned mg = lgrid(n * 16, 256, 1 << 16);
         const CInfo* cinfo_prev = level > 0 ? cinfo.p : nullptr;
-        if (fused_sides) {
-        } else if (sparse)
+        if (sparse)
             hipLaunchKernelGGL(k_lvl_margins_pos, dim3(lgrid(total, 256, 1 << 16)), dim3(256), 0, s, a, pinfo.p, T,
-                               (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P, flags8.p);
+                               (const int32_t* const*)(P + 2 * T), (const int32_t* const*)P,
+                               prev_entry ? cscan.p : nullptr, flags8.p);
         else if (mfma_levels && na <= kMfmaEntries)
             launch_margins_mfma(s, a, cur, (int)na, pinfo.p, T, cinfo_prev, nodeOf.p, mask16.p);
         else if (point_major_dpp)
@@ -1326,22 +1318,24 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         else
             hipLaunchKernelGGL(k_lvl_margins<0>, dim3(mg), dim3(256), 0, s, a, pinfo.p, T, cinfo_prev, nodeOf.p,
                                mask16.p);
-        if (!fused_sides) margins_timer.stop(s);
+        margins_timer.stop(s);
         const unsigned cgrid = lgrid(nchunks, 4, 1 << 16);  // four chunks (waves) per workgroup
         if (entry_mode) {
             const unsigned wg = lgrid(na, 4, 1 << 16), eg = lgrid(na, 1, 2048);
-            if (sparse) {
+            if (sparse) {  // flags at the positions, from k_lvl_margins_pos
                 hipLaunchKernelGGL(k_lvl_part_wave<true>, dim3(wg), dim3(256), 0, s, a, cur, na,
-                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p);
+                                   (const int32_t* const*)P, P + T, P + 2 * T, P + 3 * T, mask16.p, flags8.p, split.p,
+                                   nchild.p, leftsz.p);
                 hipLaunchKernelGGL(k_lvl_part_entry<true>, dim3(eg), dim3(256), 0, s, a, cur, na,
-                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p,
-                                   flags8.p);
-            } else {
+                                   (const int32_t* const*)P, P + T, P + 2 * T, P + 3 * T, mask16.p, split.p, nchild.p,
+                                   leftsz.p, flags8.p);
+            } else {  // flags in the members' mask words
                 hipLaunchKernelGGL(k_lvl_part_wave<false>, dim3(wg), dim3(256), 0, s, a, cur, na,
-                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p);
+                                   (const int32_t* const*)P, P + T, P + 2 * T, P + 3 * T, mask16.p, flags8.p, split.p,
+                                   nchild.p, leftsz.p);
                 hipLaunchKernelGGL(k_lvl_part_entry<false>, dim3(eg), dim3(256), 0, s, a, cur, na,
-                                   (const int32_t* const*)P, P + T, mask16.p, pinfo.p, split.p, nchild.p, leftsz.p,
-                                   flags8.p);
+                                   (const int32_t* const*)P, P + T, P + 2 * T, P + 3 * T, mask16.p, split.p, nchild.p,
+                                   leftsz.p, flags8.p);
             }
         } else {
             if (vec4)
@@ -1383,9 +1377,8 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
             // (tree, point) its node and position reads and its flag write
             const double tested = (double)(pairs_now - pairs_before);
             pairs_before = pairs_now;
-            if (!fused_sides)
-                margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim * 4.0 +
-                                      tested * 12.0);
+            margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim * 4.0 +
+                                  tested * 12.0);
             sparse_next = sparse || (dim % 4 == 0 && tested <= (double)(kSparsePairs * n));
         }
         int64_t next_na = 0;
