@@ -213,7 +213,9 @@ static void build_forest_dev(rinnd_ctx* ctx, const float* DX, int64_t Dld, int64
         // Several workgroups per tree when every tree's group fits on the device at once: the walking workgroup
         // hands its large nodes' side tests and partitions to the group.
         const int G = forest_coop_groups(dim, (int)n_trees);
-        const int64_t coop_min = std::max<int64_t>(2048, 128 * (int64_t)G);
+        // a node joins the group from 384 members per workgroup on (below, the group's barriers cost more than the
+        // walking workgroup's side tests and partition: measured at 1M x 128, 16 trees)
+        const int64_t coop_min = std::max<int64_t>(4096, 384 * (int64_t)G);
         DBuf<CoopCtl> coop_ctl;
         DBuf<float> coop_hp;
         if (G >= 2 && n >= 2 * coop_min) {

@@ -932,16 +932,32 @@ __global__ __launch_bounds__(256) void k_lvl_part_entry(LvlArgs a, const LvlNode
         uint32_t cnt = 0;
         const uint16_t* mk = mask16 + (int64_t)(t >> 4) * a.n;
         const int sh = t & 15;
-        for (int i = tid; i < m; i += 256) {
-            const int32_t pt = P[i];
-            const uint8_t sd = FLAGS ? F[i] : (uint8_t)((mk[pt] >> sh) & 1u);
-            if (staged) {
-                s_pt[i] = pt;
-                s_side[i] = sd;
-            } else if (!FLAGS) {
-                F[i] = sd;
+        for (int i0 = tid; i0 < m; i0 += 1024) {  // four members per thread in flight
+            int32_t pt[4];
+            uint8_t sd[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + 256 * k;
+                pt[k] = i < m ? P[i] : 0;
             }
-            cnt += sd;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + 256 * k;
+                sd[k] = i < m ? (FLAGS ? F[i] : (uint8_t)((mk[pt[k]] >> sh) & 1u)) : (uint8_t)0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + 256 * k;
+                if (i < m) {
+                    if (staged) {
+                        s_pt[i] = pt[k];
+                        s_side[i] = sd[k];
+                    } else if (!FLAGS) {
+                        F[i] = sd[k];
+                    }
+                }
+                cnt += sd[k];
+            }
         }
         const uint32_t L = block_sum_u32(cnt, red);  // also orders pass 1's writes before pass 2's reads
         if (L == 0 || L == (uint32_t)m) {  // uniform: degenerate split -> leaf with a zero hyperplane
