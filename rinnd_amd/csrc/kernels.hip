@@ -2922,9 +2922,56 @@ __global__ __launch_bounds__(256) void k_sort_rows(int32_t* idx, float* dist, ui
         wave_sync();
     }
 }
+// The same for rows of at most 64 entries, in registers: S (k rounded up to a power of two) lanes per row,
+// 64 / S rows per wave, the bitonic network on (distance order key, slot) by lane exchanges; each lane then
+// takes the entry its rank names from the lane that holds it.
+__global__ __launch_bounds__(256) void k_sort_rows_reg(int32_t* idx, float* dist, uint8_t* flags, int64_t n, int k,
+                                                       int S) {
+    const int lane = threadIdx.x & 63;
+    const int G = 64 / S, j = lane % S, base = lane - j;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t row0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * G; row0 < n;
+         row0 += waves * G) {  // wave-uniform
+        const int64_t row = row0 + lane / S;
+        const bool ok = row < n && j < k;
+        int32_t vi = 0;
+        float vd = 0.f;
+        uint32_t vf = 0u;
+        uint64_t key = ~0ull;
+        if (ok) {
+            vi = idx[row * k + j];
+            vd = dist[row * k + j];
+            vf = flags[row * k + j];
+            key = ((uint64_t)float_order_key(vd) << 32) | (uint32_t)j;
+        }
+        for (int kk = 2; kk <= S; kk <<= 1)
+            for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+                const uint32_t lo = __shfl_xor((uint32_t)key, jj), hi = __shfl_xor((uint32_t)(key >> 32), jj);
+                const uint64_t other = ((uint64_t)hi << 32) | lo;
+                const bool take_min = ((j & jj) == 0) == ((j & kk) == 0);
+                key = take_min ? (other < key ? other : key) : (other > key ? other : key);
+            }
+        const int src = base + (int)(key & 0x3fu);  // rank j's entry sits in lane base + its slot
+        const int32_t oi = __shfl(vi, src);
+        const float od = __shfl(vd, src);
+        const uint32_t of = __shfl(vf, src);
+        if (ok) {
+            idx[row * k + j] = oi;
+            dist[row * k + j] = od;
+            flags[row * k + j] = (uint8_t)of;
+        }
+    }
+}
+
 void launch_sort_rows(hipStream_t s, int32_t* idx, float* dist, uint8_t* flags, int64_t n, int k) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_sort_rows, dim3(grid_for(n, 4, 1 << 16)), dim3(256), 0, s, idx, dist, flags, n, k);
+    int S = 1;
+    while (S < k) S <<= 1;
+    if (S <= 64)
+        hipLaunchKernelGGL(k_sort_rows_reg, dim3(grid_for(ceil_div(n, (int64_t)(64 / S)), 4, 1 << 16)), dim3(256), 0, s,
+                           idx, dist, flags, n, k, S);
+    else
+        hipLaunchKernelGGL(k_sort_rows, dim3(grid_for(n, 4, 1 << 16)), dim3(256), 0, s, idx, dist, flags, n, k);
 }
 
 // distance corrections (core/index.rs:669-673)

@@ -316,9 +316,10 @@ def test_counter_mode_quality(ctx):
     assert np.all(gi != np.arange(len(X))[:, None])
 
 
-def test_sort_rows_matches_sort_all(ctx):
-    rng = np.random.default_rng(0)
-    n, k = 500, 23
+@pytest.mark.parametrize("k", [1, 15, 23, 64, 100])  # one row per wave (S > 64) and 64 / S rows per wave
+def test_sort_rows_matches_sort_all(ctx, k):
+    rng = np.random.default_rng(k)
+    n = 501
     idx = rng.integers(-1, 1000, (n, k)).astype(np.int32)
     dist = rng.integers(0, 6, (n, k)).astype(np.float32)  # many ties: stability matters
     dist[idx < 0] = np.inf
