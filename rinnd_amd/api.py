@@ -235,8 +235,9 @@ class RINND:
         self._ctx.check(st, "rinnd_index_query")
         return oi, od
 
-    def query_quantized(self, query_data, k=10, epsilon=0.1, candidate_width=0, rerank_width=0):
-        """Quantized query with explicit widths (py:825-870, validate_quantized_widths py:1148-1172)."""
+    @staticmethod
+    def _validate_widths(k, candidate_width, rerank_width):
+        """validate_quantized_widths (py:1148-1172)."""
         if candidate_width != 0:
             if candidate_width < k:
                 raise ValueError(f"candidate_width {candidate_width} must be at least k ({k})")
@@ -244,6 +245,10 @@ class RINND:
                 raise ValueError(f"rerank_width {rerank_width} must not exceed candidate_width {candidate_width}")
             if 0 < rerank_width < k:
                 raise ValueError(f"rerank_width {rerank_width} must be zero or at least k ({k})")
+
+    def query_quantized(self, query_data, k=10, epsilon=0.1, candidate_width=0, rerank_width=0):
+        """Quantized query with explicit widths (py:825-870)."""
+        self._validate_widths(k, candidate_width, rerank_width)
         q = self._prep_queries(query_data)
         if self._quantized is None:
             raise ValueError("query-time quantized widths require a quantized cosine_distance_mode")
@@ -267,6 +272,19 @@ class RINND:
 
     def query_one_indices(self, query, k=10, epsilon=0.1):
         return self.query_one(query, k, epsilon)[0]
+
+    def query_one_indices_quantized(self, query, k=10, epsilon=0.1, candidate_width=0, rerank_width=0):
+        """py:960-1011: the indices of one quantized query with explicit widths."""
+        self._validate_widths(k, candidate_width, rerank_width)
+        q = np.ascontiguousarray(query, np.float32)
+        if q.ndim != 1:
+            raise ValueError("query must be a 1-D array")
+        if q.shape[0] != self.dim:
+            raise ValueError(f"Query dimension {q.shape[0]} does not match data dimension {self.dim}")
+        if self._quantized is None:
+            raise ValueError("query-time quantized widths require a quantized cosine_distance_mode")
+        q2 = self._prep_queries(q[None, :], "query")
+        return self._query_i8(q2, k, epsilon, candidate_width, rerank_width)[0][0]
 
     # ---- graph / tree exports (py:1013-1025, 1087-1128) ----
     @property

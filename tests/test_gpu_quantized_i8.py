@@ -74,6 +74,22 @@ def test_constructor_widths_and_single_queries(indexed):
         np.testing.assert_array_equal(d.view(np.uint32), want[1][j].view(np.uint32))
 
 
+def test_query_one_indices_quantized(indexed):
+    """py:960-1011: the indices of query_one_quantized_widths for one query; width errors as query_quantized."""
+    mode, g, r, codes, Q = indexed
+    want = r.query_i8(codes, normalize_rows(Q[:3]), k=6, epsilon=0.1, candidate_width=18, rerank_width=12)[0]
+    for j in range(3):
+        np.testing.assert_array_equal(g.query_one_indices_quantized(Q[j], k=6, epsilon=0.1, candidate_width=18,
+                                                                    rerank_width=12), want[j])
+    with pytest.raises(ValueError, match="must be at least k"):
+        g.query_one_indices_quantized(Q[0], k=6, candidate_width=3)
+    with pytest.raises(ValueError, match="does not match data dimension"):
+        g.query_one_indices_quantized(Q[0, :5], k=6)
+    plain = RINND(data(500, 40, 1), n_neighbors=8, n_trees=2, n_iters=2)
+    with pytest.raises(ValueError, match="quantized cosine_distance_mode"):
+        plain.query_one_indices_quantized(Q[0], k=5)
+
+
 @pytest.mark.parametrize("mode", ["int8", "sq8p"])
 def test_int8_trees_with_int8_codes(mode):
     X = data(4000, 32, 5)
