@@ -391,9 +391,35 @@ __device__ int coop_node(const float* X, int64_t ld, int dim, const float* hp, f
     return split;
 }
 
+// Sequential sums 0 + a[0] + a[1] + ... (and the same of b) in that order, as the reference's scalar loops
+// add them, by one thread; the loads of eight terms go out before their additions.
+__device__ __forceinline__ void seq_sum2(const float* a, const float* b, int n, float& sa, float& sb) {
+    float x = 0.f, y = 0.f;
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        float va[8], vb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            va[u] = a[i + u];
+            vb[u] = b[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            x += va[u];
+            y += vb[u];
+        }
+    }
+    for (; i < n; ++i) {
+        x += a[i];
+        y += b[i];
+    }
+    sa = x;
+    sb = y;
+}
+
 template <int THREADS, bool WAVE_SUB, bool COOP = false>
 __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64_t so) {
-    extern __shared__ __align__(16) float fsm[];  // hp[dim], p1[dim], p2[dim]
+    extern __shared__ __align__(16) float fsm[];  // hp[dim], p1[dim], p2[dim], two product arrays q1[dim], q2[dim]
     __shared__ StackEnt stack[kForestStack];
     __shared__ int32_t wv_idx[kWaveSub], wv_tmp[kWaveSub], wv_rank[kWaveSub + 1], wv_P[kWaveSub], wv_Q[kWaveSub];
     __shared__ uint8_t wv_side[kWaveSub];
@@ -415,6 +441,8 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
     float* hp = fsm;
     float* p1s = fsm + dim;
     float* p2s = fsm + 2 * dim;
+    float* q1s = fsm + 3 * dim;  // per-dimension products, summed in order by one thread
+    float* q2s = fsm + 4 * dim;
     float* hyper = a.hyper[t];
     float* offs = a.offsets[t];
     int32_t* ch = a.children[t];
@@ -497,18 +525,18 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
             wave_sync();
             float off = 0.f;
             if (!a.angular) {  // make_euclidean_split (builder.rs:249-279)
-                for (int i = lane; i < dim; i += RINND_WAVE) hp[i] = p2s[i] - p1s[i];
+                for (int i = lane; i < dim; i += RINND_WAVE) {
+                    const float h = p2s[i] - p1s[i];
+                    hp[i] = h;
+                    q1s[i] = h * h;
+                    q2s[i] = h * ((p1s[i] + p2s[i]) * 0.5f);
+                }
                 wave_sync();
                 float inv = 0.f;
                 int scale = 0;
                 if (lane == 0) {
-                    float norm_sq = 0.f, dot_nm = 0.f;
-                    for (int i = 0; i < dim; ++i) {
-                        const float h = hp[i];
-                        const float mid = (p1s[i] + p2s[i]) * 0.5f;
-                        norm_sq += h * h;
-                        dot_nm += h * mid;
-                    }
+                    float norm_sq, dot_nm;
+                    seq_sum2(q1s, q2s, dim, norm_sq, dot_nm);
                     const float norm = sqrtf(norm_sq);
                     if (norm > 1e-8f) {
                         inv = 1.0f / norm;
@@ -523,9 +551,13 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
                     for (int i = lane; i < dim; i += RINND_WAVE) hp[i] *= inv;
             } else {  // make_angular_split (builder.rs:281-308)
                 float n1 = 0.f, n2 = 0.f;
+                for (int i = lane; i < dim; i += RINND_WAVE) {
+                    q1s[i] = p1s[i] * p1s[i];
+                    q2s[i] = p2s[i] * p2s[i];
+                }
+                wave_sync();
                 if (lane == 0) {
-                    for (int i = 0; i < dim; ++i) n1 += p1s[i] * p1s[i];
-                    for (int i = 0; i < dim; ++i) n2 += p2s[i] * p2s[i];
+                    seq_sum2(q1s, q2s, dim, n1, n2);
                     n1 = sqrtf(n1);
                     n2 = sqrtf(n2);
                 }
@@ -536,8 +568,11 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
                 wave_sync();
                 if (!deg) {
                     float nn = 0.f;
+                    for (int i = lane; i < dim; i += RINND_WAVE) q1s[i] = hp[i] * hp[i];
+                    wave_sync();
                     if (lane == 0) {
-                        for (int i = 0; i < dim; ++i) nn += hp[i] * hp[i];
+                        float unused;
+                        seq_sum2(q1s, q1s, dim, nn, unused);
                         nn = sqrtf(nn);
                     }
                     nn = __shfl(nn, 0);
@@ -720,16 +755,16 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
         }
         __syncthreads();
         if (!a.angular) {  // make_euclidean_split (builder.rs:249-279)
-            for (int i = tid; i < dim; i += blockDim.x) hp[i] = p2s[i] - p1s[i];
+            for (int i = tid; i < dim; i += blockDim.x) {
+                const float h = p2s[i] - p1s[i];
+                hp[i] = h;
+                q1s[i] = h * h;
+                q2s[i] = h * ((p1s[i] + p2s[i]) * 0.5f);
+            }
             __syncthreads();
             if (tid == 0) {
-                float norm_sq = 0.f, dot_nm = 0.f;
-                for (int i = 0; i < dim; ++i) {
-                    const float h = hp[i];
-                    const float mid = (p1s[i] + p2s[i]) * 0.5f;
-                    norm_sq += h * h;
-                    dot_nm += h * mid;
-                }
+                float norm_sq, dot_nm;
+                seq_sum2(q1s, q2s, dim, norm_sq, dot_nm);
                 const float norm = sqrtf(norm_sq);
                 if (norm > 1e-8f) {
                     const float inv = 1.0f / norm;
@@ -745,10 +780,14 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
             if (sh_scale)
                 for (int i = tid; i < dim; i += blockDim.x) hp[i] *= sh_inv;
         } else {  // make_angular_split (builder.rs:281-308)
+            for (int i = tid; i < dim; i += blockDim.x) {
+                q1s[i] = p1s[i] * p1s[i];
+                q2s[i] = p2s[i] * p2s[i];
+            }
+            __syncthreads();
             if (tid == 0) {
-                float n1 = 0.f, n2 = 0.f;
-                for (int i = 0; i < dim; ++i) n1 += p1s[i] * p1s[i];
-                for (int i = 0; i < dim; ++i) n2 += p2s[i] * p2s[i];
+                float n1, n2;
+                seq_sum2(q1s, q2s, dim, n1, n2);
                 sh_n1 = sqrtf(n1);
                 sh_n2 = sqrtf(n2);
                 sh_off = 0.f;
@@ -759,9 +798,11 @@ __device__ void forest_tree_dfs(const ForestDFSArgs& a, const int t, const int64
                 hp[i] = deg ? 0.f : p2s[i] / sh_n2 - p1s[i] / sh_n1;
             __syncthreads();
             if (!deg) {
+                for (int i = tid; i < dim; i += blockDim.x) q1s[i] = hp[i] * hp[i];
+                __syncthreads();
                 if (tid == 0) {
-                    float nn = 0.f;
-                    for (int i = 0; i < dim; ++i) nn += hp[i] * hp[i];
+                    float nn, unused;
+                    seq_sum2(q1s, q1s, dim, nn, unused);
                     sh_n1 = sqrtf(nn);
                 }
                 __syncthreads();
@@ -1059,7 +1100,7 @@ constexpr int kForestQueueThreads = 256;
 
 int forest_queue_grid(int dim) {
     int per_cu = 0;
-    const size_t lds = (size_t)3 * dim * sizeof(float);
+    const size_t lds = (size_t)5 * dim * sizeof(float);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_forest_dfs_queue<kForestQueueThreads>,
                                                      kForestQueueThreads, lds) != hipSuccess || per_cu < 1) {
         (void)hipGetLastError();
@@ -1070,7 +1111,7 @@ int forest_queue_grid(int dim) {
 
 int forest_coop_groups(int dim, int n_trees) {
     int per_cu = 0;
-    const size_t lds = (size_t)3 * dim * sizeof(float);
+    const size_t lds = (size_t)5 * dim * sizeof(float);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_forest_dfs_coop, kForestThreads, lds) != hipSuccess ||
         per_cu < 1) {
         (void)hipGetLastError();
@@ -1081,7 +1122,7 @@ int forest_coop_groups(int dim, int n_trees) {
 }
 
 void launch_forest_dfs(hipStream_t s, const ForestDFSArgs& a) {
-    const size_t lds = (size_t)3 * a.dim * sizeof(float);
+    const size_t lds = (size_t)5 * a.dim * sizeof(float);
     if (a.coop) {
         hipLaunchKernelGGL(k_forest_dfs_coop, dim3(a.n_trees * a.coop_groups), dim3(kForestThreads), lds, s, a);
     } else if (a.queue) {
