@@ -4,9 +4,11 @@
 // m <= leaf_size or depth >= max_depth; otherwise it splits by the bisector of two random members
 // (euclidean) or by the normalised difference of their directions (angular), a point goes left iff
 // its margin is < 0 (ties |margin| < 1e-8 decided by a random bit per point), and a degenerate split (all
-// points on one side) makes the node a leaf.  The unit normal is rounded to bf16 (the stored f32
-// hyperplane holds exactly those values, and a bf16 copy halves the bytes the side tests read);
-// the plane passes through the midpoint of the two members.  The random draws come from mix64 of the tree key
+// points on one side) makes the node a leaf.  The unit normal is rounded to int8 codes q times a power of two
+// s (the largest coordinate takes 7 bits; the stored f32 hyperplane holds exactly q s, and the side tests read
+// the one-byte codes: a quarter of the f32 plane's bytes); the plane passes through the midpoint of the two
+// members.  Since s is a power of two, the margin x . (q s) + off is s (x . q + off / s) exactly: the side
+// tests sum x . q and compare with -off / s (ties below 1e-8 / s).  The random draws come from mix64 of the tree key
 // and node id (the two members) or of the tree key, node id and point (a tie) instead of a depth-first
 // FastRng stream, so every node of a level is independent and one level is processed for all trees at once:
 //   k_lvl_planes   one workgroup per active node: split points, hyperplane, offset (or leaf)
@@ -41,7 +43,8 @@ struct LvlNode {
 // What the side tests of an active entry read: its node id (-1 if it does not split) and offset.
 struct PInfo {
     int32_t node;
-    float off;
+    float off;  // the plane's offset over its scale: the side test compares sum x q + off with 0
+    float tie;  // 1e-8 over the scale: |sum x q + off| below it is a margin tie
 };
 // The same for the two children of an entry of the last level (left child first), with the child's entry
 // (-1 for both when the parent did not split), so that a point moves to its next entry and finds its next
@@ -50,7 +53,7 @@ struct CInfo {
     int32_t node;
     float off;
     int32_t entry;
-    int32_t pad;
+    float tie;
 };
 
 struct LvlArgs {
@@ -61,12 +64,12 @@ struct LvlArgs {
     int64_t leaf_size, max_depth;
     int angular;
     const uint64_t* keys;  // per tree
-    float** hyper;         // per tree, cap x dim (values exactly representable in bf16)
+    float** hyper;         // per tree, cap x dim (the codes times the plane's power-of-two scale)
     float** offsets;       // per tree, cap
     int32_t** children;    // per tree, cap x 2
-    uint16_t** hyper_bf;   // per tree, cap x dim bf16 copy read by the side tests
+    int8_t** hyper_q;      // per tree, cap x dim int8 codes q of the planes read by the side tests
     int n_trees;
-    const uint16_t* hb0;   // the same copies as one array: tree t's node v at hb0 + (t * cap + v) * dim
+    const int8_t* hq0;     // the same codes as one array: tree t's node v at hq0 + (t * cap + v) * dim
     int64_t cap;
 };
 
@@ -75,12 +78,33 @@ __device__ __forceinline__ float bf16_round(float f) {  // round to nearest even
     u += 0x7fffu + ((u >> 16) & 1u);
     return __builtin_bit_cast(float, u & 0xffff0000u);
 }
-__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
-__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+__device__ __forceinline__ float q8(uint32_t w, int b) { return (float)(int8_t)(w >> (8 * b)); }  // code b of w
 
 static inline unsigned lgrid(int64_t work, int block, int64_t cap = 1 << 20) {
     const int64_t g = ceil_div(work, (int64_t)block);
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+__device__ __forceinline__ float block_max(float v, float* red) {  // order-independent
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    float t = 0.f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t = fmaxf(t, red[i]);
+    return t;
+}
+
+// The power of two s with max |h| / s in [64, 128) (1 for a zero plane): codes round(h / s) fit in [-127, 127].
+__device__ __forceinline__ float plane_scale(float mx) {
+    if (!(mx > 0.f)) return 1.f;
+    int e = 0;
+    (void)frexpf(mx, &e);  // mx = m 2^e, m in [0.5, 1)
+    return ldexpf(1.f, e - 7);
+}
+__device__ __forceinline__ int8_t plane_code(float h, float sc) {
+    return (int8_t)fminf(127.f, fmaxf(-127.f, rintf(h / sc)));
 }
 
 __device__ __forceinline__ float block_sum(float v, float* red) {
@@ -108,8 +132,8 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
                 ch[2 * nd.node] = -nd.start;
                 ch[2 * nd.node + 1] = -nd.end;
                 split[e] = 0;
-                pinfo[e] = PInfo{-1, 0.f};
-                if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{-1, 0.f, (int32_t)e, 0};
+                pinfo[e] = PInfo{-1, 0.f, 0.f};
+                if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{-1, 0.f, (int32_t)e, 0.f};
             }
             continue;  // uniform across the block
         }
@@ -122,7 +146,10 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
         const float* p1 = a.X + (int64_t)perm[nd.tree][nd.start + i1] * a.ld;
         const float* p2 = a.X + (int64_t)perm[nd.tree][nd.start + i2] * a.ld;
         float* hp = a.hyper[nd.tree] + (int64_t)nd.node * a.dim;
-        uint16_t* hb = a.hyper_bf[nd.tree] + (int64_t)nd.node * a.dim;
+        int8_t* hq = a.hyper_q[nd.tree] + (int64_t)nd.node * a.dim;
+        // the unit normal h (or the raw difference below the norm threshold), its scale and codes; the stored
+        // plane is q s exactly
+        float sc, off = 0.f;
         if (!a.angular) {
             float nsq = 0.f;
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
@@ -133,20 +160,21 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
             const float norm = sqrtf(nsq);
             const bool scale = norm > 1e-8f;
             const float inv = scale ? 1.0f / norm : 0.f;
-            // normal rounded to bf16 (what the side tests read); the plane passes through the midpoint
+            float mx = 0.f;
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x)
+                mx = fmaxf(mx, fabsf(scale ? (p2[i] - p1[i]) * inv : p2[i] - p1[i]));
+            sc = plane_scale(block_max(mx, red));
+            // the plane passes through the midpoint of the two members
             float dm = 0.f;
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
-                const float h = bf16_round(scale ? (p2[i] - p1[i]) * inv : p2[i] - p1[i]);
+                const int8_t q = plane_code(scale ? (p2[i] - p1[i]) * inv : p2[i] - p1[i], sc);
+                const float h = (float)q * sc;
                 hp[i] = h;
-                hb[i] = (uint16_t)(__builtin_bit_cast(uint32_t, h) >> 16);
+                hq[i] = q;
                 dm += h * ((p1[i] + p2[i]) * 0.5f);
             }
             dm = block_sum(dm, red);
-            if (threadIdx.x == 0) {
-                a.offsets[nd.tree][nd.node] = scale ? -dm : 0.f;
-                pinfo[e] = PInfo{nd.node, scale ? -dm : 0.f};
-                if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{nd.node, scale ? -dm : 0.f, (int32_t)e, 0};
-            }
+            off = scale ? -dm : 0.f;
         } else {
             float n1 = 0.f, n2 = 0.f;
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
@@ -162,17 +190,24 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
                 nn += h * h;
             }
             nn = sqrtf(block_sum(nn, red));
+            float mx = 0.f;
             for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
                 const float h = deg ? 0.f : p2[i] / n2 - p1[i] / n1;
-                const float hr = bf16_round(nn > 1e-8f ? h / nn : h);
-                hp[i] = hr;
-                hb[i] = (uint16_t)(__builtin_bit_cast(uint32_t, hr) >> 16);
+                mx = fmaxf(mx, fabsf(nn > 1e-8f ? h / nn : h));
             }
-            if (threadIdx.x == 0) {
-                a.offsets[nd.tree][nd.node] = 0.f;
-                pinfo[e] = PInfo{nd.node, 0.f};
-                if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{nd.node, 0.f, (int32_t)e, 0};
+            sc = plane_scale(block_max(mx, red));
+            for (int i = threadIdx.x; i < a.dim; i += blockDim.x) {
+                const float h = deg ? 0.f : p2[i] / n2 - p1[i] / n1;
+                const int8_t q = plane_code(nn > 1e-8f ? h / nn : h, sc);
+                hp[i] = (float)q * sc;
+                hq[i] = q;
             }
+        }
+        if (threadIdx.x == 0) {
+            const float off_s = off / sc, tie_s = 1e-8f / sc;
+            a.offsets[nd.tree][nd.node] = off;
+            pinfo[e] = PInfo{nd.node, off_s, tie_s};
+            if (nd.parent >= 0) cinfo[2 * nd.parent + (e & 1)] = CInfo{nd.node, off_s, (int32_t)e, tie_s};
         }
         if (threadIdx.x == 0) split[e] = 1;
         act += (unsigned long long)m;
@@ -180,8 +215,10 @@ __global__ __launch_bounds__(256) void k_lvl_planes(LvlArgs a, const LvlNode* no
     if (threadIdx.x == 0 && act) atomicAdd(active_pairs, act);
 }
 
-__device__ __forceinline__ uint32_t side_of(float margin, uint64_t key, int32_t node, int64_t point) {
-    if (fabsf(margin) < 1e-8f)  // ties: a random bit per (tree, node, point)
+// r = x . q + off / s of a plane q s (s a power of two): the point goes left iff r < 0; |r| < tie (= 1e-8 / s, a
+// margin below 1e-8) is a tie, decided by a random bit per (tree, node, point).
+__device__ __forceinline__ uint32_t side_of(float margin, float tie, uint64_t key, int32_t node, int64_t point) {
+    if (fabsf(margin) < tie)  // ties: a random bit per (tree, node, point)
         return (uint32_t)(mix64(key ^ mix64(((uint64_t)(uint32_t)node << 32) ^ (uint64_t)point)) & 1);
     return margin < 0.0f ? 1u : 0u;
 }
@@ -195,12 +232,12 @@ __device__ __forceinline__ PInfo lvl_info(int64_t n, const PInfo* __restrict__ p
                                           int32_t* nodeOf, const uint16_t* mask16, int t, int64_t i) {
     const int64_t ti = (int64_t)t * n + i;
     const int32_t e = nodeOf[ti];
-    if (e < 0) return PInfo{-1, 0.f};
+    if (e < 0) return PInfo{-1, 0.f, 0.f};
     if (!cinfo) return pinfo[e];
     const bool left = (mask16[(int64_t)(t >> 4) * n + i] >> (t & 15)) & 1u;
     const CInfo c = cinfo[2 * (int64_t)e + (left ? 0 : 1)];
     nodeOf[ti] = c.entry;
-    return PInfo{c.entry >= 0 ? c.node : -1, c.off};
+    return PInfo{c.entry >= 0 ? c.node : -1, c.off, c.tie};
 }
 
 // Side tests, point-major, 16 lanes per point (any dim).  First the lanes fetch, in parallel, the point's
@@ -219,18 +256,20 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const PInfo* __r
     for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4; i < a.n; i += groups) {
         // per-lane tables for trees t = gl + 16 j
         int32_t tnode[kMaxT / 16];
-        float toff[kMaxT / 16];
+        float toff[kMaxT / 16], ttie[kMaxT / 16];
         uint32_t mask = 0;  // bit j: tree gl + 16 j splits this point's node
 #pragma unroll
         for (int j = 0; j < kMaxT / 16; ++j) {
             const int t = gl + 16 * j;
             tnode[j] = 0;
             toff[j] = 0.f;
+            ttie[j] = 0.f;
             if (t < n_trees) {
                 const PInfo pi = lvl_info(a.n, pinfo, cinfo, nodeOf, mask16, t, i);
                 if (pi.node >= 0) {
                     tnode[j] = pi.node;
                     toff[j] = pi.off;
+                    ttie[j] = pi.tie;
                     mask |= 1u << j;
                 }
             }
@@ -263,19 +302,19 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const PInfo* __r
                     if (tl[u] < 0) continue;
                     const int t = tl[u] + 16 * j;
                     const int32_t node = __shfl(tnode[j], gbase + tl[u]);
-                    const uint16_t* hb = a.hyper_bf[t] + (int64_t)node * a.dim;
+                    const int8_t* hq = a.hyper_q[t] + (int64_t)node * a.dim;
                     if (CH > 0) {
 #pragma unroll
                         for (int c = 0; c < CH; ++c) {
                             const int d0 = 4 * gl + 64 * c;
                             if (d0 < a.dim) {
-                                const uint2 hv = *reinterpret_cast<const uint2*>(hb + d0);
-                                acc[u] += xr[c].x * bf16_lo(hv.x) + xr[c].y * bf16_hi(hv.x) +
-                                          xr[c].z * bf16_lo(hv.y) + xr[c].w * bf16_hi(hv.y);
+                                const uint32_t hv = *reinterpret_cast<const uint32_t*>(hq + d0);
+                                acc[u] += xr[c].x * q8(hv, 0) + xr[c].y * q8(hv, 1) + xr[c].z * q8(hv, 2) +
+                                          xr[c].w * q8(hv, 3);
                             }
                         }
                     } else {
-                        for (int d = gl; d < a.dim; d += 16) acc[u] += x[d] * bf16_lo((uint32_t)hb[d]);
+                        for (int d = gl; d < a.dim; d += 16) acc[u] += x[d] * (float)hq[d];
                     }
                 }
 #pragma unroll
@@ -285,7 +324,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const PInfo* __r
                 for (int u = 0; u < 4; ++u) {
                     if (tl[u] != gl) continue;  // the lane that owns tree t keeps its side
                     const int t = gl + 16 * j;
-                    lbits |= side_of(acc[u] + toff[j], a.keys[t], tnode[j], i) << j;
+                    lbits |= side_of(acc[u] + toff[j], ttie[j], a.keys[t], tnode[j], i) << j;
                 }
             }
         }
@@ -338,7 +377,7 @@ __device__ __forceinline__ float row16_sum(float v) {
 // Dense levels, point-major: a 16-lane group (one DPP row) per point holds the point's row in registers
 // (lane l: dims 8 (l + 16 r) .. +8).  Trees go in batches of 16: lane u advances tree u's entry and
 // fetches its PInfo, the group broadcasts each tree's node from its lane (row_newbcast), reads that tree's
-// bf16 plane with one coalesced 16-byte load per lane, and reduces the 16 partial dot products with row
+// plane codes with one coalesced 8-byte load per lane, and reduces the 16 partial dot products with row
 // rotations; lane u keeps tree u's margin, and the batch's 16 sides become the point's mask word.
 // Needs dim % 8 == 0 and dim <= 128 R.
 template <int R>
@@ -364,11 +403,12 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
         for (int tb = 0; tb < n_trees; tb += 16) {
             const int tl = tb + gl;
             int32_t nodeL = -1;
-            float offL = 0.f;
+            float offL = 0.f, tieL = 0.f;
             if (tl < n_trees) {
                 const PInfo pi = lvl_info(a.n, pinfo, cinfo, nodeOf, mask16, tl, i);
                 nodeL = pi.node;
                 offL = pi.off;
+                tieL = pi.tie;
             }
             // group-uniform: no tree of the batch splits this point's node (its mask word is not read again)
             if (!((__ballot(nodeL >= 0) >> (threadIdx.x & 48)) & 0xffffull)) continue;
@@ -376,14 +416,14 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
             constexpr int UB = R >= 4 ? 2 : 8 / R;  // planes in flight per batch (8 for dim <= 128)
 #pragma unroll
             for (int ub = 0; ub < 16; ub += UB) {  // UB planes in flight, then their dot products
-                uint4 h[UB][R];
+                uint2 h[UB][R];
 #pragma unroll
                 for (int u = 0; u < UB; ++u) {
                     // trees past n_trees and nodes that do not split read a valid plane whose result is unused
                     const int32_t nodeU = row_bcast(nodeL, ub + u);
                     const int tu = tb + ub + u < n_trees ? tb + ub + u : n_trees - 1;
-                    const uint4* hb = reinterpret_cast<const uint4*>(
-                        a.hb0 + (int64_t)tu * tstride + (int64_t)(nodeU > 0 ? nodeU : 0) * a.dim);
+                    const uint2* hb = reinterpret_cast<const uint2*>(
+                        a.hq0 + (int64_t)tu * tstride + (int64_t)(nodeU > 0 ? nodeU : 0) * a.dim);
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int c = gl + 16 * r;
@@ -395,21 +435,21 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
                     float acc = 0.f;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        const uint4 hv = h[u][r];
-                        acc = fmaf(xr[2 * r].x, bf16_lo(hv.x), acc);
-                        acc = fmaf(xr[2 * r].y, bf16_hi(hv.x), acc);
-                        acc = fmaf(xr[2 * r].z, bf16_lo(hv.y), acc);
-                        acc = fmaf(xr[2 * r].w, bf16_hi(hv.y), acc);
-                        acc = fmaf(xr[2 * r + 1].x, bf16_lo(hv.z), acc);
-                        acc = fmaf(xr[2 * r + 1].y, bf16_hi(hv.z), acc);
-                        acc = fmaf(xr[2 * r + 1].z, bf16_lo(hv.w), acc);
-                        acc = fmaf(xr[2 * r + 1].w, bf16_hi(hv.w), acc);
+                        const uint2 hv = h[u][r];
+                        acc = fmaf(xr[2 * r].x, q8(hv.x, 0), acc);
+                        acc = fmaf(xr[2 * r].y, q8(hv.x, 1), acc);
+                        acc = fmaf(xr[2 * r].z, q8(hv.x, 2), acc);
+                        acc = fmaf(xr[2 * r].w, q8(hv.x, 3), acc);
+                        acc = fmaf(xr[2 * r + 1].x, q8(hv.y, 0), acc);
+                        acc = fmaf(xr[2 * r + 1].y, q8(hv.y, 1), acc);
+                        acc = fmaf(xr[2 * r + 1].z, q8(hv.y, 2), acc);
+                        acc = fmaf(xr[2 * r + 1].w, q8(hv.y, 3), acc);
                     }
                     acc = row16_sum(acc);
                     if (gl == ub + u) mine = acc;
                 }
             }
-            const uint32_t left = nodeL >= 0 ? side_of(mine + offL, a.keys[tl], nodeL, i) : 0u;
+            const uint32_t left = nodeL >= 0 ? side_of(mine + offL, tieL, a.keys[tl], nodeL, i) : 0u;
             const uint64_t bits = __ballot(left != 0);
             if (gl == 0) mask16[(int64_t)(tb >> 4) * a.n + i] = (uint16_t)(bits >> (threadIdx.x & 48));
         }
@@ -417,7 +457,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
 }
 
 // Shallow dense levels on MFMA: while the level's active entries (over all trees) number at most NE, a
-// workgroup stages every split entry's bf16 plane in LDS once, and each wave multiplies blocks of 16 points by
+// workgroup stages every split entry's plane codes in LDS once (as bf16, exactly), and each wave multiplies blocks of 16 points by
 // all of them: the point's f32 row split into bf16 hi + lo parts (x = hi + lo up to 2^-16 relative), 16x16x32
 // bf16 MFMAs with f32 accumulation, the 16 x na margins staged in LDS.  Then the wave's 64 lanes take
 // (point p = lane % 16, trees lane / 16 + 4 i): the point's entry advances as in lvl_info, its margin is read
@@ -432,27 +472,36 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
     extern __shared__ __align__(16) uint16_t msm[];
     const int na16 = (na + 15) & ~15;
     const int pld = KS * 32 + 8;  // staged plane stride (bf16): 16-byte aligned, rows on different banks
-    float* eoff = (float*)(msm + (int64_t)na16 * pld);   // per entry: offset
-    int32_t* enode = (int32_t*)(eoff + na16);            // per entry: node id, -1 if it does not split
+    float* eoff = (float*)(msm + (int64_t)na16 * pld);   // per entry: offset over the plane's scale
+    float* etie = eoff + na16;                           // per entry: tie threshold over the scale
+    int32_t* enode = (int32_t*)(etie + na16);            // per entry: node id, -1 if it does not split
     float* csm = (float*)(enode + na16);                 // per wave: na16 x 16 margins (entry-major)
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int mrow = lane & 15, kqi = lane >> 4;
-    // stage the planes of the split entries (zeros elsewhere)
-    const int v8 = KS * 4;  // 16-byte chunks per plane
+    // stage the planes' codes of the split entries as bf16 (exact; zeros elsewhere)
+    const int v8 = KS * 4;  // 8-code chunks per plane
     for (int i = t; i < na16 * v8; i += blockDim.x) {
         const int e = i / v8, c = i - e * v8;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        uint2 v = make_uint2(0u, 0u);
         if (e < na) {
             const PInfo pi = pinfo[e];
             if (pi.node >= 0)
-                v = *reinterpret_cast<const uint4*>(a.hb0 + ((int64_t)cur[e].tree * a.cap + pi.node) * a.dim + 8 * c);
+                v = *reinterpret_cast<const uint2*>(a.hq0 + ((int64_t)cur[e].tree * a.cap + pi.node) * a.dim + 8 * c);
         }
-        *reinterpret_cast<uint4*>(msm + e * pld + 8 * c) = v;
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t w = j < 2 ? v.x : v.y;
+            const int b = 2 * (j & 1);
+            o[j] = (__builtin_bit_cast(uint32_t, q8(w, b)) >> 16) | (__builtin_bit_cast(uint32_t, q8(w, b + 1)) & 0xffff0000u);
+        }
+        *reinterpret_cast<uint4*>(msm + e * pld + 8 * c) = make_uint4(o[0], o[1], o[2], o[3]);
     }
     for (int e = t; e < na16; e += blockDim.x) {
-        const PInfo pi = e < na ? pinfo[e] : PInfo{-1, 0.f};
+        const PInfo pi = e < na ? pinfo[e] : PInfo{-1, 0.f, 0.f};
         enode[e] = pi.node;
         eoff[e] = pi.off;
+        etie[e] = pi.tie;
     }
     __syncthreads();
     float* cw = csm + (int64_t)w * na16 * 16;
@@ -531,12 +580,13 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
                     const int64_t ti = (int64_t)tr * a.n + p;
                     const int32_t e0 = j == 0 ? ce0[ii] : nodeOf[ti];
                     int32_t e = -1, node = -1;
-                    float off = 0.f;
+                    float off = 0.f, tie = 0.f;
                     if (e0 >= 0) {
                         if (!cinfo) {
                             e = e0;
                             node = enode[e];
                             off = eoff[e];
+                            tie = etie[e];
                         } else {
                             const uint32_t mw = j == 0 ? cmw : mask16[(int64_t)j * a.n + p];
                             const bool l0 = (mw >> (tr & 15)) & 1u;
@@ -545,9 +595,10 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
                             e = c.entry;
                             node = c.entry >= 0 ? c.node : -1;
                             off = c.off;
+                            tie = c.tie;
                         }
                     }
-                    if (node >= 0) left = side_of(cw[e * 16 + mrow] + off, a.keys[tr], node, p);
+                    if (node >= 0) left = side_of(cw[e * 16 + mrow] + off, tie, a.keys[tr], node, p);
                 }
                 const uint64_t bal = __ballot(left != 0u);
 #pragma unroll
@@ -574,7 +625,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
          w0 += nw * 64) {  // wave-uniform
         const int64_t item = w0 + lane;
         int32_t node = -1, t = 0, pos = 0;
-        float off = 0.f;
+        float off = 0.f, tie = 0.f;
         if (item < total) {
             t = (int32_t)(item / a.n);
             pos = (int32_t)(item - (int64_t)t * a.n);
@@ -584,6 +635,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
                 const PInfo pi = pinfo[e];
                 node = pi.node;
                 off = pi.off;
+                tie = pi.tie;
             }
         }
         uint64_t m = __ballot(node >= 0);
@@ -594,24 +646,25 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pos(LvlArgs a, const PInfo*
             for (int k = 0; k < 4 && m; ++k) m &= m - 1;
             const int sl = src < 0 ? 0 : src;
             const int32_t nd = __shfl(node, sl), tt = __shfl(t, sl), ps = __shfl(pos, sl);
-            const float of = __shfl(off, sl);
+            const float of = __shfl(off, sl), ti = __shfl(tie, sl);
             float acc = 0.f;
             int32_t pt = 0;
             if (src >= 0) {
                 pt = perm[tt][ps];
                 const float* x = a.X + (int64_t)pt * a.ld;
-                const uint16_t* hb = a.hyper_bf[tt] + (int64_t)nd * a.dim;
+                const int8_t* hq = a.hyper_q[tt] + (int64_t)nd * a.dim;
                 for (int d0 = 4 * gl; d0 < a.dim; d0 += 64) {
                     const float4 xv = *reinterpret_cast<const float4*>(x + d0);
-                    const uint2 hv = *reinterpret_cast<const uint2*>(hb + d0);
-                    acc = fmaf(xv.x, bf16_lo(hv.x), acc);
-                    acc = fmaf(xv.y, bf16_hi(hv.x), acc);
-                    acc = fmaf(xv.z, bf16_lo(hv.y), acc);
-                    acc = fmaf(xv.w, bf16_hi(hv.y), acc);
+                    const uint32_t hv = *reinterpret_cast<const uint32_t*>(hq + d0);
+                    acc = fmaf(xv.x, q8(hv, 0), acc);
+                    acc = fmaf(xv.y, q8(hv, 1), acc);
+                    acc = fmaf(xv.z, q8(hv, 2), acc);
+                    acc = fmaf(xv.w, q8(hv, 3), acc);
                 }
             }
             for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);  // inside the 16-lane group
-            if (src >= 0 && gl == 0) flags8[(int64_t)tt * a.n + ps] = (uint8_t)side_of(acc + of, a.keys[tt], nd, pt);
+            if (src >= 0 && gl == 0)
+                flags8[(int64_t)tt * a.n + ps] = (uint8_t)side_of(acc + of, ti, a.keys[tt], nd, pt);
         }
     }
 }
@@ -774,8 +827,8 @@ __global__ void k_lvl_assign(const LvlNode* nodes, int64_t na, const uint8_t* sp
                              int32_t** children, LvlNode* next, CInfo* cinfo) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x) {
         if (!split[e]) {
-            cinfo[2 * e] = CInfo{-1, 0.f, -1, 0};
-            cinfo[2 * e + 1] = CInfo{-1, 0.f, -1, 0};
+            cinfo[2 * e] = CInfo{-1, 0.f, -1, 0.f};
+            cinfo[2 * e + 1] = CInfo{-1, 0.f, -1, 0.f};
             continue;
         }
         const LvlNode nd = nodes[e];
@@ -1165,7 +1218,7 @@ static bool mfma_levels_ok(int dim, int T) { return dim % 32 == 0 && dim <= 256 
 static void launch_margins_mfma(hipStream_t s, const LvlArgs& a, const LvlNode* cur, int na, const PInfo* pinfo, int T,
                                 const CInfo* cinfo, int32_t* nodeOf, uint16_t* mask16) {
     const int na16 = (na + 15) & ~15;
-    const size_t lds = (size_t)na16 * (a.dim + 8) * 2 + (size_t)na16 * 8 + 4 * (size_t)na16 * 16 * 4;
+    const size_t lds = (size_t)na16 * (a.dim + 8) * 2 + (size_t)na16 * 12 + 4 * (size_t)na16 * 16 * 4;
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
     const int64_t nblk = (a.n + 15) >> 4;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nblk, (int64_t)4),
@@ -1197,7 +1250,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     WBuf<int32_t> nodeOf(W, kWsForest + 3, total);  // dense levels: each (tree, point)'s entry
     const int NB = (T + 15) / 16;                      // mask words per point
     WBuf<uint16_t> mask16(W, kWsForest + 27, (size_t)NB * n);
-    WBuf<uint16_t> hyper_bf(W, kWsForest + 4, (size_t)T * cap * dim);
+    WBuf<int8_t> hyper_q(W, kWsForest + 4, (size_t)T * cap * dim);
     for (int t = 0; t < T; ++t) {
         perm_h[t] = F.trees[t].indices.p;
         perm2_h[t] = perm2.p + (int64_t)t * n;
@@ -1227,12 +1280,12 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     HIP_TRY(hipMemcpyAsync(d_hp.p, hp_h.data(), T * sizeof(float*), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_of.p, of_h.data(), T * sizeof(float*), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_keys.p, keys.data(), T * 8, hipMemcpyHostToDevice, s));
-    std::vector<uint16_t*> hb_h(T);
-    for (int t = 0; t < T; ++t) hb_h[t] = hyper_bf.p + (size_t)t * cap * dim;
-    WBuf<uint16_t*> d_hb(W, kWsForest + 10, T);
-    HIP_TRY(hipMemcpyAsync(d_hb.p, hb_h.data(), T * sizeof(uint16_t*), hipMemcpyHostToDevice, s));
+    std::vector<int8_t*> hb_h(T);
+    for (int t = 0; t < T; ++t) hb_h[t] = hyper_q.p + (size_t)t * cap * dim;
+    WBuf<int8_t*> d_hb(W, kWsForest + 10, T);
+    HIP_TRY(hipMemcpyAsync(d_hb.p, hb_h.data(), T * sizeof(int8_t*), hipMemcpyHostToDevice, s));
     const LvlArgs a{X, ld, dim, n, std::max<int64_t>(leaf_size, 1), max_depth, angular,
-                    d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p, T, hyper_bf.p, cap};
+                    d_keys.p, d_hp.p, d_of.p, d_ch.p, d_hb.p, T, hyper_q.p, cap};
 
     const int64_t nchunks = ceil_div(total, (int64_t)kChunk);
     WBuf<uint8_t> flags8(W, kWsForest + 11, (size_t)nchunks * kChunk);
@@ -1389,11 +1442,11 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
         const unsigned* tc = reinterpret_cast<const unsigned*>(lst + 1);
         bool sparse_next = sparse;
         {
-            // algorithmic bytes: every point row once, each split node's hyperplane once, and per tested
-            // (tree, point) its node and position reads and its flag write
+            // algorithmic bytes: every point row once, each split node's plane codes once (a byte a
+            // coordinate), and per tested (tree, point) its node and position reads and its flag write
             const double tested = (double)(pairs_now - pairs_before);
             pairs_before = pairs_now;
-            margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim * 4.0 +
+            margins_timer.collect(std::min<double>(tested, (double)n) * dim * 4.0 + (double)na * dim +
                                   tested * 12.0);
             sparse_next = sparse || (dim % 4 == 0 && tested <= (double)(kSparsePairs * n));
         }
