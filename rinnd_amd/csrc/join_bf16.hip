@@ -431,14 +431,27 @@ __global__ __launch_bounds__(256) void k_leaf_dist_mfma(const uint16_t* __restri
             O -= 1;
             const v4f acc = gram_block16(bsm16 + (16 * A + mrow) * lds_ld, bsm16 + (16 * O + mrow) * lds_ld, (int)ldb,
                                          lane >> 4);
+            const int o = 16 * O + mrow;  // D[row][col] layout of the 16x16 MFMA: rows 16 A + kq + r
+            if (A < O) {  // every a < o: o's row gets its four columns 16 A + kq .. + 3 in one 16-byte store
+                if (o < m) {
+                    float d[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int a = 16 * A + kq + r;  // D[row][col] layout of the 16x16 MFMA
-                const int o = 16 * O + mrow;
-                if (a < o && o < m) {
-                    const float d = (nr[a] + nr[o]) - 2.0f * acc[r];
-                    D[(s + a) * Dl4 + o] = d;
-                    D[(s + o) * Dl4 + a] = d;
+                    for (int r = 0; r < 4; ++r) {
+                        const int a = 16 * A + kq + r;
+                        d[r] = (nr[a] + nr[o]) - 2.0f * acc[r];
+                        D[(s + a) * Dl4 + o] = d[r];
+                    }
+                    *reinterpret_cast<float4*>(D + (s + o) * Dl4 + 16 * A + kq) = make_float4(d[0], d[1], d[2], d[3]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int a = 16 * A + kq + r;
+                    if (a < o && o < m) {
+                        const float d = (nr[a] + nr[o]) - 2.0f * acc[r];
+                        D[(s + a) * Dl4 + o] = d;
+                        D[(s + o) * Dl4 + a] = d;
+                    }
                 }
             }
         }
@@ -687,10 +700,20 @@ __global__ __launch_bounds__(256) void k_leaf_surv_mfma(const uint16_t* __restri
                                 if ((mrow == 0) & (half != 0u))
                                     atomicOr(&lmask[x * 8 + (Oh >> 1)], half << (16 * (Oh & 1)));
                             }
-                            if (any_so && so[r]) Do[x] = d[r];
                             nib |= (uint32_t)so[r] << r;
                         }
                         if (!any_so) continue;  // wave-uniform: no mask bits for any column this block
+                        // o's D row, columns x0 .. x0 + 3.  Off the diagonal every one of them is a pair of this
+                        // block and no other (A < Oh puts x0 + 3 below o), so a lane with a survivor writes all
+                        // four at once: the extra values land in slots that only this pair owns and its mask bit
+                        // leaves unread.  On the diagonal the pairs x < o are written one by one.
+                        if (A < Oh) {
+                            if (nib) *reinterpret_cast<float4*>(Do + x0) = make_float4(d[0], d[1], d[2], d[3]);
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                if (so[r]) Do[x0 + r] = d[r];
+                        }
                         // row o, columns 16 A + kq .. + 3: each lane group ORs its nibble (no return, no wait)
                         if (nib) atomicOr(&lmask[o * 8 + (A >> 1)], nib << (kq + 16 * (A & 1)));
                     }
@@ -715,6 +738,8 @@ void launch_leaf_surv_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree
                            const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, const float* hdist, int k,
                            float* D, int64_t Dl4, uint32_t* Msk, int2* info, unsigned long long* surv) {
     if (mcap > 256) throw Error{RINND_ERR_INVALID_ARG, "survivor leaf kernel: leaves above 256 members"};
+    if (Dl4 % 4 || ((uintptr_t)D & 15))  // the kernel writes four D columns with one 16-byte store
+        throw Error{RINND_ERR_INVALID_ARG, "survivor leaf kernel: D rows must be 16-byte aligned"};
     // rows of at most 128 bf16: persistent, two workgroups per CU (the LDS limit), each pipelining its leaves
     const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(max_leaves, 1),
                                                       R.ldb <= 128 ? 2 * (int64_t)device_cus() : 4096);
@@ -741,6 +766,8 @@ size_t leaf_dist_mfma_lds(int64_t ldb, int mcap) {
 void launch_leaf_dist_mfma(hipStream_t s, const Bf16Rows& R, const int32_t* tree_indices, const int64_t* leaves,
                            const int64_t* n_leaves_dev, int64_t max_leaves, int mcap, float* D, int64_t Dl4,
                            unsigned long long* pairs) {
+    if (Dl4 % 4 || ((uintptr_t)D & 15))  // the kernel writes four D columns with one 16-byte store
+        throw Error{RINND_ERR_INVALID_ARG, "leaf distance kernel: D rows must be 16-byte aligned"};
     const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(max_leaves, 1), 4096);
     hipLaunchKernelGGL(k_leaf_dist_mfma, dim3(grid), dim3(256), leaf_dist_mfma_lds(R.ldb, mcap), s, R.Xb.p, R.ldb,
                        R.nrm.p, tree_indices, leaves, n_leaves_dev, mcap, D, Dl4, pairs);
