@@ -226,11 +226,13 @@ __device__ __forceinline__ uint32_t side_of(float margin, float tie, uint64_t ke
 // The side-test info of point i in tree t at this level.  At level 0 (cinfo null) its entry is tree t's root
 // and the info that entry's PInfo.  Later, its entry one level up (nodeOf) and the side bit the last level
 // left in the point's mask word choose one of that entry's two CInfo (children entries are consecutive, the
-// left one first); the point's entry moves to the child's.  nodeOf is rewritten in place (the lane that
-// owns (t, i) is its only reader and writer).
-__device__ __forceinline__ PInfo lvl_info(int64_t n, const PInfo* __restrict__ pinfo, const CInfo* __restrict__ cinfo,
-                                          int32_t* nodeOf, const uint16_t* mask16, int t, int64_t i) {
-    const int64_t ti = (int64_t)t * n + i;
+// left one first); the point's entry moves to the child's.  nodeOf is point-major (point i's entries of the
+// n_trees trees are consecutive, one segment for the lanes that test the point) and rewritten in place (the
+// lane that owns (t, i) is its only reader and writer).
+__device__ __forceinline__ PInfo lvl_info(int64_t n, int n_trees, const PInfo* __restrict__ pinfo,
+                                          const CInfo* __restrict__ cinfo, int32_t* nodeOf, const uint16_t* mask16,
+                                          int t, int64_t i) {
+    const int64_t ti = i * n_trees + t;
     const int32_t e = nodeOf[ti];
     if (e < 0) return PInfo{-1, 0.f, 0.f};
     if (!cinfo) return pinfo[e];
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins(LvlArgs a, const PInfo* __r
             toff[j] = 0.f;
             ttie[j] = 0.f;
             if (t < n_trees) {
-                const PInfo pi = lvl_info(a.n, pinfo, cinfo, nodeOf, mask16, t, i);
+                const PInfo pi = lvl_info(a.n, n_trees, pinfo, cinfo, nodeOf, mask16, t, i);
                 if (pi.node >= 0) {
                     tnode[j] = pi.node;
                     toff[j] = pi.off;
@@ -405,7 +407,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
             int32_t nodeL = -1;
             float offL = 0.f, tieL = 0.f;
             if (tl < n_trees) {
-                const PInfo pi = lvl_info(a.n, pinfo, cinfo, nodeOf, mask16, tl, i);
+                const PInfo pi = lvl_info(a.n, n_trees, pinfo, cinfo, nodeOf, mask16, tl, i);
                 nodeL = pi.node;
                 offL = pi.off;
                 tieL = pi.tie;
@@ -525,7 +527,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
 #pragma unroll
         for (int ii = 0; ii < 4; ++ii) {
             const int tr = kqi + 4 * ii;
-            pe0[ii] = ok && tr < n_trees ? nodeOf[(int64_t)(tr < n_trees ? tr : 0) * a.n + pr] : -1;
+            pe0[ii] = ok && tr < n_trees ? nodeOf[pr * n_trees + (tr < n_trees ? tr : 0)] : -1;
         }
         pmw = ok && cinfo ? mask16[pr] : (uint16_t)0;
     };
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
                 const int tr = 16 * j + kqi + 4 * ii;
                 uint32_t left = 0u;
                 if (pv && tr < n_trees) {
-                    const int64_t ti = (int64_t)tr * a.n + p;
+                    const int64_t ti = p * n_trees + tr;
                     const int32_t e0 = j == 0 ? ce0[ii] : nodeOf[ti];
                     int32_t e = -1, node = -1;
                     float off = 0.f, tie = 0.f;
@@ -1196,7 +1198,7 @@ __global__ void k_lvl_init(int n_trees, int64_t n, int32_t* const* perm, int32_t
         const int64_t pos = item - (int64_t)t * n;
         perm[t][pos] = (int32_t)pos;
         act[t][pos] = t;  // level 0: the root of tree t is active entry t
-        nodeOf[item] = t;
+        nodeOf[pos * n_trees + t] = t;
     }
 }
 
@@ -1247,7 +1249,7 @@ bool build_forest_level(hipStream_t s, const float* X, int64_t ld, int dim, int6
     std::vector<int32_t*> perm_h(T), perm2_h(T), act_h(T), act2_h(T), ch_h(T);
     std::vector<float*> hp_h(T), of_h(T);
     WBuf<int32_t> perm2(W, kWsForest + 0, total), act(W, kWsForest + 1, total), act2(W, kWsForest + 2, total);
-    WBuf<int32_t> nodeOf(W, kWsForest + 3, total);  // dense levels: each (tree, point)'s entry
+    WBuf<int32_t> nodeOf(W, kWsForest + 3, total);  // dense levels: each (point, tree)'s entry, point-major
     const int NB = (T + 15) / 16;                      // mask words per point
     WBuf<uint16_t> mask16(W, kWsForest + 27, (size_t)NB * n);
     WBuf<int8_t> hyper_q(W, kWsForest + 4, (size_t)T * cap * dim);
