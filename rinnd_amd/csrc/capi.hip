@@ -772,8 +772,8 @@ rinnd_status rinnd_ctx_create(int device, rinnd_ctx** out) {
         delete c;
         return RINND_ERR_HIP;
     }
-    for (auto& e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) return RINND_ERR_HIP;
+    for (auto& e : c->ev)  // phase and launch timings only: no system-scope fence at each record
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return RINND_ERR_HIP;
     // the FastRng jump tables (built once per process) on the device, so that no build pays for them
     try {
         const uint64_t* jt = rinnd::host_jump_tables();

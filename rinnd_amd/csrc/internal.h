@@ -146,7 +146,8 @@ struct LaunchLog {
     hipEvent_t event() {
         if (used == pool->size()) {
             hipEvent_t e = nullptr;
-            HIP_TRY(hipEventCreate(&e));
+            // timing only: no system-scope fence (no L2 writeback and invalidate between the timed kernels)
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             pool->push_back(e);
         }
         return (*pool)[used++];
