@@ -809,17 +809,26 @@ __global__ void k_lvl_counts(LvlArgs a, const LvlNode* nodes, int64_t na, const 
 // start per tree, with the total at n_trees) becomes the next level's start.
 __global__ void k_lvl_tree_base(int n_trees, const uint32_t* start, const uint32_t* cscan, unsigned* tree_children,
                                 int32_t* node_count, int32_t* base, uint32_t* cum) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t acc = 0;
-    for (int t = 0; t < n_trees; ++t) {
-        const uint32_t c = cscan[start[t + 1]] - cscan[start[t]];
-        tree_children[t] = c;
-        base[t] = node_count[t];
-        cum[t] = acc;
-        acc += c;
-        node_count[t] += (int32_t)c;
+    if (threadIdx.x >= 64 || blockIdx.x != 0) return;  // one wave: a lane per tree, 64 trees per step
+    const int lane = threadIdx.x;
+    uint32_t carry = 0;
+    for (int t0 = 0; t0 < n_trees; t0 += 64) {
+        const int t = t0 + lane;
+        const uint32_t c = t < n_trees ? cscan[start[t + 1]] - cscan[start[t]] : 0u;
+        uint32_t incl = c;  // inclusive scan of the children counts over the lanes
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (t < n_trees) {
+            tree_children[t] = c;
+            base[t] = node_count[t];
+            cum[t] = carry + incl - c;
+            node_count[t] += (int32_t)c;
+        }
+        carry += __shfl(incl, 63);
     }
-    cum[n_trees] = acc;
+    if (lane == 0) cum[n_trees] = carry;
 }
 
 // An entry that does not split gets its two CInfo marked (no child entry); the children of the others
