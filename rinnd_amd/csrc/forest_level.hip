@@ -512,8 +512,8 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
     // software pipeline: the next block's rows and (first 16 trees) entries and mask word are loaded while the
     // current block is evaluated
     float4 rf[KS][2];
-    int32_t pe0[4];
-    uint16_t pmw = 0;
+    int32_t pe0[4];  // entry of tree mrow for point 4 ii + kqi of the block
+    uint16_t pmw[4];  // and that point's first mask word
     auto prefetch = [&](int64_t blk) {
         const int64_t pr0 = (blk << 4) + mrow;
         const bool ok = blk < nblk && pr0 < a.n;
@@ -525,11 +525,12 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
             rf[kk][1] = ok ? *reinterpret_cast<const float4*>(x + 32 * kk + 8 * kqi + 4) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
-        for (int ii = 0; ii < 4; ++ii) {
-            const int tr = kqi + 4 * ii;
-            pe0[ii] = ok && tr < n_trees ? nodeOf[pr * n_trees + (tr < n_trees ? tr : 0)] : -1;
+        for (int ii = 0; ii < 4; ++ii) {  // the decisions' layout: four points of 16 trees, one segment each
+            const int64_t qp = (blk << 4) + 4 * ii + kqi;
+            const bool qok = blk < nblk && qp < a.n;
+            pe0[ii] = qok && mrow < n_trees ? nodeOf[qp * n_trees + mrow] : -1;
+            pmw[ii] = qok && cinfo ? mask16[qp] : (uint16_t)0;
         }
-        pmw = ok && cinfo ? mask16[pr] : (uint16_t)0;
     };
     int64_t blk = (int64_t)blockIdx.x * (blockDim.x >> 6) + w;
     prefetch(blk);
@@ -550,9 +551,12 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
             }
         }
         int32_t ce0[4];
+        uint16_t cmw[4];
 #pragma unroll
-        for (int ii = 0; ii < 4; ++ii) ce0[ii] = pe0[ii];
-        const uint16_t cmw = pmw;
+        for (int ii = 0; ii < 4; ++ii) {
+            ce0[ii] = pe0[ii];
+            cmw[ii] = pmw[ii];
+        }
         prefetch(blk + wstride);
         wave_sync();  // the previous block's margins are read
         for (int ct = 0; ct < (na16 >> 4); ++ct) {
@@ -568,15 +572,15 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
             *reinterpret_cast<float4*>(cw + (16 * ct + mrow) * 16 + 4 * kqi) = make_float4(acc[0], acc[1], acc[2], acc[3]);
         }
         wave_sync();
-        // sides: lane (point p0 + mrow, trees kqi + 4 i); mask word j of the point holds trees 16 j .. 16 j + 15,
-        // tree 16 j + g + 4 ii being bit (16 g + mrow) of the ballot of i = 4 j + ii
-        const int64_t p = p0 + mrow;
-        const bool pv = p < a.n;
+        // sides: in step ii lane (point p0 + 4 ii + kqi, tree 16 j + mrow), so that the 16 lanes of a point read
+        // and write its entries as one segment; the point's mask word j is bits 16 kqi .. + 15 of the ballot
         for (int j = 0; 16 * j < n_trees; ++j) {  // uniform
-            uint32_t wbits = 0u;
 #pragma unroll
             for (int ii = 0; ii < 4; ++ii) {
-                const int tr = 16 * j + kqi + 4 * ii;
+                const int pq = 4 * ii + kqi;  // the point's row in the block
+                const int64_t p = p0 + pq;
+                const bool pv = p < a.n;
+                const int tr = 16 * j + mrow;
                 uint32_t left = 0u;
                 if (pv && tr < n_trees) {
                     const int64_t ti = p * n_trees + tr;
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
                             off = eoff[e];
                             tie = etie[e];
                         } else {
-                            const uint32_t mw = j == 0 ? cmw : mask16[(int64_t)j * a.n + p];
+                            const uint32_t mw = j == 0 ? cmw[ii] : mask16[(int64_t)j * a.n + p];
                             const bool l0 = (mw >> (tr & 15)) & 1u;
                             const CInfo c = cinfo[2 * (int64_t)e0 + (l0 ? 0 : 1)];
                             nodeOf[ti] = c.entry;
@@ -600,13 +604,11 @@ __global__ __launch_bounds__(256) void k_lvl_margins_mfma(LvlArgs a, const LvlNo
                             tie = c.tie;
                         }
                     }
-                    if (node >= 0) left = side_of(cw[e * 16 + mrow] + off, tie, a.keys[tr], node, p);
+                    if (node >= 0) left = side_of(cw[e * 16 + pq] + off, tie, a.keys[tr], node, p);
                 }
                 const uint64_t bal = __ballot(left != 0u);
-#pragma unroll
-                for (int g = 0; g < 4; ++g) wbits |= (uint32_t)((bal >> (16 * g + mrow)) & 1ull) << (g + 4 * ii);
+                if (mrow == 0 && pv) mask16[(int64_t)j * a.n + p] = (uint16_t)(bal >> (16 * kqi));
             }
-            if (kqi == 0 && pv) mask16[(int64_t)j * a.n + p] = (uint16_t)wbits;
         }
     }
 }
