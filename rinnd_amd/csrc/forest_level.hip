@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void k_lvl_margins_pm(LvlArgs a, const PInfo* 
             // group-uniform: no tree of the batch splits this point's node (its mask word is not read again)
             if (!((__ballot(nodeL >= 0) >> (threadIdx.x & 48)) & 0xffffull)) continue;
             float mine = 0.f;
-            constexpr int UB = R >= 4 ? 2 : 8 / R;  // planes in flight per batch (8 for dim <= 128)
+            constexpr int UB = R >= 4 ? 2 : (R == 1 ? 16 : 8 / R);  // planes in flight per batch (all 16 for dim <= 128)
 #pragma unroll
             for (int ub = 0; ub < 16; ub += UB) {  // UB planes in flight, then their dot products
                 uint2 h[UB][R];
